@@ -1,0 +1,489 @@
+"""gooseberry_amd -- Python view of libgooseberry (the MI355X sparse-solve
+library) through its C ABI (include/gooseberry.h).
+
+The product is the C library and the `gooseberry` program; this module loads
+the in-tree libgooseberry.so with ctypes and mirrors the reference's
+interface for the sparse-solve path:
+
+  spmult(...)          gooseberry_spmult (include/gooseberry.h:174-184)
+  cgd(...)             the reference's cgd() (src/cgd.c:40-129), host arrays
+  Matrix               a device-resident CSR matrix (SpMV, SpMM, CG)
+  read_*/write_*       the .csr/.mtx/grid readers and writers (src/io.c)
+  countingsort_kv, permute_sparse, permute_dense, read_partition
+
+There is no CPU fallback: every function raises if the library is missing
+or the GPU call fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libgooseberry.so")
+CLI_PATH = os.path.join(_PKG, "bin", "gooseberry")
+
+SUCCESS = 1
+ERRORS = {
+    1: "SUCCESS", 2: "UNIMPLEMENTED", 3: "INVALIDINPUT", 4: "READ",
+    5: "WRITE", 6: "FILENOTFOUND", 7: "PERMISSIONDENIED", 8: "UNKNOWN",
+    9: "DEVICE", 10: "NOMEMORY", 11: "BREAKDOWN",
+}
+ERROR_CODES = {v: k for k, v in ERRORS.items()}
+
+FORMAT_GRID = 1
+FORMAT_CSR = 2
+FORMAT_MATRIXMARKET = 7
+
+TOL_ABSOLUTE = 0
+TOL_RELATIVE = 1
+
+# every extern "C" symbol declared in include/gooseberry.h
+EXPORTED = [
+    "gooseberry_scale", "gooseberry_spmult", "gooseberry_read_sparse_matrix",
+    "gooseberry_read_dense_matrix", "gooseberry_write_dense_matrix",
+    "gooseberry_write_sparse_matrix", "gooseberry_read_labels",
+    "gooseberry_read_partition", "gooseberry_free", "gooseberry_strerror",
+    "gooseberry_last_device_error", "gooseberry_countingsort_kv",
+    "gooseberry_permute_sparse", "gooseberry_permute_dense",
+    "gooseberry_device_alloc", "gooseberry_device_free",
+    "gooseberry_memcpy_htod", "gooseberry_memcpy_dtoh",
+    "gooseberry_device_synchronize", "gooseberry_device_memory",
+    "gooseberry_event_create", "gooseberry_event_destroy",
+    "gooseberry_event_record", "gooseberry_event_elapsed_ms",
+    "gooseberry_matrix_create", "gooseberry_matrix_create_device",
+    "gooseberry_matrix_free", "gooseberry_matrix_replan",
+    "gooseberry_matrix_info", "gooseberry_matrix_spmv",
+    "gooseberry_matrix_spmm", "gooseberry_matrix_cgd", "gooseberry_cgd",
+    "gooseberry_generate_laplacian", "gooseberry_generate_uniform",
+]
+
+
+class GooseberryError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+        msg = "%s failed: GOOSEBERRY_ERROR_%s" % (what, self.name)
+        try:
+            dev = lib().gooseberry_last_device_error().decode()
+            if dev and code == 9:
+                msg += " (" + dev + ")"
+        except Exception:
+            pass
+        super().__init__(msg)
+
+
+class _Plan(C.Structure):
+    _fields_ = [("short_max", C.c_uint32), ("wave_max", C.c_uint32),
+                ("window", C.c_uint32), ("rowcap", C.c_uint32)]
+
+
+class _MatrixInfo(C.Structure):
+    _fields_ = [("nrows", C.c_uint32), ("ncols", C.c_uint32),
+                ("nnz", C.c_uint64), ("plan", _Plan),
+                ("nblocks_stream", C.c_uint32), ("nrows_wave", C.c_uint32),
+                ("nrows_block", C.c_uint32), ("hist", C.c_uint64 * 33)]
+
+
+class CgdInfo(C.Structure):
+    _fields_ = [("niter", C.c_uint64), ("rnorm", C.c_double),
+                ("bnorm", C.c_double), ("converged", C.c_int),
+                ("status", C.c_int), ("nrefine", C.c_uint64),
+                ("true_rel_residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+_P = C.c_void_p
+_U32P = C.POINTER(C.c_uint32)
+_F64P = C.POINTER(C.c_double)
+
+
+def _sig(lib):
+    u32, u64, i32, f64, sz = C.c_uint32, C.c_uint64, C.c_int, C.c_double, C.c_size_t
+    s = {
+        "gooseberry_scale": (i32, [u32, _P, f64]),
+        "gooseberry_spmult": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, u32]),
+        "gooseberry_read_sparse_matrix": (i32, [i32, C.c_char_p, _U32P, _U32P,
+                                                C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+        "gooseberry_read_dense_matrix": (i32, [i32, C.c_char_p, _U32P, _U32P, C.POINTER(_P)]),
+        "gooseberry_write_dense_matrix": (i32, [i32, C.c_char_p, u32, u32, _P]),
+        "gooseberry_write_sparse_matrix": (i32, [i32, C.c_char_p, u32, u32, _P, _P, _P]),
+        "gooseberry_read_labels": (i32, [C.c_char_p, _U32P, C.POINTER(_P)]),
+        "gooseberry_read_partition": (i32, [C.c_char_p, _U32P, _U32P, C.POINTER(_P),
+                                            C.POINTER(_P), C.POINTER(_P)]),
+        "gooseberry_free": (None, [_P]),
+        "gooseberry_strerror": (C.c_char_p, [i32]),
+        "gooseberry_last_device_error": (C.c_char_p, []),
+        "gooseberry_countingsort_kv": (i32, [_P, _P, u32, u32, sz, _P, _P]),
+        "gooseberry_permute_sparse": (i32, [u32, u32, _P, _P, _P, _P, _P]),
+        "gooseberry_permute_dense": (i32, [u32, u32, _P, _P, _P, i32]),
+        "gooseberry_device_alloc": (i32, [C.POINTER(_P), sz]),
+        "gooseberry_device_free": (i32, [_P]),
+        "gooseberry_memcpy_htod": (i32, [_P, _P, sz]),
+        "gooseberry_memcpy_dtoh": (i32, [_P, _P, sz]),
+        "gooseberry_device_synchronize": (i32, []),
+        "gooseberry_device_memory": (i32, [C.POINTER(sz), C.POINTER(sz)]),
+        "gooseberry_event_create": (i32, [C.POINTER(_P)]),
+        "gooseberry_event_destroy": (i32, [_P]),
+        "gooseberry_event_record": (i32, [_P]),
+        "gooseberry_event_elapsed_ms": (i32, [_P, _P, C.POINTER(C.c_float)]),
+        "gooseberry_matrix_create": (i32, [u32, u32, _P, _P, _P, C.POINTER(_Plan), C.POINTER(_P)]),
+        "gooseberry_matrix_create_device": (i32, [u32, u32, _P, _P, _P, C.POINTER(_Plan),
+                                                  C.POINTER(_P)]),
+        "gooseberry_matrix_free": (i32, [_P]),
+        "gooseberry_matrix_replan": (i32, [_P, C.POINTER(_Plan)]),
+        "gooseberry_matrix_info": (i32, [_P, C.POINTER(_MatrixInfo)]),
+        "gooseberry_matrix_spmv": (i32, [_P, _P, _P]),
+        "gooseberry_matrix_spmm": (i32, [_P, u32, _P, _P]),
+        "gooseberry_matrix_cgd": (i32, [_P, _P, _P, f64, i32, sz, _P, C.POINTER(CgdInfo)]),
+        "gooseberry_cgd": (i32, [u32, u32, _P, _P, _P, _P, _P, f64, sz, C.POINTER(CgdInfo)]),
+        "gooseberry_generate_laplacian": (i32, [i32, u32, u32, u32, _U32P, C.POINTER(_P),
+                                                C.POINTER(_P), C.POINTER(_P)]),
+        "gooseberry_generate_uniform": (i32, [u64, u64, sz, f64, f64, _P]),
+    }
+    for name, (res, args) in s.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """The loaded libgooseberry (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libgooseberry.so is not built (%s); run "
+                               "python -m gooseberry_amd.build" % LIB_PATH)
+        _LIB = C.CDLL(LIB_PATH)
+        _sig(_LIB)
+    return _LIB
+
+
+def check(rv, what):
+    if rv != SUCCESS:
+        raise GooseberryError(rv, what)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _u32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------ device memory
+
+class DeviceArray:
+    """A device (HBM) buffer holding a 1-D numpy-typed array."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = _P()
+        check(lib().gooseberry_device_alloc(C.byref(p), max(self.nbytes, 8)),
+              "gooseberry_device_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().gooseberry_memcpy_htod(self.ptr, _ptr(a), self.nbytes),
+              "gooseberry_memcpy_htod")
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().gooseberry_memcpy_dtoh(_ptr(out), self.ptr, self.nbytes),
+              "gooseberry_memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gooseberry_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib().gooseberry_device_synchronize(), "gooseberry_device_synchronize")
+
+
+def device_memory():
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().gooseberry_device_memory(C.byref(f), C.byref(t)),
+          "gooseberry_device_memory")
+    return f.value, t.value
+
+
+class Event:
+    """A HIP event recorded on the library stream."""
+
+    def __init__(self):
+        p = _P()
+        check(lib().gooseberry_event_create(C.byref(p)), "gooseberry_event_create")
+        self.ptr = p.value
+
+    def record(self):
+        check(lib().gooseberry_event_record(self.ptr), "gooseberry_event_record")
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float()
+        check(lib().gooseberry_event_elapsed_ms(self.ptr, stop.ptr, C.byref(ms)),
+              "gooseberry_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().gooseberry_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- matrices
+
+def _plan(plan):
+    if plan is None:
+        return None
+    p = _Plan(plan.get("short_max", 0), plan.get("wave_max", 0),
+              plan.get("window", 0), plan.get("rowcap", 0))
+    return C.byref(p)
+
+
+class Matrix:
+    """A CSR matrix resident in HBM with its SpMV plan."""
+
+    def __init__(self, rowptr, rowind, rowval, ncols=None, plan=None):
+        rowptr, rowind, rowval = _u32(rowptr), _u32(rowind), _f64(rowval)
+        nrows = rowptr.size - 1
+        if ncols is None:
+            ncols = nrows
+        h = _P()
+        check(lib().gooseberry_matrix_create(nrows, ncols, _ptr(rowptr), _ptr(rowind),
+                                             _ptr(rowval), _plan(plan), C.byref(h)),
+              "gooseberry_matrix_create")
+        self.h = h.value
+        self.nrows, self.ncols = nrows, ncols
+        self.nnz = int(rowptr[-1])
+
+    def info(self):
+        i = _MatrixInfo()
+        check(lib().gooseberry_matrix_info(self.h, C.byref(i)), "gooseberry_matrix_info")
+        return {"nrows": i.nrows, "ncols": i.ncols, "nnz": i.nnz,
+                "plan": {"short_max": i.plan.short_max, "wave_max": i.plan.wave_max,
+                         "window": i.plan.window, "rowcap": i.plan.rowcap},
+                "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
+                "nrows_block": i.nrows_block, "hist": list(i.hist)}
+
+    def replan(self, plan=None):
+        check(lib().gooseberry_matrix_replan(self.h, _plan(plan)), "gooseberry_matrix_replan")
+
+    def spmv_device(self, d_x, d_y):
+        check(lib().gooseberry_matrix_spmv(self.h, d_x.ptr, d_y.ptr), "gooseberry_matrix_spmv")
+
+    def spmv(self, x):
+        dx = DeviceArray.from_host(x, np.float64)
+        dy = DeviceArray(self.nrows)
+        self.spmv_device(dx, dy)
+        return dy.numpy()
+
+    def spmm(self, bcol, bncols):
+        """C (row-major nrows x bncols) = A B, B column-major (ncols x bncols)."""
+        db = DeviceArray.from_host(np.ascontiguousarray(bcol, dtype=np.float64).ravel())
+        dc = DeviceArray(self.nrows * bncols)
+        check(lib().gooseberry_matrix_spmm(self.h, bncols, db.ptr, dc.ptr),
+              "gooseberry_matrix_spmm")
+        return dc.numpy()
+
+    def cgd_device(self, d_b, d_x, tol, tol_mode=TOL_RELATIVE, niter=0, log=None):
+        info = CgdInfo()
+        rv = lib().gooseberry_matrix_cgd(self.h, d_b.ptr, d_x.ptr, float(tol), tol_mode,
+                                         int(niter), log, C.byref(info))
+        if rv not in (SUCCESS, ERROR_CODES["BREAKDOWN"]):
+            check(rv, "gooseberry_matrix_cgd")
+        return info
+
+    def cgd(self, b, tol, tol_mode=TOL_RELATIVE, niter=0):
+        db = DeviceArray.from_host(b, np.float64)
+        dx = DeviceArray(self.nrows)
+        info = self.cgd_device(db, dx, tol, tol_mode, niter)
+        return dx.numpy(), info
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().gooseberry_matrix_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------- reference entry points
+
+def spmult(anrows, ancols, bncols, rowptr, rowind, rowval, bcolval, blocks=None,
+           crowval=None):
+    """gooseberry_spmult with host arrays; returns crowval (row-major)."""
+    rowptr, rowind, rowval = _u32(rowptr), _u32(rowind), _f64(rowval)
+    bcolval = _f64(bcolval)
+    if crowval is None:
+        crowval = np.zeros(anrows * bncols, dtype=np.float64)
+    blocks = _u32(blocks)
+    nblocks = 0 if blocks is None else blocks.size - 1
+    check(lib().gooseberry_spmult(anrows, ancols, bncols, _ptr(rowptr), _ptr(rowind),
+                                  _ptr(rowval), _ptr(bcolval), _ptr(crowval),
+                                  _ptr(blocks), nblocks), "gooseberry_spmult")
+    return crowval
+
+
+def scale(val, s):
+    val = _f64(val).copy()
+    check(lib().gooseberry_scale(val.size, _ptr(val), float(s)), "gooseberry_scale")
+    return val
+
+
+def cgd(rowptr, rowind, rowval, b, error, niter=0):
+    """The reference's cgd() on host arrays: absolute threshold `error`."""
+    rowptr, rowind, rowval, b = _u32(rowptr), _u32(rowind), _f64(rowval), _f64(b)
+    n = rowptr.size - 1
+    x = np.zeros(n, dtype=np.float64)
+    info = CgdInfo()
+    rv = lib().gooseberry_cgd(n, n, _ptr(rowptr), _ptr(rowind), _ptr(rowval), _ptr(b),
+                              _ptr(x), float(error), int(niter), C.byref(info))
+    if rv not in (SUCCESS, ERROR_CODES["BREAKDOWN"]):
+        check(rv, "gooseberry_cgd")
+    return x, info
+
+
+def countingsort_kv(keys, kmin, kmax, vals=None, want_dist=True):
+    keys = _u32(keys)
+    vals = _u32(vals)
+    out = np.empty(keys.size, dtype=np.uint32)
+    dist = np.empty(int(kmax) - int(kmin) + 2, dtype=np.uint32) if want_dist else None
+    check(lib().gooseberry_countingsort_kv(_ptr(keys), _ptr(vals), kmin, kmax, keys.size,
+                                           _ptr(out), _ptr(dist)),
+          "gooseberry_countingsort_kv")
+    return out, dist
+
+
+def permute_sparse(rowptr, rowind, rowval, ncols, rowperm=None, colperm=None):
+    rowptr, rowind, rowval = _u32(rowptr).copy(), _u32(rowind).copy(), _f64(rowval).copy()
+    check(lib().gooseberry_permute_sparse(rowptr.size - 1, ncols, _ptr(rowptr), _ptr(rowind),
+                                          _ptr(rowval), _ptr(_u32(rowperm)),
+                                          _ptr(_u32(colperm))),
+          "gooseberry_permute_sparse")
+    return rowptr, rowind, rowval
+
+
+def permute_dense(vals, nrows, ncols, rowperm=None, colperm=None, rev=False):
+    vals = _f64(vals).copy()
+    check(lib().gooseberry_permute_dense(nrows, ncols, _ptr(vals), _ptr(_u32(rowperm)),
+                                         _ptr(_u32(colperm)), 1 if rev else 0),
+          "gooseberry_permute_dense")
+    return vals
+
+
+def _take(p, n, dtype):
+    if n == 0:
+        lib().gooseberry_free(p)
+        return np.zeros(0, dtype=dtype)
+    ct = C.c_uint32 if dtype == np.uint32 else C.c_double
+    arr = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n,)).copy()
+    lib().gooseberry_free(p)
+    return arr
+
+
+def read_sparse(path, fmt=FORMAT_CSR):
+    nr, nc = C.c_uint32(), C.c_uint32()
+    rp, ri, rv = _P(), _P(), _P()
+    check(lib().gooseberry_read_sparse_matrix(fmt, path.encode(), C.byref(nr), C.byref(nc),
+                                              C.byref(rp), C.byref(ri), C.byref(rv)),
+          "gooseberry_read_sparse_matrix")
+    rowptr = _take(rp, nr.value + 1, np.uint32)
+    nnz = int(rowptr[-1])
+    return nr.value, nc.value, rowptr, _take(ri, nnz, np.uint32), _take(rv, nnz, np.float64)
+
+
+def read_dense(path):
+    nr, nc = C.c_uint32(), C.c_uint32()
+    rv = _P()
+    check(lib().gooseberry_read_dense_matrix(FORMAT_GRID, path.encode(), C.byref(nr),
+                                             C.byref(nc), C.byref(rv)),
+          "gooseberry_read_dense_matrix")
+    return nr.value, nc.value, _take(rv, nr.value * nc.value, np.float64)
+
+
+def write_dense(path, vals, nrows, ncols):
+    vals = _f64(vals)
+    check(lib().gooseberry_write_dense_matrix(FORMAT_GRID, path.encode(), nrows, ncols,
+                                              _ptr(vals)), "gooseberry_write_dense_matrix")
+
+
+def write_csr(path, rowptr, rowind, rowval, ncols):
+    rowptr, rowind, rowval = _u32(rowptr), _u32(rowind), _f64(rowval)
+    check(lib().gooseberry_write_sparse_matrix(FORMAT_CSR, path.encode(), rowptr.size - 1,
+                                               ncols, _ptr(rowptr), _ptr(rowind),
+                                               _ptr(rowval)),
+          "gooseberry_write_sparse_matrix")
+
+
+def read_labels(path, nrows=0):
+    n = C.c_uint32(nrows)
+    p = _P()
+    check(lib().gooseberry_read_labels(path.encode(), C.byref(n), C.byref(p)),
+          "gooseberry_read_labels")
+    return _take(p, n.value, np.uint32)
+
+
+def read_partition(path, nrows=0):
+    n, k = C.c_uint32(nrows), C.c_uint32()
+    mp, pp, dp = _P(), _P(), _P()
+    check(lib().gooseberry_read_partition(path.encode(), C.byref(n), C.byref(k),
+                                          C.byref(mp), C.byref(pp), C.byref(dp)),
+          "gooseberry_read_partition")
+    return (_take(mp, n.value, np.uint32), _take(pp, n.value, np.uint32),
+            _take(dp, k.value + 1, np.uint32))
+
+
+# ------------------------------------------------------------ generators
+
+def laplacian(dims, nx, ny=None, nz=None):
+    """2D 5-point / 3D 7-point Dirichlet Laplacian as (rowptr, rowind, rowval)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = C.c_uint32()
+    rp, ri, rv = _P(), _P(), _P()
+    check(lib().gooseberry_generate_laplacian(dims, nx, ny, nz, C.byref(n), C.byref(rp),
+                                              C.byref(ri), C.byref(rv)),
+          "gooseberry_generate_laplacian")
+    rowptr = _take(rp, n.value + 1, np.uint32)
+    nnz = int(rowptr[-1])
+    return rowptr, _take(ri, nnz, np.uint32), _take(rv, nnz, np.float64)
+
+
+def uniform(seed, n, lo=-1.0, hi=1.0, stream=0):
+    out = np.empty(n, dtype=np.float64)
+    check(lib().gooseberry_generate_uniform(seed, stream, n, lo, hi, _ptr(out)),
+          "gooseberry_generate_uniform")
+    return out

@@ -1,0 +1,154 @@
+/*
+ * gb_device.h -- the thin C ABI between the host C code (csrc/host) and
+ * the HIP kernels (csrc/hip).  Plain pointers and sizes only.
+ *
+ * Every gbd_* function returns a gooseberry_error_t value
+ * (GOOSEBERRY_SUCCESS == 1).  Pointers named d_* are device (HBM) pointers.
+ * All work is enqueued on one library stream per process (created on first
+ * use on device 0); functions that hand results to the host synchronise
+ * that stream themselves.
+ */
+#ifndef GB_DEVICE_H
+#define GB_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gooseberry.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ runtime */
+int gbd_init(void);
+int gbd_malloc(void **d_ptr, size_t bytes);
+int gbd_free(void *d_ptr);
+int gbd_host_alloc(void **h_ptr, size_t bytes); /* pinned host memory */
+int gbd_host_free(void *h_ptr);
+int gbd_htod(void *d_dst, const void *h_src, size_t bytes);
+int gbd_dtoh(void *h_dst, const void *d_src, size_t bytes);
+int gbd_dtod(void *d_dst, const void *d_src, size_t bytes);
+int gbd_zero(void *d_dst, size_t bytes); /* fill kernel, safe under capture */
+/* d[i] *= s (src/blas.c:244-257) */
+int gbd_scale(double *d, size_t n, double s);
+int gbd_sync(void);
+void *gbd_stream(void); /* the library stream as a hipStream_t */
+int gbd_device_memory(size_t *free_bytes, size_t *total_bytes);
+const char *gbd_last_error(void);
+
+/* events on the library stream (timing) */
+int gbd_event_create(void **ev);
+int gbd_event_destroy(void *ev);
+int gbd_event_record(void *ev);
+int gbd_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ------------------------------------------------------ index work */
+/* out = exclusive prefix sum of in (n entries); out may alias in.  When
+ * d_total is non-NULL it receives the sum of all n entries. */
+int gbd_scan_u32(const uint32_t *d_in, uint32_t *d_out, size_t n,
+    uint32_t *d_total);
+/* Stable counting sort of d_vals by d_keys (keys in [kmin, kmax]).
+ * d_out[n] receives vals in key order, stable in input order.  d_dist
+ * (optional, kmax-kmin+2 entries) receives the start of each key's run and
+ * n at the end.  d_vals == NULL means vals = 0..n-1. */
+int gbd_countingsort_kv(const uint32_t *d_keys, const uint32_t *d_vals,
+    uint32_t kmin, uint32_t kmax, size_t n, uint32_t *d_out,
+    uint32_t *d_dist);
+/* src/permute.c:120-205 semantics on device arrays.  Writes the permuted
+ * CSR into d_prowptr/d_prowind/d_prowval (distinct from the inputs). */
+int gbd_permute_sparse(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
+    const uint32_t *d_rowind, const double *d_rowval,
+    const uint32_t *d_rowperm, const uint32_t *d_colperm,
+    uint32_t *d_prowptr, uint32_t *d_prowind, double *d_prowval);
+/* src/permute.c:208-285: rev == 0 gathers (new[i][j] =
+ * old[rp[i]][cp[j]]), rev == 1 scatters (old[rp[i]][cp[j]] = new[i][j]).
+ * d_out is distinct from d_in. */
+int gbd_permute_dense(uint32_t nrows, uint32_t ncols, const double *d_in,
+    double *d_out, const uint32_t *d_rowperm, const uint32_t *d_colperm,
+    int rev);
+
+/* ------------------------------------------------------------- SpMV */
+typedef struct gbd_plan_params {
+  uint32_t short_max;  /* rows with len <= short_max use LDS-staged blocks */
+  uint32_t wave_max;   /* short_max < len <= wave_max: wavefront per row;
+                          longer rows: one workgroup per row */
+  uint32_t window;     /* nnz window of an LDS-staged row block */
+  uint32_t rowcap;     /* max rows per LDS-staged row block */
+} gbd_plan_params;
+
+/* Default parameters for a matrix with the given row-length histogram
+ * (hist[b] = number of rows with floor(log2(len+1)) == b, 33 bins). */
+void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
+    gbd_plan_params *p);
+
+typedef struct gbd_matrix gbd_matrix;
+
+/* Takes ownership of the device arrays d_rowptr (nrows+1), d_rowind and
+ * d_rowval (nnz each; freed by gbd_matrix_destroy) and builds the SpMV plan
+ * on the device.  params == NULL selects gbd_plan_defaults(). */
+int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
+    uint32_t *d_rowind, double *d_rowval, const gbd_plan_params *params,
+    gbd_matrix **out);
+int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params);
+void gbd_matrix_destroy(gbd_matrix *m);
+int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
+    uint64_t *nnz, gbd_plan_params *params, uint32_t *nblocks_stream,
+    uint32_t *nrows_wave, uint32_t *nrows_block, uint64_t *hist33);
+const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m);
+const uint32_t *gbd_matrix_rowind(const gbd_matrix *m);
+const double *gbd_matrix_rowval(const gbd_matrix *m);
+/* FP32 copy of the values for the mixed-precision solver (made on first
+ * request, kept with the matrix). */
+int gbd_matrix_make_f32(gbd_matrix *m);
+
+/* y = A x (FP64).  bitwise the reference's row-sequential sums for rows in
+ * LDS-staged blocks; tree-reduced for longer rows. */
+int gbd_spmv(const gbd_matrix *m, const double *d_x, double *d_y);
+/* C = A B with B column-major (ncols x bncols) and C row-major
+ * (nrows x bncols), src/blas.c:67-92 layout. */
+int gbd_spmm(const gbd_matrix *m, uint32_t bncols, const double *d_b,
+    double *d_c);
+/* y = A x in FP32 values/vectors (FP32 accumulate). */
+int gbd_spmv_f32(const gbd_matrix *m, const float *d_x, float *d_y);
+
+/* --------------------------------------------------------------- CG */
+typedef struct gbd_cg_state {
+  double dr;       /* r.r of the current residual */
+  double pap;      /* p.Ap of the current iteration */
+  double alpha;
+  double beta;
+  double rnorm;    /* sqrt(dr) */
+  double rnorm0;   /* sqrt(b.b) */
+  double tol;      /* absolute stopping threshold on rnorm */
+  uint64_t iter;   /* completed iterations (SpMVs) */
+  uint64_t maxiter;/* 0 = unbounded */
+  int32_t done;    /* 1: stop (converged, breakdown or maxiter) */
+  int32_t status;  /* 0 running, 1 converged, 2 maxiter, 3 breakdown */
+  uint32_t ticket[4];
+} gbd_cg_state;
+
+typedef struct gbd_cg gbd_cg;
+
+/* Workspace for solving with m (vectors r, p, ap, partials, state). */
+int gbd_cg_create(const gbd_matrix *m, gbd_cg **out);
+void gbd_cg_destroy(gbd_cg *cg);
+/* x = 0, r = b, p = b, dr = b.b; tol is absolute when rel == 0, else
+ * relative to ||b||_2.  trace_cap > 0 records rnorm of each iteration in a
+ * ring of trace_cap entries (iteration i at slot i % trace_cap). */
+int gbd_cg_start(gbd_cg *cg, const double *d_b, double *d_x, double tol,
+    int rel, uint64_t maxiter, uint32_t trace_cap);
+/* Enqueue k iterations (no host synchronisation). */
+int gbd_cg_enqueue(gbd_cg *cg, uint32_t k);
+/* Synchronise and copy the state (and, when h_trace != NULL, the trace
+ * ring) to the host. */
+int gbd_cg_poll(gbd_cg *cg, gbd_cg_state *st, double *h_trace);
+/* Capture k iterations into a hipGraph once and replay it for each
+ * gbd_cg_enqueue_graph call. */
+int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

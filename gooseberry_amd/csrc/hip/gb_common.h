@@ -1,0 +1,81 @@
+// gb_common.h -- shared device helpers for the gfx950 kernels.
+#ifndef GB_COMMON_H
+#define GB_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../gb_device.h"
+
+#define GB_WAVE 64
+#define GB_BLOCK 256
+#define GB_WAVES_PER_BLOCK (GB_BLOCK / GB_WAVE)
+
+// Record a HIP failure and map it to GOOSEBERRY_ERROR_DEVICE.
+int gb_hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define GB_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t gb_e_ = (call);                                              \
+    if (gb_e_ != hipSuccess)                                                \
+      return gb_hip_fail(gb_e_, #call, __FILE__, __LINE__);                 \
+  } while (0)
+#define GB_LAUNCH_CHECK() GB_HIP(hipGetLastError())
+
+hipStream_t gb_stream();
+int gb_ensure_init();
+
+static inline uint32_t gb_div_up(uint64_t a, uint64_t b) {
+  return (uint32_t)((a + b - 1) / b);
+}
+
+// Butterfly sum over the 64 lanes of a wavefront; every lane gets the same
+// value and the combine order is fixed, so the result is deterministic.
+template <typename T>
+__device__ __forceinline__ T gb_wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, GB_WAVE);
+  return v;
+}
+
+// Sum over a 256-thread block in a fixed order: wave butterflies, then the
+// four wave totals added in wave order by thread 0.  Valid on thread 0.
+template <typename T>
+__device__ __forceinline__ T gb_block_sum(T v, T *lds4) {
+  v = gb_wave_sum(v);
+  const int lane = threadIdx.x & (GB_WAVE - 1);
+  const int wave = threadIdx.x / GB_WAVE;
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0) {
+    s = lds4[0];
+#pragma unroll
+    for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) s += lds4[w];
+  }
+  return s;
+}
+
+// Inclusive scan over a wavefront.
+__device__ __forceinline__ uint32_t gb_wave_scan_incl(uint32_t v) {
+  const int lane = threadIdx.x & (GB_WAVE - 1);
+#pragma unroll
+  for (int off = 1; off < GB_WAVE; off <<= 1) {
+    uint32_t y = __shfl_up(v, off, GB_WAVE);
+    if (lane >= off) v += y;
+  }
+  return v;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8).  Map b to a logical index so that each XCD walks one
+// contiguous range of [0, n): neighbouring row blocks then share one XCD's
+// L2 for the x gather.  A bijection on [0, n) for any n.
+__device__ __forceinline__ uint32_t gb_xcd_remap(uint32_t b, uint32_t n) {
+  const uint32_t nx = 8;
+  if (n < 2 * nx) return b;
+  const uint32_t q = n / nx, r = n % nx;
+  const uint32_t x = b % nx, local = b / nx;
+  return x * q + (x < r ? x : r) + local;
+}
+
+#endif

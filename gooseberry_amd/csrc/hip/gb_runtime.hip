@@ -1,0 +1,202 @@
+// gb_runtime.hip -- device selection, the library stream, memory, events.
+#include <pthread.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "gb_common.h"
+
+static hipStream_t g_stream = nullptr;
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+static int g_init_rv = GOOSEBERRY_SUCCESS;
+static char g_last_error[512] = "";
+
+int gb_hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)",
+           hipGetErrorString(e), what, file, line);
+  fprintf(stderr, "gooseberry: HIP error %s\n", g_last_error);
+  if (e == hipErrorOutOfMemory) return GOOSEBERRY_ERROR_NOMEMORY;
+  return GOOSEBERRY_ERROR_DEVICE;
+}
+
+static void gb_do_init() {
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) {
+    g_init_rv = gb_hip_fail(e != hipSuccess ? e : hipErrorNoDevice,
+                            "hipGetDeviceCount (no GPU visible)", __FILE__,
+                            __LINE__);
+    return;
+  }
+  e = hipSetDevice(0);
+  if (e != hipSuccess) {
+    g_init_rv = gb_hip_fail(e, "hipSetDevice(0)", __FILE__, __LINE__);
+    return;
+  }
+  // default priority, non-blocking with respect to the null stream
+  e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_init_rv = gb_hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    return;
+  }
+}
+
+int gb_ensure_init() {
+  pthread_once(&g_once, gb_do_init);
+  return g_init_rv;
+}
+
+hipStream_t gb_stream() { return g_stream; }
+
+__global__ void gb_fill_zero_kernel(uint32_t *p, size_t nwords) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+__global__ void gb_fill_zero_bytes_kernel(uint8_t *p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void gb_scale_kernel(double *__restrict__ d, size_t n, double s) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] *= s;
+}
+
+extern "C" {
+
+int gbd_scale(double *d, size_t n, double s) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!n) return GOOSEBERRY_SUCCESS;
+  uint32_t grid = gb_div_up(n, GB_BLOCK);
+  if (grid > 65536u) grid = 65536u;
+  gb_scale_kernel<<<grid, GB_BLOCK, 0, g_stream>>>(d, n, s);
+  GB_LAUNCH_CHECK();
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_init(void) { return gb_ensure_init(); }
+
+const char *gbd_last_error(void) { return g_last_error; }
+
+void *gbd_stream(void) {
+  if (gb_ensure_init() != GOOSEBERRY_SUCCESS) return nullptr;
+  return (void *)g_stream;
+}
+
+int gbd_malloc(void **d_ptr, size_t bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  *d_ptr = nullptr;
+  GB_HIP(hipMalloc(d_ptr, bytes ? bytes : 16));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_free(void *d_ptr) {
+  if (!d_ptr) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipFree(d_ptr));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_host_alloc(void **h_ptr, size_t bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  GB_HIP(hipHostMalloc(h_ptr, bytes ? bytes : 16, hipHostMallocDefault));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_host_free(void *h_ptr) {
+  if (!h_ptr) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipHostFree(h_ptr));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_htod(void *d_dst, const void *h_src, size_t bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!bytes) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, g_stream));
+  GB_HIP(hipStreamSynchronize(g_stream));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_dtoh(void *h_dst, const void *d_src, size_t bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!bytes) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, g_stream));
+  GB_HIP(hipStreamSynchronize(g_stream));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_dtod(void *d_dst, const void *d_src, size_t bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!bytes) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_zero(void *d_dst, size_t bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!bytes) return GOOSEBERRY_SUCCESS;
+  if (((uintptr_t)d_dst & 3u) == 0 && (bytes & 3u) == 0) {
+    size_t nw = bytes / 4;
+    uint32_t grid = gb_div_up(nw, GB_BLOCK);
+    if (grid > 65536u) grid = 65536u;
+    gb_fill_zero_kernel<<<grid, GB_BLOCK, 0, g_stream>>>((uint32_t *)d_dst, nw);
+  } else {
+    gb_fill_zero_bytes_kernel<<<gb_div_up(bytes, GB_BLOCK), GB_BLOCK, 0,
+                                g_stream>>>((uint8_t *)d_dst, bytes);
+  }
+  GB_LAUNCH_CHECK();
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_sync(void) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  GB_HIP(hipStreamSynchronize(g_stream));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_device_memory(size_t *free_bytes, size_t *total_bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  size_t f = 0, t = 0;
+  GB_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_event_create(void **ev) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  hipEvent_t e;
+  GB_HIP(hipEventCreate(&e));
+  *ev = (void *)e;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_event_destroy(void *ev) {
+  if (!ev) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipEventDestroy((hipEvent_t)ev));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_event_record(void *ev) {
+  GB_HIP(hipEventRecord((hipEvent_t)ev, g_stream));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_event_elapsed_ms(void *start, void *stop, float *ms) {
+  GB_HIP(hipEventSynchronize((hipEvent_t)stop));
+  GB_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return GOOSEBERRY_SUCCESS;
+}
+
+}  // extern "C"

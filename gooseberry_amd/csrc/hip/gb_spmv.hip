@@ -1,0 +1,509 @@
+// gb_spmv.hip -- CSR SpMV for gfx950 with row-length binning.
+//
+// Reference: src/blas.c:67-92 (__sparse_mult_dense_block) driven by
+// src/blas.c:552-606 (blas_spmult).  Rows are binned by length on the GPU
+// (stable counting sort of row ids by bin) into
+//   * short rows (len <= short_max): LDS-staged row blocks.  A block is a
+//     contiguous run of short rows whose nonzeros start inside one window
+//     of `window` nonzeros (at most rowcap rows).  The block streams its
+//     values/indices with fully coalesced loads, stages the products
+//     val*x[col] in LDS, then one thread per row adds them left to right
+//     from +0.0 -- the reference's exact operation order, so these rows are
+//     bitwise identical to the reference (no FMA: this file is compiled
+//     with -ffp-contract=off).
+//   * medium rows (short_max < len <= wave_max): one wavefront per row,
+//     lanes stride the row, 64-lane butterfly reduction.
+//   * long rows (len > wave_max): one 256-thread workgroup per row.
+// All three run in one launch: blockIdx ranges select the bin.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gb_matrix.h"
+
+// ------------------------------------------------------------- plan build
+
+__device__ __forceinline__ uint32_t gb_lower_bound(const uint32_t *a,
+                                                   uint32_t n, uint32_t v) {
+  // first i in [0, n) with a[i] >= v (n if none)
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t gb_row_class(uint32_t len,
+                                                 const gbd_plan_params p) {
+  return len <= p.short_max ? 0u : (len <= p.wave_max ? 1u : 2u);
+}
+
+// Per row: bin (0 short / 1 wave / 2 block), the "starts a row block" flag
+// and the log2 row-length histogram.
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_plan_classify(const uint32_t *__restrict__ rowptr, uint32_t nrows,
+                 gbd_plan_params p, uint32_t *__restrict__ cls,
+                 uint32_t *__restrict__ flag,
+                 unsigned long long *__restrict__ hist) {
+  __shared__ uint32_t h[33];
+  if (threadIdx.x < 33) h[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t r = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (r < nrows) {
+    uint32_t a = rowptr[r], b = rowptr[r + 1];
+    uint32_t len = b - a;
+    uint32_t c = gb_row_class(len, p);
+    cls[r] = c;
+    // floor(log2(len + 1)); len + 1 wraps to 0 only for len = 2^32 - 1
+    uint32_t bin = (len == 0xffffffffu) ? 32u : 31u - __clz(len + 1);
+    atomicAdd(&h[bin], 1u);
+    uint32_t f = 0;
+    if (c == 0) {
+      if (r == 0) {
+        f = 1;
+      } else {
+        uint32_t pa = rowptr[r - 1];
+        uint32_t pc = gb_row_class(a - pa, p);
+        uint32_t w = a / p.window;
+        if (pc != 0 || pa / p.window != w) {
+          f = 1;
+        } else {
+          uint32_t first = gb_lower_bound(rowptr, nrows + 1, w * p.window);
+          if ((r - first) % p.rowcap == 0) f = 1;
+        }
+      }
+    }
+    flag[r] = f;
+  }
+  __syncthreads();
+  if (threadIdx.x < 33 && h[threadIdx.x])
+    atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// Structural check of a device CSR before any kernel indexes through it:
+// rowptr[0] == 0, non-decreasing, and every column index < ncols.
+__global__ void gb_validate_csr(const uint32_t *__restrict__ rowptr,
+                                const uint32_t *__restrict__ rowind,
+                                uint32_t nrows, uint32_t ncols, uint32_t nnz,
+                                uint32_t *__restrict__ bad) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  uint32_t b = 0;
+  for (size_t r = i; r < nrows; r += stride) {
+    uint32_t a = rowptr[r], e = rowptr[r + 1];
+    if (e < a || e > nnz) b = 1;
+    if (r == 0 && a != 0) b = 1;
+  }
+  for (size_t k = i; k < nnz; k += stride)
+    if (rowind[k] >= ncols) b = 1;
+  if (b) atomicOr(bad, 1u);
+}
+
+// idx = exclusive scan of flag.  Block id of a short row r is
+// idx[r] + flag[r] - 1.
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_plan_blocks(const uint32_t *__restrict__ cls,
+               const uint32_t *__restrict__ flag,
+               const uint32_t *__restrict__ idx, uint32_t nrows,
+               uint32_t *__restrict__ blk) {
+  uint32_t r = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (r >= nrows || cls[r] != 0) return;
+  uint32_t id = idx[r] + flag[r] - 1;
+  if (flag[r]) blk[2 * id] = r;
+  bool end = (r + 1 == nrows) || cls[r + 1] != 0 || flag[r + 1];
+  if (end) blk[2 * id + 1] = r + 1;
+}
+
+// --------------------------------------------------------------- kernels
+
+template <typename T>
+__device__ __forceinline__ void gb_cg_alpha_finalize(double s,
+                                                     gbd_cg_state *st) {
+  st->pap = s;
+  double alpha = st->dr / s;
+  st->alpha = alpha;
+  if (!(s != 0.0) || !isfinite(alpha)) {
+    st->done = 1;
+    st->status = 3;
+  }
+}
+
+// One launch covers all bins.  With DOT the kernel also forms p.Ap (x = p,
+// y = Ap) with a fixed-order single-pass grid reduction: each block writes
+// its partial, the last block to arrive (atomic ticket) adds the partials
+// in block order and sets alpha = dr / p.Ap in the CG state.
+template <typename T, bool DOT>
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
+               const uint32_t *__restrict__ rowind,
+               const T *__restrict__ val, const T *__restrict__ x,
+               T *__restrict__ y, uint32_t ystride,
+               const uint32_t *__restrict__ blk, uint32_t nstream,
+               const uint32_t *__restrict__ wave_rows, uint32_t nwave,
+               uint32_t nwaveblocks,
+               const uint32_t *__restrict__ block_rows,
+               double *__restrict__ partials, gbd_cg_state *st) {
+  __shared__ T prod[GB_STREAM_LDS];
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  __shared__ T tred[GB_WAVES_PER_BLOCK];
+  __shared__ uint32_t is_last;
+  if (DOT) {
+    if (st->done) return;
+  }
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & (GB_WAVE - 1);
+  const uint32_t wave = tid / GB_WAVE;
+  const uint32_t b = blockIdx.x;
+  double dotp = 0.0;
+
+  if (b < nstream) {
+    // ---- LDS-staged row block of short rows
+    const uint32_t lb = gb_xcd_remap(b, nstream);
+    const uint32_t r0 = blk[2 * lb], r1 = blk[2 * lb + 1];
+    const uint32_t k0 = rowptr[r0];
+    const uint32_t cnt = rowptr[r1] - k0;
+    // row bounds of this thread's (at most two) rows, loaded together with
+    // the block's nonzeros so that all loads are in flight at once
+    const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
+    uint32_t pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
+    if (ra < r1) { pa0 = rowptr[ra]; pa1 = rowptr[ra + 1]; }
+    if (rb < r1) { pb0 = rowptr[rb]; pb1 = rowptr[rb + 1]; }
+    if (cnt > 0) {
+      // stage val*x[col] for the block's nonzeros; out-of-range lanes
+      // re-read the last entry (in bounds) and store nothing
+      const uint32_t klast = k0 + cnt - 1;
+      uint32_t c[GB_STREAM_ITERS];
+      T v[GB_STREAM_ITERS], xv[GB_STREAM_ITERS];
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        if (i * GB_BLOCK < cnt) {
+          uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
+          c[i] = __builtin_nontemporal_load(&rowind[k]);
+          v[i] = __builtin_nontemporal_load(&val[k]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        if (i * GB_BLOCK < cnt) xv[i] = x[c[i]];
+      }
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        uint32_t e = i * GB_BLOCK + tid;
+        if (i * GB_BLOCK < cnt && e < cnt) prod[e] = v[i] * xv[i];
+      }
+    }
+    __syncthreads();
+    // one thread per row: the reference's left-to-right sum from +0.0
+    if (ra < r1) {
+      T s = 0;
+      for (uint32_t k = pa0 - k0; k < pa1 - k0; ++k) s += prod[k];
+      y[(size_t)ra * ystride] = s;
+      if (DOT) dotp += (double)x[ra] * (double)s;
+    }
+    if (rb < r1) {
+      T s = 0;
+      for (uint32_t k = pb0 - k0; k < pb1 - k0; ++k) s += prod[k];
+      y[(size_t)rb * ystride] = s;
+      if (DOT) dotp += (double)x[rb] * (double)s;
+    }
+  } else if (b < nstream + nwaveblocks) {
+    // ---- one wavefront per medium row
+    const uint32_t wid = (b - nstream) * GB_WAVES_PER_BLOCK + wave;
+    if (wid < nwave) {
+      const uint32_t row = wave_rows[wid];
+      const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+      T s = 0;
+      for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) {
+        uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+        T v = __builtin_nontemporal_load(&val[k]);
+        s += v * x[c];
+      }
+      s = gb_wave_sum(s);
+      if (lane == 0) {
+        y[(size_t)row * ystride] = s;
+        if (DOT) dotp = (double)x[row] * (double)s;
+      }
+    }
+  } else {
+    // ---- one workgroup per long row
+    const uint32_t row = block_rows[b - nstream - nwaveblocks];
+    const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    T s = 0;
+    for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) {
+      uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+      T v = __builtin_nontemporal_load(&val[k]);
+      s += v * x[c];
+    }
+    s = gb_wave_sum(s);
+    if (lane == 0) tred[wave] = s;
+    __syncthreads();
+    if (tid == 0) {
+      T t = tred[0];
+#pragma unroll
+      for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) t += tred[w];
+      y[(size_t)row * ystride] = t;
+      if (DOT) dotp = (double)x[row] * (double)t;
+    }
+  }
+
+  if (DOT) {
+    double bs = gb_block_sum(dotp, red);
+    if (tid == 0) {
+      partials[b] = bs;
+      __threadfence();
+      uint32_t t = atomicAdd(&st->ticket[0], 1u);
+      is_last = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (is_last) {
+      __threadfence();
+      double s = 0.0;
+      for (uint32_t i = tid; i < gridDim.x; i += GB_BLOCK) s += partials[i];
+      s = gb_block_sum(s, red);
+      if (tid == 0) {
+        gb_cg_alpha_finalize<T>(s, st);
+        st->ticket[0] = 0;
+      }
+    }
+  }
+}
+
+template <typename T>
+static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
+                            T *y, uint32_t ystride, double *partials,
+                            gbd_cg_state *cg) {
+  const uint32_t grid = gb_spmv_grid(m);
+  if (grid == 0) return GOOSEBERRY_SUCCESS;
+  hipStream_t s = gb_stream();
+  const uint32_t *wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
+  const uint32_t *block_rows = m->rowbins + (m->nrows - m->nblockrows);
+  if (cg) {
+    gb_spmv_kernel<T, true><<<grid, GB_BLOCK, 0, s>>>(
+        m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
+        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, partials, cg);
+  } else {
+    gb_spmv_kernel<T, false><<<grid, GB_BLOCK, 0, s>>>(
+        m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
+        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, nullptr,
+        nullptr);
+  }
+  GB_LAUNCH_CHECK();
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gb_spmv_launch(const gbd_matrix *m, const double *x, double *y,
+                   uint32_t ystride, double *partials, gbd_cg_state *cg) {
+  return gb_spmv_launch_t<double>(m, m->val, x, y, ystride, partials, cg);
+}
+
+int gb_spmv_launch_f32(const gbd_matrix *m, const float *x, float *y,
+                       double *partials, gbd_cg_state *cg) {
+  if (!m->val32) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return gb_spmv_launch_t<float>(m, m->val32, x, y, 1, partials, cg);
+}
+
+__global__ void gb_to_f32(const double *__restrict__ in, float *__restrict__ out,
+                          size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+// ------------------------------------------------------------- host side
+
+static int gb_plan_check(const gbd_plan_params *p) {
+  if (p->short_max == 0 || p->window == 0 || p->rowcap == 0) return 0;
+  if ((uint64_t)p->window + p->short_max > GB_STREAM_LDS) return 0;
+  if (p->wave_max < p->short_max) return 0;
+  // the row-block kernel gives each thread at most two rows
+  if (p->rowcap > 2 * GB_BLOCK) return 0;
+  return 1;
+}
+
+static int gb_plan_build(gbd_matrix *m) {
+  hipStream_t s = gb_stream();
+  const uint32_t n = m->nrows;
+  memset(m->hist, 0, sizeof(m->hist));
+  m->nstream = m->nwave = m->nblockrows = 0;
+  if (m->blk) { GB_HIP(hipFree(m->blk)); m->blk = nullptr; }
+  if (m->rowbins) { GB_HIP(hipFree(m->rowbins)); m->rowbins = nullptr; }
+  if (n == 0) return GOOSEBERRY_SUCCESS;
+
+  uint32_t *cls = nullptr, *flag = nullptr, *idx = nullptr, *dist = nullptr;
+  unsigned long long *hist = nullptr;
+  GB_HIP(hipMallocAsync((void **)&cls, (size_t)n * 4, s));
+  GB_HIP(hipMallocAsync((void **)&flag, (size_t)n * 4, s));
+  GB_HIP(hipMallocAsync((void **)&idx, ((size_t)n + 1) * 4, s));
+  GB_HIP(hipMallocAsync((void **)&dist, 4 * 4, s));
+  GB_HIP(hipMallocAsync((void **)&hist, 33 * 8, s));
+  GB_HIP(hipMemsetAsync(hist, 0, 33 * 8, s));
+  gb_plan_classify<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(
+      m->rowptr, n, m->params, cls, flag, hist);
+  GB_LAUNCH_CHECK();
+  GB_HIP(hipMalloc((void **)&m->rowbins, (size_t)n * 4));
+  int rv = gbd_countingsort_kv(cls, nullptr, 0, 2, n, m->rowbins, dist);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  rv = gbd_scan_u32(flag, idx, n, idx + n);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  uint32_t hd[4], nst = 0;
+  unsigned long long hh[33];
+  GB_HIP(hipMemcpyAsync(hd, dist, sizeof(hd), hipMemcpyDeviceToHost, s));
+  GB_HIP(hipMemcpyAsync(&nst, idx + n, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipMemcpyAsync(hh, hist, sizeof(hh), hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < 33; ++i) m->hist[i] = hh[i];
+  m->nstream = nst;
+  m->nwave = hd[2] - hd[1];
+  m->nblockrows = hd[3] - hd[2];
+  if (nst) {
+    GB_HIP(hipMalloc((void **)&m->blk, (size_t)nst * 2 * 4));
+    gb_plan_blocks<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(cls, flag, idx,
+                                                              n, m->blk);
+    GB_LAUNCH_CHECK();
+  }
+  GB_HIP(hipFreeAsync(cls, s));
+  GB_HIP(hipFreeAsync(flag, s));
+  GB_HIP(hipFreeAsync(idx, s));
+  GB_HIP(hipFreeAsync(dist, s));
+  GB_HIP(hipFreeAsync(hist, s));
+  GB_HIP(hipStreamSynchronize(s));
+  return GOOSEBERRY_SUCCESS;
+}
+
+extern "C" {
+
+void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
+                       gbd_plan_params *p) {
+  (void)hist;
+  (void)nrows;
+  (void)nnz;
+  p->short_max = 64;
+  p->wave_max = 4096;
+  p->window = 2048;
+  p->rowcap = 512;
+}
+
+int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
+                      uint32_t *d_rowind, double *d_rowval,
+                      const gbd_plan_params *params, gbd_matrix **out) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  gbd_matrix *m = (gbd_matrix *)calloc(1, sizeof(gbd_matrix));
+  if (!m) return GOOSEBERRY_ERROR_NOMEMORY;
+  m->nrows = nrows;
+  m->ncols = ncols;
+  m->rowptr = d_rowptr;
+  m->rowind = d_rowind;
+  m->val = d_rowval;
+  uint32_t nnz = 0;
+  GB_HIP(hipMemcpyAsync(&nnz, d_rowptr + nrows, 4, hipMemcpyDeviceToHost,
+                        gb_stream()));
+  GB_HIP(hipStreamSynchronize(gb_stream()));
+  m->nnz = nnz;
+  {
+    uint32_t *bad = nullptr, hbad = 0;
+    hipStream_t s = gb_stream();
+    GB_HIP(hipMallocAsync((void **)&bad, 4, s));
+    GB_HIP(hipMemsetAsync(bad, 0, 4, s));
+    uint64_t work = (uint64_t)nrows > nnz ? nrows : nnz;
+    uint32_t g = gb_div_up(work ? work : 1, GB_BLOCK);
+    gb_validate_csr<<<g > 8192u ? 8192u : g, GB_BLOCK, 0, s>>>(
+        d_rowptr, d_rowind, nrows, ncols, nnz, bad);
+    GB_LAUNCH_CHECK();
+    GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+    GB_HIP(hipFreeAsync(bad, s));
+    GB_HIP(hipStreamSynchronize(s));
+    if (hbad) {
+      free(m);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+  }
+  if (params) {
+    m->params = *params;
+  } else {
+    gbd_plan_defaults(nullptr, nrows, nnz, &m->params);
+  }
+  if (!gb_plan_check(&m->params)) {
+    free(m);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  rv = gb_plan_build(m);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    m->rowptr = nullptr; m->rowind = nullptr; m->val = nullptr;
+    gbd_matrix_destroy(m);
+    return rv;
+  }
+  *out = m;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
+  gbd_plan_params p;
+  if (params) p = *params;
+  else gbd_plan_defaults(m->hist, m->nrows, m->nnz, &p);
+  if (!gb_plan_check(&p)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  m->params = p;
+  return gb_plan_build(m);
+}
+
+void gbd_matrix_destroy(gbd_matrix *m) {
+  if (!m) return;
+  hipStreamSynchronize(gb_stream());
+  if (m->rowptr) hipFree(m->rowptr);
+  if (m->rowind) hipFree(m->rowind);
+  if (m->val) hipFree(m->val);
+  if (m->val32) hipFree(m->val32);
+  if (m->blk) hipFree(m->blk);
+  if (m->rowbins) hipFree(m->rowbins);
+  free(m);
+}
+
+int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
+                    uint64_t *nnz, gbd_plan_params *params,
+                    uint32_t *nblocks_stream, uint32_t *nrows_wave,
+                    uint32_t *nrows_block, uint64_t *hist33) {
+  if (nrows) *nrows = m->nrows;
+  if (ncols) *ncols = m->ncols;
+  if (nnz) *nnz = m->nnz;
+  if (params) *params = m->params;
+  if (nblocks_stream) *nblocks_stream = m->nstream;
+  if (nrows_wave) *nrows_wave = m->nwave;
+  if (nrows_block) *nrows_block = m->nblockrows;
+  if (hist33) memcpy(hist33, m->hist, sizeof(m->hist));
+  return GOOSEBERRY_SUCCESS;
+}
+
+const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m) { return m->rowptr; }
+const uint32_t *gbd_matrix_rowind(const gbd_matrix *m) { return m->rowind; }
+const double *gbd_matrix_rowval(const gbd_matrix *m) { return m->val; }
+
+int gbd_matrix_make_f32(gbd_matrix *m) {
+  if (m->val32) return GOOSEBERRY_SUCCESS;
+  GB_HIP(hipMalloc((void **)&m->val32, (m->nnz ? m->nnz : 1) * sizeof(float)));
+  if (m->nnz) {
+    gb_to_f32<<<gb_div_up(m->nnz, GB_BLOCK), GB_BLOCK, 0, gb_stream()>>>(
+        m->val, m->val32, m->nnz);
+    GB_LAUNCH_CHECK();
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_spmv(const gbd_matrix *m, const double *d_x, double *d_y) {
+  return gb_spmv_launch(m, d_x, d_y, 1, nullptr, nullptr);
+}
+
+int gbd_spmm(const gbd_matrix *m, uint32_t bncols, const double *d_b,
+             double *d_c) {
+  for (uint32_t j = 0; j < bncols; ++j) {
+    int rv = gb_spmv_launch(m, d_b + (size_t)j * m->ncols, d_c + j, bncols,
+                            nullptr, nullptr);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_spmv_f32(const gbd_matrix *m, const float *d_x, float *d_y) {
+  return gb_spmv_launch_f32(m, d_x, d_y, nullptr, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,485 @@
+/*
+ * gb_api.c -- public C entry points of libgooseberry (include/gooseberry.h)
+ * on top of the thin device ABI (gb_device.h).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gooseberry.h"
+#include "../gb_device.h"
+#include "gb_host.h"
+
+struct gooseberry_matrix_s {
+  gbd_matrix *m;
+  gbd_cg *cg;
+};
+
+void gooseberry_free(void *ptr) { free(ptr); }
+
+const char *gooseberry_strerror(int err)
+{
+  switch (err) {
+    case GOOSEBERRY_SUCCESS: return "success";
+    case GOOSEBERRY_ERROR_UNIMPLEMENTED: return "unimplemented";
+    case GOOSEBERRY_ERROR_INVALIDINPUT: return "invalid input";
+    case GOOSEBERRY_ERROR_READ: return "read error";
+    case GOOSEBERRY_ERROR_WRITE: return "write error";
+    case GOOSEBERRY_ERROR_FILENOTFOUND: return "file not found";
+    case GOOSEBERRY_ERROR_PERMISSIONDENIED: return "permission denied";
+    case GOOSEBERRY_ERROR_UNKNOWN: return "unknown error";
+    case GOOSEBERRY_ERROR_DEVICE: return "device (HIP) error";
+    case GOOSEBERRY_ERROR_NOMEMORY: return "out of device memory";
+    case GOOSEBERRY_ERROR_BREAKDOWN: return "CG breakdown";
+    default: return "unrecognised error code";
+  }
+}
+
+const char *gooseberry_last_device_error(void) { return gbd_last_error(); }
+
+/* --------------------------------------------------------- device memory */
+
+int gooseberry_device_alloc(void **d_ptr, size_t bytes)
+{
+  if (!d_ptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return gbd_malloc(d_ptr, bytes);
+}
+
+int gooseberry_device_free(void *d_ptr) { return gbd_free(d_ptr); }
+
+int gooseberry_memcpy_htod(void *d_dst, const void *h_src, size_t bytes)
+{
+  return gbd_htod(d_dst, h_src, bytes);
+}
+
+int gooseberry_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes)
+{
+  return gbd_dtoh(h_dst, d_src, bytes);
+}
+
+int gooseberry_device_synchronize(void) { return gbd_sync(); }
+
+int gooseberry_device_memory(size_t *free_bytes, size_t *total_bytes)
+{
+  return gbd_device_memory(free_bytes, total_bytes);
+}
+
+int gooseberry_event_create(void **ev) { return gbd_event_create(ev); }
+int gooseberry_event_destroy(void *ev) { return gbd_event_destroy(ev); }
+int gooseberry_event_record(void *ev) { return gbd_event_record(ev); }
+int gooseberry_event_elapsed_ms(void *a, void *b, float *ms)
+{
+  return gbd_event_elapsed_ms(a, b, ms);
+}
+
+/* upload a host array into a fresh device buffer */
+int gb_upload(void **d, const void *h, size_t bytes)
+{
+  int rv = gbd_malloc(d, bytes);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  rv = gbd_htod(*d, h, bytes);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(*d);
+    *d = NULL;
+  }
+  return rv;
+}
+
+/* --------------------------------------------------------------- scale */
+
+int gooseberry_scale(ind_t n, real_t *val, real_t s)
+{
+  if (n == 0) return GOOSEBERRY_SUCCESS;
+  if (!val) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *d = NULL;
+  int rv = gb_upload(&d, val, (size_t)n * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_scale(d, n, s);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(val, d, (size_t)n * sizeof(real_t));
+  gbd_free(d);
+  return rv;
+}
+
+/* ------------------------------------------------------------ matrices */
+
+static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
+    gbd_plan_params *out, int *have)
+{
+  *have = 0;
+  if (!p) return;
+  gbd_plan_params d;
+  gbd_plan_defaults(NULL, 0, 0, &d);
+  (void)m;
+  if (p->short_max) d.short_max = p->short_max;
+  if (p->wave_max) d.wave_max = p->wave_max;
+  if (p->window) d.window = p->window;
+  if (p->rowcap) d.rowcap = p->rowcap;
+  *out = d;
+  *have = 1;
+}
+
+/* validate a host CSR: monotone rowptr starting at 0, indices < ncols */
+int gb_check_csr(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind)
+{
+  if (!rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (rowptr[0] != 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  for (dim_t i = 0; i < nrows; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  ind_t nnz = rowptr[nrows];
+  if (nnz && !rowind) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  for (ind_t k = 0; k < nnz; ++k) {
+    if (rowind[k] >= ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+static int wrap_matrix(gbd_matrix *m, gooseberry_matrix_t *r_mat)
+{
+  struct gooseberry_matrix_s *h = calloc(1, sizeof(*h));
+  if (!h) {
+    gbd_matrix_destroy(m);
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  h->m = m;
+  *r_mat = h;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_matrix_create(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval, const gooseberry_plan_t *plan,
+    gooseberry_matrix_t *r_mat)
+{
+  if (!r_mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int rv = gb_check_csr(nrows, ncols, rowptr, rowind);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  size_t nnz = rowptr[nrows];
+  if (nnz && !rowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *dp = NULL, *di = NULL, *dv = NULL;
+  rv = gb_upload(&dp, rowptr, ((size_t)nrows + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&di, rowind, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dv, rowval, nnz * sizeof(real_t));
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  gbd_plan_params pp;
+  int have;
+  plan_to_dev(plan, NULL, &pp, &have);
+  gbd_matrix *m = NULL;
+  rv = gbd_matrix_create(nrows, ncols, dp, di, dv, have ? &pp : NULL, &m);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  return wrap_matrix(m, r_mat);
+}
+
+int gooseberry_matrix_create_device(dim_t nrows, dim_t ncols,
+    const ind_t *d_rowptr, const dim_t *d_rowind, const real_t *d_rowval,
+    const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
+{
+  if (!r_mat || !d_rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  ind_t nnz = 0;
+  int rv = gbd_dtoh(&nnz, d_rowptr + nrows, sizeof(ind_t));
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  void *dp = NULL, *di = NULL, *dv = NULL;
+  rv = gbd_malloc(&dp, ((size_t)nrows + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&di, (size_t)nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dv, (size_t)nnz * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtod(dp, d_rowptr, ((size_t)nrows + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtod(di, d_rowind, (size_t)nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtod(dv, d_rowval, (size_t)nnz * sizeof(real_t));
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  gbd_plan_params pp;
+  int have;
+  plan_to_dev(plan, NULL, &pp, &have);
+  gbd_matrix *m = NULL;
+  rv = gbd_matrix_create(nrows, ncols, dp, di, dv, have ? &pp : NULL, &m);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  return wrap_matrix(m, r_mat);
+}
+
+int gooseberry_matrix_free(gooseberry_matrix_t mat)
+{
+  if (!mat) return GOOSEBERRY_SUCCESS;
+  gbd_cg_destroy(mat->cg);
+  gbd_matrix_destroy(mat->m);
+  free(mat);
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_matrix_replan(gooseberry_matrix_t mat,
+    const gooseberry_plan_t *plan)
+{
+  if (!mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_plan_params pp;
+  int have;
+  plan_to_dev(plan, mat->m, &pp, &have);
+  /* the CG workspace is sized by the launch grid: rebuild it lazily */
+  gbd_cg_destroy(mat->cg);
+  mat->cg = NULL;
+  return gbd_matrix_replan(mat->m, have ? &pp : NULL);
+}
+
+int gooseberry_matrix_info(gooseberry_matrix_t mat,
+    gooseberry_matrix_info_t *info)
+{
+  if (!mat || !info) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_plan_params p;
+  memset(info, 0, sizeof(*info));
+  gbd_matrix_info(mat->m, &info->nrows, &info->ncols, &info->nnz, &p,
+      &info->nblocks_stream, &info->nrows_wave, &info->nrows_block,
+      info->hist);
+  info->plan.short_max = p.short_max;
+  info->plan.wave_max = p.wave_max;
+  info->plan.window = p.window;
+  info->plan.rowcap = p.rowcap;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_matrix_spmv(gooseberry_matrix_t mat, const real_t *d_x,
+    real_t *d_y)
+{
+  if (!mat || !d_x || !d_y) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return gbd_spmv(mat->m, d_x, d_y);
+}
+
+int gooseberry_matrix_spmm(gooseberry_matrix_t mat, dim_t bncols,
+    const real_t *d_b, real_t *d_c)
+{
+  if (!mat || !d_b || !d_c) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return gbd_spmm(mat->m, bncols, d_b, d_c);
+}
+
+gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat) { return mat->m; }
+
+/* -------------------------------------------------------------- spmult */
+
+/* src/blas.c:552-606 through the GPU.  With blocks, the reference computes
+ * only the rows inside [blocks[0], blocks[nblocks]) (the union of the
+ * blocks of a non-decreasing blocks array); other rows of crowval are left
+ * untouched, as there. */
+int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
+    const ind_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const real_t *bcolval, real_t *crowval, const dim_t *blocks,
+    dim_t nblocks)
+{
+  dim_t rlo = 0, rhi = anrows;
+  if (blocks) {
+    for (dim_t c = 0; c < nblocks; ++c) {
+      if (blocks[c + 1] < blocks[c]) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+    if (nblocks && blocks[nblocks] > anrows) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    rlo = nblocks ? blocks[0] : 0;
+    rhi = nblocks ? blocks[nblocks] : 0;
+  }
+  if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
+  if (!bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gooseberry_matrix_t mat = NULL;
+  int rv = gooseberry_matrix_create(anrows, ancols, arowptr, arowind, arowval,
+      NULL, &mat);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  void *db = NULL, *dc = NULL;
+  size_t bbytes = (size_t)ancols * bncols * sizeof(real_t);
+  size_t cbytes = (size_t)anrows * bncols * sizeof(real_t);
+  rv = gb_upload(&db, bcolval, bbytes);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dc, cbytes);
+  if (rv == GOOSEBERRY_SUCCESS) {
+    rv = (bncols == 1) ? gbd_spmv(mat->m, db, dc)
+                       : gbd_spmm(mat->m, bncols, db, dc);
+  }
+  if (rv == GOOSEBERRY_SUCCESS) {
+    size_t off = (size_t)rlo * bncols;
+    rv = gbd_dtoh(crowval + off, (real_t *)dc + off,
+        (size_t)(rhi - rlo) * bncols * sizeof(real_t));
+  }
+  gbd_free(db);
+  gbd_free(dc);
+  gooseberry_matrix_free(mat);
+  return rv;
+}
+
+/* ------------------------------------------------------------------- CG */
+
+/* iterations per enqueued batch (captured once into a hipGraph) */
+#define GB_CG_BATCH 32
+#define GB_CG_TRACE 64
+
+int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
+    real_t *d_x, real_t tol, int tol_mode, size_t niter, void *log,
+    gooseberry_cgd_info_t *info)
+{
+  if (!mat || !d_b || !d_x) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (tol_mode != GOOSEBERRY_TOL_ABSOLUTE && tol_mode != GOOSEBERRY_TOL_RELATIVE)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t nrows, ncols;
+  gbd_matrix_info(mat->m, &nrows, &ncols, NULL, NULL, NULL, NULL, NULL, NULL);
+  if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int rv;
+  if (!mat->cg) {
+    rv = gbd_cg_create(mat->m, &mat->cg);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  FILE *flog = (FILE *)log;
+  rv = gbd_cg_start(mat->cg, d_b, d_x, tol,
+      tol_mode == GOOSEBERRY_TOL_RELATIVE, niter, GB_CG_TRACE);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  gbd_cg_state st;
+  double trace[GB_CG_TRACE];
+  uint64_t printed = 0;
+  rv = gbd_cg_poll(mat->cg, &st, NULL);
+  while (rv == GOOSEBERRY_SUCCESS && !st.done) {
+    rv = gbd_cg_enqueue_graph(mat->cg, GB_CG_BATCH);
+    if (rv != GOOSEBERRY_SUCCESS) break;
+    rv = gbd_cg_poll(mat->cg, &st, flog ? trace : NULL);
+    if (rv != GOOSEBERRY_SUCCESS) break;
+    if (flog) {
+      for (; printed < st.iter; ++printed) {
+        /* src/cgd.c:116 */
+        fprintf(flog, "ITER: %zu RMSE: %lf\n", (size_t)printed,
+            trace[printed % GB_CG_TRACE]);
+      }
+    }
+  }
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (info) {
+    memset(info, 0, sizeof(*info));
+    info->niter = st.iter;
+    info->rnorm = st.rnorm;
+    info->bnorm = st.rnorm0;
+    info->converged = st.status == 1;
+    info->status = st.status;
+    info->true_rel_residual = NAN;
+  }
+  return st.status == 3 ? GOOSEBERRY_ERROR_BREAKDOWN : GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval, const real_t *b, real_t *x,
+    real_t error, size_t niter, gooseberry_cgd_info_t *info)
+{
+  /* src/cgd.c:51-65 shape checks (b and x are nrows/ncols long here) */
+  if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!b || !x) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gooseberry_matrix_t mat = NULL;
+  int rv = gooseberry_matrix_create(nrows, ncols, rowptr, rowind, rowval, NULL,
+      &mat);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  void *db = NULL, *dx = NULL;
+  size_t vb = (size_t)nrows * sizeof(real_t);
+  rv = gb_upload(&db, b, vb);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dx, vb);
+  int rs = GOOSEBERRY_SUCCESS;
+  if (rv == GOOSEBERRY_SUCCESS) {
+    rs = gooseberry_matrix_cgd(mat, db, dx, error, GOOSEBERRY_TOL_ABSOLUTE,
+        niter, NULL, info);
+    if (rs == GOOSEBERRY_SUCCESS || rs == GOOSEBERRY_ERROR_BREAKDOWN)
+      rv = gbd_dtoh(x, dx, vb);
+    else
+      rv = rs;
+  }
+  gbd_free(db);
+  gbd_free(dx);
+  gooseberry_matrix_free(mat);
+  return rv != GOOSEBERRY_SUCCESS ? rv : rs;
+}
+
+/* --------------------------------------------------- index work (host) */
+
+int gooseberry_countingsort_kv(const dim_t *keys, const dim_t *vals,
+    dim_t kmin, dim_t kmax, size_t n, dim_t *out, dim_t *dist)
+{
+  if (kmax < kmin || (n && (!keys || !out))) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *dk = NULL, *dv = NULL, *dout = NULL, *dd = NULL;
+  size_t nd = (size_t)kmax - kmin + 2;
+  int rv = gb_upload(&dk, keys, n * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS && vals) rv = gb_upload(&dv, vals, n * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dout, n * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS && dist) rv = gbd_malloc(&dd, nd * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_countingsort_kv(dk, dv, kmin, kmax, n, dout, dd);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(out, dout, n * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS && dist) rv = gbd_dtoh(dist, dd, nd * sizeof(dim_t));
+  gbd_free(dk); gbd_free(dv); gbd_free(dout); gbd_free(dd);
+  return rv;
+}
+
+static int check_perm(const dim_t *perm, dim_t n)
+{
+  if (!perm) return GOOSEBERRY_SUCCESS;
+  unsigned char *seen = calloc(n ? n : 1, 1);
+  if (!seen) return GOOSEBERRY_ERROR_NOMEMORY;
+  int rv = GOOSEBERRY_SUCCESS;
+  for (dim_t i = 0; i < n; ++i) {
+    if (perm[i] >= n || seen[perm[i]]) { rv = GOOSEBERRY_ERROR_INVALIDINPUT; break; }
+    seen[perm[i]] = 1;
+  }
+  free(seen);
+  return rv;
+}
+
+int gooseberry_permute_sparse(dim_t nrows, dim_t ncols, ind_t *rowptr,
+    dim_t *rowind, real_t *rowval, const dim_t *rowperm, const dim_t *colperm)
+{
+  int rv = gb_check_csr(nrows, ncols, rowptr, rowind);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if ((rv = check_perm(rowperm, nrows)) != GOOSEBERRY_SUCCESS) return rv;
+  if ((rv = check_perm(colperm, ncols)) != GOOSEBERRY_SUCCESS) return rv;
+  if (!rowperm && !colperm) return GOOSEBERRY_SUCCESS;
+  size_t nnz = rowptr[nrows];
+  size_t pb = ((size_t)nrows + 1) * sizeof(ind_t);
+  void *dp = NULL, *di = NULL, *dv = NULL, *drp = NULL, *dcp = NULL;
+  void *op = NULL, *oi = NULL, *ov = NULL;
+  rv = gb_upload(&dp, rowptr, pb);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&di, rowind, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dv, rowval, nnz * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS && rowperm)
+    rv = gb_upload(&drp, rowperm, (size_t)nrows * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS && colperm)
+    rv = gb_upload(&dcp, colperm, (size_t)ncols * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&op, pb);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&oi, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&ov, nnz * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_permute_sparse(nrows, ncols, dp, di, dv, drp, dcp, op, oi, ov);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(rowptr, op, pb);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(rowind, oi, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(rowval, ov, nnz * sizeof(real_t));
+  gbd_free(dp); gbd_free(di); gbd_free(dv); gbd_free(drp); gbd_free(dcp);
+  gbd_free(op); gbd_free(oi); gbd_free(ov);
+  return rv;
+}
+
+int gooseberry_permute_dense(dim_t nrows, dim_t ncols, real_t *rowval,
+    const dim_t *rowperm, const dim_t *colperm, int rev)
+{
+  int rv;
+  if ((rv = check_perm(rowperm, nrows)) != GOOSEBERRY_SUCCESS) return rv;
+  if ((rv = check_perm(colperm, ncols)) != GOOSEBERRY_SUCCESS) return rv;
+  size_t tot = (size_t)nrows * ncols;
+  if ((!rowperm && !colperm) || tot == 0) return GOOSEBERRY_SUCCESS;
+  if (!rowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *din = NULL, *dout = NULL, *drp = NULL, *dcp = NULL;
+  rv = gb_upload(&din, rowval, tot * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dout, tot * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS && rowperm)
+    rv = gb_upload(&drp, rowperm, (size_t)nrows * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS && colperm)
+    rv = gb_upload(&dcp, colperm, (size_t)ncols * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_permute_dense(nrows, ncols, din, dout, drp, dcp, rev ? 1 : 0);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(rowval, dout, tot * sizeof(real_t));
+  gbd_free(din); gbd_free(dout); gbd_free(drp); gbd_free(dcp);
+  return rv;
+}

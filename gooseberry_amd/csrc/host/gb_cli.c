@@ -1,0 +1,609 @@
+/*
+ * gb_cli.c -- the `gooseberry` command-line program for the sparse-solve
+ * path: `gooseberry blas -x multiply ...` and `gooseberry cgd ...`, with the
+ * reference's option letters, stdout lines, output formats and exit codes
+ * (src/gooseberry_bin.c:1509-1857, 1860-2144, 2842-2877).  The compute runs
+ * on the GPU through libgooseberry.
+ *
+ * Exit codes: 0 success, 1 usage / unknown command, 2 operation failed.
+ */
+#define _GNU_SOURCE
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "gooseberry.h"
+
+/* ------------------------------------------------------------ options */
+
+typedef enum { OPT_FLAG, OPT_STRING, OPT_INT, OPT_FLOAT, OPT_CHOICE } opt_kind_t;
+
+typedef struct {
+  int id;
+  char sflag;
+  const char *lflag;
+  const char *desc;
+  opt_kind_t kind;
+  const char *const *choices;
+} opt_t;
+
+typedef struct {
+  int id;
+  const char *s;
+  long long i;
+  double f;
+  int o;
+} arg_t;
+
+static double now(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+typedef struct { double acc, start; int on; } tmr_t;
+static void tmr_start(tmr_t *t) { t->start = now(); t->on = 1; }
+static void tmr_stop(tmr_t *t) { if (t->on) { t->acc += now() - t->start; t->on = 0; } }
+
+/* Accepted forms: -x value, -xvalue, --name value, --name=value. */
+static int parse_args(int argc, char **argv, const opt_t *opts, size_t nopts,
+    arg_t *args, size_t *nargs)
+{
+  size_t n = 0;
+  for (int a = 0; a < argc; ++a) {
+    const char *s = argv[a];
+    const opt_t *o = NULL;
+    const char *val = NULL;
+    if (s[0] == '-' && s[1] == '-') {
+      const char *eq = strchr(s + 2, '=');
+      size_t len = eq ? (size_t)(eq - s - 2) : strlen(s + 2);
+      for (size_t k = 0; k < nopts; ++k) {
+        if (strlen(opts[k].lflag) == len && strncmp(opts[k].lflag, s + 2, len) == 0) {
+          o = &opts[k];
+          break;
+        }
+      }
+      if (eq) val = eq + 1;
+    } else if (s[0] == '-' && s[1]) {
+      for (size_t k = 0; k < nopts; ++k) {
+        if (opts[k].sflag == s[1]) { o = &opts[k]; break; }
+      }
+      if (s[2]) val = s + 2;
+    }
+    if (!o) {
+      fprintf(stderr, "Unknown argument '%s'\n", s);
+      return 0;
+    }
+    arg_t *r = &args[n++];
+    memset(r, 0, sizeof(*r));
+    r->id = o->id;
+    if (o->kind == OPT_FLAG) continue;
+    if (!val) {
+      if (a + 1 >= argc) {
+        fprintf(stderr, "Missing value for '%s'\n", s);
+        return 0;
+      }
+      val = argv[++a];
+    }
+    r->s = val;
+    char *end = NULL;
+    switch (o->kind) {
+      case OPT_INT:
+        r->i = strtoll(val, &end, 10);
+        if (end == val || *end) { fprintf(stderr, "Bad integer '%s'\n", val); return 0; }
+        break;
+      case OPT_FLOAT:
+        r->f = strtod(val, &end);
+        if (end == val || *end) { fprintf(stderr, "Bad number '%s'\n", val); return 0; }
+        break;
+      case OPT_CHOICE:
+        r->o = -1;
+        for (int c = 0; o->choices[c]; ++c) {
+          if (strcmp(o->choices[c], val) == 0) r->o = c;
+        }
+        if (r->o < 0) { fprintf(stderr, "Unknown choice '%s'\n", val); return 0; }
+        break;
+      default:
+        break;
+    }
+  }
+  *nargs = n;
+  return 1;
+}
+
+static void command_usage(const char *prog, const char *cmd, const opt_t *opts,
+    size_t nopts, FILE *f)
+{
+  fprintf(f, "USAGE:\n%s %s [options]\n\nOptions:\n", prog, cmd);
+  for (size_t k = 0; k < nopts; ++k) {
+    fprintf(f, "  -%c, --%s\n    %s\n", opts[k].sflag, opts[k].lflag, opts[k].desc);
+  }
+}
+
+/* src/gooseberry_bin.c:440-470: format by file extension */
+static int file_type(const char *name)
+{
+  static const struct { const char *ext; int type; } tbl[] = {
+    {"raw", GOOSEBERRY_FORMAT_RAW}, {"mat", GOOSEBERRY_FORMAT_GRID},
+    {"grid", GOOSEBERRY_FORMAT_GRID}, {"vec", GOOSEBERRY_FORMAT_GRID},
+    {"txt", GOOSEBERRY_FORMAT_GRID}, {"csr", GOOSEBERRY_FORMAT_CSR},
+    {"svm", GOOSEBERRY_FORMAT_SVM}, {"libfm", GOOSEBERRY_FORMAT_SVM},
+    {"ij", GOOSEBERRY_FORMAT_POINT}, {"point", GOOSEBERRY_FORMAT_POINT},
+    {"metis", GOOSEBERRY_FORMAT_GRAPH}, {"chaco", GOOSEBERRY_FORMAT_GRAPH},
+    {"graph", GOOSEBERRY_FORMAT_GRAPH}, {"clu", GOOSEBERRY_FORMAT_CLU},
+    {"mm", GOOSEBERRY_FORMAT_MATRIXMARKET}, {"mtx", GOOSEBERRY_FORMAT_MATRIXMARKET},
+  };
+  if (!name) return -1;
+  size_t ln = strlen(name);
+  for (size_t k = 0; k < sizeof(tbl) / sizeof(tbl[0]); ++k) {
+    size_t le = strlen(tbl[k].ext);
+    if (ln >= le && strcmp(name + ln - le, tbl[k].ext) == 0) return tbl[k].type;
+  }
+  return -1;
+}
+
+static int is_dense(int t)
+{
+  return t == GOOSEBERRY_FORMAT_RAW || t == GOOSEBERRY_FORMAT_GRID;
+}
+
+static void print_times(const char *names[], const double *vals, int n)
+{
+  printf("################################ Times ###############################\n");
+  for (int i = 0; i < n; ++i) printf(" %s: %0.04lf\n", names[i], vals[i]);
+  printf("######################################################################\n");
+}
+
+typedef struct {
+  int dense;
+  dim_t nrows, ncols;
+  ind_t *rowptr;
+  dim_t *rowind;
+  real_t *rowval;
+} mat_t;
+
+static void mat_free(mat_t *m)
+{
+  free(m->rowptr); free(m->rowind); free(m->rowval);
+  memset(m, 0, sizeof(*m));
+}
+
+static int read_input(const char *file, mat_t *m)
+{
+  int t = file_type(file);
+  memset(m, 0, sizeof(*m));
+  if (t < 0) {
+    fprintf(stderr, "Unknown file format of '%s'\n", file);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  m->dense = is_dense(t);
+  if (m->dense)
+    return gooseberry_read_dense_matrix(t, file, &m->nrows, &m->ncols, &m->rowval);
+  return gooseberry_read_sparse_matrix(t, file, &m->nrows, &m->ncols, &m->rowptr,
+      &m->rowind, &m->rowval);
+}
+
+/* stable counting sort of row ids by label (src/gooseberry_bin.c:2038-2049) */
+static int labels_to_perm(const char *file, dim_t n, dim_t **perm)
+{
+  dim_t cnt = n;
+  dim_t *lab = NULL;
+  int rv = gooseberry_read_labels(file, &cnt, &lab);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (cnt != n) {
+    fprintf(stderr, "Label file '%s' has %u labels for %u rows\n", file, cnt, n);
+    free(lab);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  *perm = malloc(sizeof(dim_t) * (n ? n : 1));
+  rv = gooseberry_countingsort_kv(lab, NULL, 0, n, n, *perm, NULL);
+  free(lab);
+  return rv;
+}
+
+/* ---------------------------------------------------------------- blas */
+
+enum { BLAS_HELP, BLAS_OPER, BLAS_IN, BLAS_OUT, BLAS_TIME, BLAS_RUNS,
+       BLAS_THREADS, BLAS_RPERM, BLAS_CPERM, BLAS_BAND };
+static const char *const BLAS_CHOICES[] = {"multiply", NULL};
+static const opt_t BLAS_OPTS[] = {
+  {BLAS_HELP, 'h', "help", "Display this help page.", OPT_FLAG, NULL},
+  {BLAS_OPER, 'x', "operation", "The type of operation to perform.", OPT_CHOICE, BLAS_CHOICES},
+  {BLAS_IN, 'i', "infile", "An input matrix/vector file.", OPT_STRING, NULL},
+  {BLAS_OUT, 'o', "outfile", "The output matrix/vector file.", OPT_STRING, NULL},
+  {BLAS_TIME, 't', "times", "Print timing of the blas routines.", OPT_FLAG, NULL},
+  {BLAS_RUNS, 'r', "runs", "Number of repeated runs (only useful for timing purposes).", OPT_INT, NULL},
+  {BLAS_THREADS, 'T', "threads", "Number of threads (accepted; the GPU ignores it).", OPT_INT, NULL},
+  {BLAS_RPERM, 'R', "rowperm", "Row permutation file.", OPT_STRING, NULL},
+  {BLAS_CPERM, 'C', "colperm", "Column permutation file.", OPT_STRING, NULL},
+  {BLAS_BAND, 'b', "bandwidthreduce", "Re-order the matrix to reduce bandwidth.", OPT_FLAG, NULL},
+};
+#define NBLAS_OPTS (sizeof(BLAS_OPTS) / sizeof(BLAS_OPTS[0]))
+
+static int cmd_blas(int argc, char **argv)
+{
+  arg_t args[512];
+  size_t nargs = 0;
+  if (argc - 2 > 512) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!parse_args(argc - 2, argv + 2, BLAS_OPTS, NBLAS_OPTS, args, &nargs))
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int err = GOOSEBERRY_SUCCESS, times = 0, oper = -1, redband = 0;
+  long long runs = 1;
+  const char *infiles[256], *outfile = NULL, *rpf = NULL, *cpf = NULL;
+  size_t nin = 0;
+  mat_t in[2];
+  memset(in, 0, sizeof(in));
+  dim_t *rperm = NULL, *cperm = NULL, *rdist = NULL, *cdist = NULL;
+  real_t *out = NULL;
+  void *d_b = NULL, *d_c = NULL;
+  gooseberry_matrix_t dmat = NULL;
+  tmr_t io = {0}, aux = {0}, op = {0};
+
+  if (nargs < 2) {
+    command_usage(argv[0], argv[1], BLAS_OPTS, NBLAS_OPTS, stderr);
+    return GOOSEBERRY_SUCCESS;
+  }
+  for (size_t i = 0; i < nargs; ++i) {
+    switch (args[i].id) {
+      case BLAS_HELP:
+        command_usage(argv[0], argv[1], BLAS_OPTS, NBLAS_OPTS, stdout);
+        return GOOSEBERRY_SUCCESS;
+      case BLAS_OPER: oper = args[i].o; break;
+      case BLAS_IN:
+        if (nin == 256) return GOOSEBERRY_ERROR_INVALIDINPUT;
+        infiles[nin++] = args[i].s;
+        break;
+      case BLAS_OUT: outfile = args[i].s; break;
+      case BLAS_TIME: times = 1; break;
+      case BLAS_RUNS: runs = args[i].i; break;
+      case BLAS_THREADS: break;
+      case BLAS_RPERM: rpf = args[i].s; break;
+      case BLAS_CPERM: cpf = args[i].s; break;
+      case BLAS_BAND: redband = 1; break;
+    }
+  }
+  if (nin != 2) {
+    fprintf(stderr, "blas needs exactly two input files (matrix, then vector)\n");
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (!outfile || file_type(outfile) < 0) {
+    fprintf(stderr, "Unknown or missing output file '%s'\n", outfile ? outfile : "");
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (times) tmr_start(&io);
+  for (size_t i = 0; i < nin; ++i) {
+    if ((err = read_input(infiles[i], &in[i])) != GOOSEBERRY_SUCCESS) goto END;
+  }
+  dim_t nrdist = 1, ncdist = 0;
+  if (rpf) {
+    dim_t prows = in[0].nrows;
+    err = gooseberry_read_partition(rpf, &prows, &nrdist, NULL, &rperm, &rdist);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+    printf("Read %" PRIu32 "-way row partition\n", nrdist);
+  }
+  if (cpf) {
+    dim_t prows = in[0].ncols;
+    err = gooseberry_read_partition(cpf, &prows, &ncdist, NULL, &cperm, &cdist);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+    printf("Read %" PRIu32 "-way column partition\n", ncdist);
+  }
+  if (times) { tmr_stop(&io); tmr_start(&aux); }
+  if (redband) {
+    fprintf(stderr, "Bandwidth reduction (-b) is not provided by this build\n");
+    err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    goto END;
+  }
+  if (oper != 0) {
+    fprintf(stderr, "Unknown operation '%d'\n", oper);
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto END;
+  }
+  if (in[0].dense || !in[1].dense) {
+    fprintf(stderr, "This build multiplies a sparse matrix by a dense "
+        "matrix/vector only\n");
+    err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    goto END;
+  }
+  if (!is_dense(file_type(outfile))) {
+    fprintf(stderr, "A sparse-times-dense product needs a dense output file\n");
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto END;
+  }
+  /* matrix_permute(in[0], rperm, cperm); matrix_permute(in[1], cperm, NULL) */
+  if (rperm || cperm) {
+    if (rperm && in[0].nrows != in[0].ncols && cperm == NULL) { /* fine */ }
+    err = gooseberry_permute_sparse(in[0].nrows, in[0].ncols, in[0].rowptr,
+        in[0].rowind, in[0].rowval, rperm, cperm);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+    if (cperm) {
+      if (in[1].nrows != in[0].ncols) {
+        err = GOOSEBERRY_ERROR_INVALIDINPUT;
+        goto END;
+      }
+      err = gooseberry_permute_dense(in[1].nrows, in[1].ncols, in[1].rowval,
+          cperm, NULL, 0);
+      if (err != GOOSEBERRY_SUCCESS) goto END;
+    }
+  }
+  const dim_t anrows = in[0].nrows, ancols = in[0].ncols;
+  const dim_t bnrows = in[1].nrows, bncols = in[1].ncols;
+  if (ancols > bnrows) {
+    fprintf(stderr, "Matrix dimensions do not match for multiplication: "
+        "%" PRIu32 "x%" PRIu32 " and %" PRIu32 "x%" PRIu32 "\n",
+        anrows, ancols, bnrows, bncols);
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto END;
+  }
+  /* matrix_buildindex: column-major copy of B (src/matrix.c:451-471) */
+  size_t bsz = (size_t)bnrows * bncols;
+  real_t *bcol = in[1].rowval;
+  int bcol_owned = 0;
+  if (bncols > 1 && bnrows > 1) {
+    bcol = malloc(sizeof(real_t) * bsz);
+    bcol_owned = 1;
+    for (dim_t i = 0; i < bnrows; ++i)
+      for (dim_t j = 0; j < bncols; ++j)
+        bcol[(size_t)j * bnrows + i] = in[1].rowval[(size_t)i * bncols + j];
+  }
+  size_t csz = (size_t)anrows * bncols;
+  out = calloc(csz ? csz : 1, sizeof(real_t));
+  err = gooseberry_matrix_create(anrows, ancols, in[0].rowptr, in[0].rowind,
+      in[0].rowval, NULL, &dmat);
+  /* B is read as bcol[j*ancols + l] (src/blas.c:87); keep that bound */
+  size_t bneed = bncols ? (size_t)(bncols - 1) * ancols + ancols : 0;
+  if (err == GOOSEBERRY_SUCCESS)
+    err = gooseberry_device_alloc(&d_b, (bsz > bneed ? bsz : bneed) * sizeof(real_t) + 8);
+  if (err == GOOSEBERRY_SUCCESS)
+    err = gooseberry_memcpy_htod(d_b, bcol, bsz * sizeof(real_t));
+  if (bcol_owned) free(bcol);
+  if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_c, csz * sizeof(real_t) + 8);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) { tmr_stop(&aux); tmr_start(&op); }
+  for (long long r = 0; r < runs; ++r) {
+    err = (bncols == 1) ? gooseberry_matrix_spmv(dmat, d_b, d_c)
+                        : gooseberry_matrix_spmm(dmat, bncols, d_b, d_c);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  err = gooseberry_memcpy_dtoh(out, d_c, csz * sizeof(real_t));
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) { tmr_stop(&op); tmr_start(&aux); }
+  if (rperm) {
+    err = gooseberry_permute_dense(anrows, bncols, out, rperm, NULL, 1);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (times) { tmr_stop(&aux); tmr_start(&io); }
+  err = gooseberry_write_dense_matrix(GOOSEBERRY_FORMAT_GRID, outfile, anrows,
+      bncols, out);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) {
+    tmr_stop(&io);
+    const char *names[] = {"I/O", "Auxillary", "Compute"};
+    double vals[] = {io.acc, aux.acc, op.acc};
+    print_times(names, vals, 3);
+  }
+END:
+  gooseberry_device_free(d_b);
+  gooseberry_device_free(d_c);
+  gooseberry_matrix_free(dmat);
+  free(out);
+  free(rperm); free(cperm); free(rdist); free(cdist);
+  mat_free(&in[0]);
+  mat_free(&in[1]);
+  return err;
+}
+
+/* ----------------------------------------------------------------- cgd */
+
+enum { CGD_HELP, CGD_IN, CGD_OUT, CGD_ERROR, CGD_NITER, CGD_TIME, CGD_RUNS,
+       CGD_THREADS, CGD_RPERM, CGD_CPERM };
+static const opt_t CGD_OPTS[] = {
+  {CGD_HELP, 'h', "help", "Display this help page.", OPT_FLAG, NULL},
+  {CGD_IN, 'i', "infile", "An input matrix/vector file.", OPT_STRING, NULL},
+  {CGD_OUT, 'o', "outfile", "The output vector file.", OPT_STRING, NULL},
+  {CGD_ERROR, 'e', "error", "The RMSE to achieve before exiting.", OPT_FLOAT, NULL},
+  {CGD_NITER, 'I', "iter", "The number of iterations to run before exiting.", OPT_INT, NULL},
+  {CGD_TIME, 't', "times", "Print timing of the cgd routines.", OPT_FLAG, NULL},
+  {CGD_RUNS, 'r', "runs", "Number of repeated runs (only useful for timing purposes).", OPT_INT, NULL},
+  {CGD_THREADS, 'T', "threads", "Number of threads (accepted; the GPU ignores it).", OPT_INT, NULL},
+  {CGD_RPERM, 'R', "rowperm", "Row permutation file.", OPT_STRING, NULL},
+  {CGD_CPERM, 'C', "colperm", "Column permutation file.", OPT_STRING, NULL},
+};
+#define NCGD_OPTS (sizeof(CGD_OPTS) / sizeof(CGD_OPTS[0]))
+
+static int cmd_cgd(int argc, char **argv)
+{
+  arg_t args[512];
+  size_t nargs = 0;
+  if (argc - 2 > 512) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!parse_args(argc - 2, argv + 2, CGD_OPTS, NCGD_OPTS, args, &nargs))
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int err = GOOSEBERRY_SUCCESS, times = 0;
+  double error = 0;
+  long long niter = 0, runs = 1;
+  const char *matfile = NULL, *vecfile = NULL, *outfile = NULL, *rpf = NULL,
+             *cpf = NULL;
+  mat_t mat, vec;
+  memset(&mat, 0, sizeof(mat));
+  memset(&vec, 0, sizeof(vec));
+  dim_t *rperm = NULL, *cperm = NULL;
+  real_t *out = NULL;
+  void *d_b = NULL, *d_x = NULL;
+  gooseberry_matrix_t dmat = NULL;
+  tmr_t io = {0}, op = {0};
+
+  if (nargs < 2) {
+    command_usage(argv[0], argv[1], CGD_OPTS, NCGD_OPTS, stderr);
+    return GOOSEBERRY_SUCCESS;
+  }
+  for (size_t i = 0; i < nargs; ++i) {
+    switch (args[i].id) {
+      case CGD_HELP:
+        command_usage(argv[0], argv[1], CGD_OPTS, NCGD_OPTS, stdout);
+        return GOOSEBERRY_SUCCESS;
+      case CGD_ERROR: error = args[i].f; break;
+      case CGD_NITER: niter = args[i].i; break;
+      case CGD_IN:
+        if (!matfile) matfile = args[i].s;
+        else if (!vecfile) vecfile = args[i].s;
+        else {
+          fprintf(stderr, "Too many input files specified\n");
+          err = GOOSEBERRY_ERROR_INVALIDINPUT;
+        }
+        break;
+      case CGD_OUT: outfile = args[i].s; break;
+      case CGD_TIME: times = 1; break;
+      case CGD_RUNS: runs = args[i].i; break;
+      case CGD_THREADS: break;
+      case CGD_RPERM: rpf = args[i].s; break;
+      case CGD_CPERM: cpf = args[i].s; break;
+    }
+  }
+  if (!matfile || !vecfile) {
+    fprintf(stderr, "You must specify both a matrix input file and a vector "
+        "input file (in that order).\n");
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (err == GOOSEBERRY_SUCCESS && !outfile) {
+    fprintf(stderr, "You must specify an output file.\n");
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (err == GOOSEBERRY_SUCCESS && niter < 0) err = GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (err != GOOSEBERRY_SUCCESS) return err;
+
+  if (times) tmr_start(&io);
+  if ((err = read_input(matfile, &mat)) != GOOSEBERRY_SUCCESS) goto END;
+  if ((err = read_input(vecfile, &vec)) != GOOSEBERRY_SUCCESS) goto END;
+  printf("matrix = %" PRIu32 "x%" PRIu32 " vector = %" PRIu32 "x%" PRIu32 "\n",
+      mat.nrows, mat.ncols, vec.nrows, vec.ncols);
+  if (mat.dense || !vec.dense) {
+    fprintf(stderr, "cgd needs a sparse matrix and a dense vector\n");
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto END;
+  }
+  if (rpf && (err = labels_to_perm(rpf, mat.nrows, &rperm)) != GOOSEBERRY_SUCCESS)
+    goto END;
+  if (cpf && (err = labels_to_perm(cpf, mat.ncols, &cperm)) != GOOSEBERRY_SUCCESS)
+    goto END;
+  if (times) tmr_stop(&io);
+  /* src/cgd.c:51-65 */
+  if (mat.nrows != mat.ncols) {
+    fprintf(stderr, "Input matrix for CGD is not square: %" PRIu32 "x%" PRIu32 "\n",
+        mat.nrows, mat.ncols);
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto END;
+  }
+  if (vec.nrows * (size_t)vec.ncols != mat.nrows || vec.ncols != 1) {
+    fprintf(stderr, "Input vector for CGD has wrong number of rows: %" PRIu32 "\n",
+        vec.nrows);
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto END;
+  }
+  if (rperm || cperm) {
+    err = gooseberry_permute_sparse(mat.nrows, mat.ncols, mat.rowptr, mat.rowind,
+        mat.rowval, rperm, cperm);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+    if (rperm) {
+      err = gooseberry_permute_dense(vec.nrows, 1, vec.rowval, rperm, NULL, 0);
+      if (err != GOOSEBERRY_SUCCESS) goto END;
+    }
+  }
+  const dim_t n = mat.nrows;
+  out = calloc(n ? n : 1, sizeof(real_t));
+  err = gooseberry_matrix_create(n, n, mat.rowptr, mat.rowind, mat.rowval, NULL,
+      &dmat);
+  if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_b, n * sizeof(real_t));
+  if (err == GOOSEBERRY_SUCCESS) err = gooseberry_memcpy_htod(d_b, vec.rowval, n * sizeof(real_t));
+  if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_x, n * sizeof(real_t));
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) tmr_start(&op);
+  for (long long r = 0; r < runs; ++r) {
+    gooseberry_cgd_info_t info;
+    err = gooseberry_matrix_cgd(dmat, d_b, d_x, error, GOOSEBERRY_TOL_ABSOLUTE,
+        (size_t)niter, stdout, &info);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  err = gooseberry_memcpy_dtoh(out, d_x, n * sizeof(real_t));
+  if (times) tmr_stop(&op);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (cperm) {
+    err = gooseberry_permute_dense(n, 1, out, cperm, NULL, 1);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (times) tmr_start(&io);
+  /* always GRID, whatever the extension (src/gooseberry_bin.c:2102-2103) */
+  err = gooseberry_write_dense_matrix(GOOSEBERRY_FORMAT_GRID, outfile, n, 1, out);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) {
+    tmr_stop(&io);
+    const char *names[] = {"I/O", "Compute"};
+    double vals[] = {io.acc, op.acc};
+    print_times(names, vals, 2);
+  }
+END:
+  gooseberry_device_free(d_b);
+  gooseberry_device_free(d_x);
+  gooseberry_matrix_free(dmat);
+  free(out);
+  free(rperm);
+  free(cperm);
+  mat_free(&mat);
+  mat_free(&vec);
+  return err;
+}
+
+/* ---------------------------------------------------------------- main */
+
+static const struct { const char *name; const char *desc; int provided; } COMMANDS[] = {
+  {"help", "Display this help page.", 1},
+  {"analysis", "Perform analysis on a matrix.", 0},
+  {"transform", "Transform a matrix.", 0},
+  {"permute", "Permute a matrix.", 0},
+  {"generate", "Generate a matrix/vector.", 0},
+  {"blas", "Perform a blas operation.", 1},
+  {"cgd", "Perform conjugate gradient descent.", 1},
+  {"sgd", "Perform stochastic gradient descent.", 0},
+  {"pagerank", "Perform pagerank on a square matrix.", 0},
+};
+#define NCOMMANDS (sizeof(COMMANDS) / sizeof(COMMANDS[0]))
+
+static void usage(const char *prog, FILE *f)
+{
+  fprintf(f, "USAGE:\n%s <command> [options]\n\nCommands:\n", prog);
+  for (size_t i = 0; i < NCOMMANDS; ++i)
+    fprintf(f, "\t%s : %s%s\n", COMMANDS[i].name, COMMANDS[i].desc,
+        COMMANDS[i].provided ? "" : " (not provided by this build)");
+}
+
+int main(int argc, char **argv)
+{
+  if (argc < 2) {
+    fprintf(stderr, "Must supply a command.\n");
+    usage(argv[0], stderr);
+    return 1;
+  }
+  const char *cmd = argv[1];
+  int err;
+  if (strcmp(cmd, "help") == 0) {
+    usage(argv[0], stdout);
+    err = GOOSEBERRY_SUCCESS;
+  } else if (strcmp(cmd, "blas") == 0) {
+    err = cmd_blas(argc, argv);
+  } else if (strcmp(cmd, "cgd") == 0) {
+    err = cmd_cgd(argc, argv);
+  } else {
+    size_t i;
+    for (i = 0; i < NCOMMANDS; ++i) if (strcmp(cmd, COMMANDS[i].name) == 0) break;
+    if (i == NCOMMANDS) {
+      fprintf(stderr, "Unrecognized command '%s'.\n", cmd);
+      usage(argv[0], stderr);
+      return 1;
+    }
+    fprintf(stderr, "Command '%s' is not provided by this build.\n", cmd);
+    err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+  }
+  if (err == GOOSEBERRY_SUCCESS) return 0;
+  fprintf(stderr, "Operation failed.\n");
+  return 2;
+}

@@ -1,0 +1,13 @@
+/* gb_host.h -- internal declarations shared by the host C files. */
+#ifndef GB_HOST_H
+#define GB_HOST_H
+
+#include "gooseberry.h"
+#include "../gb_device.h"
+
+int gb_upload(void **d, const void *h, size_t bytes);
+int gb_check_csr(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind);
+gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat);
+
+#endif

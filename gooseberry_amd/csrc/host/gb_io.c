@@ -1,0 +1,489 @@
+/*
+ * gb_io.c -- CSR construction from text and vector I/O for the blas/cgd
+ * path: the .csr, .mtx and grid readers and the grid/.csr writers of
+ * src/io.c, with the same parsing semantics (identical integer arrays and
+ * values for identical input text).
+ *
+ * The whole file is read into memory once and split into lines in place;
+ * a line is the text up to '\n' with trailing '\r' removed.
+ */
+#define _GNU_SOURCE
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gooseberry.h"
+#include "gb_host.h"
+
+/* src/io.c:63-69 */
+static int is_comment(char c)
+{
+  return c == '#' || c == '%' || c == '\'' || c == '"' || c == '/';
+}
+
+static int errno_to_rv(int e)
+{
+  switch (e) {
+    case ENOENT: case ENOTDIR: return GOOSEBERRY_ERROR_FILENOTFOUND;
+    case EACCES: case EPERM: return GOOSEBERRY_ERROR_PERMISSIONDENIED;
+    case EINVAL: case ENAMETOOLONG: return GOOSEBERRY_ERROR_INVALIDINPUT;
+    default: return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+}
+
+typedef struct {
+  char *buf;
+  size_t len;
+  size_t pos;
+} text_t;
+
+static int text_load(const char *file, text_t *t)
+{
+  memset(t, 0, sizeof(*t));
+  if (!file) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  FILE *f = fopen(file, "rb");
+  if (!f) return errno_to_rv(errno);
+  size_t cap = 1 << 16, len = 0;
+  char *buf = malloc(cap + 1);
+  if (!buf) { fclose(f); return GOOSEBERRY_ERROR_UNKNOWN; }
+  for (;;) {
+    if (len == cap) {
+      cap *= 2;
+      char *nb = realloc(buf, cap + 1);
+      if (!nb) { free(buf); fclose(f); return GOOSEBERRY_ERROR_UNKNOWN; }
+      buf = nb;
+    }
+    size_t got = fread(buf + len, 1, cap - len, f);
+    len += got;
+    if (got == 0) break;
+  }
+  int err = ferror(f);
+  fclose(f);
+  if (err) { free(buf); return GOOSEBERRY_ERROR_READ; }
+  buf[len] = '\0';
+  t->buf = buf;
+  t->len = len;
+  t->pos = 0;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* next line as a NUL-terminated string; returns its length or -1 at end */
+static ssize_t text_line(text_t *t, char **line)
+{
+  if (t->pos >= t->len) return -1;
+  char *s = t->buf + t->pos;
+  char *nl = memchr(s, '\n', t->len - t->pos);
+  size_t ll;
+  if (nl) {
+    ll = (size_t)(nl - s);
+    *nl = '\0';
+    t->pos += ll + 1;
+  } else {
+    ll = t->len - t->pos;
+    t->pos = t->len;
+  }
+  while (ll > 0 && (s[ll - 1] == '\r' || s[ll - 1] == '\n')) s[--ll] = '\0';
+  *line = s;
+  return (ssize_t)ll;
+}
+
+typedef struct { void *p; size_t n, cap, esz; } arr_t;
+
+static int arr_push(arr_t *a, const void *e)
+{
+  if (a->n == a->cap) {
+    size_t nc = a->cap ? 2 * a->cap : 1024;
+    void *np = realloc(a->p, nc * a->esz);
+    if (!np) return 0;
+    a->p = np;
+    a->cap = nc;
+  }
+  memcpy((char *)a->p + a->n * a->esz, e, a->esz);
+  a->n++;
+  return 1;
+}
+
+/* src/io.c:369-498 */
+static int read_csr(const char *file, dim_t *nrows, dim_t *ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  arr_t ptr = {0, 0, 0, sizeof(ind_t)}, ind = {0, 0, 0, sizeof(dim_t)},
+        val = {0, 0, 0, sizeof(real_t)};
+  dim_t mincol = 1, maxcol = 0, n = 0;
+  ind_t nnz = 0;
+  int ok = arr_push(&ptr, &nnz);
+  char *line;
+  ssize_t ll;
+  while (ok && (ll = text_line(&t, &line)) >= 0) {
+    if (ll != 0 && is_comment(line[0])) continue;
+    char *sptr = line, *eptr;
+    dim_t c = (dim_t)strtoull(sptr, &eptr, 10);
+    sptr = eptr;
+    real_t v = strtod(sptr, &eptr);
+    while (ok && eptr != sptr) {
+      if (c > maxcol) maxcol = c;
+      if (c < mincol) mincol = c;
+      ok = arr_push(&ind, &c) && arr_push(&val, &v);
+      ++nnz;
+      sptr = eptr;
+      c = (dim_t)strtoull(sptr, &eptr, 10);
+      sptr = eptr;
+      v = strtod(sptr, &eptr);
+    }
+    ok = ok && arr_push(&ptr, &nnz);
+    ++n;
+  }
+  free(t.buf);
+  if (!ok) {
+    free(ptr.p); free(ind.p); free(val.p);
+    return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  dim_t *ri = ind.p;
+  for (size_t k = 0; k < ind.n; ++k) ri[k] -= mincol; /* mincol <= 1 */
+  *r_rowptr = ptr.p;
+  *r_rowind = ind.n ? ind.p : calloc(1, sizeof(dim_t));
+  *r_rowval = val.n ? val.p : calloc(1, sizeof(real_t));
+  *nrows = n;
+  *ncols = maxcol - mincol + 1;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:921-1190 */
+static int read_mtx(const char *file, dim_t *r_nrows, dim_t *r_ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  char *line;
+  ssize_t ll;
+  int sym = 0, wgts = 0;
+  dim_t nrows = 0, ncols = 0;
+  ind_t *rowptr = NULL;
+  dim_t *rowind = NULL;
+  real_t *rowval = NULL;
+  rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+  if ((ll = text_line(&t, &line)) < 1) goto ERR;
+  {
+    unsigned a, b, c;
+    if (sscanf(line, "%u %u %u", &a, &b, &c) == 3) {
+      nrows = a; ncols = b; sym = 0; /* no header: real general */
+    } else {
+      static const char ws[] = " \t";
+      char *save = NULL;
+      char *tok = strtok_r(line, ws, &save);
+      if (!tok || strcmp(tok, "%%MatrixMarket") != 0) goto ERR;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok || strcmp(tok, "matrix") != 0) goto ERR;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok || strcmp(tok, "coordinate") != 0) goto ERR;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok || strcmp(tok, "real") != 0) goto ERR;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok) goto ERR;
+      if (strcmp(tok, "general") == 0) sym = 0;
+      else if (strcmp(tok, "symmetric") == 0) sym = 1;
+      else goto ERR;
+      while ((ll = text_line(&t, &line)) >= 0) {
+        if (ll == 0 || is_comment(line[0])) continue;
+        break;
+      }
+      if (ll < 0) goto ERR;
+      char *sptr = line, *eptr;
+      nrows = (dim_t)strtoull(sptr, &eptr, 10); sptr = eptr;
+      ncols = (dim_t)strtoull(sptr, &eptr, 10); sptr = eptr;
+      (void)strtoull(sptr, &eptr, 10);
+      if (sptr == eptr) goto ERR;
+    }
+  }
+  rowptr = calloc((size_t)nrows + 2, sizeof(ind_t));
+  if (!rowptr) { rv = GOOSEBERRY_ERROR_UNKNOWN; goto ERR; }
+  /* pass 1: parse the coordinates and count entries per row in
+   * rowptr[i+1] (src/io.c:1060-1112) */
+  arr_t ti = {0, 0, 0, sizeof(dim_t)}, tj = {0, 0, 0, sizeof(dim_t)},
+        tw = {0, 0, 0, sizeof(real_t)};
+  size_t nnz = 0, nent = 0;
+  int ok = 1;
+  while (ok && (ll = text_line(&t, &line)) >= 0) {
+    if (ll == 0 || is_comment(line[0])) continue;
+    char *sptr = line, *eptr;
+    dim_t i = (dim_t)strtoull(sptr, &eptr, 10) - 1;
+    if (eptr == sptr) goto ERRT;
+    sptr = eptr;
+    dim_t j = (dim_t)strtoull(sptr, &eptr, 10) - 1;
+    if (eptr == sptr) goto ERRT;
+    sptr = eptr;
+    real_t w = strtod(sptr, &eptr);
+    if (eptr != sptr) {
+      if (nent == 0) wgts = 1;
+      else if (!wgts) goto ERRT;
+    } else if (wgts) {
+      goto ERRT;
+    }
+    if (i >= nrows || j >= ncols || (sym && j >= nrows)) goto ERRT;
+    ok = arr_push(&ti, &i) && arr_push(&tj, &j) && (!wgts || arr_push(&tw, &w));
+    ++nent;
+    if (sym) { ++rowptr[j + 1]; ++nnz; }
+    ++rowptr[i + 1];
+    ++nnz;
+  }
+  if (!ok || nnz > 0xffffffffull) goto ERRT;
+  {
+    ind_t s = 0;
+    for (size_t k = 1; k <= nrows; ++k) { ind_t c = rowptr[k]; rowptr[k] = s; s += c; }
+  }
+  rowind = malloc(sizeof(dim_t) * (nnz ? nnz : 1));
+  rowval = malloc(sizeof(real_t) * (nnz ? nnz : 1));
+  if (!rowind || !rowval) goto ERRT;
+  /* pass 2: fill in file order, mirrored entry first (src/io.c:1115-1145) */
+  for (size_t e = 0; e < nent; ++e) {
+    dim_t i = ((dim_t *)ti.p)[e], j = ((dim_t *)tj.p)[e];
+    real_t w = wgts ? ((real_t *)tw.p)[e] : 1;
+    ind_t k;
+    if (sym) { k = rowptr[j + 1]++; rowind[k] = i; rowval[k] = w; }
+    k = rowptr[i + 1]++;
+    rowind[k] = j;
+    rowval[k] = w;
+  }
+  free(ti.p); free(tj.p); free(tw.p);
+  free(t.buf);
+  *r_nrows = nrows;
+  *r_ncols = ncols;
+  *r_rowptr = rowptr;
+  *r_rowind = rowind;
+  *r_rowval = rowval;
+  return GOOSEBERRY_SUCCESS;
+ERRT:
+  free(ti.p); free(tj.p); free(tw.p);
+ERR:
+  free(t.buf);
+  free(rowptr);
+  free(rowind);
+  free(rowval);
+  return rv;
+}
+
+/* src/io.c:145-239 */
+static int read_grid(const char *file, dim_t *r_nrows, dim_t *r_ncols,
+    real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  arr_t val = {0, 0, 0, sizeof(real_t)};
+  dim_t nrows = 0, ncols = 0;
+  char *line;
+  ssize_t ll;
+  int ok = 1;
+  while (ok && (ll = text_line(&t, &line)) >= 0) {
+    if (is_comment(line[0])) continue;
+    dim_t j = 0;
+    char *sptr = line, *eptr;
+    real_t v = strtod(sptr, &eptr);
+    while (ok && eptr != sptr) {
+      ok = arr_push(&val, &v);
+      ++j;
+      sptr = eptr;
+      v = strtod(sptr, &eptr);
+    }
+    if (ncols == 0) {
+      ncols = j;
+    } else if (j != ncols) {
+      free(t.buf);
+      free(val.p);
+      fprintf(stderr, "Invalid line in grid matrix file: %u\n", nrows + 1);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+    ++nrows;
+  }
+  free(t.buf);
+  if (!ok) { free(val.p); return GOOSEBERRY_ERROR_UNKNOWN; }
+  if ((size_t)nrows * ncols != val.n) {
+    free(val.p);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  *r_nrows = nrows;
+  *r_ncols = ncols;
+  *r_rowval = val.n ? val.p : calloc(1, sizeof(real_t));
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:1195-1229 */
+static int write_grid(const char *file, dim_t nrows, dim_t ncols,
+    const real_t *rowval)
+{
+  FILE *f = fopen(file, "w");
+  if (!f) return errno_to_rv(errno);
+  static char big[1 << 20];
+  setvbuf(f, big, _IOFBF, sizeof(big));
+  for (dim_t i = 0; i < nrows; ++i) {
+    fprintf(f, "%g", rowval[(size_t)i * ncols]);
+    for (dim_t j = 1; j < ncols; ++j) fprintf(f, " %g", rowval[(size_t)i * ncols + j]);
+    fputc('\n', f);
+  }
+  int bad = ferror(f);
+  if (fclose(f) != 0) bad = 1;
+  return bad ? GOOSEBERRY_ERROR_WRITE : GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:1233-1274 */
+static int write_csr(const char *file, dim_t nrows, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval)
+{
+  FILE *f = fopen(file, "w");
+  if (!f) return errno_to_rv(errno);
+  static char big[1 << 20];
+  setvbuf(f, big, _IOFBF, sizeof(big));
+  for (dim_t i = 0; i < nrows; ++i) {
+    if (rowptr[i] < rowptr[i + 1]) {
+      fprintf(f, "%" PRIu32 " %g", rowind[rowptr[i]] + 1, rowval[rowptr[i]]);
+      for (ind_t j = rowptr[i] + 1; j < rowptr[i + 1]; ++j)
+        fprintf(f, " %" PRIu32 " %g", rowind[j] + 1, rowval[j]);
+    }
+    fputc('\n', f);
+  }
+  int bad = ferror(f);
+  if (fclose(f) != 0) bad = 1;
+  return bad ? GOOSEBERRY_ERROR_WRITE : GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:1464-1549 dispatch */
+int gooseberry_read_sparse_matrix(int type, const char *file, dim_t *nrows,
+    dim_t *ncols, ind_t **rowptr, dim_t **rowind, real_t **rowval)
+{
+  if (!nrows || !ncols || !rowptr || !rowind || !rowval)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  switch (type) {
+    case GOOSEBERRY_FORMAT_CSR:
+      return read_csr(file, nrows, ncols, rowptr, rowind, rowval);
+    case GOOSEBERRY_FORMAT_MATRIXMARKET:
+      return read_mtx(file, nrows, ncols, rowptr, rowind, rowval);
+    case GOOSEBERRY_FORMAT_GRAPH:
+    case GOOSEBERRY_FORMAT_SVM:
+    case GOOSEBERRY_FORMAT_CLU:
+    case GOOSEBERRY_FORMAT_POINT:
+      fprintf(stderr, "Sparse format %d is not provided by this build\n", type);
+      return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    default:
+      fprintf(stderr, "Unrecognized sparse matrix format '%d'\n", type);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+}
+
+int gooseberry_read_dense_matrix(int type, const char *file, dim_t *nrows,
+    dim_t *ncols, real_t **rowval)
+{
+  if (!nrows || !ncols || !rowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (type != GOOSEBERRY_FORMAT_GRID) {
+    fprintf(stderr, "Unrecognized dense matrix format '%d'\n", type);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  return read_grid(file, nrows, ncols, rowval);
+}
+
+int gooseberry_write_dense_matrix(int type, const char *file, dim_t nrows,
+    dim_t ncols, const real_t *rowval)
+{
+  if (type != GOOSEBERRY_FORMAT_GRID) {
+    fprintf(stderr, "Unrecognized dense matrix format '%d'\n", type);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (!file || (nrows && ncols && !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return write_grid(file, nrows, ncols, rowval);
+}
+
+int gooseberry_write_sparse_matrix(int type, const char *file, dim_t nrows,
+    dim_t ncols, const ind_t *rowptr, const dim_t *rowind,
+    const real_t *rowval)
+{
+  (void)ncols;
+  if (!file || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  switch (type) {
+    case GOOSEBERRY_FORMAT_CSR:
+      return write_csr(file, nrows, rowptr, rowind, rowval);
+    case GOOSEBERRY_FORMAT_GRAPH:
+    case GOOSEBERRY_FORMAT_MATRIXMARKET:
+    case GOOSEBERRY_FORMAT_POINT:
+      fprintf(stderr, "Sparse output format %d is not provided by this build\n",
+          type);
+      return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    default:
+      fprintf(stderr, "Unrecognized sparse matrix format '%d'\n", type);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+}
+
+/* src/io.c:1552-1639 */
+int gooseberry_read_labels(const char *file, dim_t *r_nrows, dim_t **r_labels)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  char *line;
+  ssize_t ll;
+  unsigned l;
+  ll = text_line(&t, &line);
+  while (ll > 0 && is_comment(line[0])) ll = text_line(&t, &line);
+  size_t first_pos = 0;
+  char *first_line = line;
+  ssize_t first_ll = ll;
+  (void)first_pos;
+  dim_t nl;
+  arr_t lab = {0, 0, 0, sizeof(dim_t)};
+  dim_t cap = (r_nrows && *r_nrows) ? *r_nrows : (dim_t)-1;
+  nl = 0;
+  line = first_line;
+  ll = first_ll;
+  int ok = 1;
+  while (ok && ll > 0 && sscanf(line, "%u", &l) == 1) {
+    if (nl == cap) {
+      fprintf(stderr, "Found excess of rows in permutation file\n");
+      break;
+    }
+    dim_t v = l;
+    ok = arr_push(&lab, &v);
+    ++nl;
+    ll = text_line(&t, &line);
+  }
+  free(t.buf);
+  if (!ok) { free(lab.p); return GOOSEBERRY_ERROR_UNKNOWN; }
+  if (r_nrows) *r_nrows = nl;
+  if (r_labels) *r_labels = lab.n ? lab.p : calloc(1, sizeof(dim_t));
+  else free(lab.p);
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/gooseberry.c:253-306 with the stable counting sort on the GPU */
+int gooseberry_read_partition(const char *const file, dim_t *const r_nrows,
+    dim_t *const r_nparts, dim_t **const r_map, dim_t **const r_perm,
+    dim_t **const r_dist)
+{
+  dim_t nrows = r_nrows ? *r_nrows : 0;
+  dim_t *map = NULL;
+  int rv = gooseberry_read_labels(file, &nrows, &map);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  dim_t mx = 0;
+  for (dim_t i = 0; i < nrows; ++i) if (map[i] > mx) mx = map[i];
+  dim_t nparts = nrows ? mx + 1 : 1;
+  dim_t *perm = malloc(sizeof(dim_t) * (nrows ? nrows : 1));
+  dim_t *dist = malloc(sizeof(dim_t) * ((size_t)nparts + 1));
+  if (!perm || !dist) {
+    free(perm); free(dist); free(map);
+    return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  rv = gooseberry_countingsort_kv(map, NULL, 0, nparts - 1, nrows, perm, dist);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    free(perm); free(dist); free(map);
+    return rv;
+  }
+  if (r_nrows) *r_nrows = nrows;
+  if (r_nparts) *r_nparts = nparts;
+  if (r_dist) *r_dist = dist; else free(dist);
+  if (r_perm) *r_perm = perm; else free(perm);
+  if (r_map) *r_map = map; else free(map);
+  return GOOSEBERRY_SUCCESS;
+}

@@ -1,0 +1,355 @@
+/*
+ * gooseberry.h -- public C API of the MI355X (gfx950) sparse-solve library.
+ *
+ * Drop-in for the sparse-solve path of the reference's include/gooseberry.h:
+ * the same types, error and format enums and the same prototypes for the
+ * functions on that path, backed by hand-written HIP kernels on one GPU.
+ * New entry points (device-resident matrices, the exported CG solver,
+ * mixed precision, batched solves and the solve server) are appended.
+ *
+ * ABI: this library is built with FP64 values (the reference's
+ * GOOSEBERRY_DOUBLE_REALS build, include/gooseberry.h:73-79) and unsigned
+ * 32-bit indices (include/gooseberry.h:45-71, default widths).
+ *
+ * Devices and streams: the library drives device 0 of the process through
+ * one library stream.  Functions that take host pointers copy in, compute
+ * and copy out before they return.  Functions taking device pointers
+ * (named d_*) enqueue work on the library stream; results are ready after
+ * gooseberry_device_synchronize() or any function that returns host data.
+ * Calls may come from one host thread at a time.
+ */
+#ifndef GOOSEBERRY_H
+#define GOOSEBERRY_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <inttypes.h>
+
+#define GOOSEBERRY_VER_MAJOR 0
+#define GOOSEBERRY_VER_MINOR 1
+#define GOOSEBERRY_VER_SUBMINOR 7
+
+/* ---------------------------------------------------------------- types */
+/* include/gooseberry.h:43-82 (values FP64, indices uint32) */
+#ifndef BOWSTRING_TYPES_DEFINED
+#if defined(GOOSEBERRY_SIGNED_TYPES) || defined(GOOSEBERRY_64BIT_DIMENSIONS) \
+    || defined(GOOSEBERRY_64BIT_INDICES)
+#error "this library is built with unsigned 32-bit dim_t/ind_t"
+#endif
+typedef uint64_t gooseberry_int64_t;
+#define PF_BSINT64_T "%" PRIu64
+typedef uint32_t gooseberry_int32_t;
+#define PF_BSINT32_T "%" PRIu32
+typedef gooseberry_int32_t dim_t;
+#define PF_DIM_T PF_BSINT32_T
+typedef gooseberry_int32_t ind_t;
+#define PF_IND_T PF_BSINT32_T
+#ifndef GOOSEBERRY_DOUBLE_REALS
+#define GOOSEBERRY_DOUBLE_REALS 1
+#endif
+typedef double real_t;
+#define PF_REAL_T "%lf"
+#define PF_MINREAL_T "%g"
+#endif
+
+/* include/gooseberry.h:85-94; codes after GOOSEBERRY_ERROR_UNKNOWN are new */
+typedef enum gooseberry_error_t {
+  GOOSEBERRY_SUCCESS=1,
+  GOOSEBERRY_ERROR_UNIMPLEMENTED,
+  GOOSEBERRY_ERROR_INVALIDINPUT,
+  GOOSEBERRY_ERROR_READ,
+  GOOSEBERRY_ERROR_WRITE,
+  GOOSEBERRY_ERROR_FILENOTFOUND,
+  GOOSEBERRY_ERROR_PERMISSIONDENIED,
+  GOOSEBERRY_ERROR_UNKNOWN,
+  GOOSEBERRY_ERROR_DEVICE,      /* a HIP call failed */
+  GOOSEBERRY_ERROR_NOMEMORY,    /* device memory or admission budget */
+  GOOSEBERRY_ERROR_BREAKDOWN    /* CG produced a non-finite or zero p.Ap */
+} gooseberry_error_t;
+
+/* include/gooseberry.h:97-106 */
+typedef enum gooseberry_format_t {
+  GOOSEBERRY_FORMAT_RAW,
+  GOOSEBERRY_FORMAT_GRID,
+  GOOSEBERRY_FORMAT_CSR,
+  GOOSEBERRY_FORMAT_POINT,
+  GOOSEBERRY_FORMAT_SVM,
+  GOOSEBERRY_FORMAT_GRAPH,
+  GOOSEBERRY_FORMAT_CLU,
+  GOOSEBERRY_FORMAT_MATRIXMARKET
+} gooseberry_format_t;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ================================================== reference entry points */
+
+/* a <- s*a on n values (include/gooseberry.h:134-137; src/blas.c:244-257).
+ * Host pointer; computed on the GPU. */
+int gooseberry_scale(
+    ind_t n,
+    real_t * val,
+    real_t s);
+
+/* C = A x B (include/gooseberry.h:174-184; src/gooseberry.c:53-67,
+ * src/blas.c:552-606).  A is CSR (anrows x ancols), B is column-major
+ * (ancols x bncols, i.e. B^t rows), C row-major (anrows x bncols).  All
+ * pointers are host pointers owned by the caller.  blocks/nblocks only
+ * steer the reference's thread scheduling and do not change results; they
+ * are validated and otherwise ignored.  Each row sum of a row with at most
+ * 64 entries is bitwise the reference's left-to-right sum. */
+int gooseberry_spmult(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const ind_t * arowptr,
+    const dim_t * arowind,
+    const real_t * arowval,
+    const real_t * bcolval,
+    real_t * crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
+/* include/gooseberry.h IO prototypes; src/io.c:1464-1549.  Sparse readers:
+ * GOOSEBERRY_FORMAT_CSR and GOOSEBERRY_FORMAT_MATRIXMARKET.  Dense: GRID.
+ * Arrays returned through pointers are malloc'd; free with free() or
+ * gooseberry_free(). */
+int gooseberry_read_sparse_matrix(
+    int type,
+    const char * file,
+    dim_t * nrows,
+    dim_t * ncols,
+    ind_t ** rowptr,
+    dim_t ** rowind,
+    real_t ** rowval);
+
+int gooseberry_read_dense_matrix(
+    int type,
+    const char * file,
+    dim_t * nrows,
+    dim_t * ncols,
+    real_t ** rowval);
+
+int gooseberry_write_dense_matrix(
+    int type,
+    const char * file,
+    dim_t nrows,
+    dim_t ncols,
+    const real_t * rowval);
+
+int gooseberry_write_sparse_matrix(
+    int type,
+    const char * file,
+    dim_t nrows,
+    dim_t ncols,
+    const ind_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval);
+
+/* src/io.c:1552-1639 */
+int gooseberry_read_labels(
+    const char * file,
+    dim_t * nrows,
+    dim_t ** labels);
+
+/* src/gooseberry.c:253-306: labels -> stable counting sort (on the GPU) ->
+ * perm and dist (nparts+1 entries). */
+int gooseberry_read_partition(
+    const char * const file,
+    dim_t * const r_nrows,
+    dim_t * const r_nparts,
+    dim_t ** const r_map,
+    dim_t ** const r_perm,
+    dim_t ** const r_dist);
+
+/* ======================================================== new entry points */
+
+void gooseberry_free(void * ptr);
+const char * gooseberry_strerror(int err);
+/* text of the last device error seen by the library ("" if none) */
+const char * gooseberry_last_device_error(void);
+
+/* ---------------------------------------------------- index work (GPU) */
+
+/* Stable counting sort of vals by keys in [kmin, kmax] (host pointers):
+ * out[] receives vals in key order, input order kept within a key; dist
+ * (optional) receives kmax-kmin+2 run boundaries.  vals == NULL means
+ * 0..n-1.  The reference's counterpart is domlib's dd_countingsort_kv as
+ * called at src/gooseberry.c:275 and src/gooseberry_bin.c:2038-2049. */
+int gooseberry_countingsort_kv(
+    const dim_t * keys,
+    const dim_t * vals,
+    dim_t kmin,
+    dim_t kmax,
+    size_t n,
+    dim_t * out,
+    dim_t * dist);
+
+/* In-place CSR permutation, src/permute.c:120-205 (host pointers): new row
+ * i is old row rowperm[i]; column c becomes i where colperm[i] == c. */
+int gooseberry_permute_sparse(
+    dim_t nrows,
+    dim_t ncols,
+    ind_t * rowptr,
+    dim_t * rowind,
+    real_t * rowval,
+    const dim_t * rowperm,
+    const dim_t * colperm);
+
+/* In-place dense permutation, src/permute.c:208-245 (rev == 0) and its
+ * inverse src/permute.c:248-285 (rev != 0). */
+int gooseberry_permute_dense(
+    dim_t nrows,
+    dim_t ncols,
+    real_t * rowval,
+    const dim_t * rowperm,
+    const dim_t * colperm,
+    int rev);
+
+/* -------------------------------------------------------- device memory */
+int gooseberry_device_alloc(void ** d_ptr, size_t bytes);
+int gooseberry_device_free(void * d_ptr);
+int gooseberry_memcpy_htod(void * d_dst, const void * h_src, size_t bytes);
+int gooseberry_memcpy_dtoh(void * h_dst, const void * d_src, size_t bytes);
+int gooseberry_device_synchronize(void);
+int gooseberry_device_memory(size_t * free_bytes, size_t * total_bytes);
+/* timing events on the library stream */
+int gooseberry_event_create(void ** ev);
+int gooseberry_event_destroy(void * ev);
+int gooseberry_event_record(void * ev);
+int gooseberry_event_elapsed_ms(void * start, void * stop, float * ms);
+
+/* ------------------------------------------- device-resident CSR matrices */
+typedef struct gooseberry_matrix_s * gooseberry_matrix_t;
+
+/* SpMV plan knobs (0 = choose from the row-length histogram). */
+typedef struct gooseberry_plan_t {
+  dim_t short_max;  /* rows up to this length: LDS-staged row blocks */
+  dim_t wave_max;   /* rows up to this length: one wavefront per row */
+  dim_t window;     /* nonzeros per LDS-staged row block */
+  dim_t rowcap;     /* rows per LDS-staged row block */
+} gooseberry_plan_t;
+
+/* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */
+int gooseberry_matrix_create(
+    dim_t nrows,
+    dim_t ncols,
+    const ind_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * r_mat);
+
+/* Same from device arrays (copied; the caller keeps its buffers). */
+int gooseberry_matrix_create_device(
+    dim_t nrows,
+    dim_t ncols,
+    const ind_t * d_rowptr,
+    const dim_t * d_rowind,
+    const real_t * d_rowval,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * r_mat);
+
+int gooseberry_matrix_free(gooseberry_matrix_t mat);
+
+int gooseberry_matrix_replan(
+    gooseberry_matrix_t mat,
+    const gooseberry_plan_t * plan);
+
+typedef struct gooseberry_matrix_info_t {
+  dim_t nrows, ncols;
+  uint64_t nnz;
+  gooseberry_plan_t plan;
+  dim_t nblocks_stream;   /* LDS-staged row blocks */
+  dim_t nrows_wave;       /* rows handled one wavefront per row */
+  dim_t nrows_block;      /* rows handled one workgroup per row */
+  uint64_t hist[33];      /* rows with floor(log2(len+1)) == b */
+} gooseberry_matrix_info_t;
+
+int gooseberry_matrix_info(
+    gooseberry_matrix_t mat,
+    gooseberry_matrix_info_t * info);
+
+/* y = A x, device vectors, enqueued on the library stream. */
+int gooseberry_matrix_spmv(
+    gooseberry_matrix_t mat,
+    const real_t * d_x,
+    real_t * d_y);
+
+/* C = A B, device arrays (B column-major, C row-major). */
+int gooseberry_matrix_spmm(
+    gooseberry_matrix_t mat,
+    dim_t bncols,
+    const real_t * d_b,
+    real_t * d_c);
+
+/* ------------------------------------------------------ conjugate gradient */
+#define GOOSEBERRY_TOL_ABSOLUTE 0   /* stop when ||r||_2 <= tol (reference) */
+#define GOOSEBERRY_TOL_RELATIVE 1   /* stop when ||r||_2 <= tol * ||b||_2 */
+
+typedef struct gooseberry_cgd_info_t {
+  uint64_t niter;      /* SpMVs performed (inner SpMVs for mixed precision) */
+  double rnorm;        /* final recurrence residual sqrt(r.r) */
+  double bnorm;        /* ||b||_2 */
+  int converged;       /* rnorm reached the threshold */
+  int status;          /* 1 converged, 2 iteration cap, 3 breakdown */
+  uint64_t nrefine;    /* outer refinement steps (mixed precision) */
+  double true_rel_residual; /* ||b - A x|| / ||b|| in FP64 (mixed only) */
+} gooseberry_cgd_info_t;
+
+/* CG of src/cgd.c:40-129 on a resident matrix: x0 = 0, the reference's
+ * update order, stop when sqrt(r.r) <= threshold or after niter SpMVs
+ * (niter == 0: no cap).  b == 0 returns x = 0 at once.  d_b, d_x device
+ * vectors.  log (a FILE *, may be NULL) receives the reference's
+ * "ITER: i RMSE: v" lines. */
+int gooseberry_matrix_cgd(
+    gooseberry_matrix_t mat,
+    const real_t * d_b,
+    real_t * d_x,
+    real_t tol,
+    int tol_mode,
+    size_t niter,
+    void * log,
+    gooseberry_cgd_info_t * info);
+
+/* Host-pointer form of the reference's internal cgd() (src/cgd.h:29-34),
+ * absolute threshold `error` as in the reference. */
+int gooseberry_cgd(
+    dim_t nrows,
+    dim_t ncols,
+    const ind_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval,
+    const real_t * b,
+    real_t * x,
+    real_t error,
+    size_t niter,
+    gooseberry_cgd_info_t * info);
+
+/* ------------------------------------------------------- synthetic inputs */
+/* 2D 5-point / 3D 7-point Dirichlet Laplacians on nx*ny(*nz) grids,
+ * lexicographic order, columns ascending within a row (host arrays). */
+int gooseberry_generate_laplacian(
+    int dims,
+    dim_t nx, dim_t ny, dim_t nz,
+    dim_t * nrows,
+    ind_t ** rowptr,
+    dim_t ** rowind,
+    real_t ** rowval);
+
+/* n values uniform in [lo, hi) from (seed, stream): xoshiro256** seeded
+ * with SplitMix64, u = (x >> 11) * 2^-53. */
+int gooseberry_generate_uniform(
+    uint64_t seed,
+    uint64_t stream,
+    size_t n,
+    real_t lo,
+    real_t hi,
+    real_t * out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,122 @@
+"""GPU parity of the CG solver (src/cgd.c:40-129) through the C ABI.
+
+Bar: the same stopping rule as the reference (sqrt(r.r) <= error), the same
+relative residual reached and an iteration count within +-2 of the oracle
+at one thread; at full size, size-independent properties (true residual,
+determinism).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def true_rel(oracle, rp, ri, rv, x, b):
+    r = b - oracle.spmv(rp, ri, rv, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def test_config1_cg_vs_oracle(gb, oracle):
+    rp, ri, rv = gb.laplacian(2, 256)
+    b = gb.uniform(0, 65536)
+    err = 1e-8 * np.linalg.norm(b)
+    x, info = gb.cgd(rp, ri, rv, b, err, niter=2000)
+    xr, it, rmse, _ = oracle.cgd(rp, ri, rv, b, err, niter=2000)
+    assert info.converged == 1 and info.status == 1
+    assert abs(int(info.niter) - it) <= 2, (info.niter, it)
+    assert info.rnorm <= err and rmse <= err
+    assert true_rel(oracle, rp, ri, rv, x, b) < 2e-8
+    assert np.max(np.abs(x - xr)) <= 1e-6 * np.max(np.abs(xr))
+
+
+def test_3d_laplacian_cg_vs_oracle(gb, oracle):
+    rp, ri, rv = gb.laplacian(3, 40)
+    n = rp.size - 1
+    b = gb.uniform(1, n)
+    err = 1e-10 * np.linalg.norm(b)
+    x, info = gb.cgd(rp, ri, rv, b, err)
+    xr, it, rmse, _ = oracle.cgd(rp, ri, rv, b, err)
+    assert info.converged == 1
+    assert abs(int(info.niter) - it) <= 2, (info.niter, it)
+    assert true_rel(oracle, rp, ri, rv, x, b) < 5e-10
+
+
+def test_first_iterations_track_oracle(gb, oracle):
+    # with the reference's update order and separately rounded products, the
+    # residual history differs only through the dot-product summation order
+    rp, ri, rv = gb.laplacian(2, 64)
+    b = gb.uniform(3, 4096)
+    for k in (1, 2, 5, 20):
+        x, info = gb.cgd(rp, ri, rv, b, 0.0, niter=k)
+        xr, it, rmse, _ = oracle.cgd(rp, ri, rv, b, 0.0, niter=k)
+        assert info.niter == k and info.status == 2 and it == k
+        assert abs(info.rnorm - rmse) <= 1e-12 * np.linalg.norm(b)
+        assert np.max(np.abs(x - xr)) <= 1e-12 * np.max(np.abs(xr))
+
+
+def test_config2_full_size_true_residual(gb, oracle):
+    rp, ri, rv = gb.laplacian(3, 160)
+    n = rp.size - 1
+    b = gb.uniform(1, n)
+    M = gb.Matrix(rp, ri, rv)
+    x, info = M.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    assert info.converged == 1
+    assert info.rnorm <= 1e-10 * info.bnorm
+    assert 200 < info.niter < 2000
+    r = b - M.spmv(x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-9
+    # deterministic: fixed reduction order
+    x2, info2 = M.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    assert info2.niter == info.niter and np.array_equal(x, x2)
+
+
+def test_irregular_spd_vs_oracle(gb, oracle):
+    # symmetric, diagonally dominant, rows of very different lengths
+    rng = np.random.default_rng(4)
+    n = 6000
+    import scipy.sparse as sp
+    L = sp.random(n, n, density=0.0015, random_state=5, format="csr")
+    L.data = -rng.uniform(0, 1, L.nnz)
+    hub = sp.csr_matrix((-rng.uniform(0, 1, 3000), (np.zeros(3000, int),
+                                                    rng.choice(n, 3000, replace=False))),
+                        shape=(n, n))
+    A = L + L.T + hub + hub.T
+    A = sp.csr_matrix(A)
+    A.setdiag(0)
+    A.eliminate_zeros()
+    d = 1.01 * np.asarray(abs(A).sum(axis=1)).ravel() + 1e-3
+    A = sp.csr_matrix(A + sp.diags(d))
+    A.sort_indices()
+    b = rng.uniform(-1, 1, n)
+    err = 1e-10 * np.linalg.norm(b)
+    x, info = gb.cgd(A.indptr, A.indices, A.data, b, err)
+    xr, it, rmse, _ = oracle.cgd(A.indptr, A.indices, A.data, b, err)
+    assert info.converged == 1 and abs(int(info.niter) - it) <= 2
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-9
+
+
+def test_zero_rhs_and_caps(gb):
+    rp, ri, rv = gb.laplacian(2, 32)
+    n = rp.size - 1
+    x, info = gb.cgd(rp, ri, rv, np.zeros(n), 1e-8)
+    assert info.niter == 0 and info.converged == 1 and np.all(x == 0)
+    x, info = gb.cgd(rp, ri, rv, gb.uniform(5, n), 0.0, niter=7)
+    assert info.niter == 7 and info.status == 2 and info.converged == 0
+
+
+def test_nonsquare_rejected(gb):
+    with pytest.raises(gb.GooseberryError) as e:
+        M = gb.Matrix(np.array([0, 1, 2], np.uint32), np.array([0, 2], np.uint32),
+                      np.ones(2), ncols=3)
+        M.cgd(np.ones(2), 1e-8)
+    assert e.value.name == "INVALIDINPUT"
+
+
+def test_indefinite_breakdown_reported(gb):
+    # p.Ap == 0 at the first step: the reference would produce NaNs; the
+    # build stops and reports a breakdown
+    rp = np.array([0, 1, 2], np.uint32)
+    ri = np.array([0, 1], np.uint32)
+    rv = np.array([1.0, -1.0])
+    x, info = gb.cgd(rp, ri, rv, np.array([1.0, 1.0]), 1e-12)
+    assert info.status == 3

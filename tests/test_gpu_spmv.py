@@ -1,0 +1,186 @@
+"""GPU parity of the CSR SpMV (src/blas.c:67-92, 552-606) through the C ABI.
+
+Bar: rows handled by LDS-staged row blocks (length <= short_max) are
+bitwise the reference's left-to-right sums; every row is within
+1e-14 relative inf-norm error of the oracle (only the summation order of
+longer rows differs).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-14
+
+
+def rel_inf(a, b):
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    s = np.max(np.abs(b)) if b.size else 0.0
+    return d / s if s > 0 else d
+
+
+def irregular(n, seed, lengths=(), density=0.002, ncols=None):
+    """Random CSR with uniform columns plus rows of prescribed lengths."""
+    ncols = ncols or n
+    rng = np.random.default_rng(seed)
+    deg = rng.poisson(density * ncols, n).astype(np.int64)
+    for r, L in lengths:
+        deg[r] = L
+    rowptr = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint32)
+    rowind = rng.integers(0, ncols, int(rowptr[-1])).astype(np.uint32)
+    rowval = rng.uniform(-1, 1, int(rowptr[-1]))
+    return rowptr, rowind, rowval
+
+
+def check(gb, oracle, rowptr, rowind, rowval, x, plan=None, short_max=64):
+    n = rowptr.size - 1
+    M = gb.Matrix(rowptr, rowind, rowval, ncols=x.size, plan=plan)
+    y = M.spmv(x)
+    ref = oracle.spmult(n, x.size, 1, rowptr, rowind, rowval, x)
+    lens = np.diff(rowptr.astype(np.int64))
+    sm = (plan or {}).get("short_max", short_max) or short_max
+    short = lens <= sm
+    assert np.array_equal(y[short], ref[short]), "short rows must be bitwise"
+    assert rel_inf(y, ref) <= TOL
+    return M, y, ref
+
+
+def test_config1_laplacian2d_bitwise(gb, oracle):
+    rp, ri, rv = gb.laplacian(2, 256)
+    x = gb.uniform(0, 65536, stream=1)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    assert np.array_equal(y, ref)
+    info = M.info()
+    assert info["nrows_wave"] == 0 and info["nrows_block"] == 0
+
+
+def test_config2_laplacian3d_full_size_bitwise(gb, oracle):
+    rp, ri, rv = gb.laplacian(3, 160)
+    assert rp.size - 1 == 4096000 and rp[-1] == 28518400
+    x = gb.uniform(1, 4096000, stream=0)
+    oracle.set_threads(8)
+    try:
+        M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    finally:
+        oracle.set_threads(1)
+    assert np.array_equal(y, ref)
+    # closed form at full size: A 1 = number of missing neighbours
+    ones = M.spmv(np.ones(4096000))
+    g = np.arange(160)
+    edge = (g == 0).astype(int) + (g == 159)
+    missing = edge[None, None, :] + edge[None, :, None] + edge[:, None, None]
+    assert np.array_equal(ones, missing.ravel().astype(float))
+
+
+def test_irregular_rows_all_bins(gb, oracle):
+    n = 20000
+    lengths = [(5, 65), (6, 64), (7, 4096), (8, 4097), (9, 20000), (100, 300),
+               (19999, 50000), (200, 0), (201, 1)]
+    rp, ri, rv = irregular(n, 1, lengths)
+    x = np.random.default_rng(2).uniform(-1, 1, n)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    info = M.info()
+    assert info["nrows_wave"] >= 3 and info["nrows_block"] >= 3
+
+
+@pytest.mark.parametrize("plan", [
+    {"short_max": 1, "wave_max": 8, "window": 256, "rowcap": 1},
+    {"short_max": 8, "wave_max": 64, "window": 512, "rowcap": 64},
+    {"short_max": 64, "wave_max": 1 << 30, "window": 2240, "rowcap": 512},
+    {"short_max": 200, "wave_max": 1000, "window": 2048, "rowcap": 300},
+    {"short_max": 2303, "wave_max": 4096, "window": 1, "rowcap": 512},
+])
+def test_plans_same_results(gb, oracle, plan):
+    n = 5000
+    rp, ri, rv = irregular(n, 3, [(10, 3000), (11, 200), (12, 70), (4999, 9000)],
+                           density=0.004)
+    x = np.random.default_rng(4).uniform(-1, 1, n)
+    check(gb, oracle, rp, ri, rv, x, plan=plan)
+
+
+def test_empty_rows_and_empty_matrix(gb, oracle):
+    n = 3000
+    rng = np.random.default_rng(5)
+    deg = np.where(rng.random(n) < 0.7, 0, rng.integers(1, 5, n))
+    rp = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint32)
+    ri = rng.integers(0, n, int(rp[-1])).astype(np.uint32)
+    rv = rng.uniform(-1, 1, int(rp[-1]))
+    x = rng.uniform(-1, 1, n)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    assert np.array_equal(y, ref)
+    # no nonzeros at all
+    rp0 = np.zeros(n + 1, dtype=np.uint32)
+    M0 = gb.Matrix(rp0, np.zeros(0, np.uint32), np.zeros(0), ncols=n)
+    assert np.array_equal(M0.spmv(x), np.zeros(n))
+
+
+def test_single_row_and_single_entry(gb, oracle):
+    rp = np.array([0, 100000], dtype=np.uint32)
+    rng = np.random.default_rng(6)
+    ri = rng.integers(0, 70000, 100000).astype(np.uint32)
+    rv = rng.uniform(-1, 1, 100000)
+    x = rng.uniform(-1, 1, 70000)
+    check(gb, oracle, rp, ri, rv, x)
+    M = gb.Matrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([3.0]))
+    assert M.spmv(np.array([2.0]))[0] == 6.0
+
+
+def test_nonsquare_and_special_values(gb, oracle):
+    rp, ri, rv = irregular(700, 7, [(3, 500)], ncols=1300, density=0.01)
+    rv[::97] = 0.0
+    x = np.random.default_rng(8).uniform(-1, 1, 1300)
+    x[::50] = -0.0
+    check(gb, oracle, rp, ri, rv, x)
+
+
+def test_repeat_is_deterministic(gb):
+    rp, ri, rv = irregular(30000, 9, [(1, 10000), (2, 300)])
+    x = np.random.default_rng(10).uniform(-1, 1, 30000)
+    M = gb.Matrix(rp, ri, rv)
+    y1, y2 = M.spmv(x), M.spmv(x)
+    assert np.array_equal(y1, y2)
+
+
+def test_spmult_dropin_host_pointers(gb, oracle):
+    rp, ri, rv = irregular(4000, 11, [(0, 5000)], ncols=4500)
+    x = np.random.default_rng(12).uniform(-1, 1, 4500)
+    y = gb.spmult(4000, 4500, 1, rp, ri, rv, x)
+    ref = oracle.spmult(4000, 4500, 1, rp, ri, rv, x)
+    assert rel_inf(y, ref) <= TOL
+    # blocks: only rows inside [blocks[0], blocks[-1]) are written
+    c = np.full(4000, 7.0)
+    blocks = np.array([100, 900, 2000], dtype=np.uint32)
+    gb.spmult(4000, 4500, 1, rp, ri, rv, x, blocks=blocks, crowval=c)
+    assert np.array_equal(c[100:2000], y[100:2000])
+    assert np.all(c[:100] == 7.0) and np.all(c[2000:] == 7.0)
+    # nblocks == 0 computes nothing
+    c2 = np.full(4000, 5.0)
+    gb.spmult(4000, 4500, 1, rp, ri, rv, x, blocks=np.array([0], np.uint32), crowval=c2)
+    assert np.all(c2 == 5.0)
+
+
+def test_spmm_bncols3(gb, oracle):
+    rp, ri, rv = irregular(3000, 13, [(4, 700)], ncols=2000)
+    B = np.random.default_rng(14).uniform(-1, 1, (2000, 3))
+    bcol = B.T.ravel()
+    c = gb.spmult(3000, 2000, 3, rp, ri, rv, bcol)
+    ref = oracle.spmult(3000, 2000, 3, rp, ri, rv, bcol)
+    assert rel_inf(c, ref) <= TOL
+    lens = np.diff(rp.astype(np.int64))
+    short = np.repeat(lens <= 64, 3)
+    assert np.array_equal(c[short], ref[short])
+
+
+def test_scale_dropin(gb):
+    v = np.random.default_rng(15).uniform(-1, 1, 1000)
+    assert np.array_equal(gb.scale(v, 0.37), v * 0.37)
+
+
+def test_invalid_inputs_rejected(gb):
+    with pytest.raises(gb.GooseberryError) as e:
+        gb.Matrix(np.array([0, 2], np.uint32), np.array([0, 5], np.uint32), np.ones(2), ncols=3)
+    assert e.value.name == "INVALIDINPUT"
+    with pytest.raises(gb.GooseberryError):
+        gb.Matrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.ones(1),
+                  plan={"short_max": 2000, "window": 2000})
