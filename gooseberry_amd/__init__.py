@@ -75,7 +75,8 @@ class GooseberryError(RuntimeError):
 
 class _Plan(C.Structure):
     _fields_ = [("short_max", C.c_uint32), ("wave_max", C.c_uint32),
-                ("window", C.c_uint32), ("rowcap", C.c_uint32)]
+                ("window", C.c_uint32), ("rowcap", C.c_uint32),
+                ("exact", C.c_uint32)]
 
 
 class _MatrixInfo(C.Structure):
@@ -265,7 +266,8 @@ def _plan(plan):
     if plan is None:
         return None
     p = _Plan(plan.get("short_max", 0), plan.get("wave_max", 0),
-              plan.get("window", 0), plan.get("rowcap", 0))
+              plan.get("window", 0), plan.get("rowcap", 0),
+              1 if plan.get("exact") else 0)
     return C.byref(p)
 
 
@@ -290,7 +292,8 @@ class Matrix:
         check(lib().gooseberry_matrix_info(self.h, C.byref(i)), "gooseberry_matrix_info")
         return {"nrows": i.nrows, "ncols": i.ncols, "nnz": i.nnz,
                 "plan": {"short_max": i.plan.short_max, "wave_max": i.plan.wave_max,
-                         "window": i.plan.window, "rowcap": i.plan.rowcap},
+                         "window": i.plan.window, "rowcap": i.plan.rowcap,
+                         "exact": i.plan.exact},
                 "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
                 "nrows_block": i.nrows_block, "hist": list(i.hist)}
 

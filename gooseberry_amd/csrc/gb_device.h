@@ -75,6 +75,8 @@ typedef struct gbd_plan_params {
                           longer rows: one workgroup per row */
   uint32_t window;     /* nnz window of an LDS-staged row block */
   uint32_t rowcap;     /* max rows per LDS-staged row block */
+  uint32_t exact;      /* 1: every row summed left to right (the reference's
+                          order) also in the wavefront/workgroup paths */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram

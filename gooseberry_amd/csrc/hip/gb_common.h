@@ -55,6 +55,18 @@ __device__ __forceinline__ T gb_block_sum(T v, T *lds4) {
   return s;
 }
 
+// Value of v in lane j (j uniform across the wavefront).
+__device__ __forceinline__ double gb_readlane(double v, uint32_t j) {
+  union { double d; int i[2]; } u;
+  u.d = v;
+  u.i[0] = __builtin_amdgcn_readlane(u.i[0], j);
+  u.i[1] = __builtin_amdgcn_readlane(u.i[1], j);
+  return u.d;
+}
+__device__ __forceinline__ float gb_readlane(float v, uint32_t j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
 // Inclusive scan over a wavefront.
 __device__ __forceinline__ uint32_t gb_wave_scan_incl(uint32_t v) {
   const int lane = threadIdx.x & (GB_WAVE - 1);

@@ -142,7 +142,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ blk, uint32_t nstream,
                const uint32_t *__restrict__ wave_rows, uint32_t nwave,
                uint32_t nwaveblocks,
-               const uint32_t *__restrict__ block_rows,
+               const uint32_t *__restrict__ block_rows, uint32_t exact,
                double *__restrict__ partials, gbd_cg_state *st) {
   __shared__ T prod[GB_STREAM_LDS];
   __shared__ double red[GB_WAVES_PER_BLOCK];
@@ -214,12 +214,27 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
       const uint32_t row = wave_rows[wid];
       const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
       T s = 0;
-      for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) {
-        uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-        T v = __builtin_nontemporal_load(&val[k]);
-        s += v * x[c];
+      if (exact) {
+        // 64 products at a time, folded left to right across the lanes
+        for (uint32_t base = k0; base < k1; base += GB_WAVE) {
+          const uint32_t k = base + lane;
+          T pr = 0;
+          if (k < k1) {
+            uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+            T v = __builtin_nontemporal_load(&val[k]);
+            pr = v * x[c];
+          }
+          const uint32_t cnt = min((uint32_t)GB_WAVE, k1 - base);
+          for (uint32_t j = 0; j < cnt; ++j) s += gb_readlane(pr, j);
+        }
+      } else {
+        for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) {
+          uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+          T v = __builtin_nontemporal_load(&val[k]);
+          s += v * x[c];
+        }
+        s = gb_wave_sum(s);
       }
-      s = gb_wave_sum(s);
       if (lane == 0) {
         y[(size_t)row * ystride] = s;
         if (DOT) dotp = (double)x[row] * (double)s;
@@ -229,21 +244,42 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
     // ---- one workgroup per long row
     const uint32_t row = block_rows[b - nstream - nwaveblocks];
     const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    T s = 0;
-    for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) {
-      uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-      T v = __builtin_nontemporal_load(&val[k]);
-      s += v * x[c];
-    }
-    s = gb_wave_sum(s);
-    if (lane == 0) tred[wave] = s;
-    __syncthreads();
-    if (tid == 0) {
-      T t = tred[0];
+    if (exact) {
+      // products staged in LDS a chunk at a time, thread 0 adds them in order
+      T s = 0;
+      for (uint32_t base = k0; base < k1; base += GB_STREAM_LDS) {
+        const uint32_t cnt = min((uint32_t)GB_STREAM_LDS, k1 - base);
+        for (uint32_t e = tid; e < cnt; e += GB_BLOCK) {
+          uint32_t c = __builtin_nontemporal_load(&rowind[base + e]);
+          T v = __builtin_nontemporal_load(&val[base + e]);
+          prod[e] = v * x[c];
+        }
+        __syncthreads();
+        if (tid == 0)
+          for (uint32_t e = 0; e < cnt; ++e) s += prod[e];
+        __syncthreads();
+      }
+      if (tid == 0) {
+        y[(size_t)row * ystride] = s;
+        if (DOT) dotp = (double)x[row] * (double)s;
+      }
+    } else {
+      T s = 0;
+      for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) {
+        uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+        T v = __builtin_nontemporal_load(&val[k]);
+        s += v * x[c];
+      }
+      s = gb_wave_sum(s);
+      if (lane == 0) tred[wave] = s;
+      __syncthreads();
+      if (tid == 0) {
+        T t = tred[0];
 #pragma unroll
-      for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) t += tred[w];
-      y[(size_t)row * ystride] = t;
-      if (DOT) dotp = (double)x[row] * (double)t;
+        for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) t += tred[w];
+        y[(size_t)row * ystride] = t;
+        if (DOT) dotp = (double)x[row] * (double)t;
+      }
     }
   }
 
@@ -281,12 +317,13 @@ static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
   if (cg) {
     gb_spmv_kernel<T, true><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, partials, cg);
+        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, m->params.exact,
+        partials, cg);
   } else {
     gb_spmv_kernel<T, false><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, nullptr,
-        nullptr);
+        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, m->params.exact,
+        nullptr, nullptr);
   }
   GB_LAUNCH_CHECK();
   return GOOSEBERRY_SUCCESS;
@@ -381,6 +418,7 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->wave_max = 4096;
   p->window = 2048;
   p->rowcap = 512;
+  p->exact = 0;
 }
 
 int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,

@@ -114,6 +114,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   if (p->wave_max) d.wave_max = p->wave_max;
   if (p->window) d.window = p->window;
   if (p->rowcap) d.rowcap = p->rowcap;
+  d.exact = p->exact ? 1 : 0;
   *out = d;
   *have = 1;
 }
@@ -245,6 +246,7 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.wave_max = p.wave_max;
   info->plan.window = p.window;
   info->plan.rowcap = p.rowcap;
+  info->plan.exact = p.exact;
   return GOOSEBERRY_SUCCESS;
 }
 

@@ -229,6 +229,9 @@ typedef struct gooseberry_plan_t {
   dim_t wave_max;   /* rows up to this length: one wavefront per row */
   dim_t window;     /* nonzeros per LDS-staged row block */
   dim_t rowcap;     /* rows per LDS-staged row block */
+  dim_t exact;      /* 1: rows of any length are summed left to right from
+                       +0.0 (bitwise the reference for every row); 0: rows
+                       longer than short_max use tree reductions */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */

@@ -115,15 +115,39 @@ def test_empty_rows_and_empty_matrix(gb, oracle):
     assert np.array_equal(M0.spmv(x), np.zeros(n))
 
 
-def test_single_row_and_single_entry(gb, oracle):
+def test_single_long_cancelling_row(gb, oracle):
+    # 100000 random-sign terms: the reference's own sequential rounding error
+    # is ~1e-14 of |y| here, so a reordered sum cannot promise 1e-14.  The
+    # exact-order plan reproduces the reference bitwise; the default tree
+    # reduction stays within the textbook bound for two summation orders,
+    # |y1 - y2| <= 2 (n-1) eps sum|a_ij x_j|.
     rp = np.array([0, 100000], dtype=np.uint32)
     rng = np.random.default_rng(6)
     ri = rng.integers(0, 70000, 100000).astype(np.uint32)
     rv = rng.uniform(-1, 1, 100000)
     x = rng.uniform(-1, 1, 70000)
-    check(gb, oracle, rp, ri, rv, x)
+    ref = oracle.spmult(1, 70000, 1, rp, ri, rv, x)
+    exact = gb.Matrix(rp, ri, rv, ncols=70000, plan={"exact": 1}).spmv(x)
+    assert np.array_equal(exact, ref)
+    y = gb.Matrix(rp, ri, rv, ncols=70000).spmv(x)
+    bound = 2 * (100000 - 1) * np.finfo(float).eps * np.sum(np.abs(rv * x[ri]))
+    assert abs(y[0] - ref[0]) <= bound
+
+
+def test_single_entry(gb):
     M = gb.Matrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.array([3.0]))
     assert M.spmv(np.array([2.0]))[0] == 6.0
+
+
+@pytest.mark.parametrize("wave_max", [64, 4096, 1 << 30])
+def test_exact_plan_bitwise_every_row(gb, oracle, wave_max):
+    n = 20000
+    lengths = [(5, 65), (6, 64), (7, 4096), (8, 4097), (9, 20000), (100, 300),
+               (19999, 50000), (200, 0), (201, 1), (202, 2304), (203, 2305)]
+    rp, ri, rv = irregular(n, 16, lengths)
+    x = np.random.default_rng(17).uniform(-1, 1, n)
+    M = gb.Matrix(rp, ri, rv, plan={"exact": 1, "wave_max": wave_max})
+    assert np.array_equal(M.spmv(x), oracle.spmult(n, n, 1, rp, ri, rv, x))
 
 
 def test_nonsquare_and_special_values(gb, oracle):
