@@ -77,6 +77,7 @@ typedef struct gbd_plan_params {
   uint32_t rowcap;     /* max rows per LDS-staged row block */
   uint32_t exact;      /* 1: every row summed left to right (the reference's
                           order) also in the wavefront/workgroup paths */
+  uint32_t grid;       /* workgroups of the persistent SpMV grid (0: 2048) */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram
@@ -120,10 +121,12 @@ typedef struct gbd_cg_state {
   double pap;      /* p.Ap of the current iteration */
   double alpha;
   double beta;
+  double dr_prev;  /* dr of the iteration in flight (read by the p update) */
   double rnorm;    /* sqrt(dr) */
   double rnorm0;   /* sqrt(b.b) */
   double tol;      /* absolute stopping threshold on rnorm */
   uint64_t iter;   /* completed iterations (SpMVs) */
+  uint64_t iter_next; /* iteration in flight + 1 */
   uint64_t maxiter;/* 0 = unbounded */
   int32_t done;    /* 1: stop (converged, breakdown or maxiter) */
   int32_t status;  /* 0 running, 1 converged, 2 maxiter, 3 breakdown */

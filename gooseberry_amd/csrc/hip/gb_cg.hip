@@ -25,7 +25,8 @@ struct gbd_cg {
   const gbd_matrix *m;
   uint32_t n;
   double *r, *p, *ap;
-  double *partials;
+  double *partials;   // SpMV p.Ap partials (one per SpMV workgroup)
+  double *partials2;  // r.r partials (one per elementwise workgroup)
   uint32_t npartials;
   gbd_cg_state *st;
   gbd_cg_state *h_st;  // pinned
@@ -41,42 +42,30 @@ struct gbd_cg {
   double *graph_x;
 };
 
-// Single-pass grid reduction: block partial -> partials[blockIdx]; the
-// last block (ticket) sums partials in block order.  Returns true on
-// thread 0 of the last block with *total set.
-__device__ __forceinline__ bool gb_grid_sum(double v, double *partials,
-                                            uint32_t *ticket, double *red,
-                                            uint32_t *is_last,
-                                            double *total) {
-  double bs = gb_block_sum(v, red);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = bs;
-    __threadfence();
-    uint32_t t = atomicAdd(ticket, 1u);
-    *is_last = (t == gridDim.x - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!*is_last) return false;
-  __threadfence();
+// Fixed-order sum of n partials, computed identically by every workgroup
+// that calls it and returned to all of its threads.
+__device__ __forceinline__ double gb_sum_partials(const double *partials,
+                                                  uint32_t n, double *red) {
   double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += GB_BLOCK) s += partials[i];
-  s = gb_block_sum(s, red);
-  if (threadIdx.x == 0) {
-    *ticket = 0;
-    *total = s;
-    return true;
-  }
-  return false;
+  for (uint32_t i = threadIdx.x; i < n; i += GB_BLOCK) s += partials[i];
+  s = gb_wave_sum(s);
+  const int lane = threadIdx.x & (GB_WAVE - 1);
+  const int wave = threadIdx.x / GB_WAVE;
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  double t = red[0];
+#pragma unroll
+  for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) t += red[w];
+  __syncthreads();
+  return t;
 }
 
-// src/cgd.c:81-85: x = 0; r = b; p = r; dr = r.r
+// src/cgd.c:81-85: x = 0; r = b; p = r; partial sums of r.r
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_cg_init(const double *__restrict__ b, double *__restrict__ x,
            double *__restrict__ r, double *__restrict__ p, uint32_t n,
-           double *__restrict__ partials, gbd_cg_state *st, double tol,
-           int rel, unsigned long long maxiter) {
+           double *__restrict__ partials) {
   __shared__ double red[GB_WAVES_PER_BLOCK];
-  __shared__ uint32_t is_last;
   double d = 0.0;
   for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
        i += gridDim.x * GB_BLOCK) {
@@ -86,9 +75,20 @@ gb_cg_init(const double *__restrict__ b, double *__restrict__ x,
     p[i] = v;
     d += v * v;
   }
-  double s;
-  if (gb_grid_sum(d, partials, &st->ticket[1], red, &is_last, &s)) {
+  double bs = gb_block_sum(d, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// dr = r.r, ||b||, the absolute threshold; one workgroup
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_cg_init_state(const double *__restrict__ partials, uint32_t np,
+                 gbd_cg_state *st, double tol, int rel,
+                 unsigned long long maxiter) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  double s = gb_sum_partials(partials, np, red);
+  if (threadIdx.x == 0) {
     st->dr = s;
+    st->dr_prev = s;
     st->rnorm0 = sqrt(s);
     st->rnorm = st->rnorm0;
     st->tol = rel ? tol * st->rnorm0 : tol;
@@ -96,6 +96,7 @@ gb_cg_init(const double *__restrict__ b, double *__restrict__ x,
     st->alpha = 0.0;
     st->beta = 0.0;
     st->iter = 0;
+    st->iter_next = 0;
     st->maxiter = maxiter;
     st->status = 0;
     st->done = 0;
@@ -109,16 +110,35 @@ gb_cg_init(const double *__restrict__ b, double *__restrict__ x,
   }
 }
 
-// src/cgd.c:92-101: ps = alpha p; nx = x + ps; nr = r - alpha ap; dnr = nr.nr
+// src/cgd.c:90-97: alpha = dr / p.Ap (every workgroup adds the SpMV's
+// partials in the same order); ps = alpha p; x += ps; r -= alpha Ap;
+// partial sums of r.r
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_cg_update_xr(double *__restrict__ x, double *__restrict__ r,
                 const double *__restrict__ p, const double *__restrict__ ap,
-                uint32_t n, double *__restrict__ partials, gbd_cg_state *st,
-                double *__restrict__ trace, uint32_t trace_cap) {
+                uint32_t n, const double *__restrict__ pap_partials,
+                uint32_t npap, double *__restrict__ partials,
+                gbd_cg_state *st) {
   __shared__ double red[GB_WAVES_PER_BLOCK];
-  __shared__ uint32_t is_last;
-  if (st->done) return;
-  const double alpha = st->alpha;
+  if (gb_block_flag(&st->done)) return;
+  const double pap = gb_sum_partials(pap_partials, npap, red);
+  const double dr = st->dr;
+  const double alpha = dr / pap;
+  if (!(pap != 0.0) || !isfinite(alpha)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->pap = pap;
+      st->alpha = alpha;
+      st->done = 1;
+      st->status = 3;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pap = pap;
+    st->alpha = alpha;
+    st->dr_prev = dr;
+    st->iter_next = st->iter + 1;
+  }
   double d = 0.0;
   for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
        i += gridDim.x * GB_BLOCK) {
@@ -129,36 +149,41 @@ gb_cg_update_xr(double *__restrict__ x, double *__restrict__ r,
     r[i] = rn;
     d += rn * rn;
   }
-  double dnr;
-  if (gb_grid_sum(d, partials, &st->ticket[1], red, &is_last, &dnr)) {
-    // src/cgd.c:100-119
-    st->beta = dnr / st->dr;
-    st->dr = dnr;
-    double rnorm = sqrt(dnr);
-    st->rnorm = rnorm;
-    unsigned long long it = st->iter;
-    if (trace_cap) trace[it % trace_cap] = rnorm;
-    st->iter = it + 1;
-    if (rnorm <= st->tol) {
-      st->done = 1;
-      st->status = 1;
-    } else if (!isfinite(dnr)) {
-      st->done = 1;
-      st->status = 3;
-    } else if (st->maxiter && it + 1 >= st->maxiter) {
-      st->done = 1;
-      st->status = 2;
-    }
-  }
+  double bs = gb_block_sum(d, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
-// src/cgd.c:104-105: p = nr + (beta/alpha) * (alpha p)
+// src/cgd.c:100-119: beta = r.r / dr; p = nr + (beta/alpha) (alpha p);
+// stop when sqrt(r.r) <= tol or at the iteration cap
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_cg_update_p(const double *__restrict__ r, double *__restrict__ p,
-               uint32_t n, const gbd_cg_state *st) {
-  if (st->done) return;
+               uint32_t n, const double *__restrict__ dnr_partials,
+               uint32_t ndnr, gbd_cg_state *st, double *__restrict__ trace,
+               uint32_t trace_cap) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  if (gb_block_flag(&st->done)) return;
+  const double dnr = gb_sum_partials(dnr_partials, ndnr, red);
   const double alpha = st->alpha;
-  const double boa = st->beta / alpha;
+  const double beta = dnr / st->dr_prev;
+  const double rnorm = sqrt(dnr);
+  const unsigned long long it1 = st->iter_next;
+  int status = 0;
+  if (rnorm <= st->tol) status = 1;
+  else if (!isfinite(dnr)) status = 3;
+  else if (st->maxiter && it1 >= st->maxiter) status = 2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->beta = beta;
+    st->dr = dnr;
+    st->rnorm = rnorm;
+    if (trace_cap) trace[(it1 - 1) % trace_cap] = rnorm;
+    st->iter = it1;
+    if (status) {
+      st->status = status;
+      st->done = 1;
+    }
+  }
+  if (status) return;
+  const double boa = beta / alpha;
   for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
        i += gridDim.x * GB_BLOCK) {
     double ps = alpha * p[i];
@@ -170,14 +195,17 @@ gb_cg_update_p(const double *__restrict__ r, double *__restrict__ p,
 static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
   hipStream_t s = gb_stream();
   const uint32_t n = cg->n;
+  const uint32_t g1 = gb_spmv_grid(cg->m);
   for (uint32_t it = 0; it < k; ++it) {
     int rv = gb_spmv_launch(cg->m, cg->p, cg->ap, 1, cg->partials, cg->st);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
     gb_cg_update_xr<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
-        cg->x, cg->r, cg->p, cg->ap, n, cg->partials, cg->st, cg->trace,
-        cg->trace_cap);
+        cg->x, cg->r, cg->p, cg->ap, n, cg->partials, g1, cg->partials2,
+        cg->st);
     GB_LAUNCH_CHECK();
-    gb_cg_update_p<<<cg->grid_elem, GB_BLOCK, 0, s>>>(cg->r, cg->p, n, cg->st);
+    gb_cg_update_p<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
+        cg->r, cg->p, n, cg->partials2, cg->grid_elem, cg->st, cg->trace,
+        cg->trace_cap);
     GB_LAUNCH_CHECK();
   }
   return GOOSEBERRY_SUCCESS;
@@ -212,6 +240,8 @@ int gbd_cg_create(const gbd_matrix *m, gbd_cg **out) {
   if (e == hipSuccess) e = hipMalloc((void **)&cg->ap, nb);
   if (e == hipSuccess)
     e = hipMalloc((void **)&cg->partials, (size_t)cg->npartials * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc((void **)&cg->partials2, (size_t)cg->grid_elem * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&cg->st, sizeof(gbd_cg_state));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&cg->h_st, sizeof(gbd_cg_state), hipHostMallocDefault);
@@ -232,6 +262,7 @@ void gbd_cg_destroy(gbd_cg *cg) {
   hipFree(cg->p);
   hipFree(cg->ap);
   hipFree(cg->partials);
+  hipFree(cg->partials2);
   hipFree(cg->st);
   hipFree(cg->trace);
   if (cg->h_st) hipHostFree(cg->h_st);
@@ -258,8 +289,10 @@ int gbd_cg_start(gbd_cg *cg, const double *d_b, double *d_x, double tol,
   cg->b = d_b;
   cg->x = d_x;
   gb_cg_init<<<cg->grid_elem, GB_BLOCK, 0, s>>>(d_b, d_x, cg->r, cg->p, cg->n,
-                                                cg->partials, cg->st, tol, rel,
-                                                (unsigned long long)maxiter);
+                                                cg->partials2);
+  GB_LAUNCH_CHECK();
+  gb_cg_init_state<<<1, GB_BLOCK, 0, s>>>(cg->partials2, cg->grid_elem, cg->st,
+                                          tol, rel, (unsigned long long)maxiter);
   GB_LAUNCH_CHECK();
   return GOOSEBERRY_SUCCESS;
 }

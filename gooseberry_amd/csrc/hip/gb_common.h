@@ -55,6 +55,18 @@ __device__ __forceinline__ T gb_block_sum(T v, T *lds4) {
   return s;
 }
 
+// Reads a device flag once per workgroup (thread 0) and hands the same
+// value to every thread, so a flag that another workgroup of the same
+// launch may be writing still gives one decision per workgroup.
+__device__ __forceinline__ int gb_block_flag(const int32_t *flag) {
+  __shared__ int32_t v;
+  if (threadIdx.x == 0) v = *(const volatile int32_t *)flag;
+  __syncthreads();
+  int32_t r = v;
+  __syncthreads();
+  return r;
+}
+
 // Value of v in lane j (j uniform across the wavefront).
 __device__ __forceinline__ double gb_readlane(double v, uint32_t j) {
   union { double d; int i[2]; } u;

@@ -182,6 +182,24 @@ __global__ void gb_cs_hist(const uint32_t *__restrict__ keys, size_t n,
   for (; i < n; i += stride) atomicAdd(&hist[keys[i] - kmin], 1u);
 }
 
+// Few keys: count in LDS per workgroup, then one global add per key, so
+// that millions of increments of a handful of counters do not serialise
+// on the same global addresses.
+#define CS_HIST_LDS 4096
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_cs_hist_lds(const uint32_t *__restrict__ keys, size_t n, uint32_t kmin,
+               uint32_t range, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[CS_HIST_LDS];
+  for (uint32_t k = threadIdx.x; k < range; k += GB_BLOCK) h[k] = 0;
+  __syncthreads();
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) atomicAdd(&h[keys[i] - kmin], 1u);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < range; k += GB_BLOCK)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
 __global__ void gb_cs_check_keys(const uint32_t *__restrict__ keys, size_t n,
                                  uint32_t kmin, uint32_t kmax,
                                  uint32_t *__restrict__ bad) {
@@ -317,7 +335,13 @@ int gbd_countingsort_kv(const uint32_t *d_keys, const uint32_t *d_vals,
   if (d_dist) {
     if (range > 0xfffffffeull) return GOOSEBERRY_ERROR_INVALIDINPUT;
     GB_HIP(hipMemsetAsync(d_dist, 0, (range + 1) * sizeof(uint32_t), s));
-    if (n) {
+    if (n && range <= CS_HIST_LDS) {
+      uint32_t g = gb_grid_stride_blocks(n);
+      if (g > 1024u) g = 1024u;
+      gb_cs_hist_lds<<<g, GB_BLOCK, 0, s>>>(d_keys, n, kmin, (uint32_t)range,
+                                            d_dist);
+      GB_LAUNCH_CHECK();
+    } else if (n) {
       gb_cs_hist<<<gb_grid_stride_blocks(n), GB_BLOCK, 0, s>>>(d_keys, n, kmin,
                                                               d_dist);
       GB_LAUNCH_CHECK();

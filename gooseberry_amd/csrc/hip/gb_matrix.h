@@ -31,15 +31,20 @@ struct gbd_matrix {
 static inline uint32_t gb_spmv_nwaveblocks(const gbd_matrix *m) {
   return (m->nwave + GB_WAVES_PER_BLOCK - 1) / GB_WAVES_PER_BLOCK;
 }
-static inline uint32_t gb_spmv_grid(const gbd_matrix *m) {
+static inline uint32_t gb_spmv_nwork(const gbd_matrix *m) {
   return m->nstream + gb_spmv_nwaveblocks(m) + m->nblockrows;
 }
+// persistent grid: 8 workgroups of 256 threads on each of the 256 CUs
+#define GB_GRID_DEFAULT 2048u
+// workgroups of one SpMV launch (the number of p.Ap partials with DOT)
+uint32_t gb_spmv_grid(const gbd_matrix *m);
 
-// Enqueue y = A x; when cg != nullptr also reduce p.Ap (x = p, y = Ap) into
-// the CG state (alpha = dr / p.Ap) using partials[gb_spmv_grid(m)].
+// Enqueue y = A x; when cg != nullptr (and cg->done is 0) also write the
+// per-workgroup partial sums of p.Ap (x = p, y = Ap) to
+// partials[gb_spmv_grid(m)].
 int gb_spmv_launch(const gbd_matrix *m, const double *x, double *y,
-                   uint32_t ystride, double *partials, gbd_cg_state *cg);
+                   uint32_t ystride, double *partials, const gbd_cg_state *cg);
 int gb_spmv_launch_f32(const gbd_matrix *m, const float *x, float *y,
-                       double *partials, gbd_cg_state *cg);
+                       double *partials, const gbd_cg_state *cg);
 
 #endif

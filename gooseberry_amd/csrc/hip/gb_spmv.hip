@@ -117,22 +117,13 @@ gb_plan_blocks(const uint32_t *__restrict__ cls,
 
 // --------------------------------------------------------------- kernels
 
-template <typename T>
-__device__ __forceinline__ void gb_cg_alpha_finalize(double s,
-                                                     gbd_cg_state *st) {
-  st->pap = s;
-  double alpha = st->dr / s;
-  st->alpha = alpha;
-  if (!(s != 0.0) || !isfinite(alpha)) {
-    st->done = 1;
-    st->status = 3;
-  }
-}
-
-// One launch covers all bins.  With DOT the kernel also forms p.Ap (x = p,
-// y = Ap) with a fixed-order single-pass grid reduction: each block writes
-// its partial, the last block to arrive (atomic ticket) adds the partials
-// in block order and sets alpha = dr / p.Ap in the CG state.
+// One launch covers all bins.  The grid is persistent: at most
+// `grid` workgroups (a multiple of 8 when capped) walk the work items
+// w = blockIdx.x, blockIdx.x + gridDim.x, ...: first the LDS-staged row
+// blocks (XCD-remapped), then groups of four wave rows, then long rows.
+// With DOT the kernel also forms the block's share of p.Ap (x = p, y = Ap)
+// and writes it to partials[blockIdx.x]; the next CG kernel adds the
+// partials in a fixed order (no fences or atomics: deterministic).
 template <typename T, bool DOT>
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
@@ -142,200 +133,195 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ blk, uint32_t nstream,
                const uint32_t *__restrict__ wave_rows, uint32_t nwave,
                uint32_t nwaveblocks,
-               const uint32_t *__restrict__ block_rows, uint32_t exact,
-               double *__restrict__ partials, gbd_cg_state *st) {
+               const uint32_t *__restrict__ block_rows, uint32_t nwork,
+               uint32_t exact, double *__restrict__ partials,
+               const gbd_cg_state *st) {
   __shared__ T prod[GB_STREAM_LDS];
   __shared__ double red[GB_WAVES_PER_BLOCK];
   __shared__ T tred[GB_WAVES_PER_BLOCK];
-  __shared__ uint32_t is_last;
   if (DOT) {
-    if (st->done) return;
+    if (gb_block_flag(&st->done)) return;
   }
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & (GB_WAVE - 1);
   const uint32_t wave = tid / GB_WAVE;
-  const uint32_t b = blockIdx.x;
   double dotp = 0.0;
 
-  if (b < nstream) {
-    // ---- LDS-staged row block of short rows
-    const uint32_t lb = gb_xcd_remap(b, nstream);
-    const uint32_t r0 = blk[2 * lb], r1 = blk[2 * lb + 1];
-    const uint32_t k0 = rowptr[r0];
-    const uint32_t cnt = rowptr[r1] - k0;
-    // row bounds of this thread's (at most two) rows, loaded together with
-    // the block's nonzeros so that all loads are in flight at once
-    const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
-    uint32_t pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
-    if (ra < r1) { pa0 = rowptr[ra]; pa1 = rowptr[ra + 1]; }
-    if (rb < r1) { pb0 = rowptr[rb]; pb1 = rowptr[rb + 1]; }
-    if (cnt > 0) {
-      // stage val*x[col] for the block's nonzeros; out-of-range lanes
-      // re-read the last entry (in bounds) and store nothing
-      const uint32_t klast = k0 + cnt - 1;
-      uint32_t c[GB_STREAM_ITERS];
-      T v[GB_STREAM_ITERS], xv[GB_STREAM_ITERS];
+  for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+    if (w < nstream) {
+      // ---- LDS-staged row block of short rows
+      const uint32_t lb = gb_xcd_remap(w, nstream);
+      const uint32_t r0 = blk[2 * lb], r1 = blk[2 * lb + 1];
+      const uint32_t k0 = rowptr[r0];
+      const uint32_t cnt = rowptr[r1] - k0;
+      // row bounds of this thread's (at most two) rows, loaded together
+      // with the block's nonzeros so that all loads are in flight at once
+      const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
+      uint32_t pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
+      if (ra < r1) { pa0 = rowptr[ra]; pa1 = rowptr[ra + 1]; }
+      if (rb < r1) { pb0 = rowptr[rb]; pb1 = rowptr[rb + 1]; }
+      if (cnt > 0) {
+        // stage val*x[col] for the block's nonzeros; out-of-range lanes
+        // re-read the last entry (in bounds) and store nothing
+        const uint32_t klast = k0 + cnt - 1;
+        uint32_t c[GB_STREAM_ITERS];
+        T v[GB_STREAM_ITERS], xv[GB_STREAM_ITERS];
 #pragma unroll
-      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-        if (i * GB_BLOCK < cnt) {
-          uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
-          c[i] = __builtin_nontemporal_load(&rowind[k]);
-          v[i] = __builtin_nontemporal_load(&val[k]);
+        for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+          if (i * GB_BLOCK < cnt) {
+            uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
+            c[i] = __builtin_nontemporal_load(&rowind[k]);
+            v[i] = __builtin_nontemporal_load(&val[k]);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+          if (i * GB_BLOCK < cnt) xv[i] = x[c[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+          uint32_t e = i * GB_BLOCK + tid;
+          if (i * GB_BLOCK < cnt && e < cnt) prod[e] = v[i] * xv[i];
         }
       }
-#pragma unroll
-      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-        if (i * GB_BLOCK < cnt) xv[i] = x[c[i]];
+      __syncthreads();
+      // one thread per row: the reference's left-to-right sum from +0.0
+      if (ra < r1) {
+        T s = 0;
+        for (uint32_t k = pa0 - k0; k < pa1 - k0; ++k) s += prod[k];
+        y[(size_t)ra * ystride] = s;
+        if (DOT) dotp += (double)x[ra] * (double)s;
       }
-#pragma unroll
-      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-        uint32_t e = i * GB_BLOCK + tid;
-        if (i * GB_BLOCK < cnt && e < cnt) prod[e] = v[i] * xv[i];
+      if (rb < r1) {
+        T s = 0;
+        for (uint32_t k = pb0 - k0; k < pb1 - k0; ++k) s += prod[k];
+        y[(size_t)rb * ystride] = s;
+        if (DOT) dotp += (double)x[rb] * (double)s;
       }
-    }
-    __syncthreads();
-    // one thread per row: the reference's left-to-right sum from +0.0
-    if (ra < r1) {
-      T s = 0;
-      for (uint32_t k = pa0 - k0; k < pa1 - k0; ++k) s += prod[k];
-      y[(size_t)ra * ystride] = s;
-      if (DOT) dotp += (double)x[ra] * (double)s;
-    }
-    if (rb < r1) {
-      T s = 0;
-      for (uint32_t k = pb0 - k0; k < pb1 - k0; ++k) s += prod[k];
-      y[(size_t)rb * ystride] = s;
-      if (DOT) dotp += (double)x[rb] * (double)s;
-    }
-  } else if (b < nstream + nwaveblocks) {
-    // ---- one wavefront per medium row
-    const uint32_t wid = (b - nstream) * GB_WAVES_PER_BLOCK + wave;
-    if (wid < nwave) {
-      const uint32_t row = wave_rows[wid];
-      const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-      T s = 0;
-      if (exact) {
-        // 64 products at a time, folded left to right across the lanes
-        for (uint32_t base = k0; base < k1; base += GB_WAVE) {
-          const uint32_t k = base + lane;
-          T pr = 0;
-          if (k < k1) {
+    } else if (w < nstream + nwaveblocks) {
+      // ---- one wavefront per medium row
+      const uint32_t wid = (w - nstream) * GB_WAVES_PER_BLOCK + wave;
+      if (wid < nwave) {
+        const uint32_t row = wave_rows[wid];
+        const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+        T s = 0;
+        if (exact) {
+          // 64 products at a time, folded left to right across the lanes
+          for (uint32_t base = k0; base < k1; base += GB_WAVE) {
+            const uint32_t k = base + lane;
+            T pr = 0;
+            if (k < k1) {
+              uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+              T v = __builtin_nontemporal_load(&val[k]);
+              pr = v * x[c];
+            }
+            const uint32_t cnt = min((uint32_t)GB_WAVE, k1 - base);
+            for (uint32_t j = 0; j < cnt; ++j) s += gb_readlane(pr, j);
+          }
+        } else {
+          for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) {
             uint32_t c = __builtin_nontemporal_load(&rowind[k]);
             T v = __builtin_nontemporal_load(&val[k]);
-            pr = v * x[c];
+            s += v * x[c];
           }
-          const uint32_t cnt = min((uint32_t)GB_WAVE, k1 - base);
-          for (uint32_t j = 0; j < cnt; ++j) s += gb_readlane(pr, j);
+          s = gb_wave_sum(s);
+        }
+        if (lane == 0) {
+          y[(size_t)row * ystride] = s;
+          if (DOT) dotp += (double)x[row] * (double)s;
+        }
+      }
+    } else {
+      // ---- one workgroup per long row
+      const uint32_t row = block_rows[w - nstream - nwaveblocks];
+      const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+      if (exact) {
+        // products staged in LDS a chunk at a time, thread 0 adds them in
+        // order
+        T s = 0;
+        for (uint32_t base = k0; base < k1; base += GB_STREAM_LDS) {
+          const uint32_t cnt = min((uint32_t)GB_STREAM_LDS, k1 - base);
+          for (uint32_t e = tid; e < cnt; e += GB_BLOCK) {
+            uint32_t c = __builtin_nontemporal_load(&rowind[base + e]);
+            T v = __builtin_nontemporal_load(&val[base + e]);
+            prod[e] = v * x[c];
+          }
+          __syncthreads();
+          if (tid == 0)
+            for (uint32_t e = 0; e < cnt; ++e) s += prod[e];
+          __syncthreads();
+        }
+        if (tid == 0) {
+          y[(size_t)row * ystride] = s;
+          if (DOT) dotp += (double)x[row] * (double)s;
         }
       } else {
-        for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) {
+        T s = 0;
+        for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) {
           uint32_t c = __builtin_nontemporal_load(&rowind[k]);
           T v = __builtin_nontemporal_load(&val[k]);
           s += v * x[c];
         }
         s = gb_wave_sum(s);
-      }
-      if (lane == 0) {
-        y[(size_t)row * ystride] = s;
-        if (DOT) dotp = (double)x[row] * (double)s;
-      }
-    }
-  } else {
-    // ---- one workgroup per long row
-    const uint32_t row = block_rows[b - nstream - nwaveblocks];
-    const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    if (exact) {
-      // products staged in LDS a chunk at a time, thread 0 adds them in order
-      T s = 0;
-      for (uint32_t base = k0; base < k1; base += GB_STREAM_LDS) {
-        const uint32_t cnt = min((uint32_t)GB_STREAM_LDS, k1 - base);
-        for (uint32_t e = tid; e < cnt; e += GB_BLOCK) {
-          uint32_t c = __builtin_nontemporal_load(&rowind[base + e]);
-          T v = __builtin_nontemporal_load(&val[base + e]);
-          prod[e] = v * x[c];
-        }
+        if (lane == 0) tred[wave] = s;
         __syncthreads();
-        if (tid == 0)
-          for (uint32_t e = 0; e < cnt; ++e) s += prod[e];
-        __syncthreads();
-      }
-      if (tid == 0) {
-        y[(size_t)row * ystride] = s;
-        if (DOT) dotp = (double)x[row] * (double)s;
-      }
-    } else {
-      T s = 0;
-      for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) {
-        uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-        T v = __builtin_nontemporal_load(&val[k]);
-        s += v * x[c];
-      }
-      s = gb_wave_sum(s);
-      if (lane == 0) tred[wave] = s;
-      __syncthreads();
-      if (tid == 0) {
-        T t = tred[0];
+        if (tid == 0) {
+          T t = tred[0];
 #pragma unroll
-        for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) t += tred[w];
-        y[(size_t)row * ystride] = t;
-        if (DOT) dotp = (double)x[row] * (double)t;
+          for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += tred[q];
+          y[(size_t)row * ystride] = t;
+          if (DOT) dotp += (double)x[row] * (double)t;
+        }
       }
     }
+    // the LDS buffers are reused by the next work item
+    __syncthreads();
   }
 
   if (DOT) {
     double bs = gb_block_sum(dotp, red);
-    if (tid == 0) {
-      partials[b] = bs;
-      __threadfence();
-      uint32_t t = atomicAdd(&st->ticket[0], 1u);
-      is_last = (t == gridDim.x - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (is_last) {
-      __threadfence();
-      double s = 0.0;
-      for (uint32_t i = tid; i < gridDim.x; i += GB_BLOCK) s += partials[i];
-      s = gb_block_sum(s, red);
-      if (tid == 0) {
-        gb_cg_alpha_finalize<T>(s, st);
-        st->ticket[0] = 0;
-      }
-    }
+    if (tid == 0) partials[blockIdx.x] = bs;
   }
+}
+
+uint32_t gb_spmv_grid(const gbd_matrix *m) {
+  const uint64_t nwork = gb_spmv_nwork(m);
+  const uint64_t gmax = m->params.grid ? m->params.grid : GB_GRID_DEFAULT;
+  return (uint32_t)(nwork < gmax ? nwork : gmax);
 }
 
 template <typename T>
 static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
                             T *y, uint32_t ystride, double *partials,
-                            gbd_cg_state *cg) {
+                            const gbd_cg_state *cg) {
   const uint32_t grid = gb_spmv_grid(m);
   if (grid == 0) return GOOSEBERRY_SUCCESS;
   hipStream_t s = gb_stream();
   const uint32_t *wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
   const uint32_t *block_rows = m->rowbins + (m->nrows - m->nblockrows);
+  const uint32_t nwork = gb_spmv_nwork(m);
   if (cg) {
     gb_spmv_kernel<T, true><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, m->params.exact,
-        partials, cg);
+        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, nwork,
+        m->params.exact, partials, cg);
   } else {
     gb_spmv_kernel<T, false><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, m->params.exact,
-        nullptr, nullptr);
+        wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, nwork,
+        m->params.exact, nullptr, nullptr);
   }
   GB_LAUNCH_CHECK();
   return GOOSEBERRY_SUCCESS;
 }
 
 int gb_spmv_launch(const gbd_matrix *m, const double *x, double *y,
-                   uint32_t ystride, double *partials, gbd_cg_state *cg) {
+                   uint32_t ystride, double *partials, const gbd_cg_state *cg) {
   return gb_spmv_launch_t<double>(m, m->val, x, y, ystride, partials, cg);
 }
 
 int gb_spmv_launch_f32(const gbd_matrix *m, const float *x, float *y,
-                       double *partials, gbd_cg_state *cg) {
+                       double *partials, const gbd_cg_state *cg) {
   if (!m->val32) return GOOSEBERRY_ERROR_INVALIDINPUT;
   return gb_spmv_launch_t<float>(m, m->val32, x, y, 1, partials, cg);
 }
@@ -354,6 +340,7 @@ static int gb_plan_check(const gbd_plan_params *p) {
   if (p->wave_max < p->short_max) return 0;
   // the row-block kernel gives each thread at most two rows
   if (p->rowcap > 2 * GB_BLOCK) return 0;
+  if (p->grid > 65536) return 0;
   return 1;
 }
 
@@ -419,6 +406,7 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->window = 2048;
   p->rowcap = 512;
   p->exact = 0;
+  p->grid = 0;
 }
 
 int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,

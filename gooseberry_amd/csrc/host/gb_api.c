@@ -115,6 +115,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   if (p->window) d.window = p->window;
   if (p->rowcap) d.rowcap = p->rowcap;
   d.exact = p->exact ? 1 : 0;
+  if (p->grid) d.grid = p->grid;
   *out = d;
   *have = 1;
 }
@@ -247,6 +248,7 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.window = p.window;
   info->plan.rowcap = p.rowcap;
   info->plan.exact = p.exact;
+  info->plan.grid = p.grid;
   return GOOSEBERRY_SUCCESS;
 }
 

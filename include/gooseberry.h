@@ -232,6 +232,7 @@ typedef struct gooseberry_plan_t {
   dim_t exact;      /* 1: rows of any length are summed left to right from
                        +0.0 (bitwise the reference for every row); 0: rows
                        longer than short_max use tree reductions */
+  dim_t grid;       /* workgroups of the persistent SpMV grid */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */
