@@ -76,7 +76,8 @@ class GooseberryError(RuntimeError):
 class _Plan(C.Structure):
     _fields_ = [("short_max", C.c_uint32), ("wave_max", C.c_uint32),
                 ("window", C.c_uint32), ("rowcap", C.c_uint32),
-                ("exact", C.c_uint32), ("grid", C.c_uint32)]
+                ("exact", C.c_uint32), ("grid", C.c_uint32),
+                ("xcd_remap", C.c_uint32)]
 
 
 class _MatrixInfo(C.Structure):
@@ -267,7 +268,8 @@ def _plan(plan):
         return None
     p = _Plan(plan.get("short_max", 0), plan.get("wave_max", 0),
               plan.get("window", 0), plan.get("rowcap", 0),
-              1 if plan.get("exact") else 0, plan.get("grid", 0))
+              1 if plan.get("exact") else 0, plan.get("grid", 0),
+              1 if plan.get("xcd_remap") else 0)
     return C.byref(p)
 
 
@@ -293,7 +295,8 @@ class Matrix:
         return {"nrows": i.nrows, "ncols": i.ncols, "nnz": i.nnz,
                 "plan": {"short_max": i.plan.short_max, "wave_max": i.plan.wave_max,
                          "window": i.plan.window, "rowcap": i.plan.rowcap,
-                         "exact": i.plan.exact, "grid": i.plan.grid},
+                         "exact": i.plan.exact, "grid": i.plan.grid,
+                         "xcd_remap": i.plan.xcd_remap},
                 "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
                 "nrows_block": i.nrows_block, "hist": list(i.hist)}
 

@@ -77,7 +77,9 @@ typedef struct gbd_plan_params {
   uint32_t rowcap;     /* max rows per LDS-staged row block */
   uint32_t exact;      /* 1: every row summed left to right (the reference's
                           order) also in the wavefront/workgroup paths */
-  uint32_t grid;       /* workgroups of the persistent SpMV grid (0: 2048) */
+  uint32_t grid;       /* cap on SpMV workgroups (0: one per work item) */
+  uint32_t xcd_remap;  /* 1: renumber row blocks so each XCD walks a
+                          contiguous range */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram

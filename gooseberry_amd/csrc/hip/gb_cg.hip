@@ -110,35 +110,38 @@ gb_cg_init_state(const double *__restrict__ partials, uint32_t np,
   }
 }
 
-// src/cgd.c:90-97: alpha = dr / p.Ap (every workgroup adds the SpMV's
-// partials in the same order); ps = alpha p; x += ps; r -= alpha Ap;
-// partial sums of r.r
+// src/cgd.c:90: alpha = dr / p.Ap from the SpMV's per-row-block partials,
+// added in a fixed order; one workgroup
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_cg_update_xr(double *__restrict__ x, double *__restrict__ r,
-                const double *__restrict__ p, const double *__restrict__ ap,
-                uint32_t n, const double *__restrict__ pap_partials,
-                uint32_t npap, double *__restrict__ partials,
-                gbd_cg_state *st) {
+gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
+            gbd_cg_state *st) {
   __shared__ double red[GB_WAVES_PER_BLOCK];
   if (gb_block_flag(&st->done)) return;
   const double pap = gb_sum_partials(pap_partials, npap, red);
-  const double dr = st->dr;
-  const double alpha = dr / pap;
-  if (!(pap != 0.0) || !isfinite(alpha)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->pap = pap;
-      st->alpha = alpha;
-      st->done = 1;
-      st->status = 3;
-    }
-    return;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
+    const double dr = st->dr;
+    const double alpha = dr / pap;
     st->pap = pap;
     st->alpha = alpha;
-    st->dr_prev = dr;
-    st->iter_next = st->iter + 1;
+    if (!(pap != 0.0) || !isfinite(alpha)) {
+      st->done = 1;
+      st->status = 3;
+    } else {
+      st->dr_prev = dr;
+      st->iter_next = st->iter + 1;
+    }
   }
+}
+
+// src/cgd.c:92-97: ps = alpha p; x += ps; r -= alpha Ap; partial sums of r.r
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_cg_update_xr(double *__restrict__ x, double *__restrict__ r,
+                const double *__restrict__ p, const double *__restrict__ ap,
+                uint32_t n, double *__restrict__ partials,
+                const gbd_cg_state *st) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  if (gb_block_flag(&st->done)) return;
+  const double alpha = st->alpha;
   double d = 0.0;
   for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
        i += gridDim.x * GB_BLOCK) {
@@ -199,9 +202,10 @@ static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
   for (uint32_t it = 0; it < k; ++it) {
     int rv = gb_spmv_launch(cg->m, cg->p, cg->ap, 1, cg->partials, cg->st);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
+    gb_cg_alpha<<<1, GB_BLOCK, 0, s>>>(cg->partials, g1, cg->st);
+    GB_LAUNCH_CHECK();
     gb_cg_update_xr<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
-        cg->x, cg->r, cg->p, cg->ap, n, cg->partials, g1, cg->partials2,
-        cg->st);
+        cg->x, cg->r, cg->p, cg->ap, n, cg->partials2, cg->st);
     GB_LAUNCH_CHECK();
     gb_cg_update_p<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
         cg->r, cg->p, n, cg->partials2, cg->grid_elem, cg->st, cg->trace,

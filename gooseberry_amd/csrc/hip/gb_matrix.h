@@ -34,8 +34,6 @@ static inline uint32_t gb_spmv_nwaveblocks(const gbd_matrix *m) {
 static inline uint32_t gb_spmv_nwork(const gbd_matrix *m) {
   return m->nstream + gb_spmv_nwaveblocks(m) + m->nblockrows;
 }
-// persistent grid: 8 workgroups of 256 threads on each of the 256 CUs
-#define GB_GRID_DEFAULT 2048u
 // workgroups of one SpMV launch (the number of p.Ap partials with DOT)
 uint32_t gb_spmv_grid(const gbd_matrix *m);
 

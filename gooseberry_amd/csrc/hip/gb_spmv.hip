@@ -117,10 +117,12 @@ gb_plan_blocks(const uint32_t *__restrict__ cls,
 
 // --------------------------------------------------------------- kernels
 
-// One launch covers all bins.  The grid is persistent: at most
-// `grid` workgroups (a multiple of 8 when capped) walk the work items
+// One launch covers all bins.  Workgroups walk the work items
 // w = blockIdx.x, blockIdx.x + gridDim.x, ...: first the LDS-staged row
-// blocks (XCD-remapped), then groups of four wave rows, then long rows.
+// blocks, then groups of four wave rows, then long rows.  By default the
+// grid has one workgroup per work item; a capped grid (plan.grid) makes it
+// persistent.  With xcd_remap the row blocks are renumbered so that each
+// XCD walks one contiguous range.
 // With DOT the kernel also forms the block's share of p.Ap (x = p, y = Ap)
 // and writes it to partials[blockIdx.x]; the next CG kernel adds the
 // partials in a fixed order (no fences or atomics: deterministic).
@@ -134,7 +136,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ wave_rows, uint32_t nwave,
                uint32_t nwaveblocks,
                const uint32_t *__restrict__ block_rows, uint32_t nwork,
-               uint32_t exact, double *__restrict__ partials,
+               uint32_t exact, uint32_t remap, double *__restrict__ partials,
                const gbd_cg_state *st) {
   __shared__ T prod[GB_STREAM_LDS];
   __shared__ double red[GB_WAVES_PER_BLOCK];
@@ -150,7 +152,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     if (w < nstream) {
       // ---- LDS-staged row block of short rows
-      const uint32_t lb = gb_xcd_remap(w, nstream);
+      const uint32_t lb = remap ? gb_xcd_remap(w, nstream) : w;
       const uint32_t r0 = blk[2 * lb], r1 = blk[2 * lb + 1];
       const uint32_t k0 = rowptr[r0];
       const uint32_t cnt = rowptr[r1] - k0;
@@ -286,7 +288,9 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
 
 uint32_t gb_spmv_grid(const gbd_matrix *m) {
   const uint64_t nwork = gb_spmv_nwork(m);
-  const uint64_t gmax = m->params.grid ? m->params.grid : GB_GRID_DEFAULT;
+  // default: one workgroup per work item (measured faster than a capped,
+  // persistent grid for the LDS-staged blocks)
+  const uint64_t gmax = m->params.grid ? m->params.grid : 0xffffffffull;
   return (uint32_t)(nwork < gmax ? nwork : gmax);
 }
 
@@ -304,12 +308,12 @@ static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
     gb_spmv_kernel<T, true><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
         wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, nwork,
-        m->params.exact, partials, cg);
+        m->params.exact, m->params.xcd_remap, partials, cg);
   } else {
     gb_spmv_kernel<T, false><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, val, x, y, ystride, m->blk, m->nstream,
         wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows, nwork,
-        m->params.exact, nullptr, nullptr);
+        m->params.exact, m->params.xcd_remap, nullptr, nullptr);
   }
   GB_LAUNCH_CHECK();
   return GOOSEBERRY_SUCCESS;
@@ -407,6 +411,7 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->rowcap = 512;
   p->exact = 0;
   p->grid = 0;
+  p->xcd_remap = 0;
 }
 
 int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,

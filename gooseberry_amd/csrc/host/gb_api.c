@@ -116,6 +116,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   if (p->rowcap) d.rowcap = p->rowcap;
   d.exact = p->exact ? 1 : 0;
   if (p->grid) d.grid = p->grid;
+  d.xcd_remap = p->xcd_remap ? 1 : 0;
   *out = d;
   *have = 1;
 }
@@ -249,6 +250,7 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.rowcap = p.rowcap;
   info->plan.exact = p.exact;
   info->plan.grid = p.grid;
+  info->plan.xcd_remap = p.xcd_remap;
   return GOOSEBERRY_SUCCESS;
 }
 

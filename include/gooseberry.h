@@ -232,7 +232,8 @@ typedef struct gooseberry_plan_t {
   dim_t exact;      /* 1: rows of any length are summed left to right from
                        +0.0 (bitwise the reference for every row); 0: rows
                        longer than short_max use tree reductions */
-  dim_t grid;       /* workgroups of the persistent SpMV grid */
+  dim_t grid;       /* cap on SpMV workgroups (0: one per row block) */
+  dim_t xcd_remap;  /* 1: each XCD walks a contiguous range of blocks */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */

@@ -90,6 +90,9 @@ def test_irregular_rows_all_bins(gb, oracle):
     {"short_max": 64, "wave_max": 1 << 30, "window": 2240, "rowcap": 512},
     {"short_max": 200, "wave_max": 1000, "window": 2048, "rowcap": 300},
     {"short_max": 2303, "wave_max": 4096, "window": 1, "rowcap": 512},
+    {"grid": 64},
+    {"grid": 8, "xcd_remap": 1},
+    {"xcd_remap": 1, "window": 700, "rowcap": 100},
 ])
 def test_plans_same_results(gb, oracle, plan):
     n = 5000
