@@ -20,6 +20,7 @@
 #include "gb_matrix.h"
 
 #define GB_ELEM_GRID_MAX 2048u
+#define GB_CG_NGRAPH 7  // captured batches of 1, 2, 4, ..., 64 iterations
 
 struct gbd_cg {
   const gbd_matrix *m;
@@ -36,10 +37,9 @@ struct gbd_cg {
   const double *b;
   double *x;
   uint32_t grid_elem;
-  hipGraph_t graph;
-  hipGraphExec_t gexec;
-  uint32_t graph_k;
-  double *graph_x;
+  hipGraph_t graph[GB_CG_NGRAPH];       // graph[i]: 2^i iterations
+  hipGraphExec_t gexec[GB_CG_NGRAPH];
+  double *graph_x;                      // x the graphs were captured with
 };
 
 // Fixed-order sum of n partials, computed identically by every workgroup
@@ -111,14 +111,35 @@ gb_cg_init_state(const double *__restrict__ partials, uint32_t np,
 }
 
 // src/cgd.c:90: alpha = dr / p.Ap from the SpMV's per-row-block partials,
-// added in a fixed order; one workgroup
-__global__ void __launch_bounds__(GB_BLOCK)
+// added in a fixed order by one 1024-thread workgroup; each thread issues
+// all of its loads before adding (one memory round trip per 8K partials).
+#define GB_RED_THREADS 1024
+#define GB_RED_UNROLL 8
+__global__ void __launch_bounds__(GB_RED_THREADS)
 gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
             gbd_cg_state *st) {
-  __shared__ double red[GB_WAVES_PER_BLOCK];
-  if (gb_block_flag(&st->done)) return;
-  const double pap = gb_sum_partials(pap_partials, npap, red);
+  __shared__ double red[GB_RED_THREADS / GB_WAVE];
+  __shared__ int32_t stop;
+  if (threadIdx.x == 0) stop = st->done;
+  __syncthreads();
+  if (stop) return;
+  double s = 0.0;
+  for (uint32_t base = 0; base < npap; base += GB_RED_THREADS * GB_RED_UNROLL) {
+    double v[GB_RED_UNROLL];
+#pragma unroll
+    for (int j = 0; j < GB_RED_UNROLL; ++j) {
+      uint32_t i = base + j * GB_RED_THREADS + threadIdx.x;
+      v[j] = i < npap ? pap_partials[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < GB_RED_UNROLL; ++j) s += v[j];
+  }
+  s = gb_wave_sum(s);
+  if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = s;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double pap = red[0];
+    for (int w = 1; w < GB_RED_THREADS / GB_WAVE; ++w) pap += red[w];
     const double dr = st->dr;
     const double alpha = dr / pap;
     st->pap = pap;
@@ -202,7 +223,7 @@ static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
   for (uint32_t it = 0; it < k; ++it) {
     int rv = gb_spmv_launch(cg->m, cg->p, cg->ap, 1, cg->partials, cg->st);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
-    gb_cg_alpha<<<1, GB_BLOCK, 0, s>>>(cg->partials, g1, cg->st);
+    gb_cg_alpha<<<1, GB_RED_THREADS, 0, s>>>(cg->partials, g1, cg->st);
     GB_LAUNCH_CHECK();
     gb_cg_update_xr<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
         cg->x, cg->r, cg->p, cg->ap, n, cg->partials2, cg->st);
@@ -216,11 +237,12 @@ static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
 }
 
 static void gb_cg_drop_graph(gbd_cg *cg) {
-  if (cg->gexec) hipGraphExecDestroy(cg->gexec);
-  if (cg->graph) hipGraphDestroy(cg->graph);
-  cg->gexec = nullptr;
-  cg->graph = nullptr;
-  cg->graph_k = 0;
+  for (int i = 0; i < GB_CG_NGRAPH; ++i) {
+    if (cg->gexec[i]) hipGraphExecDestroy(cg->gexec[i]);
+    if (cg->graph[i]) hipGraphDestroy(cg->graph[i]);
+    cg->gexec[i] = nullptr;
+    cg->graph[i] = nullptr;
+  }
   cg->graph_x = nullptr;
 }
 
@@ -303,25 +325,34 @@ int gbd_cg_start(gbd_cg *cg, const double *d_b, double *d_x, double tol,
 
 int gbd_cg_enqueue(gbd_cg *cg, uint32_t k) { return gb_cg_enqueue_raw(cg, k); }
 
+// k iterations as replays of captured graphs of 2^i iterations (one graph
+// per power of two, captured on first use and kept while x stays the same).
 int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k) {
   hipStream_t s = gb_stream();
-  if (!cg->gexec || cg->graph_k != k || cg->graph_x != cg->x) {
+  if (cg->graph_x != cg->x) {
     gb_cg_drop_graph(cg);
-    GB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rv = gb_cg_enqueue_raw(cg, k);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(s, &g);
-    if (rv != GOOSEBERRY_SUCCESS) {
-      if (g) hipGraphDestroy(g);
-      return rv;
-    }
-    GB_HIP(e);
-    cg->graph = g;
-    GB_HIP(hipGraphInstantiate(&cg->gexec, g, nullptr, nullptr, 0));
-    cg->graph_k = k;
     cg->graph_x = cg->x;
   }
-  GB_HIP(hipGraphLaunch(cg->gexec, s));
+  for (int i = GB_CG_NGRAPH - 1; i >= 0 && k; --i) {
+    const uint32_t len = 1u << i;
+    while (k >= len) {
+      if (!cg->gexec[i]) {
+        GB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rv = gb_cg_enqueue_raw(cg, len);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rv != GOOSEBERRY_SUCCESS) {
+          if (g) hipGraphDestroy(g);
+          return rv;
+        }
+        GB_HIP(e);
+        cg->graph[i] = g;
+        GB_HIP(hipGraphInstantiate(&cg->gexec[i], g, nullptr, nullptr, 0));
+      }
+      GB_HIP(hipGraphLaunch(cg->gexec[i], s));
+      k -= len;
+    }
+  }
   return GOOSEBERRY_SUCCESS;
 }
 

@@ -141,9 +141,10 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
   __shared__ T prod[GB_STREAM_LDS];
   __shared__ double red[GB_WAVES_PER_BLOCK];
   __shared__ T tred[GB_WAVES_PER_BLOCK];
-  if (DOT) {
-    if (gb_block_flag(&st->done)) return;
-  }
+  // With DOT the launch writes only Ap and the partials, which nothing reads
+  // once the solve has stopped, so it does not wait on the stop flag (that
+  // read would add a dependent memory round trip to every workgroup).
+  (void)st;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & (GB_WAVE - 1);
   const uint32_t wave = tid / GB_WAVE;
@@ -160,8 +161,17 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
       // with the block's nonzeros so that all loads are in flight at once
       const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
       uint32_t pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
-      if (ra < r1) { pa0 = rowptr[ra]; pa1 = rowptr[ra + 1]; }
-      if (rb < r1) { pb0 = rowptr[rb]; pb1 = rowptr[rb + 1]; }
+      T xa = 0, xb = 0;  // p[row] for the dot product, fetched up front
+      if (ra < r1) {
+        pa0 = rowptr[ra];
+        pa1 = rowptr[ra + 1];
+        if (DOT) xa = x[ra];
+      }
+      if (rb < r1) {
+        pb0 = rowptr[rb];
+        pb1 = rowptr[rb + 1];
+        if (DOT) xb = x[rb];
+      }
       if (cnt > 0) {
         // stage val*x[col] for the block's nonzeros; out-of-range lanes
         // re-read the last entry (in bounds) and store nothing
@@ -192,13 +202,13 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
         T s = 0;
         for (uint32_t k = pa0 - k0; k < pa1 - k0; ++k) s += prod[k];
         y[(size_t)ra * ystride] = s;
-        if (DOT) dotp += (double)x[ra] * (double)s;
+        if (DOT) dotp += (double)xa * (double)s;
       }
       if (rb < r1) {
         T s = 0;
         for (uint32_t k = pb0 - k0; k < pb1 - k0; ++k) s += prod[k];
         y[(size_t)rb * ystride] = s;
-        if (DOT) dotp += (double)x[rb] * (double)s;
+        if (DOT) dotp += (double)xb * (double)s;
       }
     } else if (w < nstream + nwaveblocks) {
       // ---- one wavefront per medium row

@@ -318,9 +318,31 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
 
 /* ------------------------------------------------------------------- CG */
 
-/* iterations per enqueued batch (captured once into a hipGraph) */
-#define GB_CG_BATCH 32
+/* Iterations are enqueued in batches replayed from captured hipGraphs and
+ * the host polls the device state between batches.  The device stops by
+ * itself at the reference's stopping point; iterations launched after it
+ * cost time, so the next batch is sized from the residual decrease seen in
+ * the last one (a little short of the predicted remainder). */
+#define GB_CG_FIRST 8
+#define GB_CG_MAXBATCH 64
 #define GB_CG_TRACE 64
+
+static uint32_t gb_cg_next_batch(uint32_t k, uint64_t it0, double r0,
+    const gbd_cg_state *st)
+{
+  uint64_t done_its = st->iter - it0;
+  if (done_its == 0 || !(r0 > 0) || !(st->rnorm > 0) || !(st->rnorm < r0) ||
+      !(st->tol > 0)) {
+    k *= 2;
+    return k > GB_CG_MAXBATCH ? GB_CG_MAXBATCH : k;
+  }
+  double rate = log(st->rnorm / r0) / (double)done_its;   /* < 0 */
+  double need = log(st->tol / st->rnorm) / rate;          /* > 0 */
+  double want = 0.8 * need;
+  if (!(want >= 1.0)) return 1;
+  if (want > GB_CG_MAXBATCH) return GB_CG_MAXBATCH;
+  return (uint32_t)want;
+}
 
 int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
     real_t *d_x, real_t tol, int tol_mode, size_t niter, void *log,
@@ -345,8 +367,11 @@ int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
   double trace[GB_CG_TRACE];
   uint64_t printed = 0;
   rv = gbd_cg_poll(mat->cg, &st, NULL);
+  uint32_t k = GB_CG_FIRST;
   while (rv == GOOSEBERRY_SUCCESS && !st.done) {
-    rv = gbd_cg_enqueue_graph(mat->cg, GB_CG_BATCH);
+    const uint64_t it0 = st.iter;
+    const double r0 = st.rnorm;
+    rv = gbd_cg_enqueue_graph(mat->cg, k);
     if (rv != GOOSEBERRY_SUCCESS) break;
     rv = gbd_cg_poll(mat->cg, &st, flog ? trace : NULL);
     if (rv != GOOSEBERRY_SUCCESS) break;
@@ -357,6 +382,7 @@ int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
             trace[printed % GB_CG_TRACE]);
       }
     }
+    k = gb_cg_next_batch(k, it0, r0, &st);
   }
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   if (info) {
