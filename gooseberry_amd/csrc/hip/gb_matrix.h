@@ -37,9 +37,13 @@ static inline uint32_t gb_spmv_nwork(const gbd_matrix *m) {
 // workgroups of one SpMV launch (the number of p.Ap partials with DOT)
 uint32_t gb_spmv_grid(const gbd_matrix *m);
 
-// Enqueue y = A x; when cg != nullptr (and cg->done is 0) also write the
-// per-workgroup partial sums of p.Ap (x = p, y = Ap) to
-// partials[gb_spmv_grid(m)].
+// p.Ap partials written by one SpMV launch with DOT (one per wavefront)
+static inline uint32_t gb_spmv_npartials(const gbd_matrix *m) {
+  return gb_spmv_grid(m) * GB_WAVES_PER_BLOCK;
+}
+
+// Enqueue y = A x; when cg != nullptr also write the per-wavefront partial
+// sums of p.Ap (x = p, y = Ap) to partials[gb_spmv_npartials(m)].
 int gb_spmv_launch(const gbd_matrix *m, const double *x, double *y,
                    uint32_t ystride, double *partials, const gbd_cg_state *cg);
 int gb_spmv_launch_f32(const gbd_matrix *m, const float *x, float *y,

@@ -287,12 +287,14 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
       }
     }
     // the LDS buffers are reused by the next work item
-    __syncthreads();
+    if (w + gridDim.x < nwork) __syncthreads();
   }
 
   if (DOT) {
-    double bs = gb_block_sum(dotp, red);
-    if (tid == 0) partials[blockIdx.x] = bs;
+    // one partial per wavefront (no workgroup barrier at the end)
+    (void)red;
+    double ws = gb_wave_sum(dotp);
+    if (lane == 0) partials[blockIdx.x * GB_WAVES_PER_BLOCK + wave] = ws;
   }
 }
 
