@@ -105,6 +105,10 @@ def main():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 4 device copies of exactly 256 MiB, the "
+                         "known-size kernel that scales the rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build is single-GPU (the path "
@@ -137,6 +141,14 @@ def main():
         info = M.cgd_device(db, dsol, TOL, gb.TOL_RELATIVE)
         t1 = time.perf_counter()  # cgd_device returns after its final poll
         return spmv_ms, t1 - t0, info
+
+    if args.calibrate_copy:
+        src = gb.DeviceArray(32 << 20)  # 256 MiB of float64
+        dst = gb.DeviceArray(32 << 20)
+        for _ in range(4):
+            dst.copy_from(src)
+        gb.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         step()

@@ -47,7 +47,7 @@ EXPORTED = [
     "gooseberry_last_device_error", "gooseberry_countingsort_kv",
     "gooseberry_permute_sparse", "gooseberry_permute_dense",
     "gooseberry_device_alloc", "gooseberry_device_free",
-    "gooseberry_memcpy_htod", "gooseberry_memcpy_dtoh",
+    "gooseberry_memcpy_htod", "gooseberry_memcpy_dtoh", "gooseberry_memcpy_dtod",
     "gooseberry_device_synchronize", "gooseberry_device_memory",
     "gooseberry_event_create", "gooseberry_event_destroy",
     "gooseberry_event_record", "gooseberry_event_elapsed_ms",
@@ -126,6 +126,7 @@ def _sig(lib):
         "gooseberry_device_free": (i32, [_P]),
         "gooseberry_memcpy_htod": (i32, [_P, _P, sz]),
         "gooseberry_memcpy_dtoh": (i32, [_P, _P, sz]),
+        "gooseberry_memcpy_dtod": (i32, [_P, _P, sz]),
         "gooseberry_device_synchronize": (i32, []),
         "gooseberry_device_memory": (i32, [C.POINTER(sz), C.POINTER(sz)]),
         "gooseberry_event_create": (i32, [C.POINTER(_P)]),
@@ -207,6 +208,12 @@ class DeviceArray:
         assert a.size == self.n
         check(lib().gooseberry_memcpy_htod(self.ptr, _ptr(a), self.nbytes),
               "gooseberry_memcpy_htod")
+
+    def copy_from(self, other):
+        """Device-to-device copy of other (same size) into this buffer."""
+        assert other.nbytes == self.nbytes
+        check(lib().gooseberry_memcpy_dtod(self.ptr, other.ptr, self.nbytes),
+              "gooseberry_memcpy_dtod")
 
     def numpy(self):
         out = np.empty(self.n, dtype=self.dtype)

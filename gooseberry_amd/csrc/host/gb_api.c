@@ -58,6 +58,11 @@ int gooseberry_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes)
   return gbd_dtoh(h_dst, d_src, bytes);
 }
 
+int gooseberry_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes)
+{
+  return gbd_dtod(d_dst, d_src, bytes);
+}
+
 int gooseberry_device_synchronize(void) { return gbd_sync(); }
 
 int gooseberry_device_memory(size_t *free_bytes, size_t *total_bytes)

@@ -212,6 +212,8 @@ int gooseberry_device_alloc(void ** d_ptr, size_t bytes);
 int gooseberry_device_free(void * d_ptr);
 int gooseberry_memcpy_htod(void * d_dst, const void * h_src, size_t bytes);
 int gooseberry_memcpy_dtoh(void * h_dst, const void * d_src, size_t bytes);
+/* device-to-device copy, enqueued on the library stream */
+int gooseberry_memcpy_dtod(void * d_dst, const void * d_src, size_t bytes);
 int gooseberry_device_synchronize(void);
 int gooseberry_device_memory(size_t * free_bytes, size_t * total_bytes);
 /* timing events on the library stream */
