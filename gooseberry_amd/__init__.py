@@ -56,7 +56,17 @@ EXPORTED = [
     "gooseberry_matrix_info", "gooseberry_matrix_spmv",
     "gooseberry_matrix_spmm", "gooseberry_matrix_cgd", "gooseberry_cgd",
     "gooseberry_generate_laplacian", "gooseberry_generate_uniform",
+    "gooseberry_batch_create", "gooseberry_batch_free", "gooseberry_batch_add",
+    "gooseberry_batch_remove", "gooseberry_batch_run", "gooseberry_batch_solve",
+    "gooseberry_batch_info", "gooseberry_batch_workspace_bytes",
+    "gooseberry_server_create", "gooseberry_server_free", "gooseberry_server_submit_sdd",
+    "gooseberry_server_submit", "gooseberry_server_progress", "gooseberry_server_status",
+    "gooseberry_server_take_result", "gooseberry_matrix_generate_sdd",
+    "gooseberry_matrix_download", "gooseberry_generate_uniform_device",
 ]
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
 
 
 class GooseberryError(RuntimeError):
@@ -92,6 +102,16 @@ class CgdInfo(C.Structure):
                 ("bnorm", C.c_double), ("converged", C.c_int),
                 ("status", C.c_int), ("nrefine", C.c_uint64),
                 ("true_rel_residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("niter", C.c_uint64),
+                ("nrefine", C.c_uint64), ("rel_residual", C.c_double),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("n", C.c_uint32), ("nnz", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -146,6 +166,27 @@ def _sig(lib):
         "gooseberry_generate_laplacian": (i32, [i32, u32, u32, u32, _U32P, C.POINTER(_P),
                                                 C.POINTER(_P), C.POINTER(_P)]),
         "gooseberry_generate_uniform": (i32, [u64, u64, sz, f64, f64, _P]),
+        "gooseberry_batch_create": (i32, [u32, f64, C.POINTER(_P)]),
+        "gooseberry_batch_free": (i32, [_P]),
+        "gooseberry_batch_add": (i32, [_P, _P, _P, _P, f64, i32, sz, i32, C.POINTER(u32)]),
+        "gooseberry_batch_remove": (i32, [_P, u32]),
+        "gooseberry_batch_run": (i32, [_P, u32, C.POINTER(u32)]),
+        "gooseberry_batch_solve": (i32, [_P]),
+        "gooseberry_batch_info": (i32, [_P, u32, C.POINTER(CgdInfo)]),
+        "gooseberry_batch_workspace_bytes": (sz, [_P, i32]),
+        "gooseberry_server_create": (i32, [sz, u32, i32, f64, i32, C.POINTER(_P)]),
+        "gooseberry_server_free": (i32, [_P]),
+        "gooseberry_server_submit_sdd": (i32, [_P, u64, u32, u64, u32, u32, f64]),
+        "gooseberry_server_submit": (i32, [_P, u64, _P, _P, _P, f64]),
+        "gooseberry_server_progress": (i32, [_P, u32, C.POINTER(RequestResult), u32,
+                                             C.POINTER(u32)]),
+        "gooseberry_server_status": (i32, [_P, C.POINTER(u32), C.POINTER(u32),
+                                           C.POINTER(sz)]),
+        "gooseberry_server_take_result": (i32, [_P, u64, _P, sz]),
+        "gooseberry_matrix_generate_sdd": (i32, [u32, u64, u32, u32, C.POINTER(_Plan),
+                                                 C.POINTER(_P)]),
+        "gooseberry_matrix_download": (i32, [_P, _P, _P, _P]),
+        "gooseberry_generate_uniform_device": (i32, [u64, u64, sz, f64, f64, _P]),
     }
     for name, (res, args) in s.items():
         f = getattr(lib, name)
@@ -295,6 +336,23 @@ class Matrix:
         self.h = h.value
         self.nrows, self.ncols = nrows, ncols
         self.nnz = int(rowptr[-1])
+
+    @classmethod
+    def _from_handle(cls, h):
+        self = cls.__new__(cls)
+        self.h = h
+        i = self.info()
+        self.nrows, self.ncols, self.nnz = i["nrows"], i["ncols"], i["nnz"]
+        return self
+
+    def download(self):
+        """(rowptr, rowind, rowval) copied back from HBM."""
+        rp = np.empty(self.nrows + 1, dtype=np.uint32)
+        ri = np.empty(max(self.nnz, 1), dtype=np.uint32)
+        rv = np.empty(max(self.nnz, 1), dtype=np.float64)
+        check(lib().gooseberry_matrix_download(self.h, _ptr(rp), _ptr(ri), _ptr(rv)),
+              "gooseberry_matrix_download")
+        return rp, ri[:self.nnz], rv[:self.nnz]
 
     def info(self):
         i = _MatrixInfo()
@@ -500,3 +558,123 @@ def uniform(seed, n, lo=-1.0, hi=1.0, stream=0):
     check(lib().gooseberry_generate_uniform(seed, stream, n, lo, hi, _ptr(out)),
           "gooseberry_generate_uniform")
     return out
+
+
+def generate_sdd(n, seed, dmin=3, dmax=27, plan=None):
+    """Random symmetric diagonally dominant SPD matrix generated in HBM."""
+    h = _P()
+    check(lib().gooseberry_matrix_generate_sdd(n, seed, dmin, dmax, _plan(plan), C.byref(h)),
+          "gooseberry_matrix_generate_sdd")
+    return Matrix._from_handle(h.value)
+
+
+def uniform_device(seed, n, lo=-1.0, hi=1.0, stream=0):
+    d = DeviceArray(n)
+    check(lib().gooseberry_generate_uniform_device(seed, stream, n, lo, hi, d.ptr),
+          "gooseberry_generate_uniform_device")
+    return d
+
+
+# --------------------------------------------------------- batched solves
+
+class Batch:
+    """Independent CG solves advanced together (gooseberry_batch_*)."""
+
+    def __init__(self, max_systems, delta=0.0):
+        h = _P()
+        check(lib().gooseberry_batch_create(max_systems, float(delta), C.byref(h)),
+              "gooseberry_batch_create")
+        self.h = h.value
+        self._keep = {}
+
+    def add(self, mat, d_b, d_x, tol, tol_mode=TOL_RELATIVE, niter=0,
+            precision=PRECISION_FP64):
+        slot = C.c_uint32()
+        check(lib().gooseberry_batch_add(self.h, mat.h, d_b.ptr, d_x.ptr, float(tol), tol_mode,
+                                         int(niter), precision, C.byref(slot)),
+              "gooseberry_batch_add")
+        self._keep[slot.value] = (mat, d_b, d_x)
+        return slot.value
+
+    def remove(self, slot):
+        check(lib().gooseberry_batch_remove(self.h, slot), "gooseberry_batch_remove")
+        self._keep.pop(slot, None)
+
+    def run(self, steps):
+        running = C.c_uint32()
+        check(lib().gooseberry_batch_run(self.h, steps, C.byref(running)), "gooseberry_batch_run")
+        return running.value
+
+    def solve(self):
+        check(lib().gooseberry_batch_solve(self.h), "gooseberry_batch_solve")
+
+    def info(self, slot):
+        i = CgdInfo()
+        check(lib().gooseberry_batch_info(self.h, slot, C.byref(i)), "gooseberry_batch_info")
+        return i
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().gooseberry_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching solve server (gooseberry_server_*)."""
+
+    def __init__(self, mem_budget=0, max_active=256, precision=PRECISION_MIXED, delta=0.0,
+                 keep_results=False):
+        h = _P()
+        check(lib().gooseberry_server_create(int(mem_budget), max_active, precision, float(delta),
+                                             1 if keep_results else 0, C.byref(h)),
+              "gooseberry_server_create")
+        self.h = h.value
+        self._keep = {}
+
+    def submit_sdd(self, rid, n, seed, dmin=3, dmax=27, tol=1e-10):
+        check(lib().gooseberry_server_submit_sdd(self.h, rid, n, seed, dmin, dmax, float(tol)),
+              "gooseberry_server_submit_sdd")
+
+    def submit(self, rid, mat, d_b, d_x, tol=1e-10):
+        check(lib().gooseberry_server_submit(self.h, rid, mat.h, d_b.ptr, d_x.ptr, float(tol)),
+              "gooseberry_server_submit")
+        self._keep[rid] = (mat, d_b, d_x)
+
+    def progress(self, steps, max_done=4096):
+        buf = (RequestResult * max_done)()
+        nd = C.c_uint32()
+        check(lib().gooseberry_server_progress(self.h, steps, buf, max_done, C.byref(nd)),
+              "gooseberry_server_progress")
+        out = [buf[i].as_dict() for i in range(min(nd.value, max_done))]
+        for r in out:
+            self._keep.pop(r["id"], None)
+        return out
+
+    def status(self):
+        a, q, m = C.c_uint32(), C.c_uint32(), C.c_size_t()
+        check(lib().gooseberry_server_status(self.h, C.byref(a), C.byref(q), C.byref(m)),
+              "gooseberry_server_status")
+        return a.value, q.value, m.value
+
+    def take_result(self, rid, n):
+        x = np.empty(n, dtype=np.float64)
+        check(lib().gooseberry_server_take_result(self.h, rid, _ptr(x), n),
+              "gooseberry_server_take_result")
+        return x
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().gooseberry_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -154,6 +154,64 @@ int gbd_cg_poll(gbd_cg *cg, gbd_cg_state *st, double *h_trace);
  * gbd_cg_enqueue_graph call. */
 int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k);
 
+/* ----------------------------------------------------- batched solves */
+/* Per-system state of a batched solve (device layout, polled to host). */
+typedef struct gbd_bstate {
+  double dr;        /* r.r of the (inner) recurrence residual */
+  double pap;
+  double alpha;
+  double beta;
+  double dr_prev;
+  double rnorm;     /* sqrt(dr) */
+  double rnorm0;    /* ||b|| */
+  double rnorm64;   /* last FP64 residual norm (MIXED refresh steps) */
+  double rnorm_ref; /* FP64 residual norm at the last refresh */
+  double tol;       /* absolute threshold */
+  uint64_t iter;    /* CG iterations (inner iterations for MIXED) */
+  uint64_t iter_next;
+  uint64_t maxiter;
+  uint64_t nrefine; /* FP64 residual refreshes */
+  uint64_t inner;   /* inner iterations since the last refresh */
+  int32_t done;
+  int32_t status;   /* 1 converged, 2 iteration cap, 3 breakdown */
+  int32_t mode;     /* step kind of the current step (0 CG, 1 refresh) */
+  int32_t mode_next;
+  int32_t precision;
+  int32_t final_refresh;
+} gbd_bstate;
+
+typedef struct gbd_batch gbd_batch;
+
+/* delta: FP32 residual reduction that triggers an FP64 refresh (MIXED). */
+int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out);
+void gbd_batch_destroy(gbd_batch *b);
+/* Device bytes the batch allocates for one system solving with m. */
+size_t gbd_batch_workspace_bytes(const gbd_matrix *m, int precision);
+/* Allocate a slot's workspace and start its solve (x = 0); the slot joins
+ * the work lists at the next gbd_batch_commit. */
+int gbd_batch_add(gbd_batch *b, const gbd_matrix *m, const double *d_b,
+    double *d_x, double tol, int rel, uint64_t maxiter, int precision,
+    uint32_t *slot);
+int gbd_batch_remove(gbd_batch *b, uint32_t slot);
+/* Rebuild the work lists over the used slots with active[slot] != 0
+ * (active == NULL: all used slots). */
+int gbd_batch_commit(gbd_batch *b, const uint8_t *active);
+/* Enqueue k steps of every listed system (graph replays). */
+int gbd_batch_step(gbd_batch *b, uint32_t k);
+/* Synchronise and copy all max_slots states to the host. */
+int gbd_batch_poll(gbd_batch *b, gbd_bstate *states);
+int gbd_batch_slot_used(const gbd_batch *b, uint32_t slot);
+
+/* ------------------------------------------------ generated matrices */
+/* Random symmetric, strictly diagonally dominant matrix built on the GPU
+ * (see gb_gen.hip); the caller owns the returned device arrays. */
+int gbd_generate_sdd(uint32_t n, uint64_t seed, uint32_t dmin, uint32_t dmax,
+    uint32_t **d_rowptr, uint32_t **d_rowind, double **d_rowval);
+/* d_out[i] = lo + (hi - lo) u, u from a counter-based hash of (seed,
+ * stream, i) */
+int gbd_fill_uniform(double *d_out, size_t n, uint64_t seed, uint64_t stream,
+    double lo, double hi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,855 @@
+// gb_batch.hip -- many independent CG solves advanced together.
+//
+// A batch holds up to max_slots systems, each a resident matrix (its own CSR
+// and SpMV plan) with its own vectors, partial-sum arrays and state.  One
+// device step advances every active system by one step with four launches
+// over work lists:
+//   K1 gb_b_spmv     (system, SpMV work item): Ap and p.Ap partials, or in a
+//                    refresh step the FP64 residual b - A x and its partials
+//   K2 gb_b_reduce   (system): alpha, or the refreshed residual norm and the
+//                    convergence test
+//   K3 gb_b_update   (system, chunk of GB_BCHUNK rows): x/r update and r.r
+//                    partials, or the FP32 copy of a refreshed residual
+//   K4 gb_b_pupdate  (system, chunk): beta (every chunk adds the system's
+//                    partials in the same order), stopping / refresh
+//                    decision, p update
+// Every sum of a system is taken over that system's own fixed decomposition
+// (its SpMV plan, its chunks) in a fixed order, so a system gives the same
+// bits alone and in any mix.
+//
+// Precision per system:
+//   FP64  -- the reference's CG (src/cgd.c:40-129) in FP64.
+//   MIXED -- FP32 matrix values and inner vectors (d, r, p, Ap), dot
+//            products accumulated in FP64, and reliable updates: when the
+//            FP32 recurrence residual has dropped by `delta` since the last
+//            refresh (or reaches the tolerance), the step folds d into the
+//            FP64 solution x and the next step recomputes r = b - A x with
+//            the FP64 matrix; the Krylov direction p is kept.  Convergence
+//            is only declared on the FP64 residual.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gb_matrix.h"
+
+#define GB_BCHUNK 2048           // rows per elementwise work item
+#define GB_BCHUNK_ITEMS (GB_BCHUNK / GB_BLOCK)
+#define GB_BGRID_SPMV 4096u      // fixed grids: the graphs stay valid for any
+#define GB_BGRID_ELEM 4096u      // membership of the batch
+#define GB_BGRID_SYS 256u
+#define GB_BRED 1024             // threads of the per-system reduction
+#define GB_B_NGRAPH 7
+
+enum { GB_MODE_CG = 0, GB_MODE_REFRESH = 1 };
+
+struct gb_bsys {
+  const uint32_t *rowptr, *rowind, *blk, *wave_rows, *block_rows;
+  const double *val;
+  const float *val32;
+  uint32_t n, nstream, nwave, nwaveblocks, nblockrows, nwork, nchunks, exact;
+  const double *b;
+  double *x;        // solution (FP64 in both precisions)
+  double *r, *p, *ap;  // FP64: residual, direction, A p.  MIXED: r = FP64
+                       // residual of refresh steps; p, ap unused
+  float *d32, *r32, *p32, *ap32;  // MIXED inner vectors
+  double *part1;    // 4 * nwork partials of K1
+  double *part2;    // nchunks partials of K3
+  gbd_bstate *st;
+  int32_t precision;
+  int32_t pad;
+};
+
+struct gb_blists {
+  uint32_t n_spmv, n_elem, n_sys, pad;
+};
+
+struct gbd_batch {
+  uint32_t max_slots;
+  gb_bsys *d_sys;          // device table
+  gb_bsys *h_sys;          // host mirror
+  gbd_bstate *d_st;        // device states, one per slot
+  gbd_bstate *h_st;        // pinned copy
+  uint8_t *used;           // host: slot in use
+  void **slot_mem;         // host: per-slot workspace allocation
+  gb_blists *d_lists;      // device counts
+  gb_blists *h_lists;      // pinned
+  uint32_t *d_spmv, *d_elem, *d_sysl;  // device work lists
+  uint32_t cap_spmv, cap_elem;
+  uint32_t *h_spmv, *h_elem, *h_sysl;  // pinned staging
+  double delta;
+  hipGraph_t graph[GB_B_NGRAPH];
+  hipGraphExec_t gexec[GB_B_NGRAPH];
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double gb_b_sum_chunks(const double *part, uint32_t n,
+                                                  double *red) {
+  // same code and order in every workgroup of K4 and in the init reduction
+  double s = 0.0;
+  for (uint32_t base = 0; base < n; base += GB_BLOCK * 8) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      uint32_t i = base + j * GB_BLOCK + threadIdx.x;
+      v[j] = i < n ? part[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += v[j];
+  }
+  s = gb_wave_sum(s);
+  const int lane = threadIdx.x & (GB_WAVE - 1), wave = threadIdx.x / GB_WAVE;
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  double t = red[0];
+#pragma unroll
+  for (int w = 1; w < GB_WAVES_PER_BLOCK; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// One SpMV work item of system s (same bins and orders as gb_spmv_kernel).
+// EPI 0: y = A x, dot += x[row] * y   (CG step; T = value/vector type)
+// EPI 1: out = b - A x in FP64, dot += out^2   (refresh step; T = double)
+template <typename T, int EPI>
+__device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
+                                                 const T *__restrict__ val,
+                                                 const T *__restrict__ x,
+                                                 T *__restrict__ y,
+                                                 const double *__restrict__ b,
+                                                 T *prod, T *tred) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & (GB_WAVE - 1), wave = tid / GB_WAVE;
+  const uint32_t *__restrict__ rowptr = S.rowptr;
+  const uint32_t *__restrict__ rowind = S.rowind;
+  double dotp = 0.0;
+  if (w < S.nstream) {
+    const uint32_t r0 = S.blk[2 * w], r1 = S.blk[2 * w + 1];
+    const uint32_t k0 = rowptr[r0];
+    const uint32_t cnt = rowptr[r1] - k0;
+    const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
+    uint32_t pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
+    T xa = 0, xb = 0;
+    double ba = 0, bb = 0;
+    if (ra < r1) {
+      pa0 = rowptr[ra]; pa1 = rowptr[ra + 1];
+      if (EPI == 0) xa = x[ra]; else ba = b[ra];
+    }
+    if (rb < r1) {
+      pb0 = rowptr[rb]; pb1 = rowptr[rb + 1];
+      if (EPI == 0) xb = x[rb]; else bb = b[rb];
+    }
+    if (cnt > 0) {
+      const uint32_t klast = k0 + cnt - 1;
+      uint32_t c[GB_STREAM_ITERS];
+      T v[GB_STREAM_ITERS], xv[GB_STREAM_ITERS];
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        if (i * GB_BLOCK < cnt) {
+          uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
+          c[i] = __builtin_nontemporal_load(&rowind[k]);
+          v[i] = __builtin_nontemporal_load(&val[k]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i)
+        if (i * GB_BLOCK < cnt) xv[i] = x[c[i]];
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        uint32_t e = i * GB_BLOCK + tid;
+        if (i * GB_BLOCK < cnt && e < cnt) prod[e] = v[i] * xv[i];
+      }
+    }
+    __syncthreads();
+    if (ra < r1) {
+      T s = 0;
+      for (uint32_t k = pa0 - k0; k < pa1 - k0; ++k) s += prod[k];
+      if (EPI == 0) { y[ra] = s; dotp += (double)xa * (double)s; }
+      else { double o = ba - (double)s; ((double *)y)[ra] = o; dotp += o * o; }
+    }
+    if (rb < r1) {
+      T s = 0;
+      for (uint32_t k = pb0 - k0; k < pb1 - k0; ++k) s += prod[k];
+      if (EPI == 0) { y[rb] = s; dotp += (double)xb * (double)s; }
+      else { double o = bb - (double)s; ((double *)y)[rb] = o; dotp += o * o; }
+    }
+    __syncthreads();
+  } else if (w < S.nstream + S.nwaveblocks) {
+    const uint32_t wid = (w - S.nstream) * GB_WAVES_PER_BLOCK + wave;
+    if (wid < S.nwave) {
+      const uint32_t row = S.wave_rows[wid];
+      const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+      T s = 0;
+      if (S.exact) {
+        for (uint32_t base = k0; base < k1; base += GB_WAVE) {
+          const uint32_t k = base + lane;
+          T pr = 0;
+          if (k < k1) pr = val[k] * x[rowind[k]];
+          const uint32_t cn = min((uint32_t)GB_WAVE, k1 - base);
+          for (uint32_t j = 0; j < cn; ++j) s += gb_readlane(pr, j);
+        }
+      } else {
+        for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) s += val[k] * x[rowind[k]];
+        s = gb_wave_sum(s);
+      }
+      if (lane == 0) {
+        if (EPI == 0) { y[row] = s; dotp += (double)x[row] * (double)s; }
+        else { double o = b[row] - (double)s; ((double *)y)[row] = o; dotp += o * o; }
+      }
+    }
+  } else {
+    const uint32_t row = S.block_rows[w - S.nstream - S.nwaveblocks];
+    const uint32_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    T s = 0;
+    if (S.exact) {
+      for (uint32_t base = k0; base < k1; base += GB_STREAM_LDS) {
+        const uint32_t cn = min((uint32_t)GB_STREAM_LDS, k1 - base);
+        for (uint32_t e = tid; e < cn; e += GB_BLOCK)
+          prod[e] = val[base + e] * x[rowind[base + e]];
+        __syncthreads();
+        if (tid == 0)
+          for (uint32_t e = 0; e < cn; ++e) s += prod[e];
+        __syncthreads();
+      }
+    } else {
+      for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) s += val[k] * x[rowind[k]];
+      s = gb_wave_sum(s);
+      if (lane == 0) tred[wave] = s;
+      __syncthreads();
+      if (tid == 0) {
+        T t = tred[0];
+#pragma unroll
+        for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += tred[q];
+        s = t;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      if (EPI == 0) { y[row] = s; dotp += (double)x[row] * (double)s; }
+      else { double o = b[row] - (double)s; ((double *)y)[row] = o; dotp += o * o; }
+    }
+  }
+  return dotp;
+}
+
+// ----------------------------------------------------------------- kernels
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
+          const gb_blists *__restrict__ lists) {
+  __shared__ double prodd[GB_STREAM_LDS];
+  __shared__ double tredd[GB_WAVES_PER_BLOCK];
+  const uint32_t count = lists->n_spmv;
+  const uint32_t lane = threadIdx.x & (GB_WAVE - 1), wave = threadIdx.x / GB_WAVE;
+  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
+    const uint32_t s = items[2 * it], w = items[2 * it + 1];
+    const gb_bsys &S = sys[s];
+    const gbd_bstate *st = S.st;
+    if (st->done) continue;  // not written during this kernel: uniform
+    const int mode = st->mode_next;
+    double dotp;
+    if (mode == GB_MODE_REFRESH) {
+      dotp = gb_b_spmv_item<double, 1>(S, w, S.val, S.x, S.r, S.b, prodd, tredd);
+    } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      dotp = gb_b_spmv_item<float, 0>(S, w, S.val32, S.p32, S.ap32, nullptr,
+                                      (float *)prodd, (float *)tredd);
+    } else {
+      dotp = gb_b_spmv_item<double, 0>(S, w, S.val, S.p, S.ap, nullptr, prodd, tredd);
+    }
+    double ws = gb_wave_sum(dotp);
+    if (lane == 0) S.part1[w * GB_WAVES_PER_BLOCK + wave] = ws;
+  }
+}
+
+__global__ void __launch_bounds__(GB_BRED)
+gb_b_reduce(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ syslist,
+            const gb_blists *__restrict__ lists) {
+  __shared__ double red[GB_BRED / GB_WAVE];
+  const uint32_t count = lists->n_sys;
+  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
+    const gb_bsys &S = sys[syslist[it]];
+    gbd_bstate *st = S.st;
+    if (st->done) continue;
+    const int mode = st->mode_next;
+    const uint32_t np = S.nwork * GB_WAVES_PER_BLOCK;
+    double s = 0.0;
+    for (uint32_t base = 0; base < np; base += GB_BRED * 8) {
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        uint32_t i = base + j * GB_BRED + threadIdx.x;
+        v[j] = i < np ? S.part1[i] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += v[j];
+    }
+    s = gb_wave_sum(s);
+    if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = red[0];
+      for (int q = 1; q < GB_BRED / GB_WAVE; ++q) t += red[q];
+      st->mode = mode;
+      if (mode == GB_MODE_REFRESH) {
+        // t = ||b - A x||^2 in FP64
+        const double rn = sqrt(t);
+        st->nrefine += 1;
+        st->rnorm64 = rn;
+        st->rnorm_ref = rn;
+        if (rn <= st->tol) {
+          st->done = 1;
+          st->status = 1;
+        } else if (!isfinite(rn)) {
+          st->done = 1;
+          st->status = 3;
+        } else if (st->final_refresh) {
+          st->done = 1;
+          st->status = 2;
+        }
+      } else {
+        const double dr = st->dr;
+        const double alpha = dr / t;
+        st->pap = t;
+        st->alpha = alpha;
+        if (!(t != 0.0) || !isfinite(alpha)) {
+          st->done = 1;
+          st->status = 3;
+        } else {
+          st->dr_prev = dr;
+          st->iter_next = st->iter + 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_b_update(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
+            const gb_blists *__restrict__ lists) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  const uint32_t count = lists->n_elem;
+  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
+    const uint32_t sidx = items[2 * it], c = items[2 * it + 1];
+    const gb_bsys &S = sys[sidx];
+    const gbd_bstate *st = S.st;
+    if (st->done) continue;
+    const int mode = st->mode;
+    const uint32_t i0 = c * GB_BCHUNK;
+    double d = 0.0;
+    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      if (mode == GB_MODE_REFRESH) {
+        // r32 = (float) r64 from the refresh SpMV; d = 0 was set when x was
+        // updated
+        for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
+          uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
+          if (i < S.n) {
+            float rf = (float)S.r[i];
+            S.r32[i] = rf;
+            d += (double)rf * (double)rf;
+          }
+        }
+      } else {
+        const float a = (float)st->alpha;
+        for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
+          uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
+          if (i < S.n) {
+            S.d32[i] = S.d32[i] + a * S.p32[i];
+            float rn = S.r32[i] - a * S.ap32[i];
+            S.r32[i] = rn;
+            d += (double)rn * (double)rn;
+          }
+        }
+      }
+    } else {
+      // src/cgd.c:92-100
+      const double alpha = st->alpha;
+      for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
+        uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
+        if (i < S.n) {
+          double ps = alpha * S.p[i];
+          S.x[i] = S.x[i] + ps;
+          double aps = alpha * S.ap[i];
+          double rn = S.r[i] - aps;
+          S.r[i] = rn;
+          d += rn * rn;
+        }
+      }
+    }
+    double bs = gb_block_sum(d, red);
+    if (threadIdx.x == 0) S.part2[c] = bs;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
+             const gb_blists *__restrict__ lists, double delta) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  __shared__ int32_t s_done;
+  const uint32_t count = lists->n_elem;
+  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
+    const uint32_t sidx = items[2 * it], c = items[2 * it + 1];
+    const gb_bsys &S = sys[sidx];
+    gbd_bstate *st = S.st;
+    // workgroup 0 of a system may set done during this kernel: decide once
+    if (threadIdx.x == 0) s_done = *(volatile int32_t *)&st->done;
+    __syncthreads();
+    const int32_t was_done = s_done;
+    __syncthreads();
+    if (was_done) continue;
+    const int mode = st->mode;
+    const double dnr = gb_b_sum_chunks(S.part2, S.nchunks, red);
+    const uint32_t i0 = c * GB_BCHUNK;
+    if (mode == GB_MODE_REFRESH) {
+      // MIXED only: restart the FP32 recurrence from the FP64 residual,
+      // keep the direction p
+      if (c == 0 && threadIdx.x == 0) {
+        st->dr = dnr;
+        st->rnorm = sqrt(dnr);
+        st->inner = 0;
+        st->mode_next = GB_MODE_CG;
+      }
+      continue;
+    }
+    // src/cgd.c:100-119 (and its FP32-inner counterpart)
+    const double alpha = st->alpha;
+    const double beta = dnr / st->dr_prev;
+    const double rnorm = sqrt(dnr);
+    const unsigned long long it1 = st->iter_next;
+    const bool cap = st->maxiter && it1 >= st->maxiter;
+    int status = 0;        // FP64: stop here
+    bool refresh = false;  // MIXED: fold d into x, FP64 residual next step
+    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      if (!isfinite(dnr)) status = 3;
+      else if (rnorm <= st->tol || rnorm <= delta * st->rnorm_ref || cap)
+        refresh = true;
+    } else {
+      if (rnorm <= st->tol) status = 1;
+      else if (!isfinite(dnr)) status = 3;
+      else if (cap) status = 2;
+    }
+    if (c == 0 && threadIdx.x == 0) {
+      st->beta = beta;
+      st->dr = dnr;
+      st->rnorm = rnorm;
+      st->iter = it1;
+      st->inner += 1;
+      if (status) {
+        st->status = status;
+        st->done = 1;
+      }
+      if (refresh) {
+        st->mode_next = GB_MODE_REFRESH;
+        if (cap) st->final_refresh = 1;
+      }
+    }
+    if (status) continue;
+    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      const float boa = (float)beta;
+      for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
+        uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
+        if (i < S.n) {
+          S.p32[i] = S.r32[i] + boa * S.p32[i];
+          if (refresh) {
+            // the FP64 solution update
+            S.x[i] = S.x[i] + (double)S.d32[i];
+            S.d32[i] = 0.0f;
+          }
+        }
+      }
+    } else {
+      const double boa = beta / alpha;
+      for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
+        uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
+        if (i < S.n) {
+          double ps = alpha * S.p[i];
+          double t = ps * boa;
+          S.p[i] = S.r[i] + t;
+        }
+      }
+    }
+  }
+}
+
+// Start of one system (not part of the step graph): x = 0, r = b, p = r
+// (FP64) or r32 = p32 = (float) b, d = 0 (MIXED); chunk partials of b.b
+// (part2) and of r32.r32 (part1, MIXED).
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_b_init(const gb_bsys *__restrict__ sys, uint32_t slot) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  const gb_bsys &S = sys[slot];
+  for (uint32_t c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
+    const uint32_t i0 = c * GB_BCHUNK;
+    double d = 0.0, f = 0.0;
+    for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
+      uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
+      if (i < S.n) {
+        double v = S.b[i];
+        S.x[i] = 0.0;
+        d += v * v;
+        if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+          float vf = (float)v;
+          S.r32[i] = vf;
+          S.p32[i] = vf;
+          S.d32[i] = 0.0f;
+          f += (double)vf * (double)vf;
+        } else {
+          S.r[i] = v;
+          S.p[i] = v;
+        }
+      }
+    }
+    double bs = gb_block_sum(d, red);
+    __syncthreads();
+    double fs = gb_block_sum(f, red);
+    if (threadIdx.x == 0) {
+      S.part2[c] = bs;
+      S.part1[c] = fs;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_b_init_state(const gb_bsys *__restrict__ sys, uint32_t slot, double tol,
+                int rel, unsigned long long maxiter) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  const gb_bsys &S = sys[slot];
+  const double bb = gb_b_sum_chunks(S.part2, S.nchunks, red);
+  const double ff = gb_b_sum_chunks(S.part1, S.nchunks, red);
+  if (threadIdx.x == 0) {
+    gbd_bstate *st = S.st;
+    uint32_t *words = (uint32_t *)st;
+    for (uint32_t i = 0; i < sizeof(gbd_bstate) / 4; ++i) words[i] = 0u;
+    st->rnorm0 = sqrt(bb);
+    st->tol = rel ? tol * st->rnorm0 : tol;
+    st->maxiter = maxiter;
+    st->precision = S.precision;
+    st->mode = GB_MODE_CG;
+    st->mode_next = GB_MODE_CG;
+    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      st->dr = ff;
+      st->rnorm = sqrt(ff);
+      st->rnorm64 = st->rnorm0;
+      st->rnorm_ref = st->rnorm0;
+    } else {
+      st->dr = bb;
+      st->rnorm = st->rnorm0;
+    }
+    if (bb == 0.0) {
+      st->done = 1;
+      st->status = 1;
+    } else if (!isfinite(bb)) {
+      st->done = 1;
+      st->status = 3;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host
+
+static void gb_b_drop_graphs(gbd_batch *B) {
+  for (int i = 0; i < GB_B_NGRAPH; ++i) {
+    if (B->gexec[i]) hipGraphExecDestroy(B->gexec[i]);
+    if (B->graph[i]) hipGraphDestroy(B->graph[i]);
+    B->gexec[i] = nullptr;
+    B->graph[i] = nullptr;
+  }
+}
+
+static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
+  hipStream_t s = gb_stream();
+  for (uint32_t i = 0; i < k; ++i) {
+    gb_b_spmv<<<GB_BGRID_SPMV, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists);
+    GB_LAUNCH_CHECK();
+    gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
+    GB_LAUNCH_CHECK();
+    gb_b_update<<<GB_BGRID_ELEM, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists);
+    GB_LAUNCH_CHECK();
+    gb_b_pupdate<<<GB_BGRID_ELEM, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists,
+                                                     B->delta);
+    GB_LAUNCH_CHECK();
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+static uint32_t gb_b_nchunks(uint32_t n) { return (n + GB_BCHUNK - 1) / GB_BCHUNK; }
+
+// Workspace of one system: byte offsets of its buffers inside one
+// allocation (each 256-byte aligned); the same function sizes and carves it.
+struct gb_b_layout {
+  size_t part1, part2, r, p, ap, d32, r32, p32, ap32, total;
+};
+
+static gb_b_layout gb_b_plan_layout(const gbd_matrix *m, int precision) {
+  const size_t n = m->nrows ? m->nrows : 1;
+  const size_t nwork = gb_spmv_nwork(m);
+  const size_t nch = gb_b_nchunks(m->nrows);
+  const size_t np1 = (nwork * GB_WAVES_PER_BLOCK > nch ? nwork * GB_WAVES_PER_BLOCK : nch) + 1;
+  const size_t np2 = nch + 1;
+  gb_b_layout L;
+  memset(&L, 0xff, sizeof(L));
+  size_t off = 0;
+  auto take = [&](size_t nb) {
+    size_t o = off;
+    off += (nb + 255) & ~(size_t)255;
+    return o;
+  };
+  L.part1 = take(np1 * 8);
+  L.part2 = take(np2 * 8);
+  L.r = take(n * 8);
+  if (precision == GOOSEBERRY_PRECISION_MIXED) {
+    L.d32 = take(n * 4);
+    L.r32 = take(n * 4);
+    L.p32 = take(n * 4);
+    L.ap32 = take(n * 4);
+  } else {
+    L.p = take(n * 8);
+    L.ap = take(n * 8);
+  }
+  L.total = off;
+  return L;
+}
+
+extern "C" {
+
+size_t gbd_batch_workspace_bytes(const gbd_matrix *m, int precision) {
+  return gb_b_plan_layout(m, precision).total;
+}
+
+int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (max_slots == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_batch *B = (gbd_batch *)calloc(1, sizeof(gbd_batch));
+  if (!B) return GOOSEBERRY_ERROR_NOMEMORY;
+  B->max_slots = max_slots;
+  B->delta = delta > 0 && delta < 1 ? delta : 1e-4;
+  B->used = (uint8_t *)calloc(max_slots, 1);
+  B->slot_mem = (void **)calloc(max_slots, sizeof(void *));
+  B->h_sys = (gb_bsys *)calloc(max_slots, sizeof(gb_bsys));
+  hipError_t e = hipMalloc((void **)&B->d_sys, max_slots * sizeof(gb_bsys));
+  if (e == hipSuccess) e = hipMalloc((void **)&B->d_st, max_slots * sizeof(gbd_bstate));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&B->h_st, max_slots * sizeof(gbd_bstate), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void **)&B->d_lists, sizeof(gb_blists));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&B->h_lists, sizeof(gb_blists), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void **)&B->d_sysl, max_slots * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void **)&B->h_sysl, max_slots * sizeof(uint32_t), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemsetAsync(B->d_lists, 0, sizeof(gb_blists), gb_stream());
+  if (e == hipSuccess)
+    e = hipMemsetAsync(B->d_st, 0, max_slots * sizeof(gbd_bstate), gb_stream());
+  if (e == hipSuccess)
+    e = hipMemsetAsync(B->d_sys, 0, max_slots * sizeof(gb_bsys), gb_stream());
+  if (e != hipSuccess || !B->used || !B->slot_mem || !B->h_sys) {
+    gbd_batch_destroy(B);
+    return e != hipSuccess ? gb_hip_fail(e, "gbd_batch_create", __FILE__, __LINE__)
+                           : GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  *out = B;
+  return GOOSEBERRY_SUCCESS;
+}
+
+void gbd_batch_destroy(gbd_batch *B) {
+  if (!B) return;
+  hipStreamSynchronize(gb_stream());
+  gb_b_drop_graphs(B);
+  if (B->slot_mem)
+    for (uint32_t i = 0; i < B->max_slots; ++i) hipFree(B->slot_mem[i]);
+  hipFree(B->d_sys);
+  hipFree(B->d_st);
+  hipFree(B->d_lists);
+  hipFree(B->d_spmv);
+  hipFree(B->d_elem);
+  hipFree(B->d_sysl);
+  if (B->h_st) hipHostFree(B->h_st);
+  if (B->h_lists) hipHostFree(B->h_lists);
+  if (B->h_spmv) hipHostFree(B->h_spmv);
+  if (B->h_elem) hipHostFree(B->h_elem);
+  if (B->h_sysl) hipHostFree(B->h_sysl);
+  free(B->used);
+  free(B->slot_mem);
+  free(B->h_sys);
+  free(B);
+}
+
+int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
+                  double *d_x, double tol, int rel, uint64_t maxiter,
+                  int precision, uint32_t *slot_out) {
+  if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (precision != GOOSEBERRY_PRECISION_FP64 &&
+      precision != GOOSEBERRY_PRECISION_MIXED)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (precision == GOOSEBERRY_PRECISION_MIXED && !m->val32)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t slot = 0;
+  while (slot < B->max_slots && B->used[slot]) ++slot;
+  if (slot == B->max_slots) return GOOSEBERRY_ERROR_NOMEMORY;
+  const uint32_t nwork = gb_spmv_nwork(m);
+  const uint32_t nch = gb_b_nchunks(m->nrows);
+  const gb_b_layout L = gb_b_plan_layout(m, precision);
+  uint8_t *mem = nullptr;
+  GB_HIP(hipMalloc((void **)&mem, L.total));
+  gb_bsys S;
+  memset(&S, 0, sizeof(S));
+  S.rowptr = m->rowptr;
+  S.rowind = m->rowind;
+  S.blk = m->blk;
+  S.wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
+  S.block_rows = m->rowbins + (m->nrows - m->nblockrows);
+  S.val = m->val;
+  S.val32 = m->val32;
+  S.n = m->nrows;
+  S.nstream = m->nstream;
+  S.nwave = m->nwave;
+  S.nwaveblocks = gb_spmv_nwaveblocks(m);
+  S.nblockrows = m->nblockrows;
+  S.nwork = nwork;
+  S.nchunks = nch;
+  S.exact = m->params.exact;
+  S.b = d_b;
+  S.x = d_x;
+  S.part1 = (double *)(mem + L.part1);
+  S.part2 = (double *)(mem + L.part2);
+  S.r = (double *)(mem + L.r);
+  if (precision == GOOSEBERRY_PRECISION_MIXED) {
+    S.d32 = (float *)(mem + L.d32);
+    S.r32 = (float *)(mem + L.r32);
+    S.p32 = (float *)(mem + L.p32);
+    S.ap32 = (float *)(mem + L.ap32);
+  } else {
+    S.p = (double *)(mem + L.p);
+    S.ap = (double *)(mem + L.ap);
+  }
+  S.st = B->d_st + slot;
+  S.precision = precision;
+  hipStream_t s = gb_stream();
+  B->h_sys[slot] = S;
+  GB_HIP(hipMemcpyAsync(B->d_sys + slot, &B->h_sys[slot], sizeof(gb_bsys),
+                        hipMemcpyHostToDevice, s));
+  uint32_t g = nch > 1024 ? 1024 : (nch ? nch : 1);
+  gb_b_init<<<g, GB_BLOCK, 0, s>>>(B->d_sys, slot);
+  GB_LAUNCH_CHECK();
+  gb_b_init_state<<<1, GB_BLOCK, 0, s>>>(B->d_sys, slot, tol, rel,
+                                         (unsigned long long)maxiter);
+  GB_LAUNCH_CHECK();
+  // the staging copy of h_sys[slot] must not change before the copy runs
+  GB_HIP(hipStreamSynchronize(s));
+  B->used[slot] = 1;
+  B->slot_mem[slot] = mem;
+  *slot_out = slot;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_batch_remove(gbd_batch *B, uint32_t slot) {
+  if (slot >= B->max_slots || !B->used[slot]) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  GB_HIP(hipStreamSynchronize(gb_stream()));
+  GB_HIP(hipFree(B->slot_mem[slot]));
+  B->slot_mem[slot] = nullptr;
+  B->used[slot] = 0;
+  return GOOSEBERRY_SUCCESS;
+}
+
+// Rebuild the work lists from the active slots (active[i] != 0 and in use).
+int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
+  hipStream_t s = gb_stream();
+  uint64_t ns = 0, ne = 0;
+  uint32_t nsys = 0;
+  for (uint32_t i = 0; i < B->max_slots; ++i) {
+    if (!B->used[i] || (active && !active[i])) continue;
+    ns += B->h_sys[i].nwork;
+    ne += B->h_sys[i].nchunks;
+    ++nsys;
+  }
+  if (ns > 0xffffffffull || ne > 0xffffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  GB_HIP(hipStreamSynchronize(s));  // staging buffers are reused
+  if (ns > B->cap_spmv || !B->d_spmv) {
+    uint32_t cap = (uint32_t)(ns * 2 + 1024);
+    if (B->d_spmv) GB_HIP(hipFree(B->d_spmv));
+    if (B->h_spmv) GB_HIP(hipHostFree(B->h_spmv));
+    GB_HIP(hipMalloc((void **)&B->d_spmv, (size_t)cap * 8));
+    GB_HIP(hipHostMalloc((void **)&B->h_spmv, (size_t)cap * 8, hipHostMallocDefault));
+    B->cap_spmv = cap;
+    gb_b_drop_graphs(B);  // the graphs hold the old list pointer
+  }
+  if (ne > B->cap_elem || !B->d_elem) {
+    uint32_t cap = (uint32_t)(ne * 2 + 1024);
+    if (B->d_elem) GB_HIP(hipFree(B->d_elem));
+    if (B->h_elem) GB_HIP(hipHostFree(B->h_elem));
+    GB_HIP(hipMalloc((void **)&B->d_elem, (size_t)cap * 8));
+    GB_HIP(hipHostMalloc((void **)&B->h_elem, (size_t)cap * 8, hipHostMallocDefault));
+    B->cap_elem = cap;
+    gb_b_drop_graphs(B);
+  }
+  uint64_t a = 0, c = 0;
+  uint32_t q = 0;
+  for (uint32_t i = 0; i < B->max_slots; ++i) {
+    if (!B->used[i] || (active && !active[i])) continue;
+    for (uint32_t w = 0; w < B->h_sys[i].nwork; ++w) {
+      B->h_spmv[2 * a] = i;
+      B->h_spmv[2 * a + 1] = w;
+      ++a;
+    }
+    for (uint32_t k = 0; k < B->h_sys[i].nchunks; ++k) {
+      B->h_elem[2 * c] = i;
+      B->h_elem[2 * c + 1] = k;
+      ++c;
+    }
+    B->h_sysl[q++] = i;
+  }
+  B->h_lists->n_spmv = (uint32_t)ns;
+  B->h_lists->n_elem = (uint32_t)ne;
+  B->h_lists->n_sys = nsys;
+  if (ns) GB_HIP(hipMemcpyAsync(B->d_spmv, B->h_spmv, ns * 8, hipMemcpyHostToDevice, s));
+  if (ne) GB_HIP(hipMemcpyAsync(B->d_elem, B->h_elem, ne * 8, hipMemcpyHostToDevice, s));
+  if (nsys)
+    GB_HIP(hipMemcpyAsync(B->d_sysl, B->h_sysl, (size_t)nsys * 4, hipMemcpyHostToDevice, s));
+  GB_HIP(hipMemcpyAsync(B->d_lists, B->h_lists, sizeof(gb_blists), hipMemcpyHostToDevice, s));
+  return GOOSEBERRY_SUCCESS;
+}
+
+// k steps, replayed from graphs of 2^i steps captured on first use.
+int gbd_batch_step(gbd_batch *B, uint32_t k) {
+  hipStream_t s = gb_stream();
+  if (!B->d_spmv || !B->d_elem) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  for (int i = GB_B_NGRAPH - 1; i >= 0 && k; --i) {
+    const uint32_t len = 1u << i;
+    while (k >= len) {
+      if (!B->gexec[i]) {
+        GB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rv = gb_b_enqueue_raw(B, len);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (rv != GOOSEBERRY_SUCCESS) {
+          if (g) hipGraphDestroy(g);
+          return rv;
+        }
+        GB_HIP(e);
+        B->graph[i] = g;
+        GB_HIP(hipGraphInstantiate(&B->gexec[i], g, nullptr, nullptr, 0));
+      }
+      GB_HIP(hipGraphLaunch(B->gexec[i], s));
+      k -= len;
+    }
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_batch_poll(gbd_batch *B, gbd_bstate *states) {
+  hipStream_t s = gb_stream();
+  GB_HIP(hipMemcpyAsync(B->h_st, B->d_st, (size_t)B->max_slots * sizeof(gbd_bstate),
+                        hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  memcpy(states, B->h_st, (size_t)B->max_slots * sizeof(gbd_bstate));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_batch_slot_used(const gbd_batch *B, uint32_t slot) {
+  return slot < B->max_slots && B->used[slot];
+}
+
+}  // extern "C"

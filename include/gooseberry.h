@@ -334,7 +334,182 @@ int gooseberry_cgd(
     size_t niter,
     gooseberry_cgd_info_t * info);
 
+/* ------------------------------------------------------- batched solves */
+/* Independent CG solves advanced together: one device step advances every
+ * active system by one CG iteration (or one FP64 residual refresh) through
+ * work lists over all systems, replayed from captured hipGraphs.  Every
+ * sum of a system runs over that system's own fixed decomposition, so its
+ * result is the same bits alone and in any mix.
+ *
+ * GOOSEBERRY_PRECISION_FP64: the reference's CG (src/cgd.c:40-129).
+ * GOOSEBERRY_PRECISION_MIXED: FP32 matrix values and inner vectors, dot
+ * products accumulated in FP64, FP64 solution; when the FP32 recurrence
+ * residual has dropped by `delta` (batch parameter) the solution update is
+ * folded into the FP64 x and the residual recomputed as b - A x with the FP64
+ * matrix (reliable update); convergence is declared on that FP64 residual. */
+#define GOOSEBERRY_PRECISION_FP64 0
+#define GOOSEBERRY_PRECISION_MIXED 1
+
+typedef struct gooseberry_batch_s * gooseberry_batch_t;
+
+/* delta <= 0 selects 1e-4. */
+int gooseberry_batch_create(
+    uint32_t max_systems,
+    double delta,
+    gooseberry_batch_t * r_batch);
+
+int gooseberry_batch_free(gooseberry_batch_t batch);
+
+/* Add a solve of mat x = b (device vectors, x overwritten; mat, d_b, d_x
+ * must stay alive until the solve is removed).  The batch makes the FP32
+ * copy of the values for MIXED. */
+int gooseberry_batch_add(
+    gooseberry_batch_t batch,
+    gooseberry_matrix_t mat,
+    const real_t * d_b,
+    real_t * d_x,
+    real_t tol,
+    int tol_mode,
+    size_t niter,
+    int precision,
+    uint32_t * r_slot);
+
+int gooseberry_batch_remove(
+    gooseberry_batch_t batch,
+    uint32_t slot);
+
+/* Advance all unfinished systems by up to `steps` steps; returns the number
+ * still running. */
+int gooseberry_batch_run(
+    gooseberry_batch_t batch,
+    uint32_t steps,
+    uint32_t * r_running);
+
+/* Run until every system has stopped. */
+int gooseberry_batch_solve(gooseberry_batch_t batch);
+
+/* State of one slot after the last run (niter = CG iterations; nrefine =
+ * FP64 refreshes for MIXED; rnorm = last FP64 residual norm for MIXED). */
+int gooseberry_batch_info(
+    gooseberry_batch_t batch,
+    uint32_t slot,
+    gooseberry_cgd_info_t * info);
+
+/* Device bytes of one solve's workspace (excluding the matrix and the
+ * caller's b and x). */
+size_t gooseberry_batch_workspace_bytes(
+    gooseberry_matrix_t mat,
+    int precision);
+
+/* ------------------------------------------------------------ solve server */
+/* Continuous batching: requests are queued, admitted first-in first-out
+ * while their device-memory footprint fits the budget and a slot is free,
+ * advanced together one CG step per device step with every other admitted
+ * solve (requests that arrive while others run join at the next step), and
+ * retired as they finish. */
+typedef struct gooseberry_server_s * gooseberry_server_t;
+
+typedef struct gooseberry_request_result_t {
+  uint64_t id;
+  int status;            /* 1 converged, 2 iteration cap, 3 breakdown */
+  uint64_t niter;        /* CG (inner) iterations */
+  uint64_t nrefine;      /* FP64 refreshes (MIXED) */
+  double rel_residual;   /* final residual norm / ||b|| */
+  double submit_s;       /* CLOCK_MONOTONIC seconds: submission, */
+  double admit_s;        /*   admission, */
+  double done_s;         /*   completion seen by the host */
+  dim_t n;
+  uint64_t nnz;
+} gooseberry_request_result_t;
+
+/* mem_budget 0: 90% of the device's free memory at creation.
+ * keep_results: keep each generated request's solution on the device
+ * until gooseberry_server_take_result. */
+int gooseberry_server_create(
+    size_t mem_budget,
+    uint32_t max_active,
+    int precision,
+    double delta,
+    int keep_results,
+    gooseberry_server_t * r_server);
+
+int gooseberry_server_free(gooseberry_server_t server);
+
+/* A request generated on the device at admission: the random symmetric
+ * diagonally dominant matrix of gooseberry_matrix_generate_sdd(n, seed,
+ * dmin, dmax) and b uniform in [-1, 1) from (seed, stream 1); relative
+ * tolerance tol. */
+int gooseberry_server_submit_sdd(
+    gooseberry_server_t server,
+    uint64_t id,
+    dim_t n,
+    uint64_t seed,
+    dim_t dmin,
+    dim_t dmax,
+    real_t tol);
+
+/* A request on a resident matrix and device vectors owned by the caller
+ * (kept alive until the request is reported done); relative tolerance. */
+int gooseberry_server_submit(
+    gooseberry_server_t server,
+    uint64_t id,
+    gooseberry_matrix_t mat,
+    const real_t * d_b,
+    real_t * d_x,
+    real_t tol);
+
+/* Admit what fits, advance every admitted solve by up to `steps` steps,
+ * retire the finished ones and report up to max_done of them. */
+int gooseberry_server_progress(
+    gooseberry_server_t server,
+    uint32_t steps,
+    gooseberry_request_result_t * done,
+    uint32_t max_done,
+    uint32_t * r_ndone);
+
+int gooseberry_server_status(
+    gooseberry_server_t server,
+    uint32_t * r_active,
+    uint32_t * r_queued,
+    size_t * r_mem_in_use);
+
+/* Copy a finished generated request's solution (n values) and release it. */
+int gooseberry_server_take_result(
+    gooseberry_server_t server,
+    uint64_t id,
+    real_t * h_x,
+    size_t n);
+
 /* ------------------------------------------------------- synthetic inputs */
+
+/* The random symmetric, strictly diagonally dominant SPD matrix described
+ * in gb_gen.hip, generated in HBM: row lengths up to a target uniform in
+ * [dmin, dmax], uniform column positions, off-diagonals in [-1, 0),
+ * diagonal 1.01 x the row's absolute off-diagonal sum. */
+int gooseberry_matrix_generate_sdd(
+    dim_t n,
+    uint64_t seed,
+    dim_t dmin,
+    dim_t dmax,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * r_mat);
+
+/* Copy a resident matrix back to host arrays (nrows+1, nnz, nnz). */
+int gooseberry_matrix_download(
+    gooseberry_matrix_t mat,
+    ind_t * rowptr,
+    dim_t * rowind,
+    real_t * rowval);
+
+/* d_out[i] uniform in [lo, hi) from a counter-based hash of (seed, stream,
+ * i), filled on the device. */
+int gooseberry_generate_uniform_device(
+    uint64_t seed,
+    uint64_t stream,
+    size_t n,
+    real_t lo,
+    real_t hi,
+    real_t * d_out);
 /* 2D 5-point / 3D 7-point Dirichlet Laplacians on nx*ny(*nz) grids,
  * lexicographic order, columns ascending within a row (host arrays). */
 int gooseberry_generate_laplacian(

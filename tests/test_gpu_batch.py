@@ -1,0 +1,212 @@
+"""GPU tests of the batched solver, the mixed-precision (FP32 inner, FP64
+reliable-update) CG, the generated SDD systems and the continuous-batching
+server, through the C ABI.
+
+Bars: every solve reaches its relative residual (checked with the oracle's
+FP64 SpMV); a system gives bitwise the same solution and iteration count
+alone and inside any mix; the mixed-precision result meets the FP64
+tolerance against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def true_rel(oracle, rp, ri, rv, x, b):
+    r = b - oracle.spmv(rp, ri, rv, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def lap(gb, dims, side):
+    rp, ri, rv = gb.laplacian(dims, side)
+    return gb.Matrix(rp, ri, rv), (rp, ri, rv)
+
+
+# ------------------------------------------------------------- generator
+
+def test_generated_sdd_structure(gb):
+    n = 50000
+    M = gb.generate_sdd(n, 7, 3, 27)
+    rp, ri, rv = M.download()
+    A = sp.csr_matrix((rv, ri, rp), shape=(n, n))
+    lens = np.diff(rp.astype(np.int64))
+    assert lens.min() >= 1 and lens.max() <= 28
+    assert (A - A.T).nnz == 0 or abs(A - A.T).max() == 0.0
+    # diagonal is the last entry of each row: 1.01 x the absolute
+    # off-diagonal sum in row order (1 for an empty row)
+    for i in np.random.default_rng(0).integers(0, n, 300):
+        a, b = int(rp[i]), int(rp[i + 1])
+        assert ri[b - 1] == i
+        s = 0.0
+        for k in range(a, b - 1):
+            assert -1.0 <= rv[k] < 0.0 and ri[k] != i
+            s = s + (-rv[k])
+        assert rv[b - 1] == (1.01 * s if b - 1 > a else 1.0)
+    # columns spread over all rows
+    counts = np.bincount(ri, minlength=n)
+    assert np.count_nonzero(counts) == n
+    M2 = gb.generate_sdd(n, 7, 3, 27)
+    assert all(np.array_equal(u, v) for u, v in zip(M2.download(), (rp, ri, rv)))
+    M3 = gb.generate_sdd(n, 8, 3, 27)
+    assert not np.array_equal(M3.download()[1][:1000], ri[:1000])
+
+
+# ---------------------------------------------------------------- batches
+
+def solve_batch(gb, systems, tol=1e-10, precision=0, niter=0, order=None):
+    """systems: list of (Matrix, b) -> list of (x, info) in input order."""
+    B = gb.Batch(max(len(systems), 1))
+    bufs = []
+    idx = list(range(len(systems))) if order is None else order
+    slots = {}
+    for i in idx:
+        M, b = systems[i]
+        db = gb.DeviceArray.from_host(b)
+        dx = gb.DeviceArray(M.nrows)
+        slots[i] = B.add(M, db, dx, tol, gb.TOL_RELATIVE, niter, precision)
+        bufs.append((db, dx))
+    B.solve()
+    out = [None] * len(systems)
+    for k, i in enumerate(idx):
+        out[i] = (bufs[k][1].numpy(), B.info(slots[i]))
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_solves_converge(gb, oracle, precision):
+    rng = np.random.default_rng(1)
+    mats = [lap(gb, 2, 60), lap(gb, 3, 24), lap(gb, 2, 17)]
+    sdd = gb.generate_sdd(30000, 3, 3, 27)
+    systems = [(M, rng.uniform(-1, 1, M.nrows)) for M, _ in mats]
+    systems.append((sdd, rng.uniform(-1, 1, sdd.nrows)))
+    res = solve_batch(gb, systems, 1e-10, precision)
+    hosts = [h for _, h in mats] + [sdd.download()]
+    for (x, info), (M, b), (rp, ri, rv) in zip(res, systems, hosts):
+        assert info.status == 1, info.as_dict()
+        tr = true_rel(oracle, rp, ri, rv, x, b)
+        assert tr <= 1.05e-10, tr
+        if precision == 1:
+            assert info.nrefine >= 1 and info.true_rel_residual <= 1e-10
+
+
+def test_batch_fp64_matches_single_solver_iterations(gb, oracle):
+    M, (rp, ri, rv) = lap(gb, 2, 96)
+    b = gb.uniform(4, M.nrows)
+    (x, info), = solve_batch(gb, [(M, b)], 1e-10, 0)
+    xs, info_s = M.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    assert abs(int(info.niter) - int(info_s.niter)) <= 2
+    xr, it, _, _ = oracle.cgd(rp, ri, rv, b, 1e-10 * np.linalg.norm(b))
+    assert abs(int(info.niter) - it) <= 2
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_alone_and_in_the_mix_bitwise(gb, precision):
+    rng = np.random.default_rng(2)
+    target, _ = lap(gb, 3, 20)
+    bt = rng.uniform(-1, 1, target.nrows)
+    others = [lap(gb, 2, 40)[0], gb.generate_sdd(20000, 11), lap(gb, 3, 12)[0],
+              gb.generate_sdd(70000, 12)]
+    (x_alone, i_alone), = solve_batch(gb, [(target, bt)], 1e-10, precision)
+    mix = [(M, rng.uniform(-1, 1, M.nrows)) for M in others]
+    mix.insert(2, (target, bt))
+    res = solve_batch(gb, mix, 1e-10, precision, order=[4, 1, 2, 0, 3])
+    x_mix, i_mix = res[2]
+    assert i_mix.niter == i_alone.niter and i_mix.nrefine == i_alone.nrefine
+    assert np.array_equal(x_mix, x_alone)
+
+
+def test_batch_caps_and_zero_rhs(gb):
+    M, _ = lap(gb, 2, 50)
+    b = gb.uniform(5, M.nrows)
+    res = solve_batch(gb, [(M, b), (M, np.zeros(M.nrows))], 1e-12, 0, niter=9)
+    assert res[0][1].status == 2 and res[0][1].niter == 9
+    assert res[1][1].status == 1 and res[1][1].niter == 0 and np.all(res[1][0] == 0)
+    res = solve_batch(gb, [(M, b)], 1e-12, 1, niter=9)
+    assert res[0][1].status == 2 and res[0][1].niter == 9
+
+
+def test_mixed_matches_fp64_oracle_solution(gb, oracle):
+    # the FP64 oracle: the reference's double CG run to 1e-12
+    M, (rp, ri, rv) = lap(gb, 3, 32)
+    b = gb.uniform(6, M.nrows)
+    xr, _, _, _ = oracle.cgd(rp, ri, rv, b, 1e-12 * np.linalg.norm(b))
+    (x, info), = solve_batch(gb, [(M, b)], 1e-10, 1)
+    assert info.status == 1
+    err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    assert err < 1e-8, err
+
+
+# ----------------------------------------------------------------- server
+
+def test_server_generated_requests(gb, oracle):
+    S = gb.Server(max_active=8, precision=gb.PRECISION_MIXED, keep_results=True)
+    sizes = [4096, 30000, 9000, 120000, 5000, 60000, 17000, 4096, 80000, 25000]
+    for i, n in enumerate(sizes):
+        S.submit_sdd(i, n, 1000 + i, 3, 27, 1e-10)
+    done = []
+    for _ in range(2000):
+        done += S.progress(8)
+        if len(done) == len(sizes):
+            break
+    assert sorted(r["id"] for r in done) == list(range(len(sizes)))
+    for r in done:
+        assert r["status"] == 1 and r["rel_residual"] <= 1e-10
+        assert r["submit_s"] <= r["admit_s"] <= r["done_s"]
+    # every solution checked against the matrix regenerated alone, and the
+    # solution equal bitwise to the same request solved alone
+    for rid in (1, 3, 8):
+        n = sizes[rid]
+        x = S.take_result(rid, n)
+        M = gb.generate_sdd(n, 1000 + rid, 3, 27)
+        rp, ri, rv = M.download()
+        b = gb.uniform_device(1000 + rid, n, stream=1).numpy()
+        assert true_rel(oracle, rp, ri, rv, x, b) <= 1.05e-10
+        S1 = gb.Server(max_active=1, precision=gb.PRECISION_MIXED, keep_results=True)
+        S1.submit_sdd(0, n, 1000 + rid, 3, 27, 1e-10)
+        while not S1.progress(8):
+            pass
+        assert np.array_equal(S1.take_result(0, n), x)
+
+
+def test_server_memory_admission_queues(gb):
+    # a budget that fits one request at a time: the rest wait in the queue
+    n = 20000
+    one = None
+    S = gb.Server(mem_budget=1, max_active=4, precision=gb.PRECISION_FP64)
+    S.submit_sdd(0, n, 1, 3, 27, 1e-8)
+    with pytest.raises(gb.GooseberryError) as e:
+        S.progress(1)
+    assert e.value.name == "NOMEMORY"
+    S.free()
+    S = gb.Server(mem_budget=40 << 20, max_active=4, precision=gb.PRECISION_FP64)
+    for i in range(4):
+        S.submit_sdd(i, n, 10 + i, 3, 27, 1e-8)
+    peak = 0
+    done = []
+    for _ in range(1000):
+        done += S.progress(4)
+        a, q, m = S.status()
+        peak = max(peak, a)
+        assert m <= 40 << 20
+        if len(done) == 4:
+            break
+    assert len(done) == 4 and peak == 1
+    assert all(r["status"] == 1 for r in done)
+
+
+def test_server_user_matrices(gb, oracle):
+    S = gb.Server(max_active=4, precision=gb.PRECISION_FP64)
+    items = []
+    for i, side in enumerate((30, 45, 22)):
+        M, host = lap(gb, 2, side)
+        b = gb.uniform(20 + i, M.nrows)
+        db, dx = gb.DeviceArray.from_host(b), gb.DeviceArray(M.nrows)
+        S.submit(i, M, db, dx, 1e-9)
+        items.append((M, host, b, dx))
+    done = []
+    while len(done) < 3:
+        done += S.progress(8)
+    for M, (rp, ri, rv), b, dx in items:
+        assert true_rel(oracle, rp, ri, rv, dx.numpy(), b) <= 1.05e-9
