@@ -171,16 +171,18 @@ def test_server_generated_requests(gb, oracle):
 
 
 def test_server_memory_admission_queues(gb):
-    # a budget that fits one request at a time: the rest wait in the queue
+    # a 20000-row generated FP64 request is charged ~7.8 MB (matrix upper
+    # bound, workspace, b and x): a 12 MB budget admits one at a time and the
+    # rest wait in the queue
     n = 20000
-    one = None
+    budget = 12 << 20
     S = gb.Server(mem_budget=1, max_active=4, precision=gb.PRECISION_FP64)
     S.submit_sdd(0, n, 1, 3, 27, 1e-8)
     with pytest.raises(gb.GooseberryError) as e:
         S.progress(1)
     assert e.value.name == "NOMEMORY"
     S.free()
-    S = gb.Server(mem_budget=40 << 20, max_active=4, precision=gb.PRECISION_FP64)
+    S = gb.Server(mem_budget=budget, max_active=4, precision=gb.PRECISION_FP64)
     for i in range(4):
         S.submit_sdd(i, n, 10 + i, 3, 27, 1e-8)
     peak = 0
@@ -189,7 +191,7 @@ def test_server_memory_admission_queues(gb):
         done += S.progress(4)
         a, q, m = S.status()
         peak = max(peak, a)
-        assert m <= 40 << 20
+        assert m <= budget
         if len(done) == 4:
             break
     assert len(done) == 4 and peak == 1
