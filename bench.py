@@ -1,16 +1,33 @@
 """Benchmark of the sparse-solve path on one MI355X.
 
-Workload (the large FP64 configuration): 3D 7-point Dirichlet Laplacian on a
-160^3 grid (n = 4,096,000, nnz = 28,518,400, FP64 values 6/-1, int32
-indices), x and b uniform in [-1, 1) from seed 1.  One step = 100 SpMVs
-y = A x, then one CG solve from x0 = 0 to relative residual 1e-10 -- the
-whole configuration, inputs resident in HBM before the timed region.
+Default workload (the large FP64 configuration): 3D 7-point Dirichlet
+Laplacian on a 160^3 grid (n = 4,096,000, nnz = 28,518,400, FP64 values
+6/-1, int32 indices), x and b uniform in [-1, 1) from seed 1.  One step =
+100 SpMVs y = A x, then one CG solve from x0 = 0 to relative residual 1e-10
+-- the whole configuration, inputs resident in HBM before the timed region.
 
 Prints one JSON line: value = steps (= CG solves) per second, the SpMV
 roofline (algorithmic bytes 12 nnz + 4 (n+1) + 16 n per SpMV over the SpMV
 kernel's average duration, from HIP events on the library stream) and the
 CPU baseline (the oracle, the C restatement of the reference's OpenMP path,
 timed on a bounded sample on this host).
+
+--workload selects the other configurations, each printing its own line:
+  zipf     irregular rows: n = 2,000,000 random symmetric diagonally
+           dominant SPD matrix, degrees Zipf(1.5) clipped to [2, 4096] before
+           symmetrising, seed 2; FP32-inner CG with FP64 reliable updates to
+           1e-10 against plain FP64 CG, errors against FP64 CG run to 1e-12
+  batched  512 independent 2D/3D Laplacians (alternating), n log-uniform in
+           [1000, 200000], b from seed 3 + i; all advanced together by the
+           batched engine (graph-replayed steps), mixed precision to 1e-10;
+           default plans against autotuned plans, and against one-at-a-time
+           FP64 GPU CG
+  served   Poisson arrivals (200/s for --duration s) of random SDD requests,
+           n log-uniform in [4096, 1048576], degrees uniform in [3, 27]
+           before symmetrising, seed per request; continuous batching with
+           memory admission, mixed precision to 1e-10; p50/p99 latency, and
+           the capacity on a backlog of the same requests against the plain
+           one-at-a-time FP64 GPU CG
 """
 import argparse
 import json
@@ -49,10 +66,11 @@ def load_traffic():
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(rp, ri, rv, x, b, gpu_iters, seconds=15.0):
+def cpu_baseline(rp, ri, rv, x, b, gpu_iters, seconds=15.0, nspmv=NSPMV,
+                 what="the 160^3 Laplacian"):
     """The oracle on this host's cores: a bounded sample of SpMVs and CG
-    iterations; solves/s extrapolated with the full solve's iteration
-    count."""
+    iterations; solves/s extrapolated with the full solve's iteration count
+    (a step = nspmv SpMVs + one solve)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
@@ -80,44 +98,24 @@ def cpu_baseline(rp, ri, rv, x, b, gpu_iters, seconds=15.0):
         orc.cgd(rp, ri, rv, b, 0.0, niter=more)
         t_it = (time.perf_counter() - t0) / more
         iters = more
-    t_solve = NSPMV * t_spmv + gpu_iters * t_it
+    t_solve = nspmv * t_spmv + gpu_iters * t_it
     return {
         "value": 1.0 / t_solve,
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": ("%d SpMVs and %d CG iterations of the 160^3 Laplacian with the oracle "
+        "sample": ("%d SpMVs and %d CG iterations of %s with the oracle "
                    "(C + OpenMP restatement of src/blas.c and src/cgd.c, -O3, no FMA) on "
-                   "%d threads; a step's time = 100 SpMVs + %d iterations (the full "
+                   "%d threads; a step's time = %d SpMVs + %d iterations (the full FP64 "
                    "solve's count) at the sampled per-iteration time"
-                   % (k, iters, threads, gpu_iters)),
+                   % (k, iters, what, threads, nspmv, gpu_iters)),
         "spmv_gbps": spmv_bytes(n, nnz) / t_spmv / 1e9,
         "spmv_ms": t_spmv * 1e3,
         "cg_iter_ms": t_it * 1e3,
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--grid", type=int, default=GRID)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run 4 device copies of exactly 256 MiB, the "
-                         "known-size kernel that scales the rocprofv3 "
-                         "FETCH_SIZE/WRITE_SIZE counters to bytes")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build is single-GPU (the path "
-              "would shard by row blocks with a halo exchange; see DESIGN.md). "
-              "Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-
-    import gooseberry_amd as gb
-
+def run_laplacian(args, gb):
     g = args.grid
     rp, ri, rv = gb.laplacian(3, g)
     n, nnz = rp.size - 1, int(rp[-1])
@@ -215,6 +213,378 @@ def main():
         out["cpu_baseline"] = cpu_baseline(rp, ri, rv, x, b, it, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    return out
+
+
+# ------------------------------------------------------------- shared bits
+
+def line(value, unit, args, ms_per_step, dtype, config, **extra):
+    out = {
+        "metric": METRIC, "value": value, "unit": unit, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic", "config": config,
+    }
+    out.update(extra)
+    return out
+
+
+def true_rel_residual(gb, M, db, dx, bh=None):
+    """||b - A x|| / ||b|| with the FP64 SpMV on the device."""
+    dy = gb.DeviceArray(M.nrows)
+    M.spmv_device(dx, dy)
+    b = db.numpy() if bh is None else bh
+    r = b - dy.numpy()
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def spmv_roofline(gb, M, dx, dy, nrep=20, val_bytes=8):
+    """Average FP64 SpMV launch time from HIP events on the library stream."""
+    e0, e1 = gb.Event(), gb.Event()
+    for _ in range(3):
+        M.spmv_device(dx, dy)
+    e0.record()
+    for _ in range(nrep):
+        M.spmv_device(dx, dy)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / nrep
+    inf = M.info()
+    ab = spmv_bytes(inf["nrows"], int(inf["nnz"]), val_bytes)
+    a = ab / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": a, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": a / PEAK_HBM_GBPS, "traffic": None,
+            "kernel": "gb_spmv_kernel<double,false>",
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": ms}
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
+
+
+# ------------------------------------------------ irregular rows (Zipf)
+
+def run_zipf(args, gb):
+    n = args.zipf_n
+    t0 = time.perf_counter()
+    M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    gen_s = time.perf_counter() - t0
+    inf = M.info()
+    nnz = int(inf["nnz"])
+    db = gb.uniform_device(2, n, stream=1)
+    bh = db.numpy()
+    log("zipf: n=%d nnz=%d (mean row %.2f) generated in %.2f s" % (n, nnz, nnz / n, gen_s))
+
+    # the FP64 reference solution: FP64 CG run to 1e-12
+    dref = gb.DeviceArray(n)
+    iref = M.cgd_device(db, dref, 1e-12, gb.TOL_RELATIVE)
+    xref = dref.numpy()
+    log("zipf: FP64 CG to 1e-12: %d iterations, status %d" % (iref.niter, iref.status))
+
+    def fp64_solve():
+        dx = gb.DeviceArray(n)
+        gb.synchronize()
+        t = time.perf_counter()
+        info = M.cgd_device(db, dx, TOL, gb.TOL_RELATIVE)
+        return time.perf_counter() - t, info, dx
+
+    def mixed_solve():
+        dx = gb.DeviceArray(n)
+        B = gb.Batch(1)
+        gb.synchronize()
+        t = time.perf_counter()
+        slot = B.add(M, db, dx, TOL, gb.TOL_RELATIVE, 0, gb.PRECISION_MIXED)
+        B.solve()
+        dt = time.perf_counter() - t
+        info = B.info(slot)
+        B.free()
+        return dt, info, dx
+
+    for _ in range(args.warmup):
+        fp64_solve()
+        mixed_solve()
+    f64 = [fp64_solve() for _ in range(args.steps)]
+    mix = [mixed_solve() for _ in range(args.steps)]
+    log("zipf: FP64 %.1f ms, mixed %.1f ms" % (np.median([a for a, _, _ in f64]) * 1e3,
+                                               np.median([a for a, _, _ in mix]) * 1e3))
+
+    def errors(dx, info):
+        x = dx.numpy()
+        return {"status": int(info.status), "iterations": int(info.niter),
+                "refreshes": int(info.nrefine),
+                "true_rel_residual": true_rel_residual(gb, M, db, dx, bh),
+                "rel_error_vs_fp64_1e-12": float(np.linalg.norm(x - xref) /
+                                                 np.linalg.norm(xref))}
+
+    t64 = float(np.median([a for a, _, _ in f64]))
+    tmx = float(np.median([a for a, _, _ in mix]))
+    dx = gb.DeviceArray(n)
+    roof = spmv_roofline(gb, M, db, dx)
+    cfg = {"workload": ("random symmetric diagonally dominant SPD, n=%d, degrees Zipf(1.5) "
+                        "clipped to [2,4096] then symmetrised (nnz=%d, mean row length "
+                        "%.2f), off-diagonals U[-1,0), diagonal 1.01 x |row sum|, seed 2; "
+                        "b U[-1,1) seed 2; step = one FP32-inner / FP64-refined CG solve "
+                        "to relative residual %g" % (n, nnz, nnz / n, TOL)),
+           "n": n, "nnz": nnz, "tol": TOL}
+    out = line(args.steps / sum(a for a, _, _ in mix), "solves/s", args, tmx * 1e3,
+               "f32+f64", cfg, roofline=roof,
+               mixed=dict(errors(mix[-1][2], mix[-1][1]), solve_ms=tmx * 1e3),
+               fp64=dict(errors(f64[-1][2], f64[-1][1]), solve_ms=t64 * 1e3),
+               speedup_mixed_vs_fp64=t64 / tmx,
+               p50_solve_ms=pct([a * 1e3 for a, _, _ in mix], 50),
+               p99_solve_ms=pct([a * 1e3 for a, _, _ in mix], 99),
+               fp64_reference={"tol": 1e-12, "iterations": int(iref.niter)},
+               generate_s=gen_s)
+    if not args.no_cpu_baseline:
+        rp, ri, rv = M.download()
+        x = gb.uniform(2, n, stream=0)
+        cb = cpu_baseline(rp, ri, rv, x, bh, int(f64[-1][1].niter), args.cpu_seconds,
+                          nspmv=0, what="the Zipf matrix (n=%d, nnz=%d)" % (n, nnz))
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+# ------------------------------------------------------ batched systems
+
+def batched_systems(gb, nsys):
+    """512 (by default) 2D/3D Laplacians, alternating, n log-uniform in
+    [1000, 200000] from the library's host generator (seed 3, stream 2)."""
+    u = gb.uniform(3, nsys, 0.0, 1.0, stream=2)
+    out = []
+    for i in range(nsys):
+        dims = 2 if i % 2 == 0 else 3
+        target = float(np.exp(np.log(1000.0) + u[i] * (np.log(200000.0) - np.log(1000.0))))
+        side = max(2, int(round(target ** (1.0 / dims))))
+        out.append((dims, side))
+    return out
+
+
+def run_batched(args, gb):
+    shapes = batched_systems(gb, args.nsys)
+    mats, rhs = [], []
+    t0 = time.perf_counter()
+    for i, (dims, side) in enumerate(shapes):
+        rp, ri, rv = gb.laplacian(dims, side)
+        M = gb.Matrix(rp, ri, rv)
+        mats.append(M)
+        rhs.append(gb.DeviceArray.from_host(gb.uniform(3 + i, M.nrows, stream=1)))
+    ns = np.array([M.nrows for M in mats])
+    nnz = int(sum(int(M.info()["nnz"]) for M in mats))
+    log("batched: %d systems, sum n=%d, nnz=%d, built in %.1f s"
+        % (len(mats), ns.sum(), nnz, time.perf_counter() - t0))
+    xs = [gb.DeviceArray(M.nrows) for M in mats]
+
+    def batch_solve(precision):
+        B = gb.Batch(len(mats))
+        gb.synchronize()
+        t = time.perf_counter()
+        slots = [B.add(M, b, x, TOL, gb.TOL_RELATIVE, 0, precision)
+                 for M, b, x in zip(mats, rhs, xs)]
+        B.solve()
+        dt = time.perf_counter() - t
+        infos = [B.info(s) for s in slots]
+        B.free()
+        return dt, infos
+
+    def summary(dt, infos):
+        its = [int(i.niter) for i in infos]
+        return {"solves_per_s": len(infos) / dt, "wall_ms": dt * 1e3,
+                "all_converged": all(int(i.status) == 1 for i in infos),
+                "iterations_max": max(its), "iterations_mean": float(np.mean(its))}
+
+    for _ in range(args.warmup):
+        batch_solve(gb.PRECISION_MIXED)
+    mixed_default = [batch_solve(gb.PRECISION_MIXED) for _ in range(args.steps)]
+    fp64_batched = batch_solve(gb.PRECISION_FP64)
+    # residuals of the last mixed batch, checked on a sample of systems
+    check = list(range(0, len(mats), max(1, len(mats) // 16)))
+    worst = max(true_rel_residual(gb, mats[i], rhs[i], xs[i]) for i in check)
+
+    # autotuned plans: one tuning per distinct matrix shape class
+    t = time.perf_counter()
+    tuned = gb.autotune_all(mats) if hasattr(gb, "autotune_all") else None
+    tune_s = time.perf_counter() - t
+    mixed_tuned = [batch_solve(gb.PRECISION_MIXED) for _ in range(args.steps)] if tuned else []
+    for M in mats:
+        M.replan(None)
+
+    # one at a time: the plain FP64 GPU CG, a sample of the systems
+    sample = list(range(0, len(mats), max(1, len(mats) // 64)))
+    t1 = time.perf_counter()
+    single_iters = []
+    for i in sample:
+        info = mats[i].cgd_device(rhs[i], xs[i], TOL, gb.TOL_RELATIVE)
+        single_iters.append(int(info.niter))
+    gb.synchronize()
+    t_single = (time.perf_counter() - t1) / len(sample)
+
+    tm = float(np.median([a for a, _ in mixed_default]))
+    cfg = {"workload": ("%d independent Laplacians (2D 5-point / 3D 7-point alternating), n "
+                        "log-uniform in [1000, 200000] (sum n=%d, nnz=%d), b U[-1,1) seed "
+                        "3+i; step = all systems solved together by the batched engine, "
+                        "FP32-inner / FP64-refined CG to relative residual %g"
+                        % (len(mats), ns.sum(), nnz, TOL)),
+           "systems": len(mats), "sum_n": int(ns.sum()), "nnz": nnz, "tol": TOL}
+    out = line(len(mats) * args.steps / sum(a for a, _ in mixed_default), "solves/s", args,
+               tm * 1e3, "f32+f64", cfg,
+               mixed_default_plans=summary(*mixed_default[-1]),
+               worst_sampled_true_rel_residual=worst,
+               fp64_batched=summary(*fp64_batched),
+               one_at_a_time_fp64={"solves_per_s": 1.0 / t_single,
+                                   "systems_sampled": len(sample),
+                                   "iterations_mean": float(np.mean(single_iters))},
+               speedup_vs_one_at_a_time_fp64=(len(mats) / tm) * t_single,
+               roofline=None, cpu_baseline=None)
+    if mixed_tuned:
+        out["mixed_autotuned_plans"] = summary(*mixed_tuned[-1])
+        out["autotune"] = dict(tuned, seconds=tune_s)
+    return out
+
+
+# --------------------------------------------------------- served stream
+
+def served_requests(gb, rate, duration):
+    """Poisson arrivals at `rate`/s for `duration` s (seed 5): arrival times,
+    n log-uniform in [4096, 1048576], request seeds 1000003 + k."""
+    m = int(rate * duration * 1.5) + 64
+    u = gb.uniform(5, 2 * m, 0.0, 1.0, stream=3)
+    gaps = -np.log1p(-u[:m]) / rate
+    t = np.cumsum(gaps)
+    t = t[t < duration]
+    k = t.size
+    ns = np.exp(np.log(4096.0) + u[m:m + k] * (np.log(1048576.0) - np.log(4096.0)))
+    ns = np.maximum(4096, np.round(ns)).astype(np.int64)
+    return t, ns, 1000003 + np.arange(k)
+
+
+def now_mono():
+    return time.clock_gettime(time.CLOCK_MONOTONIC)
+
+
+def serve(gb, S, t_arr, ns, seeds, steps=4, progress_every=10.0):
+    """Submit each request at its arrival time, advance the server, return
+    per-request latency (done - arrival) and the done records."""
+    k = len(t_arr)
+    t0 = now_mono()
+    nxt, done, lat = 0, [], {}
+    last_log = t0
+    while len(done) < k:
+        now = now_mono() - t0
+        while nxt < k and t_arr[nxt] <= now:
+            S.submit_sdd(int(nxt), int(ns[nxt]), int(seeds[nxt]), 3, 27, TOL)
+            nxt += 1
+        a, q, _ = S.status()
+        if a == 0 and q == 0:
+            if nxt < k:
+                time.sleep(max(0.0, min(0.002, t_arr[nxt] - (now_mono() - t0))))
+            continue
+        for r in S.progress(steps):
+            done.append(r)
+            lat[r["id"]] = r["done_s"] - (t0 + t_arr[r["id"]])
+        if now_mono() - last_log > progress_every:
+            last_log = now_mono()
+            log("served: t=%.1f s submitted %d done %d active %d queued %d"
+                % (last_log - t0, nxt, len(done), a, q))
+    return done, lat, now_mono() - t0
+
+
+def run_served(args, gb):
+    t_arr, ns, seeds = served_requests(gb, args.rate, args.duration)
+    log("served: %d requests over %.0f s, mean n %.0f" % (len(ns), args.duration, ns.mean()))
+    # the Poisson stream through the continuous-batching server
+    S = gb.Server(max_active=64, precision=gb.PRECISION_MIXED)
+    done, lat, wall = serve(gb, S, t_arr, ns, seeds)
+    S.free()
+    lats = np.array([lat[r["id"]] for r in done]) * 1e3
+    ok = all(r["status"] == 1 and r["rel_residual"] <= TOL for r in done)
+    nnz = np.array([r["nnz"] for r in done])
+    its = np.array([r["niter"] for r in done])
+
+    # capacity on a backlog of the first requests: served vs one at a time
+    kc = min(args.capacity_requests, len(ns))
+    S = gb.Server(max_active=64, precision=gb.PRECISION_MIXED)
+    zeros = np.zeros(kc)
+    _, _, t_served = serve(gb, S, zeros, ns[:kc], seeds[:kc])
+    S.free()
+    t = time.perf_counter()
+    single_its = []
+    for i in range(kc):
+        M = gb.generate_sdd(int(ns[i]), int(seeds[i]), 3, 27)
+        db = gb.uniform_device(int(seeds[i]), int(ns[i]), stream=1)
+        dx = gb.DeviceArray(int(ns[i]))
+        info = M.cgd_device(db, dx, TOL, gb.TOL_RELATIVE)
+        single_its.append(int(info.niter))
+        M.free()
+    gb.synchronize()
+    t_single = time.perf_counter() - t
+    cap_served = kc / t_served
+    cap_single = kc / t_single
+    log("served: capacity %.1f solves/s batched mixed vs %.1f one-at-a-time FP64"
+        % (cap_served, cap_single))
+
+    cfg = {"workload": ("served stream: Poisson arrivals %.0f/s for %.0f s (%d requests), "
+                        "random SDD SPD n log-uniform in [4096, 1048576], degrees U[3,27] "
+                        "then symmetrised, values as the Zipf workload, seed per request; "
+                        "continuous batching (64 slots, memory admission), FP32-inner / "
+                        "FP64-refined CG to relative residual %g"
+                        % (args.rate, args.duration, len(ns), TOL)),
+           "requests": int(len(ns)), "rate": args.rate, "duration_s": args.duration,
+           "tol": TOL}
+    return line(len(done) / wall, "solves/s", args, wall * 1e3, "f32+f64", cfg,
+                p50_latency_ms=pct(lats, 50), p99_latency_ms=pct(lats, 99),
+                all_converged=ok, mean_nnz=float(nnz.mean()), mean_iterations=float(its.mean()),
+                capacity={"requests": kc, "served_mixed_solves_per_s": cap_served,
+                          "one_at_a_time_fp64_solves_per_s": cap_single,
+                          "ratio": cap_served / cap_single,
+                          "one_at_a_time_mean_iterations": float(np.mean(single_its))},
+                roofline=None, cpu_baseline=None)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 10; zipf 3; batched 3)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps (default 2; zipf 1; batched 1)")
+    ap.add_argument("--workload", default="laplacian",
+                    choices=["laplacian", "zipf", "batched", "served"])
+    ap.add_argument("--grid", type=int, default=GRID)
+    ap.add_argument("--zipf-n", type=int, default=2000000)
+    ap.add_argument("--nsys", type=int, default=512)
+    ap.add_argument("--duration", type=float, default=60.0,
+                    help="served: seconds of Poisson arrivals")
+    ap.add_argument("--rate", type=float, default=200.0,
+                    help="served: arrivals per second")
+    ap.add_argument("--capacity-requests", type=int, default=200,
+                    help="served: requests of the backlog capacity comparison")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 4 device copies of exactly 256 MiB, the "
+                         "known-size kernel that scales the rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE counters to bytes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build is single-GPU (the path "
+              "would shard by row blocks with a halo exchange; see DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0)}
+    st, wu = defaults[args.workload]
+    args.steps = st if args.steps is None else args.steps
+    args.warmup = wu if args.warmup is None else args.warmup
+
+    import gooseberry_amd as gb
+
+    run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
+           "served": run_served}[args.workload]
+    out = run(args, gb)
     print(json.dumps(out), flush=True)
 
 

@@ -63,10 +63,13 @@ EXPORTED = [
     "gooseberry_server_submit", "gooseberry_server_progress", "gooseberry_server_status",
     "gooseberry_server_take_result", "gooseberry_matrix_generate_sdd",
     "gooseberry_matrix_download", "gooseberry_generate_uniform_device",
+    "gooseberry_matrix_generate_spd", "gooseberry_matrix_autotune",
 ]
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+DEGREE_UNIFORM = 0
+DEGREE_ZIPF = 1
 
 
 class GooseberryError(RuntimeError):
@@ -185,7 +188,11 @@ def _sig(lib):
         "gooseberry_server_take_result": (i32, [_P, u64, _P, sz]),
         "gooseberry_matrix_generate_sdd": (i32, [u32, u64, u32, u32, C.POINTER(_Plan),
                                                  C.POINTER(_P)]),
+        "gooseberry_matrix_generate_spd": (i32, [u32, u64, C.c_int, f64, u32, u32,
+                                                 C.POINTER(_Plan), C.POINTER(_P)]),
         "gooseberry_matrix_download": (i32, [_P, _P, _P, _P]),
+        "gooseberry_matrix_autotune": (i32, [_P, C.c_int, u32, C.POINTER(_Plan),
+                                             C.POINTER(f64)]),
         "gooseberry_generate_uniform_device": (i32, [u64, u64, sz, f64, f64, _P]),
     }
     for name, (res, args) in s.items():
@@ -367,6 +374,15 @@ class Matrix:
 
     def replan(self, plan=None):
         check(lib().gooseberry_matrix_replan(self.h, _plan(plan)), "gooseberry_matrix_replan")
+
+    def autotune(self, precision=PRECISION_FP64, reps=5):
+        """Time candidate plans from the row-length histogram, keep the
+        fastest; returns (plan dict, SpMV ms)."""
+        p = _Plan()
+        ms = C.c_double()
+        check(lib().gooseberry_matrix_autotune(self.h, precision, reps, C.byref(p),
+                                               C.byref(ms)), "gooseberry_matrix_autotune")
+        return {k: getattr(p, k) for k, _ in _Plan._fields_}, ms.value
 
     def spmv_device(self, d_x, d_y):
         check(lib().gooseberry_matrix_spmv(self.h, d_x.ptr, d_y.ptr), "gooseberry_matrix_spmv")
@@ -561,11 +577,44 @@ def uniform(seed, n, lo=-1.0, hi=1.0, stream=0):
 
 
 def generate_sdd(n, seed, dmin=3, dmax=27, plan=None):
-    """Random symmetric diagonally dominant SPD matrix generated in HBM."""
+    """Random symmetric diagonally dominant SPD matrix generated in HBM,
+    degrees uniform in [dmin, dmax] before symmetrising."""
     h = _P()
     check(lib().gooseberry_matrix_generate_sdd(n, seed, dmin, dmax, _plan(plan), C.byref(h)),
           "gooseberry_matrix_generate_sdd")
     return Matrix._from_handle(h.value)
+
+
+def generate_spd(n, seed, law=DEGREE_UNIFORM, zipf_s=1.5, dmin=2, dmax=4096, plan=None):
+    """Random symmetric diagonally dominant SPD matrix generated in HBM with
+    degrees drawn by `law` (DEGREE_UNIFORM or DEGREE_ZIPF) before
+    symmetrising."""
+    h = _P()
+    check(lib().gooseberry_matrix_generate_spd(n, seed, law, zipf_s, dmin, dmax,
+                                                 _plan(plan), C.byref(h)),
+          "gooseberry_matrix_generate_spd")
+    return Matrix._from_handle(h.value)
+
+
+def autotune_all(mats, precision=PRECISION_MIXED, reps=5):
+    """Autotune many matrices: one timed search per class of matrices with
+    the same occupied row-length buckets and nnz within a factor of 4 (the
+    largest of the class is tuned), its plan applied to the whole class."""
+    groups = {}
+    for M in mats:
+        inf = M.info()
+        occupied = tuple(int(b) for b in np.nonzero(np.asarray(inf["hist"]))[0])
+        key = (occupied, int(np.log2(max(int(inf["nnz"]), 1))) // 2)
+        groups.setdefault(key, []).append(M)
+    plans = {}
+    for key, ms in groups.items():
+        rep = max(ms, key=lambda m: int(m.info()["nnz"]))
+        plan, t = rep.autotune(precision, reps)
+        for M in ms:
+            if M is not rep:
+                M.replan(plan)
+        plans[str(key)] = {"plan": plan, "spmv_ms": t, "matrices": len(ms)}
+    return {"classes": len(groups), "plans": plans}
 
 
 def uniform_device(seed, n, lo=-1.0, hi=1.0, stream=0):

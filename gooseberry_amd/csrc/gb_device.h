@@ -204,9 +204,11 @@ int gbd_batch_slot_used(const gbd_batch *b, uint32_t slot);
 
 /* ------------------------------------------------ generated matrices */
 /* Random symmetric, strictly diagonally dominant matrix built on the GPU
- * (see gb_gen.hip); the caller owns the returned device arrays. */
-int gbd_generate_sdd(uint32_t n, uint64_t seed, uint32_t dmin, uint32_t dmax,
-    uint32_t **d_rowptr, uint32_t **d_rowind, double **d_rowval);
+ * (see gb_gen.hip): degrees drawn by `law` (GOOSEBERRY_DEGREE_*), then
+ * symmetrised.  The caller owns the returned device arrays. */
+int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
+    uint32_t dmin, uint32_t dmax, uint32_t **d_rowptr, uint32_t **d_rowind,
+    double **d_rowval);
 /* d_out[i] = lo + (hi - lo) u, u from a counter-based hash of (seed,
  * stream, i) */
 int gbd_fill_uniform(double *d_out, size_t n, uint64_t seed, uint64_t stream,

@@ -155,6 +155,25 @@ static int wrap_matrix(gbd_matrix *m, gooseberry_matrix_t *r_mat)
   return GOOSEBERRY_SUCCESS;
 }
 
+/* wrap device arrays the library allocated itself (no copy); the arrays
+ * are freed here when the matrix cannot be made */
+int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
+    dim_t *d_rowind, real_t *d_rowval, const gooseberry_plan_t *plan,
+    gooseberry_matrix_t *r_mat)
+{
+  gbd_plan_params pp;
+  int have;
+  plan_to_dev(plan, NULL, &pp, &have);
+  gbd_matrix *m = NULL;
+  int rv = gbd_matrix_create(nrows, ncols, d_rowptr, d_rowind, d_rowval,
+      have ? &pp : NULL, &m);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(d_rowptr); gbd_free(d_rowind); gbd_free(d_rowval);
+    return rv;
+  }
+  return wrap_matrix(m, r_mat);
+}
+
 int gooseberry_matrix_create(dim_t nrows, dim_t ncols, const ind_t *rowptr,
     const dim_t *rowind, const real_t *rowval, const gooseberry_plan_t *plan,
     gooseberry_matrix_t *r_mat)

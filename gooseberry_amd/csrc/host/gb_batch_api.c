@@ -165,20 +165,23 @@ int gooseberry_batch_info(gooseberry_batch_t B, uint32_t slot,
 
 /* ------------------------------------------------------ generated inputs */
 
-int gooseberry_matrix_generate_sdd(dim_t n, uint64_t seed, dim_t dmin,
-    dim_t dmax, const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
+int gooseberry_matrix_generate_spd(dim_t n, uint64_t seed, int law,
+    double zipf_s, dim_t dmin, dim_t dmax, const gooseberry_plan_t *plan,
+    gooseberry_matrix_t *r_mat)
 {
   if (!r_mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t *rp = NULL, *ri = NULL;
   double *rv_ = NULL;
-  int rv = gbd_generate_sdd(n, seed, dmin, dmax, &rp, &ri, &rv_);
+  int rv = gbd_generate_spd(n, seed, law, zipf_s, dmin, dmax, &rp, &ri, &rv_);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  /* the device arrays are copied into the matrix; free the originals */
-  rv = gooseberry_matrix_create_device(n, n, rp, ri, rv_, plan, r_mat);
-  gbd_free(rp);
-  gbd_free(ri);
-  gbd_free(rv_);
-  return rv;
+  return gb_matrix_adopt(n, n, rp, ri, rv_, plan, r_mat);
+}
+
+int gooseberry_matrix_generate_sdd(dim_t n, uint64_t seed, dim_t dmin,
+    dim_t dmax, const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
+{
+  return gooseberry_matrix_generate_spd(n, seed, GOOSEBERRY_DEGREE_UNIFORM,
+      0.0, dmin, dmax, plan, r_mat);
 }
 
 int gooseberry_matrix_download(gooseberry_matrix_t mat, ind_t *rowptr,
@@ -238,19 +241,30 @@ struct gooseberry_server_s {
   uint32_t nactive;
 };
 
-/* Device bytes a request holds while admitted (an upper bound for
- * generated requests, whose nnz is known only after generation). */
+/* Device bytes a request holds while admitted: its workspace, and for a
+ * generated request an upper bound of its matrix (nnz <= n (2 dmax - 1)),
+ * b and x. */
 static size_t req_footprint(const struct gooseberry_server_s *S, const req_t *r)
 {
   const size_t n = r->n;
   size_t vec = (S->precision == GOOSEBERRY_PRECISION_MIXED ? 8 + 16 : 24) * n;
   size_t ws = vec + 64 * 1024 + n / 4;   /* vectors + partials and slack */
   if (!r->generated) return ws;
-  size_t nnz = n * ((size_t)r->dmax + 1);
+  size_t nnz = n * (2 * (size_t)r->dmax - 1);
   size_t mat = nnz * (4 + 8 + (S->precision == GOOSEBERRY_PRECISION_MIXED ? 4 : 0))
              + 4 * (n + 1)       /* rowptr */
              + 4 * n + 8 * (n / 8 + 16); /* plan: row bins, blocks */
   return ws + mat + 16 * n;      /* b and x */
+}
+
+/* Device bytes a generated request needs on top of its footprint while it
+ * is generated (edge lists and the counting sort's buffers, 30 bytes per
+ * drawn edge at most, all freed before admission ends). */
+static size_t req_transient(const req_t *r)
+{
+  if (!r->generated) return 0;
+  const size_t m = (size_t)r->n * ((size_t)r->dmax - 1);
+  return 30 * m + 12 * (size_t)r->n + (1u << 20);
 }
 
 int gooseberry_server_create(size_t mem_budget, uint32_t max_active,
@@ -332,7 +346,7 @@ static int enqueue(struct gooseberry_server_s *S, req_t *r)
 int gooseberry_server_submit_sdd(gooseberry_server_t S, uint64_t id, dim_t n,
     uint64_t seed, dim_t dmin, dim_t dmax, real_t tol)
 {
-  if (!S || n == 0 || dmin < 1 || dmax < dmin || dmax > 255 || !(tol > 0))
+  if (!S || n < 2 || dmin < 1 || dmax < dmin || dmax > 65535 || !(tol > 0))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   req_t *r = calloc(1, sizeof(*r));
   if (!r) return GOOSEBERRY_ERROR_NOMEMORY;
@@ -381,26 +395,29 @@ static int admit(struct gooseberry_server_s *S, req_t *r)
     gooseberry_matrix_info(r->mat, &mi);
     r->res.nnz = mi.nnz;
     if ((rv = gbd_malloc((void **)&r->own_b, (size_t)r->n * 8)) != GOOSEBERRY_SUCCESS ||
-        (rv = gbd_malloc((void **)&r->own_x, (size_t)r->n * 8)) != GOOSEBERRY_SUCCESS) {
-      gooseberry_matrix_free(r->mat);
-      gbd_free(r->own_b);
-      r->mat = NULL;
-      r->own_b = NULL;
-      return rv;
-    }
-    rv = gbd_fill_uniform(r->own_b, r->n, r->seed, 1, -1.0, 1.0);
-    if (rv != GOOSEBERRY_SUCCESS) return rv;
+        (rv = gbd_malloc((void **)&r->own_x, (size_t)r->n * 8)) != GOOSEBERRY_SUCCESS ||
+        (rv = gbd_fill_uniform(r->own_b, r->n, r->seed, 1, -1.0, 1.0)) != GOOSEBERRY_SUCCESS)
+      goto fail;
     r->d_b = r->own_b;
     r->d_x = r->own_x;
   }
   rv = gooseberry_batch_add(S->batch, r->mat, r->d_b, r->d_x, r->tol,
       GOOSEBERRY_TOL_RELATIVE, 0, S->precision, &r->slot);
-  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (rv != GOOSEBERRY_SUCCESS) goto fail;
   r->state = REQ_ACTIVE;
   r->res.admit_s = now_s();
   S->slots[r->slot] = r;
   S->nactive++;
   return GOOSEBERRY_SUCCESS;
+fail:
+  if (r->generated) {
+    gooseberry_matrix_free(r->mat);
+    gbd_free(r->own_b);
+    gbd_free(r->own_x);
+    r->mat = NULL;
+    r->own_b = r->own_x = NULL;
+  }
+  return rv;
 }
 
 int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
@@ -411,7 +428,7 @@ int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
   /* 1. admission: first in, first out, while the head fits */
   while (S->qhead < S->qtail && S->nactive < S->max_active) {
     req_t *r = S->queue[S->qhead];
-    if (S->in_use + r->footprint > S->budget) {
+    if (S->in_use + r->footprint + req_transient(r) > S->budget) {
       if (S->nactive == 0 && S->nkept == 0) {
         /* can never fit: fail the request instead of blocking the queue */
         S->qhead++;

@@ -264,6 +264,19 @@ int gooseberry_matrix_replan(
     gooseberry_matrix_t mat,
     const gooseberry_plan_t * plan);
 
+/* Choose the SpMV plan by timing candidates drawn from the row-length
+ * histogram (HIP events over `reps` SpMVs each, 0 = 5; FP64 SpMV, or FP32
+ * for GOOSEBERRY_PRECISION_MIXED) and keep the fastest; *r_plan and *r_ms
+ * (may be NULL) receive it and its SpMV time.  Results do not depend on the
+ * plan beyond the SpMV's rounding tolerance.  Not while the matrix is in a
+ * batch or a server (they hold its plan arrays). */
+int gooseberry_matrix_autotune(
+    gooseberry_matrix_t mat,
+    int precision,
+    uint32_t reps,
+    gooseberry_plan_t * r_plan,
+    double * r_ms);
+
 typedef struct gooseberry_matrix_info_t {
   dim_t nrows, ncols;
   uint64_t nnz;
@@ -437,8 +450,8 @@ int gooseberry_server_free(gooseberry_server_t server);
 
 /* A request generated on the device at admission: the random symmetric
  * diagonally dominant matrix of gooseberry_matrix_generate_sdd(n, seed,
- * dmin, dmax) and b uniform in [-1, 1) from (seed, stream 1); relative
- * tolerance tol. */
+ * dmin, dmax) (n >= 2) and b uniform in [-1, 1) from (seed, stream 1);
+ * relative tolerance tol. */
 int gooseberry_server_submit_sdd(
     gooseberry_server_t server,
     uint64_t id,
@@ -482,10 +495,28 @@ int gooseberry_server_take_result(
 
 /* ------------------------------------------------------- synthetic inputs */
 
-/* The random symmetric, strictly diagonally dominant SPD matrix described
- * in gb_gen.hip, generated in HBM: row lengths up to a target uniform in
- * [dmin, dmax], uniform column positions, off-diagonals in [-1, 0),
- * diagonal 1.01 x the row's absolute off-diagonal sum. */
+/* Degree laws of the generated matrices. */
+#define GOOSEBERRY_DEGREE_UNIFORM 0   /* d uniform in [dmin, dmax] */
+#define GOOSEBERRY_DEGREE_ZIPF 1      /* d = Zipf(s) clipped to [dmin, dmax] */
+
+/* Random symmetric, strictly diagonally dominant SPD matrix generated in
+ * HBM (n >= 2).  Row i draws a degree d_i by `law` and d_i - 1 directed
+ * off-diagonals at uniform columns != i with values in [-1, 0); the pattern
+ * is then symmetrised (each drawn entry also placed at its transpose), and
+ * the diagonal, stored last in its row, is 1.01 x the row's absolute
+ * off-diagonal sum.  nnz = n + 2 sum(d_i - 1), which must stay below 2^32.
+ * Deterministic in (n, seed, law, s, dmin, dmax). */
+int gooseberry_matrix_generate_spd(
+    dim_t n,
+    uint64_t seed,
+    int law,
+    double zipf_s,
+    dim_t dmin,
+    dim_t dmax,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * r_mat);
+
+/* gooseberry_matrix_generate_spd with degrees uniform in [dmin, dmax]. */
 int gooseberry_matrix_generate_sdd(
     dim_t n,
     uint64_t seed,

@@ -19,6 +19,20 @@ def true_rel(oracle, rp, ri, rv, x, b):
     return np.linalg.norm(r) / np.linalg.norm(b)
 
 
+def assert_symmetric_entries(rp, ri, rv):
+    """The stored entries of A and of A^T are the same multiset of
+    (row, column, value) triples (positions drawn more than once are kept as
+    separate entries, whose sums scipy would add in different orders)."""
+    n = rp.size - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    cols = ri.astype(np.int64)
+    o1 = np.lexsort((rv, cols, rows))
+    o2 = np.lexsort((rv, rows, cols))
+    assert np.array_equal(rows[o1], cols[o2])
+    assert np.array_equal(cols[o1], rows[o2])
+    assert np.array_equal(rv[o1], rv[o2])
+
+
 def lap(gb, dims, side):
     rp, ri, rv = gb.laplacian(dims, side)
     return gb.Matrix(rp, ri, rv), (rp, ri, rv)
@@ -26,16 +40,27 @@ def lap(gb, dims, side):
 
 # ------------------------------------------------------------- generator
 
+def test_generated_sdd_matches_restatement(gb):
+    # the generator restated in numpy (tests/spd_restatement.py): bitwise
+    # equal CSR arrays for the uniform degree law
+    from spd_restatement import generate_uniform
+    for n, seed, dmin, dmax in ((5000, 7, 3, 27), (3001, 8, 2, 9), (2, 1, 1, 3)):
+        M = gb.generate_sdd(n, seed, dmin, dmax)
+        got = M.download()
+        want = generate_uniform(n, seed, dmin, dmax)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+
+
 def test_generated_sdd_structure(gb):
     n = 50000
     M = gb.generate_sdd(n, 7, 3, 27)
     rp, ri, rv = M.download()
-    A = sp.csr_matrix((rv, ri, rp), shape=(n, n))
+    assert_symmetric_entries(rp, ri, rv)
     lens = np.diff(rp.astype(np.int64))
-    assert lens.min() >= 1 and lens.max() <= 28
-    assert (A - A.T).nnz == 0 or abs(A - A.T).max() == 0.0
-    # diagonal is the last entry of each row: 1.01 x the absolute
-    # off-diagonal sum in row order (1 for an empty row)
+    # out-degree d - 1 in [2, 26] plus in-edges (mean 14) plus the diagonal
+    assert lens.min() >= 3 and abs(lens.mean() - 29.0) < 0.2
+    # diagonal last: 1.01 x the absolute off-diagonal sum in row order
     for i in np.random.default_rng(0).integers(0, n, 300):
         a, b = int(rp[i]), int(rp[i + 1])
         assert ri[b - 1] == i
@@ -43,14 +68,33 @@ def test_generated_sdd_structure(gb):
         for k in range(a, b - 1):
             assert -1.0 <= rv[k] < 0.0 and ri[k] != i
             s = s + (-rv[k])
-        assert rv[b - 1] == (1.01 * s if b - 1 > a else 1.0)
-    # columns spread over all rows
+        assert rv[b - 1] == 1.01 * s
     counts = np.bincount(ri, minlength=n)
     assert np.count_nonzero(counts) == n
     M2 = gb.generate_sdd(n, 7, 3, 27)
     assert all(np.array_equal(u, v) for u, v in zip(M2.download(), (rp, ri, rv)))
     M3 = gb.generate_sdd(n, 8, 3, 27)
     assert not np.array_equal(M3.download()[1][:1000], ri[:1000])
+
+
+def test_generated_zipf_degrees(gb):
+    # Zipf(1.5) clipped to [2, 4096]: nnz = n + 2 sum(d_i - 1); the degree
+    # draws restated in numpy agree except where the two pow() differ in the
+    # last bit at an acceptance boundary
+    from spd_restatement import zipf_degrees
+    n = 200000
+    M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    info = M.info()
+    d = zipf_degrees(n, 2, 1.5, 2, 4096)
+    want = n + 2 * int((d - 1).sum())
+    assert abs(int(info["nnz"]) - want) <= 2 * 4096 * 2
+    rp, ri, rv = M.download()
+    assert_symmetric_entries(rp, ri, rv)
+    A = sp.csr_matrix((rv.copy(), ri.copy(), rp.copy()), shape=(n, n))
+    diag = rv[rp[1:].astype(np.int64) - 1]
+    assert np.all(ri[rp[1:].astype(np.int64) - 1] == np.arange(n))
+    off = np.abs(A).sum(axis=1).A1 - diag
+    assert np.allclose(diag, 1.01 * off, rtol=1e-12)
 
 
 # ---------------------------------------------------------------- batches
@@ -171,11 +215,12 @@ def test_server_generated_requests(gb, oracle):
 
 
 def test_server_memory_admission_queues(gb):
-    # a 20000-row generated FP64 request is charged ~7.8 MB (matrix upper
-    # bound, workspace, b and x): a 12 MB budget admits one at a time and the
-    # rest wait in the queue
+    # a 20000-row generated FP64 request (degrees up to 27) is charged
+    # ~13.8 MB while admitted (matrix upper bound, workspace, b and x) and
+    # needs ~16.8 MB more while it is generated: a 36 MiB budget admits one
+    # at a time and the rest wait in the queue
     n = 20000
-    budget = 12 << 20
+    budget = 36 << 20
     S = gb.Server(mem_budget=1, max_active=4, precision=gb.PRECISION_FP64)
     S.submit_sdd(0, n, 1, 3, 27, 1e-8)
     with pytest.raises(gb.GooseberryError) as e:
@@ -212,3 +257,35 @@ def test_server_user_matrices(gb, oracle):
         done += S.progress(8)
     for M, (rp, ri, rv), b, dx in items:
         assert true_rel(oracle, rp, ri, rv, dx.numpy(), b) <= 1.05e-9
+
+
+# ---------------------------------------------------------------- autotune
+
+def _row_bound(rp, ri, rv, x):
+    # |fl(sum) - sum| <= (len - 1) eps sum |terms| for any summation order
+    terms = np.abs(rv * x[ri.astype(np.int64)])
+    lens = np.diff(rp.astype(np.int64))
+    s = np.add.reduceat(terms, rp[:-1].astype(np.int64)) if terms.size else terms
+    s = np.where(lens > 0, s, 0.0)
+    return 2.0 * np.maximum(lens - 1, 0) * np.finfo(np.float64).eps * s
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_keeps_results(gb, oracle, precision):
+    n = 30000
+    M = gb.generate_spd(n, 4, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    rp, ri, rv = M.download()
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    yr = oracle.spmv(rp, ri, rv, x)
+    plan, ms = M.autotune(precision, 3)
+    assert ms > 0
+    got = M.info()["plan"]
+    assert all(got[k] == plan[k] for k in ("short_max", "wave_max", "window", "rowcap"))
+    assert plan["window"] + plan["short_max"] <= 2304 and plan["rowcap"] <= 512
+    y = M.spmv(x)
+    assert np.all(np.abs(y - yr) <= _row_bound(rp, ri, rv, x))
+    # a tuned matrix solves in a batch as before
+    b = gb.uniform(9, n)
+    (xs, info), = solve_batch(gb, [(M, b)], 1e-10, precision)
+    assert info.status == 1
+    assert true_rel(oracle, rp, ri, rv, xs, b) <= 1.05e-10
