@@ -64,6 +64,7 @@ EXPORTED = [
     "gooseberry_server_take_result", "gooseberry_matrix_generate_sdd",
     "gooseberry_matrix_download", "gooseberry_generate_uniform_device",
     "gooseberry_matrix_generate_spd", "gooseberry_matrix_autotune",
+    "gooseberry_add_scalar", "gooseberry_matrix_pagerank", "gooseberry_pagerank",
 ]
 
 PRECISION_FP64 = 0
@@ -108,6 +109,10 @@ class CgdInfo(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PagerankInfo(C.Structure):
+    _fields_ = [("niter", C.c_uint64), ("rmse", C.c_double), ("nsinks", C.c_uint32)]
 
 
 class RequestResult(C.Structure):
@@ -193,6 +198,11 @@ def _sig(lib):
         "gooseberry_matrix_download": (i32, [_P, _P, _P, _P]),
         "gooseberry_matrix_autotune": (i32, [_P, C.c_int, u32, C.POINTER(_Plan),
                                              C.POINTER(f64)]),
+        "gooseberry_add_scalar": (i32, [u32, _P, f64]),
+        "gooseberry_matrix_pagerank": (i32, [_P, f64, sz, f64, _P,
+                                             C.POINTER(PagerankInfo)]),
+        "gooseberry_pagerank": (i32, [u32, _P, _P, _P, f64, sz, f64, _P,
+                                      C.POINTER(PagerankInfo)]),
         "gooseberry_generate_uniform_device": (i32, [u64, u64, sz, f64, f64, _P]),
     }
     for name, (res, args) in s.items():
@@ -375,6 +385,16 @@ class Matrix:
     def replan(self, plan=None):
         check(lib().gooseberry_matrix_replan(self.h, _plan(plan)), "gooseberry_matrix_replan")
 
+    def pagerank(self, damping=0.85, niter=0, minerror=0.0):
+        """`gooseberry pagerank` on this matrix: (vector, iterations, rmse,
+        number of sinks)."""
+        d = DeviceArray(self.nrows)
+        info = PagerankInfo()
+        check(lib().gooseberry_matrix_pagerank(self.h, float(damping), int(niter),
+                                               float(minerror), d.ptr, C.byref(info)),
+              "gooseberry_matrix_pagerank")
+        return d.numpy(), int(info.niter), float(info.rmse), int(info.nsinks)
+
     def autotune(self, precision=PRECISION_FP64, reps=5):
         """Time candidate plans from the row-length histogram, keep the
         fastest; returns (plan dict, SpMV ms)."""
@@ -448,6 +468,25 @@ def scale(val, s):
     val = _f64(val).copy()
     check(lib().gooseberry_scale(val.size, _ptr(val), float(s)), "gooseberry_scale")
     return val
+
+
+def add_scalar(vals, s):
+    """gooseberry_add_scalar on a host array (returns a new array)."""
+    out = np.array(vals, dtype=np.float64, copy=True)
+    check(lib().gooseberry_add_scalar(out.size, _ptr(out), float(s)), "gooseberry_add_scalar")
+    return out
+
+
+def pagerank(rowptr, rowind, rowval, damping=0.85, niter=0, minerror=0.0):
+    """gooseberry_pagerank on host CSR arrays: (vector, iterations, rmse)."""
+    rowptr, rowind, rowval = _u32(rowptr), _u32(rowind), _f64(rowval)
+    n = rowptr.size - 1
+    out = np.zeros(n, dtype=np.float64)
+    info = PagerankInfo()
+    check(lib().gooseberry_pagerank(n, _ptr(rowptr), _ptr(rowind), _ptr(rowval),
+                                    float(damping), int(niter), float(minerror), _ptr(out),
+                                    C.byref(info)), "gooseberry_pagerank")
+    return out, int(info.niter), float(info.rmse)
 
 
 def cgd(rowptr, rowind, rowval, b, error, niter=0):

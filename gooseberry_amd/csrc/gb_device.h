@@ -32,6 +32,8 @@ int gbd_dtod(void *d_dst, const void *d_src, size_t bytes);
 int gbd_zero(void *d_dst, size_t bytes); /* fill kernel, safe under capture */
 /* d[i] *= s (src/blas.c:244-257) */
 int gbd_scale(double *d, size_t n, double s);
+/* d[i] += s (src/blas.c:397-411) */
+int gbd_add_scalar(double *d, size_t n, double s);
 int gbd_sync(void);
 void *gbd_stream(void); /* the library stream as a hipStream_t */
 int gbd_device_memory(size_t *free_bytes, size_t *total_bytes);
@@ -55,6 +57,11 @@ int gbd_scan_u32(const uint32_t *d_in, uint32_t *d_out, size_t n,
 int gbd_countingsort_kv(const uint32_t *d_keys, const uint32_t *d_vals,
     uint32_t kmin, uint32_t kmax, size_t n, uint32_t *d_out,
     uint32_t *d_dist);
+/* The same for keys known to lie in [kmin, kmax] (made by the library
+ * itself): no range check, no synchronisation. */
+int gbd_countingsort_kv_unchecked(const uint32_t *d_keys,
+    const uint32_t *d_vals, uint32_t kmin, uint32_t kmax, size_t n,
+    uint32_t *d_out, uint32_t *d_dist);
 /* src/permute.c:120-205 semantics on device arrays.  Writes the permuted
  * CSR into d_prowptr/d_prowind/d_prowval (distinct from the inputs). */
 int gbd_permute_sparse(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
@@ -95,6 +102,11 @@ typedef struct gbd_matrix gbd_matrix;
 int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
     uint32_t *d_rowind, double *d_rowval, const gbd_plan_params *params,
     gbd_matrix **out);
+/* The same for a CSR the library built itself (valid by construction, nnz
+ * known on the host): no read-back and no validation pass. */
+int gbd_matrix_create_trusted(uint32_t nrows, uint32_t ncols,
+    uint32_t *d_rowptr, uint32_t *d_rowind, double *d_rowval, uint32_t nnz,
+    const gbd_plan_params *params, gbd_matrix **out);
 int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params);
 void gbd_matrix_destroy(gbd_matrix *m);
 int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
@@ -208,7 +220,14 @@ int gbd_batch_slot_used(const gbd_batch *b, uint32_t slot);
  * symmetrised.  The caller owns the returned device arrays. */
 int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
     uint32_t dmin, uint32_t dmax, uint32_t **d_rowptr, uint32_t **d_rowind,
-    double **d_rowval);
+    double **d_rowval, uint32_t *r_nnz);
+/* ------------------------------------------------------------ PageRank */
+/* src/gooseberry_bin.c:2622-2737 on a resident square matrix (see
+ * gb_pagerank.hip): d_out receives the vector the reference writes. */
+int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
+    double minerror, double *d_out, uint64_t *r_iters, double *r_rmse,
+    uint32_t *r_nsinks);
+
 /* d_out[i] = lo + (hi - lo) u, u from a counter-based hash of (seed,
  * stream, i) */
 int gbd_fill_uniform(double *d_out, size_t n, uint64_t seed, uint64_t stream,

@@ -35,9 +35,14 @@
 
 #define GB_BCHUNK 2048           // rows per elementwise work item
 #define GB_BCHUNK_ITEMS (GB_BCHUNK / GB_BLOCK)
-#define GB_BGRID_SPMV 4096u      // fixed grids: the graphs stay valid for any
-#define GB_BGRID_ELEM 4096u      // membership of the batch
+// The SpMV and elementwise launches get one workgroup per entry of their
+// work-list allocation (workgroups past the current count exit at once), so
+// items are dispatched in list order, as one launch per matrix would be,
+// and a graph stays valid while the lists fit their allocation.  The
+// allocation is resized (and the graphs recaptured) when the lists outgrow
+// it or shrink below a quarter of it.
 #define GB_BGRID_SYS 256u
+#define GB_BLIST_MIN 1024u
 #define GB_BRED 1024             // threads of the per-system reduction
 #define GB_B_NGRAPH 7
 
@@ -78,6 +83,7 @@ struct gbd_batch {
   uint32_t cap_spmv, cap_elem;
   uint32_t *h_spmv, *h_elem, *h_sysl;  // pinned staging
   double delta;
+  uint32_t xcd_remap;
   hipGraph_t graph[GB_B_NGRAPH];
   hipGraphExec_t gexec[GB_B_NGRAPH];
 };
@@ -190,8 +196,7 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
           for (uint32_t j = 0; j < cn; ++j) s += gb_readlane(pr, j);
         }
       } else {
-        for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) s += val[k] * x[rowind[k]];
-        s = gb_wave_sum(s);
+        s = gb_wave_sum(gb_row_lane_sum<T, GB_WAVE>(rowind, val, x, k0, k1, lane));
       }
       if (lane == 0) {
         if (EPI == 0) { y[row] = s; dotp += (double)x[row] * (double)s; }
@@ -213,8 +218,7 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
         __syncthreads();
       }
     } else {
-      for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) s += val[k] * x[rowind[k]];
-      s = gb_wave_sum(s);
+      s = gb_wave_sum(gb_row_lane_sum<T, GB_BLOCK>(rowind, val, x, k0, k1, tid));
       if (lane == 0) tred[wave] = s;
       __syncthreads();
       if (tid == 0) {
@@ -237,12 +241,15 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
 
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
-          const gb_blists *__restrict__ lists) {
+          const gb_blists *__restrict__ lists, uint32_t remap) {
   __shared__ double prodd[GB_STREAM_LDS];
   __shared__ double tredd[GB_WAVES_PER_BLOCK];
   const uint32_t count = lists->n_spmv;
   const uint32_t lane = threadIdx.x & (GB_WAVE - 1), wave = threadIdx.x / GB_WAVE;
-  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
+  for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
+    // remap: each XCD walks one contiguous range of the list, so a system's
+    // items (and its gathered vector) mostly stay in one XCD's L2
+    const uint32_t it = (remap && gridDim.x >= count) ? gb_xcd_remap(b, count) : b;
     const uint32_t s = items[2 * it], w = items[2 * it + 1];
     const gb_bsys &S = sys[s];
     const gbd_bstate *st = S.st;
@@ -562,13 +569,14 @@ static void gb_b_drop_graphs(gbd_batch *B) {
 static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
   hipStream_t s = gb_stream();
   for (uint32_t i = 0; i < k; ++i) {
-    gb_b_spmv<<<GB_BGRID_SPMV, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists);
+    gb_b_spmv<<<B->cap_spmv, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists,
+                                                B->xcd_remap);
     GB_LAUNCH_CHECK();
     gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
     GB_LAUNCH_CHECK();
-    gb_b_update<<<GB_BGRID_ELEM, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists);
+    gb_b_update<<<B->cap_elem, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists);
     GB_LAUNCH_CHECK();
-    gb_b_pupdate<<<GB_BGRID_ELEM, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists,
+    gb_b_pupdate<<<B->cap_elem, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists,
                                                      B->delta);
     GB_LAUNCH_CHECK();
   }
@@ -626,18 +634,22 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
   gbd_batch *B = (gbd_batch *)calloc(1, sizeof(gbd_batch));
   if (!B) return GOOSEBERRY_ERROR_NOMEMORY;
   B->max_slots = max_slots;
-  B->delta = delta > 0 && delta < 1 ? delta : 1e-4;
+  B->delta = delta > 0 && delta < 1 ? delta : 0.05;
+  {
+    const char *e = getenv("GB_BATCH_XCD_REMAP");
+    B->xcd_remap = e && e[0] == '1';
+  }
   B->used = (uint8_t *)calloc(max_slots, 1);
   B->slot_mem = (void **)calloc(max_slots, sizeof(void *));
   B->h_sys = (gb_bsys *)calloc(max_slots, sizeof(gb_bsys));
-  hipError_t e = hipMalloc((void **)&B->d_sys, max_slots * sizeof(gb_bsys));
-  if (e == hipSuccess) e = hipMalloc((void **)&B->d_st, max_slots * sizeof(gbd_bstate));
+  hipError_t e = gb_dev_alloc((void **)&B->d_sys, max_slots * sizeof(gb_bsys));
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_st, max_slots * sizeof(gbd_bstate));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&B->h_st, max_slots * sizeof(gbd_bstate), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc((void **)&B->d_lists, sizeof(gb_blists));
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_lists, sizeof(gb_blists));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&B->h_lists, sizeof(gb_blists), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc((void **)&B->d_sysl, max_slots * sizeof(uint32_t));
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_sysl, max_slots * sizeof(uint32_t));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&B->h_sysl, max_slots * sizeof(uint32_t), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMemsetAsync(B->d_lists, 0, sizeof(gb_blists), gb_stream());
@@ -659,13 +671,13 @@ void gbd_batch_destroy(gbd_batch *B) {
   hipStreamSynchronize(gb_stream());
   gb_b_drop_graphs(B);
   if (B->slot_mem)
-    for (uint32_t i = 0; i < B->max_slots; ++i) hipFree(B->slot_mem[i]);
-  hipFree(B->d_sys);
-  hipFree(B->d_st);
-  hipFree(B->d_lists);
-  hipFree(B->d_spmv);
-  hipFree(B->d_elem);
-  hipFree(B->d_sysl);
+    for (uint32_t i = 0; i < B->max_slots; ++i) gb_dev_free(B->slot_mem[i]);
+  gb_dev_free(B->d_sys);
+  gb_dev_free(B->d_st);
+  gb_dev_free(B->d_lists);
+  gb_dev_free(B->d_spmv);
+  gb_dev_free(B->d_elem);
+  gb_dev_free(B->d_sysl);
   if (B->h_st) hipHostFree(B->h_st);
   if (B->h_lists) hipHostFree(B->h_lists);
   if (B->h_spmv) hipHostFree(B->h_spmv);
@@ -693,7 +705,7 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   const uint32_t nch = gb_b_nchunks(m->nrows);
   const gb_b_layout L = gb_b_plan_layout(m, precision);
   uint8_t *mem = nullptr;
-  GB_HIP(hipMalloc((void **)&mem, L.total));
+  GB_HIP(gb_dev_alloc((void **)&mem, L.total));
   gb_bsys S;
   memset(&S, 0, sizeof(S));
   S.rowptr = m->rowptr;
@@ -747,8 +759,8 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
 
 int gbd_batch_remove(gbd_batch *B, uint32_t slot) {
   if (slot >= B->max_slots || !B->used[slot]) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  GB_HIP(hipStreamSynchronize(gb_stream()));
-  GB_HIP(hipFree(B->slot_mem[slot]));
+  // ordered after the steps already queued that still read the workspace
+  GB_HIP(gb_dev_free(B->slot_mem[slot]));
   B->slot_mem[slot] = nullptr;
   B->used[slot] = 0;
   return GOOSEBERRY_SUCCESS;
@@ -767,20 +779,23 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   }
   if (ns > 0xffffffffull || ne > 0xffffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
   GB_HIP(hipStreamSynchronize(s));  // staging buffers are reused
-  if (ns > B->cap_spmv || !B->d_spmv) {
-    uint32_t cap = (uint32_t)(ns * 2 + 1024);
-    if (B->d_spmv) GB_HIP(hipFree(B->d_spmv));
+  auto resize = [](uint64_t need, uint32_t cap) {
+    return need > cap || (cap > 4 * GB_BLIST_MIN && need < cap / 4);
+  };
+  if (resize(ns, B->cap_spmv) || !B->d_spmv) {
+    uint32_t cap = (uint32_t)(ns + ns / 2 + GB_BLIST_MIN);
+    if (B->d_spmv) GB_HIP(gb_dev_free(B->d_spmv));
     if (B->h_spmv) GB_HIP(hipHostFree(B->h_spmv));
-    GB_HIP(hipMalloc((void **)&B->d_spmv, (size_t)cap * 8));
+    GB_HIP(gb_dev_alloc((void **)&B->d_spmv, (size_t)cap * 8));
     GB_HIP(hipHostMalloc((void **)&B->h_spmv, (size_t)cap * 8, hipHostMallocDefault));
     B->cap_spmv = cap;
     gb_b_drop_graphs(B);  // the graphs hold the old list pointer
   }
-  if (ne > B->cap_elem || !B->d_elem) {
-    uint32_t cap = (uint32_t)(ne * 2 + 1024);
-    if (B->d_elem) GB_HIP(hipFree(B->d_elem));
+  if (resize(ne, B->cap_elem) || !B->d_elem) {
+    uint32_t cap = (uint32_t)(ne + ne / 2 + GB_BLIST_MIN);
+    if (B->d_elem) GB_HIP(gb_dev_free(B->d_elem));
     if (B->h_elem) GB_HIP(hipHostFree(B->h_elem));
-    GB_HIP(hipMalloc((void **)&B->d_elem, (size_t)cap * 8));
+    GB_HIP(gb_dev_alloc((void **)&B->d_elem, (size_t)cap * 8));
     GB_HIP(hipHostMalloc((void **)&B->h_elem, (size_t)cap * 8, hipHostMallocDefault));
     B->cap_elem = cap;
     gb_b_drop_graphs(B);

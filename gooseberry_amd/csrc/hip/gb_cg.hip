@@ -272,14 +272,14 @@ int gbd_cg_create(const gbd_matrix *m, gbd_cg **out) {
   cg->grid_elem = ge > GB_ELEM_GRID_MAX ? GB_ELEM_GRID_MAX : ge;
   uint32_t gs = gb_spmv_npartials(m);
   cg->npartials = gs > cg->grid_elem ? gs : cg->grid_elem;
-  hipError_t e = hipMalloc((void **)&cg->r, nb);
-  if (e == hipSuccess) e = hipMalloc((void **)&cg->p, nb);
-  if (e == hipSuccess) e = hipMalloc((void **)&cg->ap, nb);
+  hipError_t e = gb_dev_alloc((void **)&cg->r, nb);
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&cg->p, nb);
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&cg->ap, nb);
   if (e == hipSuccess)
-    e = hipMalloc((void **)&cg->partials, (size_t)cg->npartials * sizeof(double));
+    e = gb_dev_alloc((void **)&cg->partials, (size_t)cg->npartials * sizeof(double));
   if (e == hipSuccess)
-    e = hipMalloc((void **)&cg->partials2, (size_t)cg->grid_elem * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc((void **)&cg->st, sizeof(gbd_cg_state));
+    e = gb_dev_alloc((void **)&cg->partials2, (size_t)cg->grid_elem * sizeof(double));
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&cg->st, sizeof(gbd_cg_state));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&cg->h_st, sizeof(gbd_cg_state), hipHostMallocDefault);
   if (e == hipSuccess) e = hipMemsetAsync(cg->st, 0, sizeof(gbd_cg_state), gb_stream());
@@ -295,13 +295,13 @@ void gbd_cg_destroy(gbd_cg *cg) {
   if (!cg) return;
   hipStreamSynchronize(gb_stream());
   gb_cg_drop_graph(cg);
-  hipFree(cg->r);
-  hipFree(cg->p);
-  hipFree(cg->ap);
-  hipFree(cg->partials);
-  hipFree(cg->partials2);
-  hipFree(cg->st);
-  hipFree(cg->trace);
+  gb_dev_free(cg->r);
+  gb_dev_free(cg->p);
+  gb_dev_free(cg->ap);
+  gb_dev_free(cg->partials);
+  gb_dev_free(cg->partials2);
+  gb_dev_free(cg->st);
+  gb_dev_free(cg->trace);
   if (cg->h_st) hipHostFree(cg->h_st);
   if (cg->h_trace) hipHostFree(cg->h_trace);
   free(cg);
@@ -313,10 +313,10 @@ int gbd_cg_start(gbd_cg *cg, const double *d_b, double *d_x, double tol,
   if (trace_cap != cg->trace_cap) {
     GB_HIP(hipStreamSynchronize(s));
     gb_cg_drop_graph(cg);
-    if (cg->trace) { GB_HIP(hipFree(cg->trace)); cg->trace = nullptr; }
+    if (cg->trace) { GB_HIP(gb_dev_free(cg->trace)); cg->trace = nullptr; }
     if (cg->h_trace) { GB_HIP(hipHostFree(cg->h_trace)); cg->h_trace = nullptr; }
     if (trace_cap) {
-      GB_HIP(hipMalloc((void **)&cg->trace, (size_t)trace_cap * sizeof(double)));
+      GB_HIP(gb_dev_alloc((void **)&cg->trace, (size_t)trace_cap * sizeof(double)));
       GB_HIP(hipHostMalloc((void **)&cg->h_trace, (size_t)trace_cap * sizeof(double),
                            hipHostMallocDefault));
     }

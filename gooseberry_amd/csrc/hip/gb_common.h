@@ -24,6 +24,13 @@ int gb_hip_fail(hipError_t e, const char *what, const char *file, int line);
 hipStream_t gb_stream();
 int gb_ensure_init();
 
+// Device memory from the device's pool, ordered on the library stream: every
+// use of library memory is on that stream, so a block freed here is reused
+// only after the work queued before the free, and freed blocks stay in the
+// pool (no device-wide synchronisation, no trip to the driver per request).
+hipError_t gb_dev_alloc(void **p, size_t bytes);
+hipError_t gb_dev_free(void *p);
+
 static inline uint32_t gb_div_up(uint64_t a, uint64_t b) {
   return (uint32_t)((a + b - 1) / b);
 }

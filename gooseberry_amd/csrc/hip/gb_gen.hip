@@ -190,10 +190,10 @@ int gbd_fill_uniform(double *d_out, size_t n, uint64_t seed, uint64_t stream,
 
 int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
                      uint32_t dmin, uint32_t dmax, uint32_t **d_rowptr,
-                     uint32_t **d_rowind, double **d_rowval) {
+                     uint32_t **d_rowind, double **d_rowval, uint32_t *r_nnz) {
   int rv = gb_ensure_init();
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  if (!d_rowptr || !d_rowind || !d_rowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!d_rowptr || !d_rowind || !d_rowval || !r_nnz) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (n < 2 || n == 0xffffffffu || dmin < 1 || dmax < dmin || dmax > (1u << 20))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (law != GOOSEBERRY_DEGREE_UNIFORM && law != GOOSEBERRY_DEGREE_ZIPF)
@@ -253,13 +253,14 @@ int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
   GB_GEN_TRY(hipMallocAsync((void **)&inptr, ((size_t)n + 1) * 4, s));
   gb_gen_edges<<<grid, GB_BLOCK, 0, s>>>(P, off, src, dst);
   GB_GEN_TRY(hipGetLastError());
-  GB_GEN_RV(gbd_countingsort_kv(dst, nullptr, 0, n - 1, m, ine, inptr));
-  GB_GEN_TRY(hipMalloc((void **)&rowptr, ((size_t)n + 1) * 4));
+  // every dst is in [0, n) by construction
+  GB_GEN_RV(gbd_countingsort_kv_unchecked(dst, nullptr, 0, n - 1, m, ine, inptr));
+  GB_GEN_TRY(gb_dev_alloc((void **)&rowptr, ((size_t)n + 1) * 4));
   gb_gen_rowlen<<<grid, GB_BLOCK, 0, s>>>(n, off, inptr, rowptr);
   GB_GEN_TRY(hipGetLastError());
   GB_GEN_RV(gbd_scan_u32(rowptr, rowptr, n, rowptr + n));
-  GB_GEN_TRY(hipMalloc((void **)&rowind, (size_t)nnz * 4));
-  GB_GEN_TRY(hipMalloc((void **)&rowval, (size_t)nnz * 8));
+  GB_GEN_TRY(gb_dev_alloc((void **)&rowind, (size_t)nnz * 4));
+  GB_GEN_TRY(gb_dev_alloc((void **)&rowval, (size_t)nnz * 8));
   gb_gen_fill_out<<<grid, GB_BLOCK, 0, s>>>(P, off, dst, rowptr, rowind, rowval);
   GB_GEN_TRY(hipGetLastError());
   if (m) {
@@ -271,10 +272,10 @@ int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
   }
   gb_gen_diag<<<grid, GB_BLOCK, 0, s>>>(n, rowptr, rowind, rowval);
   GB_GEN_TRY(hipGetLastError());
-  GB_GEN_TRY(hipStreamSynchronize(s));
   *d_rowptr = rowptr;
   *d_rowind = rowind;
   *d_rowval = rowval;
+  *r_nnz = nnz;
   rowptr = rowind = nullptr;
   rowval = nullptr;
   rv = GOOSEBERRY_SUCCESS;
@@ -285,10 +286,9 @@ done:
   hipFreeAsync(dst, s);
   hipFreeAsync(ine, s);
   hipFreeAsync(inptr, s);
-  hipStreamSynchronize(s);
-  hipFree(rowptr);
-  hipFree(rowind);
-  hipFree(rowval);
+  gb_dev_free(rowptr);
+  gb_dev_free(rowind);
+  gb_dev_free(rowval);
   return rv;
 #undef GB_GEN_TRY
 #undef GB_GEN_RV

@@ -311,16 +311,16 @@ int gbd_scan_u32(const uint32_t *d_in, uint32_t *d_out, size_t n,
   return gb_scan_u32(d_in, d_out, n, d_total);
 }
 
-int gbd_countingsort_kv(const uint32_t *d_keys, const uint32_t *d_vals,
-                        uint32_t kmin, uint32_t kmax, size_t n,
-                        uint32_t *d_out, uint32_t *d_dist) {
+static int gb_countingsort(const uint32_t *d_keys, const uint32_t *d_vals,
+                           uint32_t kmin, uint32_t kmax, size_t n,
+                           uint32_t *d_out, uint32_t *d_dist, bool check) {
   int rv = gb_ensure_init();
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   if (kmax < kmin || n > 0xffffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
   hipStream_t s = gb_stream();
   const uint64_t range = (uint64_t)kmax - kmin + 1;
   // keys outside [kmin, kmax] would index out of bounds: reject them
-  if (n) {
+  if (n && check) {
     uint32_t *bad = nullptr, hbad = 0;
     GB_HIP(hipMallocAsync((void **)&bad, sizeof(uint32_t), s));
     GB_HIP(hipMemsetAsync(bad, 0, sizeof(uint32_t), s));
@@ -385,6 +385,18 @@ int gbd_countingsort_kv(const uint32_t *d_keys, const uint32_t *d_vals,
   GB_HIP(hipFreeAsync(tv[0], s));
   if (tv[1]) GB_HIP(hipFreeAsync(tv[1], s));
   return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_countingsort_kv(const uint32_t *d_keys, const uint32_t *d_vals,
+                        uint32_t kmin, uint32_t kmax, size_t n,
+                        uint32_t *d_out, uint32_t *d_dist) {
+  return gb_countingsort(d_keys, d_vals, kmin, kmax, n, d_out, d_dist, true);
+}
+
+int gbd_countingsort_kv_unchecked(const uint32_t *d_keys, const uint32_t *d_vals,
+                                  uint32_t kmin, uint32_t kmax, size_t n,
+                                  uint32_t *d_out, uint32_t *d_dist) {
+  return gb_countingsort(d_keys, d_vals, kmin, kmax, n, d_out, d_dist, false);
 }
 
 int gbd_permute_sparse(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,

@@ -26,6 +26,41 @@ struct gbd_matrix {
   uint64_t hist[33];
 };
 
+// One lane's share of a row: the products val[k] * x[rowind[k]] for
+// k = k0 + lane, k0 + lane + STRIDE, ... added in k order from +0.0.  Four
+// strides of index and value loads, then their four gathers, are issued
+// before the products are added, so a lane keeps four gathers in flight;
+// the additions are those of the plain strided loop, in the same order.
+template <typename T, uint32_t STRIDE>
+__device__ __forceinline__ T gb_row_lane_sum(const uint32_t *__restrict__ rowind,
+                                             const T *__restrict__ val,
+                                             const T *__restrict__ x, uint32_t k0,
+                                             uint32_t k1, uint32_t lane) {
+  T s = 0;
+  uint32_t k = k0 + lane;
+  for (; (uint64_t)k + 3 * STRIDE < k1; k += 4 * STRIDE) {
+    const uint32_t c0 = __builtin_nontemporal_load(&rowind[k]);
+    const uint32_t c1 = __builtin_nontemporal_load(&rowind[k + STRIDE]);
+    const uint32_t c2 = __builtin_nontemporal_load(&rowind[k + 2 * STRIDE]);
+    const uint32_t c3 = __builtin_nontemporal_load(&rowind[k + 3 * STRIDE]);
+    const T v0 = __builtin_nontemporal_load(&val[k]);
+    const T v1 = __builtin_nontemporal_load(&val[k + STRIDE]);
+    const T v2 = __builtin_nontemporal_load(&val[k + 2 * STRIDE]);
+    const T v3 = __builtin_nontemporal_load(&val[k + 3 * STRIDE]);
+    const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    s += v0 * x0;
+    s += v1 * x1;
+    s += v2 * x2;
+    s += v3 * x3;
+  }
+  for (; k < k1; k += STRIDE) {
+    const uint32_t c = __builtin_nontemporal_load(&rowind[k]);
+    const T v = __builtin_nontemporal_load(&val[k]);
+    s += v * x[c];
+  }
+  return s;
+}
+
 // Launch geometry of one SpMV: stream blocks, then blocks of 4 wave rows,
 // then one block per long row.
 static inline uint32_t gb_spmv_nwaveblocks(const gbd_matrix *m) {

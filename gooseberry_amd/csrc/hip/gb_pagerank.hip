@@ -1,0 +1,259 @@
+// gb_pagerank.hip -- PageRank power iteration over the resident SpMV
+// (src/gooseberry_bin.c:2622-2737).
+//
+// Preparation, once per call:
+//   column sums  indeg[c] = the values of column c added in row order (a
+//                stable counting sort of the entries by column, then one
+//                sequential sum per column: the reference's order)
+//   normalise    nval[l] = val[l] / indeg[rowind[l]] (IEEE division)
+//   sinks        the rows without entries, ascending (scan + compaction)
+// One iteration, three launches:
+//   SpMV         out = N rank (the matrix's own plan over nval)
+//   gb_pr_shift  c = (1 - d) / n + sum of rank[sinks] (the sink ranks are
+//                added in sink order, as the reference does, up to
+//                GB_PR_SEQ_SINKS sinks)
+//   gb_pr_update out = out * d + c (scale, then add, separately rounded)
+// and every tenth iteration the squared difference of rank and out
+// (fixed-order two-level reduction) for the stopping test.
+#include <math.h>
+
+#include "gb_matrix.h"
+
+// Sink ranks added one at a time in sink order up to this many sinks; above
+// it, lane-strided sums and a butterfly (fixed order, not the reference's).
+#define GB_PR_SEQ_SINKS 65536u
+#define GB_PR_ERR_GRID 1024u
+// niter == 0 iterates until the error test passes; the reference has no
+// bound, this build stops after this many iterations.
+#define GB_PR_MAX_ITER 1000000ull
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_colsum(uint32_t ncols, const uint32_t *__restrict__ dist,
+             const uint32_t *__restrict__ perm, const double *__restrict__ val,
+             double *__restrict__ indeg) {
+  const uint32_t c = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (c >= ncols) return;
+  double s = 0.0;
+  for (uint32_t p = dist[c]; p < dist[c + 1]; ++p) s += val[perm[p]];
+  indeg[c] = s;
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_normalise(uint64_t nnz, const uint32_t *__restrict__ rowind,
+                const double *__restrict__ val, const double *__restrict__ indeg,
+                double *__restrict__ nval) {
+  for (uint64_t l = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; l < nnz;
+       l += (uint64_t)gridDim.x * GB_BLOCK)
+    nval[l] = val[l] / indeg[rowind[l]];
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_sink_flags(uint32_t n, const uint32_t *__restrict__ rowptr,
+                 uint32_t *__restrict__ flag) {
+  const uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (k < n) flag[k] = rowptr[k] == rowptr[k + 1] ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_sink_compact(uint32_t n, const uint32_t *__restrict__ rowptr,
+                   const uint32_t *__restrict__ idx, uint32_t *__restrict__ sinks) {
+  const uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (k < n && rowptr[k] == rowptr[k + 1]) sinks[idx[k]] = k;
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_fill(uint32_t n, double v, double *__restrict__ x) {
+  for (uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x; k < n; k += gridDim.x * GB_BLOCK)
+    x[k] = v;
+}
+
+// one wavefront: c = ((1 - d) / n) + dist
+__global__ void __launch_bounds__(GB_WAVE)
+gb_pr_shift(const double *__restrict__ rank, const uint32_t *__restrict__ sinks,
+            uint32_t nsinks, double damping, uint32_t n, double *__restrict__ c) {
+  const uint32_t lane = threadIdx.x;
+  double dist = 0;
+  if (nsinks <= GB_PR_SEQ_SINKS) {
+    // 64 ranks loaded at once, added in sink order by every lane
+    for (uint32_t base = 0; base < nsinks; base += GB_WAVE) {
+      const uint32_t k = base + lane;
+      const double v = k < nsinks ? rank[sinks[k]] : 0.0;
+      const uint32_t cnt = min((uint32_t)GB_WAVE, nsinks - base);
+      for (uint32_t j = 0; j < cnt; ++j) dist += gb_readlane(v, j);
+    }
+  } else {
+    for (uint32_t k = lane; k < nsinks; k += GB_WAVE) dist += rank[sinks[k]];
+    dist = gb_wave_sum(dist);
+  }
+  if (lane == 0) c[0] = ((1.0 - damping) / n) + dist;
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_update(uint32_t n, double *__restrict__ out, double damping,
+             const double *__restrict__ c) {
+  const double cc = c[0];
+  for (uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x; k < n; k += gridDim.x * GB_BLOCK) {
+    const double t = out[k] * damping;  // gooseberry_scale
+    out[k] = t + cc;                    // gooseberry_add_scalar
+  }
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_err_partials(uint32_t n, const double *__restrict__ a,
+                   const double *__restrict__ b, double *__restrict__ part) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  double e = 0.0;
+  for (uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x; k < n; k += gridDim.x * GB_BLOCK) {
+    const double d = a[k] - b[k];
+    e += d * d;
+  }
+  e = gb_block_sum(e, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = e;
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_err_final(const double *__restrict__ part, uint32_t np, double *__restrict__ err) {
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  double e = 0.0;
+  for (uint32_t i = threadIdx.x; i < np; i += GB_BLOCK) e += part[i];
+  e = gb_block_sum(e, red);
+  if (threadIdx.x == 0) err[0] = e;
+}
+
+static uint32_t gb_pr_grid(uint64_t n) {
+  uint64_t g = (n + GB_BLOCK - 1) / GB_BLOCK;
+  if (g > 65536) g = 65536;
+  return g ? (uint32_t)g : 1u;
+}
+
+extern "C" {
+
+int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
+                 double minerror, double *d_out, uint64_t *r_iters,
+                 double *r_rmse, uint32_t *r_nsinks) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!m || !d_out || m->nrows != m->ncols || m->nrows == 0)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  hipStream_t s = gb_stream();
+  const uint32_t n = m->nrows;
+  const uint64_t nnz = m->nnz;
+  const uint32_t g = gb_pr_grid(n);
+  double *indeg = nullptr, *nval = nullptr, *rank = nullptr, *out = nullptr,
+         *c = nullptr, *part = nullptr, *err = nullptr;
+  uint32_t *perm = nullptr, *dist = nullptr, *flag = nullptr, *idx = nullptr,
+           *sinks = nullptr;
+  uint32_t nsinks = 0;
+  uint64_t i = 0;
+  double e = 0.0;
+  bool finished = false;
+  gbd_matrix mm;
+  hipError_t he = hipSuccess;
+#define GB_PR_TRY(call)                                         \
+  do {                                                          \
+    he = (call);                                                \
+    if (he != hipSuccess) {                                     \
+      rv = gb_hip_fail(he, #call, __FILE__, __LINE__);          \
+      goto done;                                                \
+    }                                                           \
+  } while (0)
+#define GB_PR_RV(call)                                          \
+  do {                                                          \
+    rv = (call);                                                \
+    if (rv != GOOSEBERRY_SUCCESS) goto done;                    \
+  } while (0)
+
+  GB_PR_TRY(gb_dev_alloc((void **)&indeg, (size_t)n * 8));
+  GB_PR_TRY(gb_dev_alloc((void **)&nval, (size_t)(nnz ? nnz : 1) * 8));
+  GB_PR_TRY(gb_dev_alloc((void **)&perm, (size_t)(nnz ? nnz : 1) * 4));
+  GB_PR_TRY(gb_dev_alloc((void **)&dist, ((size_t)n + 1) * 4));
+  GB_PR_TRY(gb_dev_alloc((void **)&flag, (size_t)n * 4));
+  GB_PR_TRY(gb_dev_alloc((void **)&idx, ((size_t)n + 1) * 4));
+  GB_PR_TRY(gb_dev_alloc((void **)&sinks, (size_t)n * 4));
+  GB_PR_TRY(gb_dev_alloc((void **)&rank, (size_t)n * 8));
+  GB_PR_TRY(gb_dev_alloc((void **)&out, (size_t)n * 8));
+  GB_PR_TRY(gb_dev_alloc((void **)&c, 8));
+  GB_PR_TRY(gb_dev_alloc((void **)&part, GB_PR_ERR_GRID * 8));
+  GB_PR_TRY(gb_dev_alloc((void **)&err, 8));
+
+  // column sums in row order (column indices were validated with the matrix)
+  GB_PR_RV(gbd_countingsort_kv_unchecked(m->rowind, nullptr, 0, n - 1, nnz, perm, dist));
+  gb_pr_colsum<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm, m->val, indeg);
+  GB_PR_TRY(hipGetLastError());
+  if (nnz) {
+    gb_pr_normalise<<<gb_pr_grid(nnz), GB_BLOCK, 0, s>>>(nnz, m->rowind, m->val, indeg,
+                                                         nval);
+    GB_PR_TRY(hipGetLastError());
+  }
+  // sinks
+  gb_pr_sink_flags<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(n, m->rowptr, flag);
+  GB_PR_TRY(hipGetLastError());
+  GB_PR_RV(gbd_scan_u32(flag, idx, n, idx + n));
+  gb_pr_sink_compact<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(n, m->rowptr, idx, sinks);
+  GB_PR_TRY(hipGetLastError());
+  GB_PR_TRY(hipMemcpyAsync(&nsinks, idx + n, 4, hipMemcpyDeviceToHost, s));
+  GB_PR_TRY(hipStreamSynchronize(s));
+
+  // the normalised matrix: the same structure and plan over nval
+  mm = *m;
+  mm.val = nval;
+  mm.val32 = nullptr;
+  gb_pr_fill<<<g, GB_BLOCK, 0, s>>>(n, 1.0 / n, rank);
+  GB_PR_TRY(hipGetLastError());
+  for (i = 0; (niter == 0 && i < GB_PR_MAX_ITER) || i < niter; ++i) {
+    GB_PR_RV(gbd_spmv(&mm, rank, out));
+    gb_pr_shift<<<1, GB_WAVE, 0, s>>>(rank, sinks, nsinks, damping, n, c);
+    GB_PR_TRY(hipGetLastError());
+    gb_pr_update<<<g, GB_BLOCK, 0, s>>>(n, out, damping, c);
+    GB_PR_TRY(hipGetLastError());
+    if (i % 10 == 0) {
+      gb_pr_err_partials<<<GB_PR_ERR_GRID, GB_BLOCK, 0, s>>>(n, rank, out, part);
+      GB_PR_TRY(hipGetLastError());
+      gb_pr_err_final<<<1, GB_BLOCK, 0, s>>>(part, GB_PR_ERR_GRID, err);
+      GB_PR_TRY(hipGetLastError());
+      GB_PR_TRY(hipMemcpyAsync(&e, err, 8, hipMemcpyDeviceToHost, s));
+      GB_PR_TRY(hipStreamSynchronize(s));
+      if (e <= minerror * minerror) {
+        ++i;
+        finished = true;
+        break;
+      }
+    }
+    double *t = rank;
+    rank = out;
+    out = t;
+  }
+  if (!finished) {
+    gb_pr_err_partials<<<GB_PR_ERR_GRID, GB_BLOCK, 0, s>>>(n, rank, out, part);
+    GB_PR_TRY(hipGetLastError());
+    gb_pr_err_final<<<1, GB_BLOCK, 0, s>>>(part, GB_PR_ERR_GRID, err);
+    GB_PR_TRY(hipGetLastError());
+    GB_PR_TRY(hipMemcpyAsync(&e, err, 8, hipMemcpyDeviceToHost, s));
+  }
+  // the reference writes its `out` buffer: after a completed loop that is
+  // the iterate before the last one
+  GB_PR_TRY(hipMemcpyAsync(d_out, out, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  GB_PR_TRY(hipStreamSynchronize(s));
+  if (r_iters) *r_iters = i;
+  if (r_rmse) *r_rmse = sqrt(e);
+  if (r_nsinks) *r_nsinks = nsinks;
+  rv = GOOSEBERRY_SUCCESS;
+done:
+  gb_dev_free(indeg);
+  gb_dev_free(nval);
+  gb_dev_free(perm);
+  gb_dev_free(dist);
+  gb_dev_free(flag);
+  gb_dev_free(idx);
+  gb_dev_free(sinks);
+  gb_dev_free(rank);
+  gb_dev_free(out);
+  gb_dev_free(c);
+  gb_dev_free(part);
+  gb_dev_free(err);
+  return rv;
+#undef GB_PR_TRY
+#undef GB_PR_RV
+}
+
+}  // extern "C"

@@ -38,6 +38,28 @@ static void gb_do_init() {
     g_init_rv = gb_hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     return;
   }
+  // keep freed blocks in the pool instead of returning them at every
+  // synchronisation
+  hipMemPool_t pool;
+  e = hipDeviceGetDefaultMemPool(&pool, 0);
+  if (e == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  if (e != hipSuccess) {
+    g_init_rv = gb_hip_fail(e, "default memory pool", __FILE__, __LINE__);
+    return;
+  }
+}
+
+hipError_t gb_dev_alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  return hipMallocAsync(p, bytes ? bytes : 16, g_stream);
+}
+
+hipError_t gb_dev_free(void *p) {
+  if (!p) return hipSuccess;
+  return hipFreeAsync(p, g_stream);
 }
 
 int gb_ensure_init() {
@@ -64,7 +86,24 @@ __global__ void gb_scale_kernel(double *__restrict__ d, size_t n, double s) {
   for (; i < n; i += stride) d[i] *= s;
 }
 
+__global__ void gb_add_scalar_kernel(double *__restrict__ d, size_t n, double s) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] += s;
+}
+
 extern "C" {
+
+int gbd_add_scalar(double *d, size_t n, double s) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (!n) return GOOSEBERRY_SUCCESS;
+  uint32_t grid = gb_div_up(n, GB_BLOCK);
+  if (grid > 65536u) grid = 65536u;
+  gb_add_scalar_kernel<<<grid, GB_BLOCK, 0, g_stream>>>(d, n, s);
+  GB_LAUNCH_CHECK();
+  return GOOSEBERRY_SUCCESS;
+}
 
 int gbd_scale(double *d, size_t n, double s) {
   int rv = gb_ensure_init();
@@ -90,13 +129,13 @@ int gbd_malloc(void **d_ptr, size_t bytes) {
   int rv = gb_ensure_init();
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   *d_ptr = nullptr;
-  GB_HIP(hipMalloc(d_ptr, bytes ? bytes : 16));
+  GB_HIP(gb_dev_alloc(d_ptr, bytes ? bytes : 16));
   return GOOSEBERRY_SUCCESS;
 }
 
 int gbd_free(void *d_ptr) {
   if (!d_ptr) return GOOSEBERRY_SUCCESS;
-  GB_HIP(hipFree(d_ptr));
+  GB_HIP(gb_dev_free(d_ptr));
   return GOOSEBERRY_SUCCESS;
 }
 

@@ -231,12 +231,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
             for (uint32_t j = 0; j < cnt; ++j) s += gb_readlane(pr, j);
           }
         } else {
-          for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE) {
-            uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-            T v = __builtin_nontemporal_load(&val[k]);
-            s += v * x[c];
-          }
-          s = gb_wave_sum(s);
+          s = gb_wave_sum(gb_row_lane_sum<T, GB_WAVE>(rowind, val, x, k0, k1, lane));
         }
         if (lane == 0) {
           y[(size_t)row * ystride] = s;
@@ -268,13 +263,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
           if (DOT) dotp += (double)x[row] * (double)s;
         }
       } else {
-        T s = 0;
-        for (uint32_t k = k0 + tid; k < k1; k += GB_BLOCK) {
-          uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-          T v = __builtin_nontemporal_load(&val[k]);
-          s += v * x[c];
-        }
-        s = gb_wave_sum(s);
+        T s = gb_wave_sum(gb_row_lane_sum<T, GB_BLOCK>(rowind, val, x, k0, k1, tid));
         if (lane == 0) tred[wave] = s;
         __syncthreads();
         if (tid == 0) {
@@ -365,8 +354,8 @@ static int gb_plan_build(gbd_matrix *m) {
   const uint32_t n = m->nrows;
   memset(m->hist, 0, sizeof(m->hist));
   m->nstream = m->nwave = m->nblockrows = 0;
-  if (m->blk) { GB_HIP(hipFree(m->blk)); m->blk = nullptr; }
-  if (m->rowbins) { GB_HIP(hipFree(m->rowbins)); m->rowbins = nullptr; }
+  if (m->blk) { GB_HIP(gb_dev_free(m->blk)); m->blk = nullptr; }
+  if (m->rowbins) { GB_HIP(gb_dev_free(m->rowbins)); m->rowbins = nullptr; }
   if (n == 0) return GOOSEBERRY_SUCCESS;
 
   uint32_t *cls = nullptr, *flag = nullptr, *idx = nullptr, *dist = nullptr;
@@ -380,8 +369,8 @@ static int gb_plan_build(gbd_matrix *m) {
   gb_plan_classify<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(
       m->rowptr, n, m->params, cls, flag, hist);
   GB_LAUNCH_CHECK();
-  GB_HIP(hipMalloc((void **)&m->rowbins, (size_t)n * 4));
-  int rv = gbd_countingsort_kv(cls, nullptr, 0, 2, n, m->rowbins, dist);
+  GB_HIP(gb_dev_alloc((void **)&m->rowbins, (size_t)n * 4));
+  int rv = gbd_countingsort_kv_unchecked(cls, nullptr, 0, 2, n, m->rowbins, dist);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   rv = gbd_scan_u32(flag, idx, n, idx + n);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
@@ -396,7 +385,7 @@ static int gb_plan_build(gbd_matrix *m) {
   m->nwave = hd[2] - hd[1];
   m->nblockrows = hd[3] - hd[2];
   if (nst) {
-    GB_HIP(hipMalloc((void **)&m->blk, (size_t)nst * 2 * 4));
+    GB_HIP(gb_dev_alloc((void **)&m->blk, (size_t)nst * 2 * 4));
     gb_plan_blocks<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(cls, flag, idx,
                                                               n, m->blk);
     GB_LAUNCH_CHECK();
@@ -406,7 +395,6 @@ static int gb_plan_build(gbd_matrix *m) {
   GB_HIP(hipFreeAsync(idx, s));
   GB_HIP(hipFreeAsync(dist, s));
   GB_HIP(hipFreeAsync(hist, s));
-  GB_HIP(hipStreamSynchronize(s));
   return GOOSEBERRY_SUCCESS;
 }
 
@@ -426,26 +414,18 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->xcd_remap = 0;
 }
 
-int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
-                      uint32_t *d_rowind, double *d_rowval,
-                      const gbd_plan_params *params, gbd_matrix **out) {
+static int gb_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
+                            uint32_t *d_rowind, double *d_rowval, bool trusted,
+                            uint32_t nnz, const gbd_plan_params *params,
+                            gbd_matrix **out) {
   int rv = gb_ensure_init();
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  gbd_matrix *m = (gbd_matrix *)calloc(1, sizeof(gbd_matrix));
-  if (!m) return GOOSEBERRY_ERROR_NOMEMORY;
-  m->nrows = nrows;
-  m->ncols = ncols;
-  m->rowptr = d_rowptr;
-  m->rowind = d_rowind;
-  m->val = d_rowval;
-  uint32_t nnz = 0;
-  GB_HIP(hipMemcpyAsync(&nnz, d_rowptr + nrows, 4, hipMemcpyDeviceToHost,
-                        gb_stream()));
-  GB_HIP(hipStreamSynchronize(gb_stream()));
-  m->nnz = nnz;
-  {
+  hipStream_t s = gb_stream();
+  if (!trusted) {
+    // nnz from the device, then every row pointer and column index checked
     uint32_t *bad = nullptr, hbad = 0;
-    hipStream_t s = gb_stream();
+    GB_HIP(hipMemcpyAsync(&nnz, d_rowptr + nrows, 4, hipMemcpyDeviceToHost, s));
+    GB_HIP(hipStreamSynchronize(s));
     GB_HIP(hipMallocAsync((void **)&bad, 4, s));
     GB_HIP(hipMemsetAsync(bad, 0, 4, s));
     uint64_t work = (uint64_t)nrows > nnz ? nrows : nnz;
@@ -456,28 +436,44 @@ int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
     GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
     GB_HIP(hipFreeAsync(bad, s));
     GB_HIP(hipStreamSynchronize(s));
-    if (hbad) {
-      free(m);
-      return GOOSEBERRY_ERROR_INVALIDINPUT;
-    }
+    if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
-  if (params) {
-    m->params = *params;
-  } else {
-    gbd_plan_defaults(nullptr, nrows, nnz, &m->params);
-  }
-  if (!gb_plan_check(&m->params)) {
-    free(m);
-    return GOOSEBERRY_ERROR_INVALIDINPUT;
-  }
+  gbd_plan_params p;
+  if (params) p = *params;
+  else gbd_plan_defaults(nullptr, nrows, nnz, &p);
+  if (!gb_plan_check(&p)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_matrix *m = (gbd_matrix *)calloc(1, sizeof(gbd_matrix));
+  if (!m) return GOOSEBERRY_ERROR_NOMEMORY;
+  m->nrows = nrows;
+  m->ncols = ncols;
+  m->rowptr = d_rowptr;
+  m->rowind = d_rowind;
+  m->val = d_rowval;
+  m->nnz = nnz;
+  m->params = p;
   rv = gb_plan_build(m);
   if (rv != GOOSEBERRY_SUCCESS) {
+    // the caller keeps the arrays it passed in
     m->rowptr = nullptr; m->rowind = nullptr; m->val = nullptr;
     gbd_matrix_destroy(m);
     return rv;
   }
   *out = m;
   return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
+                      uint32_t *d_rowind, double *d_rowval,
+                      const gbd_plan_params *params, gbd_matrix **out) {
+  return gb_matrix_create(nrows, ncols, d_rowptr, d_rowind, d_rowval, false, 0,
+                          params, out);
+}
+
+int gbd_matrix_create_trusted(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
+                              uint32_t *d_rowind, double *d_rowval, uint32_t nnz,
+                              const gbd_plan_params *params, gbd_matrix **out) {
+  return gb_matrix_create(nrows, ncols, d_rowptr, d_rowind, d_rowval, true, nnz,
+                          params, out);
 }
 
 int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
@@ -491,13 +487,13 @@ int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
 
 void gbd_matrix_destroy(gbd_matrix *m) {
   if (!m) return;
-  hipStreamSynchronize(gb_stream());
-  if (m->rowptr) hipFree(m->rowptr);
-  if (m->rowind) hipFree(m->rowind);
-  if (m->val) hipFree(m->val);
-  if (m->val32) hipFree(m->val32);
-  if (m->blk) hipFree(m->blk);
-  if (m->rowbins) hipFree(m->rowbins);
+  // pool frees are ordered after the queued work that reads the arrays
+  if (m->rowptr) gb_dev_free(m->rowptr);
+  if (m->rowind) gb_dev_free(m->rowind);
+  if (m->val) gb_dev_free(m->val);
+  if (m->val32) gb_dev_free(m->val32);
+  if (m->blk) gb_dev_free(m->blk);
+  if (m->rowbins) gb_dev_free(m->rowbins);
   free(m);
 }
 
@@ -522,7 +518,7 @@ const double *gbd_matrix_rowval(const gbd_matrix *m) { return m->val; }
 
 int gbd_matrix_make_f32(gbd_matrix *m) {
   if (m->val32) return GOOSEBERRY_SUCCESS;
-  GB_HIP(hipMalloc((void **)&m->val32, (m->nnz ? m->nnz : 1) * sizeof(float)));
+  GB_HIP(gb_dev_alloc((void **)&m->val32, (m->nnz ? m->nnz : 1) * sizeof(float)));
   if (m->nnz) {
     gb_to_f32<<<gb_div_up(m->nnz, GB_BLOCK), GB_BLOCK, 0, gb_stream()>>>(
         m->val, m->val32, m->nnz);

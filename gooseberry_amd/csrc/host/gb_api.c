@@ -105,6 +105,18 @@ int gooseberry_scale(ind_t n, real_t *val, real_t s)
   return rv;
 }
 
+int gooseberry_add_scalar(ind_t n, real_t *val, real_t s)
+{
+  if (n == 0) return GOOSEBERRY_SUCCESS;
+  if (!val) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *d = NULL;
+  int rv = gb_upload(&d, val, (size_t)n * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_add_scalar(d, n, s);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(val, d, (size_t)n * sizeof(real_t));
+  gbd_free(d);
+  return rv;
+}
+
 /* ------------------------------------------------------------ matrices */
 
 static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
@@ -155,18 +167,19 @@ static int wrap_matrix(gbd_matrix *m, gooseberry_matrix_t *r_mat)
   return GOOSEBERRY_SUCCESS;
 }
 
-/* wrap device arrays the library allocated itself (no copy); the arrays
- * are freed here when the matrix cannot be made */
+/* wrap a CSR the library built itself on the device (valid by
+ * construction, nnz known; no copy); the arrays are freed here when the
+ * matrix cannot be made */
 int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
-    dim_t *d_rowind, real_t *d_rowval, const gooseberry_plan_t *plan,
-    gooseberry_matrix_t *r_mat)
+    dim_t *d_rowind, real_t *d_rowval, ind_t nnz,
+    const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
 {
   gbd_plan_params pp;
   int have;
   plan_to_dev(plan, NULL, &pp, &have);
   gbd_matrix *m = NULL;
-  int rv = gbd_matrix_create(nrows, ncols, d_rowptr, d_rowind, d_rowval,
-      have ? &pp : NULL, &m);
+  int rv = gbd_matrix_create_trusted(nrows, ncols, d_rowptr, d_rowind,
+      d_rowval, nnz, have ? &pp : NULL, &m);
   if (rv != GOOSEBERRY_SUCCESS) {
     gbd_free(d_rowptr); gbd_free(d_rowind); gbd_free(d_rowval);
     return rv;
@@ -537,5 +550,41 @@ int gooseberry_permute_dense(dim_t nrows, dim_t ncols, real_t *rowval,
     rv = gbd_permute_dense(nrows, ncols, din, dout, drp, dcp, rev ? 1 : 0);
   if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(rowval, dout, tot * sizeof(real_t));
   gbd_free(din); gbd_free(dout); gbd_free(drp); gbd_free(dcp);
+  return rv;
+}
+
+/* -------------------------------------------------------------- PageRank */
+
+int gooseberry_matrix_pagerank(gooseberry_matrix_t mat, real_t damping,
+    size_t niter, real_t minerror, real_t *d_out,
+    gooseberry_pagerank_info_t *info)
+{
+  if (!mat || !d_out) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint64_t it = 0;
+  double rmse = 0;
+  uint32_t ns = 0;
+  int rv = gbd_pagerank(mat->m, damping, niter, minerror, d_out, &it, &rmse, &ns);
+  if (rv == GOOSEBERRY_SUCCESS && info) {
+    info->niter = it;
+    info->rmse = rmse;
+    info->nsinks = ns;
+  }
+  return rv;
+}
+
+int gooseberry_pagerank(dim_t n, const ind_t *rowptr, const dim_t *rowind,
+    const real_t *rowval, real_t damping, size_t niter, real_t minerror,
+    real_t *out, gooseberry_pagerank_info_t *info)
+{
+  if (!out || n == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gooseberry_matrix_t m = NULL;
+  void *d = NULL;
+  int rv = gooseberry_matrix_create(n, n, rowptr, rowind, rowval, NULL, &m);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&d, (size_t)n * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gooseberry_matrix_pagerank(m, damping, niter, minerror, d, info);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(out, d, (size_t)n * sizeof(real_t));
+  gbd_free(d);
+  gooseberry_matrix_free(m);
   return rv;
 }

@@ -170,11 +170,11 @@ int gooseberry_matrix_generate_spd(dim_t n, uint64_t seed, int law,
     gooseberry_matrix_t *r_mat)
 {
   if (!r_mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  uint32_t *rp = NULL, *ri = NULL;
+  uint32_t *rp = NULL, *ri = NULL, nnz = 0;
   double *rv_ = NULL;
-  int rv = gbd_generate_spd(n, seed, law, zipf_s, dmin, dmax, &rp, &ri, &rv_);
+  int rv = gbd_generate_spd(n, seed, law, zipf_s, dmin, dmax, &rp, &ri, &rv_, &nnz);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  return gb_matrix_adopt(n, n, rp, ri, rv_, plan, r_mat);
+  return gb_matrix_adopt(n, n, rp, ri, rv_, nnz, plan, r_mat);
 }
 
 int gooseberry_matrix_generate_sdd(dim_t n, uint64_t seed, dim_t dmin,

@@ -553,6 +553,171 @@ END:
   return err;
 }
 
+/* ------------------------------------------------------------ pagerank */
+
+/* src/gooseberry_bin.c:421-444 */
+enum { PR_HELP, PR_IN, PR_OUT, PR_TIME, PR_RUNS, PR_NITER, PR_ERROR, PR_DAMPING,
+       PR_THREADS, PR_PERM };
+static const opt_t PR_OPTS[] = {
+  {PR_HELP, 'h', "help", "Display this help page.", OPT_FLAG, NULL},
+  {PR_IN, 'i', "infile", "An input matrix file.", OPT_STRING, NULL},
+  {PR_OUT, 'o', "outfile", "The output matrix/vector file.", OPT_STRING, NULL},
+  {PR_TIME, 't', "times", "Print timing of the pagerank calcuation.", OPT_FLAG, NULL},
+  {PR_RUNS, 'r', "runs", "Number of repeated runs (only useful for timing purposes).",
+      OPT_INT, NULL},
+  {PR_NITER, 'I', "iterations", "Maximum number of iterations.", OPT_INT, NULL},
+  {PR_ERROR, 'e', "error", "Error threshold for stopping.", OPT_FLOAT, NULL},
+  {PR_DAMPING, 'd', "damping", "Damping factor to use.", OPT_FLOAT, NULL},
+  {PR_THREADS, 'T', "threads", "Number of threads.", OPT_INT, NULL},
+  {PR_PERM, 'p', "perm", "Row and column permutation file.", OPT_STRING, NULL},
+};
+static const size_t NPR_OPTS = sizeof(PR_OPTS) / sizeof(PR_OPTS[0]);
+
+/* src/gooseberry_bin.c:2434-2815 */
+static int cmd_pagerank(int argc, char **argv)
+{
+  arg_t args[512];
+  size_t nargs = 0;
+  if (argc - 2 > 512) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!parse_args(argc - 2, argv + 2, PR_OPTS, NPR_OPTS, args, &nargs))
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int err = GOOSEBERRY_SUCCESS, times = 0;
+  double minerror = 0.0, damping = 0.85;
+  long long niter = 0, runs = 1;
+  const char *matfile = NULL, *outfile = NULL, *pf = NULL;
+  mat_t mat;
+  memset(&mat, 0, sizeof(mat));
+  dim_t *perm = NULL;
+  real_t *out = NULL;
+  void *d_out = NULL;
+  gooseberry_matrix_t dmat = NULL;
+  gooseberry_pagerank_info_t info;
+  memset(&info, 0, sizeof(info));
+  tmr_t io = {0}, pre = {0}, op = {0};
+
+  if (nargs < 2) {
+    command_usage(argv[0], argv[1], PR_OPTS, NPR_OPTS, stderr);
+    return GOOSEBERRY_SUCCESS;
+  }
+  for (size_t i = 0; i < nargs; ++i) {
+    switch (args[i].id) {
+      case PR_HELP:
+        command_usage(argv[0], argv[1], PR_OPTS, NPR_OPTS, stdout);
+        return GOOSEBERRY_SUCCESS;
+      case PR_IN:
+        if (!matfile) matfile = args[i].s;
+        else {
+          fprintf(stderr, "Too many input files specified\n");
+          err = GOOSEBERRY_ERROR_INVALIDINPUT;
+        }
+        break;
+      case PR_OUT: outfile = args[i].s; break;
+      case PR_TIME: times = 1; break;
+      case PR_PERM: pf = args[i].s; break;
+      case PR_RUNS: runs = args[i].i; break;
+      case PR_NITER: niter = args[i].i; break;
+      case PR_DAMPING: damping = args[i].f; break;
+      case PR_ERROR: minerror = args[i].f; break;
+      case PR_THREADS: break;
+    }
+  }
+  if (!matfile) {
+    fprintf(stderr, "You must specify a matrix input file.\n");
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (!outfile) {
+    fprintf(stderr, "You must specify an output vector file.\n");
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (err == GOOSEBERRY_SUCCESS && (niter < 0 || runs < 0))
+    err = GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (err != GOOSEBERRY_SUCCESS) return err;
+
+  if (times) tmr_start(&io);
+  if ((err = read_input(matfile, &mat)) != GOOSEBERRY_SUCCESS) goto END;
+  if (mat.dense) {
+    /* the reference also accepts dense matrices; this build runs the
+     * sparse path only */
+    fprintf(stderr, "pagerank: dense input matrices are not supported\n");
+    err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    goto END;
+  }
+  /* src/gooseberry_bin.c:2561-2577: a non-square sparse matrix is squared */
+  if (mat.nrows > mat.ncols) {
+    mat.ncols = mat.nrows;
+  } else if (mat.ncols > mat.nrows) {
+    ind_t *rp = realloc(mat.rowptr, sizeof(ind_t) * ((size_t)mat.ncols + 1));
+    if (!rp) { err = GOOSEBERRY_ERROR_UNKNOWN; goto END; }
+    for (dim_t k = mat.nrows + 1; k <= mat.ncols; ++k) rp[k] = rp[mat.nrows];
+    mat.rowptr = rp;
+    mat.nrows = mat.ncols;
+  }
+  if (pf && (err = labels_to_perm(pf, mat.nrows, &perm)) != GOOSEBERRY_SUCCESS)
+    goto END;
+  if (times) {
+    tmr_stop(&io);
+    tmr_start(&pre);
+  }
+  const dim_t n = mat.nrows;
+  if (perm) {
+    err = gooseberry_permute_sparse(n, n, mat.rowptr, mat.rowind, mat.rowval, perm, perm);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  out = calloc(n ? n : 1, sizeof(real_t));
+  err = gooseberry_matrix_create(n, n, mat.rowptr, mat.rowind, mat.rowval, NULL, &dmat);
+  if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_out, (n ? n : 1) * sizeof(real_t));
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) {
+    tmr_stop(&pre);
+    tmr_start(&op);
+  }
+  for (long long r = 0; r < runs; ++r) {
+    err = gooseberry_matrix_pagerank(dmat, damping, (size_t)niter, minerror, d_out, &info);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  err = gooseberry_memcpy_dtoh(out, d_out, n * sizeof(real_t));
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  printf("PageRank finished in %zu iterations with an RMSE of %lf\n",
+      (size_t)info.niter, info.rmse);
+  if (times) tmr_stop(&op);
+  if (perm) {
+    err = gooseberry_permute_dense(n, 1, out, perm, NULL, 1);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (times) tmr_start(&io);
+  {
+    int t = file_type(outfile);
+    if (t < 0) {
+      fprintf(stderr, "Unknown file format of '%s'\n", outfile);
+      err = GOOSEBERRY_ERROR_INVALIDINPUT;
+      goto END;
+    }
+    if (is_dense(t)) {
+      err = gooseberry_write_dense_matrix(t, outfile, n, 1, out);
+    } else {
+      /* the reference's matrix_sparsify rejects a dense vector
+       * (src/matrix.c:185-200), so a sparse output file is an error */
+      fprintf(stderr, "Unknown matrix type: %d\n", 0);
+      err = GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+  }
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) {
+    tmr_stop(&io);
+    /* this build times the whole power iteration (SpMV included) as one */
+    const char *names[] = {"I/O", "Preprocessing", "Compute", "  SpMV"};
+    double vals[] = {io.acc, pre.acc, op.acc, op.acc};
+    print_times(names, vals, 4);
+  }
+END:
+  gooseberry_device_free(d_out);
+  gooseberry_matrix_free(dmat);
+  free(out);
+  free(perm);
+  mat_free(&mat);
+  return err;
+}
+
 /* ---------------------------------------------------------------- main */
 
 static const struct { const char *name; const char *desc; int provided; } COMMANDS[] = {
@@ -564,7 +729,7 @@ static const struct { const char *name; const char *desc; int provided; } COMMAN
   {"blas", "Perform a blas operation.", 1},
   {"cgd", "Perform conjugate gradient descent.", 1},
   {"sgd", "Perform stochastic gradient descent.", 0},
-  {"pagerank", "Perform pagerank on a square matrix.", 0},
+  {"pagerank", "Perform pagerank on a square matrix.", 1},
 };
 #define NCOMMANDS (sizeof(COMMANDS) / sizeof(COMMANDS[0]))
 
@@ -592,6 +757,8 @@ int main(int argc, char **argv)
     err = cmd_blas(argc, argv);
   } else if (strcmp(cmd, "cgd") == 0) {
     err = cmd_cgd(argc, argv);
+  } else if (strcmp(cmd, "pagerank") == 0) {
+    err = cmd_pagerank(argc, argv);
   } else {
     size_t i;
     for (i = 0; i < NCOMMANDS; ++i) if (strcmp(cmd, COMMANDS[i].name) == 0) break;

@@ -10,7 +10,7 @@ int gb_check_csr(dim_t nrows, dim_t ncols, const ind_t *rowptr,
     const dim_t *rowind);
 gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat);
 int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
-    dim_t *d_rowind, real_t *d_rowval, const gooseberry_plan_t *plan,
-    gooseberry_matrix_t *r_mat);
+    dim_t *d_rowind, real_t *d_rowval, ind_t nnz,
+    const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat);
 
 #endif

@@ -92,6 +92,13 @@ int gooseberry_scale(
     real_t * val,
     real_t s);
 
+/* val[i] += s on n values (include/gooseberry.h:150-153; src/blas.c:397-411).
+ * Host pointer; computed on the GPU. */
+int gooseberry_add_scalar(
+    ind_t n,
+    real_t * val,
+    real_t s);
+
 /* C = A x B (include/gooseberry.h:174-184; src/gooseberry.c:53-67,
  * src/blas.c:552-606).  A is CSR (anrows x ancols), B is column-major
  * (ancols x bncols, i.e. B^t rows), C row-major (anrows x bncols).  All
@@ -365,7 +372,10 @@ int gooseberry_cgd(
 
 typedef struct gooseberry_batch_s * gooseberry_batch_t;
 
-/* delta <= 0 selects 1e-4. */
+/* delta <= 0 (or >= 1) selects 0.05: a refresh every 1.3 decades of the
+ * FP32 recurrence residual keeps the Krylov direction consistent with the
+ * refreshed residual (much smaller deltas let the two drift apart and the
+ * FP32 recurrence can stall on ill-conditioned systems). */
 int gooseberry_batch_create(
     uint32_t max_systems,
     double delta,
@@ -492,6 +502,45 @@ int gooseberry_server_take_result(
     uint64_t id,
     real_t * h_x,
     size_t n);
+
+/* --------------------------------------------------------------- PageRank */
+/* The power iteration of `gooseberry pagerank` (src/gooseberry_bin.c:
+ * 2622-2737) on the GPU SpMV.  The matrix's values are divided by their
+ * column sums (added in row order), rows without entries are the sinks,
+ * rank starts at 1/n, and each iteration computes out = damping * (A rank)
+ * + ((1 - damping) / n + sum of rank over the sinks) -- the reference adds
+ * the whole sink mass to every entry.  Every tenth iteration (i % 10 == 0)
+ * stops when sum (rank - out)^2 <= minerror^2.  niter == 0 runs until that
+ * test passes (at most 10^6 iterations here; the reference has no bound).
+ * The result is the vector the reference writes: the last iterate when
+ * the test passed, else the one before the last. */
+typedef struct gooseberry_pagerank_info_t {
+  uint64_t niter;   /* the iteration count the reference prints */
+  double rmse;      /* the RMSE the reference prints */
+  dim_t nsinks;
+} gooseberry_pagerank_info_t;
+
+/* On a resident square matrix; d_out: nrows values on the device.  The
+ * matrix itself is not modified. */
+int gooseberry_matrix_pagerank(
+    gooseberry_matrix_t mat,
+    real_t damping,
+    size_t niter,
+    real_t minerror,
+    real_t * d_out,
+    gooseberry_pagerank_info_t * info);
+
+/* The same on host CSR arrays (n x n) and a host output vector. */
+int gooseberry_pagerank(
+    dim_t n,
+    const ind_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval,
+    real_t damping,
+    size_t niter,
+    real_t minerror,
+    real_t * out,
+    gooseberry_pagerank_info_t * info);
 
 /* ------------------------------------------------------- synthetic inputs */
 

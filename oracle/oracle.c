@@ -668,3 +668,81 @@ int orc_permute_dense_rev(dim_t nrows, dim_t ncols, real_t *rowval,
   free(p);
   return ORC_SUCCESS;
 }
+
+/* src/blas.c:397-411 */
+int orc_add_scalar(ind_t n, real_t *v, real_t s)
+{
+  #pragma omp parallel for schedule(static, 256)
+  for (ind_t i = 0; i < n; ++i) v[i] += s;
+  return ORC_SUCCESS;
+}
+
+/* src/gooseberry_bin.c:2622-2641 (sparse matrix case) */
+int orc_pagerank_prepare(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, real_t *rowval, dim_t *sinks, dim_t *nsinks)
+{
+  real_t *indeg = calloc(ncols ? ncols : 1, sizeof(real_t));
+  if (!indeg) return ORC_ERROR_UNKNOWN;
+  dim_t ns = 0;
+  for (dim_t k = 0; k < nrows; ++k) {
+    if (rowptr[k] == rowptr[k + 1]) {
+      sinks[ns++] = k;
+    } else {
+      for (ind_t l = rowptr[k]; l < rowptr[k + 1]; ++l)
+        indeg[rowind[l]] += rowval[l];
+    }
+  }
+  for (dim_t k = 0; k < nrows; ++k)
+    for (ind_t l = rowptr[k]; l < rowptr[k + 1]; ++l)
+      rowval[l] /= indeg[rowind[l]];
+  free(indeg);
+  *nsinks = ns;
+  return ORC_SUCCESS;
+}
+
+static real_t pr_sqerr(dim_t n, const real_t *a, const real_t *b)
+{
+  real_t e = 0.0, d;
+  for (dim_t k = 0; k < n; ++k) {
+    d = a[k] - b[k];
+    e += d * d;
+  }
+  return e;
+}
+
+/* src/gooseberry_bin.c:2669-2737 */
+int orc_pagerank(dim_t n, const ind_t *rowptr, const dim_t *rowind,
+    const real_t *nrowval, const dim_t *sinks, dim_t nsinks, real_t damping,
+    size_t niter, real_t minerror, real_t *out, size_t *iters, real_t *rmse)
+{
+  real_t *a = malloc(sizeof(real_t) * (n ? n : 1));
+  real_t *b = malloc(sizeof(real_t) * (n ? n : 1));
+  if (!a || !b) { free(a); free(b); return ORC_ERROR_UNKNOWN; }
+  real_t *rank = a, *o = b, *t;
+  real_t error = 0.0, dist;
+  size_t i;
+  for (dim_t k = 0; k < n; ++k) rank[k] = 1.0 / n;
+  for (i = 0; niter == 0 || i < niter; ++i) {
+    orc_spmult(n, n, 1, rowptr, rowind, nrowval, rank, o, NULL, 0);
+    orc_scale(n, o, damping);
+    dist = 0;
+    for (dim_t k = 0; k < nsinks; ++k) dist += rank[sinks[k]];
+    orc_add_scalar(n, o, ((1.0 - damping) / n) + dist);
+    if (i % 10 == 0) {
+      error = pr_sqerr(n, rank, o);
+      if (error <= minerror * minerror) {
+        ++i;
+        goto finish;
+      }
+    }
+    t = rank; rank = o; o = t;
+  }
+  error = pr_sqerr(n, rank, o);
+finish:
+  *rmse = sqrt(error);
+  *iters = i;
+  memcpy(out, o, sizeof(real_t) * n);
+  free(a);
+  free(b);
+  return ORC_SUCCESS;
+}

@@ -111,6 +111,28 @@ int orc_permute_dense(orc_dim_t nrows, orc_dim_t ncols, orc_real_t *rowval,
 int orc_permute_dense_rev(orc_dim_t nrows, orc_dim_t ncols,
     orc_real_t *rowval, const orc_dim_t *rowperm, const orc_dim_t *colperm);
 
+/* src/blas.c:397-411: v[i] += s */
+int orc_add_scalar(orc_ind_t n, orc_real_t *v, orc_real_t s);
+
+/* src/gooseberry_bin.c:2622-2641 (sparse case): indeg[c] = sum of the
+ * values in column c, added in row order; every value is divided by its
+ * column's sum; sinks = the rows without entries, ascending.  rowval is
+ * normalised in place; sinks holds nrows entries, *nsinks is set. */
+int orc_pagerank_prepare(orc_dim_t nrows, orc_dim_t ncols,
+    const orc_ind_t *rowptr, const orc_dim_t *rowind, orc_real_t *rowval,
+    orc_dim_t *sinks, orc_dim_t *nsinks);
+
+/* src/gooseberry_bin.c:2669-2737: one run of the power iteration on the
+ * normalised matrix (square, n rows).  niter == 0 runs until the error
+ * test passes.  out receives the vector the reference writes (its `out`
+ * after the last swap), *iters the printed iteration count and *rmse the
+ * printed RMSE.  The error sums are sequential (the reference's OpenMP
+ * reduction at one thread). */
+int orc_pagerank(orc_dim_t n, const orc_ind_t *rowptr, const orc_dim_t *rowind,
+    const orc_real_t *nrowval, const orc_dim_t *sinks, orc_dim_t nsinks,
+    orc_real_t damping, size_t niter, orc_real_t minerror, orc_real_t *out,
+    size_t *iters, orc_real_t *rmse);
+
 void orc_free(void *p);
 int orc_num_threads(void);
 void orc_set_num_threads(int t);

@@ -52,6 +52,13 @@ def lib():
         L.orc_permute_dense.restype = i32
         L.orc_permute_dense_rev.argtypes = [u32, u32, _P, _P, _P]
         L.orc_permute_dense_rev.restype = i32
+        L.orc_add_scalar.argtypes = [u32, _P, f64]
+        L.orc_add_scalar.restype = i32
+        L.orc_pagerank_prepare.argtypes = [u32, u32, _P, _P, _P, _P, C.POINTER(u32)]
+        L.orc_pagerank_prepare.restype = i32
+        L.orc_pagerank.argtypes = [u32, _P, _P, _P, _P, u32, f64, sz, f64, _P,
+                                   C.POINTER(sz), C.POINTER(f64)]
+        L.orc_pagerank.restype = i32
         L.orc_free.argtypes = [_P]
         L.orc_set_num_threads.argtypes = [i32]
         L.orc_num_threads.restype = i32
@@ -201,3 +208,26 @@ def permute_dense(vals, nrows, ncols, rowperm=None, colperm=None, rev=False):
     f = lib().orc_permute_dense_rev if rev else lib().orc_permute_dense
     f(nrows, ncols, _p(vals), _p(_u32(rowperm)), _p(_u32(colperm)))
     return vals
+
+
+def pagerank_prepare(rowptr, rowind, rowval, ncols):
+    """Normalised values (column sums in row order) and the sink rows."""
+    rowptr, rowind = _u32(rowptr), _u32(rowind)
+    nval = np.array(rowval, dtype=np.float64, copy=True)
+    n = rowptr.size - 1
+    sinks = np.zeros(max(n, 1), dtype=np.uint32)
+    ns = C.c_uint32()
+    lib().orc_pagerank_prepare(n, ncols, _p(rowptr), _p(rowind), _p(nval), _p(sinks),
+                               C.byref(ns))
+    return nval, sinks[:ns.value].copy()
+
+
+def pagerank(rowptr, rowind, nval, sinks, damping=0.85, niter=0, minerror=0.0):
+    """Returns (out vector, iterations, rmse) as the reference prints them."""
+    rowptr, rowind, nval, sinks = _u32(rowptr), _u32(rowind), _f64(nval), _u32(sinks)
+    n = rowptr.size - 1
+    out = np.zeros(n, dtype=np.float64)
+    it, err = C.c_size_t(), C.c_double()
+    lib().orc_pagerank(n, _p(rowptr), _p(rowind), _p(nval), _p(sinks), sinks.size,
+                       damping, niter, minerror, _p(out), C.byref(it), C.byref(err))
+    return out, it.value, err.value

@@ -1,0 +1,114 @@
+"""GPU PageRank (src/gooseberry_bin.c:2622-2737) against the oracle.
+
+The reference adds the whole sink mass to every entry
+(src/gooseberry_bin.c:2689-2694), so with empty rows present the iteration
+grows without bound and the error test never passes: runs with sinks use
+a fixed iteration count, runs to an error threshold use sink-free graphs.
+
+The normalisation (column sums in row order, IEEE division), the SpMV of
+rows with at most 64 entries, the sink sum (sink order) and the
+scale-then-add update are all the reference's operations in the
+reference's order, so the written vector is bitwise the oracle's and the
+iteration count is equal; only the stopping error is summed in another
+order (checked to 1e-12 relative).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_oracle_pinning import _random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_pr(oracle, rp, ri, rv, n, d, niter, minerror):
+    nval, sinks = oracle.pagerank_prepare(rp, ri, rv, n)
+    return oracle.pagerank(rp, ri, nval, sinks, d, niter, minerror) + (sinks.size,)
+
+
+@pytest.mark.parametrize("sink_every,niter,minerror",
+                         [(0, 0, 1e-10), (11, 7, 0.0), (11, 1, 0.0), (0, 40, 1e-6),
+                          (11, 25, 0.0)])
+def test_pagerank_bitwise_with_oracle(gb, oracle, sink_every, niter, minerror):
+    n = 3000
+    rp, ri, rv = _random_graph(n, 5, sink_every=sink_every)
+    M = gb.Matrix(rp, ri, rv)
+    x, it, err, ns = M.pagerank(0.85, niter, minerror)
+    xo, ito, erro, nso = oracle_pr(oracle, rp, ri, rv, n, 0.85, niter, minerror)
+    assert ns == nso and it == ito
+    assert np.array_equal(x, xo)
+    assert abs(err - erro) <= 1e-12 * max(erro, 1e-300)
+
+
+def test_pagerank_long_rows_and_many_sinks(gb, oracle):
+    # rows above 64 entries use tree reductions: close, not bitwise
+    n = 5000
+    rng = np.random.default_rng(8)
+    rows = []
+    for i in range(n):
+        k = 0 if i % 3 == 0 else (300 if i % 97 == 0 else 5)
+        rows.append(np.sort(rng.choice(n, size=k, replace=False)))
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ri = np.concatenate(rows).astype(np.uint32)
+    rv = rng.uniform(0.1, 1.0, ri.size)
+    x, it, err, ns = gb.Matrix(rp, ri, rv).pagerank(0.85, 30, 0.0)
+    xo, ito, erro, nso = oracle_pr(oracle, rp, ri, rv, n, 0.85, 30, 0.0)
+    assert ns == nso and it == ito == 30
+    assert np.max(np.abs(x - xo)) <= 1e-13 * np.max(np.abs(xo))
+
+
+def test_pagerank_all_sinks(gb):
+    n = 5
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    x, it, _, ns = gb.Matrix(rp, np.zeros(0, np.uint32), np.zeros(0)).pagerank(0.85, 2, 0.0)
+    assert ns == 5 and it == 2
+    assert np.all(x == (1 - 0.85) / n + 1.0)
+
+
+def test_pagerank_host_dropin_and_add_scalar(gb, oracle):
+    n = 1000
+    rp, ri, rv = _random_graph(n, 9)
+    x, it, err = gb.pagerank(rp, ri, rv, 0.9, 0, 1e-9)
+    xo, ito, erro, _ = oracle_pr(oracle, rp, ri, rv, n, 0.9, 0, 1e-9)
+    assert it == ito and np.array_equal(x, xo)
+    v = np.linspace(-1, 1, 777)
+    assert np.array_equal(gb.add_scalar(v, 0.125), v + 0.125)
+
+
+def test_pagerank_cli(gb, oracle, tmp_path):
+    from gooseberry_amd import CLI_PATH
+    n = 2000
+    rp, ri, rv = _random_graph(n, 12)
+    a = os.path.join(str(tmp_path), "G.csr")
+    assert oracle.write_csr(a, rp, ri, rv) == 1
+    out = os.path.join(str(tmp_path), "pr.vec")
+    r = subprocess.run([CLI_PATH, "pagerank", "-i", a, "-o", out, "-e", "1e-9", "-t"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    # the oracle through the same text input
+    nr, nc, orp, ori, orv = oracle.read_sparse(a)
+    xo, ito, erro, _ = oracle_pr(oracle, orp, ori, orv, nr, 0.85, 0, 1e-9)
+    assert "PageRank finished in %d iterations with an RMSE of %f" % (ito, erro) in r.stdout
+    assert "Compute:" in r.stdout
+    ref = os.path.join(str(tmp_path), "ref.vec")
+    oracle.write_grid(ref, xo, nr, 1)
+    assert open(out).read() == open(ref).read()
+    # with sinks and an iteration cap: the vector before the last iterate
+    rp, ri, rv = _random_graph(n, 13, sink_every=9)
+    assert oracle.write_csr(a, rp, ri, rv) == 1
+    r = subprocess.run([CLI_PATH, "pagerank", "-i", a, "-o", out, "-I", "12", "-d", "0.7"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    nr, nc, orp, ori, orv = oracle.read_sparse(a)
+    xo, ito, erro, _ = oracle_pr(oracle, orp, ori, orv, nr, 0.7, 12, 0.0)
+    assert ito == 12 and "PageRank finished in 12 iterations" in r.stdout
+    oracle.write_grid(ref, xo, nr, 1)
+    assert open(out).read() == open(ref).read()
+    # a sparse output file is rejected, as matrix_sparsify does
+    r = subprocess.run([CLI_PATH, "pagerank", "-i", a, "-I", "3", "-o",
+                        os.path.join(str(tmp_path), "pr.csr")],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 2

@@ -208,7 +208,10 @@ def test_cli_exit_codes_without_compute(tmp_path):
     assert run_cli("nosuch").returncode == 1
     r = run_cli("help")
     assert r.returncode == 0 and "cgd" in r.stdout
-    assert run_cli("pagerank").returncode == 2
+    assert run_cli("sgd").returncode == 2           # not provided by this build
+    assert run_cli("pagerank").returncode == 0      # usage only (nargs < 2)
+    r = run_cli("pagerank", "-i", os.path.join(str(tmp_path), "no.csr"), "-t")
+    assert r.returncode == 2 and "output vector file" in r.stderr
     assert run_cli("blas").returncode == 0          # usage only (nargs < 2)
     r = run_cli("cgd", "-i", os.path.join(str(tmp_path), "no.csr"), "-i",
                 os.path.join(str(tmp_path), "no.vec"), "-o", os.path.join(str(tmp_path), "x.vec"))

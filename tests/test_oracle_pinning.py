@@ -343,3 +343,66 @@ def test_read_labels_and_partition(oracle, tmp_path):
     assert m.tolist() == labels
     assert perm.tolist() == np.argsort(labels, kind="stable").tolist()
     assert dist.tolist() == [0, 3, 4, 6]
+
+
+# ------------------------------------------------------------------ PageRank
+# src/gooseberry_bin.c:2622-2737
+
+def _random_graph(n, seed, sink_every=0, deg=6):
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        if sink_every and i % sink_every == 0:
+            rows.append([])
+            continue
+        cols = rng.choice(n, size=deg, replace=False)
+        rows.append(sorted(int(c) for c in cols))
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ri = np.array([c for r in rows for c in r], dtype=np.uint32)
+    rv = rng.uniform(0.5, 2.0, ri.size)
+    return rp, ri, rv
+
+
+def test_pagerank_normalisation_is_by_column(oracle):
+    rp, ri, rv = _random_graph(400, 1, sink_every=7)
+    nval, sinks = oracle.pagerank_prepare(rp, ri, rv, 400)
+    A = sp.csr_matrix((nval, ri, rp), shape=(400, 400))
+    colsum = np.asarray(A.sum(axis=0)).ravel()
+    used = np.bincount(ri, minlength=400) > 0
+    assert np.allclose(colsum[used], 1.0, rtol=0, atol=1e-14)
+    assert np.array_equal(sinks, np.flatnonzero(np.diff(rp.astype(np.int64)) == 0))
+
+
+def test_pagerank_without_sinks_solves_the_linear_system(oracle):
+    # with no sinks the fixed point is (I - d N) r = (1 - d) / n
+    n, d = 300, 0.85
+    rp, ri, rv = _random_graph(n, 2)
+    nval, sinks = oracle.pagerank_prepare(rp, ri, rv, n)
+    assert sinks.size == 0
+    out, it, err = oracle.pagerank(rp, ri, nval, sinks, d, 0, 1e-15)
+    N = sp.csr_matrix((nval, ri, rp), shape=(n, n))
+    r = spla.spsolve((sp.identity(n) - d * N).tocsc(), np.full(n, (1 - d) / n))
+    assert np.allclose(out, r, rtol=1e-10, atol=0)
+    assert it % 10 == 1 and err <= 1e-15
+
+
+def test_pagerank_sinks_add_their_whole_mass(oracle):
+    # every row empty: out = 0 * d + (1 - d) / n + sum(rank); with two
+    # iterations the written vector is the first iterate (the reference's
+    # `out` after its last swap)
+    n, d = 5, 0.85
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    ri = np.zeros(0, dtype=np.uint32)
+    nval, sinks = oracle.pagerank_prepare(rp, ri, np.zeros(0), n)
+    assert list(sinks) == [0, 1, 2, 3, 4]
+    out, it, _ = oracle.pagerank(rp, ri, nval, sinks, d, 2, 0.0)
+    assert it == 2
+    s = 0.0
+    for _ in range(n):
+        s += 1.0 / n
+    assert np.all(out == (1 - d) / n + s)
+    assert abs(out[0] - 1.03) < 1e-15
+    # one iteration: the written vector is the starting rank
+    out1, it1, _ = oracle.pagerank(rp, ri, nval, sinks, d, 1, 0.0)
+    assert it1 == 1 and np.all(out1 == 1.0 / n)
