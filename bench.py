@@ -273,6 +273,14 @@ def run_zipf(args, gb):
     db = gb.uniform_device(2, n, stream=1)
     bh = db.numpy()
     log("zipf: n=%d nnz=%d (mean row %.2f) generated in %.2f s" % (n, nnz, nnz / n, gen_s))
+    dxs = gb.uniform_device(2, n, stream=0)
+    dys = gb.DeviceArray(n)
+    roof_default = spmv_roofline(gb, M, dxs, dys)
+    plan, tuned_ms = M.autotune(gb.PRECISION_FP64, 5)
+    roof = spmv_roofline(gb, M, dxs, dys)
+    log("zipf: SpMV default plan %.1f us, autotuned %.1f us (%s)"
+        % (roof_default["avg_launch_ms"] * 1e3, roof["avg_launch_ms"] * 1e3, plan))
+    del dxs, dys
 
     # the FP64 reference solution: FP64 CG run to 1e-12
     dref = gb.DeviceArray(n)
@@ -317,8 +325,6 @@ def run_zipf(args, gb):
 
     t64 = float(np.median([a for a, _, _ in f64]))
     tmx = float(np.median([a for a, _, _ in mix]))
-    dx = gb.DeviceArray(n)
-    roof = spmv_roofline(gb, M, db, dx)
     cfg = {"workload": ("random symmetric diagonally dominant SPD, n=%d, degrees Zipf(1.5) "
                         "clipped to [2,4096] then symmetrised (nnz=%d, mean row length "
                         "%.2f), off-diagonals U[-1,0), diagonal 1.01 x |row sum|, seed 2; "
@@ -333,6 +339,8 @@ def run_zipf(args, gb):
                p50_solve_ms=pct([a * 1e3 for a, _, _ in mix], 50),
                p99_solve_ms=pct([a * 1e3 for a, _, _ in mix], 99),
                fp64_reference={"tol": 1e-12, "iterations": int(iref.niter)},
+               autotune={"plan": plan, "spmv_ms_default": roof_default["avg_launch_ms"],
+                         "spmv_ms_tuned": roof["avg_launch_ms"]},
                generate_s=gen_s)
     if not args.no_cpu_baseline:
         rp, ri, rv = M.download()
@@ -505,6 +513,12 @@ def run_served(args, gb):
     # before the timed regions: the plain FP64 CG one solve at a time, then
     # the same systems through the server (mixed precision, 64 slots)
     kc = min(args.capacity_requests, len(ns))
+    if kc == 0:
+        return line(len(done) / wall, "solves/s", args, wall * 1e3, "f32+f64",
+                    {"workload": "served stream (no capacity comparison)",
+                     "requests": int(len(ns))},
+                    p50_latency_ms=pct(lats, 50), p99_latency_ms=pct(lats, 99),
+                    all_converged=ok, roofline=None, cpu_baseline=None)
     mats = [gb.generate_sdd(int(ns[i]), int(seeds[i]), 3, 27) for i in range(kc)]
     bs = [gb.uniform_device(int(seeds[i]), int(ns[i]), stream=1) for i in range(kc)]
     xs = [gb.DeviceArray(int(ns[i])) for i in range(kc)]

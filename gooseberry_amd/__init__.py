@@ -635,13 +635,17 @@ def generate_spd(n, seed, law=DEGREE_UNIFORM, zipf_s=1.5, dmin=2, dmax=4096, pla
     return Matrix._from_handle(h.value)
 
 
-def autotune_all(mats, precision=PRECISION_MIXED, reps=5):
+def autotune_all(mats, precision=PRECISION_MIXED, reps=5, min_nnz=1 << 20):
     """Autotune many matrices: one timed search per class of matrices with
     the same occupied row-length buckets and nnz within a factor of 4 (the
-    largest of the class is tuned), its plan applied to the whole class."""
+    largest of the class is tuned), its plan applied to the whole class.
+    Classes below min_nnz keep their default plans: their SpMV is
+    launch-latency bound, and timings that short cannot rank plans."""
     groups = {}
     for M in mats:
         inf = M.info()
+        if int(inf["nnz"]) < min_nnz:
+            continue
         occupied = tuple(int(b) for b in np.nonzero(np.asarray(inf["hist"]))[0])
         key = (occupied, int(np.log2(max(int(inf["nnz"]), 1))) // 2)
         groups.setdefault(key, []).append(M)

@@ -83,7 +83,6 @@ struct gbd_batch {
   uint32_t cap_spmv, cap_elem;
   uint32_t *h_spmv, *h_elem, *h_sysl;  // pinned staging
   double delta;
-  uint32_t xcd_remap;
   hipGraph_t graph[GB_B_NGRAPH];
   hipGraphExec_t gexec[GB_B_NGRAPH];
 };
@@ -241,15 +240,12 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
 
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
-          const gb_blists *__restrict__ lists, uint32_t remap) {
+          const gb_blists *__restrict__ lists) {
   __shared__ double prodd[GB_STREAM_LDS];
   __shared__ double tredd[GB_WAVES_PER_BLOCK];
   const uint32_t count = lists->n_spmv;
   const uint32_t lane = threadIdx.x & (GB_WAVE - 1), wave = threadIdx.x / GB_WAVE;
-  for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
-    // remap: each XCD walks one contiguous range of the list, so a system's
-    // items (and its gathered vector) mostly stay in one XCD's L2
-    const uint32_t it = (remap && gridDim.x >= count) ? gb_xcd_remap(b, count) : b;
+  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
     const uint32_t s = items[2 * it], w = items[2 * it + 1];
     const gb_bsys &S = sys[s];
     const gbd_bstate *st = S.st;
@@ -569,8 +565,7 @@ static void gb_b_drop_graphs(gbd_batch *B) {
 static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
   hipStream_t s = gb_stream();
   for (uint32_t i = 0; i < k; ++i) {
-    gb_b_spmv<<<B->cap_spmv, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists,
-                                                B->xcd_remap);
+    gb_b_spmv<<<B->cap_spmv, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists);
     GB_LAUNCH_CHECK();
     gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
     GB_LAUNCH_CHECK();
@@ -635,10 +630,6 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
   if (!B) return GOOSEBERRY_ERROR_NOMEMORY;
   B->max_slots = max_slots;
   B->delta = delta > 0 && delta < 1 ? delta : 0.05;
-  {
-    const char *e = getenv("GB_BATCH_XCD_REMAP");
-    B->xcd_remap = e && e[0] == '1';
-  }
   B->used = (uint8_t *)calloc(max_slots, 1);
   B->slot_mem = (void **)calloc(max_slots, sizeof(void *));
   B->h_sys = (gb_bsys *)calloc(max_slots, sizeof(gb_bsys));

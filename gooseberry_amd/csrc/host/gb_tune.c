@@ -7,7 +7,8 @@
  * Each candidate is built on the device (gbd_matrix_replan) and timed with
  * HIP events over `reps` SpMVs after one warm-up launch; the search is a
  * coordinate descent (short_max, then window x rowcap, then wave_max) from
- * the current plan, and the fastest plan is kept.  Every plan gives the
+ * the current plan, and a candidate is kept when it is at least 3% faster
+ * than the best so far.  Every plan gives the
  * same results to the SpMV's stated tolerance (exact plans: bitwise), so
  * tuning changes speed only.
  */
@@ -18,6 +19,9 @@
 #include "gb_host.h"
 
 #define GB_TUNE_LDS 2304   /* window + short_max must fit the LDS stage */
+/* a candidate replaces the best plan only when it is this much faster
+ * (timing noise of short launches must not pick a plan) */
+#define GB_TUNE_GAIN 0.97
 
 typedef struct {
   gbd_matrix *m;
@@ -103,7 +107,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
     if (p.window + p.short_max > GB_TUNE_LDS) p.window = GB_TUNE_LDS - p.short_max;
     if (p.wave_max < p.short_max) p.wave_max = p.short_max;
     if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
-    if (ms < best_ms) { best_ms = ms; best = p; }
+    if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
   }
   /* 2. the row-block window and row cap */
   static const uint32_t windows[] = {512, 1024, 2048};
@@ -117,7 +121,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
         if (p.window + p.short_max > GB_TUNE_LDS) continue;
         if (p.window == best.window && p.rowcap == best.rowcap) continue;
         if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
-        if (ms < best_ms) { best_ms = ms; best = p; }
+        if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
       }
   }
   /* 3. wavefront-per-row against workgroup-per-row for the long rows */
@@ -131,7 +135,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
                         umax(p.wave_max, best.wave_max)))
         continue;
       if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
-      if (ms < best_ms) { best_ms = ms; best = p; }
+      if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
     }
   }
   gooseberry_plan_t pub;
