@@ -91,7 +91,7 @@ class _Plan(C.Structure):
     _fields_ = [("short_max", C.c_uint32), ("wave_max", C.c_uint32),
                 ("window", C.c_uint32), ("rowcap", C.c_uint32),
                 ("exact", C.c_uint32), ("grid", C.c_uint32),
-                ("xcd_remap", C.c_uint32)]
+                ("xcd_remap", C.c_uint32), ("csplit", C.c_uint32)]
 
 
 class _MatrixInfo(C.Structure):
@@ -334,7 +334,7 @@ def _plan(plan):
     p = _Plan(plan.get("short_max", 0), plan.get("wave_max", 0),
               plan.get("window", 0), plan.get("rowcap", 0),
               1 if plan.get("exact") else 0, plan.get("grid", 0),
-              1 if plan.get("xcd_remap") else 0)
+              1 if plan.get("xcd_remap") else 0, plan.get("csplit", 0))
     return C.byref(p)
 
 
@@ -378,7 +378,7 @@ class Matrix:
                 "plan": {"short_max": i.plan.short_max, "wave_max": i.plan.wave_max,
                          "window": i.plan.window, "rowcap": i.plan.rowcap,
                          "exact": i.plan.exact, "grid": i.plan.grid,
-                         "xcd_remap": i.plan.xcd_remap},
+                         "xcd_remap": i.plan.xcd_remap, "csplit": i.plan.csplit},
                 "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
                 "nrows_block": i.nrows_block, "hist": list(i.hist)}
 

@@ -87,6 +87,10 @@ typedef struct gbd_plan_params {
   uint32_t grid;       /* cap on SpMV workgroups (0: one per work item) */
   uint32_t xcd_remap;  /* 1: renumber row blocks so each XCD walks a
                           contiguous range */
+  uint32_t csplit;     /* > 1: the columns are split into this many slices,
+                          each multiplied by its own launch into a partial
+                          vector, the partials added after (x gathers stay
+                          in one slice of x at a time); 0 or 1: off */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram

@@ -24,7 +24,27 @@ struct gbd_matrix {
   uint32_t nblockrows;  // rows processed one workgroup per row
   uint32_t *rowbins;    // stable bin permutation: short | wave | block rows
   uint64_t hist[33];
+  // column split (params.csplit = P > 1): P views, view r holding the
+  // entries of columns [r W, (r + 1) W), W = ceil(ncols / P), in row order;
+  // the views share the arrays below (row pointers of view r start at
+  // srowptr + r * nrows) and own only their plans
+  uint32_t nsplit;
+  gbd_matrix **split;
+  uint32_t *srowptr;  // P * nrows + 1
+  uint32_t *srowind;  // nnz
+  double *sval;       // nnz
+  float *sval32;      // nnz, with val32
+  double *ypart;      // P * nrows partial results (FP64, or FP32 in place)
+  bool view;          // rowptr/rowind/val belong to a parent matrix
 };
+
+#define GB_SPLIT_MAX 64
+
+// workgroups of the split sum (fixed, so the partial count is known)
+static inline uint32_t gb_split_sum_grid(const gbd_matrix *m) {
+  uint32_t g = (m->nrows + GB_BLOCK - 1) / GB_BLOCK;
+  return g > 1024u ? 1024u : (g ? g : 1u);
+}
 
 // One lane's share of a row: the products val[k] * x[rowind[k]] for
 // k = k0 + lane, k0 + lane + STRIDE, ... added in k order from +0.0.  Four
@@ -74,7 +94,7 @@ uint32_t gb_spmv_grid(const gbd_matrix *m);
 
 // p.Ap partials written by one SpMV launch with DOT (one per wavefront)
 static inline uint32_t gb_spmv_npartials(const gbd_matrix *m) {
-  return gb_spmv_grid(m) * GB_WAVES_PER_BLOCK;
+  return (m->nsplit ? gb_split_sum_grid(m) : gb_spmv_grid(m)) * GB_WAVES_PER_BLOCK;
 }
 
 // Enqueue y = A x; when cg != nullptr also write the per-wavefront partial

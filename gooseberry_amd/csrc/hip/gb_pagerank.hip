@@ -198,6 +198,8 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   mm = *m;
   mm.val = nval;
   mm.val32 = nullptr;
+  mm.nsplit = 0;  // the column-split views hold the unnormalised values
+  mm.split = nullptr;
   gb_pr_fill<<<g, GB_BLOCK, 0, s>>>(n, 1.0 / n, rank);
   GB_PR_TRY(hipGetLastError());
   for (i = 0; (niter == 0 && i < GB_PR_MAX_ITER) || i < niter; ++i) {

@@ -295,10 +295,100 @@ uint32_t gb_spmv_grid(const gbd_matrix *m) {
   return (uint32_t)(nwork < gmax ? nwork : gmax);
 }
 
+// ------------------------------------------------------- column split
+
+// cnt[r * nrows + i] = entries of row i in column slice r
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_split_count(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
+               uint32_t nrows, uint32_t width, uint32_t P, uint32_t *__restrict__ cnt) {
+  const uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (i >= nrows) return;
+  const uint32_t k0 = rowptr[i], k1 = rowptr[i + 1];
+  for (uint32_t r = 0; r < P; ++r) {
+    uint32_t c = 0;
+    for (uint32_t k = k0; k < k1; ++k) c += (rowind[k] / width == r);
+    cnt[(size_t)r * nrows + i] = c;
+  }
+}
+
+// the entries of row i in slice r, in row order, from srowptr[r nrows + i]
+template <typename T>
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_split_fill(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
+              const T *__restrict__ val, uint32_t nrows, uint32_t width, uint32_t P,
+              const uint32_t *__restrict__ srowptr, uint32_t *__restrict__ srowind,
+              T *__restrict__ sval) {
+  const uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (i >= nrows) return;
+  const uint32_t k0 = rowptr[i], k1 = rowptr[i + 1];
+  for (uint32_t r = 0; r < P; ++r) {
+    uint32_t p = srowptr[(size_t)r * nrows + i];
+    for (uint32_t k = k0; k < k1; ++k) {
+      const uint32_t c = rowind[k];
+      if (c / width == r) {
+        if (srowind) srowind[p] = c;
+        sval[p] = val[k];
+        ++p;
+      }
+    }
+  }
+}
+
+// y[i] = ((yp_0[i] + yp_1[i]) + ...) + yp_{P-1}[i]; with DOT also the
+// per-wavefront partials of x.y
+template <typename T, bool DOT>
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_split_sum(const T *__restrict__ yp, uint32_t n, uint32_t P, T *__restrict__ y,
+             uint32_t ystride, const T *__restrict__ x, double *__restrict__ partials) {
+  double dotp = 0.0;
+  for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n; i += gridDim.x * GB_BLOCK) {
+    T s = yp[i];
+    for (uint32_t r = 1; r < P; ++r) s += yp[(size_t)r * n + i];
+    y[(size_t)i * ystride] = s;
+    if (DOT) dotp += (double)x[i] * (double)s;
+  }
+  if (DOT) {
+    const double ws = gb_wave_sum(dotp);
+    if ((threadIdx.x & (GB_WAVE - 1)) == 0)
+      partials[blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE] = ws;
+  }
+}
+
+template <typename T>
+static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
+                            T *y, uint32_t ystride, double *partials,
+                            const gbd_cg_state *cg);
+
+// a split matrix: one launch per slice into its partial vector, then the sum
+template <typename T>
+static int gb_spmv_launch_split(const gbd_matrix *m, const T *x, T *y, uint32_t ystride,
+                                double *partials, const gbd_cg_state *cg) {
+  T *yp = (T *)m->ypart;
+  for (uint32_t r = 0; r < m->nsplit; ++r) {
+    const gbd_matrix *v = m->split[r];
+    const T *vval;
+    if constexpr (sizeof(T) == 8) vval = (const T *)v->val;
+    else vval = (const T *)v->val32;
+    int rv = gb_spmv_launch_t<T>(v, vval, x, yp + (size_t)r * m->nrows, 1, nullptr, nullptr);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  hipStream_t s = gb_stream();
+  if (cg) {
+    gb_split_sum<T, true><<<gb_split_sum_grid(m), GB_BLOCK, 0, s>>>(
+        yp, m->nrows, m->nsplit, y, ystride, x, partials);
+  } else {
+    gb_split_sum<T, false><<<gb_split_sum_grid(m), GB_BLOCK, 0, s>>>(
+        yp, m->nrows, m->nsplit, y, ystride, nullptr, nullptr);
+  }
+  GB_LAUNCH_CHECK();
+  return GOOSEBERRY_SUCCESS;
+}
+
 template <typename T>
 static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
                             T *y, uint32_t ystride, double *partials,
                             const gbd_cg_state *cg) {
+  if (m->nsplit) return gb_spmv_launch_split<T>(m, x, y, ystride, partials, cg);
   const uint32_t grid = gb_spmv_grid(m);
   if (grid == 0) return GOOSEBERRY_SUCCESS;
   hipStream_t s = gb_stream();
@@ -346,12 +436,88 @@ static int gb_plan_check(const gbd_plan_params *p) {
   // the row-block kernel gives each thread at most two rows
   if (p->rowcap > 2 * GB_BLOCK) return 0;
   if (p->grid > 65536) return 0;
+  if (p->csplit > GB_SPLIT_MAX) return 0;
+  // slices change the summation order: not with the bitwise plan
+  if (p->csplit > 1 && p->exact) return 0;
   return 1;
+}
+
+static void gb_split_free(gbd_matrix *m) {
+  for (uint32_t r = 0; r < m->nsplit; ++r) gbd_matrix_destroy(m->split[r]);
+  free(m->split);
+  m->split = nullptr;
+  m->nsplit = 0;
+  gb_dev_free(m->srowptr);
+  gb_dev_free(m->srowind);
+  gb_dev_free(m->sval);
+  gb_dev_free(m->sval32);
+  gb_dev_free(m->ypart);
+  m->srowptr = m->srowind = nullptr;
+  m->sval = m->ypart = nullptr;
+  m->sval32 = nullptr;
+}
+
+static int gb_plan_build(gbd_matrix *m);
+
+// P column slices of width ceil(ncols / P) as views with their own plans
+static int gb_split_build(gbd_matrix *m, uint32_t P) {
+  hipStream_t s = gb_stream();
+  const uint32_t n = m->nrows;
+  const uint32_t width = (m->ncols + P - 1) / P;
+  const uint64_t tot = (uint64_t)P * n;
+  if (tot >= 0xffffffffull || width == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t *cnt = nullptr;
+  uint32_t bounds[GB_SPLIT_MAX + 1];
+  GB_HIP(gb_dev_alloc((void **)&m->srowptr, (tot + 1) * 4));
+  GB_HIP(gb_dev_alloc((void **)&m->srowind, (m->nnz ? m->nnz : 1) * 4));
+  GB_HIP(gb_dev_alloc((void **)&m->sval, (m->nnz ? m->nnz : 1) * 8));
+  GB_HIP(gb_dev_alloc((void **)&m->ypart, tot * 8));
+  GB_HIP(hipMallocAsync((void **)&cnt, tot * 4, s));
+  const uint32_t g = gb_div_up(n, GB_BLOCK);
+  gb_split_count<<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, n, width, P, cnt);
+  GB_LAUNCH_CHECK();
+  int rv = gbd_scan_u32(cnt, m->srowptr, tot, m->srowptr + tot);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  GB_HIP(hipFreeAsync(cnt, s));
+  gb_split_fill<double><<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->val, n, width, P,
+                                               m->srowptr, m->srowind, m->sval);
+  GB_LAUNCH_CHECK();
+  if (m->val32) {
+    GB_HIP(gb_dev_alloc((void **)&m->sval32, (m->nnz ? m->nnz : 1) * 4));
+    gb_split_fill<float><<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->val32, n, width, P,
+                                                m->srowptr, nullptr, m->sval32);
+    GB_LAUNCH_CHECK();
+  }
+  // slice boundaries for the views' nonzero counts
+  for (uint32_t r = 0; r <= P; ++r)
+    GB_HIP(hipMemcpyAsync(&bounds[r], m->srowptr + (size_t)r * n, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  m->split = (gbd_matrix **)calloc(P, sizeof(gbd_matrix *));
+  if (!m->split) return GOOSEBERRY_ERROR_NOMEMORY;
+  m->nsplit = P;
+  for (uint32_t r = 0; r < P; ++r) {
+    gbd_matrix *v = (gbd_matrix *)calloc(1, sizeof(gbd_matrix));
+    if (!v) return GOOSEBERRY_ERROR_NOMEMORY;
+    m->split[r] = v;
+    v->view = true;
+    v->nrows = n;
+    v->ncols = m->ncols;
+    v->nnz = bounds[r + 1] - bounds[r];
+    v->rowptr = m->srowptr + (size_t)r * n;
+    v->rowind = m->srowind;
+    v->val = m->sval;
+    v->val32 = m->sval32;
+    v->params = m->params;
+    v->params.csplit = 0;
+    if ((rv = gb_plan_build(v)) != GOOSEBERRY_SUCCESS) return rv;
+  }
+  return GOOSEBERRY_SUCCESS;
 }
 
 static int gb_plan_build(gbd_matrix *m) {
   hipStream_t s = gb_stream();
   const uint32_t n = m->nrows;
+  if (!m->view) gb_split_free(m);
   memset(m->hist, 0, sizeof(m->hist));
   m->nstream = m->nwave = m->nblockrows = 0;
   if (m->blk) { GB_HIP(gb_dev_free(m->blk)); m->blk = nullptr; }
@@ -395,6 +561,13 @@ static int gb_plan_build(gbd_matrix *m) {
   GB_HIP(hipFreeAsync(idx, s));
   GB_HIP(hipFreeAsync(dist, s));
   GB_HIP(hipFreeAsync(hist, s));
+  if (!m->view && m->params.csplit > 1) {
+    int rv2 = gb_split_build(m, m->params.csplit);
+    if (rv2 != GOOSEBERRY_SUCCESS) {
+      gb_split_free(m);
+      return rv2;
+    }
+  }
   return GOOSEBERRY_SUCCESS;
 }
 
@@ -412,6 +585,7 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->exact = 0;
   p->grid = 0;
   p->xcd_remap = 0;
+  p->csplit = 0;
 }
 
 static int gb_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
@@ -488,6 +662,13 @@ int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
 void gbd_matrix_destroy(gbd_matrix *m) {
   if (!m) return;
   // pool frees are ordered after the queued work that reads the arrays
+  if (m->view) {
+    gb_dev_free(m->blk);
+    gb_dev_free(m->rowbins);
+    free(m);
+    return;
+  }
+  gb_split_free(m);
   if (m->rowptr) gb_dev_free(m->rowptr);
   if (m->rowind) gb_dev_free(m->rowind);
   if (m->val) gb_dev_free(m->val);
@@ -523,6 +704,15 @@ int gbd_matrix_make_f32(gbd_matrix *m) {
     gb_to_f32<<<gb_div_up(m->nnz, GB_BLOCK), GB_BLOCK, 0, gb_stream()>>>(
         m->val, m->val32, m->nnz);
     GB_LAUNCH_CHECK();
+  }
+  if (m->nsplit) {
+    GB_HIP(gb_dev_alloc((void **)&m->sval32, (m->nnz ? m->nnz : 1) * sizeof(float)));
+    if (m->nnz) {
+      gb_to_f32<<<gb_div_up(m->nnz, GB_BLOCK), GB_BLOCK, 0, gb_stream()>>>(
+          m->sval, m->sval32, m->nnz);
+      GB_LAUNCH_CHECK();
+    }
+    for (uint32_t r = 0; r < m->nsplit; ++r) m->split[r]->val32 = m->sval32;
   }
   return GOOSEBERRY_SUCCESS;
 }

@@ -134,6 +134,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   d.exact = p->exact ? 1 : 0;
   if (p->grid) d.grid = p->grid;
   d.xcd_remap = p->xcd_remap ? 1 : 0;
+  d.csplit = p->csplit > 1 ? p->csplit : 0;
   *out = d;
   *have = 1;
 }
@@ -288,6 +289,7 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.exact = p.exact;
   info->plan.grid = p.grid;
   info->plan.xcd_remap = p.xcd_remap;
+  info->plan.csplit = p.csplit;
   return GOOSEBERRY_SUCCESS;
 }
 

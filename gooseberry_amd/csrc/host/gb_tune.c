@@ -6,7 +6,9 @@
  * LDS row-block, wavefront-per-row and workgroup-per-row bins are tried.
  * Each candidate is built on the device (gbd_matrix_replan) and timed with
  * HIP events over `reps` SpMVs after one warm-up launch; the search is a
- * coordinate descent (short_max, then window x rowcap, then wave_max) from
+ * coordinate descent (short_max, then window x rowcap, then wave_max, then
+ * the number of column slices for a gathered vector larger than half an
+ * XCD's L2) from
  * the current plan, and a candidate is kept when it is at least 3% faster
  * than the best so far.  Every plan gives the
  * same results to the SpMV's stated tolerance (exact plans: bitwise), so
@@ -138,6 +140,18 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
       if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
     }
   }
+  /* 4. column slices, for a gathered vector larger than half an XCD's L2 */
+  static const uint32_t splits[] = {4, 8, 16, 32};
+  if (!best.exact && (uint64_t)ncols * es > (2u << 20)) {
+    gbd_plan_params base = best;
+    for (size_t i = 0; i < 4; ++i) {
+      gbd_plan_params p = base;
+      p.csplit = splits[i];
+      if ((uint64_t)ncols * es / splits[i] < (256u << 10)) continue;
+      if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
+      if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
+    }
+  }
   gooseberry_plan_t pub;
   pub.short_max = best.short_max;
   pub.wave_max = best.wave_max;
@@ -146,6 +160,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
   pub.exact = best.exact;
   pub.grid = best.grid;
   pub.xcd_remap = best.xcd_remap;
+  pub.csplit = best.csplit;
   /* through the public replan: it also drops the CG workspace, which is
    * sized by the plan */
   if ((rv = gooseberry_matrix_replan(mat, &pub)) != GOOSEBERRY_SUCCESS) goto done;

@@ -243,6 +243,10 @@ typedef struct gooseberry_plan_t {
                        longer than short_max use tree reductions */
   dim_t grid;       /* cap on SpMV workgroups (0: one per row block) */
   dim_t xcd_remap;  /* 1: each XCD walks a contiguous range of blocks */
+  dim_t csplit;     /* > 1: split the columns into this many slices, one
+                       launch per slice into a partial vector, partials
+                       added after; keeps the gathered slice of x in L2 for
+                       matrices with scattered columns (not with exact) */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */

@@ -211,3 +211,74 @@ def test_invalid_inputs_rejected(gb):
     with pytest.raises(gb.GooseberryError):
         gb.Matrix(np.array([0, 1], np.uint32), np.array([0], np.uint32), np.ones(1),
                   plan={"short_max": 2000, "window": 2000})
+
+
+# ------------------------------------------------------------ column split
+
+def _row_sum_bound(rp, ri, rv, x):
+    # |fl(sum) - sum| <= (len - 1) eps sum |terms| for any summation order
+    terms = np.abs(rv * x[ri.astype(np.int64)])
+    lens = np.diff(rp.astype(np.int64))
+    cs = np.concatenate([[0.0], np.cumsum(terms)])
+    s = cs[rp[1:].astype(np.int64)] - cs[rp[:-1].astype(np.int64)]
+    return 2.0 * np.maximum(lens - 1, 0) * np.finfo(np.float64).eps * s * 1.01 + 1e-300
+
+
+@pytest.mark.parametrize("P", [2, 8, 13])
+def test_column_split_spmv(gb, oracle, P):
+    # uniform random columns, empty rows, a long row and a non-square shape
+    rp, ri, rv = irregular(20000, 40 + P, [(7, 5000), (11, 0), (12, 300)], density=0.0015,
+                           ncols=30011)
+    x = np.random.default_rng(P).uniform(-1, 1, 30011)
+    M = gb.Matrix(rp, ri, rv, ncols=x.size, plan={"csplit": P})
+    assert M.info()["plan"]["csplit"] == P
+    y = M.spmv(x)
+    ref = oracle.spmult(20000, 30011, 1, rp, ri, rv, x)
+    assert np.all(np.abs(y - ref) <= _row_sum_bound(rp, ri, rv, x))
+    assert y[11] == 0.0
+    # repeatable bits; back to the unsplit plan
+    assert np.array_equal(M.spmv(x), y)
+    M.replan(None)
+    assert M.info()["plan"]["csplit"] == 0
+    assert np.all(np.abs(M.spmv(x) - ref) <= _row_sum_bound(rp, ri, rv, x))
+
+
+def test_column_split_more_slices_than_columns_and_spmm(gb, oracle):
+    rp, ri, rv = irregular(500, 3, density=0.3, ncols=5)
+    x = np.random.default_rng(4).uniform(-1, 1, 5)
+    M = gb.Matrix(rp, ri, rv, ncols=5, plan={"csplit": 16})
+    ref = oracle.spmult(500, 5, 1, rp, ri, rv, x)
+    assert np.all(np.abs(M.spmv(x) - ref) <= _row_sum_bound(rp, ri, rv, x))
+    B = np.random.default_rng(5).uniform(-1, 1, (5, 3))
+    c = M.spmm(B.T.ravel(), 3) if hasattr(M, "spmm") else None
+    if c is not None:
+        refc = oracle.spmult(500, 5, 3, rp, ri, rv, B.T.ravel())
+        assert rel_inf(c, refc) <= 1e-14
+
+
+def test_column_split_rejects_exact(gb):
+    rp, ri, rv = irregular(100, 5)
+    with pytest.raises(gb.GooseberryError) as e:
+        gb.Matrix(rp, ri, rv, plan={"csplit": 4, "exact": 1})
+    assert e.value.name == "INVALIDINPUT"
+
+
+def test_column_split_cg_and_batch(gb, oracle):
+    # CG on a split plan: iterations within 2 of the oracle, same residual;
+    # the batched solver keeps using the unsplit plan of the same matrix
+    M0 = gb.generate_sdd(60000, 21, 3, 27)
+    rp, ri, rv = M0.download()
+    M = gb.Matrix(rp, ri, rv, plan={"csplit": 8})
+    b = gb.uniform(22, 60000)
+    x, info = M.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    xo, it, _, _ = oracle.cgd(rp, ri, rv, b, 1e-10 * np.linalg.norm(b))
+    assert abs(int(info.niter) - it) <= 2
+    r = b - oracle.spmv(rp, ri, rv, x)
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
+    B = gb.Batch(1)
+    db, dx = gb.DeviceArray.from_host(b), gb.DeviceArray(60000)
+    s = B.add(M, db, dx, 1e-10, gb.TOL_RELATIVE, 0, gb.PRECISION_MIXED)
+    B.solve()
+    assert B.info(s).status == 1
+    r = b - oracle.spmv(rp, ri, rv, dx.numpy())
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
