@@ -334,7 +334,7 @@ def _plan(plan):
     p = _Plan(plan.get("short_max", 0), plan.get("wave_max", 0),
               plan.get("window", 0), plan.get("rowcap", 0),
               1 if plan.get("exact") else 0, plan.get("grid", 0),
-              1 if plan.get("xcd_remap") else 0, plan.get("csplit", 0))
+              int(plan.get("xcd_remap", 0)), plan.get("csplit", 0))
     return C.byref(p)
 
 

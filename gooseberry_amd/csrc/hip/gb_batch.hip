@@ -4,8 +4,11 @@
 // and SpMV plan) with its own vectors, partial-sum arrays and state.  One
 // device step advances every active system by one step with four launches
 // over work lists:
-//   K1 gb_b_spmv     (system, SpMV work item): Ap and p.Ap partials, or in a
-//                    refresh step the FP64 residual b - A x and its partials
+//   K1 gb_b_spmv     (system, SpMV work item): Ap and one p.Ap partial per
+//                    item, or in a refresh step the FP64 residual b - A x and
+//                    its partials.  A column-split matrix runs over its
+//                    slices into partial vectors (FP64 ones in refresh
+//                    steps), which K3 adds; its refresh norm is tested in K4.
 //   K2 gb_b_reduce   (system): alpha, or the refreshed residual norm and the
 //                    convergence test
 //   K3 gb_b_update   (system, chunk of GB_BCHUNK rows): x/r update and r.r
@@ -48,25 +51,42 @@
 
 enum { GB_MODE_CG = 0, GB_MODE_REFRESH = 1 };
 
-struct gb_bsys {
+// One SpMV plan as the batched kernels read it: the system's own plan, or
+// one column slice of a split matrix (its items numbered from `base`).
+struct gb_bplan {
   const uint32_t *rowptr, *rowind, *blk, *wave_rows, *block_rows;
   const double *val;
   const float *val32;
-  uint32_t n, nstream, nwave, nwaveblocks, nblockrows, nwork, nchunks, exact;
+  uint32_t nstream, nwave, nwaveblocks, nblockrows, nwork, exact, base, pad;
+};
+
+struct gb_bsys {
+  gb_bplan pl;      // the matrix's plan (refresh steps, and every step when
+                    // the matrix is not split)
+  const gb_bplan *slices;  // column slices (nsplit of them) for CG steps
+  void *ypart;      // nsplit * n partial products (FP32 for MIXED, else FP64)
+  uint32_t nsplit, nwork_split;
+  uint32_t n, nchunks;
   const double *b;
   double *x;        // solution (FP64 in both precisions)
   double *r, *p, *ap;  // FP64: residual, direction, A p.  MIXED: r = FP64
                        // residual of refresh steps; p, ap unused
   float *d32, *r32, *p32, *ap32;  // MIXED inner vectors
-  double *part1;    // 4 * nwork partials of K1
+  double *part1;    // one partial per K1 work item
   double *part2;    // nchunks partials of K3
+  double *part3;    // nchunks partials of ||b - A x||^2 (split matrices)
   gbd_bstate *st;
   int32_t precision;
   int32_t pad;
 };
 
+// The SpMV list is cut into segments, one launch each: segment k holds the
+// items of column slice k of every split matrix, and segment 0 also the
+// items of the matrices that are not split.  Launch boundaries make the GPU
+// finish one slice of x before it starts gathering from the next.
 struct gb_blists {
-  uint32_t n_spmv, n_elem, n_sys, pad;
+  uint32_t n_spmv, n_elem, n_sys, nseg;
+  uint32_t seg[GB_SPLIT_MAX + 1];
 };
 
 struct gbd_batch {
@@ -81,6 +101,10 @@ struct gbd_batch {
   gb_blists *h_lists;      // pinned
   uint32_t *d_spmv, *d_elem, *d_sysl;  // device work lists
   uint32_t cap_spmv, cap_elem;
+  uint32_t nseg;                       // SpMV launches per step
+  uint32_t cap_seg[GB_SPLIT_MAX];      // their grids
+  uint32_t *h_slice_nwork;             // host: max_slots x GB_SPLIT_MAX
+  gb_bplan *d_slices;                  // max_slots x GB_SPLIT_MAX slice plans
   uint32_t *h_spmv, *h_elem, *h_sysl;  // pinned staging
   double delta;
   hipGraph_t graph[GB_B_NGRAPH];
@@ -118,7 +142,7 @@ __device__ __forceinline__ double gb_b_sum_chunks(const double *part, uint32_t n
 // EPI 0: y = A x, dot += x[row] * y   (CG step; T = value/vector type)
 // EPI 1: out = b - A x in FP64, dot += out^2   (refresh step; T = double)
 template <typename T, int EPI>
-__device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
+__device__ __forceinline__ double gb_b_spmv_item(const gb_bplan &S, uint32_t w,
                                                  const T *__restrict__ val,
                                                  const T *__restrict__ x,
                                                  T *__restrict__ y,
@@ -240,28 +264,52 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bsys &S, uint32_t w,
 
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
-          const gb_blists *__restrict__ lists) {
+          const gb_blists *__restrict__ lists, uint32_t seg,
+          const gbd_bstate *__restrict__ states, const gb_bplan *__restrict__ slice_tab) {
   __shared__ double prodd[GB_STREAM_LDS];
   __shared__ double tredd[GB_WAVES_PER_BLOCK];
-  const uint32_t count = lists->n_spmv;
-  const uint32_t lane = threadIdx.x & (GB_WAVE - 1), wave = threadIdx.x / GB_WAVE;
-  for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
-    const uint32_t s = items[2 * it], w = items[2 * it + 1];
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  const uint32_t end = lists->seg[seg + 1];
+  for (uint32_t it = lists->seg[seg] + blockIdx.x; it < end; it += gridDim.x) {
+    const uint32_t s = items[2 * it], lw = items[2 * it + 1];
+    // the system, its state and its slice plan are addressed from the slot
+    // alone, so their loads go out together
     const gb_bsys &S = sys[s];
-    const gbd_bstate *st = S.st;
+    const gbd_bstate *st = states + s;
     if (st->done) continue;  // not written during this kernel: uniform
     const int mode = st->mode_next;
     double dotp;
-    if (mode == GB_MODE_REFRESH) {
-      dotp = gb_b_spmv_item<double, 1>(S, w, S.val, S.x, S.r, S.b, prodd, tredd);
+    uint32_t w = lw;  // the item's partial slot
+    if (S.nsplit) {
+      // a column-split matrix runs every step over its slices, slice r in
+      // launch r, each into its partial vector: CG steps in the system's
+      // precision, refresh steps A x in FP64 (the update kernel adds them)
+      const uint32_t r = seg;
+      const gb_bplan &L = slice_tab[(size_t)s * GB_SPLIT_MAX + r];
+      w = L.base + lw;
+      if (mode == GB_MODE_REFRESH) {
+        double *yp = (double *)S.ypart + (size_t)r * S.n;
+        dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.x, yp, nullptr, prodd, tredd);
+      } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+        float *yp = (float *)S.ypart + (size_t)r * S.n;
+        dotp = gb_b_spmv_item<float, 0>(L, lw, L.val32, S.p32, yp, nullptr,
+                                        (float *)prodd, (float *)tredd);
+      } else {
+        double *yp = (double *)S.ypart + (size_t)r * S.n;
+        dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.p, yp, nullptr, prodd, tredd);
+      }
+    } else if (mode == GB_MODE_REFRESH) {
+      dotp = gb_b_spmv_item<double, 1>(S.pl, w, S.pl.val, S.x, S.r, S.b, prodd, tredd);
     } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
-      dotp = gb_b_spmv_item<float, 0>(S, w, S.val32, S.p32, S.ap32, nullptr,
+      dotp = gb_b_spmv_item<float, 0>(S.pl, w, S.pl.val32, S.p32, S.ap32, nullptr,
                                       (float *)prodd, (float *)tredd);
     } else {
-      dotp = gb_b_spmv_item<double, 0>(S, w, S.val, S.p, S.ap, nullptr, prodd, tredd);
+      dotp = gb_b_spmv_item<double, 0>(S.pl, w, S.pl.val, S.p, S.ap, nullptr, prodd, tredd);
     }
-    double ws = gb_wave_sum(dotp);
-    if (lane == 0) S.part1[w * GB_WAVES_PER_BLOCK + wave] = ws;
+    // one partial per work item
+    const double bs = gb_block_sum(dotp, red);
+    if (threadIdx.x == 0) S.part1[w] = bs;
+    __syncthreads();
   }
 }
 
@@ -275,14 +323,22 @@ gb_b_reduce(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ syslis
     gbd_bstate *st = S.st;
     if (st->done) continue;
     const int mode = st->mode_next;
-    const uint32_t np = S.nwork * GB_WAVES_PER_BLOCK;
+    if (S.nsplit && mode == GB_MODE_REFRESH) {
+      // A x is still in slices: the update kernel forms b - A x and the
+      // p-update kernel tests it
+      if (threadIdx.x == 0) st->mode = mode;
+      continue;
+    }
+    // one partial per SpMV work item
+    const uint32_t np = S.nsplit ? S.nwork_split : S.pl.nwork;
+    const double *part = S.part1;
     double s = 0.0;
     for (uint32_t base = 0; base < np; base += GB_BRED * 8) {
       double v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         uint32_t i = base + j * GB_BRED + threadIdx.x;
-        v[j] = i < np ? S.part1[i] : 0.0;
+        v[j] = i < np ? part[i] : 0.0;
       }
 #pragma unroll
       for (int j = 0; j < 8; ++j) s += v[j];
@@ -343,23 +399,49 @@ gb_b_update(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
     double d = 0.0;
     if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
       if (mode == GB_MODE_REFRESH) {
-        // r32 = (float) r64 from the refresh SpMV; d = 0 was set when x was
-        // updated
+        // r32 = (float) r64 from the refresh SpMV (for a split matrix r64 =
+        // b - A x from the slices' FP64 partials, and its squared norm goes
+        // to part3); d = 0 was set when x was updated
+        const double *yp = (const double *)S.ypart;
+        double d64 = 0.0;
         for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
           uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
           if (i < S.n) {
-            float rf = (float)S.r[i];
+            double rr;
+            if (S.nsplit) {
+              double ax = yp[i];
+              for (uint32_t r = 1; r < S.nsplit; ++r) ax += yp[(size_t)r * S.n + i];
+              rr = S.b[i] - ax;
+              S.r[i] = rr;
+              d64 += rr * rr;
+            } else {
+              rr = S.r[i];
+            }
+            float rf = (float)rr;
             S.r32[i] = rf;
             d += (double)rf * (double)rf;
           }
         }
+        if (S.nsplit) {
+          const double b64 = gb_block_sum(d64, red);
+          if (threadIdx.x == 0) S.part3[c] = b64;
+          __syncthreads();
+        }
       } else {
         const float a = (float)st->alpha;
+        const float *yp = (const float *)S.ypart;
         for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
           uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
           if (i < S.n) {
+            float ap;
+            if (S.nsplit) {  // slice partials added in slice order
+              ap = yp[i];
+              for (uint32_t r = 1; r < S.nsplit; ++r) ap += yp[(size_t)r * S.n + i];
+            } else {
+              ap = S.ap32[i];
+            }
             S.d32[i] = S.d32[i] + a * S.p32[i];
-            float rn = S.r32[i] - a * S.ap32[i];
+            float rn = S.r32[i] - a * ap;
             S.r32[i] = rn;
             d += (double)rn * (double)rn;
           }
@@ -368,12 +450,20 @@ gb_b_update(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
     } else {
       // src/cgd.c:92-100
       const double alpha = st->alpha;
+      const double *yp = (const double *)S.ypart;
       for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
         uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
         if (i < S.n) {
+          double ap;
+          if (S.nsplit) {
+            ap = yp[i];
+            for (uint32_t r = 1; r < S.nsplit; ++r) ap += yp[(size_t)r * S.n + i];
+          } else {
+            ap = S.ap[i];
+          }
           double ps = alpha * S.p[i];
           S.x[i] = S.x[i] + ps;
-          double aps = alpha * S.ap[i];
+          double aps = alpha * ap;
           double rn = S.r[i] - aps;
           S.r[i] = rn;
           d += rn * rn;
@@ -407,7 +497,27 @@ gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items
     const uint32_t i0 = c * GB_BCHUNK;
     if (mode == GB_MODE_REFRESH) {
       // MIXED only: restart the FP32 recurrence from the FP64 residual,
-      // keep the direction p
+      // keep the direction p.  A split matrix's FP64 residual norm is
+      // tested here (the reduction kernel tests the others').
+      if (S.nsplit) {
+        const double t = gb_b_sum_chunks(S.part3, S.nchunks, red);
+        if (c == 0 && threadIdx.x == 0) {
+          const double rn = sqrt(t);
+          st->nrefine += 1;
+          st->rnorm64 = rn;
+          st->rnorm_ref = rn;
+          if (rn <= st->tol) {
+            st->done = 1;
+            st->status = 1;
+          } else if (!isfinite(rn)) {
+            st->done = 1;
+            st->status = 3;
+          } else if (st->final_refresh) {
+            st->done = 1;
+            st->status = 2;
+          }
+        }
+      }
       if (c == 0 && threadIdx.x == 0) {
         st->dr = dnr;
         st->rnorm = sqrt(dnr);
@@ -565,8 +675,11 @@ static void gb_b_drop_graphs(gbd_batch *B) {
 static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
   hipStream_t s = gb_stream();
   for (uint32_t i = 0; i < k; ++i) {
-    gb_b_spmv<<<B->cap_spmv, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists);
-    GB_LAUNCH_CHECK();
+    for (uint32_t g = 0; g < B->nseg; ++g) {
+      gb_b_spmv<<<B->cap_seg[g], GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists, g,
+                                                    B->d_st, B->d_slices);
+      GB_LAUNCH_CHECK();
+    }
     gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
     GB_LAUNCH_CHECK();
     gb_b_update<<<B->cap_elem, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists);
@@ -583,14 +696,21 @@ static uint32_t gb_b_nchunks(uint32_t n) { return (n + GB_BCHUNK - 1) / GB_BCHUN
 // Workspace of one system: byte offsets of its buffers inside one
 // allocation (each 256-byte aligned); the same function sizes and carves it.
 struct gb_b_layout {
-  size_t part1, part2, r, p, ap, d32, r32, p32, ap32, total;
+  size_t part1, part2, part3, r, p, ap, d32, r32, p32, ap32, ypart, slices, total;
 };
+
+// SpMV items of the matrix's column slices (0 when it is not split)
+static uint32_t gb_b_nwork_split(const gbd_matrix *m) {
+  uint32_t t = 0;
+  for (uint32_t r = 0; r < m->nsplit; ++r) t += gb_spmv_nwork(m->split[r]);
+  return t;
+}
 
 static gb_b_layout gb_b_plan_layout(const gbd_matrix *m, int precision) {
   const size_t n = m->nrows ? m->nrows : 1;
-  const size_t nwork = gb_spmv_nwork(m);
+  const size_t nwork = m->nsplit ? gb_b_nwork_split(m) : gb_spmv_nwork(m);
   const size_t nch = gb_b_nchunks(m->nrows);
-  const size_t np1 = (nwork * GB_WAVES_PER_BLOCK > nch ? nwork * GB_WAVES_PER_BLOCK : nch) + 1;
+  const size_t np1 = (nwork > nch ? nwork : nch) + 1;
   const size_t np2 = nch + 1;
   gb_b_layout L;
   memset(&L, 0xff, sizeof(L));
@@ -612,8 +732,34 @@ static gb_b_layout gb_b_plan_layout(const gbd_matrix *m, int precision) {
     L.p = take(n * 8);
     L.ap = take(n * 8);
   }
+  if (m->nsplit) {
+    // FP64-sized: the refresh steps of a mixed system use it in FP64
+    L.ypart = take((size_t)m->nsplit * n * 8);
+    L.slices = take((size_t)m->nsplit * sizeof(gb_bplan));
+    L.part3 = take((nch + 1) * 8);
+  }
   L.total = off;
   return L;
+}
+
+static gb_bplan gb_b_plan_of(const gbd_matrix *m, uint32_t base) {
+  gb_bplan P;
+  memset(&P, 0, sizeof(P));
+  P.rowptr = m->rowptr;
+  P.rowind = m->rowind;
+  P.blk = m->blk;
+  P.wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
+  P.block_rows = m->rowbins + (m->nrows - m->nblockrows);
+  P.val = m->val;
+  P.val32 = m->val32;
+  P.nstream = m->nstream;
+  P.nwave = m->nwave;
+  P.nwaveblocks = gb_spmv_nwaveblocks(m);
+  P.nblockrows = m->nblockrows;
+  P.nwork = gb_spmv_nwork(m);
+  P.exact = m->params.exact;
+  P.base = base;
+  return P;
 }
 
 extern "C" {
@@ -632,12 +778,15 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
   B->delta = delta > 0 && delta < 1 ? delta : 0.05;
   B->used = (uint8_t *)calloc(max_slots, 1);
   B->slot_mem = (void **)calloc(max_slots, sizeof(void *));
+  B->h_slice_nwork = (uint32_t *)calloc((size_t)max_slots * GB_SPLIT_MAX, sizeof(uint32_t));
   B->h_sys = (gb_bsys *)calloc(max_slots, sizeof(gb_bsys));
   hipError_t e = gb_dev_alloc((void **)&B->d_sys, max_slots * sizeof(gb_bsys));
   if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_st, max_slots * sizeof(gbd_bstate));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&B->h_st, max_slots * sizeof(gbd_bstate), hipHostMallocDefault);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_lists, sizeof(gb_blists));
+  if (e == hipSuccess)
+    e = gb_dev_alloc((void **)&B->d_slices, (size_t)max_slots * GB_SPLIT_MAX * sizeof(gb_bplan));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&B->h_lists, sizeof(gb_blists), hipHostMallocDefault);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_sysl, max_slots * sizeof(uint32_t));
@@ -648,7 +797,7 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
     e = hipMemsetAsync(B->d_st, 0, max_slots * sizeof(gbd_bstate), gb_stream());
   if (e == hipSuccess)
     e = hipMemsetAsync(B->d_sys, 0, max_slots * sizeof(gb_bsys), gb_stream());
-  if (e != hipSuccess || !B->used || !B->slot_mem || !B->h_sys) {
+  if (e != hipSuccess || !B->used || !B->slot_mem || !B->h_sys || !B->h_slice_nwork) {
     gbd_batch_destroy(B);
     return e != hipSuccess ? gb_hip_fail(e, "gbd_batch_create", __FILE__, __LINE__)
                            : GOOSEBERRY_ERROR_NOMEMORY;
@@ -666,6 +815,7 @@ void gbd_batch_destroy(gbd_batch *B) {
   gb_dev_free(B->d_sys);
   gb_dev_free(B->d_st);
   gb_dev_free(B->d_lists);
+  gb_dev_free(B->d_slices);
   gb_dev_free(B->d_spmv);
   gb_dev_free(B->d_elem);
   gb_dev_free(B->d_sysl);
@@ -676,6 +826,7 @@ void gbd_batch_destroy(gbd_batch *B) {
   if (B->h_sysl) hipHostFree(B->h_sysl);
   free(B->used);
   free(B->slot_mem);
+  free(B->h_slice_nwork);
   free(B->h_sys);
   free(B);
 }
@@ -692,28 +843,19 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   uint32_t slot = 0;
   while (slot < B->max_slots && B->used[slot]) ++slot;
   if (slot == B->max_slots) return GOOSEBERRY_ERROR_NOMEMORY;
-  const uint32_t nwork = gb_spmv_nwork(m);
+  if (precision == GOOSEBERRY_PRECISION_MIXED && m->nsplit && !m->sval32)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
   const uint32_t nch = gb_b_nchunks(m->nrows);
   const gb_b_layout L = gb_b_plan_layout(m, precision);
   uint8_t *mem = nullptr;
   GB_HIP(gb_dev_alloc((void **)&mem, L.total));
   gb_bsys S;
   memset(&S, 0, sizeof(S));
-  S.rowptr = m->rowptr;
-  S.rowind = m->rowind;
-  S.blk = m->blk;
-  S.wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
-  S.block_rows = m->rowbins + (m->nrows - m->nblockrows);
-  S.val = m->val;
-  S.val32 = m->val32;
+  S.pl = gb_b_plan_of(m, 0);
   S.n = m->nrows;
-  S.nstream = m->nstream;
-  S.nwave = m->nwave;
-  S.nwaveblocks = gb_spmv_nwaveblocks(m);
-  S.nblockrows = m->nblockrows;
-  S.nwork = nwork;
   S.nchunks = nch;
-  S.exact = m->params.exact;
+  S.nsplit = m->nsplit;
+  S.nwork_split = gb_b_nwork_split(m);
   S.b = d_b;
   S.x = d_x;
   S.part1 = (double *)(mem + L.part1);
@@ -731,6 +873,35 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   S.st = B->d_st + slot;
   S.precision = precision;
   hipStream_t s = gb_stream();
+  gb_bplan *hsl = nullptr;
+  if (m->nsplit) {
+    // the slices' plans, their items numbered after the matrix's own
+    S.ypart = mem + L.ypart;
+    S.slices = (const gb_bplan *)(mem + L.slices);
+    S.part3 = (double *)(mem + L.part3);
+    hsl = (gb_bplan *)calloc(m->nsplit, sizeof(gb_bplan));
+    if (!hsl) {
+      gb_dev_free(mem);
+      return GOOSEBERRY_ERROR_NOMEMORY;
+    }
+    uint32_t base = 0;
+    for (uint32_t r = 0; r < m->nsplit; ++r) {
+      hsl[r] = gb_b_plan_of(m->split[r], base);
+      base += hsl[r].nwork;
+      B->h_slice_nwork[(size_t)slot * GB_SPLIT_MAX + r] = hsl[r].nwork;
+    }
+    hipError_t e = hipMemcpyAsync(mem + L.slices, hsl, m->nsplit * sizeof(gb_bplan),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(B->d_slices + (size_t)slot * GB_SPLIT_MAX, hsl,
+                         m->nsplit * sizeof(gb_bplan), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    free(hsl);
+    if (e != hipSuccess) {
+      gb_dev_free(mem);
+      return gb_hip_fail(e, "gbd_batch_add slices", __FILE__, __LINE__);
+    }
+  }
   B->h_sys[slot] = S;
   GB_HIP(hipMemcpyAsync(B->d_sys + slot, &B->h_sys[slot], sizeof(gb_bsys),
                         hipMemcpyHostToDevice, s));
@@ -764,7 +935,7 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   uint32_t nsys = 0;
   for (uint32_t i = 0; i < B->max_slots; ++i) {
     if (!B->used[i] || (active && !active[i])) continue;
-    ns += B->h_sys[i].nwork;
+    ns += B->h_sys[i].nsplit ? B->h_sys[i].nwork_split : B->h_sys[i].pl.nwork;
     ne += B->h_sys[i].nchunks;
     ++nsys;
   }
@@ -791,15 +962,50 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
     B->cap_elem = cap;
     gb_b_drop_graphs(B);
   }
-  uint64_t a = 0, c = 0;
-  uint32_t q = 0;
+  // SpMV segments: slice k of every split matrix in segment k, the other
+  // matrices in segment 0
+  uint32_t nseg = 1;
+  uint64_t cnt[GB_SPLIT_MAX] = {0};
   for (uint32_t i = 0; i < B->max_slots; ++i) {
     if (!B->used[i] || (active && !active[i])) continue;
-    for (uint32_t w = 0; w < B->h_sys[i].nwork; ++w) {
-      B->h_spmv[2 * a] = i;
-      B->h_spmv[2 * a + 1] = w;
-      ++a;
+    const gb_bsys &S = B->h_sys[i];
+    if (S.nsplit) {
+      if (S.nsplit > nseg) nseg = S.nsplit;
+      for (uint32_t r = 0; r < S.nsplit; ++r)
+        cnt[r] += B->h_slice_nwork[(size_t)i * GB_SPLIT_MAX + r];
+    } else {
+      cnt[0] += S.pl.nwork;
     }
+  }
+  bool regrid = nseg != B->nseg;
+  for (uint32_t g = 0; g < nseg; ++g) regrid = regrid || resize(cnt[g], B->cap_seg[g]);
+  if (regrid) {
+    for (uint32_t g = 0; g < GB_SPLIT_MAX; ++g)
+      B->cap_seg[g] = g < nseg ? (uint32_t)(cnt[g] + cnt[g] / 2 + GB_BLIST_MIN) : 0;
+    B->nseg = nseg;
+    gb_b_drop_graphs(B);  // the graphs hold the launch grids
+  }
+  uint64_t a = 0, c = 0;
+  uint32_t q = 0;
+  for (uint32_t g = 0; g < nseg; ++g) {
+    B->h_lists->seg[g] = (uint32_t)a;
+    for (uint32_t i = 0; i < B->max_slots; ++i) {
+      if (!B->used[i] || (active && !active[i])) continue;
+      const gb_bsys &S = B->h_sys[i];
+      uint32_t nw = 0;
+      if (S.nsplit) nw = g < S.nsplit ? B->h_slice_nwork[(size_t)i * GB_SPLIT_MAX + g] : 0;
+      else if (g == 0) nw = S.pl.nwork;
+      for (uint32_t w = 0; w < nw; ++w) {
+        B->h_spmv[2 * a] = i;
+        B->h_spmv[2 * a + 1] = w;
+        ++a;
+      }
+    }
+  }
+  B->h_lists->seg[nseg] = (uint32_t)a;
+  B->h_lists->nseg = nseg;
+  for (uint32_t i = 0; i < B->max_slots; ++i) {
+    if (!B->used[i] || (active && !active[i])) continue;
     for (uint32_t k = 0; k < B->h_sys[i].nchunks; ++k) {
       B->h_elem[2 * c] = i;
       B->h_elem[2 * c + 1] = k;

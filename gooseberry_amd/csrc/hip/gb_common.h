@@ -109,4 +109,17 @@ __device__ __forceinline__ uint32_t gb_xcd_remap(uint32_t b, uint32_t n) {
   return x * q + (x < r ? x : r) + local;
 }
 
+// Chunked variant: chunks of C consecutive indices go round-robin to the
+// XCDs, each XCD walking its chunks in order.  Neighbouring indices share an
+// XCD (and its L2) within a chunk, while the eight XCDs still work on
+// nearby addresses.  A bijection on [0, n): the tail past the last whole
+// round of 8 chunks keeps its order.
+__device__ __forceinline__ uint32_t gb_xcd_chunk_remap(uint32_t b, uint32_t n, uint32_t C) {
+  const uint32_t nx = 8;
+  const uint32_t full = (n / (nx * C)) * (nx * C);
+  if (b >= full) return b;
+  const uint32_t x = b % nx, l = b / nx;
+  return ((l / C) * nx + x) * C + l % C;
+}
+
 #endif

@@ -153,7 +153,9 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     if (w < nstream) {
       // ---- LDS-staged row block of short rows
-      const uint32_t lb = remap ? gb_xcd_remap(w, nstream) : w;
+      const uint32_t lb = remap == 0 ? w
+                          : remap == 1 ? gb_xcd_remap(w, nstream)
+                                       : gb_xcd_chunk_remap(w, nstream, remap);
       const uint32_t r0 = blk[2 * lb], r1 = blk[2 * lb + 1];
       const uint32_t k0 = rowptr[r0];
       const uint32_t cnt = rowptr[r1] - k0;
@@ -437,6 +439,7 @@ static int gb_plan_check(const gbd_plan_params *p) {
   if (p->rowcap > 2 * GB_BLOCK) return 0;
   if (p->grid > 65536) return 0;
   if (p->csplit > GB_SPLIT_MAX) return 0;
+  if (p->xcd_remap > 4096) return 0;
   // slices change the summation order: not with the bitwise plan
   if (p->csplit > 1 && p->exact) return 0;
   return 1;

@@ -133,7 +133,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   if (p->rowcap) d.rowcap = p->rowcap;
   d.exact = p->exact ? 1 : 0;
   if (p->grid) d.grid = p->grid;
-  d.xcd_remap = p->xcd_remap ? 1 : 0;
+  d.xcd_remap = p->xcd_remap;
   d.csplit = p->csplit > 1 ? p->csplit : 0;
   *out = d;
   *have = 1;

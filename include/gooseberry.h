@@ -242,7 +242,9 @@ typedef struct gooseberry_plan_t {
                        +0.0 (bitwise the reference for every row); 0: rows
                        longer than short_max use tree reductions */
   dim_t grid;       /* cap on SpMV workgroups (0: one per row block) */
-  dim_t xcd_remap;  /* 1: each XCD walks a contiguous range of blocks */
+  dim_t xcd_remap;  /* 1: each XCD walks a contiguous range of blocks;
+                       C >= 2: chunks of C consecutive blocks go to the
+                       XCDs in turn (same results, another order) */
   dim_t csplit;     /* > 1: split the columns into this many slices, one
                        launch per slice into a partial vector, partials
                        added after; keeps the gathered slice of x in L2 for

@@ -289,3 +289,27 @@ def test_autotune_keeps_results(gb, oracle, precision):
     (xs, info), = solve_batch(gb, [(M, b)], 1e-10, precision)
     assert info.status == 1
     assert true_rel(oracle, rp, ri, rv, xs, b) <= 1.05e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_split_matrix_in_batch_alone_and_mix(gb, oracle, precision):
+    # a column-split matrix runs its CG steps over its slices (refresh steps
+    # over its own plan); same bits alone and in a mix, residual met
+    M0 = gb.generate_sdd(80000, 31, 3, 27)
+    rp, ri, rv = M0.download()
+    Ms = gb.Matrix(rp, ri, rv, plan={"csplit": 6})
+    rng = np.random.default_rng(7)
+    b = rng.uniform(-1, 1, 80000)
+    (x_alone, i_alone), = solve_batch(gb, [(Ms, b)], 1e-10, precision)
+    assert i_alone.status == 1
+    assert true_rel(oracle, rp, ri, rv, x_alone, b) <= 1.05e-10
+    others = [lap(gb, 2, 40)[0], gb.generate_sdd(30000, 32)]
+    mix = [(M, rng.uniform(-1, 1, M.nrows)) for M in others]
+    mix.insert(1, (Ms, b))
+    res = solve_batch(gb, mix, 1e-10, precision, order=[2, 1, 0])
+    x_mix, i_mix = res[1]
+    assert i_mix.niter == i_alone.niter and np.array_equal(x_mix, x_alone)
+    # iterations close to the unsplit plan's (FP32 slice sums of a few terms
+    # round less than whole-row FP32 sums, which can save mixed iterations)
+    (x_u, i_u), = solve_batch(gb, [(M0, b)], 1e-10, precision)
+    assert abs(int(i_u.niter) - int(i_alone.niter)) <= max(3, 0.15 * int(i_u.niter))
