@@ -65,6 +65,7 @@ EXPORTED = [
     "gooseberry_matrix_download", "gooseberry_generate_uniform_device",
     "gooseberry_matrix_generate_spd", "gooseberry_matrix_autotune",
     "gooseberry_add_scalar", "gooseberry_matrix_pagerank", "gooseberry_pagerank",
+    "gooseberry_spmultsp", "gooseberry_spmultsp_sp",
 ]
 
 PRECISION_FP64 = 0
@@ -136,6 +137,10 @@ def _sig(lib):
     s = {
         "gooseberry_scale": (i32, [u32, _P, f64]),
         "gooseberry_spmult": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, u32]),
+        "gooseberry_spmultsp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, _P, _P, u32]),
+        "gooseberry_spmultsp_sp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P,
+                                         C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                                         _P, u32]),
         "gooseberry_read_sparse_matrix": (i32, [i32, C.c_char_p, _U32P, _U32P,
                                                 C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
         "gooseberry_read_dense_matrix": (i32, [i32, C.c_char_p, _U32P, _U32P, C.POINTER(_P)]),
@@ -462,6 +467,38 @@ def spmult(anrows, ancols, bncols, rowptr, rowind, rowval, bcolval, blocks=None,
                                   _ptr(rowval), _ptr(bcolval), _ptr(crowval),
                                   _ptr(blocks), nblocks), "gooseberry_spmult")
     return crowval
+
+
+def spmultsp(anrows, ancols, bncols, a, b, blocks=None, crowval=None):
+    """gooseberry_spmultsp: a and b are (rowptr, rowind, rowval) CSR triples
+    (b has ancols rows); returns the dense row-major product."""
+    arp, ari, arv = _u32(a[0]), _u32(a[1]), _f64(a[2])
+    brp, bri, brv = _u32(b[0]), _u32(b[1]), _f64(b[2])
+    if crowval is None:
+        crowval = np.zeros(anrows * bncols, dtype=np.float64)
+    blocks = _u32(blocks)
+    nblocks = 0 if blocks is None else blocks.size - 1
+    check(lib().gooseberry_spmultsp(anrows, ancols, bncols, _ptr(arp), _ptr(ari), _ptr(arv),
+                                    _ptr(brp), _ptr(bri), _ptr(brv), _ptr(crowval),
+                                    _ptr(blocks), nblocks), "gooseberry_spmultsp")
+    return crowval
+
+
+def spmultsp_sp(anrows, ancols, bncols, a, b, blocks=None):
+    """gooseberry_spmultsp_sp: the sparse product as (rowptr, rowind, rowval),
+    each row's columns in the order of their first product."""
+    arp, ari, arv = _u32(a[0]), _u32(a[1]), _f64(a[2])
+    brp, bri, brv = _u32(b[0]), _u32(b[1]), _f64(b[2])
+    blocks = _u32(blocks)
+    nblocks = 0 if blocks is None else blocks.size - 1
+    rp, ri, rv = _P(), _P(), _P()
+    check(lib().gooseberry_spmultsp_sp(anrows, ancols, bncols, _ptr(arp), _ptr(ari),
+                                       _ptr(arv), _ptr(brp), _ptr(bri), _ptr(brv),
+                                       C.byref(rp), C.byref(ri), C.byref(rv),
+                                       _ptr(blocks), nblocks), "gooseberry_spmultsp_sp")
+    rowptr = _take(rp, anrows + 1, np.uint32)
+    nnz = int(rowptr[-1])
+    return rowptr, _take(ri, nnz, np.uint32), _take(rv, nnz, np.float64)
 
 
 def scale(val, s):

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
 HIP_SOURCES = ["gb_runtime.hip", "gb_index.hip", "gb_spmv.hip", "gb_cg.hip",
-               "gb_batch.hip", "gb_gen.hip", "gb_pagerank.hip"]
+               "gb_batch.hip", "gb_gen.hip", "gb_pagerank.hip", "gb_spgemm.hip"]
 C_SOURCES = ["gb_api.c", "gb_io.c", "gb_gen.c", "gb_batch_api.c", "gb_tune.c"]
 
 HIP_FLAGS = [

@@ -232,6 +232,19 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
     double minerror, double *d_out, uint64_t *r_iters, double *r_rmse,
     uint32_t *r_nsinks);
 
+/* C = A B for CSR A (anrows x ancols) and CSR B (ancols x bncols), each
+ * element summed in the reference's scan order (src/blas.c:94-234).
+ * d_cdense non-NULL: C written there row-major (anrows x bncols).
+ * Otherwise C returned as new device CSR arrays (free with gbd_free), each
+ * row's columns in the order of their first product.  Inputs are trusted
+ * (checked by the caller); more than 2^32 - 2 products is INVALIDINPUT. */
+int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
+    const uint32_t *d_arowptr, const uint32_t *d_arowind,
+    const double *d_arowval, const uint32_t *d_browptr,
+    const uint32_t *d_browind, const double *d_browval, double *d_cdense,
+    uint32_t **d_crowptr, uint32_t **d_crowind, double **d_crowval,
+    uint32_t *r_cnnz);
+
 /* d_out[i] = lo + (hi - lo) u, u from a counter-based hash of (seed,
  * stream, i) */
 int gbd_fill_uniform(double *d_out, size_t n, uint64_t seed, uint64_t stream,

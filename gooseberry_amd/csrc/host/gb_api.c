@@ -355,6 +355,133 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
   return rv;
 }
 
+/* ------------------------------------------------------ sparse x sparse */
+
+/* blocks given: monotone row ranges; the dense product writes rows
+ * [blocks[0], blocks[nblocks]) as the reference's block loop does */
+static int spgemm_blocks(dim_t anrows, const dim_t *blocks, dim_t nblocks,
+    dim_t *rlo, dim_t *rhi)
+{
+  *rlo = 0;
+  *rhi = anrows;
+  if (!blocks) return GOOSEBERRY_SUCCESS;
+  for (dim_t c = 0; c < nblocks; ++c) {
+    if (blocks[c + 1] < blocks[c]) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (nblocks && blocks[nblocks] > anrows) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  *rlo = nblocks ? blocks[0] : 0;
+  *rhi = nblocks ? blocks[nblocks] : 0;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* A (anrows x ancols) and B (ancols x bncols) checked and uploaded */
+static int spgemm_upload(dim_t anrows, dim_t ancols, dim_t bncols,
+    const ind_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const ind_t *browptr, const dim_t *browind, const real_t *browval,
+    void *d[6])
+{
+  int rv = gb_check_csr(anrows, ancols, arowptr, arowind);
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gb_check_csr(ancols, bncols, browptr, browind);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  const size_t annz = arowptr[anrows], bnnz = browptr[ancols];
+  if ((annz && !arowval) || (bnnz && !browval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  rv = gb_upload(&d[0], arowptr, ((size_t)anrows + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&d[1], arowind, annz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&d[2], arowval, annz * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gb_upload(&d[3], browptr, ((size_t)ancols + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&d[4], browind, bnnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&d[5], browval, bnnz * sizeof(real_t));
+  return rv;
+}
+
+static void spgemm_free(void *d[6])
+{
+  for (int i = 0; i < 6; ++i) gbd_free(d[i]);
+}
+
+int gooseberry_spmultsp(dim_t anrows, dim_t ancols, dim_t bncols,
+    const ind_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const ind_t *browptr, const dim_t *browind, const real_t *browval,
+    real_t *crowval, const dim_t *blocks, dim_t nblocks)
+{
+  dim_t rlo, rhi;
+  int rv = spgemm_blocks(anrows, blocks, nblocks, &rlo, &rhi);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
+  if (!crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *d[6] = {NULL, NULL, NULL, NULL, NULL, NULL}, *dc = NULL;
+  rv = spgemm_upload(anrows, ancols, bncols, arowptr, arowind, arowval,
+      browptr, browind, browval, d);
+  const size_t cbytes = (size_t)anrows * bncols * sizeof(real_t);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dc, cbytes);
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_spgemm(anrows, ancols, bncols, d[0], d[1], d[2], d[3], d[4],
+        d[5], dc, NULL, NULL, NULL, NULL);
+  if (rv == GOOSEBERRY_SUCCESS) {
+    const size_t off = (size_t)rlo * bncols;
+    rv = gbd_dtoh(crowval + off, (real_t *)dc + off,
+        (size_t)(rhi - rlo) * bncols * sizeof(real_t));
+  }
+  gbd_free(dc);
+  spgemm_free(d);
+  return rv;
+}
+
+int gooseberry_spmultsp_sp(dim_t anrows, dim_t ancols, dim_t bncols,
+    const ind_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const ind_t *browptr, const dim_t *browind, const real_t *browval,
+    ind_t **r_crowptr, dim_t **r_crowind, real_t **r_crowval,
+    const dim_t *blocks, dim_t nblocks)
+{
+  if (!r_crowptr || !r_crowind || !r_crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* the reference fills every row whatever the blocks, and its row
+   * pointers are only whole when the blocks partition the rows */
+  if (blocks) {
+    dim_t rlo, rhi;
+    int brv = spgemm_blocks(anrows, blocks, nblocks, &rlo, &rhi);
+    if (brv != GOOSEBERRY_SUCCESS) return brv;
+    if (rlo != 0 || rhi != anrows) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  void *d[6] = {NULL, NULL, NULL, NULL, NULL, NULL};
+  uint32_t *dptr = NULL, *dind = NULL;
+  double *dval = NULL;
+  uint32_t cnnz = 0;
+  ind_t *cptr = NULL;
+  dim_t *cind = NULL;
+  real_t *cval = NULL;
+  int rv = spgemm_upload(anrows, ancols, bncols, arowptr, arowind, arowval,
+      browptr, browind, browval, d);
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_spgemm(anrows, ancols, bncols, d[0], d[1], d[2], d[3], d[4],
+        d[5], NULL, &dptr, &dind, &dval, &cnnz);
+  if (rv == GOOSEBERRY_SUCCESS) {
+    cptr = malloc(((size_t)anrows + 1) * sizeof(ind_t));
+    cind = malloc((cnnz ? cnnz : 1) * sizeof(dim_t));
+    cval = malloc((cnnz ? cnnz : 1) * sizeof(real_t));
+    if (!cptr || !cind || !cval) rv = GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtoh(cptr, dptr, ((size_t)anrows + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(cind, dind, (size_t)cnnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(cval, dval, (size_t)cnnz * sizeof(real_t));
+  gbd_free(dptr);
+  gbd_free(dind);
+  gbd_free(dval);
+  spgemm_free(d);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    free(cptr);
+    free(cind);
+    free(cval);
+    return rv;
+  }
+  *r_crowptr = cptr;
+  *r_crowind = cind;
+  *r_crowval = cval;
+  return GOOSEBERRY_SUCCESS;
+}
+
 /* ------------------------------------------------------------------- CG */
 
 /* Iterations are enqueued in batches replayed from captured hipGraphs and

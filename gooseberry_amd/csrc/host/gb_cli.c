@@ -222,6 +222,87 @@ static const opt_t BLAS_OPTS[] = {
 };
 #define NBLAS_OPTS (sizeof(BLAS_OPTS) / sizeof(BLAS_OPTS[0]))
 
+/* sparse x sparse (src/gooseberry_bin.c:1727-1750): the output file's
+ * type picks the dense (blas_spmultsp) or sparse (blas_spmultsp_sp)
+ * product; rows are unpermuted afterwards as matrix_unpermute does
+ * (src/matrix.c:348-378) */
+static int blas_sparse_sparse(mat_t *a, mat_t *b, const char *outfile,
+    const dim_t *rperm, const dim_t *cperm, const dim_t *blocks,
+    dim_t nblocks, long long runs, int times, tmr_t *io, tmr_t *aux,
+    tmr_t *op)
+{
+  int err = GOOSEBERRY_SUCCESS;
+  const int otype = file_type(outfile);
+  const dim_t anrows = a->nrows, ancols = a->ncols, bncols = b->ncols;
+  real_t *dense = NULL;
+  ind_t *cptr = NULL;
+  dim_t *cind = NULL, *rren = NULL;
+  real_t *cval = NULL;
+  if (cperm) {
+    if (b->nrows != ancols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    err = gooseberry_permute_sparse(b->nrows, b->ncols, b->rowptr, b->rowind,
+        b->rowval, (dim_t *)cperm, NULL);
+    if (err != GOOSEBERRY_SUCCESS) return err;
+  }
+  if (ancols > b->nrows) {
+    fprintf(stderr, "Matrix dimensions do not match for multiplication: "
+        "%" PRIu32 "x%" PRIu32 " and %" PRIu32 "x%" PRIu32 "\n",
+        anrows, ancols, b->nrows, bncols);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (is_dense(otype)) {
+    dense = calloc((size_t)anrows * bncols ? (size_t)anrows * bncols : 1,
+        sizeof(real_t));
+    if (!dense) return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  if (times) { tmr_stop(aux); tmr_start(op); }
+  for (long long r = 0; r < runs; ++r) {
+    if (dense) {
+      err = gooseberry_spmultsp(anrows, ancols, bncols, a->rowptr, a->rowind,
+          a->rowval, b->rowptr, b->rowind, b->rowval, dense, blocks, nblocks);
+    } else {
+      free(cptr); free(cind); free(cval);
+      cptr = NULL; cind = NULL; cval = NULL;
+      err = gooseberry_spmultsp_sp(anrows, ancols, bncols, a->rowptr,
+          a->rowind, a->rowval, b->rowptr, b->rowind, b->rowval, &cptr, &cind,
+          &cval, blocks, nblocks);
+    }
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (times) { tmr_stop(op); tmr_start(aux); }
+  if (rperm) {
+    if (dense) {
+      err = gooseberry_permute_dense(anrows, bncols, dense, (dim_t *)rperm,
+          NULL, 1);
+    } else {
+      rren = malloc(sizeof(dim_t) * (anrows ? anrows : 1));
+      if (!rren) { err = GOOSEBERRY_ERROR_NOMEMORY; goto END; }
+      for (dim_t i = 0; i < anrows; ++i) rren[rperm[i]] = i;
+      err = gooseberry_permute_sparse(anrows, bncols, cptr, cind, cval, rren,
+          NULL);
+    }
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (times) { tmr_stop(aux); tmr_start(io); }
+  if (dense) {
+    err = gooseberry_write_dense_matrix(GOOSEBERRY_FORMAT_GRID, outfile,
+        anrows, bncols, dense);
+  } else {
+    err = gooseberry_write_sparse_matrix(otype, outfile, anrows, bncols, cptr,
+        cind, cval);
+  }
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) {
+    tmr_stop(io);
+    const char *names[] = {"I/O", "Auxillary", "Compute"};
+    double vals[] = {io->acc, aux->acc, op->acc};
+    print_times(names, vals, 3);
+  }
+END:
+  free(dense); free(cptr); free(cind); free(cval); free(rren);
+  return err;
+}
+
 static int cmd_blas(int argc, char **argv)
 {
   arg_t args[512];
@@ -300,10 +381,20 @@ static int cmd_blas(int argc, char **argv)
     err = GOOSEBERRY_ERROR_INVALIDINPUT;
     goto END;
   }
-  if (in[0].dense || !in[1].dense) {
-    fprintf(stderr, "This build multiplies a sparse matrix by a dense "
-        "matrix/vector only\n");
+  if (in[0].dense) {
+    fprintf(stderr, "This build multiplies a sparse first matrix only\n");
     err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    goto END;
+  }
+  if (!in[1].dense) {
+    /* matrix_permute(in[0], rperm, cperm) here, in[1] in the helper */
+    if (rperm || cperm) {
+      err = gooseberry_permute_sparse(in[0].nrows, in[0].ncols, in[0].rowptr,
+          in[0].rowind, in[0].rowval, rperm, cperm);
+      if (err != GOOSEBERRY_SUCCESS) goto END;
+    }
+    err = blas_sparse_sparse(&in[0], &in[1], outfile, rperm, cperm,
+        rpf ? rdist : NULL, rpf ? nrdist : 0, runs, times, &io, &aux, &op);
     goto END;
   }
   if (!is_dense(file_type(outfile))) {

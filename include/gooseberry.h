@@ -118,6 +118,50 @@ int gooseberry_spmult(
     const dim_t * blocks,
     dim_t nblocks);
 
+/* include/gooseberry.h:187-216; src/blas.c:609-791.  C = A B for a sparse
+ * A (anrows x ancols) and a sparse B (ancols x bncols, so browptr holds
+ * ancols + 1 entries).  Every C element is the sum of its products in the
+ * reference's scan order (row i of A entry by entry, each selecting a row
+ * of B entry by entry), so results are bitwise the reference's.
+ * gooseberry_spmultsp writes the dense row-major C into crowval: with
+ * blocks, only rows [blocks[0], blocks[nblocks]) are written (the
+ * reference writes exactly the block rows).  gooseberry_spmultsp_sp returns
+ * a new CSR C (malloc'd arrays, free with free()), each row's columns in
+ * the order of their first product as the reference's fill loop leaves
+ * them.  Its blocks, when given, must partition [0, anrows): the reference
+ * computes every row whatever the blocks and its row pointers are only
+ * whole in that case.  The product count must stay below 2^32 - 1
+ * (GOOSEBERRY_ERROR_INVALIDINPUT beyond). */
+int gooseberry_spmultsp(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const ind_t * arowptr,
+    const dim_t * arowind,
+    const real_t * arowval,
+    const ind_t * browptr,
+    const dim_t * browind,
+    const real_t * browval,
+    real_t * crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
+int gooseberry_spmultsp_sp(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const ind_t * arowptr,
+    const dim_t * arowind,
+    const real_t * arowval,
+    const ind_t * browptr,
+    const dim_t * browind,
+    const real_t * browval,
+    ind_t ** crowptr,
+    dim_t ** crowind,
+    real_t ** crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
 /* include/gooseberry.h IO prototypes; src/io.c:1464-1549.  Sparse readers:
  * GOOSEBERRY_FORMAT_CSR and GOOSEBERRY_FORMAT_MATRIXMARKET.  Dense: GRID.
  * Arrays returned through pointers are malloc'd; free with free() or

@@ -746,3 +746,88 @@ finish:
   free(b);
   return ORC_SUCCESS;
 }
+
+/* src/blas.c:94-121 (one row range), driven as src/blas.c:609-648 */
+int orc_spmultsp(dim_t anrows, dim_t ancols, dim_t bncols, const ind_t *arowptr,
+    const dim_t *arowind, const real_t *arowval, const ind_t *browptr,
+    const dim_t *browind, const real_t *browval, real_t *crowval)
+{
+  (void)ancols;
+  #pragma omp parallel for schedule(static)
+  for (dim_t i = 0; i < anrows; ++i) {
+    real_t *c = crowval + (size_t)i * bncols;
+    for (dim_t m = 0; m < bncols; ++m) c[m] = 0;
+    for (ind_t j = arowptr[i]; j < arowptr[i + 1]; ++j) {
+      const dim_t l = arowind[j];
+      const real_t x = arowval[j];
+      for (ind_t k = browptr[l]; k < browptr[l + 1]; ++k)
+        c[browind[k]] += x * browval[k];
+    }
+  }
+  return ORC_SUCCESS;
+}
+
+/* src/blas.c:124-234 (count, fill) driven as src/blas.c:651-791 without
+ * blocks; sequential here (the result does not depend on the threads) */
+int orc_spmultsp_sp(dim_t anrows, dim_t ancols, dim_t bncols,
+    const ind_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const ind_t *browptr, const dim_t *browind, const real_t *browval,
+    ind_t **r_crowptr, dim_t **r_crowind, real_t **r_crowval)
+{
+  (void)ancols;
+  ind_t *crowptr = calloc((size_t)anrows + 1, sizeof(ind_t));
+  int *row = calloc(bncols ? bncols : 1, sizeof(int));
+  dim_t *mark = malloc(sizeof(dim_t) * (bncols ? bncols : 1));
+  if (!crowptr || !row || !mark) { free(crowptr); free(row); free(mark); return ORC_ERROR_UNKNOWN; }
+  /* count */
+  for (dim_t i = 0; i < anrows; ++i) {
+    dim_t rowsize = 0;
+    for (ind_t j = arowptr[i]; j < arowptr[i + 1]; ++j) {
+      const dim_t l = arowind[j];
+      for (ind_t k = browptr[l]; k < browptr[l + 1]; ++k) {
+        const dim_t m = browind[k];
+        if (!row[m]) { row[m] = 1; mark[rowsize++] = m; }
+      }
+    }
+    crowptr[i + 1] = crowptr[i] + rowsize;
+    for (dim_t m = 0; m < rowsize; ++m) row[mark[m]] = 0;
+  }
+  free(row);
+  free(mark);
+  const ind_t nnz = crowptr[anrows];
+  dim_t *crowind = malloc(sizeof(dim_t) * (nnz ? nnz : 1));
+  real_t *crowval = malloc(sizeof(real_t) * (nnz ? nnz : 1));
+  ind_t *pos = malloc(sizeof(ind_t) * (bncols ? bncols : 1));
+  if (!crowind || !crowval || !pos) {
+    free(crowptr); free(crowind); free(crowval); free(pos);
+    return ORC_ERROR_UNKNOWN;
+  }
+  const ind_t NONE = (ind_t)-1;
+  for (dim_t m = 0; m < bncols; ++m) pos[m] = NONE;
+  /* fill */
+  for (dim_t i = 0; i < anrows; ++i) {
+    ind_t ptr = crowptr[i];
+    for (ind_t j = arowptr[i]; j < arowptr[i + 1]; ++j) {
+      const dim_t l = arowind[j];
+      const real_t x = arowval[j];
+      for (ind_t k = browptr[l]; k < browptr[l + 1]; ++k) {
+        const dim_t m = browind[k];
+        ind_t q = pos[m];
+        if (q == NONE) {
+          q = ptr++;
+          crowind[q] = m;
+          crowval[q] = x * browval[k];
+          pos[m] = q;
+        } else {
+          crowval[q] += x * browval[k];
+        }
+      }
+    }
+    for (ind_t q = crowptr[i]; q < crowptr[i + 1]; ++q) pos[crowind[q]] = NONE;
+  }
+  free(pos);
+  *r_crowptr = crowptr;
+  *r_crowind = crowind;
+  *r_crowval = crowval;
+  return ORC_SUCCESS;
+}

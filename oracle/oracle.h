@@ -111,6 +111,25 @@ int orc_permute_dense(orc_dim_t nrows, orc_dim_t ncols, orc_real_t *rowval,
 int orc_permute_dense_rev(orc_dim_t nrows, orc_dim_t ncols,
     orc_real_t *rowval, const orc_dim_t *rowperm, const orc_dim_t *colperm);
 
+/* src/blas.c:94-121 + 609-648: C (dense, row-major anrows x bncols) = A B
+ * with A and B sparse; each C element starts at 0 and adds the products in
+ * the scan order of A's row and B's rows. */
+int orc_spmultsp(orc_dim_t anrows, orc_dim_t ancols, orc_dim_t bncols,
+    const orc_ind_t *arowptr, const orc_dim_t *arowind,
+    const orc_real_t *arowval, const orc_ind_t *browptr,
+    const orc_dim_t *browind, const orc_real_t *browval,
+    orc_real_t *crowval);
+
+/* src/blas.c:124-234 + 651-791: sparse C = A B (two passes, count and
+ * fill); the columns of a C row in order of first appearance in the scan,
+ * each value the first product plus the later ones in scan order.  The
+ * three arrays are malloc'd (free with orc_free). */
+int orc_spmultsp_sp(orc_dim_t anrows, orc_dim_t ancols, orc_dim_t bncols,
+    const orc_ind_t *arowptr, const orc_dim_t *arowind,
+    const orc_real_t *arowval, const orc_ind_t *browptr,
+    const orc_dim_t *browind, const orc_real_t *browval,
+    orc_ind_t **crowptr, orc_dim_t **crowind, orc_real_t **crowval);
+
 /* src/blas.c:397-411: v[i] += s */
 int orc_add_scalar(orc_ind_t n, orc_real_t *v, orc_real_t s);
 

@@ -52,6 +52,11 @@ def lib():
         L.orc_permute_dense.restype = i32
         L.orc_permute_dense_rev.argtypes = [u32, u32, _P, _P, _P]
         L.orc_permute_dense_rev.restype = i32
+        L.orc_spmultsp.argtypes = [u32, u32, u32, _P, _P, _P, _P, _P, _P, _P]
+        L.orc_spmultsp.restype = i32
+        L.orc_spmultsp_sp.argtypes = [u32, u32, u32, _P, _P, _P, _P, _P, _P,
+                                      C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]
+        L.orc_spmultsp_sp.restype = i32
         L.orc_add_scalar.argtypes = [u32, _P, f64]
         L.orc_add_scalar.restype = i32
         L.orc_pagerank_prepare.argtypes = [u32, u32, _P, _P, _P, _P, C.POINTER(u32)]
@@ -231,3 +236,29 @@ def pagerank(rowptr, rowind, nval, sinks, damping=0.85, niter=0, minerror=0.0):
     lib().orc_pagerank(n, _p(rowptr), _p(rowind), _p(nval), _p(sinks), sinks.size,
                        damping, niter, minerror, _p(out), C.byref(it), C.byref(err))
     return out, it.value, err.value
+
+
+def spmultsp(a, b, bncols):
+    """Dense C = A B for sparse A, B given as (rowptr, rowind, rowval)."""
+    (arp, ari, arv), (brp, bri, brv) = a, b
+    arp, ari, arv = _u32(arp), _u32(ari), _f64(arv)
+    brp, bri, brv = _u32(brp), _u32(bri), _f64(brv)
+    n = arp.size - 1
+    out = np.zeros(n * bncols, dtype=np.float64)
+    lib().orc_spmultsp(n, brp.size - 1, bncols, _p(arp), _p(ari), _p(arv), _p(brp),
+                       _p(bri), _p(brv), _p(out))
+    return out
+
+
+def spmultsp_sp(a, b, bncols):
+    """Sparse C = A B: (rowptr, rowind, rowval) in the reference's order."""
+    (arp, ari, arv), (brp, bri, brv) = a, b
+    arp, ari, arv = _u32(arp), _u32(ari), _f64(arv)
+    brp, bri, brv = _u32(brp), _u32(bri), _f64(brv)
+    n = arp.size - 1
+    cp, ci, cv = _P(), _P(), _P()
+    lib().orc_spmultsp_sp(n, brp.size - 1, bncols, _p(arp), _p(ari), _p(arv), _p(brp),
+                          _p(bri), _p(brv), C.byref(cp), C.byref(ci), C.byref(cv))
+    rowptr = _take(cp, n + 1, C.c_uint32, np.uint32)
+    nnz = int(rowptr[-1])
+    return rowptr, _take(ci, nnz, C.c_uint32, np.uint32), _take(cv, nnz, C.c_double, np.float64)
