@@ -28,6 +28,11 @@ timed on a bounded sample on this host).
            memory admission, mixed precision to 1e-10; p50/p99 latency, and
            the capacity on a backlog of the same requests against the plain
            one-at-a-time FP64 GPU CG
+  spgemm   C = A A for the 3D 7-point Laplacian on a --spgemm-grid^3 grid
+           (default 150: n = 3,375,000, 164 M products, 83 M entries of C),
+           resident A in, resident C out (CSR, columns in first-appearance
+           order, bitwise the reference's sums); the oracle's sequential
+           product on the same matrix as the CPU baseline
 """
 import argparse
 import json
@@ -568,6 +573,78 @@ def run_served(args, gb):
                 roofline=None, cpu_baseline=None)
 
 
+# ------------------------------------------------ sparse x sparse (SpGEMM)
+
+def spgemm_counts(rp, ri):
+    """Number of products of A A: each entry (i, l) of A meets row l."""
+    lens = np.diff(rp.astype(np.int64))
+    return int(lens[ri.astype(np.int64)].sum())
+
+
+def run_spgemm(args, gb):
+    g = args.spgemm_grid
+    rp, ri, rv = gb.laplacian(3, g)
+    n, nnz = rp.size - 1, int(rp[-1])
+    products = spgemm_counts(rp, ri)
+    A = gb.Matrix(rp, ri, rv)
+    e0, e1 = gb.Event(), gb.Event()
+    nnz_c = None
+    for _ in range(args.warmup):
+        Cm = A.spmultsp(A)
+        nnz_c = Cm.nnz
+        Cm.free()
+    gb.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0.record()
+        Cm = A.spmultsp(A)
+        e1.record()
+        dev_ms.append(e0.elapsed_ms(e1))
+        nnz_c = Cm.nnz
+        Cm.free()
+    gb.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    ab = 2 * (12 * nnz + 4 * (n + 1)) + 12 * nnz_c + 4 * (n + 1)
+    achieved = ab / (ms * 1e-3) / 1e9
+    extra = {
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+            "kernel": "whole product (expansion, two stable radix sorts, run sums, "
+                      "first-appearance sort, output, plan of C)",
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": float(np.mean(dev_ms)),
+        },
+        "products": products, "nnz_c": nnz_c,
+        "products_per_s": products / (ms * 1e-3),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        orc.set_threads(1)
+        t = time.perf_counter()
+        k = 0
+        while True:
+            orc.spmultsp_sp((rp, ri, rv), (rp, ri, rv), n)
+            k += 1
+            if time.perf_counter() - t > args.cpu_seconds or k >= 5:
+                break
+        t = (time.perf_counter() - t) / k
+        cpu = {"value": 1.0 / t, "unit": "spgemm/s", "cores": 1, "kind": "port",
+               "sample": "%d sequential products A A of the same %d^3 Laplacian by the "
+                         "oracle (C restatement of src/blas.c:124-234, count then fill, "
+                         "-O3, no FMA) on one core" % (k, g),
+               "products_per_s": products / t}
+    extra["cpu_baseline"] = cpu
+    return line(args.steps / elapsed * 1.0, "spgemm/s", args, ms, "f64",
+                {"workload": "C = A A, 3D 7-point Laplacian %d^3 (n=%d, nnz=%d), %d products, "
+                             "C nnz %d; resident A in, resident C out" % (g, n, nnz, products,
+                                                                          nnz_c),
+                 "n": n, "nnz": nnz}, **extra)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -580,10 +657,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps (default 2; zipf 1; batched 1)")
     ap.add_argument("--workload", default="laplacian",
-                    choices=["laplacian", "zipf", "batched", "served"])
+                    choices=["laplacian", "zipf", "batched", "served", "spgemm"])
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
+    ap.add_argument("--spgemm-grid", type=int, default=150)
     ap.add_argument("--duration", type=float, default=60.0,
                     help="served: seconds of Poisson arrivals")
     ap.add_argument("--rate", type=float, default=200.0,
@@ -599,10 +677,11 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build is single-GPU (the path "
-              "would shard by row blocks with a halo exchange; see DESIGN.md). "
-              "Run with --gpus 1.", file=sys.stderr)
+              "would shard by row blocks with a halo exchange). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
-    defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0)}
+    defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0),
+                "spgemm": (5, 1)}
     st, wu = defaults[args.workload]
     args.steps = st if args.steps is None else args.steps
     args.warmup = wu if args.warmup is None else args.warmup
@@ -610,7 +689,7 @@ def main():
     import gooseberry_amd as gb
 
     run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
-           "served": run_served}[args.workload]
+           "served": run_served, "spgemm": run_spgemm}[args.workload]
     out = run(args, gb)
     print(json.dumps(out), flush=True)
 

@@ -34,6 +34,7 @@ ERROR_CODES = {v: k for k, v in ERRORS.items()}
 FORMAT_GRID = 1
 FORMAT_CSR = 2
 FORMAT_MATRIXMARKET = 7
+FORMAT_BCSR = 8
 
 TOL_ABSOLUTE = 0
 TOL_RELATIVE = 1
@@ -65,7 +66,9 @@ EXPORTED = [
     "gooseberry_matrix_download", "gooseberry_generate_uniform_device",
     "gooseberry_matrix_generate_spd", "gooseberry_matrix_autotune",
     "gooseberry_add_scalar", "gooseberry_matrix_pagerank", "gooseberry_pagerank",
-    "gooseberry_spmultsp", "gooseberry_spmultsp_sp",
+    "gooseberry_spmultsp", "gooseberry_spmultsp_sp", "gooseberry_matrix_spmultsp",
+    "gooseberry_matrix_spmultsp_dense", "gooseberry_read_sparse_matrix_device",
+    "gooseberry_parse_csr_device", "gooseberry_matrix_read",
 ]
 
 PRECISION_FP64 = 0
@@ -137,6 +140,14 @@ def _sig(lib):
     s = {
         "gooseberry_scale": (i32, [u32, _P, f64]),
         "gooseberry_spmult": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, u32]),
+        "gooseberry_matrix_spmultsp": (i32, [_P, _P, _P, C.POINTER(_P)]),
+        "gooseberry_read_sparse_matrix_device": (i32, [i32, C.c_char_p, _U32P, _U32P,
+                                                       C.POINTER(_P), C.POINTER(_P),
+                                                       C.POINTER(_P)]),
+        "gooseberry_parse_csr_device": (i32, [_P, sz, _U32P, _U32P, C.POINTER(_P),
+                                              C.POINTER(_P), C.POINTER(_P)]),
+        "gooseberry_matrix_read": (i32, [i32, C.c_char_p, _P, C.POINTER(_P)]),
+        "gooseberry_matrix_spmultsp_dense": (i32, [_P, _P, _P]),
         "gooseberry_spmultsp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, _P, _P, u32]),
         "gooseberry_spmultsp_sp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P,
                                          C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
@@ -360,6 +371,14 @@ class Matrix:
         self.nnz = int(rowptr[-1])
 
     @classmethod
+    def read(cls, path, fmt=FORMAT_CSR, plan=None):
+        """gooseberry_matrix_read: the file parsed on the GPU, resident."""
+        h = _P()
+        check(lib().gooseberry_matrix_read(fmt, path.encode(), _plan(plan), C.byref(h)),
+              "gooseberry_matrix_read")
+        return cls._from_handle(h.value)
+
+    @classmethod
     def _from_handle(cls, h):
         self = cls.__new__(cls)
         self.h = h
@@ -425,6 +444,18 @@ class Matrix:
         check(lib().gooseberry_matrix_spmm(self.h, bncols, db.ptr, dc.ptr),
               "gooseberry_matrix_spmm")
         return dc.numpy()
+
+    def spmultsp(self, other, plan=None):
+        """C = self @ other as a new resident Matrix (rows list their columns
+        in the order of their first product)."""
+        h = _P()
+        check(lib().gooseberry_matrix_spmultsp(self.h, other.h, _plan(plan), C.byref(h)),
+              "gooseberry_matrix_spmultsp")
+        return Matrix._from_handle(h.value)
+
+    def spmultsp_dense_device(self, other, d_c):
+        check(lib().gooseberry_matrix_spmultsp_dense(self.h, other.h, d_c.ptr),
+              "gooseberry_matrix_spmultsp_dense")
 
     def cgd_device(self, d_b, d_x, tol, tol_mode=TOL_RELATIVE, niter=0, log=None):
         info = CgdInfo()
@@ -603,12 +634,54 @@ def write_dense(path, vals, nrows, ncols):
                                               _ptr(vals)), "gooseberry_write_dense_matrix")
 
 
-def write_csr(path, rowptr, rowind, rowval, ncols):
+def write_csr(path, rowptr, rowind, rowval, ncols, fmt=FORMAT_CSR):
     rowptr, rowind, rowval = _u32(rowptr), _u32(rowind), _f64(rowval)
-    check(lib().gooseberry_write_sparse_matrix(FORMAT_CSR, path.encode(), rowptr.size - 1,
+    check(lib().gooseberry_write_sparse_matrix(fmt, path.encode(), rowptr.size - 1,
                                                ncols, _ptr(rowptr), _ptr(rowind),
                                                _ptr(rowval)),
           "gooseberry_write_sparse_matrix")
+
+
+def _take_device(nr, rp, ri, rv):
+    """Device CSR arrays returned by the library, copied to numpy and freed."""
+    rowptr = np.empty(nr + 1, dtype=np.uint32)
+    check(lib().gooseberry_memcpy_dtoh(_ptr(rowptr), rp, rowptr.nbytes), "dtoh")
+    nnz = int(rowptr[-1])
+    rowind = np.empty(nnz, dtype=np.uint32)
+    rowval = np.empty(nnz, dtype=np.float64)
+    if nnz:
+        check(lib().gooseberry_memcpy_dtoh(_ptr(rowind), ri, rowind.nbytes), "dtoh")
+        check(lib().gooseberry_memcpy_dtoh(_ptr(rowval), rv, rowval.nbytes), "dtoh")
+    for p in (rp, ri, rv):
+        lib().gooseberry_device_free(p)
+    return rowptr, rowind, rowval
+
+
+def read_sparse_device(path, fmt=FORMAT_CSR):
+    """gooseberry_read_sparse_matrix_device: parsed on the GPU, returned as
+    (nrows, ncols, rowptr, rowind, rowval) host copies."""
+    nr, nc = C.c_uint32(), C.c_uint32()
+    rp, ri, rv = _P(), _P(), _P()
+    check(lib().gooseberry_read_sparse_matrix_device(fmt, path.encode(), C.byref(nr),
+                                                     C.byref(nc), C.byref(rp), C.byref(ri),
+                                                     C.byref(rv)),
+          "gooseberry_read_sparse_matrix_device")
+    return (nr.value, nc.value) + _take_device(nr.value, rp.value, ri.value, rv.value)
+
+
+def parse_csr_device(text):
+    """gooseberry_parse_csr_device on bytes uploaded to the device."""
+    buf = np.frombuffer(bytes(text), dtype=np.uint8)
+    d = DeviceArray(buf.size, np.uint8)
+    if buf.size:
+        d.upload(buf)
+    nr, nc = C.c_uint32(), C.c_uint32()
+    rp, ri, rv = _P(), _P(), _P()
+    check(lib().gooseberry_parse_csr_device(d.ptr, buf.size, C.byref(nr), C.byref(nc),
+                                            C.byref(rp), C.byref(ri), C.byref(rv)),
+          "gooseberry_parse_csr_device")
+    d.free()
+    return (nr.value, nc.value) + _take_device(nr.value, rp.value, ri.value, rv.value)
 
 
 def read_labels(path, nrows=0):

@@ -22,8 +22,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
 HIP_SOURCES = ["gb_runtime.hip", "gb_index.hip", "gb_spmv.hip", "gb_cg.hip",
-               "gb_batch.hip", "gb_gen.hip", "gb_pagerank.hip", "gb_spgemm.hip"]
-C_SOURCES = ["gb_api.c", "gb_io.c", "gb_gen.c", "gb_batch_api.c", "gb_tune.c"]
+               "gb_batch.hip", "gb_gen.hip", "gb_pagerank.hip", "gb_spgemm.hip",
+               "gb_ingest.hip"]
+C_SOURCES = ["gb_api.c", "gb_io.c", "gb_gen.c", "gb_batch_api.c", "gb_tune.c",
+             "gb_ingest_api.c"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

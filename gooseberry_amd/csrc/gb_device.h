@@ -100,6 +100,10 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
 
 typedef struct gbd_matrix gbd_matrix;
 
+/* Structural check of a device CSR: rowptr[0] == 0, non-decreasing, every
+ * column index < ncols; *r_nnz = rowptr[nrows]. */
+int gbd_csr_check(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
+    const uint32_t *d_rowind, uint32_t *r_nnz);
 /* Takes ownership of the device arrays d_rowptr (nrows+1), d_rowind and
  * d_rowval (nnz each; freed by gbd_matrix_destroy) and builds the SpMV plan
  * on the device.  params == NULL selects gbd_plan_defaults(). */
@@ -244,6 +248,19 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
     const uint32_t *d_browind, const double *d_browval, double *d_cdense,
     uint32_t **d_crowptr, uint32_t **d_crowind, double **d_crowval,
     uint32_t *r_cnnz);
+
+/* ------------------------------------------------------------ ingest */
+/* The whole file into new device memory (*d_txt, len + 16 bytes; free
+ * with gbd_free), through pinned staging buffers. */
+int gbd_load_file(const char *file, char **d_txt, uint64_t *r_len);
+/* bytes [offset, offset + bytes) of a file into device memory at d_dst */
+int gbd_load_file_range(const char *file, uint64_t offset, uint64_t bytes,
+    void *d_dst);
+/* .csr text at d_txt (len bytes) parsed on the device with the semantics
+ * of src/io.c:369-498; new device arrays (free with gbd_free). */
+int gbd_parse_csr(const char *d_txt, uint64_t len, uint32_t *r_nrows,
+    uint32_t *r_ncols, uint32_t *r_nnz, uint32_t **d_rowptr,
+    uint32_t **d_rowind, double **d_rowval);
 
 /* d_out[i] = lo + (hi - lo) u, u from a counter-based hash of (seed,
  * stream, i) */

@@ -97,6 +97,28 @@ __device__ __forceinline__ uint32_t gb_wave_scan_incl(uint32_t v) {
   return v;
 }
 
+// Block-wide exclusive scan of one value per thread; returns the thread's
+// exclusive prefix, *total receives the block total.
+__device__ __forceinline__ uint32_t gb_block_scan_excl(uint32_t v,
+                                                       uint32_t *lds4,
+                                                       uint32_t *total) {
+  const int lane = threadIdx.x & (GB_WAVE - 1);
+  const int wave = threadIdx.x / GB_WAVE;
+  uint32_t inc = gb_wave_scan_incl(v);
+  if (lane == GB_WAVE - 1) lds4[wave] = inc;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < GB_WAVES_PER_BLOCK; ++w) {
+    uint32_t t = lds4[w];
+    if (w < wave) woff += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + inc - v;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8).  Map b to a logical index so that each XCD walks one
 // contiguous range of [0, n): neighbouring row blocks then share one XCD's

@@ -25,28 +25,6 @@ gb_scan_tile_sums(const uint32_t *__restrict__ in, size_t n,
   if (threadIdx.x == 0) sums[blockIdx.x] = s;
 }
 
-// Block-wide exclusive scan of one value per thread; returns the thread's
-// exclusive prefix, *total receives the block total.
-__device__ __forceinline__ uint32_t gb_block_scan_excl(uint32_t v,
-                                                       uint32_t *lds4,
-                                                       uint32_t *total) {
-  const int lane = threadIdx.x & (GB_WAVE - 1);
-  const int wave = threadIdx.x / GB_WAVE;
-  uint32_t inc = gb_wave_scan_incl(v);
-  if (lane == GB_WAVE - 1) lds4[wave] = inc;
-  __syncthreads();
-  uint32_t woff = 0, tot = 0;
-#pragma unroll
-  for (int w = 0; w < GB_WAVES_PER_BLOCK; ++w) {
-    uint32_t t = lds4[w];
-    if (w < wave) woff += t;
-    tot += t;
-  }
-  __syncthreads();
-  *total = tot;
-  return woff + inc - v;
-}
-
 // One block: exclusive scan of sums[0..nb) in place; *total = grand total.
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_scan_top(uint32_t *__restrict__ sums, size_t nb, uint32_t *total) {

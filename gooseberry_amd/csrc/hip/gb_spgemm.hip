@@ -131,10 +131,17 @@ gb_sg_runsum(uint32_t nruns, uint32_t n, const uint32_t *__restrict__ runstart,
   }
 }
 
+// row pointers of C from the runs' rows (ascending in run order): thread q
+// writes the pointers of the rows between run q-1's row and run q's row
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_sg_rowcount(uint32_t nruns, const uint32_t *__restrict__ rrow, uint32_t *__restrict__ cnt) {
-  for (uint32_t q = blockIdx.x * GB_BLOCK + threadIdx.x; q < nruns; q += gridDim.x * GB_BLOCK)
-    atomicAdd(&cnt[rrow[q]], 1u);
+gb_sg_rowptr(uint32_t nruns, uint32_t anrows, const uint32_t *__restrict__ rrow,
+             uint32_t *__restrict__ crowptr) {
+  for (uint32_t q = blockIdx.x * GB_BLOCK + threadIdx.x; q <= nruns;
+       q += gridDim.x * GB_BLOCK) {
+    const uint32_t lo = q == 0 ? 0u : rrow[q - 1] + 1u;
+    const uint32_t hi = q == nruns ? anrows : rrow[q];
+    for (uint32_t i = lo; i <= hi; ++i) crowptr[i] = q;
+  }
 }
 
 // runs in first-appearance order (ordr) written out as the C arrays
@@ -180,7 +187,7 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
   uint32_t *len = nullptr, *off = nullptr, *erow = nullptr, *ecol = nullptr,
            *perm1 = nullptr, *row1 = nullptr, *ord = nullptr, *flag = nullptr,
            *rid = nullptr, *runstart = nullptr, *rrow = nullptr, *rcol = nullptr,
-           *rfirst = nullptr, *ordr = nullptr, *cnt = nullptr, *crowptr = nullptr,
+           *rfirst = nullptr, *ordr = nullptr, *crowptr = nullptr,
            *crowind = nullptr;
   double *eval = nullptr, *rval = nullptr, *crowval = nullptr;
   unsigned long long e64 = 0, *tot = nullptr;
@@ -280,19 +287,17 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
     }
   }
   if (!dense) {
-    GB_SG_TRY(gb_dev_alloc((void **)&cnt, ((size_t)anrows + 1) * 4));
     GB_SG_TRY(gb_dev_alloc((void **)&crowptr, ((size_t)anrows + 1) * 4));
     GB_SG_TRY(gb_dev_alloc((void **)&crowind, ((size_t)nruns + 1) * 4));
     GB_SG_TRY(gb_dev_alloc((void **)&crowval, ((size_t)nruns + 1) * 8));
-    GB_SG_RV(gbd_zero(cnt, ((size_t)anrows + 1) * 4));
+    gb_sg_rowptr<<<gb_sg_grid((uint64_t)nruns + 1), GB_BLOCK, 0, s>>>(nruns, anrows, rrow,
+                                                                      crowptr);
+    GB_SG_TRY(hipGetLastError());
     if (nruns) {
-      gb_sg_rowcount<<<gb_sg_grid(nruns), GB_BLOCK, 0, s>>>(nruns, rrow, cnt);
-      GB_SG_TRY(hipGetLastError());
       gb_sg_out<<<gb_sg_grid(nruns), GB_BLOCK, 0, s>>>(nruns, ordr, rcol, rval, crowind,
                                                        crowval);
       GB_SG_TRY(hipGetLastError());
     }
-    GB_SG_RV(gbd_scan_u32(cnt, crowptr, anrows, crowptr + anrows));
     *d_crowptr = crowptr;
     *d_crowind = crowind;
     *d_crowval = crowval;
@@ -307,7 +312,7 @@ done:
   gb_dev_free(eval); gb_dev_free(perm1); gb_dev_free(row1); gb_dev_free(ord);
   gb_dev_free(flag); gb_dev_free(rid); gb_dev_free(runstart); gb_dev_free(rrow);
   gb_dev_free(rcol); gb_dev_free(rfirst); gb_dev_free(rval); gb_dev_free(ordr);
-  gb_dev_free(cnt); gb_dev_free(crowptr); gb_dev_free(crowind); gb_dev_free(crowval);
+  gb_dev_free(crowptr); gb_dev_free(crowind); gb_dev_free(crowval);
   gb_dev_free(tot);
   return rv;
 #undef GB_SG_TRY

@@ -591,6 +591,30 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->csplit = 0;
 }
 
+int gbd_csr_check(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
+                  const uint32_t *d_rowind, uint32_t *r_nnz) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  hipStream_t s = gb_stream();
+  // nnz from the device, then every row pointer and column index checked
+  uint32_t *bad = nullptr, hbad = 0, nnz = 0;
+  GB_HIP(hipMemcpyAsync(&nnz, d_rowptr + nrows, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  GB_HIP(hipMallocAsync((void **)&bad, 4, s));
+  GB_HIP(hipMemsetAsync(bad, 0, 4, s));
+  uint64_t work = (uint64_t)nrows > nnz ? nrows : nnz;
+  uint32_t g = gb_div_up(work ? work : 1, GB_BLOCK);
+  gb_validate_csr<<<g > 8192u ? 8192u : g, GB_BLOCK, 0, s>>>(
+      d_rowptr, d_rowind, nrows, ncols, nnz, bad);
+  GB_LAUNCH_CHECK();
+  GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipFreeAsync(bad, s));
+  GB_HIP(hipStreamSynchronize(s));
+  if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  *r_nnz = nnz;
+  return GOOSEBERRY_SUCCESS;
+}
+
 static int gb_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
                             uint32_t *d_rowind, double *d_rowval, bool trusted,
                             uint32_t nnz, const gbd_plan_params *params,
@@ -598,22 +622,10 @@ static int gb_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
   int rv = gb_ensure_init();
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   hipStream_t s = gb_stream();
+  (void)s;
   if (!trusted) {
-    // nnz from the device, then every row pointer and column index checked
-    uint32_t *bad = nullptr, hbad = 0;
-    GB_HIP(hipMemcpyAsync(&nnz, d_rowptr + nrows, 4, hipMemcpyDeviceToHost, s));
-    GB_HIP(hipStreamSynchronize(s));
-    GB_HIP(hipMallocAsync((void **)&bad, 4, s));
-    GB_HIP(hipMemsetAsync(bad, 0, 4, s));
-    uint64_t work = (uint64_t)nrows > nnz ? nrows : nnz;
-    uint32_t g = gb_div_up(work ? work : 1, GB_BLOCK);
-    gb_validate_csr<<<g > 8192u ? 8192u : g, GB_BLOCK, 0, s>>>(
-        d_rowptr, d_rowind, nrows, ncols, nnz, bad);
-    GB_LAUNCH_CHECK();
-    GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
-    GB_HIP(hipFreeAsync(bad, s));
-    GB_HIP(hipStreamSynchronize(s));
-    if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    rv = gbd_csr_check(nrows, ncols, d_rowptr, d_rowind, &nnz);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
   }
   gbd_plan_params p;
   if (params) p = *params;

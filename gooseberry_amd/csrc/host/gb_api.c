@@ -482,6 +482,40 @@ int gooseberry_spmultsp_sp(dim_t anrows, dim_t ancols, dim_t bncols,
   return GOOSEBERRY_SUCCESS;
 }
 
+int gooseberry_matrix_spmultsp(gooseberry_matrix_t a, gooseberry_matrix_t b,
+    const gooseberry_plan_t *plan, gooseberry_matrix_t *r_c)
+{
+  if (!a || !b || !r_c) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t anr, anc, bnr, bnc;
+  gbd_matrix_info(a->m, &anr, &anc, NULL, NULL, NULL, NULL, NULL, NULL);
+  gbd_matrix_info(b->m, &bnr, &bnc, NULL, NULL, NULL, NULL, NULL, NULL);
+  if (bnr < anc) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t *dptr = NULL, *dind = NULL, cnnz = 0;
+  double *dval = NULL;
+  int rv = gbd_spgemm(anr, anc, bnc, gbd_matrix_rowptr(a->m),
+      gbd_matrix_rowind(a->m), gbd_matrix_rowval(a->m),
+      gbd_matrix_rowptr(b->m), gbd_matrix_rowind(b->m),
+      gbd_matrix_rowval(b->m), NULL, &dptr, &dind, &dval, &cnnz);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  return gb_matrix_adopt(anr, bnc, dptr, dind, dval, cnnz, plan, r_c);
+}
+
+int gooseberry_matrix_spmultsp_dense(gooseberry_matrix_t a,
+    gooseberry_matrix_t b, real_t *d_c)
+{
+  if (!a || !b) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t anr, anc, bnr, bnc;
+  gbd_matrix_info(a->m, &anr, &anc, NULL, NULL, NULL, NULL, NULL, NULL);
+  gbd_matrix_info(b->m, &bnr, &bnc, NULL, NULL, NULL, NULL, NULL, NULL);
+  if (bnr < anc) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if ((size_t)anr * bnc == 0) return GOOSEBERRY_SUCCESS;
+  if (!d_c) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return gbd_spgemm(anr, anc, bnc, gbd_matrix_rowptr(a->m),
+      gbd_matrix_rowind(a->m), gbd_matrix_rowval(a->m),
+      gbd_matrix_rowptr(b->m), gbd_matrix_rowind(b->m),
+      gbd_matrix_rowval(b->m), d_c, NULL, NULL, NULL, NULL);
+}
+
 /* ------------------------------------------------------------------- CG */
 
 /* Iterations are enqueued in batches replayed from captured hipGraphs and

@@ -2,6 +2,9 @@
 #ifndef GB_HOST_H
 #define GB_HOST_H
 
+#include <stdint.h>
+#include <stdio.h>
+
 #include "gooseberry.h"
 #include "../gb_device.h"
 
@@ -12,5 +15,9 @@ gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat);
 int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
     dim_t *d_rowind, real_t *d_rowval, ind_t nnz,
     const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat);
+
+/* reads and checks a binary CSR header (the file must be exactly the size
+ * it announces); leaves f at the row pointers */
+int gb_bcsr_header(FILE *f, dim_t *nrows, dim_t *ncols, uint64_t *nnz);
 
 #endif

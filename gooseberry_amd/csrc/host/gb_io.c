@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include "gooseberry.h"
 #include "gb_host.h"
@@ -351,6 +352,84 @@ static int write_csr(const char *file, dim_t nrows, const ind_t *rowptr,
   return bad ? GOOSEBERRY_ERROR_WRITE : GOOSEBERRY_SUCCESS;
 }
 
+/* Binary CSR: the 8-byte magic, uint32 nrows, uint32 ncols, uint64 nnz,
+ * then rowptr, rowind, rowval as stored in memory (little-endian) */
+static const unsigned char BCSR_MAGIC[8] = {'G', 'B', 'C', 'S', 'R', 0, 1, 0};
+
+int gb_bcsr_header(FILE *f, dim_t *nrows, dim_t *ncols, uint64_t *nnz)
+{
+  unsigned char h[24];
+  if (fread(h, 1, 24, f) != 24 || memcmp(h, BCSR_MAGIC, 8) != 0)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  memcpy(nrows, h + 8, 4);
+  memcpy(ncols, h + 12, 4);
+  memcpy(nnz, h + 16, 8);
+  if (*nnz > 0xffffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* the file must hold exactly the arrays the header announces */
+  struct stat st;
+  if (fstat(fileno(f), &st) != 0) return GOOSEBERRY_ERROR_READ;
+  uint64_t want = 24 + ((uint64_t)*nrows + 1) * 4 + *nnz * 12;
+  if ((uint64_t)st.st_size != want) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return GOOSEBERRY_SUCCESS;
+}
+
+static int write_bcsr(const char *file, dim_t nrows, dim_t ncols,
+    const ind_t *rowptr, const dim_t *rowind, const real_t *rowval)
+{
+  FILE *f = fopen(file, "wb");
+  if (!f) return errno_to_rv(errno);
+  uint64_t nnz = rowptr[nrows];
+  unsigned char h[24];
+  memcpy(h, BCSR_MAGIC, 8);
+  memcpy(h + 8, &nrows, 4);
+  memcpy(h + 12, &ncols, 4);
+  memcpy(h + 16, &nnz, 8);
+  int bad = fwrite(h, 1, 24, f) != 24;
+  bad = bad || fwrite(rowptr, 4, (size_t)nrows + 1, f) != (size_t)nrows + 1;
+  bad = bad || (nnz && fwrite(rowind, 4, nnz, f) != nnz);
+  bad = bad || (nnz && fwrite(rowval, 8, nnz, f) != nnz);
+  if (fclose(f) != 0) bad = 1;
+  return bad ? GOOSEBERRY_ERROR_WRITE : GOOSEBERRY_SUCCESS;
+}
+
+static int read_bcsr(const char *file, dim_t *r_nrows, dim_t *r_ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  FILE *f = fopen(file, "rb");
+  if (!f) return errno_to_rv(errno);
+  dim_t nrows, ncols;
+  uint64_t nnz;
+  int rv = gb_bcsr_header(f, &nrows, &ncols, &nnz);
+  ind_t *rp = NULL;
+  dim_t *ri = NULL;
+  real_t *rv8 = NULL;
+  if (rv == GOOSEBERRY_SUCCESS) {
+    rp = malloc(((size_t)nrows + 1) * 4);
+    ri = malloc((nnz ? nnz : 1) * 4);
+    rv8 = malloc((nnz ? nnz : 1) * 8);
+    if (!rp || !ri || !rv8) rv = GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  if (rv == GOOSEBERRY_SUCCESS &&
+      (fread(rp, 4, (size_t)nrows + 1, f) != (size_t)nrows + 1 ||
+       (nnz && fread(ri, 4, nnz, f) != nnz) ||
+       (nnz && fread(rv8, 8, nnz, f) != nnz)))
+    rv = GOOSEBERRY_ERROR_READ;
+  fclose(f);
+  if (rv == GOOSEBERRY_SUCCESS &&
+      (rp[nrows] != nnz || gb_check_csr(nrows, ncols, rp, ri) != GOOSEBERRY_SUCCESS))
+    rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (rv != GOOSEBERRY_SUCCESS) {
+    free(rp); free(ri); free(rv8);
+    return rv;
+  }
+  *r_nrows = nrows;
+  *r_ncols = ncols;
+  *r_rowptr = rp;
+  *r_rowind = ri;
+  *r_rowval = rv8;
+  return GOOSEBERRY_SUCCESS;
+}
+
 /* src/io.c:1464-1549 dispatch */
 int gooseberry_read_sparse_matrix(int type, const char *file, dim_t *nrows,
     dim_t *ncols, ind_t **rowptr, dim_t **rowind, real_t **rowval)
@@ -362,6 +441,8 @@ int gooseberry_read_sparse_matrix(int type, const char *file, dim_t *nrows,
       return read_csr(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_MATRIXMARKET:
       return read_mtx(file, nrows, ncols, rowptr, rowind, rowval);
+    case GOOSEBERRY_FORMAT_BCSR:
+      return read_bcsr(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_GRAPH:
     case GOOSEBERRY_FORMAT_SVM:
     case GOOSEBERRY_FORMAT_CLU:
@@ -400,11 +481,12 @@ int gooseberry_write_sparse_matrix(int type, const char *file, dim_t nrows,
     dim_t ncols, const ind_t *rowptr, const dim_t *rowind,
     const real_t *rowval)
 {
-  (void)ncols;
   if (!file || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
   switch (type) {
     case GOOSEBERRY_FORMAT_CSR:
       return write_csr(file, nrows, rowptr, rowind, rowval);
+    case GOOSEBERRY_FORMAT_BCSR:
+      return write_bcsr(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_GRAPH:
     case GOOSEBERRY_FORMAT_MATRIXMARKET:
     case GOOSEBERRY_FORMAT_POINT:

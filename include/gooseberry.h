@@ -76,7 +76,8 @@ typedef enum gooseberry_format_t {
   GOOSEBERRY_FORMAT_SVM,
   GOOSEBERRY_FORMAT_GRAPH,
   GOOSEBERRY_FORMAT_CLU,
-  GOOSEBERRY_FORMAT_MATRIXMARKET
+  GOOSEBERRY_FORMAT_MATRIXMARKET,
+  GOOSEBERRY_FORMAT_BCSR          /* new: binary CSR, see below */
 } gooseberry_format_t;
 
 #ifdef __cplusplus
@@ -163,9 +164,15 @@ int gooseberry_spmultsp_sp(
     dim_t nblocks);
 
 /* include/gooseberry.h IO prototypes; src/io.c:1464-1549.  Sparse readers:
- * GOOSEBERRY_FORMAT_CSR and GOOSEBERRY_FORMAT_MATRIXMARKET.  Dense: GRID.
+ * GOOSEBERRY_FORMAT_CSR, GOOSEBERRY_FORMAT_MATRIXMARKET and
+ * GOOSEBERRY_FORMAT_BCSR; sparse writers: CSR and BCSR.  Dense: GRID.
  * Arrays returned through pointers are malloc'd; free with free() or
- * gooseberry_free(). */
+ * gooseberry_free().
+ *
+ * GOOSEBERRY_FORMAT_BCSR (new, extension .bcsr) is a binary CSR: the 8
+ * bytes "GBCSR\0\1\0", uint32 nrows, uint32 ncols, uint64 nnz, then
+ * rowptr (nrows + 1 uint32), rowind (nnz uint32) and rowval (nnz
+ * float64), little-endian, nothing else. */
 int gooseberry_read_sparse_matrix(
     int type,
     const char * file,
@@ -197,6 +204,31 @@ int gooseberry_write_sparse_matrix(
     const ind_t * rowptr,
     const dim_t * rowind,
     const real_t * rowval);
+
+/* New: a sparse matrix file straight into device memory.  CSR text is
+ * parsed on the GPU with the reader semantics of src/io.c:369-498 -- the
+ * arrays are those of gooseberry_read_sparse_matrix, bit for bit -- and
+ * BCSR is copied through pinned buffers and checked on the GPU (other
+ * formats: GOOSEBERRY_ERROR_UNIMPLEMENTED).  The arrays are device memory
+ * (free with gooseberry_device_free). */
+int gooseberry_read_sparse_matrix_device(
+    int type,
+    const char * file,
+    dim_t * nrows,
+    dim_t * ncols,
+    ind_t ** d_rowptr,
+    dim_t ** d_rowind,
+    real_t ** d_rowval);
+
+/* The same parse of CSR text already in device memory (len bytes). */
+int gooseberry_parse_csr_device(
+    const char * d_text,
+    size_t len,
+    dim_t * nrows,
+    dim_t * ncols,
+    ind_t ** d_rowptr,
+    dim_t ** d_rowind,
+    real_t ** d_rowval);
 
 /* src/io.c:1552-1639 */
 int gooseberry_read_labels(
@@ -314,6 +346,14 @@ int gooseberry_matrix_create_device(
     const real_t * d_rowval,
     const gooseberry_plan_t * plan,
     gooseberry_matrix_t * r_mat);
+
+/* A resident matrix read from a file as gooseberry_read_sparse_matrix_device
+ * reads it, planned with `plan` (NULL: defaults). */
+int gooseberry_matrix_read(
+    int type,
+    const char * file,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * mat);
 
 int gooseberry_matrix_free(gooseberry_matrix_t mat);
 
@@ -569,6 +609,21 @@ typedef struct gooseberry_pagerank_info_t {
   double rmse;      /* the RMSE the reference prints */
   dim_t nsinks;
 } gooseberry_pagerank_info_t;
+
+/* The same products on resident matrices (B needs at least as many rows as
+ * A has columns; only the first A-ncols rows are used).  The sparse form
+ * returns C as a new resident matrix planned with `plan` (NULL: defaults);
+ * the dense form writes row-major C (A-nrows x B-ncols) to device memory. */
+int gooseberry_matrix_spmultsp(
+    gooseberry_matrix_t a,
+    gooseberry_matrix_t b,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * c);
+
+int gooseberry_matrix_spmultsp_dense(
+    gooseberry_matrix_t a,
+    gooseberry_matrix_t b,
+    real_t * d_c);
 
 /* On a resident square matrix; d_out: nrows values on the device.  The
  * matrix itself is not modified. */

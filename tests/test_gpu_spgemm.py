@@ -156,3 +156,28 @@ def test_cli_sparse_times_sparse(gb, oracle, tmp_path):
     # B B: 1100 columns against 900 rows do not chain and are refused
     r = run("blas", "-x", "multiply", "-i", fb, "-i", fb, "-o", out)
     assert r.returncode != 0 and "do not match" in r.stderr
+
+
+def test_resident_spgemm(gb, oracle):
+    a = random_csr(3000, 2500, 6, 21, empty_every=11, dups=True)
+    b = random_csr(2600, 2000, 5, 22, dups=True)   # more rows than A has columns
+    A = gb.Matrix(*a, ncols=2500)
+    B = gb.Matrix(*b, ncols=2000)
+    Cm = A.spmultsp(B)
+    assert (Cm.nrows, Cm.ncols) == (3000, 2000)
+    rp, ri, rv = Cm.download()
+    bb = (b[0][:2501], b[1][:b[0][2500]], b[2][:b[0][2500]])
+    orp, ori, orv = oracle.spmultsp_sp(a, bb, 2000)
+    assert np.array_equal(rp, orp) and np.array_equal(ri, ori)
+    assert np.array_equal(bits(rv), bits(orv))
+    # the product is an ordinary resident matrix: its SpMV works
+    x = np.random.default_rng(23).uniform(-1, 1, 2000)
+    y = Cm.spmv(x)
+    ref = oracle.spmult(3000, 2000, 1, orp, ori, orv, x)
+    assert np.max(np.abs(y - ref)) <= 1e-14 * np.max(np.abs(ref))
+    d = gb.DeviceArray(3000 * 2000)
+    A.spmultsp_dense_device(B, d)
+    assert np.array_equal(bits(d.numpy()), bits(oracle.spmultsp(a, bb, 2000)))
+    S = gb.Matrix(*random_csr(100, 50, 3, 24), ncols=50)
+    with pytest.raises(gb.GooseberryError):
+        B.spmultsp(S)   # 2000 columns against 100 rows

@@ -228,3 +228,40 @@ def test_compute_fails_loudly_without_gpu(gb):
     with pytest.raises(gb.GooseberryError) as e:
         gb.spmult(16, 16, 1, rp, ri, rv, np.ones(16))
     assert e.value.name == "DEVICE"
+
+
+def test_binary_csr_round_trip_and_checks(gb, oracle, tmp_path):
+    """The binary CSR format: header, arrays as stored, exact size; the
+    host reader gives back the written arrays and refuses a bad file."""
+    from test_oracle_spgemm import random_csr
+    rp, ri, rv = random_csr(3000, 2000, 6, 41, empty_every=7, dups=True)
+    rv[::97] = -0.0
+    path = os.path.join(str(tmp_path), "m.bcsr")
+    gb.write_csr(path, rp, ri, rv, 2000, fmt=gb.FORMAT_BCSR)
+    data = open(path, "rb").read()
+    assert data[:8] == b"GBCSR\x00\x01\x00"
+    assert len(data) == 24 + 4 * 3001 + 12 * int(rp[-1])
+    nr, nc, grp, gri, grv = gb.read_sparse(path, gb.FORMAT_BCSR)
+    assert (nr, nc) == (3000, 2000)
+    assert np.array_equal(grp, rp) and np.array_equal(gri, ri)
+    assert np.array_equal(grv.view(np.uint64), rv.view(np.uint64))
+    for name, blob in (("short", data[:-1]), ("magic", b"X" + data[1:]),
+                       ("col", data[:24 + 4 * 3001] + b"\xff\xff\xff\xff" +
+                        data[24 + 4 * 3001 + 4:])):
+        bad = os.path.join(str(tmp_path), name + ".bcsr")
+        with open(bad, "wb") as f:
+            f.write(blob)
+        with pytest.raises(gb.GooseberryError) as e:
+            gb.read_sparse(bad, gb.FORMAT_BCSR)
+        assert e.value.name == "INVALIDINPUT", name
+
+
+def test_device_reader_fails_loudly_without_gpu(gb, tmp_path):
+    if HAVE_GPU:
+        pytest.skip("a GPU is present")
+    path = os.path.join(str(tmp_path), "a.csr")
+    with open(path, "w") as f:
+        f.write("1 2\n")
+    with pytest.raises(gb.GooseberryError) as e:
+        gb.read_sparse_device(path)
+    assert e.value.name == "DEVICE"
