@@ -33,6 +33,11 @@ timed on a bounded sample on this host).
            resident A in, resident C out (CSR, columns in first-appearance
            order, bitwise the reference's sums); the oracle's sequential
            product on the same matrix as the CPU baseline
+  ingest   the 160^3 Laplacian as .csr text (written as the reference
+           writes it, %g) read into a resident matrix: text parsed on the
+           GPU; also the parse alone with the text already in HBM, and the
+           binary CSR file; the oracle's reader (the reference's loop with
+           the C library's strtod) as the CPU baseline
 """
 import argparse
 import json
@@ -645,6 +650,97 @@ def run_spgemm(args, gb):
                  "n": n, "nnz": nnz}, **extra)
 
 
+# ------------------------------------------------ CSR ingest
+
+def run_ingest(args, gb):
+    import ctypes as C
+    import tempfile
+    g = args.grid
+    rp, ri, rv = gb.laplacian(3, g)
+    n, nnz = rp.size - 1, int(rp[-1])
+    d = tempfile.mkdtemp(prefix="gb_ingest_")
+    csr, bcsr = os.path.join(d, "lap.csr"), os.path.join(d, "lap.bcsr")
+    try:
+        gb.write_csr(csr, rp, ri, rv, n)
+        gb.write_csr(bcsr, rp, ri, rv, n, fmt=gb.FORMAT_BCSR)
+        text_bytes = os.path.getsize(csr)
+        for _ in range(args.warmup):
+            gb.Matrix.read(csr).free()
+        gb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            M = gb.Matrix.read(csr)
+            M.free()
+        gb.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms = elapsed / args.steps * 1e3
+        # the binary file
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            gb.Matrix.read(bcsr, gb.FORMAT_BCSR).free()
+        bin_ms = (time.perf_counter() - t) / args.steps * 1e3
+        # the parse alone, text resident in HBM
+        data = np.fromfile(csr, dtype=np.uint8)
+        dt = gb.DeviceArray(data.size, np.uint8)
+        dt.upload(data)
+        e0, e1 = gb.Event(), gb.Event()
+        parse_ms = []
+        for rep in range(args.steps + 1):
+            nr, nc = C.c_uint32(), C.c_uint32()
+            p = [C.c_void_p(), C.c_void_p(), C.c_void_p()]
+            e0.record()
+            gb.check(gb.lib().gooseberry_parse_csr_device(
+                dt.ptr, data.size, C.byref(nr), C.byref(nc), C.byref(p[0]), C.byref(p[1]),
+                C.byref(p[2])), "gooseberry_parse_csr_device")
+            e1.record()
+            if rep:
+                parse_ms.append(e0.elapsed_ms(e1))
+            for q in p:
+                gb.lib().gooseberry_device_free(q.value)
+        dt.free()
+        pms = float(np.mean(parse_ms))
+        ab = text_bytes + 4 * (n + 1) + 12 * nnz   # text in, CSR out
+        achieved = ab / (pms * 1e-3) / 1e9
+        # the library's own host reader and the oracle's, same file
+        t = time.perf_counter()
+        gb.read_sparse(csr)
+        host_s = time.perf_counter() - t
+        cpu = None
+        if not args.no_cpu_baseline:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as orc
+            k, t = 0, time.perf_counter()
+            while True:
+                orc.read_sparse(csr)
+                k += 1
+                if time.perf_counter() - t > args.cpu_seconds or k >= 3:
+                    break
+            t = (time.perf_counter() - t) / k
+            cpu = {"value": text_bytes / t / 1e6, "unit": "MB/s of text", "cores": 1,
+                   "kind": "port",
+                   "sample": "%d reads of the same %.0f MB file by the oracle's reader (C "
+                             "restatement of src/io.c:369-498: getline, strtoull, strtod) "
+                             "on one core" % (k, text_bytes / 1e6)}
+    finally:
+        for f in (csr, bcsr):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+    return line(text_bytes / (ms * 1e-3) / 1e6, "MB/s of text", args, ms, "u8",
+                {"workload": ".csr text of the 3D 7-point Laplacian %d^3 (n=%d, nnz=%d, %.0f "
+                             "MB, %%g values) read into a resident matrix; file in the page "
+                             "cache" % (g, n, nnz, text_bytes / 1e6),
+                 "n": n, "nnz": nnz, "text_bytes": text_bytes},
+                roofline={"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
+                          "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                          "kernel": "whole parse with the text in HBM (line split, row "
+                                    "flags, count and fill passes, scans)",
+                          "algorithmic_bytes_per_launch": ab, "avg_launch_ms": pms},
+                parse_in_hbm_gbps=text_bytes / (pms * 1e-3) / 1e9,
+                binary_read_ms=bin_ms, binary_bytes=24 + 4 * (n + 1) + 12 * nnz,
+                host_reader_ms=host_s * 1e3, cpu_baseline=cpu)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -657,7 +753,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps (default 2; zipf 1; batched 1)")
     ap.add_argument("--workload", default="laplacian",
-                    choices=["laplacian", "zipf", "batched", "served", "spgemm"])
+                    choices=["laplacian", "zipf", "batched", "served", "spgemm", "ingest"])
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
@@ -681,7 +777,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0),
-                "spgemm": (5, 1)}
+                "spgemm": (5, 1), "ingest": (5, 1)}
     st, wu = defaults[args.workload]
     args.steps = st if args.steps is None else args.steps
     args.warmup = wu if args.warmup is None else args.warmup
@@ -689,7 +785,7 @@ def main():
     import gooseberry_amd as gb
 
     run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
-           "served": run_served, "spgemm": run_spgemm}[args.workload]
+           "served": run_served, "spgemm": run_spgemm, "ingest": run_ingest}[args.workload]
     out = run(args, gb)
     print(json.dumps(out), flush=True)
 

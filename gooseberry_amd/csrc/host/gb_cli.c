@@ -128,7 +128,8 @@ static int file_type(const char *name)
   static const struct { const char *ext; int type; } tbl[] = {
     {"raw", GOOSEBERRY_FORMAT_RAW}, {"mat", GOOSEBERRY_FORMAT_GRID},
     {"grid", GOOSEBERRY_FORMAT_GRID}, {"vec", GOOSEBERRY_FORMAT_GRID},
-    {"txt", GOOSEBERRY_FORMAT_GRID}, {"csr", GOOSEBERRY_FORMAT_CSR},
+    {"txt", GOOSEBERRY_FORMAT_GRID}, {"bcsr", GOOSEBERRY_FORMAT_BCSR},
+    {"csr", GOOSEBERRY_FORMAT_CSR},
     {"svm", GOOSEBERRY_FORMAT_SVM}, {"libfm", GOOSEBERRY_FORMAT_SVM},
     {"ij", GOOSEBERRY_FORMAT_POINT}, {"point", GOOSEBERRY_FORMAT_POINT},
     {"metis", GOOSEBERRY_FORMAT_GRAPH}, {"chaco", GOOSEBERRY_FORMAT_GRAPH},
@@ -183,6 +184,43 @@ static int read_input(const char *file, mat_t *m)
     return gooseberry_read_dense_matrix(t, file, &m->nrows, &m->ncols, &m->rowval);
   return gooseberry_read_sparse_matrix(t, file, &m->nrows, &m->ncols, &m->rowptr,
       &m->rowind, &m->rowval);
+}
+
+/* CSR text and binary CSR go straight to a resident matrix: parsed on the
+ * GPU with the host reader's semantics (the same arrays, bit for bit) */
+static int device_readable(const char *file)
+{
+  int t = file_type(file);
+  return t == GOOSEBERRY_FORMAT_CSR || t == GOOSEBERRY_FORMAT_BCSR;
+}
+
+static int read_resident(const char *file, gooseberry_matrix_t *dmat, mat_t *m)
+{
+  memset(m, 0, sizeof(*m));
+  int err = gooseberry_matrix_read(file_type(file), file, NULL, dmat);
+  if (err != GOOSEBERRY_SUCCESS) {
+    if (err == GOOSEBERRY_ERROR_INVALIDINPUT)
+      fprintf(stderr, "Could not read '%s' as a matrix\n", file);
+    return err;
+  }
+  gooseberry_matrix_info_t info;
+  err = gooseberry_matrix_info(*dmat, &info);
+  m->nrows = info.nrows;
+  m->ncols = info.ncols;
+  return err;
+}
+
+/* host arrays of a resident matrix (when a host-side step needs them) */
+static int download_resident(gooseberry_matrix_t dmat, mat_t *m)
+{
+  gooseberry_matrix_info_t info;
+  int err = gooseberry_matrix_info(dmat, &info);
+  if (err != GOOSEBERRY_SUCCESS) return err;
+  m->rowptr = malloc(((size_t)info.nrows + 1) * sizeof(ind_t));
+  m->rowind = malloc((info.nnz ? info.nnz : 1) * sizeof(dim_t));
+  m->rowval = malloc((info.nnz ? info.nnz : 1) * sizeof(real_t));
+  if (!m->rowptr || !m->rowind || !m->rowval) return GOOSEBERRY_ERROR_UNKNOWN;
+  return gooseberry_matrix_download(dmat, m->rowptr, m->rowind, m->rowval);
 }
 
 /* stable counting sort of row ids by label (src/gooseberry_bin.c:2038-2049) */
@@ -303,6 +341,69 @@ END:
   return err;
 }
 
+/* sparse x sparse on resident matrices read on the GPU (no permutations):
+ * the product stays on the device until it is written */
+static int blas_resident_sparse(gooseberry_matrix_t a, gooseberry_matrix_t b,
+    const mat_t *ma, const mat_t *mb, const char *outfile, long long runs,
+    int times, tmr_t *aux, tmr_t *op, tmr_t *io)
+{
+  const dim_t anrows = ma->nrows, ancols = ma->ncols, bncols = mb->ncols;
+  if (ancols > mb->nrows) {
+    fprintf(stderr, "Matrix dimensions do not match for multiplication: "
+        "%" PRIu32 "x%" PRIu32 " and %" PRIu32 "x%" PRIu32 "\n",
+        anrows, ancols, mb->nrows, bncols);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  const int otype = file_type(outfile);
+  int err = GOOSEBERRY_SUCCESS;
+  real_t *dense = NULL;
+  void *d_c = NULL;
+  gooseberry_matrix_t c = NULL;
+  mat_t mc;
+  memset(&mc, 0, sizeof(mc));
+  const size_t csz = (size_t)anrows * bncols;
+  if (is_dense(otype)) {
+    dense = calloc(csz ? csz : 1, sizeof(real_t));
+    if (!dense) return GOOSEBERRY_ERROR_NOMEMORY;
+    err = gooseberry_device_alloc(&d_c, (csz ? csz : 1) * sizeof(real_t));
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (times) { tmr_stop(aux); tmr_start(op); }
+  for (long long r = 0; r < runs; ++r) {
+    if (dense) {
+      err = gooseberry_matrix_spmultsp_dense(a, b, d_c);
+    } else {
+      gooseberry_matrix_free(c);
+      c = NULL;
+      err = gooseberry_matrix_spmultsp(a, b, NULL, &c);
+    }
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (dense) err = gooseberry_memcpy_dtoh(dense, d_c, csz * sizeof(real_t));
+  else err = download_resident(c, &mc);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) { tmr_stop(op); tmr_start(io); }
+  if (dense)
+    err = gooseberry_write_dense_matrix(GOOSEBERRY_FORMAT_GRID, outfile, anrows,
+        bncols, dense);
+  else
+    err = gooseberry_write_sparse_matrix(otype, outfile, anrows, bncols,
+        mc.rowptr, mc.rowind, mc.rowval);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (times) {
+    tmr_stop(io);
+    const char *names[] = {"I/O", "Auxillary", "Compute"};
+    double vals[] = {io->acc, aux->acc, op->acc};
+    print_times(names, vals, 3);
+  }
+END:
+  free(dense);
+  gooseberry_device_free(d_c);
+  gooseberry_matrix_free(c);
+  mat_free(&mc);
+  return err;
+}
+
 static int cmd_blas(int argc, char **argv)
 {
   arg_t args[512];
@@ -319,7 +420,7 @@ static int cmd_blas(int argc, char **argv)
   dim_t *rperm = NULL, *cperm = NULL, *rdist = NULL, *cdist = NULL;
   real_t *out = NULL;
   void *d_b = NULL, *d_c = NULL;
-  gooseberry_matrix_t dmat = NULL;
+  gooseberry_matrix_t dmat = NULL, dmatB = NULL;
   tmr_t io = {0}, aux = {0}, op = {0};
 
   if (nargs < 2) {
@@ -354,8 +455,18 @@ static int cmd_blas(int argc, char **argv)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
   if (times) tmr_start(&io);
-  for (size_t i = 0; i < nin; ++i) {
-    if ((err = read_input(infiles[i], &in[i])) != GOOSEBERRY_SUCCESS) goto END;
+  /* without host-side permutations a CSR / BCSR matrix is read on the GPU */
+  const int devA = !rpf && !cpf && !redband && device_readable(infiles[0]);
+  const int devB = devA && device_readable(infiles[1]);
+  err = devA ? read_resident(infiles[0], &dmat, &in[0]) : read_input(infiles[0], &in[0]);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  err = devB ? read_resident(infiles[1], &dmatB, &in[1]) : read_input(infiles[1], &in[1]);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
+  if (devA && !devB && !in[1].dense) {
+    /* a sparse second input in another format: the host product */
+    if ((err = download_resident(dmat, &in[0])) != GOOSEBERRY_SUCCESS) goto END;
+    gooseberry_matrix_free(dmat);
+    dmat = NULL;
   }
   dim_t nrdist = 1, ncdist = 0;
   if (rpf) {
@@ -384,6 +495,11 @@ static int cmd_blas(int argc, char **argv)
   if (in[0].dense) {
     fprintf(stderr, "This build multiplies a sparse first matrix only\n");
     err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    goto END;
+  }
+  if (!in[1].dense && dmat && dmatB) {
+    err = blas_resident_sparse(dmat, dmatB, &in[0], &in[1], outfile, runs, times,
+        &aux, &op, &io);
     goto END;
   }
   if (!in[1].dense) {
@@ -440,8 +556,9 @@ static int cmd_blas(int argc, char **argv)
   }
   size_t csz = (size_t)anrows * bncols;
   out = calloc(csz ? csz : 1, sizeof(real_t));
-  err = gooseberry_matrix_create(anrows, ancols, in[0].rowptr, in[0].rowind,
-      in[0].rowval, NULL, &dmat);
+  if (!dmat)
+    err = gooseberry_matrix_create(anrows, ancols, in[0].rowptr, in[0].rowind,
+        in[0].rowval, NULL, &dmat);
   /* B is read as bcol[j*ancols + l] (src/blas.c:87); keep that bound */
   size_t bneed = bncols ? (size_t)(bncols - 1) * ancols + ancols : 0;
   if (err == GOOSEBERRY_SUCCESS)
@@ -478,6 +595,7 @@ END:
   gooseberry_device_free(d_b);
   gooseberry_device_free(d_c);
   gooseberry_matrix_free(dmat);
+  gooseberry_matrix_free(dmatB);
   free(out);
   free(rperm); free(cperm); free(rdist); free(cdist);
   mat_free(&in[0]);
@@ -564,7 +682,9 @@ static int cmd_cgd(int argc, char **argv)
   if (err != GOOSEBERRY_SUCCESS) return err;
 
   if (times) tmr_start(&io);
-  if ((err = read_input(matfile, &mat)) != GOOSEBERRY_SUCCESS) goto END;
+  err = (!rpf && !cpf && device_readable(matfile)) ? read_resident(matfile, &dmat, &mat)
+                                                  : read_input(matfile, &mat);
+  if (err != GOOSEBERRY_SUCCESS) goto END;
   if ((err = read_input(vecfile, &vec)) != GOOSEBERRY_SUCCESS) goto END;
   printf("matrix = %" PRIu32 "x%" PRIu32 " vector = %" PRIu32 "x%" PRIu32 "\n",
       mat.nrows, mat.ncols, vec.nrows, vec.ncols);
@@ -602,8 +722,9 @@ static int cmd_cgd(int argc, char **argv)
   }
   const dim_t n = mat.nrows;
   out = calloc(n ? n : 1, sizeof(real_t));
-  err = gooseberry_matrix_create(n, n, mat.rowptr, mat.rowind, mat.rowval, NULL,
-      &dmat);
+  if (!dmat)
+    err = gooseberry_matrix_create(n, n, mat.rowptr, mat.rowind, mat.rowval, NULL,
+        &dmat);
   if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_b, n * sizeof(real_t));
   if (err == GOOSEBERRY_SUCCESS) err = gooseberry_memcpy_htod(d_b, vec.rowval, n * sizeof(real_t));
   if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_x, n * sizeof(real_t));
@@ -725,7 +846,18 @@ static int cmd_pagerank(int argc, char **argv)
   if (err != GOOSEBERRY_SUCCESS) return err;
 
   if (times) tmr_start(&io);
-  if ((err = read_input(matfile, &mat)) != GOOSEBERRY_SUCCESS) goto END;
+  if (!pf && device_readable(matfile)) {
+    err = read_resident(matfile, &dmat, &mat);
+    if (err == GOOSEBERRY_SUCCESS && mat.nrows != mat.ncols) {
+      /* squaring (below) works on the host arrays */
+      err = download_resident(dmat, &mat);
+      gooseberry_matrix_free(dmat);
+      dmat = NULL;
+    }
+  } else {
+    err = read_input(matfile, &mat);
+  }
+  if (err != GOOSEBERRY_SUCCESS) goto END;
   if (mat.dense) {
     /* the reference also accepts dense matrices; this build runs the
      * sparse path only */
@@ -755,7 +887,8 @@ static int cmd_pagerank(int argc, char **argv)
     if (err != GOOSEBERRY_SUCCESS) goto END;
   }
   out = calloc(n ? n : 1, sizeof(real_t));
-  err = gooseberry_matrix_create(n, n, mat.rowptr, mat.rowind, mat.rowval, NULL, &dmat);
+  if (!dmat)
+    err = gooseberry_matrix_create(n, n, mat.rowptr, mat.rowind, mat.rowval, NULL, &dmat);
   if (err == GOOSEBERRY_SUCCESS) err = gooseberry_device_alloc(&d_out, (n ? n : 1) * sizeof(real_t));
   if (err != GOOSEBERRY_SUCCESS) goto END;
   if (times) {
