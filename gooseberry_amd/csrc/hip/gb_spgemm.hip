@@ -5,18 +5,36 @@
 // arowind[j] of B entry by entry (k): the product x_j * b_k goes to column
 // browind[k] of C.  Each C element is the sum of its products in that scan
 // order; in the sparse result the columns of a row appear in the order of
-// their first product.  Here every product is materialised with its scan
-// position t (row-major, so t orders rows too), then
-//   1. stable counting sort of t by column, then by row: (row, col, t) order
-//   2. one sequential sum per (row, col) run, in t order (the first product,
-//      then += the others; the dense variant adds them to +0.0 as the
-//      reference's zeroed row does)
-//   3. dense result: scatter the sums; sparse result: the runs sorted by
-//      their first t (unique, row-major), so each row lists its columns by
-//      first appearance.
+// their first product.  A row's products are numbered t = 0, 1, ... in that
+// scan order.  Rows take one of two paths:
+//
+// Short rows (at most 64 products from at most 64 entries of A): one
+// wavefront per row, lane t holding product t.  Every lane compares its
+// column with the other lanes' (read one after another, in t order): a lane
+// with no earlier equal column is the head of its column, and its sum is its
+// own product followed by the later equal ones, added in t order.  The heads
+// in lane order are the columns in first-appearance order (a ballot gives
+// each head its rank).
+//
+// Other rows: every product is materialised at its position t (numbered
+// over these rows, row-major), then
+//   1. stable radix sort of t by column, then by row: (row, col, t) order
+//   2. one sequential sum per (row, col) run, in t order
+//   3. the runs sorted by their first t (unique, row-major), so each row
+//      lists its columns by first appearance.
+//
+// Both paths add a column's products one after another in t order: the
+// first product, then += the others (the dense variant starts from +0.0 as
+// the reference's zeroed row does, so a lone -0.0 becomes +0.0).  The sparse
+// result is built in two passes: counts per row (their scan gives the row
+// pointers), then the entries.
 #include <stdlib.h>
 
 #include "gb_common.h"
+
+#define GB_SG_SHORT 64  // products (and A entries) of a wavefront row
+
+enum { GB_SG_COUNT = 0, GB_SG_FILL = 1, GB_SG_DENSE = 2 };
 
 // products per A entry: the length of the B row it selects
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -26,6 +44,123 @@ gb_sg_len(uint64_t annz, const uint32_t *__restrict__ arowind,
        j += (uint64_t)gridDim.x * GB_BLOCK) {
     const uint32_t l = arowind[j];
     len[j] = browptr[l + 1] - browptr[l];
+  }
+}
+
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_sg_sum64(const uint32_t *__restrict__ a, uint64_t n, unsigned long long *__restrict__ total) {
+  __shared__ unsigned long long lds[GB_WAVES_PER_BLOCK];
+  unsigned long long v = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * GB_BLOCK)
+    v += a[i];
+  v = gb_block_sum(v, lds);
+  if (threadIdx.x == 0) atomicAdd(total, v);
+}
+
+// row classes: isshort[i] = 1 for a wavefront row, islong[i] = 1 for a row
+// of the sort path (off = exclusive scan of the products per A entry)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_sg_classify(uint32_t anrows, const uint32_t *__restrict__ arowptr,
+               const uint32_t *__restrict__ off, uint32_t *__restrict__ isshort,
+               uint32_t *__restrict__ islong) {
+  for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < anrows;
+       i += gridDim.x * GB_BLOCK) {
+    const uint32_t j0 = arowptr[i], j1 = arowptr[i + 1];
+    const uint32_t p = off[j1] - off[j0];
+    const bool s = p <= GB_SG_SHORT && j1 - j0 <= GB_SG_SHORT;
+    isshort[i] = s ? 1u : 0u;
+    islong[i] = s ? 0u : 1u;
+  }
+}
+
+// list[pos[i]] = i for flagged rows (pos = exclusive scan of the flags)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_sg_compact(uint32_t n, const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos,
+              uint32_t *__restrict__ list) {
+  for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n; i += gridDim.x * GB_BLOCK)
+    if (flag[i]) list[pos[i]] = i;
+}
+
+__device__ __forceinline__ double gb_sg_readlane(double v, uint32_t lane) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// One wavefront per short row.  COUNT: cnt[i] = distinct columns; FILL:
+// the row's entries at crowptr[i]; DENSE: the sums into cdense row i.
+template <int MODE>
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_sg_wave(uint32_t nlist, const uint32_t *__restrict__ list,
+           const uint32_t *__restrict__ arowptr, const uint32_t *__restrict__ arowind,
+           const double *__restrict__ arowval, const uint32_t *__restrict__ browptr,
+           const uint32_t *__restrict__ browind, const double *__restrict__ browval,
+           const uint32_t *__restrict__ crowptr, uint32_t *__restrict__ cnt,
+           uint32_t *__restrict__ crowind, double *__restrict__ crowval,
+           double *__restrict__ cdense, uint32_t bncols) {
+  const uint32_t w = blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE;
+  const uint32_t lane = threadIdx.x & (GB_WAVE - 1);
+  if (w >= nlist) return;  // the whole wavefront
+  // one row per wavefront: its scalars live in scalar registers
+  const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)list[w]);
+  const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)arowptr[i]);
+  const uint32_t m = (uint32_t)__builtin_amdgcn_readfirstlane((int)arowptr[i + 1]) - j0;
+  // lane a < m: A entry j0 + a and the B row it selects
+  uint32_t bstart = 0, blen = 0;
+  double x = 0.0;
+  if (lane < m) {
+    const uint32_t l = arowind[j0 + lane];
+    bstart = browptr[l];
+    blen = browptr[l + 1] - bstart;
+    x = arowval[j0 + lane];
+  }
+  const uint32_t incl = gb_wave_scan_incl(blen);
+  const uint32_t excl = incl - blen;
+  const uint32_t P = (uint32_t)__builtin_amdgcn_readlane((int)incl, GB_WAVE - 1);  // <= 64
+  // product t = lane comes from the last A entry a whose products start at
+  // or before t (entries selecting empty rows start where the next starts)
+  uint32_t a = 0;
+  for (uint32_t u = 0; u < m; ++u)
+    if ((uint32_t)__builtin_amdgcn_readlane((int)excl, (int)u) <= lane) a = u;
+  const uint32_t ex_a = (uint32_t)__shfl((int)excl, (int)a, GB_WAVE);
+  const uint32_t bs_a = (uint32_t)__shfl((int)bstart, (int)a, GB_WAVE);
+  const double x_a = __shfl(x, (int)a, GB_WAVE);
+  const bool live = lane < P;
+  uint32_t col = 0xffffffffu;
+  double val = 0.0;
+  if (live) {
+    const uint32_t k = bs_a + (lane - ex_a);
+    col = browind[k];
+    val = x_a * browval[k];
+  }
+  bool head = live;
+  double sum = MODE == GB_SG_DENSE ? 0.0 + val : val;
+  for (uint32_t u = 0; u < P; ++u) {
+    const uint32_t cu = (uint32_t)__builtin_amdgcn_readlane((int)col, (int)u);
+    if (MODE == GB_SG_COUNT) {
+      if (cu == col && u < lane) head = false;
+    } else {
+      const double vu = gb_sg_readlane(val, u);
+      if (cu == col) {
+        if (u < lane) head = false;
+        else if (u > lane) sum += vu;
+      }
+    }
+  }
+  const uint64_t heads = __ballot(head);
+  if (MODE == GB_SG_COUNT) {
+    if (lane == 0) cnt[i] = (uint32_t)__popcll(heads);
+  } else if (head) {
+    if (MODE == GB_SG_FILL) {
+      const uint32_t rank = (uint32_t)__popcll(heads & ((1ull << lane) - 1ull));
+      const uint32_t o = crowptr[i] + rank;
+      crowind[o] = col;
+      crowval[o] = sum;
+    } else {
+      cdense[(size_t)i * bncols + col] = sum;
+    }
   }
 }
 
@@ -40,6 +175,18 @@ __device__ __forceinline__ uint32_t gb_sg_row_of(const uint32_t *arowptr, uint32
   return lo;
 }
 
+// products per A entry of the sort path's rows (0 for short rows)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_sg_len_long(uint32_t anrows, uint64_t annz, const uint32_t *__restrict__ arowptr,
+               const uint32_t *__restrict__ islong, const uint32_t *__restrict__ len,
+               uint32_t *__restrict__ len2) {
+  for (uint64_t j = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; j < annz;
+       j += (uint64_t)gridDim.x * GB_BLOCK) {
+    const uint32_t i = gb_sg_row_of(arowptr, anrows, (uint32_t)j);
+    len2[j] = islong[i] ? len[j] : 0u;
+  }
+}
+
 // the products of A entry j (row i, column l) with B row l, at positions
 // off[j] .. off[j] + len[j] - 1: their scan order
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -47,31 +194,23 @@ gb_sg_expand(uint32_t anrows, uint64_t annz, const uint32_t *__restrict__ arowpt
              const uint32_t *__restrict__ arowind, const double *__restrict__ arowval,
              const uint32_t *__restrict__ browptr, const uint32_t *__restrict__ browind,
              const double *__restrict__ browval, const uint32_t *__restrict__ off,
-             uint32_t *__restrict__ erow, uint32_t *__restrict__ ecol,
-             double *__restrict__ eval) {
+             const uint32_t *__restrict__ len, uint32_t *__restrict__ erow,
+             uint32_t *__restrict__ ecol, double *__restrict__ eval) {
   for (uint64_t j = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; j < annz;
        j += (uint64_t)gridDim.x * GB_BLOCK) {
+    const uint32_t nk = len[j];
+    if (!nk) continue;
     const uint32_t i = gb_sg_row_of(arowptr, anrows, (uint32_t)j);
     const uint32_t l = arowind[j];
     const double x = arowval[j];
+    const uint32_t k0 = browptr[l];
     uint32_t t = off[j];
-    for (uint32_t k = browptr[l]; k < browptr[l + 1]; ++k, ++t) {
+    for (uint32_t k = k0; k < k0 + nk; ++k, ++t) {
       erow[t] = i;
       ecol[t] = browind[k];
       eval[t] = x * browval[k];
     }
   }
-}
-
-__global__ void __launch_bounds__(GB_BLOCK)
-gb_sg_sum64(const uint32_t *__restrict__ a, uint64_t n, unsigned long long *__restrict__ total) {
-  __shared__ unsigned long long lds[GB_WAVES_PER_BLOCK];
-  unsigned long long v = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * GB_BLOCK)
-    v += a[i];
-  v = gb_block_sum(v, lds);
-  if (threadIdx.x == 0) atomicAdd(total, v);
 }
 
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -131,28 +270,40 @@ gb_sg_runsum(uint32_t nruns, uint32_t n, const uint32_t *__restrict__ runstart,
   }
 }
 
-// row pointers of C from the runs' rows (ascending in run order): thread q
-// writes the pointers of the rows between run q-1's row and run q's row
+// runs are in (row, col) order: rbeg[r] = the first run of row r and
+// cnt[r] = its number of runs, for every row that has runs (a row's runs
+// are contiguous; its last one is found by binary search)
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_sg_rowptr(uint32_t nruns, uint32_t anrows, const uint32_t *__restrict__ rrow,
-             uint32_t *__restrict__ crowptr) {
-  for (uint32_t q = blockIdx.x * GB_BLOCK + threadIdx.x; q <= nruns;
+gb_sg_rowruns(uint32_t nruns, const uint32_t *__restrict__ rrow, uint32_t *__restrict__ rbeg,
+              uint32_t *__restrict__ cnt) {
+  for (uint32_t q = blockIdx.x * GB_BLOCK + threadIdx.x; q < nruns;
        q += gridDim.x * GB_BLOCK) {
-    const uint32_t lo = q == 0 ? 0u : rrow[q - 1] + 1u;
-    const uint32_t hi = q == nruns ? anrows : rrow[q];
-    for (uint32_t i = lo; i <= hi; ++i) crowptr[i] = q;
+    const uint32_t r = rrow[q];
+    if (q == 0 || rrow[q - 1] != r) {
+      uint32_t lo = q, hi = nruns - 1;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (rrow[mid] == r) lo = mid; else hi = mid - 1;
+      }
+      rbeg[r] = q;
+      cnt[r] = lo + 1 - q;
+    }
   }
 }
 
-// runs in first-appearance order (ordr) written out as the C arrays
+// runs in first-appearance order (ordr): run q is entry q - rbeg[r] of its
+// row r (the order is row-major, so a row's runs are q = rbeg[r] ..)
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_sg_out(uint32_t nruns, const uint32_t *__restrict__ ordr, const uint32_t *__restrict__ rcol,
-          const double *__restrict__ rval, uint32_t *__restrict__ crowind,
-          double *__restrict__ crowval) {
+gb_sg_out(uint32_t nruns, const uint32_t *__restrict__ ordr, const uint32_t *__restrict__ rrow,
+          const uint32_t *__restrict__ rcol, const double *__restrict__ rval,
+          const uint32_t *__restrict__ rbeg, const uint32_t *__restrict__ crowptr,
+          uint32_t *__restrict__ crowind, double *__restrict__ crowval) {
   for (uint32_t q = blockIdx.x * GB_BLOCK + threadIdx.x; q < nruns; q += gridDim.x * GB_BLOCK) {
-    const uint32_t r = ordr[q];
-    crowind[q] = rcol[r];
-    crowval[q] = rval[r];
+    const uint32_t sidx = ordr[q];
+    const uint32_t r = rrow[sidx];
+    const uint32_t o = crowptr[r] + (q - rbeg[r]);
+    crowind[o] = rcol[sidx];
+    crowval[o] = rval[sidx];
   }
 }
 
@@ -183,12 +334,13 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
   if (!dense && (!d_crowptr || !d_crowind || !d_crowval || !r_cnnz))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   hipStream_t s = gb_stream();
-  uint32_t annz = 0, E = 0, nruns = 0;
-  uint32_t *len = nullptr, *off = nullptr, *erow = nullptr, *ecol = nullptr,
-           *perm1 = nullptr, *row1 = nullptr, *ord = nullptr, *flag = nullptr,
-           *rid = nullptr, *runstart = nullptr, *rrow = nullptr, *rcol = nullptr,
-           *rfirst = nullptr, *ordr = nullptr, *crowptr = nullptr,
-           *crowind = nullptr;
+  uint32_t annz = 0, E = 0, nruns = 0, nshort = 0, nlong = 0, cnnz = 0;
+  uint32_t *len = nullptr, *off = nullptr, *isshort = nullptr, *islong = nullptr,
+           *slist = nullptr, *len2 = nullptr, *off2 = nullptr, *erow = nullptr,
+           *ecol = nullptr, *perm1 = nullptr, *row1 = nullptr, *ord = nullptr,
+           *flag = nullptr, *rid = nullptr, *runstart = nullptr, *rrow = nullptr,
+           *rcol = nullptr, *rfirst = nullptr, *ordr = nullptr, *rbeg = nullptr,
+           *cnt = nullptr, *crowptr = nullptr, *crowind = nullptr;
   double *eval = nullptr, *rval = nullptr, *crowval = nullptr;
   unsigned long long e64 = 0, *tot = nullptr;
   hipError_t he = hipSuccess;
@@ -234,8 +386,51 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
     rv = GOOSEBERRY_ERROR_INVALIDINPUT;
     goto done;
   }
-  E = (uint32_t)e64;
   GB_SG_RV(gbd_scan_u32(len, off, annz, off + annz));
+  // row classes and the list of wavefront rows
+  GB_SG_TRY(gb_dev_alloc((void **)&isshort, ((size_t)anrows + 1) * 4));
+  GB_SG_TRY(gb_dev_alloc((void **)&islong, ((size_t)anrows + 1) * 4));
+  GB_SG_TRY(gb_dev_alloc((void **)&slist, ((size_t)anrows + 1) * 4));
+  GB_SG_TRY(gb_dev_alloc((void **)&cnt, ((size_t)anrows + 1) * 4));
+  if (anrows) {
+    gb_sg_classify<<<gb_sg_grid(anrows), GB_BLOCK, 0, s>>>(anrows, d_arowptr, off, isshort,
+                                                          islong);
+    GB_SG_TRY(hipGetLastError());
+  }
+  // cnt holds the list positions until the counts replace them
+  GB_SG_RV(gbd_scan_u32(isshort, cnt, anrows, cnt + anrows));
+  GB_SG_TRY(hipMemcpyAsync(&nshort, cnt + anrows, 4, hipMemcpyDeviceToHost, s));
+  GB_SG_TRY(hipStreamSynchronize(s));
+  nlong = anrows - nshort;
+  if (nshort) {
+    gb_sg_compact<<<gb_sg_grid(anrows), GB_BLOCK, 0, s>>>(anrows, isshort, cnt, slist);
+    GB_SG_TRY(hipGetLastError());
+  }
+  if (!dense) {
+    GB_SG_RV(gbd_zero(cnt, ((size_t)anrows + 1) * 4));
+    if (nshort) {
+      gb_sg_wave<GB_SG_COUNT><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
+          nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
+          nullptr, cnt, nullptr, nullptr, nullptr, 0);
+      GB_SG_TRY(hipGetLastError());
+    }
+  } else if (nshort) {
+    gb_sg_wave<GB_SG_DENSE><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
+        nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
+        nullptr, nullptr, nullptr, nullptr, d_cdense, bncols);
+    GB_SG_TRY(hipGetLastError());
+  }
+  // the sort path's products, numbered over its rows
+  if (nlong && annz) {
+    GB_SG_TRY(gb_dev_alloc((void **)&len2, ((size_t)annz + 1) * 4));
+    GB_SG_TRY(gb_dev_alloc((void **)&off2, ((size_t)annz + 1) * 4));
+    gb_sg_len_long<<<gb_sg_grid(annz), GB_BLOCK, 0, s>>>(anrows, annz, d_arowptr, islong, len,
+                                                        len2);
+    GB_SG_TRY(hipGetLastError());
+    GB_SG_RV(gbd_scan_u32(len2, off2, annz, off2 + annz));
+    GB_SG_TRY(hipMemcpyAsync(&E, off2 + annz, 4, hipMemcpyDeviceToHost, s));
+    GB_SG_TRY(hipStreamSynchronize(s));
+  }
   if (E) {
     GB_SG_TRY(gb_dev_alloc((void **)&erow, (size_t)E * 4));
     GB_SG_TRY(gb_dev_alloc((void **)&ecol, (size_t)E * 4));
@@ -246,8 +441,8 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
     GB_SG_TRY(gb_dev_alloc((void **)&flag, (size_t)E * 4));
     GB_SG_TRY(gb_dev_alloc((void **)&rid, ((size_t)E + 1) * 4));
     gb_sg_expand<<<gb_sg_grid(annz), GB_BLOCK, 0, s>>>(
-        anrows, annz, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval, off,
-        erow, ecol, eval);
+        anrows, annz, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval, off2,
+        len2, erow, ecol, eval);
     GB_SG_TRY(hipGetLastError());
     // 1. (row, col, t) order: stable by column, then stable by row
     GB_SG_RV(gbd_countingsort_kv_unchecked(ecol, nullptr, 0, bncols ? bncols - 1 : 0, E,
@@ -276,9 +471,12 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
       GB_SG_TRY(gb_dev_alloc((void **)&rfirst, ((size_t)nruns + 1) * 4));
       GB_SG_TRY(gb_dev_alloc((void **)&rval, ((size_t)nruns + 1) * 8));
       GB_SG_TRY(gb_dev_alloc((void **)&ordr, ((size_t)nruns + 1) * 4));
+      GB_SG_TRY(gb_dev_alloc((void **)&rbeg, ((size_t)anrows + 1) * 4));
       gb_sg_runsum<false><<<gb_sg_grid(nruns), GB_BLOCK, 0, s>>>(
           nruns, E, runstart, ord, erow, ecol, eval, bncols, nullptr, rrow, rcol, rfirst,
           rval);
+      GB_SG_TRY(hipGetLastError());
+      gb_sg_rowruns<<<gb_sg_grid(nruns), GB_BLOCK, 0, s>>>(nruns, rrow, rbeg, cnt);
       GB_SG_TRY(hipGetLastError());
       // 3. first-appearance order: the first positions are unique and
       // row-major, so sorting by them alone orders the rows too
@@ -288,32 +486,39 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
   }
   if (!dense) {
     GB_SG_TRY(gb_dev_alloc((void **)&crowptr, ((size_t)anrows + 1) * 4));
-    GB_SG_TRY(gb_dev_alloc((void **)&crowind, ((size_t)nruns + 1) * 4));
-    GB_SG_TRY(gb_dev_alloc((void **)&crowval, ((size_t)nruns + 1) * 8));
-    gb_sg_rowptr<<<gb_sg_grid((uint64_t)nruns + 1), GB_BLOCK, 0, s>>>(nruns, anrows, rrow,
-                                                                      crowptr);
-    GB_SG_TRY(hipGetLastError());
+    GB_SG_RV(gbd_scan_u32(cnt, crowptr, anrows, crowptr + anrows));
+    GB_SG_TRY(hipMemcpyAsync(&cnnz, crowptr + anrows, 4, hipMemcpyDeviceToHost, s));
+    GB_SG_TRY(hipStreamSynchronize(s));
+    GB_SG_TRY(gb_dev_alloc((void **)&crowind, ((size_t)cnnz + 1) * 4));
+    GB_SG_TRY(gb_dev_alloc((void **)&crowval, ((size_t)cnnz + 1) * 8));
+    if (nshort) {
+      gb_sg_wave<GB_SG_FILL><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
+          nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
+          crowptr, nullptr, crowind, crowval, nullptr, 0);
+      GB_SG_TRY(hipGetLastError());
+    }
     if (nruns) {
-      gb_sg_out<<<gb_sg_grid(nruns), GB_BLOCK, 0, s>>>(nruns, ordr, rcol, rval, crowind,
-                                                       crowval);
+      gb_sg_out<<<gb_sg_grid(nruns), GB_BLOCK, 0, s>>>(nruns, ordr, rrow, rcol, rval, rbeg,
+                                                       crowptr, crowind, crowval);
       GB_SG_TRY(hipGetLastError());
     }
     *d_crowptr = crowptr;
     *d_crowind = crowind;
     *d_crowval = crowval;
-    *r_cnnz = nruns;
+    *r_cnnz = cnnz;
     crowptr = crowind = nullptr;
     crowval = nullptr;
   }
   GB_SG_TRY(hipStreamSynchronize(s));
   rv = GOOSEBERRY_SUCCESS;
 done:
-  gb_dev_free(len); gb_dev_free(off); gb_dev_free(erow); gb_dev_free(ecol);
-  gb_dev_free(eval); gb_dev_free(perm1); gb_dev_free(row1); gb_dev_free(ord);
-  gb_dev_free(flag); gb_dev_free(rid); gb_dev_free(runstart); gb_dev_free(rrow);
-  gb_dev_free(rcol); gb_dev_free(rfirst); gb_dev_free(rval); gb_dev_free(ordr);
-  gb_dev_free(crowptr); gb_dev_free(crowind); gb_dev_free(crowval);
-  gb_dev_free(tot);
+  gb_dev_free(len); gb_dev_free(off); gb_dev_free(isshort); gb_dev_free(islong);
+  gb_dev_free(slist); gb_dev_free(len2); gb_dev_free(off2); gb_dev_free(erow);
+  gb_dev_free(ecol); gb_dev_free(eval); gb_dev_free(perm1); gb_dev_free(row1);
+  gb_dev_free(ord); gb_dev_free(flag); gb_dev_free(rid); gb_dev_free(runstart);
+  gb_dev_free(rrow); gb_dev_free(rcol); gb_dev_free(rfirst); gb_dev_free(rval);
+  gb_dev_free(ordr); gb_dev_free(rbeg); gb_dev_free(cnt); gb_dev_free(crowptr);
+  gb_dev_free(crowind); gb_dev_free(crowval); gb_dev_free(tot);
   return rv;
 #undef GB_SG_TRY
 #undef GB_SG_RV
