@@ -69,6 +69,7 @@ EXPORTED = [
     "gooseberry_spmultsp", "gooseberry_spmultsp_sp", "gooseberry_matrix_spmultsp",
     "gooseberry_matrix_spmultsp_dense", "gooseberry_read_sparse_matrix_device",
     "gooseberry_parse_csr_device", "gooseberry_matrix_read",
+    "gooseberry_permute_cuthillmckee", "gooseberry_matrix_cuthillmckee",
 ]
 
 PRECISION_FP64 = 0
@@ -147,6 +148,8 @@ def _sig(lib):
         "gooseberry_parse_csr_device": (i32, [_P, sz, _U32P, _U32P, C.POINTER(_P),
                                               C.POINTER(_P), C.POINTER(_P)]),
         "gooseberry_matrix_read": (i32, [i32, C.c_char_p, _P, C.POINTER(_P)]),
+        "gooseberry_permute_cuthillmckee": (i32, [u32, u32, _P, _P, _P, _P, u32, _P]),
+        "gooseberry_matrix_cuthillmckee": (i32, [_P, _P, u32, _P]),
         "gooseberry_matrix_spmultsp_dense": (i32, [_P, _P, _P]),
         "gooseberry_spmultsp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, _P, _P, u32]),
         "gooseberry_spmultsp_sp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P,
@@ -445,6 +448,15 @@ class Matrix:
               "gooseberry_matrix_spmm")
         return dc.numpy()
 
+    def cuthillmckee(self, blocks=None):
+        """gooseberry_matrix_cuthillmckee, copied back to the host."""
+        d = DeviceArray(self.nrows, np.uint32)
+        b = _u32(blocks)
+        check(lib().gooseberry_matrix_cuthillmckee(self.h, _ptr(b),
+                                                   0 if b is None else b.size - 1, d.ptr),
+              "gooseberry_matrix_cuthillmckee")
+        return d.numpy()
+
     def spmultsp(self, other, plan=None):
         """C = self @ other as a new resident Matrix (rows list their columns
         in the order of their first product)."""
@@ -588,6 +600,19 @@ def permute_sparse(rowptr, rowind, rowval, ncols, rowperm=None, colperm=None):
                                           _ptr(_u32(colperm))),
           "gooseberry_permute_sparse")
     return rowptr, rowind, rowval
+
+
+def cuthillmckee(rowptr, rowind, ncols=None, blocks=None, perm=None):
+    """gooseberry_permute_cuthillmckee: perm[p] = the row placed at p."""
+    rowptr, rowind = _u32(rowptr), _u32(rowind)
+    n = rowptr.size - 1
+    out = np.zeros(n, dtype=np.uint32) if perm is None else _u32(perm).copy()
+    b = _u32(blocks)
+    check(lib().gooseberry_permute_cuthillmckee(n, n if ncols is None else ncols, _ptr(rowptr),
+                                                _ptr(rowind), None, _ptr(b),
+                                                0 if b is None else b.size - 1, _ptr(out)),
+          "gooseberry_permute_cuthillmckee")
+    return out
 
 
 def permute_dense(vals, nrows, ncols, rowperm=None, colperm=None, rev=False):

@@ -23,7 +23,7 @@ CC = os.environ.get("CC", "gcc")
 
 HIP_SOURCES = ["gb_runtime.hip", "gb_index.hip", "gb_spmv.hip", "gb_cg.hip",
                "gb_batch.hip", "gb_gen.hip", "gb_pagerank.hip", "gb_spgemm.hip",
-               "gb_ingest.hip"]
+               "gb_ingest.hip", "gb_cm.hip"]
 C_SOURCES = ["gb_api.c", "gb_io.c", "gb_gen.c", "gb_batch_api.c", "gb_tune.c",
              "gb_ingest_api.c"]
 

@@ -249,6 +249,14 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
     uint32_t **d_crowptr, uint32_t **d_crowind, double **d_crowval,
     uint32_t *r_cnnz);
 
+/* src/permute.c:49-110,288-332: Cuthill-McKee order of a square CSR's rows
+ * (device arrays), per diagonal block when blocks (host array, nblocks + 1
+ * monotone entries within [0, nrows]) are given: d_perm[p] = the row at
+ * position p.  Positions outside the blocks are left as they are. */
+int gbd_cuthillmckee(uint32_t nrows, const uint32_t *d_rowptr,
+    const uint32_t *d_rowind, const uint32_t *blocks, uint32_t nblocks,
+    uint32_t *d_perm);
+
 /* ------------------------------------------------------------ ingest */
 /* The whole file into new device memory (*d_txt, len + 16 bytes; free
  * with gbd_free), through pinned staging buffers. */

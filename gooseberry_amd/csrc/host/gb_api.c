@@ -4,6 +4,7 @@
  */
 #include <math.h>
 #include <stdio.h>
+#include <inttypes.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -514,6 +515,59 @@ int gooseberry_matrix_spmultsp_dense(gooseberry_matrix_t a,
       gbd_matrix_rowind(a->m), gbd_matrix_rowval(a->m),
       gbd_matrix_rowptr(b->m), gbd_matrix_rowind(b->m),
       gbd_matrix_rowval(b->m), d_c, NULL, NULL, NULL, NULL);
+}
+
+/* ------------------------------------------------------- Cuthill-McKee */
+
+static int cm_blocks_ok(dim_t nrows, const dim_t *blocks, dim_t nblocks)
+{
+  if (!blocks) return 1;
+  for (dim_t c = 0; c < nblocks; ++c)
+    if (blocks[c + 1] < blocks[c]) return 0;
+  return nblocks == 0 || blocks[nblocks] <= nrows;
+}
+
+int gooseberry_permute_cuthillmckee(dim_t nrows, dim_t ncols,
+    const ind_t *rowptr, const dim_t *rowind, const real_t *rowval,
+    const dim_t *blocks, dim_t nblocks, dim_t *perm)
+{
+  (void)rowval;
+  if (nrows != ncols) {
+    fprintf(stderr, "Cuthill-McKee requires a structurally symmetric matrix: "
+        "Given %" PRIu32 "x%" PRIu32 "\n", nrows, ncols);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (!perm || !cm_blocks_ok(nrows, blocks, nblocks))
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int rv = gb_check_csr(nrows, ncols, rowptr, rowind);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (nrows == 0) return GOOSEBERRY_SUCCESS;
+  void *drp = NULL, *dri = NULL, *dp = NULL;
+  const size_t nnz = rowptr[nrows];
+  rv = gb_upload(&drp, rowptr, ((size_t)nrows + 1) * sizeof(ind_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dri, rowind, nnz * sizeof(dim_t));
+  /* positions outside the blocks keep the caller's values */
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dp, perm, (size_t)nrows * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_cuthillmckee(nrows, drp, dri, blocks, nblocks, dp);
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtoh(perm, dp, (size_t)nrows * sizeof(dim_t));
+  gbd_free(drp);
+  gbd_free(dri);
+  gbd_free(dp);
+  return rv;
+}
+
+int gooseberry_matrix_cuthillmckee(gooseberry_matrix_t mat,
+    const dim_t *blocks, dim_t nblocks, dim_t *d_perm)
+{
+  if (!mat || !d_perm) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t nrows, ncols;
+  gbd_matrix_info(mat->m, &nrows, &ncols, NULL, NULL, NULL, NULL, NULL, NULL);
+  if (nrows != ncols || !cm_blocks_ok(nrows, blocks, nblocks))
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  return gbd_cuthillmckee(nrows, gbd_matrix_rowptr(mat->m),
+      gbd_matrix_rowind(mat->m), blocks, nblocks, d_perm);
 }
 
 /* ------------------------------------------------------------------- CG */

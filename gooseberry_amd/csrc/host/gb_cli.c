@@ -260,14 +260,13 @@ static const opt_t BLAS_OPTS[] = {
 };
 #define NBLAS_OPTS (sizeof(BLAS_OPTS) / sizeof(BLAS_OPTS[0]))
 
-/* sparse x sparse (src/gooseberry_bin.c:1727-1750): the output file's
- * type picks the dense (blas_spmultsp) or sparse (blas_spmultsp_sp)
- * product; rows are unpermuted afterwards as matrix_unpermute does
- * (src/matrix.c:348-378) */
+/* sparse x sparse (src/gooseberry_bin.c:1727-1750) on inputs already
+ * permuted: the output file's type picks the dense (blas_spmultsp) or
+ * sparse (blas_spmultsp_sp) product; rows are unpermuted afterwards as
+ * matrix_unpermute does (src/matrix.c:348-378) */
 static int blas_sparse_sparse(mat_t *a, mat_t *b, const char *outfile,
-    const dim_t *rperm, const dim_t *cperm, const dim_t *blocks,
-    dim_t nblocks, long long runs, int times, tmr_t *io, tmr_t *aux,
-    tmr_t *op)
+    const dim_t *rperm, const dim_t *blocks, dim_t nblocks, long long runs,
+    int times, tmr_t *io, tmr_t *aux, tmr_t *op)
 {
   int err = GOOSEBERRY_SUCCESS;
   const int otype = file_type(outfile);
@@ -276,12 +275,6 @@ static int blas_sparse_sparse(mat_t *a, mat_t *b, const char *outfile,
   ind_t *cptr = NULL;
   dim_t *cind = NULL, *rren = NULL;
   real_t *cval = NULL;
-  if (cperm) {
-    if (b->nrows != ancols) return GOOSEBERRY_ERROR_INVALIDINPUT;
-    err = gooseberry_permute_sparse(b->nrows, b->ncols, b->rowptr, b->rowind,
-        b->rowval, (dim_t *)cperm, NULL);
-    if (err != GOOSEBERRY_SUCCESS) return err;
-  }
   if (ancols > b->nrows) {
     fprintf(stderr, "Matrix dimensions do not match for multiplication: "
         "%" PRIu32 "x%" PRIu32 " and %" PRIu32 "x%" PRIu32 "\n",
@@ -417,7 +410,7 @@ static int cmd_blas(int argc, char **argv)
   size_t nin = 0;
   mat_t in[2];
   memset(in, 0, sizeof(in));
-  dim_t *rperm = NULL, *cperm = NULL, *rdist = NULL, *cdist = NULL;
+  dim_t *rperm = NULL, *cperm = NULL, *rdist = NULL, *cdist = NULL, *bperm = NULL;
   real_t *out = NULL;
   void *d_b = NULL, *d_c = NULL;
   gooseberry_matrix_t dmat = NULL, dmatB = NULL;
@@ -482,11 +475,6 @@ static int cmd_blas(int argc, char **argv)
     printf("Read %" PRIu32 "-way column partition\n", ncdist);
   }
   if (times) { tmr_stop(&io); tmr_start(&aux); }
-  if (redband) {
-    fprintf(stderr, "Bandwidth reduction (-b) is not provided by this build\n");
-    err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
-    goto END;
-  }
   if (oper != 0) {
     fprintf(stderr, "Unknown operation '%d'\n", oper);
     err = GOOSEBERRY_ERROR_INVALIDINPUT;
@@ -502,25 +490,14 @@ static int cmd_blas(int argc, char **argv)
         &aux, &op, &io);
     goto END;
   }
-  if (!in[1].dense) {
-    /* matrix_permute(in[0], rperm, cperm) here, in[1] in the helper */
-    if (rperm || cperm) {
-      err = gooseberry_permute_sparse(in[0].nrows, in[0].ncols, in[0].rowptr,
-          in[0].rowind, in[0].rowval, rperm, cperm);
-      if (err != GOOSEBERRY_SUCCESS) goto END;
-    }
-    err = blas_sparse_sparse(&in[0], &in[1], outfile, rperm, cperm,
-        rpf ? rdist : NULL, rpf ? nrdist : 0, runs, times, &io, &aux, &op);
-    goto END;
-  }
-  if (!is_dense(file_type(outfile))) {
+  if (in[1].dense && !is_dense(file_type(outfile))) {
     fprintf(stderr, "A sparse-times-dense product needs a dense output file\n");
     err = GOOSEBERRY_ERROR_INVALIDINPUT;
     goto END;
   }
-  /* matrix_permute(in[0], rperm, cperm); matrix_permute(in[1], cperm, NULL) */
+  /* matrix_permute(in[0], rperm, cperm); matrix_permute(in[1], cperm, NULL)
+   * (src/gooseberry_bin.c:1655-1656) */
   if (rperm || cperm) {
-    if (rperm && in[0].nrows != in[0].ncols && cperm == NULL) { /* fine */ }
     err = gooseberry_permute_sparse(in[0].nrows, in[0].ncols, in[0].rowptr,
         in[0].rowind, in[0].rowval, rperm, cperm);
     if (err != GOOSEBERRY_SUCCESS) goto END;
@@ -529,10 +506,47 @@ static int cmd_blas(int argc, char **argv)
         err = GOOSEBERRY_ERROR_INVALIDINPUT;
         goto END;
       }
-      err = gooseberry_permute_dense(in[1].nrows, in[1].ncols, in[1].rowval,
-          cperm, NULL, 0);
+      err = in[1].dense
+          ? gooseberry_permute_dense(in[1].nrows, in[1].ncols, in[1].rowval, cperm, NULL, 0)
+          : gooseberry_permute_sparse(in[1].nrows, in[1].ncols, in[1].rowptr,
+                in[1].rowind, in[1].rowval, cperm, NULL);
       if (err != GOOSEBERRY_SUCCESS) goto END;
     }
+  }
+  /* -b (src/gooseberry_bin.c:1658-1675): Cuthill-McKee within the row
+   * partition's blocks, applied to rows and columns of A and rows of B;
+   * with both partitions the two permutations are composed with it as the
+   * reference composes them, and only rperm undoes the output's order */
+  if (redband) {
+    const dim_t n = in[0].nrows;
+    bperm = malloc(sizeof(dim_t) * (n ? n : 1));
+    if (!bperm) { err = GOOSEBERRY_ERROR_NOMEMORY; goto END; }
+    err = gooseberry_permute_cuthillmckee(n, in[0].ncols, in[0].rowptr, in[0].rowind,
+        in[0].rowval, rpf ? rdist : NULL, rpf ? nrdist : 0, bperm);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+    if (rperm && cperm) {
+      for (dim_t i = 0; i < n; ++i) rperm[i] = bperm[rperm[i]];
+      for (dim_t i = 0; i < n; ++i) cperm[i] = bperm[cperm[i]];
+    }
+    err = gooseberry_permute_sparse(n, in[0].ncols, in[0].rowptr, in[0].rowind,
+        in[0].rowval, bperm, bperm);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+    if (in[1].nrows != n) {
+      fprintf(stderr, "The second input has %" PRIu32 " rows, the matrix %" PRIu32 "\n",
+          in[1].nrows, n);
+      err = GOOSEBERRY_ERROR_INVALIDINPUT;
+      goto END;
+    }
+    err = in[1].dense
+        ? gooseberry_permute_dense(in[1].nrows, in[1].ncols, in[1].rowval, bperm, NULL, 0)
+        : gooseberry_permute_sparse(in[1].nrows, in[1].ncols, in[1].rowptr,
+              in[1].rowind, in[1].rowval, bperm, NULL);
+    if (err != GOOSEBERRY_SUCCESS) goto END;
+  }
+  if (!in[1].dense) {
+    err = blas_sparse_sparse(&in[0], &in[1], outfile, rperm, rpf ? rdist : NULL,
+        rpf ? nrdist : 0, runs, times, &io, &aux, &op);
+    goto END;
   }
   const dim_t anrows = in[0].nrows, ancols = in[0].ncols;
   const dim_t bnrows = in[1].nrows, bncols = in[1].ncols;
@@ -597,7 +611,7 @@ END:
   gooseberry_matrix_free(dmat);
   gooseberry_matrix_free(dmatB);
   free(out);
-  free(rperm); free(cperm); free(rdist); free(cdist);
+  free(rperm); free(cperm); free(rdist); free(cdist); free(bperm);
   mat_free(&in[0]);
   mat_free(&in[1]);
   return err;

@@ -163,6 +163,24 @@ int gooseberry_spmultsp_sp(
     const dim_t * blocks,
     dim_t nblocks);
 
+/* src/permute.c:288-332 (not in the reference's public header; the CLI's
+ * -b option): the Cuthill-McKee order of a square matrix's rows, per
+ * diagonal block when blocks (nblocks + 1 monotone row boundaries) are
+ * given; perm[p] = the row placed at position p, and positions outside the
+ * blocks keep their values.  Each component starts at its least-degree row
+ * and each row places its unplaced neighbours by ascending degree (entries
+ * in the block counted); equal degrees are taken in ascending row order.
+ * Host arrays; computed on the GPU. */
+int gooseberry_permute_cuthillmckee(
+    dim_t nrows,
+    dim_t ncols,
+    const ind_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval,
+    const dim_t * blocks,
+    dim_t nblocks,
+    dim_t * perm);
+
 /* include/gooseberry.h IO prototypes; src/io.c:1464-1549.  Sparse readers:
  * GOOSEBERRY_FORMAT_CSR, GOOSEBERRY_FORMAT_MATRIXMARKET and
  * GOOSEBERRY_FORMAT_BCSR; sparse writers: CSR and BCSR.  Dense: GRID.
@@ -624,6 +642,14 @@ int gooseberry_matrix_spmultsp_dense(
     gooseberry_matrix_t a,
     gooseberry_matrix_t b,
     real_t * d_c);
+
+/* The Cuthill-McKee order of a resident square matrix into device memory
+ * d_perm (nrows entries); blocks is a host array. */
+int gooseberry_matrix_cuthillmckee(
+    gooseberry_matrix_t mat,
+    const dim_t * blocks,
+    dim_t nblocks,
+    dim_t * d_perm);
 
 /* On a resident square matrix; d_out: nrows values on the device.  The
  * matrix itself is not modified. */

@@ -831,3 +831,164 @@ int orc_spmultsp_sp(dim_t anrows, dim_t ancols, dim_t bncols,
   *r_crowval = crowval;
   return ORC_SUCCESS;
 }
+
+/* ---------------------------------------------------------- Cuthill-McKee */
+
+/* A min-heap of (key, vertex) pairs with membership, standing in for
+ * domlib's dd_pq (DLPQ_MIN, src/permute.c:29-39).  domlib is not in the
+ * reference tree, so its order among equal keys is not known: here equal
+ * keys leave in ascending vertex order. */
+typedef struct {
+  dim_t *key, *val, *pos;  /* pos[v - lo]: heap slot of v, or NONE */
+  dim_t n, lo;
+} orc_heap_t;
+
+#define ORC_NONE ((dim_t)-1)
+
+static int hp_less(const orc_heap_t *h, dim_t a, dim_t b)
+{
+  return h->key[a] < h->key[b] || (h->key[a] == h->key[b] && h->val[a] < h->val[b]);
+}
+
+static void hp_swap(orc_heap_t *h, dim_t a, dim_t b)
+{
+  dim_t k = h->key[a], v = h->val[a];
+  h->key[a] = h->key[b];
+  h->val[a] = h->val[b];
+  h->key[b] = k;
+  h->val[b] = v;
+  h->pos[h->val[a] - h->lo] = a;
+  h->pos[h->val[b] - h->lo] = b;
+}
+
+static void hp_up(orc_heap_t *h, dim_t i)
+{
+  while (i > 0) {
+    dim_t p = (i - 1) / 2;
+    if (!hp_less(h, i, p)) break;
+    hp_swap(h, i, p);
+    i = p;
+  }
+}
+
+static void hp_down(orc_heap_t *h, dim_t i)
+{
+  for (;;) {
+    dim_t l = 2 * i + 1, r = l + 1, m = i;
+    if (l < h->n && hp_less(h, l, m)) m = l;
+    if (r < h->n && hp_less(h, r, m)) m = r;
+    if (m == i) return;
+    hp_swap(h, i, m);
+    i = m;
+  }
+}
+
+static int hp_init(orc_heap_t *h, dim_t lo, dim_t hi)
+{
+  size_t cap = (size_t)(hi - lo) + 1;
+  h->key = malloc(cap * sizeof(dim_t));
+  h->val = malloc(cap * sizeof(dim_t));
+  h->pos = malloc(cap * sizeof(dim_t));
+  h->n = 0;
+  h->lo = lo;
+  if (!h->key || !h->val || !h->pos) return 0;
+  for (size_t i = 0; i < cap; ++i) h->pos[i] = ORC_NONE;
+  return 1;
+}
+
+static void hp_free(orc_heap_t *h)
+{
+  free(h->key); free(h->val); free(h->pos);
+}
+
+static void hp_push(orc_heap_t *h, dim_t key, dim_t v)
+{
+  dim_t i = h->n++;
+  h->key[i] = key;
+  h->val[i] = v;
+  h->pos[v - h->lo] = i;
+  hp_up(h, i);
+}
+
+static dim_t hp_pop(orc_heap_t *h)
+{
+  dim_t v = h->val[0];
+  h->pos[v - h->lo] = ORC_NONE;
+  if (--h->n > 0) {
+    h->key[0] = h->key[h->n];
+    h->val[0] = h->val[h->n];
+    h->pos[h->val[0] - h->lo] = 0;
+    hp_down(h, 0);
+  }
+  return v;
+}
+
+static void hp_remove(orc_heap_t *h, dim_t v)
+{
+  dim_t i = h->pos[v - h->lo];
+  h->pos[v - h->lo] = ORC_NONE;
+  if (i != --h->n) {
+    const dim_t mv = h->val[h->n];  /* the last element fills the hole */
+    h->key[i] = h->key[h->n];
+    h->val[i] = mv;
+    h->pos[mv - h->lo] = i;
+    hp_up(h, i);
+    hp_down(h, h->pos[mv - h->lo]);
+  }
+}
+
+/* src/permute.c:49-110: Cuthill-McKee on the rows [start, end), counting
+ * only the entries whose column lies in the block */
+static int cm_block(dim_t start, dim_t end, const ind_t *rowptr,
+    const dim_t *rowind, dim_t *perm)
+{
+  if (start >= end) return ORC_SUCCESS;
+  orc_heap_t q, rem;
+  dim_t *deg = malloc(sizeof(dim_t) * (size_t)(end - start));
+  int ok = deg && hp_init(&q, start, end) && hp_init(&rem, start, end);
+  if (!ok) { free(deg); return ORC_ERROR_UNKNOWN; }
+  for (dim_t i = start; i < end; ++i) {
+    dim_t d = 0;
+    for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+      dim_t k = rowind[j];
+      if (k < end && k >= start) ++d;
+    }
+    deg[i - start] = d;
+    hp_push(&rem, d, i);
+  }
+  dim_t sr = start, nordered = start;
+  while (rem.n > 0) {
+    dim_t i = hp_pop(&rem);
+    perm[nordered++] = i;
+    while (sr < nordered) {
+      i = perm[sr++];
+      for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+        dim_t k = rowind[j];
+        if (k < end && k >= start && rem.pos[k - start] != ORC_NONE) {
+          hp_remove(&rem, k);
+          hp_push(&q, deg[k - start], k);
+        }
+      }
+      while (q.n > 0) perm[nordered++] = hp_pop(&q);
+    }
+  }
+  hp_free(&q);
+  hp_free(&rem);
+  free(deg);
+  return ORC_SUCCESS;
+}
+
+/* src/permute.c:288-332: perm[p] = the row placed at position p */
+int orc_cuthillmckee(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const dim_t *blocks, dim_t nblocks, dim_t *perm)
+{
+  if (nrows != ncols) return ORC_ERROR_INVALIDINPUT;
+  if (blocks) {
+    for (dim_t c = 0; c < nblocks; ++c) {
+      int rv = cm_block(blocks[c], blocks[c + 1], rowptr, rowind, perm);
+      if (rv != ORC_SUCCESS) return rv;
+    }
+    return ORC_SUCCESS;
+  }
+  return cm_block(0, nrows, rowptr, rowind, perm);
+}

@@ -152,6 +152,11 @@ int orc_pagerank(orc_dim_t n, const orc_ind_t *rowptr, const orc_dim_t *rowind,
     orc_real_t damping, size_t niter, orc_real_t minerror, orc_real_t *out,
     size_t *iters, orc_real_t *rmse);
 
+/* src/permute.c:49-110,288-332 (Cuthill-McKee; equal degrees in ascending
+ * vertex order, see oracle.c) */
+int orc_cuthillmckee(orc_dim_t nrows, orc_dim_t ncols, const orc_ind_t *rowptr,
+    const orc_dim_t *rowind, const orc_dim_t *blocks, orc_dim_t nblocks,
+    orc_dim_t *perm);
 void orc_free(void *p);
 int orc_num_threads(void);
 void orc_set_num_threads(int t);

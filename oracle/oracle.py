@@ -64,6 +64,8 @@ def lib():
         L.orc_pagerank.argtypes = [u32, _P, _P, _P, _P, u32, f64, sz, f64, _P,
                                    C.POINTER(sz), C.POINTER(f64)]
         L.orc_pagerank.restype = i32
+        L.orc_cuthillmckee.argtypes = [u32, u32, _P, _P, _P, u32, _P]
+        L.orc_cuthillmckee.restype = i32
         L.orc_free.argtypes = [_P]
         L.orc_set_num_threads.argtypes = [i32]
         L.orc_num_threads.restype = i32
@@ -262,3 +264,16 @@ def spmultsp_sp(a, b, bncols):
     rowptr = _take(cp, n + 1, C.c_uint32, np.uint32)
     nnz = int(rowptr[-1])
     return rowptr, _take(ci, nnz, C.c_uint32, np.uint32), _take(cv, nnz, C.c_double, np.float64)
+
+
+def cuthillmckee(rowptr, rowind, ncols=None, blocks=None):
+    """perm[p] = the row placed at position p (src/permute.c:49-110,288-332)."""
+    rowptr, rowind = _u32(rowptr), _u32(rowind)
+    n = rowptr.size - 1
+    perm = np.full(n, 0xffffffff, dtype=np.uint32)
+    b = None if blocks is None else _u32(blocks)
+    code = lib().orc_cuthillmckee(n, n if ncols is None else ncols, _p(rowptr), _p(rowind),
+                                  _p(b), 0 if b is None else b.size - 1, _p(perm))
+    if code != 1:
+        raise ValueError("orc_cuthillmckee: %d" % code)
+    return perm
