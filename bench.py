@@ -38,6 +38,11 @@ timed on a bounded sample on this host).
            GPU; also the parse alone with the text already in HBM, and the
            binary CSR file; the oracle's reader (the reference's loop with
            the C library's strtod) as the CPU baseline
+  rcm      Cuthill-McKee ordering (the CLI's -b) of the 100^3 Laplacian with
+           its rows and columns scrambled by a random permutation (seed 11):
+           orderings per second, the bandwidth before and after, and the
+           FP64 SpMV on the scrambled and on the reordered matrix; the
+           oracle's sequential ordering as the CPU baseline
 """
 import argparse
 import json
@@ -741,6 +746,67 @@ def run_ingest(args, gb):
                 host_reader_ms=host_s * 1e3, cpu_baseline=cpu)
 
 
+# ------------------------------------------------ Cuthill-McKee
+
+def bandwidth(rp, ri):
+    rows = np.repeat(np.arange(rp.size - 1, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    return int(np.max(np.abs(rows - ri.astype(np.int64)))) if ri.size else 0
+
+
+def run_rcm(args, gb):
+    g = 100
+    rp, ri, rv = gb.laplacian(3, g)
+    n, nnz = rp.size - 1, int(rp[-1])
+    q = np.random.default_rng(11).permutation(n).astype(np.uint32)
+    srp, sri, srv = gb.permute_sparse(rp, ri, rv, n, q, q)
+    M = gb.Matrix(srp, sri, srv)
+    for _ in range(args.warmup):
+        M.cuthillmckee()
+    gb.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        perm = M.cuthillmckee()
+    gb.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    prp, pri, prv = gb.permute_sparse(srp, sri, srv, n, perm, perm)
+    M2 = gb.Matrix(prp, pri, prv)
+    dx, dy = gb.DeviceArray.from_host(gb.uniform(1, n)), gb.DeviceArray(n)
+    before = spmv_roofline(gb, M, dx, dy)
+    after = spmv_roofline(gb, M2, dx, dy)
+    ab = 4 * (n + 1) + 4 * nnz + 4 * n          # structure read once, perm written
+    achieved = ab / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        k, t = 0, time.perf_counter()
+        while True:
+            ref = orc.cuthillmckee(srp, sri)
+            k += 1
+            if time.perf_counter() - t > args.cpu_seconds or k >= 3:
+                break
+        t = (time.perf_counter() - t) / k
+        assert np.array_equal(ref, perm)
+        cpu = {"value": 1.0 / t, "unit": "orderings/s", "cores": 1, "kind": "port",
+               "sample": "%d orderings of the same scrambled matrix by the oracle (C "
+                         "restatement of src/permute.c:49-110 with a binary heap) on one "
+                         "core; its permutation equals the GPU's" % k}
+    return line(args.steps / elapsed, "orderings/s", args, ms, "u32",
+                {"workload": "Cuthill-McKee of the 3D 7-point Laplacian %d^3 (n=%d, nnz=%d) "
+                             "with rows and columns scrambled (seed 11)" % (g, n, nnz),
+                 "n": n, "nnz": nnz},
+                roofline={"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
+                          "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                          "kernel": "whole ordering (degree, sort, claim steps, radix sorts, "
+                                    "one-workgroup stepping)",
+                          "algorithmic_bytes_per_launch": ab, "avg_launch_ms": ms},
+                bandwidth_before=bandwidth(srp, sri), bandwidth_after=bandwidth(prp, pri),
+                spmv_scrambled_ms=before["avg_launch_ms"], spmv_reordered_ms=after["avg_launch_ms"],
+                spmv_scrambled_gbps=before["achieved"], spmv_reordered_gbps=after["achieved"],
+                cpu_baseline=cpu)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -753,7 +819,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps (default 2; zipf 1; batched 1)")
     ap.add_argument("--workload", default="laplacian",
-                    choices=["laplacian", "zipf", "batched", "served", "spgemm", "ingest"])
+                    choices=["laplacian", "zipf", "batched", "served", "spgemm", "ingest",
+                             "rcm"])
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
@@ -777,7 +844,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0),
-                "spgemm": (5, 1), "ingest": (5, 1)}
+                "spgemm": (5, 1), "ingest": (5, 1), "rcm": (3, 1)}
     st, wu = defaults[args.workload]
     args.steps = st if args.steps is None else args.steps
     args.warmup = wu if args.warmup is None else args.warmup
@@ -785,7 +852,8 @@ def main():
     import gooseberry_amd as gb
 
     run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
-           "served": run_served, "spgemm": run_spgemm, "ingest": run_ingest}[args.workload]
+           "served": run_served, "spgemm": run_spgemm, "ingest": run_ingest,
+           "rcm": run_rcm}[args.workload]
     out = run(args, gb)
     print(json.dumps(out), flush=True)
 

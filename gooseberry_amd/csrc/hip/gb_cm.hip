@@ -163,6 +163,51 @@ gb_cm_placelist(uint32_t L, uint32_t p0, uint32_t start, const uint32_t *__restr
   }
 }
 
+// A step's new vertices (at most GB_CM_LDS_SORT) sorted in one workgroup:
+// keys (claiming position - b0, srank) are distinct, so a bitonic sort in
+// LDS gives the exact order; the vertex is S[srank]
+#define GB_CM_LDS_SORT 8192
+#define GB_CM_SORT_THREADS 1024
+__global__ void __launch_bounds__(GB_CM_SORT_THREADS)
+gb_cm_sortplace(uint32_t L, uint32_t b0, uint32_t p0, uint32_t start,
+                const uint32_t *__restrict__ list, const uint32_t *__restrict__ claim,
+                const uint32_t *__restrict__ srank, const uint32_t *__restrict__ S,
+                uint32_t *__restrict__ posof, uint32_t *__restrict__ perm) {
+  __shared__ uint64_t key[GB_CM_LDS_SORT];
+  uint32_t P2 = 1;
+  while (P2 < L) P2 <<= 1;
+  for (uint32_t t = threadIdx.x; t < P2; t += GB_CM_SORT_THREADS) {
+    uint64_t k = ~0ull;
+    if (t < L) {
+      const uint32_t v = list[t];
+      k = ((uint64_t)(claim[v] - b0) << 32) | srank[v];
+    }
+    key[t] = k;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= P2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P2; i += GB_CM_SORT_THREADS) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const uint64_t a = key[i], b = key[l];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            key[i] = b;
+            key[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t t = threadIdx.x; t < L; t += GB_CM_SORT_THREADS) {
+    const uint32_t v = S[(uint32_t)key[t]];
+    posof[v] = p0 + t;
+    perm[p0 + t] = start + v;
+  }
+}
+
 // One workgroup processes positions one by one from st[0] (sr) while
 // st[1] (nordered) - sr < GB_CM_BATCH.  Stops with st[2] = 0 when the
 // component is done, 1 when the range has grown, 2 before a vertex whose
@@ -363,7 +408,11 @@ static int gb_cm_block(uint32_t start, uint32_t end, const uint32_t *rowptr,
       uint32_t L = 0;
       GB_CM_TRY(hipMemcpyAsync(&L, dv + 1, 4, hipMemcpyDeviceToHost, s));
       GB_CM_TRY(hipStreamSynchronize(s));
-      if (L) {
+      if (L && L <= GB_CM_LDS_SORT) {
+        gb_cm_sortplace<<<1, GB_CM_SORT_THREADS, 0, s>>>(L, sr, no, start, list, claim, srank,
+                                                        S, posof, perm);
+        GB_CM_TRY(hipGetLastError());
+      } else if (L) {
         // (position, degree, vertex) order: by srank, then stably by position
         gb_cm_keys<0><<<gb_cm_grid(L), GB_BLOCK, 0, s>>>(L, list, srank, 0, keys);
         GB_CM_TRY(hipGetLastError());
