@@ -126,8 +126,11 @@ gb_plan_blocks(const uint32_t *__restrict__ cls,
 // With DOT the kernel also forms the block's share of p.Ap (x = p, y = Ap)
 // and writes it to partials[blockIdx.x]; the next CG kernel adds the
 // partials in a fixed order (no fences or atomics: deterministic).
+#ifndef GB_SPMV_WAVES
+#define GB_SPMV_WAVES 5  // wavefronts per SIMD the register budget must allow
+#endif
 template <typename T, bool DOT>
-__global__ void __launch_bounds__(GB_BLOCK)
+__global__ void __launch_bounds__(GB_BLOCK) __attribute__((amdgpu_waves_per_eu(GB_SPMV_WAVES)))
 gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ rowind,
                const T *__restrict__ val, const T *__restrict__ x,
