@@ -9,12 +9,12 @@
 // scan order.  Rows take one of two paths:
 //
 // Short rows (at most 64 products from at most 64 entries of A): one
-// wavefront per row, lane t holding product t.  Every lane compares its
-// column with the other lanes' (read one after another, in t order): a lane
-// with no earlier equal column is the head of its column, and its sum is its
-// own product followed by the later equal ones, added in t order.  The heads
-// in lane order are the columns in first-appearance order (a ballot gives
-// each head its rank).
+// wavefront per row, lane t holding product t.  The lanes enter their
+// columns in a 128-slot LDS hash table, each slot keeping the mask of lanes
+// with its column: a lane with no lower lane in its mask is the head of its
+// column, and its sum is its own product followed by the other lanes' in
+// the mask, added in lane (= t) order.  The heads in lane order are the
+// columns in first-appearance order (a ballot gives each head its rank).
 //
 // Other rows: every product is materialised at its position t (numbered
 // over these rows, row-major), then
@@ -82,11 +82,15 @@ gb_sg_compact(uint32_t n, const uint32_t *__restrict__ flag, const uint32_t *__r
     if (flag[i]) list[pos[i]] = i;
 }
 
-__device__ __forceinline__ double gb_sg_readlane(double v, uint32_t lane) {
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)lane);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)lane);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+#define GB_SG_HASH_BITS 7
+#define GB_SG_HASH (1u << GB_SG_HASH_BITS)  // slots per wavefront row (>= 2 x 64)
+#define GB_SG_EMPTY 0xffffffffu            // no column reaches 2^32 - 1
+
+// LDS operations of one wavefront complete in order; this keeps the
+// compiler from moving them across the step and waits for them
+__device__ __forceinline__ void gb_sg_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // One wavefront per short row.  COUNT: cnt[i] = distinct columns; FILL:
@@ -100,6 +104,9 @@ gb_sg_wave(uint32_t nlist, const uint32_t *__restrict__ list,
            const uint32_t *__restrict__ crowptr, uint32_t *__restrict__ cnt,
            uint32_t *__restrict__ crowind, double *__restrict__ crowval,
            double *__restrict__ cdense, uint32_t bncols) {
+  __shared__ uint32_t hkey[GB_WAVES_PER_BLOCK][GB_SG_HASH];
+  __shared__ unsigned long long hmask[GB_WAVES_PER_BLOCK][GB_SG_HASH];
+  __shared__ double hval[GB_WAVES_PER_BLOCK][GB_WAVE];
   const uint32_t w = blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE;
   const uint32_t lane = threadIdx.x & (GB_WAVE - 1);
   if (w >= nlist) return;  // the whole wavefront
@@ -135,17 +142,42 @@ gb_sg_wave(uint32_t nlist, const uint32_t *__restrict__ list,
     col = browind[k];
     val = x_a * browval[k];
   }
-  bool head = live;
-  double sum = MODE == GB_SG_DENSE ? 0.0 + val : val;
-  for (uint32_t u = 0; u < P; ++u) {
-    const uint32_t cu = (uint32_t)__builtin_amdgcn_readlane((int)col, (int)u);
-    if (MODE == GB_SG_COUNT) {
-      if (cu == col && u < lane) head = false;
-    } else {
-      const double vu = gb_sg_readlane(val, u);
-      if (cu == col) {
-        if (u < lane) head = false;
-        else if (u > lane) sum += vu;
+  // the row's columns in a small LDS hash table: slot -> column and the
+  // mask of lanes holding it; a lane is its column's head when no lower
+  // lane is in the mask, and the head adds the other lanes' products in
+  // lane (= scan) order
+  uint32_t *key = hkey[threadIdx.x / GB_WAVE];
+  unsigned long long *msk = hmask[threadIdx.x / GB_WAVE];
+  double *hv = hval[threadIdx.x / GB_WAVE];
+  key[lane] = GB_SG_EMPTY;
+  key[lane + GB_WAVE] = GB_SG_EMPTY;
+  msk[lane] = 0ull;
+  msk[lane + GB_WAVE] = 0ull;
+  hv[lane] = val;
+  gb_sg_wave_sync();
+  uint32_t slot = 0;
+  if (live) {
+    uint32_t h = (col * 2654435761u) >> (32 - GB_SG_HASH_BITS);
+    for (;;) {  // at most 64 columns in 128 slots: a free or equal slot exists
+      const uint32_t prev = atomicCAS(&key[h], GB_SG_EMPTY, col);
+      if (prev == GB_SG_EMPTY || prev == col) break;
+      h = (h + 1) & (GB_SG_HASH - 1);
+    }
+    slot = h;
+    atomicOr(&msk[slot], 1ull << lane);
+  }
+  gb_sg_wave_sync();
+  bool head = false;
+  double sum = 0.0;
+  if (live) {
+    const unsigned long long mk = msk[slot];
+    head = (mk & ((1ull << lane) - 1ull)) == 0;
+    if (head && MODE != GB_SG_COUNT) {
+      sum = MODE == GB_SG_DENSE ? 0.0 + val : val;
+      unsigned long long rest = mk & ~((2ull << lane) - 1ull);  // lanes above this one
+      while (rest) {
+        sum += hv[__ffsll(rest) - 1];
+        rest &= rest - 1ull;
       }
     }
   }
