@@ -264,6 +264,12 @@ int gbd_load_file(const char *file, char **d_txt, uint64_t *r_len);
 /* bytes [offset, offset + bytes) of a file into device memory at d_dst */
 int gbd_load_file_range(const char *file, uint64_t offset, uint64_t bytes,
     void *d_dst);
+/* The body of a MatrixMarket file (the lines after the size line, at
+ * d_txt[off, len)) parsed on the device with the semantics of
+ * src/io.c:1060-1145; nrows, ncols and sym come from the header. */
+int gbd_parse_mtx_body(const char *d_txt, uint64_t len, uint64_t off,
+    uint32_t nrows, uint32_t ncols, int sym, uint32_t *r_nnz,
+    uint32_t **d_rowptr, uint32_t **d_rowind, double **d_rowval);
 /* .csr text at d_txt (len bytes) parsed on the device with the semantics
  * of src/io.c:369-498; new device arrays (free with gbd_free). */
 int gbd_parse_csr(const char *d_txt, uint64_t len, uint32_t *r_nrows,

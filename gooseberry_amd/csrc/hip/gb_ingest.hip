@@ -231,6 +231,106 @@ gb_in_sum64(const uint32_t *__restrict__ a, uint64_t n, unsigned long long *__re
   if (threadIdx.x == 0) atomicAdd(total, v);
 }
 
+// ---- MatrixMarket body (src/io.c:1060-1145): one coordinate per line
+
+// flag[k] = 1 for an entry line (not blank after '\r' stripping, not a
+// comment)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_mx_flag(const char *__restrict__ txt, uint64_t len, const uint64_t *__restrict__ ls,
+           uint32_t nl, uint32_t nlines, uint32_t *__restrict__ flag) {
+  for (uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x; k < nlines;
+       k += gridDim.x * GB_BLOCK) {
+    uint64_t a, b;
+    gb_in_line(txt, len, ls, nl, k, &a, &b);
+    flag[k] = (b == a || gb_in_is_comment((unsigned char)txt[a])) ? 0u : 1u;
+  }
+}
+
+// entry e (= rank of its line): row i, column j (1-based in the file),
+// weight w when one converts (a weight the first 19 digits do not settle is
+// listed for gb_in_fixup).  bad: 1 when a coordinate does not convert or is
+// out of range; nwt counts the entries with a weight
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_mx_parse(const char *__restrict__ txt, uint64_t len, const uint64_t *__restrict__ ls,
+            uint32_t nl, uint32_t nlines, const uint32_t *__restrict__ ent,
+            uint32_t nrows, uint32_t ncols, int sym, uint32_t *__restrict__ ei,
+            uint32_t *__restrict__ ej, double *__restrict__ ew, uint32_t *__restrict__ bad,
+            uint32_t *__restrict__ nwt, gb_in_fix *__restrict__ fix, uint32_t fixcap,
+            uint32_t *__restrict__ nfix) {
+  __shared__ uint32_t swt;
+  if (threadIdx.x == 0) swt = 0;
+  __syncthreads();
+  uint32_t b_ = 0, wt = 0;
+  for (uint32_t k = blockIdx.x * GB_BLOCK + threadIdx.x; k < nlines;
+       k += gridDim.x * GB_BLOCK) {
+    const uint32_t e = ent[k];
+    if (ent[k + 1] == e) continue;
+    uint64_t a, b;
+    gb_in_line(txt, len, ls, nl, k, &a, &b);
+    uint64_t v = 0;
+    uint64_t q = gbt_strtoull(txt, b, a, 10, &v);
+    if (q == a) { b_ = 1; continue; }
+    const uint32_t i = (uint32_t)v - 1u;
+    uint64_t q2 = gbt_strtoull(txt, b, q, 10, &v);
+    if (q2 == q) { b_ = 1; continue; }
+    const uint32_t j = (uint32_t)v - 1u;
+    double w = 1.0;
+    bool hard = false;
+    const uint64_t q3 = gbt_strtod(txt, b, q2, &w, &hard, true);
+    if (q3 != q2) {
+      ++wt;
+      if (hard) {  // settled by gb_in_fixup
+        const uint32_t f = atomicAdd(nfix, 1u);
+        if (f < fixcap) {
+          fix[f].p = q2;
+          fix[f].end = b;
+          fix[f].out = e;
+        }
+      }
+    } else {
+      w = 1.0;
+    }
+    if (i >= nrows || j >= ncols || (sym && j >= nrows)) b_ = 1;
+    ei[e] = i;
+    ej[e] = j;
+    ew[e] = w;
+  }
+  if (b_) atomicOr(bad, 1u);
+  if (wt) atomicAdd(&swt, wt);
+  __syncthreads();
+  if (threadIdx.x == 0 && swt) atomicAdd(nwt, swt);
+}
+
+// record keys in file order, a symmetric file's mirrored entry first:
+// record 2e = (j, i), 2e + 1 = (i, j); otherwise record e = (i, j)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_mx_keys(uint32_t nent, int sym, const uint32_t *__restrict__ ei,
+           const uint32_t *__restrict__ ej, uint32_t *__restrict__ key) {
+  for (uint32_t e = blockIdx.x * GB_BLOCK + threadIdx.x; e < nent; e += gridDim.x * GB_BLOCK) {
+    if (sym) {
+      key[2 * (uint64_t)e] = ej[e];
+      key[2 * (uint64_t)e + 1] = ei[e];
+    } else {
+      key[e] = ei[e];
+    }
+  }
+}
+
+// CSR entries from the records in row order (ord = stable sort by row)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_mx_gather(uint32_t nnz, int sym, int wgts, const uint32_t *__restrict__ ord,
+             const uint32_t *__restrict__ ei, const uint32_t *__restrict__ ej,
+             const double *__restrict__ ew, uint32_t *__restrict__ rowind,
+             double *__restrict__ rowval) {
+  for (uint32_t p = blockIdx.x * GB_BLOCK + threadIdx.x; p < nnz; p += gridDim.x * GB_BLOCK) {
+    const uint32_t r = ord[p];
+    const uint32_t e = sym ? r >> 1 : r;
+    const bool mirror = sym && !(r & 1u);
+    rowind[p] = mirror ? ei[e] : ej[e];
+    rowval[p] = wgts ? ew[e] : 1.0;
+  }
+}
+
 static uint32_t gb_in_grid(uint64_t n) {
   uint64_t g = (n + GB_BLOCK - 1) / GB_BLOCK;
   if (g > 16384) g = 16384;
@@ -487,6 +587,146 @@ done:
   return rv;
 #undef GB_IN_TRY
 #undef GB_IN_RV
+}
+
+// The body of a MatrixMarket file (text at d_txt[off, len)): the lines
+// after the size line.  nrows, ncols and sym come from the header, parsed
+// on the host.  Rows are filled by a stable sort of the entries' records by
+// row, a symmetric file's mirrored entry first (src/io.c:1103-1144).
+int gbd_parse_mtx_body(const char *d_txt, uint64_t len, uint64_t off, uint32_t nrows,
+                       uint32_t ncols, int sym, uint32_t *r_nnz, uint32_t **d_rowptr,
+                       uint32_t **d_rowind, double **d_rowval) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (off > len) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  hipStream_t s = gb_stream();
+  hipError_t he = hipSuccess;
+  const char *txt = d_txt + off;
+  const uint64_t blen = len - off;
+  const uint64_t nchunks = (blen + GB_IN_CHUNK - 1) / GB_IN_CHUNK;
+  if (nchunks >= 0xffffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t *ccnt = nullptr, *ent = nullptr, *ei = nullptr, *ej = nullptr, *flags = nullptr,
+           *key = nullptr, *ord = nullptr, *rowptr = nullptr, *rowind = nullptr,
+           *nfix = nullptr;
+  uint64_t *ls = nullptr;
+  double *ew = nullptr, *rowval = nullptr;
+  gb_in_fix *fix = nullptr;
+  uint32_t nl = 0, nlines = 0, nent = 0, hflags[3] = {0, 0, 0}, hnfix = 0, fixcap = GB_IN_FIX0;
+  uint64_t last = 0, nnz64 = 0;
+  uint32_t nnz = 0;
+#define GB_MX_TRY(call)                                         \
+  do {                                                          \
+    he = (call);                                                \
+    if (he != hipSuccess) {                                     \
+      rv = gb_hip_fail(he, #call, __FILE__, __LINE__);          \
+      goto done;                                                \
+    }                                                           \
+  } while (0)
+#define GB_MX_RV(call)                                          \
+  do {                                                          \
+    rv = (call);                                                \
+    if (rv != GOOSEBERRY_SUCCESS) goto done;                    \
+  } while (0)
+  // lines of the body
+  GB_MX_TRY(gb_dev_alloc((void **)&ccnt, (nchunks + 1) * 4));
+  if (nchunks) {
+    gb_in_nl_count<<<(uint32_t)nchunks, GB_BLOCK, 0, s>>>(txt, blen, ccnt);
+    GB_MX_TRY(hipGetLastError());
+  }
+  GB_MX_RV(gbd_scan_u32(ccnt, ccnt, nchunks, ccnt + nchunks));
+  GB_MX_TRY(hipMemcpyAsync(&nl, ccnt + nchunks, 4, hipMemcpyDeviceToHost, s));
+  GB_MX_TRY(hipStreamSynchronize(s));
+  if (nl == 0xffffffffu) {
+    rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto done;
+  }
+  GB_MX_TRY(gb_dev_alloc((void **)&ls, ((uint64_t)nl + 2) * 8));
+  if (nchunks) {
+    gb_in_nl_pos<<<(uint32_t)nchunks, GB_BLOCK, 0, s>>>(txt, blen, ccnt, ls);
+    GB_MX_TRY(hipGetLastError());
+    GB_MX_TRY(hipMemcpyAsync(&last, ls + nl, 8, hipMemcpyDeviceToHost, s));
+    GB_MX_TRY(hipStreamSynchronize(s));
+  }
+  nlines = nl + (blen > last ? 1u : 0u);
+  // entry lines and their ranks
+  GB_MX_TRY(gb_dev_alloc((void **)&ent, ((uint64_t)nlines + 1) * 4));
+  if (nlines) {
+    gb_mx_flag<<<gb_in_grid(nlines), GB_BLOCK, 0, s>>>(txt, blen, ls, nl, nlines, ent);
+    GB_MX_TRY(hipGetLastError());
+  }
+  GB_MX_RV(gbd_scan_u32(ent, ent, nlines, ent + nlines));
+  GB_MX_TRY(hipMemcpyAsync(&nent, ent + nlines, 4, hipMemcpyDeviceToHost, s));
+  GB_MX_TRY(hipStreamSynchronize(s));
+  nnz64 = (uint64_t)nent * (sym ? 2u : 1u);
+  if (nnz64 > 0xffffffffull) {
+    rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto done;
+  }
+  nnz = (uint32_t)nnz64;
+  GB_MX_TRY(gb_dev_alloc((void **)&ei, ((uint64_t)nent + 1) * 4));
+  GB_MX_TRY(gb_dev_alloc((void **)&ej, ((uint64_t)nent + 1) * 4));
+  GB_MX_TRY(gb_dev_alloc((void **)&ew, ((uint64_t)nent + 1) * 8));
+  GB_MX_TRY(gb_dev_alloc((void **)&flags, 8));
+  GB_MX_TRY(gb_dev_alloc((void **)&nfix, 4));
+  for (int pass = 0; pass < 2; ++pass) {
+    GB_MX_TRY(gb_dev_alloc((void **)&fix, (uint64_t)fixcap * sizeof(gb_in_fix)));
+    GB_MX_RV(gbd_zero(flags, 8));
+    GB_MX_RV(gbd_zero(nfix, 4));
+    if (nlines) {
+      gb_mx_parse<<<gb_in_grid(nlines), GB_BLOCK, 0, s>>>(txt, blen, ls, nl, nlines, ent, nrows,
+                                                         ncols, sym, ei, ej, ew, flags,
+                                                         flags + 1, fix, fixcap, nfix);
+      GB_MX_TRY(hipGetLastError());
+    }
+    GB_MX_TRY(hipMemcpyAsync(hflags, flags, 8, hipMemcpyDeviceToHost, s));
+    GB_MX_TRY(hipMemcpyAsync(&hnfix, nfix, 4, hipMemcpyDeviceToHost, s));
+    GB_MX_TRY(hipStreamSynchronize(s));
+    if (hnfix <= fixcap) break;
+    gb_dev_free(fix);
+    fix = nullptr;
+    fixcap = hnfix;
+  }
+  // a coordinate missing or out of range; weights on some lines only
+  if (hflags[0] || (hflags[1] != 0 && hflags[1] != nent) || hnfix > fixcap) {
+    rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+    goto done;
+  }
+  if (hnfix) {
+    gb_in_fixup<<<gb_div_up(hnfix, 64), 64, 0, s>>>(txt, fix, hnfix, ew);
+    GB_MX_TRY(hipGetLastError());
+  }
+  // rows: a stable sort of the records by row gives the fill order and,
+  // through its run starts, the row pointers
+  GB_MX_TRY(gb_dev_alloc((void **)&rowptr, ((uint64_t)nrows + 2) * 4));
+  GB_MX_TRY(gb_dev_alloc((void **)&rowind, ((uint64_t)nnz + 1) * 4));
+  GB_MX_TRY(gb_dev_alloc((void **)&rowval, ((uint64_t)nnz + 1) * 8));
+  if (nnz) {
+    GB_MX_TRY(gb_dev_alloc((void **)&key, (uint64_t)nnz * 4));
+    GB_MX_TRY(gb_dev_alloc((void **)&ord, (uint64_t)nnz * 4));
+    gb_mx_keys<<<gb_in_grid(nent), GB_BLOCK, 0, s>>>(nent, sym, ei, ej, key);
+    GB_MX_TRY(hipGetLastError());
+    GB_MX_RV(gbd_countingsort_kv_unchecked(key, nullptr, 0, nrows - 1, nnz, ord, rowptr));
+    gb_mx_gather<<<gb_in_grid(nnz), GB_BLOCK, 0, s>>>(nnz, sym, hflags[1] != 0, ord, ei, ej, ew,
+                                                     rowind, rowval);
+    GB_MX_TRY(hipGetLastError());
+  } else {
+    GB_MX_RV(gbd_zero(rowptr, ((uint64_t)nrows + 1) * 4));
+  }
+  GB_MX_TRY(hipStreamSynchronize(s));
+  *r_nnz = nnz;
+  *d_rowptr = rowptr;
+  *d_rowind = rowind;
+  *d_rowval = rowval;
+  rowptr = rowind = nullptr;
+  rowval = nullptr;
+  rv = GOOSEBERRY_SUCCESS;
+done:
+  gb_dev_free(ccnt); gb_dev_free(ls); gb_dev_free(ent); gb_dev_free(ei); gb_dev_free(ej);
+  gb_dev_free(ew); gb_dev_free(flags); gb_dev_free(nfix); gb_dev_free(fix); gb_dev_free(key);
+  gb_dev_free(ord); gb_dev_free(rowptr); gb_dev_free(rowind); gb_dev_free(rowval);
+  return rv;
+#undef GB_MX_TRY
+#undef GB_MX_RV
 }
 
 }  // extern "C"

@@ -186,12 +186,14 @@ static int read_input(const char *file, mat_t *m)
       &m->rowind, &m->rowval);
 }
 
-/* CSR text and binary CSR go straight to a resident matrix: parsed on the
- * GPU with the host reader's semantics (the same arrays, bit for bit) */
+/* CSR text, MatrixMarket and binary CSR go straight to a resident matrix:
+ * parsed on the GPU with the host reader's semantics (the same arrays, bit
+ * for bit) */
 static int device_readable(const char *file)
 {
   int t = file_type(file);
-  return t == GOOSEBERRY_FORMAT_CSR || t == GOOSEBERRY_FORMAT_BCSR;
+  return t == GOOSEBERRY_FORMAT_CSR || t == GOOSEBERRY_FORMAT_BCSR ||
+      t == GOOSEBERRY_FORMAT_MATRIXMARKET;
 }
 
 static int read_resident(const char *file, gooseberry_matrix_t *dmat, mat_t *m)

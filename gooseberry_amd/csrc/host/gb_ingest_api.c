@@ -1,15 +1,24 @@
 /*
  * gb_ingest_api.c -- sparse matrix files straight into device memory.
  * CSR text is parsed on the GPU (gb_ingest.hip) with the semantics of the
- * reference's reader (src/io.c:369-498); the binary CSR format is copied
- * through pinned staging buffers and checked on the GPU.
+ * reference's reader (src/io.c:369-498), and so is a MatrixMarket file's
+ * body (src/io.c:1060-1145; its header lines are read here); the binary CSR
+ * format is copied through pinned staging buffers and checked on the GPU.
  */
+#define _GNU_SOURCE
 #include <errno.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "gooseberry.h"
 #include "gb_host.h"
+
+static ssize_t strip_eol(char *line, size_t n)
+{
+  while (n > 0 && (line[n - 1] == '\n' || line[n - 1] == '\r')) line[--n] = '\0';
+  return (ssize_t)n;
+}
 
 static int read_device_bcsr(const char *file, dim_t *nrows, dim_t *ncols,
     ind_t **d_rowptr, dim_t **d_rowind, real_t **d_rowval, ind_t *r_nnz)
@@ -52,6 +61,78 @@ static int read_device_bcsr(const char *file, dim_t *nrows, dim_t *ncols,
   return GOOSEBERRY_SUCCESS;
 }
 
+/* src/io.c:921-1059 (as the host reader restates it): the first line is
+ * either "nrows ncols nnz" (no banner: real general) or the banner
+ * "%%MatrixMarket matrix coordinate real general|symmetric"; after a banner
+ * blank and comment lines are skipped and the size line follows.  Leaves the
+ * byte offset of the body in *off. */
+static int mtx_header(const char *file, dim_t *nrows, dim_t *ncols, int *sym,
+    uint64_t *off)
+{
+  FILE *f = fopen(file, "rb");
+  if (!f) {
+    switch (errno) {
+      case ENOENT: case ENOTDIR: return GOOSEBERRY_ERROR_FILENOTFOUND;
+      case EACCES: case EPERM: return GOOSEBERRY_ERROR_PERMISSIONDENIED;
+      default: return GOOSEBERRY_ERROR_UNKNOWN;
+    }
+  }
+  char *line = NULL;
+  size_t cap = 0;
+  ssize_t got;
+  uint64_t pos = 0;
+  int rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* a line: up to '\n', trailing '\r' and '\n' removed */
+#define MTX_LINE()                                                        \
+  ((got = getline(&line, &cap, f)) < 0 ? -1 : (pos += (uint64_t)got,      \
+    strip_eol(line, (size_t)got)))
+  ssize_t ll = MTX_LINE();
+  if (ll < 1) goto done;
+  {
+    unsigned a, b, c;
+    if (sscanf(line, "%u %u %u", &a, &b, &c) == 3) {
+      *nrows = a;
+      *ncols = b;
+      *sym = 0;
+    } else {
+      static const char ws[] = " \t";
+      char *save = NULL;
+      char *tok = strtok_r(line, ws, &save);
+      if (!tok || strcmp(tok, "%%MatrixMarket") != 0) goto done;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok || strcmp(tok, "matrix") != 0) goto done;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok || strcmp(tok, "coordinate") != 0) goto done;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok || strcmp(tok, "real") != 0) goto done;
+      tok = strtok_r(NULL, ws, &save);
+      if (!tok) goto done;
+      if (strcmp(tok, "general") == 0) *sym = 0;
+      else if (strcmp(tok, "symmetric") == 0) *sym = 1;
+      else goto done;
+      while ((ll = MTX_LINE()) >= 0) {
+        if (ll == 0 || (line[0] != '\0' && strchr("#%'\"/", line[0]) != NULL)) continue;
+        break;
+      }
+      if (ll < 0) goto done;
+      char *sptr = line, *eptr;
+      *nrows = (dim_t)strtoull(sptr, &eptr, 10);
+      sptr = eptr;
+      *ncols = (dim_t)strtoull(sptr, &eptr, 10);
+      sptr = eptr;
+      (void)strtoull(sptr, &eptr, 10);
+      if (sptr == eptr) goto done;
+    }
+  }
+#undef MTX_LINE
+  *off = pos;
+  rv = GOOSEBERRY_SUCCESS;
+done:
+  free(line);
+  fclose(f);
+  return rv;
+}
+
 static int read_device(int type, const char *file, dim_t *nrows, dim_t *ncols,
     ind_t **d_rowptr, dim_t **d_rowind, real_t **d_rowval, ind_t *r_nnz)
 {
@@ -60,17 +141,34 @@ static int read_device(int type, const char *file, dim_t *nrows, dim_t *ncols,
   if (type == GOOSEBERRY_FORMAT_BCSR)
     return read_device_bcsr(file, nrows, ncols, d_rowptr, d_rowind, d_rowval,
         r_nnz);
-  if (type != GOOSEBERRY_FORMAT_CSR) {
+  if (type != GOOSEBERRY_FORMAT_CSR && type != GOOSEBERRY_FORMAT_MATRIXMARKET) {
     fprintf(stderr, "Device reading of sparse format %d is not provided by "
         "this build\n", type);
     return GOOSEBERRY_ERROR_UNIMPLEMENTED;
   }
+  dim_t mr = 0, mc = 0;
+  int sym = 0;
+  uint64_t off = 0;
+  int rv;
+  if (type == GOOSEBERRY_FORMAT_MATRIXMARKET) {
+    rv = mtx_header(file, &mr, &mc, &sym, &off);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
   char *d_txt = NULL;
   uint64_t len = 0;
-  int rv = gbd_load_file(file, &d_txt, &len);
+  rv = gbd_load_file(file, &d_txt, &len);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  rv = gbd_parse_csr(d_txt, len, nrows, ncols, r_nnz, d_rowptr, d_rowind,
-      d_rowval);
+  if (type == GOOSEBERRY_FORMAT_CSR) {
+    rv = gbd_parse_csr(d_txt, len, nrows, ncols, r_nnz, d_rowptr, d_rowind,
+        d_rowval);
+  } else {
+    rv = gbd_parse_mtx_body(d_txt, len, off, mr, mc, sym, r_nnz, d_rowptr,
+        d_rowind, d_rowval);
+    if (rv == GOOSEBERRY_SUCCESS) {
+      *nrows = mr;
+      *ncols = mc;
+    }
+  }
   gbd_free(d_txt);
   return rv;
 }

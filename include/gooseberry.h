@@ -223,12 +223,13 @@ int gooseberry_write_sparse_matrix(
     const dim_t * rowind,
     const real_t * rowval);
 
-/* New: a sparse matrix file straight into device memory.  CSR text is
- * parsed on the GPU with the reader semantics of src/io.c:369-498 -- the
- * arrays are those of gooseberry_read_sparse_matrix, bit for bit -- and
- * BCSR is copied through pinned buffers and checked on the GPU (other
- * formats: GOOSEBERRY_ERROR_UNIMPLEMENTED).  The arrays are device memory
- * (free with gooseberry_device_free). */
+/* New: a sparse matrix file straight into device memory.  CSR text and
+ * MatrixMarket bodies are parsed on the GPU with the reader semantics of
+ * src/io.c:369-498 and :921-1190 -- the arrays are those of
+ * gooseberry_read_sparse_matrix, bit for bit -- and BCSR is copied through
+ * pinned buffers and checked on the GPU (other formats:
+ * GOOSEBERRY_ERROR_UNIMPLEMENTED).  The arrays are device memory (free with
+ * gooseberry_device_free). */
 int gooseberry_read_sparse_matrix_device(
     int type,
     const char * file,

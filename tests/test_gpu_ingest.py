@@ -160,3 +160,66 @@ def test_binary_csr_and_resident_read(gb, oracle, tmp_path):
     with pytest.raises(gb.GooseberryError) as e:
         gb.read_sparse_device(os.path.join(str(tmp_path), "missing.csr"))
     assert e.value.name == "FILENOTFOUND"
+
+
+def check_mtx(gb, oracle, tmp_path, text, name="m.mtx"):
+    path = os.path.join(str(tmp_path), name)
+    with open(path, "wb") as f:
+        f.write(text)
+    ref = oracle.read_sparse(path, mtx=True)
+    if isinstance(ref, int):          # the reader refuses the file
+        with pytest.raises(gb.GooseberryError) as e:
+            gb.read_sparse_device(path, gb.FORMAT_MATRIXMARKET)
+        assert e.value.name == "INVALIDINPUT"
+        return None
+    same(gb.read_sparse_device(path, gb.FORMAT_MATRIXMARKET), ref)
+    return ref
+
+
+def test_matrixmarket_files(gb, oracle, tmp_path):
+    """MatrixMarket bodies parsed on the GPU (src/io.c:921-1190): general
+    and symmetric files (the mirrored entry first, diagonals twice), files
+    without a banner, blank and comment lines, CRLF, long and hard
+    weights; and the files the reader refuses."""
+    import scipy.io
+    import scipy.sparse as sp
+    rng = np.random.default_rng(40)
+    A = sp.random(700, 500, density=0.01, random_state=41, format="coo")
+    A.data = rng.uniform(-1, 1, A.nnz) * 10.0 ** rng.integers(-5, 5, A.nnz)
+    p = os.path.join(str(tmp_path), "gen.mtx")
+    scipy.io.mmwrite(p, A, precision=17)
+    same(gb.read_sparse_device(p, gb.FORMAT_MATRIXMARKET), oracle.read_sparse(p, mtx=True))
+    S = sp.random(600, 600, density=0.01, random_state=42)
+    S = (S + S.T + sp.eye(600)).tocoo()
+    p = os.path.join(str(tmp_path), "sym.mtx")
+    scipy.io.mmwrite(p, S, symmetry="symmetric")
+    ref = oracle.read_sparse(p, mtx=True)
+    same(gb.read_sparse_device(p, gb.FORMAT_MATRIXMARKET), ref)
+    assert ref[2][-1] > S.nnz              # mirrored entries and doubled diagonals
+    B = b"%%MatrixMarket matrix coordinate real general\n"
+    cases = [
+        B + b"% c\n\n3 4 5\n1 1 1.5\n\n% mid\n3 4 -2\n2 2 1e-300\n1 4 0x1p3\n3 1 inf\n",
+        B.replace(b"\n", b"\r\n") + b"3 3 2\r\n1 2 0.5\r\n3 3 7\r\n",
+        b"3 3 3\n1 1\n2 3\n3 2\n",                           # no banner, no weights
+        b"%%MatrixMarket matrix coordinate real symmetric\n4 4 3\n2 1 1\n3 3 2\n4 2 3",
+        B + b"2 2 1\n1 1 " + b"1" * 30 + b"e-20\n2 1 2.4703282292062327208828439643411068e-324\n",
+        B + b"2 2 2\n1 1 1\n2 2\n",                         # weights on some lines only
+        B + b"2 2 2\n1 1\n2 2 1\n",
+        B + b"2 2 1\n3 1 1\n",                              # row out of range
+        B + b"2 2 1\n1 x 1\n",                              # column does not convert
+        b"%%MatrixMarket matrix coordinate real symmetric\n3 2 1\n1 3 1\n",
+        b"%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n",
+        b"%%MatrixMarket matrix coordinate complex general\n1 1 1\n1 1 1 0\n",
+        b"",
+        b"\n3 3 1\n1 1 1\n",
+        B + b"0 0 0\n",
+        B + b"5 5 0\n",
+    ]
+    refused = 0
+    for t, text in enumerate(cases):
+        if check_mtx(gb, oracle, tmp_path, text, "c%d.mtx" % t) is None:
+            refused += 1
+    assert refused >= 6
+    # a resident matrix from a MatrixMarket file
+    M = gb.Matrix.read(os.path.join(str(tmp_path), "sym.mtx"), gb.FORMAT_MATRIXMARKET)
+    assert (M.nrows, M.ncols, M.nnz) == (600, 600, int(ref[2][-1]))
