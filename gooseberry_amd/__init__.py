@@ -70,6 +70,7 @@ EXPORTED = [
     "gooseberry_matrix_spmultsp_dense", "gooseberry_read_sparse_matrix_device",
     "gooseberry_parse_csr_device", "gooseberry_matrix_read",
     "gooseberry_permute_cuthillmckee", "gooseberry_matrix_cuthillmckee",
+    "gooseberry_matrix_spmv_f32", "gooseberry_matrix_cgd_mixed", "gooseberry_spmult_f32",
 ]
 
 PRECISION_FP64 = 0
@@ -150,6 +151,9 @@ def _sig(lib):
         "gooseberry_matrix_read": (i32, [i32, C.c_char_p, _P, C.POINTER(_P)]),
         "gooseberry_permute_cuthillmckee": (i32, [u32, u32, _P, _P, _P, _P, u32, _P]),
         "gooseberry_matrix_cuthillmckee": (i32, [_P, _P, u32, _P]),
+        "gooseberry_matrix_spmv_f32": (i32, [_P, _P, _P]),
+        "gooseberry_matrix_cgd_mixed": (i32, [_P, _P, _P, f64, i32, sz, C.POINTER(CgdInfo)]),
+        "gooseberry_spmult_f32": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, u32]),
         "gooseberry_matrix_spmultsp_dense": (i32, [_P, _P, _P]),
         "gooseberry_spmultsp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, _P, _P, u32]),
         "gooseberry_spmultsp_sp": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P,
@@ -477,6 +481,24 @@ class Matrix:
             check(rv, "gooseberry_matrix_cgd")
         return info
 
+    def spmv_f32(self, x):
+        """gooseberry_matrix_spmv_f32 on a host float32 vector."""
+        dx = DeviceArray.from_host(np.asarray(x, dtype=np.float32), np.float32)
+        dy = DeviceArray(self.nrows, np.float32)
+        check(lib().gooseberry_matrix_spmv_f32(self.h, dx.ptr, dy.ptr),
+              "gooseberry_matrix_spmv_f32")
+        return dy.numpy()
+
+    def cgd_mixed(self, b, tol, tol_mode=TOL_RELATIVE, niter=0):
+        """gooseberry_matrix_cgd_mixed: (x, info)."""
+        db = DeviceArray.from_host(b, np.float64)
+        dx = DeviceArray(self.nrows)
+        info = CgdInfo()
+        check(lib().gooseberry_matrix_cgd_mixed(self.h, db.ptr, dx.ptr, float(tol), tol_mode,
+                                                int(niter), C.byref(info)),
+              "gooseberry_matrix_cgd_mixed")
+        return dx.numpy(), info
+
     def cgd(self, b, tol, tol_mode=TOL_RELATIVE, niter=0):
         db = DeviceArray.from_host(b, np.float64)
         dx = DeviceArray(self.nrows)
@@ -542,6 +564,22 @@ def spmultsp_sp(anrows, ancols, bncols, a, b, blocks=None):
     rowptr = _take(rp, anrows + 1, np.uint32)
     nnz = int(rowptr[-1])
     return rowptr, _take(ri, nnz, np.uint32), _take(rv, nnz, np.float64)
+
+
+def spmult_f32(anrows, ancols, bncols, rowptr, rowind, rowval, bcolval, blocks=None,
+               crowval=None):
+    """gooseberry_spmult_f32 (the reference's float build) on host arrays."""
+    rowptr, rowind = _u32(rowptr), _u32(rowind)
+    rowval = np.ascontiguousarray(rowval, dtype=np.float32)
+    bcolval = np.ascontiguousarray(bcolval, dtype=np.float32)
+    if crowval is None:
+        crowval = np.zeros(anrows * bncols, dtype=np.float32)
+    blocks = _u32(blocks)
+    nblocks = 0 if blocks is None else blocks.size - 1
+    check(lib().gooseberry_spmult_f32(anrows, ancols, bncols, _ptr(rowptr), _ptr(rowind),
+                                      _ptr(rowval), _ptr(bcolval), _ptr(crowval),
+                                      _ptr(blocks), nblocks), "gooseberry_spmult_f32")
+    return crowval
 
 
 def scale(val, s):

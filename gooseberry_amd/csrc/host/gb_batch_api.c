@@ -513,3 +513,78 @@ int gooseberry_server_take_result(gooseberry_server_t S, uint64_t id,
   }
   return GOOSEBERRY_ERROR_INVALIDINPUT;
 }
+
+/* ------------------------------------------------ explicitly typed FP32 */
+
+int gooseberry_matrix_spmv_f32(gooseberry_matrix_t mat, const float *d_x,
+    float *d_y)
+{
+  if (!mat || !d_x || !d_y) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_matrix *m = gb_matrix_dev(mat);
+  int rv = gbd_matrix_make_f32(m);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  return gbd_spmv_f32(m, d_x, d_y);
+}
+
+int gooseberry_matrix_cgd_mixed(gooseberry_matrix_t mat, const real_t *d_b,
+    real_t *d_x, real_t tol, int tol_mode, size_t niter,
+    gooseberry_cgd_info_t *info)
+{
+  if (!mat || !d_b || !d_x) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gooseberry_batch_t B = NULL;
+  uint32_t slot = 0;
+  int rv = gooseberry_batch_create(1, 0.0, &B);
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gooseberry_batch_add(B, mat, d_b, d_x, tol, tol_mode, niter,
+        GOOSEBERRY_PRECISION_MIXED, &slot);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gooseberry_batch_solve(B);
+  gooseberry_cgd_info_t local;
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gooseberry_batch_info(B, slot, info ? info : &local);
+  if (B) gooseberry_batch_free(B);
+  return rv;
+}
+
+/* the reference's float build (real_t = float): the SpMV in FP32 with
+ * host pointers, B column-major, C row-major, as gooseberry_spmult */
+int gooseberry_spmult_f32(dim_t anrows, dim_t ancols, dim_t bncols,
+    const ind_t *arowptr, const dim_t *arowind, const float *arowval,
+    const float *bcolval, float *crowval, const dim_t *blocks, dim_t nblocks)
+{
+  dim_t rlo = 0, rhi = anrows;
+  if (blocks) {
+    for (dim_t c = 0; c < nblocks; ++c)
+      if (blocks[c + 1] < blocks[c]) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    if (nblocks && blocks[nblocks] > anrows) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    rlo = nblocks ? blocks[0] : 0;
+    rhi = nblocks ? blocks[nblocks] : 0;
+  }
+  if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
+  if (!arowptr || !bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  const size_t nnz = arowptr[anrows];
+  if (nnz && !arowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* FP64 copies of FP32 values are exact, and so is their FP32 copy back */
+  real_t *v64 = malloc((nnz ? nnz : 1) * sizeof(real_t));
+  float *col = malloc((size_t)anrows * sizeof(float));
+  if (!v64 || !col) { free(v64); free(col); return GOOSEBERRY_ERROR_NOMEMORY; }
+  for (size_t k = 0; k < nnz; ++k) v64[k] = arowval[k];
+  gooseberry_matrix_t mat = NULL;
+  void *dx = NULL, *dy = NULL;
+  int rv = gooseberry_matrix_create(anrows, ancols, arowptr, arowind, v64, NULL,
+      &mat);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dx, (size_t)ancols * sizeof(float));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dy, (size_t)anrows * sizeof(float));
+  for (dim_t j = 0; rv == GOOSEBERRY_SUCCESS && j < bncols; ++j) {
+    rv = gbd_htod(dx, bcolval + (size_t)j * ancols, (size_t)ancols * sizeof(float));
+    if (rv == GOOSEBERRY_SUCCESS) rv = gooseberry_matrix_spmv_f32(mat, dx, dy);
+    if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(col, dy, (size_t)anrows * sizeof(float));
+    for (dim_t i = rlo; rv == GOOSEBERRY_SUCCESS && i < rhi; ++i)
+      crowval[(size_t)i * bncols + j] = col[i];
+  }
+  gbd_free(dx);
+  gbd_free(dy);
+  gooseberry_matrix_free(mat);
+  free(v64);
+  free(col);
+  return rv;
+}

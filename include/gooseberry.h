@@ -449,6 +449,42 @@ int gooseberry_matrix_cgd(
     void * log,
     gooseberry_cgd_info_t * info);
 
+/* Mixed precision for one system (a one-system batch, see the batched
+ * solves below): FP32 inner iterations, FP64 dots, solution and reliable
+ * updates; converged on the FP64 residual.  info->niter counts inner
+ * iterations, info->nrefine the FP64 refreshes. */
+int gooseberry_matrix_cgd_mixed(
+    gooseberry_matrix_t mat,
+    const real_t * d_b,
+    real_t * d_x,
+    real_t tol,
+    int tol_mode,
+    size_t niter,
+    gooseberry_cgd_info_t * info);
+
+/* y = A x in FP32 (the matrix's FP32 value copy, made on first use), device
+ * vectors; rows of up to 64 entries are the row-sequential FP32 sums of the
+ * reference's float build. */
+int gooseberry_matrix_spmv_f32(
+    gooseberry_matrix_t mat,
+    const float * d_x,
+    float * d_y);
+
+/* include/gooseberry.h:174-184 for the reference's float build (real_t =
+ * float, the default without GOOSEBERRY_DOUBLE_REALS): the same call as
+ * gooseberry_spmult with FP32 values and vectors, host pointers. */
+int gooseberry_spmult_f32(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const ind_t * arowptr,
+    const dim_t * arowind,
+    const float * arowval,
+    const float * bcolval,
+    float * crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
 /* Host-pointer form of the reference's internal cgd() (src/cgd.h:29-34),
  * absolute threshold `error` as in the reference. */
 int gooseberry_cgd(
