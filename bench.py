@@ -43,6 +43,12 @@ timed on a bounded sample on this host).
            orderings per second, the bandwidth before and after, and the
            FP64 SpMV on the scrambled and on the reordered matrix; the
            oracle's sequential ordering as the CPU baseline
+  pagerank `gooseberry pagerank` on the Zipf graph of the zipf workload
+           (n = 2,000,000, weights |a_ij|, no sinks): damping 0.85, a fixed
+           100 iterations from the resident matrix (column sums,
+           normalisation and the power iteration all timed) with the
+           autotuned SpMV plan, and with the default plan; the oracle's
+           iterations on the same graph as the CPU baseline and check
 """
 import argparse
 import json
@@ -807,6 +813,95 @@ def run_rcm(args, gb):
                 cpu_baseline=cpu)
 
 
+# ------------------------------------------------------------ PageRank
+
+PR_ITERS = 100
+
+
+def run_pagerank(args, gb):
+    n = args.zipf_n
+    M0 = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    rp, ri, rv = M0.download()
+    del M0
+    rv = np.abs(rv)
+    nnz = int(rp[-1])
+    M = gb.Matrix(rp, ri, rv)
+    log("pagerank: graph n=%d nnz=%d (mean row %.2f)" % (n, nnz, nnz / n))
+    dout = gb.DeviceArray(n)
+
+    def timed(steps, warmup):
+        for _ in range(warmup):
+            M.pagerank_device(dout, 0.85, PR_ITERS, 0.0)
+        gb.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            it, rmse, nsinks = M.pagerank_device(dout, 0.85, PR_ITERS, 0.0)
+        gb.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms = elapsed / steps * 1e3
+        # the per-iteration cost: the same run capped at one iteration
+        # removes the preparation (column sums, normalisation, sinks)
+        M.pagerank_device(dout, 0.85, 1, 0.0)
+        gb.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            M.pagerank_device(dout, 0.85, 1, 0.0)
+        gb.synchronize()
+        ms1 = (time.perf_counter() - t1) / 3 * 1e3
+        return elapsed, ms, (ms - ms1) / (PR_ITERS - 1), ms1, nsinks
+
+    _, ms_default, it_ms_default, _, _ = timed(1, min(1, args.warmup))
+    plan, _ = M.autotune(gb.PRECISION_FP64, 5)
+    log("pagerank: default plan %.3f ms per iteration; autotuned plan %s"
+        % (it_ms_default, plan))
+    elapsed, ms, it_ms, ms1, nsinks = timed(args.steps, args.warmup)
+    # one iteration: the SpMV over the normalised values (12 nnz + 4 (n+1)
+    # + 16 n) and out = out d + c (16 n)
+    ab = 12 * nnz + 4 * (n + 1) + 32 * n
+    achieved = ab / (it_ms * 1e-3) / 1e9
+    cpu = None
+    check = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        orc.set_threads(threads)
+        nval, sinks = orc.pagerank_prepare(rp, ri, rv, n)
+        t = time.perf_counter()
+        orc.pagerank(rp, ri, nval, sinks, 0.85, 2, 0.0)
+        t = (time.perf_counter() - t) / 2
+        k = int(max(2, min(50, args.cpu_seconds * 0.7 / max(t, 1e-6))))
+        t = time.perf_counter()
+        ref, _, _ = orc.pagerank(rp, ri, nval, sinks, 0.85, k, 0.0)
+        t = time.perf_counter() - t
+        got = gb.DeviceArray(n)
+        M.pagerank_device(got, 0.85, k, 0.0)
+        g = got.numpy()
+        check = float(np.abs(g - ref).max() / np.abs(ref).max())
+        assert check <= 1e-12, check
+        cpu = {"value": k / t, "unit": "iterations/s", "cores": threads, "kind": "port",
+               "sample": "%d power iterations of the oracle (C + OpenMP restatement of "
+                         "src/gooseberry_bin.c:2622-2737) on %d threads on the same graph, "
+                         "column sums and normalisation excluded; its vector matches the "
+                         "GPU's to %.1e relative" % (k, threads, check)}
+    return line(PR_ITERS * args.steps / elapsed, "iterations/s", args, ms, "f64",
+                {"workload": "PageRank (damping 0.85, %d iterations) of the Zipf(1.5) graph "
+                             "n=%d nnz=%d (random symmetric pattern, degrees clipped to "
+                             "[2,4096] before symmetrising, seed 2), weights |a_ij|; "
+                             "autotuned SpMV plan" % (PR_ITERS, n, nnz),
+                 "n": n, "nnz": nnz, "iterations": PR_ITERS},
+                edges_per_s=nnz * PR_ITERS * args.steps / elapsed,
+                iteration_ms=it_ms, preparation_ms=ms1 - it_ms, sinks=nsinks,
+                plan=plan, default_plan_ms_per_step=ms_default,
+                default_plan_iteration_ms=it_ms_default,
+                roofline={"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
+                          "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                          "kernel": "one power iteration (SpMV over the normalised values, "
+                                    "sink sum, update)",
+                          "algorithmic_bytes_per_launch": ab, "avg_launch_ms": it_ms},
+                oracle_check_rel=check, cpu_baseline=cpu)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -820,7 +915,7 @@ def main():
                     help="untimed steps (default 2; zipf 1; batched 1)")
     ap.add_argument("--workload", default="laplacian",
                     choices=["laplacian", "zipf", "batched", "served", "spgemm", "ingest",
-                             "rcm"])
+                             "rcm", "pagerank"])
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
@@ -844,7 +939,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0),
-                "spgemm": (5, 1), "ingest": (5, 1), "rcm": (3, 1)}
+                "spgemm": (5, 1), "ingest": (5, 1), "rcm": (3, 1), "pagerank": (3, 1)}
     st, wu = defaults[args.workload]
     args.steps = st if args.steps is None else args.steps
     args.warmup = wu if args.warmup is None else args.warmup
@@ -853,7 +948,7 @@ def main():
 
     run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
            "served": run_served, "spgemm": run_spgemm, "ingest": run_ingest,
-           "rcm": run_rcm}[args.workload]
+           "rcm": run_rcm, "pagerank": run_pagerank}[args.workload]
     out = run(args, gb)
     print(json.dumps(out), flush=True)
 

@@ -426,6 +426,15 @@ class Matrix:
               "gooseberry_matrix_pagerank")
         return d.numpy(), int(info.niter), float(info.rmse), int(info.nsinks)
 
+    def pagerank_device(self, d_out, damping=0.85, niter=0, minerror=0.0):
+        """gooseberry_matrix_pagerank into a device vector: (iterations, rmse,
+        number of sinks)."""
+        info = PagerankInfo()
+        check(lib().gooseberry_matrix_pagerank(self.h, float(damping), int(niter),
+                                               float(minerror), d_out.ptr, C.byref(info)),
+              "gooseberry_matrix_pagerank")
+        return int(info.niter), float(info.rmse), int(info.nsinks)
+
     def autotune(self, precision=PRECISION_FP64, reps=5):
         """Time candidate plans from the row-length histogram, keep the
         fastest; returns (plan dict, SpMV ms)."""

@@ -8,7 +8,10 @@
 //   normalise    nval[l] = val[l] / indeg[rowind[l]] (IEEE division)
 //   sinks        the rows without entries, ascending (scan + compaction)
 // One iteration, three launches:
-//   SpMV         out = N rank (the matrix's own plan over nval)
+//   SpMV         out = N rank (the matrix's own plan over nval; with a
+//                column-split plan, over the slices' copy of the values
+//                normalised the same way, so each row is then summed per
+//                slice and the slices added in order)
 //   gb_pr_shift  c = (1 - d) / n + sum of rank[sinks] (the sink ranks are
 //                added in sink order, as the reference does, up to
 //                GB_PR_SEQ_SINKS sinks)
@@ -148,6 +151,9 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   double e = 0.0;
   bool finished = false;
   gbd_matrix mm;
+  gbd_matrix sv[GB_SPLIT_MAX];
+  gbd_matrix *svp[GB_SPLIT_MAX];
+  double *snval = nullptr;
   hipError_t he = hipSuccess;
 #define GB_PR_TRY(call)                                         \
   do {                                                          \
@@ -198,8 +204,26 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   mm = *m;
   mm.val = nval;
   mm.val32 = nullptr;
-  mm.nsplit = 0;  // the column-split views hold the unnormalised values
+  mm.nsplit = 0;
   mm.split = nullptr;
+  if (m->nsplit && nnz) {
+    // a column-split plan: the slices' copy of the values divided by the
+    // same column sums (the same quotient per entry), and views over it
+    GB_PR_TRY(gb_dev_alloc((void **)&snval, (size_t)nnz * 8));
+    gb_pr_normalise<<<gb_pr_grid(nnz), GB_BLOCK, 0, s>>>(nnz, m->srowind, m->sval, indeg,
+                                                         snval);
+    GB_PR_TRY(hipGetLastError());
+    for (uint32_t r = 0; r < m->nsplit; ++r) {
+      sv[r] = *m->split[r];
+      sv[r].val = snval;
+      sv[r].val32 = nullptr;
+      svp[r] = &sv[r];
+    }
+    mm.nsplit = m->nsplit;
+    mm.split = svp;
+    mm.sval = snval;
+    mm.sval32 = nullptr;
+  }
   gb_pr_fill<<<g, GB_BLOCK, 0, s>>>(n, 1.0 / n, rank);
   GB_PR_TRY(hipGetLastError());
   for (i = 0; (niter == 0 && i < GB_PR_MAX_ITER) || i < niter; ++i) {
@@ -243,6 +267,7 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
 done:
   gb_dev_free(indeg);
   gb_dev_free(nval);
+  gb_dev_free(snval);
   gb_dev_free(perm);
   gb_dev_free(dist);
   gb_dev_free(flag);

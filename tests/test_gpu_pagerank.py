@@ -60,6 +60,27 @@ def test_pagerank_long_rows_and_many_sinks(gb, oracle):
     assert np.max(np.abs(x - xo)) <= 1e-13 * np.max(np.abs(xo))
 
 
+@pytest.mark.parametrize("csplit", [2, 5])
+def test_pagerank_column_split_plan(gb, oracle, csplit):
+    # the iteration through column slices: each row summed per slice, then
+    # the slices added in order -- close to the oracle, with the same
+    # iteration count and sinks; the default plan on the same graph stays
+    # bitwise
+    n = 4000
+    rp, ri, rv = _random_graph(n, 7, sink_every=0)
+    xo, ito, erro, nso = oracle_pr(oracle, rp, ri, rv, n, 0.85, 0, 1e-9)
+    M = gb.Matrix(rp, ri, rv, plan={"csplit": csplit})
+    assert M.info()["plan"]["csplit"] == csplit
+    x, it, err, ns = M.pagerank(0.85, 0, 1e-9)
+    assert ns == nso and abs(it - ito) <= 1
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+    rp2, ri2, rv2 = _random_graph(n, 7, sink_every=9)
+    xo2, _, _, nso2 = oracle_pr(oracle, rp2, ri2, rv2, n, 0.85, 12, 0.0)
+    x2, it2, _, ns2 = gb.Matrix(rp2, ri2, rv2, plan={"csplit": csplit}).pagerank(0.85, 12, 0.0)
+    assert ns2 == nso2 and it2 == 12
+    assert np.max(np.abs(x2 - xo2)) <= 1e-12 * np.max(np.abs(xo2))
+
+
 def test_pagerank_all_sinks(gb):
     n = 5
     rp = np.zeros(n + 1, dtype=np.uint32)
