@@ -92,12 +92,12 @@ static inline uint32_t gb_spmv_nwork(const gbd_matrix *m) {
 // workgroups of one SpMV launch (the number of p.Ap partials with DOT)
 uint32_t gb_spmv_grid(const gbd_matrix *m);
 
-// p.Ap partials written by one SpMV launch with DOT (one per wavefront)
+// p.Ap partials written by one SpMV launch with DOT (one per workgroup)
 static inline uint32_t gb_spmv_npartials(const gbd_matrix *m) {
-  return (m->nsplit ? gb_split_sum_grid(m) : gb_spmv_grid(m)) * GB_WAVES_PER_BLOCK;
+  return m->nsplit ? gb_split_sum_grid(m) : gb_spmv_grid(m);
 }
 
-// Enqueue y = A x; when cg != nullptr also write the per-wavefront partial
+// Enqueue y = A x; when cg != nullptr also write the per-workgroup partial
 // sums of p.Ap (x = p, y = Ap) to partials[gb_spmv_npartials(m)].
 int gb_spmv_launch(const gbd_matrix *m, const double *x, double *y,
                    uint32_t ystride, double *partials, const gbd_cg_state *cg);

@@ -285,10 +285,17 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
   }
 
   if (DOT) {
-    // one partial per wavefront (no workgroup barrier at the end)
-    (void)red;
-    double ws = gb_wave_sum(dotp);
-    if (lane == 0) partials[blockIdx.x * GB_WAVES_PER_BLOCK + wave] = ws;
+    // one partial per workgroup: the wavefront sums added in wave order
+    // (`red` is used nowhere else, so no barrier is needed before it)
+    const double ws = gb_wave_sum(dotp);
+    if (lane == 0) red[wave] = ws;
+    __syncthreads();
+    if (tid == 0) {
+      double t = red[0];
+#pragma unroll
+      for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += red[q];
+      partials[blockIdx.x] = t;
+    }
   }
 }
 
@@ -340,7 +347,7 @@ gb_split_fill(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ 
 }
 
 // y[i] = ((yp_0[i] + yp_1[i]) + ...) + yp_{P-1}[i]; with DOT also the
-// per-wavefront partials of x.y
+// per-workgroup partials of x.y
 template <typename T, bool DOT>
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_split_sum(const T *__restrict__ yp, uint32_t n, uint32_t P, T *__restrict__ y,
@@ -353,9 +360,16 @@ gb_split_sum(const T *__restrict__ yp, uint32_t n, uint32_t P, T *__restrict__ y
     if (DOT) dotp += (double)x[i] * (double)s;
   }
   if (DOT) {
+    __shared__ double red[GB_WAVES_PER_BLOCK];
     const double ws = gb_wave_sum(dotp);
-    if ((threadIdx.x & (GB_WAVE - 1)) == 0)
-      partials[blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE] = ws;
+    if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = red[0];
+#pragma unroll
+      for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += red[q];
+      partials[blockIdx.x] = t;
+    }
   }
 }
 
