@@ -31,8 +31,13 @@ ERRORS = {
 }
 ERROR_CODES = {v: k for k, v in ERRORS.items()}
 
+FORMAT_RAW = 0
 FORMAT_GRID = 1
 FORMAT_CSR = 2
+FORMAT_POINT = 3
+FORMAT_SVM = 4
+FORMAT_GRAPH = 5
+FORMAT_CLU = 6
 FORMAT_MATRIXMARKET = 7
 FORMAT_BCSR = 8
 

@@ -9,6 +9,7 @@
  */
 #define _GNU_SOURCE
 #include <errno.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -430,6 +431,415 @@ static int read_bcsr(const char *file, dim_t *r_nrows, dim_t *r_ncols,
   return GOOSEBERRY_SUCCESS;
 }
 
+/* ---- the other sparse formats of src/io.c (METIS/Chaco graph, CLUTO,
+ * libsvm, coordinate points), read on the host.  Each follows its reference
+ * reader statement for statement.  The reference trusts every count a header
+ * gives and every 1-based index a line gives; where it would then write past
+ * its arrays or store an index wrapped below zero or beyond the matrix, these
+ * readers stop with GOOSEBERRY_ERROR_INVALIDINPUT and say why. */
+
+typedef struct {
+  ind_t *rowptr;
+  dim_t *rowind;
+  real_t *rowval;
+} csr_bufs_t;
+
+static void csr_bufs_free(csr_bufs_t *b)
+{
+  free(b->rowptr);
+  free(b->rowind);
+  free(b->rowval);
+  memset(b, 0, sizeof(*b));
+}
+
+static int csr_bufs_alloc(csr_bufs_t *b, size_t nrows, size_t nnz)
+{
+  b->rowptr = malloc((nrows + 1) * sizeof(ind_t));
+  b->rowind = malloc((nnz ? nnz : 1) * sizeof(dim_t));
+  b->rowval = malloc((nnz ? nnz : 1) * sizeof(real_t));
+  if (!b->rowptr || !b->rowind || !b->rowval) {
+    csr_bufs_free(b);
+    return 0;
+  }
+  return 1;
+}
+
+/* the first non-empty line that is not a comment, or NULL */
+static char *header_line(text_t *t)
+{
+  char *line;
+  ssize_t ll;
+  while ((ll = text_line(t, &line)) >= 0)
+    if (ll != 0 && !is_comment(line[0])) return line;
+  return NULL;
+}
+
+static int refuse(text_t *t, csr_bufs_t *b, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vfprintf(stderr, fmt, ap);
+  va_end(ap);
+  free(t->buf);
+  t->buf = NULL;
+  csr_bufs_free(b);
+  return GOOSEBERRY_ERROR_INVALIDINPUT;
+}
+
+/* columns must fall inside the matrix the reader reports */
+static int columns_in_range(const dim_t *rowind, ind_t nnz, dim_t ncols)
+{
+  for (ind_t k = 0; k < nnz; ++k)
+    if (rowind[k] >= ncols) return 0;
+  return 1;
+}
+
+/* src/io.c:242-367: header "n m [fmt]" (2m entries, weights when fmt is
+ * a nonzero number), then one line per vertex -- an empty line is a vertex
+ * without edges -- listing 1-based neighbours, each followed by its weight
+ * when there are weights; rows = lines read, ncols = nrows */
+static int read_graph(const char *file, dim_t *nrows, dim_t *ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  csr_bufs_t b = {0, 0, 0};
+  char *line = header_line(&t), *sptr, *eptr;
+  dim_t hn = 0;
+  ind_t cap = 0;
+  int do_wgt = 0;
+  if (line) {
+    sptr = line;
+    hn = (dim_t)strtoull(sptr, &eptr, 10);
+    sptr = eptr;
+    cap = (ind_t)strtoull(sptr, &eptr, 10) * 2;
+    sptr = eptr;
+    do_wgt = (int)strtoull(sptr, &eptr, 10);
+    if (sptr == eptr) do_wgt = 0;
+  }
+  if (hn == 0 && cap == 0)
+    return refuse(&t, &b, "Unable to read graph file or empty graph specified\n");
+  if (!csr_bufs_alloc(&b, hn, cap)) {
+    free(t.buf);
+    return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  dim_t n = 0;
+  ind_t j = 0;
+  ssize_t ll;
+  b.rowptr[0] = 0;
+  while ((ll = text_line(&t, &line)) >= 0) {
+    if (ll != 0 && is_comment(line[0])) continue;
+    if (n == hn)
+      return refuse(&t, &b, "Graph file has more vertex lines than its header's %" PRIu32 "\n", hn);
+    real_t f = 0.0;
+    sptr = line;
+    dim_t i = (dim_t)strtoull(sptr, &eptr, 10);
+    if (do_wgt) {
+      sptr = eptr;
+      f = strtod(sptr, &eptr);
+    }
+    while (eptr != sptr) {
+      if (j == cap)
+        return refuse(&t, &b, "Graph file has more edges than its header's %" PRIu32 "\n", cap / 2);
+      if (i == 0)
+        return refuse(&t, &b, "Graph vertex numbers start at 1 (line of vertex %" PRIu32 ")\n",
+            n + 1);
+      b.rowind[j] = i - 1;
+      b.rowval[j] = do_wgt ? f : 1.0;
+      ++j;
+      sptr = eptr;
+      i = (dim_t)strtoull(sptr, &eptr, 10);
+      if (do_wgt) {
+        sptr = eptr;
+        f = strtod(sptr, &eptr);
+      }
+    }
+    b.rowptr[++n] = j;
+  }
+  if (!columns_in_range(b.rowind, j, n))
+    return refuse(&t, &b, "Graph file names a vertex beyond its %" PRIu32 " vertex lines\n", n);
+  free(t.buf);
+  *r_rowptr = b.rowptr;
+  *r_rowind = b.rowind;
+  *r_rowval = b.rowval;
+  *nrows = n;
+  *ncols = n;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:501-611: header "nrows ncols nnz", then one line per row (empty
+ * lines and comments skipped) of 1-based "col val" pairs; ncols from the
+ * header */
+static int read_clu(const char *file, dim_t *nrows, dim_t *ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  csr_bufs_t b = {0, 0, 0};
+  char *line = header_line(&t), *sptr, *eptr;
+  dim_t hn = 0, maxcol = 0;
+  ind_t cap = 0;
+  if (!line) return refuse(&t, &b, "CLUTO file has no header line\n");
+  /* fields the header lacks stay 0, as in the reference */
+  (void)sscanf(line, "%" SCNu32 " %" SCNu32 " %" SCNu32, &hn, &maxcol, &cap);
+  if (!csr_bufs_alloc(&b, hn, cap)) {
+    free(t.buf);
+    return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  dim_t n = 0;
+  ind_t j = 0;
+  ssize_t ll;
+  b.rowptr[0] = 0;
+  while ((ll = text_line(&t, &line)) >= 0) {
+    if (ll == 0 || is_comment(line[0])) continue;
+    if (n == hn)
+      return refuse(&t, &b, "CLUTO file has more rows than its header's %" PRIu32 "\n", hn);
+    sptr = line;
+    dim_t i = (dim_t)strtoull(sptr, &eptr, 10);
+    sptr = eptr;
+    real_t f = strtod(sptr, &eptr);
+    while (eptr != sptr) {
+      if (j == cap)
+        return refuse(&t, &b, "CLUTO file has more entries than its header's %" PRIu32 "\n", cap);
+      if (i == 0 || i > maxcol)
+        return refuse(&t, &b, "CLUTO column %" PRIu32 " is outside 1..%" PRIu32 "\n", i, maxcol);
+      b.rowind[j] = i - 1;
+      b.rowval[j] = f;
+      ++j;
+      sptr = eptr;
+      i = (dim_t)strtoull(sptr, &eptr, 10);
+      sptr = eptr;
+      f = strtod(sptr, &eptr);
+    }
+    b.rowptr[++n] = j;
+  }
+  free(t.buf);
+  *r_rowptr = b.rowptr;
+  *r_rowind = b.rowind;
+  *r_rowval = b.rowval;
+  *nrows = n;
+  *ncols = maxcol;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:612-745: one line per row (empty lines and comments skipped):
+ * a label, then 1-based "index:value" items; the one character after an
+ * index is skipped whatever it is.  ncols = the largest index.  Where an
+ * index ends its line the reference steps past the line's end before
+ * reading a value; here the row ends there. */
+static int read_svm(const char *file, dim_t *nrows, dim_t *ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  csr_bufs_t b = {0, 0, 0};
+  arr_t ptr = {0, 0, 0, sizeof(ind_t)}, ind = {0, 0, 0, sizeof(dim_t)},
+        val = {0, 0, 0, sizeof(real_t)};
+  dim_t n = 0, maxcol = 0;
+  ind_t nnz = 0;
+  int ok = arr_push(&ptr, &nnz);
+  char *line, *sptr, *eptr;
+  ssize_t ll;
+  while (ok && (ll = text_line(&t, &line)) >= 0) {
+    if (ll == 0 || is_comment(line[0])) continue;
+    sptr = line;
+    (void)strtoll(sptr, &eptr, 10); /* the label */
+    sptr = eptr;
+    for (;;) {
+      dim_t i = (dim_t)strtoull(sptr, &eptr, 10);
+      if (*eptr == '\0') break;
+      sptr = eptr + 1;
+      real_t f = strtod(sptr, &eptr);
+      if (eptr == sptr) break;
+      if (i == 0) {
+        free(ptr.p); free(ind.p); free(val.p);
+        return refuse(&t, &b, "libsvm indices start at 1 (row %" PRIu32 ")\n", n + 1);
+      }
+      if (i > maxcol) maxcol = i;
+      const dim_t c = i - 1;
+      ok = arr_push(&ind, &c) && arr_push(&val, &f);
+      if (!ok) break;
+      ++nnz;
+      sptr = eptr;
+    }
+    ok = ok && arr_push(&ptr, &nnz);
+    ++n;
+  }
+  free(t.buf);
+  if (!ok) {
+    free(ptr.p); free(ind.p); free(val.p);
+    return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  *r_rowptr = ptr.p;
+  *r_rowind = ind.n ? ind.p : calloc(1, sizeof(dim_t));
+  *r_rowval = val.n ? val.p : calloc(1, sizeof(real_t));
+  *nrows = n;
+  *ncols = maxcol;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:746-920: "row col value" lines (empty lines and comments
+ * skipped) in any order.  The index bases follow the reference's rule: a
+ * row number that is not a new maximum becomes minrow (likewise columns),
+ * nrows = max + 1 - minrow; entries are grouped by row in file order. */
+static int read_point(const char *file, dim_t *r_nrows, dim_t *r_ncols,
+    ind_t **r_rowptr, dim_t **r_rowind, real_t **r_rowval)
+{
+  text_t t;
+  int rv = text_load(file, &t);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  csr_bufs_t b = {0, 0, 0};
+  arr_t ri = {0, 0, 0, sizeof(dim_t)}, ci = {0, 0, 0, sizeof(dim_t)},
+        wv = {0, 0, 0, sizeof(real_t)};
+  dim_t nrows = 0, ncols = 0, minrow = 1, mincol = 1;
+  ind_t nnz = 0;
+  int ok = 1;
+  char *line, *sptr, *eptr;
+  ssize_t ll;
+  while (ok && (ll = text_line(&t, &line)) >= 0) {
+    if (ll == 0 || is_comment(line[0])) continue;
+    sptr = line;
+    dim_t i = (dim_t)strtoull(sptr, &eptr, 10);
+    sptr = eptr;
+    dim_t j = (dim_t)strtoull(sptr, &eptr, 10);
+    sptr = eptr;
+    real_t w = strtod(sptr, &eptr);
+    if (eptr == sptr) {
+      free(ri.p); free(ci.p); free(wv.p);
+      return refuse(&t, &b, "Error in point list at line %" PRIu32 "\n", nnz);
+    }
+    if (i > nrows) nrows = i;
+    else minrow = i;
+    if (j > ncols) ncols = j;
+    else mincol = j;
+    ok = arr_push(&ri, &i) && arr_push(&ci, &j) && arr_push(&wv, &w);
+    ++nnz;
+  }
+  free(t.buf);
+  t.buf = NULL;
+  nrows += 1 - minrow;
+  ncols += 1 - mincol;
+  const dim_t *pi = ri.p, *pj = ci.p;
+  for (ind_t k = 0; ok && k < nnz; ++k) {
+    if ((dim_t)(pi[k] - minrow) >= nrows || (dim_t)(pj[k] - mincol) >= ncols) {
+      free(ri.p); free(ci.p); free(wv.p);
+      fprintf(stderr, "Point list entry %llu falls outside the %" PRIu32 " x %" PRIu32
+          " matrix its index bases give\n", (unsigned long long)k + 1, nrows, ncols);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+  }
+  if (!ok || !csr_bufs_alloc(&b, nrows, nnz)) {
+    free(ri.p); free(ci.p); free(wv.p);
+    return GOOSEBERRY_ERROR_UNKNOWN;
+  }
+  memset(b.rowptr, 0, ((size_t)nrows + 1) * sizeof(ind_t));
+  for (ind_t k = 0; k < nnz; ++k) ++b.rowptr[pi[k] - minrow + 1];
+  for (dim_t r = 0; r < nrows; ++r) b.rowptr[r + 1] += b.rowptr[r];
+  const real_t *pw = wv.p;
+  for (ind_t k = 0; k < nnz; ++k) {
+    const ind_t at = b.rowptr[pi[k] - minrow]++;
+    b.rowind[at] = pj[k] - mincol;
+    b.rowval[at] = pw[k];
+  }
+  for (dim_t r = nrows; r > 0; --r) b.rowptr[r] = b.rowptr[r - 1];
+  b.rowptr[0] = 0;
+  free(ri.p); free(ci.p); free(wv.p);
+  *r_rowptr = b.rowptr;
+  *r_rowind = b.rowind;
+  *r_rowval = b.rowval;
+  *r_nrows = nrows;
+  *r_ncols = ncols;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* src/io.c:1277-1366: loops left out (the edge count drops by one per row
+ * holding one), weights written when any two consecutive values among the
+ * first nnz differ, integral weights as integers */
+static int write_graph(const char *file, dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval)
+{
+  if (nrows != ncols) {
+    fprintf(stderr, "Graphs must have an equal number of rows and columns: %" PRIu32
+        "x%" PRIu32 "\n", nrows, ncols);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  FILE *f = fopen(file, "w");
+  if (!f) return errno_to_rv(errno);
+  ind_t nnz = rowptr[nrows];
+  for (dim_t i = 0; i < nrows; ++i)
+    for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j)
+      if (rowind[j] == i) {
+        --nnz;
+        break;
+      }
+  int ewgts = 0;
+  if (rowval)
+    for (ind_t j = 1; j < nnz; ++j)
+      if (rowval[j] != rowval[j - 1]) {
+        ewgts = 1;
+        break;
+      }
+  if (ewgts) {
+    fprintf(f, "%" PRIu32 " %" PRIu32 " 01\n", ncols, nnz / 2);
+    for (dim_t i = 0; i < nrows; ++i) {
+      for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+        if (rowind[j] == i) continue;
+        const real_t v = rowval[j];
+        /* the reference's v == (real_t)(long long)v, with the conversion
+         * defined: integral values inside the range of long long */
+        if (v >= -9223372036854775808.0 && v < 9223372036854775808.0 &&
+            v == (real_t)(long long)v)
+          fprintf(f, " %" PRIu32 " %lld", rowind[j] + 1, (long long)v);
+        else
+          fprintf(f, " %" PRIu32 " %g", rowind[j] + 1, v);
+      }
+      fprintf(f, "\n");
+    }
+  } else {
+    fprintf(f, "%" PRIu32 " %" PRIu32 "\n", ncols, nnz / 2);
+    for (dim_t i = 0; i < nrows; ++i) {
+      for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j)
+        if (rowind[j] != i) fprintf(f, " %" PRIu32, rowind[j] + 1);
+      fprintf(f, "\n");
+    }
+  }
+  return fclose(f) == 0 ? GOOSEBERRY_SUCCESS : GOOSEBERRY_ERROR_UNKNOWN;
+}
+
+/* src/io.c:1367-1418 (the banner names the reference's version, 0.1.7) */
+static int write_mtx(const char *file, dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval)
+{
+  FILE *f = fopen(file, "w");
+  if (!f) return errno_to_rv(errno);
+  fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n");
+  fprintf(f, "%%\n");
+  fprintf(f, "%% Written by Gooseberry version %d.%d.%d\n", GOOSEBERRY_VER_MAJOR,
+      GOOSEBERRY_VER_MINOR, GOOSEBERRY_VER_SUBMINOR);
+  fprintf(f, "%%   http://cs.umn.edu/~lasalle/gooseberry\n");
+  fprintf(f, "%%\n");
+  fprintf(f, "%" PRIu32 " %" PRIu32 " %" PRIu32 "\n", nrows, ncols, rowptr[nrows]);
+  for (dim_t i = 0; i < nrows; ++i)
+    for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j)
+      fprintf(f, "%" PRIu32 "\t%" PRIu32 "\t%lf\n", i + 1, rowind[j] + 1, rowval[j]);
+  return fclose(f) == 0 ? GOOSEBERRY_SUCCESS : GOOSEBERRY_ERROR_UNKNOWN;
+}
+
+/* src/io.c:1419-1462 */
+static int write_point(const char *file, dim_t nrows, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval)
+{
+  FILE *f = fopen(file, "w");
+  if (!f) return errno_to_rv(errno);
+  for (dim_t i = 0; i < nrows; ++i)
+    for (ind_t j = rowptr[i]; j < rowptr[i + 1]; ++j)
+      fprintf(f, "%" PRIu32 "\t%" PRIu32 "\t%lf\n", i + 1, rowind[j] + 1, rowval[j]);
+  return fclose(f) == 0 ? GOOSEBERRY_SUCCESS : GOOSEBERRY_ERROR_UNKNOWN;
+}
+
 /* src/io.c:1464-1549 dispatch */
 int gooseberry_read_sparse_matrix(int type, const char *file, dim_t *nrows,
     dim_t *ncols, ind_t **rowptr, dim_t **rowind, real_t **rowval)
@@ -444,11 +854,13 @@ int gooseberry_read_sparse_matrix(int type, const char *file, dim_t *nrows,
     case GOOSEBERRY_FORMAT_BCSR:
       return read_bcsr(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_GRAPH:
+      return read_graph(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_SVM:
+      return read_svm(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_CLU:
+      return read_clu(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_POINT:
-      fprintf(stderr, "Sparse format %d is not provided by this build\n", type);
-      return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+      return read_point(file, nrows, ncols, rowptr, rowind, rowval);
     default:
       fprintf(stderr, "Unrecognized sparse matrix format '%d'\n", type);
       return GOOSEBERRY_ERROR_INVALIDINPUT;
@@ -488,11 +900,13 @@ int gooseberry_write_sparse_matrix(int type, const char *file, dim_t nrows,
     case GOOSEBERRY_FORMAT_BCSR:
       return write_bcsr(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_GRAPH:
+      return write_graph(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_MATRIXMARKET:
+      if (rowptr[nrows] && (!rowind || !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+      return write_mtx(file, nrows, ncols, rowptr, rowind, rowval);
     case GOOSEBERRY_FORMAT_POINT:
-      fprintf(stderr, "Sparse output format %d is not provided by this build\n",
-          type);
-      return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+      if (rowptr[nrows] && (!rowind || !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+      return write_point(file, nrows, rowptr, rowind, rowval);
     default:
       fprintf(stderr, "Unrecognized sparse matrix format '%d'\n", type);
       return GOOSEBERRY_ERROR_INVALIDINPUT;

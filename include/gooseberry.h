@@ -182,10 +182,13 @@ int gooseberry_permute_cuthillmckee(
     dim_t * perm);
 
 /* include/gooseberry.h IO prototypes; src/io.c:1464-1549.  Sparse readers:
- * GOOSEBERRY_FORMAT_CSR, GOOSEBERRY_FORMAT_MATRIXMARKET and
- * GOOSEBERRY_FORMAT_BCSR; sparse writers: CSR and BCSR.  Dense: GRID.
- * Arrays returned through pointers are malloc'd; free with free() or
- * gooseberry_free().
+ * CSR, MATRIXMARKET, GRAPH, CLU, SVM, POINT and BCSR; sparse writers: CSR,
+ * GRAPH, MATRIXMARKET, POINT and BCSR.  Dense: GRID.  Arrays returned
+ * through pointers are malloc'd; free with free() or gooseberry_free().
+ * Where the reference's readers would write past their arrays or keep an
+ * index wrapped below zero or beyond the matrix (more rows or entries than
+ * a header announces, a 0 where indices are 1-based, a column past ncols),
+ * these return GOOSEBERRY_ERROR_INVALIDINPUT with a message on stderr.
  *
  * GOOSEBERRY_FORMAT_BCSR (new, extension .bcsr) is a binary CSR: the 8
  * bytes "GBCSR\0\1\0", uint32 nrows, uint32 ncols, uint64 nnz, then

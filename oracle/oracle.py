@@ -277,3 +277,288 @@ def cuthillmckee(rowptr, rowind, ncols=None, blocks=None):
     if code != 1:
         raise ValueError("orc_cuthillmckee: %d" % code)
     return perm
+
+
+# ---------------------------------------------------------------------------
+# The other sparse formats of src/io.c, restated in Python for small files.
+# Number conversions call the C library's strtoull/strtod/strtoll on the
+# line's bytes, so end positions and values are the reference's.  Lines are
+# split at '\n' with trailing '\r' removed; where the reference would write
+# past its arrays or store a wrapped index, these raise ValueError.
+
+_LIBC = None
+
+
+def _libc():
+    global _LIBC
+    if _LIBC is None:
+        c = C.CDLL(None)
+        for nm, rt in (("strtoull", C.c_ulonglong), ("strtoll", C.c_longlong),
+                       ("strtod", C.c_double)):
+            f = getattr(c, nm)
+            f.restype = rt
+            f.argtypes = ([C.c_void_p, C.POINTER(C.c_void_p)] if nm == "strtod"
+                          else [C.c_void_p, C.POINTER(C.c_void_p), C.c_int])
+        _LIBC = c
+    return _LIBC
+
+
+class _Line:
+    """One NUL-terminated line with the C conversions at byte offsets."""
+
+    def __init__(self, text):
+        self.buf = C.create_string_buffer(text)
+        self.base = C.addressof(self.buf)
+        self.n = len(text)
+
+    def _call(self, nm, off, *extra):
+        end = C.c_void_p()
+        v = getattr(_libc(), nm)(C.c_void_p(self.base + off), C.byref(end), *extra)
+        return v, end.value - self.base
+
+    def ull(self, off):
+        v, e = self._call("strtoull", off, 10)
+        return v & 0xFFFFFFFF, e, v
+
+    def ll(self, off):
+        return self._call("strtoll", off, 10)
+
+    def dbl(self, off):
+        return self._call("strtod", off)
+
+    def at(self, off):
+        return self.buf.raw[off:off + 1] if off <= self.n else b""
+
+
+def _lines(path):
+    data = open(path, "rb").read()
+    parts = data.split(b"\n")
+    if parts and parts[-1] == b"" and data.endswith(b"\n"):
+        parts.pop()
+    return [p.rstrip(b"\r\n") for p in parts] if data else []
+
+
+def _is_comment(line):
+    return line[:1] in (b"#", b"%", b"'", b'"', b"/")
+
+
+def _csr(rows, ncols):
+    rowptr = np.zeros(len(rows) + 1, dtype=np.uint32)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    ri = np.array([c for r in rows for c, _ in r], dtype=np.uint32)
+    rv = np.array([v for r in rows for _, v in r], dtype=np.float64)
+    return len(rows), ncols, rowptr, ri, rv
+
+
+def read_graph(path):
+    """src/io.c:242-367: (nrows, ncols, rowptr, rowind, rowval)."""
+    lines = _lines(path)
+    k = 0
+    while k < len(lines) and (lines[k] == b"" or _is_comment(lines[k])):
+        k += 1
+    hn = cap = wgt = 0
+    if k < len(lines):
+        L = _Line(lines[k])
+        hn, e, _ = L.ull(0)
+        m, e2, _ = L.ull(e)
+        cap = (m * 2) & 0xFFFFFFFF
+        _, e3, wfull = L.ull(e2)
+        # (int) of the conversion: its low 32 bits decide
+        wgt = (wfull & 0xFFFFFFFF) if e3 != e2 else 0
+    if hn == 0 and cap == 0:
+        raise ValueError("empty graph")
+    rows, nnz = [], 0
+    for line in lines[k + 1:]:
+        if line != b"" and _is_comment(line):
+            continue
+        if len(rows) == hn:
+            raise ValueError("more vertex lines than the header")
+        L = _Line(line)
+        row = []
+        i, e, _ = L.ull(0)
+        s = 0
+        f = 0.0
+        if wgt:
+            s = e
+            f, e = L.dbl(s)
+        while e != s:
+            if nnz == cap:
+                raise ValueError("more edges than the header")
+            if i == 0:
+                raise ValueError("vertex 0")
+            row.append((i - 1, f if wgt else 1.0))
+            nnz += 1
+            s = e
+            i, e, _ = L.ull(s)
+            if wgt:
+                s = e
+                f, e = L.dbl(s)
+        rows.append(row)
+    n = len(rows)
+    if any(c >= n for r in rows for c, _ in r):
+        raise ValueError("vertex beyond the vertex lines")
+    return _csr(rows, n)
+
+
+def read_clu(path):
+    """src/io.c:501-611."""
+    lines = [l for l in _lines(path)]
+    k = 0
+    while k < len(lines) and (lines[k] == b"" or _is_comment(lines[k])):
+        k += 1
+    if k == len(lines):
+        raise ValueError("no header")
+    fields = []
+    L = _Line(lines[k])
+    off = 0
+    for _ in range(3):
+        # sscanf %u: a conversion that fails stops the rest
+        v, e, _ = L.ull(off)
+        if e == off:
+            break
+        fields.append(v)
+        off = e
+    hn, maxcol, cap = (fields + [0, 0, 0])[:3]
+    rows, nnz = [], 0
+    for line in lines[k + 1:]:
+        if line == b"" or _is_comment(line):
+            continue
+        if len(rows) == hn:
+            raise ValueError("more rows than the header")
+        L = _Line(line)
+        i, e, _ = L.ull(0)
+        s = e
+        f, e = L.dbl(s)
+        row = []
+        while e != s:
+            if nnz == cap:
+                raise ValueError("more entries than the header")
+            if i == 0 or i > maxcol:
+                raise ValueError("column outside 1..ncols")
+            row.append((i - 1, f))
+            nnz += 1
+            s = e
+            i, e, _ = L.ull(s)
+            s = e
+            f, e = L.dbl(s)
+        rows.append(row)
+    return _csr(rows, maxcol)
+
+
+def read_svm(path):
+    """src/io.c:612-745 (a row ends where an index ends its line)."""
+    rows, maxcol = [], 0
+    for line in _lines(path):
+        if line == b"" or _is_comment(line):
+            continue
+        L = _Line(line)
+        _, e = L.ll(0)
+        s = e
+        row = []
+        while True:
+            i, e, _ = L.ull(s)
+            if L.at(e) in (b"\x00", b""):
+                break
+            s = e + 1
+            f, e = L.dbl(s)
+            if e == s:
+                break
+            if i == 0:
+                raise ValueError("index 0")
+            maxcol = max(maxcol, i)
+            row.append((i - 1, f))
+            s = e
+        rows.append(row)
+    return _csr(rows, maxcol)
+
+
+def read_point(path):
+    """src/io.c:746-920, with the reference's index-base rule."""
+    ents = []
+    nrows = ncols = 0
+    minrow = mincol = 1
+    for line in _lines(path):
+        if line == b"" or _is_comment(line):
+            continue
+        L = _Line(line)
+        i, e, _ = L.ull(0)
+        j, e, _ = L.ull(e)
+        s = e
+        w, e = L.dbl(s)
+        if e == s:
+            raise ValueError("error in point list at line %d" % len(ents))
+        if i > nrows:
+            nrows = i
+        else:
+            minrow = i
+        if j > ncols:
+            ncols = j
+        else:
+            mincol = j
+        ents.append((i, j, w))
+    nrows = (nrows + 1 - minrow) & 0xFFFFFFFF
+    ncols = (ncols + 1 - mincol) & 0xFFFFFFFF
+    rows = [[] for _ in range(nrows)]
+    for i, j, w in ents:
+        r, c = (i - minrow) & 0xFFFFFFFF, (j - mincol) & 0xFFFFFFFF
+        if r >= nrows or c >= ncols:
+            raise ValueError("entry outside the matrix")
+        rows[r].append((c, w))
+    return _csr(rows, ncols)
+
+
+def write_graph(path, rowptr, rowind, rowval, ncols):
+    """src/io.c:1277-1366 (text)."""
+    n = len(rowptr) - 1
+    if n != ncols:
+        raise ValueError("not square")
+    nnz = int(rowptr[n])
+    for i in range(n):
+        if any(int(rowind[j]) == i for j in range(rowptr[i], rowptr[i + 1])):
+            nnz -= 1
+    ew = rowval is not None and any(rowval[j] != rowval[j - 1] for j in range(1, nnz))
+    out = []
+    if ew:
+        out.append("%d %d 01\n" % (ncols, nnz // 2))
+        for i in range(n):
+            s = ""
+            for j in range(rowptr[i], rowptr[i + 1]):
+                c, v = int(rowind[j]), float(rowval[j])
+                if c == i:
+                    continue
+                if -2.0 ** 63 <= v < 2.0 ** 63 and v == float(int(v)):
+                    s += " %d %d" % (c + 1, int(v))
+                else:
+                    s += " %d %g" % (c + 1, v)
+            out.append(s + "\n")
+    else:
+        out.append("%d %d\n" % (ncols, nnz // 2))
+        for i in range(n):
+            out.append("".join(" %d" % (int(rowind[j]) + 1)
+                               for j in range(rowptr[i], rowptr[i + 1])
+                               if int(rowind[j]) != i) + "\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def write_mtx(path, rowptr, rowind, rowval, ncols):
+    """src/io.c:1367-1418 (text)."""
+    n = len(rowptr) - 1
+    out = ["%%MatrixMarket matrix coordinate real general\n", "%\n",
+           "% Written by Gooseberry version 0.1.7\n",
+           "%   http://cs.umn.edu/~lasalle/gooseberry\n", "%\n",
+           "%d %d %d\n" % (n, ncols, int(rowptr[n]))]
+    for i in range(n):
+        for j in range(rowptr[i], rowptr[i + 1]):
+            out.append("%d\t%d\t%f\n" % (i + 1, int(rowind[j]) + 1, float(rowval[j])))
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def write_point(path, rowptr, rowind, rowval):
+    """src/io.c:1419-1462 (text)."""
+    n = len(rowptr) - 1
+    with open(path, "w") as f:
+        for i in range(n):
+            for j in range(rowptr[i], rowptr[i + 1]):
+                f.write("%d\t%d\t%f\n" % (i + 1, int(rowind[j]) + 1, float(rowval[j])))

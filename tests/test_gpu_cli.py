@@ -107,3 +107,31 @@ def test_cgd_rejects_nonsquare(gb, oracle, tmp_path):
         f.write("1\n2\n")
     r = run("cgd", "-i", a, "-i", v, "-o", os.path.join(str(tmp_path), "x.vec"))
     assert r.returncode == 2 and "not square" in r.stderr
+
+
+@pytest.mark.parametrize("name,fmt", [("g.graph", 5), ("m.clu", 6), ("m.svm", 4),
+                                      ("m.ij", 3)])
+def test_blas_reads_the_other_sparse_formats(gb, oracle, tmp_path, name, fmt):
+    """`gooseberry blas` picks the reader by extension (.graph/.metis/.chaco,
+    .clu, .svm/.libfm, .ij/.point; src/gooseberry_bin.c:453-508); the output
+    equals the oracle's SpMV of the oracle's reading of the same file."""
+    from test_formats import generated_files
+    text = generated_files(7)[(name, fmt)]
+    a = os.path.join(str(tmp_path), name)
+    with open(a, "w") as f:
+        f.write(text)
+    ref_read = {5: oracle.read_graph, 6: oracle.read_clu, 4: oracle.read_svm,
+                3: oracle.read_point}[fmt]
+    nr, nc, orp, ori, orv = ref_read(a)
+    x = gb.uniform(3, nc, stream=1)
+    v = os.path.join(str(tmp_path), "x.vec")
+    with open(v, "w") as f:
+        f.write("\n".join("%.17g" % t for t in x) + "\n")
+    out = os.path.join(str(tmp_path), "y.vec")
+    r = run("blas", "-x", "multiply", "-i", a, "-i", v, "-o", out)
+    assert r.returncode == 0, r.stderr
+    _, _, ox = oracle.read_grid(v)
+    y = oracle.spmult(nr, nc, 1, orp, ori, orv, ox)
+    ref = os.path.join(str(tmp_path), "ref.vec")
+    oracle.write_grid(ref, y, nr, 1)
+    assert open(out).read() == open(ref).read()
