@@ -422,6 +422,15 @@ def run_batched(args, gb):
                 "all_converged": all(int(i.status) == 1 for i in infos),
                 "iterations_max": max(its), "iterations_mean": float(np.mean(its))}
 
+    nnzs = np.array([int(M.info()["nnz"]) for M in mats], dtype=np.float64)
+
+    def inner_bytes(infos):
+        """FP32 inner iterations' algorithmic bytes, each system's own count
+        (8 nnz + 4 (n+1) + 24 n per iteration; the FP64 refreshes not
+        counted, so a lower bound)"""
+        its = np.array([int(i.niter) for i in infos], dtype=np.float64)
+        return float(np.sum(its * (8 * nnzs + 4 * (ns + 1) + 24 * ns)))
+
     for _ in range(args.warmup):
         batch_solve(gb.PRECISION_MIXED)
     mixed_default = [batch_solve(gb.PRECISION_MIXED) for _ in range(args.steps)]
@@ -439,16 +448,24 @@ def run_batched(args, gb):
         M.replan(None)
 
     # one at a time: the plain FP64 GPU CG, a sample of the systems
-    sample = list(range(0, len(mats), max(1, len(mats) // 64)))
+    sample = [] if args.no_single else list(range(0, len(mats), max(1, len(mats) // 64)))
     t1 = time.perf_counter()
     single_iters = []
     for i in sample:
         info = mats[i].cgd_device(rhs[i], xs[i], TOL, gb.TOL_RELATIVE)
         single_iters.append(int(info.niter))
     gb.synchronize()
-    t_single = (time.perf_counter() - t1) / len(sample)
+    t_single = (time.perf_counter() - t1) / len(sample) if sample else None
 
     tm = float(np.median([a for a, _ in mixed_default]))
+    ab = inner_bytes(mixed_default[-1][1])
+    achieved = ab / mixed_default[-1][0] / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+            "kernel": "whole batched step (gb_b_spmv, gb_b_reduce, gb_b_update, gb_b_pupdate "
+                      "over all systems): FP32 inner-iteration bytes of every system over the "
+                      "solve's wall time",
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": mixed_default[-1][0] * 1e3}
     cfg = {"workload": ("%d independent Laplacians (2D 5-point / 3D 7-point alternating), n "
                         "log-uniform in [1000, 200000] (sum n=%d, nnz=%d), b U[-1,1) seed "
                         "3+i; step = all systems solved together by the batched engine, "
@@ -460,11 +477,11 @@ def run_batched(args, gb):
                mixed_default_plans=summary(*mixed_default[-1]),
                worst_sampled_true_rel_residual=worst,
                fp64_batched=summary(*fp64_batched),
-               one_at_a_time_fp64={"solves_per_s": 1.0 / t_single,
-                                   "systems_sampled": len(sample),
-                                   "iterations_mean": float(np.mean(single_iters))},
-               speedup_vs_one_at_a_time_fp64=(len(mats) / tm) * t_single,
-               roofline=None, cpu_baseline=None)
+               one_at_a_time_fp64=None if not sample else {
+                   "solves_per_s": 1.0 / t_single, "systems_sampled": len(sample),
+                   "iterations_mean": float(np.mean(single_iters))},
+               speedup_vs_one_at_a_time_fp64=None if not sample else (len(mats) / tm) * t_single,
+               roofline=roof, cpu_baseline=None)
     if mixed_tuned:
         out["mixed_autotuned_plans"] = summary(*mixed_tuned[-1])
         out["autotune"] = dict(tuned, seconds=tune_s)
@@ -927,6 +944,8 @@ def main():
     ap.add_argument("--capacity-requests", type=int, default=200,
                     help="served: requests of the backlog capacity comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single", action="store_true",
+                    help="batched: skip the one-at-a-time FP64 comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 4 device copies of exactly 256 MiB, the "
