@@ -582,6 +582,17 @@ def run_served(args, gb):
     del mats, bs, xs
     cap_served = kc / t_served
     cap_single = kc / t_single
+    # whole-run roofline of the served capacity: every request's FP32
+    # inner-iteration bytes (8 nnz + 4 (n+1) + 24 n per iteration, the FP64
+    # refreshes not counted) over the run's wall time
+    ab = float(sum(int(r["niter"]) * (8 * int(r["nnz"]) + 4 * (int(r["n"]) + 1)
+                                       + 24 * int(r["n"])) for r in cap_done))
+    achieved = ab / t_served / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+            "kernel": "whole served capacity run (batched SpMV, reductions and updates over "
+                      "all admitted requests): FP32 inner-iteration bytes over its wall time",
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": t_served * 1e3}
     log("served: capacity %.1f solves/s batched mixed vs %.1f one-at-a-time FP64"
         % (cap_served, cap_single))
 
@@ -603,7 +614,7 @@ def run_served(args, gb):
                           "served_all_converged": cap_ok,
                           "worst_sampled_true_rel_residual": worst,
                           "one_at_a_time_mean_iterations": float(np.mean(single_its))},
-                roofline=None, cpu_baseline=None)
+                roofline=roof, cpu_baseline=None)
 
 
 # ------------------------------------------------ sparse x sparse (SpGEMM)
