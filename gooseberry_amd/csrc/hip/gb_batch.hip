@@ -665,8 +665,8 @@ gb_b_init_state(const gb_bsys *__restrict__ sys, uint32_t slot, double tol,
 
 static void gb_b_drop_graphs(gbd_batch *B) {
   for (int i = 0; i < GB_B_NGRAPH; ++i) {
-    if (B->gexec[i]) hipGraphExecDestroy(B->gexec[i]);
-    if (B->graph[i]) hipGraphDestroy(B->graph[i]);
+    if (B->gexec[i]) (void)hipGraphExecDestroy(B->gexec[i]);
+    if (B->graph[i]) (void)hipGraphDestroy(B->graph[i]);
     B->gexec[i] = nullptr;
     B->graph[i] = nullptr;
   }
@@ -808,22 +808,22 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
 
 void gbd_batch_destroy(gbd_batch *B) {
   if (!B) return;
-  hipStreamSynchronize(gb_stream());
+  (void)hipStreamSynchronize(gb_stream());
   gb_b_drop_graphs(B);
   if (B->slot_mem)
-    for (uint32_t i = 0; i < B->max_slots; ++i) gb_dev_free(B->slot_mem[i]);
-  gb_dev_free(B->d_sys);
-  gb_dev_free(B->d_st);
-  gb_dev_free(B->d_lists);
-  gb_dev_free(B->d_slices);
-  gb_dev_free(B->d_spmv);
-  gb_dev_free(B->d_elem);
-  gb_dev_free(B->d_sysl);
-  if (B->h_st) hipHostFree(B->h_st);
-  if (B->h_lists) hipHostFree(B->h_lists);
-  if (B->h_spmv) hipHostFree(B->h_spmv);
-  if (B->h_elem) hipHostFree(B->h_elem);
-  if (B->h_sysl) hipHostFree(B->h_sysl);
+    for (uint32_t i = 0; i < B->max_slots; ++i) (void)gb_dev_free(B->slot_mem[i]);
+  (void)gb_dev_free(B->d_sys);
+  (void)gb_dev_free(B->d_st);
+  (void)gb_dev_free(B->d_lists);
+  (void)gb_dev_free(B->d_slices);
+  (void)gb_dev_free(B->d_spmv);
+  (void)gb_dev_free(B->d_elem);
+  (void)gb_dev_free(B->d_sysl);
+  if (B->h_st) (void)hipHostFree(B->h_st);
+  if (B->h_lists) (void)hipHostFree(B->h_lists);
+  if (B->h_spmv) (void)hipHostFree(B->h_spmv);
+  if (B->h_elem) (void)hipHostFree(B->h_elem);
+  if (B->h_sysl) (void)hipHostFree(B->h_sysl);
   free(B->used);
   free(B->slot_mem);
   free(B->h_slice_nwork);
@@ -881,7 +881,7 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
     S.part3 = (double *)(mem + L.part3);
     hsl = (gb_bplan *)calloc(m->nsplit, sizeof(gb_bplan));
     if (!hsl) {
-      gb_dev_free(mem);
+      (void)gb_dev_free(mem);
       return GOOSEBERRY_ERROR_NOMEMORY;
     }
     uint32_t base = 0;
@@ -898,7 +898,7 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     free(hsl);
     if (e != hipSuccess) {
-      gb_dev_free(mem);
+      (void)gb_dev_free(mem);
       return gb_hip_fail(e, "gbd_batch_add slices", __FILE__, __LINE__);
     }
   }
@@ -1037,7 +1037,7 @@ int gbd_batch_step(gbd_batch *B, uint32_t k) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
         if (rv != GOOSEBERRY_SUCCESS) {
-          if (g) hipGraphDestroy(g);
+          if (g) (void)hipGraphDestroy(g);
           return rv;
         }
         GB_HIP(e);

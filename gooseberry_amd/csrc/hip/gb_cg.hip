@@ -249,8 +249,8 @@ static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
 
 static void gb_cg_drop_graph(gbd_cg *cg) {
   for (int i = 0; i < GB_CG_NGRAPH; ++i) {
-    if (cg->gexec[i]) hipGraphExecDestroy(cg->gexec[i]);
-    if (cg->graph[i]) hipGraphDestroy(cg->graph[i]);
+    if (cg->gexec[i]) (void)hipGraphExecDestroy(cg->gexec[i]);
+    if (cg->graph[i]) (void)hipGraphDestroy(cg->graph[i]);
     cg->gexec[i] = nullptr;
     cg->graph[i] = nullptr;
   }
@@ -293,17 +293,17 @@ int gbd_cg_create(const gbd_matrix *m, gbd_cg **out) {
 
 void gbd_cg_destroy(gbd_cg *cg) {
   if (!cg) return;
-  hipStreamSynchronize(gb_stream());
+  (void)hipStreamSynchronize(gb_stream());
   gb_cg_drop_graph(cg);
-  gb_dev_free(cg->r);
-  gb_dev_free(cg->p);
-  gb_dev_free(cg->ap);
-  gb_dev_free(cg->partials);
-  gb_dev_free(cg->partials2);
-  gb_dev_free(cg->st);
-  gb_dev_free(cg->trace);
-  if (cg->h_st) hipHostFree(cg->h_st);
-  if (cg->h_trace) hipHostFree(cg->h_trace);
+  (void)gb_dev_free(cg->r);
+  (void)gb_dev_free(cg->p);
+  (void)gb_dev_free(cg->ap);
+  (void)gb_dev_free(cg->partials);
+  (void)gb_dev_free(cg->partials2);
+  (void)gb_dev_free(cg->st);
+  (void)gb_dev_free(cg->trace);
+  if (cg->h_st) (void)hipHostFree(cg->h_st);
+  if (cg->h_trace) (void)hipHostFree(cg->h_trace);
   free(cg);
 }
 
@@ -353,7 +353,7 @@ int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(s, &g);
         if (rv != GOOSEBERRY_SUCCESS) {
-          if (g) hipGraphDestroy(g);
+          if (g) (void)hipGraphDestroy(g);
           return rv;
         }
         GB_HIP(e);

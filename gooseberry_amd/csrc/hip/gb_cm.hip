@@ -432,9 +432,9 @@ static int gb_cm_block(uint32_t start, uint32_t end, const uint32_t *rowptr,
   GB_CM_TRY(hipStreamSynchronize(s));
   rv = GOOSEBERRY_SUCCESS;
 done:
-  gb_dev_free(deg); gb_dev_free(iso); gb_dev_free(S); gb_dev_free(srank);
-  gb_dev_free(posof); gb_dev_free(claim); gb_dev_free(list); gb_dev_free(list2);
-  gb_dev_free(keys); gb_dev_free(flag); gb_dev_free(frank); gb_dev_free(dv);
+  (void)gb_dev_free(deg); (void)gb_dev_free(iso); (void)gb_dev_free(S); (void)gb_dev_free(srank);
+  (void)gb_dev_free(posof); (void)gb_dev_free(claim); (void)gb_dev_free(list); (void)gb_dev_free(list2);
+  (void)gb_dev_free(keys); (void)gb_dev_free(flag); (void)gb_dev_free(frank); (void)gb_dev_free(dv);
   return rv;
 #undef GB_CM_TRY
 #undef GB_CM_RV

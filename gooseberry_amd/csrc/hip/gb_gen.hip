@@ -230,8 +230,8 @@ int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
     if (rv != GOOSEBERRY_SUCCESS) goto done;                   \
   } while (0)
 
-  GB_GEN_TRY(hipMallocAsync((void **)&off, ((size_t)n + 1) * 4, s));
-  GB_GEN_TRY(hipMallocAsync((void **)&tot, sizeof(*tot), s));
+  GB_GEN_TRY(gb_dev_alloc((void **)&off, ((size_t)n + 1) * 4));
+  GB_GEN_TRY(gb_dev_alloc((void **)&tot, sizeof(*tot)));
   GB_GEN_TRY(hipMemsetAsync(tot, 0, sizeof(*tot), s));
   gb_gen_outdeg<<<grid, GB_BLOCK, 0, s>>>(P, off);
   GB_GEN_TRY(hipGetLastError());
@@ -247,10 +247,10 @@ int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
   m = (uint32_t)m64;
   nnz = 2 * m + n;
   GB_GEN_RV(gbd_scan_u32(off, off, n, off + n));
-  GB_GEN_TRY(hipMallocAsync((void **)&src, (size_t)m * 4 + 4, s));
-  GB_GEN_TRY(hipMallocAsync((void **)&dst, (size_t)m * 4 + 4, s));
-  GB_GEN_TRY(hipMallocAsync((void **)&ine, (size_t)m * 4 + 4, s));
-  GB_GEN_TRY(hipMallocAsync((void **)&inptr, ((size_t)n + 1) * 4, s));
+  GB_GEN_TRY(gb_dev_alloc((void **)&src, (size_t)m * 4 + 4));
+  GB_GEN_TRY(gb_dev_alloc((void **)&dst, (size_t)m * 4 + 4));
+  GB_GEN_TRY(gb_dev_alloc((void **)&ine, (size_t)m * 4 + 4));
+  GB_GEN_TRY(gb_dev_alloc((void **)&inptr, ((size_t)n + 1) * 4));
   gb_gen_edges<<<grid, GB_BLOCK, 0, s>>>(P, off, src, dst);
   GB_GEN_TRY(hipGetLastError());
   // every dst is in [0, n) by construction
@@ -280,15 +280,15 @@ int gbd_generate_spd(uint32_t n, uint64_t seed, int law, double zipf_s,
   rowval = nullptr;
   rv = GOOSEBERRY_SUCCESS;
 done:
-  hipFreeAsync(off, s);
-  hipFreeAsync(tot, s);
-  hipFreeAsync(src, s);
-  hipFreeAsync(dst, s);
-  hipFreeAsync(ine, s);
-  hipFreeAsync(inptr, s);
-  gb_dev_free(rowptr);
-  gb_dev_free(rowind);
-  gb_dev_free(rowval);
+  (void)gb_dev_free(off);
+  (void)gb_dev_free(tot);
+  (void)gb_dev_free(src);
+  (void)gb_dev_free(dst);
+  (void)gb_dev_free(ine);
+  (void)gb_dev_free(inptr);
+  (void)gb_dev_free(rowptr);
+  (void)gb_dev_free(rowind);
+  (void)gb_dev_free(rowval);
   return rv;
 #undef GB_GEN_TRY
 #undef GB_GEN_RV

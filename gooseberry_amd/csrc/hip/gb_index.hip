@@ -75,14 +75,14 @@ static int gb_scan_u32(const uint32_t *d_in, uint32_t *d_out, size_t n,
   }
   size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   uint32_t *sums = nullptr;
-  GB_HIP(hipMallocAsync((void **)&sums, nb * sizeof(uint32_t), s));
+  GB_HIP(gb_dev_alloc((void **)&sums, nb * sizeof(uint32_t)));
   gb_scan_tile_sums<<<(uint32_t)nb, GB_BLOCK, 0, s>>>(d_in, n, sums);
   GB_LAUNCH_CHECK();
   gb_scan_top<<<1, GB_BLOCK, 0, s>>>(sums, nb, d_total);
   GB_LAUNCH_CHECK();
   gb_scan_tile_apply<<<(uint32_t)nb, GB_BLOCK, 0, s>>>(d_in, d_out, n, sums);
   GB_LAUNCH_CHECK();
-  GB_HIP(hipFreeAsync(sums, s));
+  GB_HIP(gb_dev_free(sums));
   return GOOSEBERRY_SUCCESS;
 }
 
@@ -300,13 +300,13 @@ static int gb_countingsort(const uint32_t *d_keys, const uint32_t *d_vals,
   // keys outside [kmin, kmax] would index out of bounds: reject them
   if (n && check) {
     uint32_t *bad = nullptr, hbad = 0;
-    GB_HIP(hipMallocAsync((void **)&bad, sizeof(uint32_t), s));
+    GB_HIP(gb_dev_alloc((void **)&bad, sizeof(uint32_t)));
     GB_HIP(hipMemsetAsync(bad, 0, sizeof(uint32_t), s));
     gb_cs_check_keys<<<gb_grid_stride_blocks(n), GB_BLOCK, 0, s>>>(
         d_keys, n, kmin, kmax, bad);
     GB_LAUNCH_CHECK();
     GB_HIP(hipMemcpyAsync(&hbad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    GB_HIP(hipFreeAsync(bad, s));
+    GB_HIP(gb_dev_free(bad));
     GB_HIP(hipStreamSynchronize(s));
     if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
@@ -336,11 +336,11 @@ static int gb_countingsort(const uint32_t *d_keys, const uint32_t *d_vals,
   const size_t ncounts = (size_t)CS_RADIX * nch;
   uint32_t *counts = nullptr, *tk[2] = {nullptr, nullptr},
            *tv[2] = {nullptr, nullptr};
-  GB_HIP(hipMallocAsync((void **)&counts, ncounts * sizeof(uint32_t), s));
-  GB_HIP(hipMallocAsync((void **)&tk[0], n * sizeof(uint32_t), s));
-  GB_HIP(hipMallocAsync((void **)&tk[1], n * sizeof(uint32_t), s));
-  GB_HIP(hipMallocAsync((void **)&tv[0], n * sizeof(uint32_t), s));
-  if (npass > 1) GB_HIP(hipMallocAsync((void **)&tv[1], n * sizeof(uint32_t), s));
+  GB_HIP(gb_dev_alloc((void **)&counts, ncounts * sizeof(uint32_t)));
+  GB_HIP(gb_dev_alloc((void **)&tk[0], n * sizeof(uint32_t)));
+  GB_HIP(gb_dev_alloc((void **)&tk[1], n * sizeof(uint32_t)));
+  GB_HIP(gb_dev_alloc((void **)&tv[0], n * sizeof(uint32_t)));
+  if (npass > 1) GB_HIP(gb_dev_alloc((void **)&tv[1], n * sizeof(uint32_t)));
   const uint32_t *ck = d_keys, *cv = d_vals;
   for (int p = 0; p < npass; ++p) {
     const int shift = 8 * p;
@@ -357,11 +357,11 @@ static int gb_countingsort(const uint32_t *d_keys, const uint32_t *d_vals,
     ck = ok;
     cv = ov;
   }
-  GB_HIP(hipFreeAsync(counts, s));
-  GB_HIP(hipFreeAsync(tk[0], s));
-  GB_HIP(hipFreeAsync(tk[1], s));
-  GB_HIP(hipFreeAsync(tv[0], s));
-  if (tv[1]) GB_HIP(hipFreeAsync(tv[1], s));
+  GB_HIP(gb_dev_free(counts));
+  GB_HIP(gb_dev_free(tk[0]));
+  GB_HIP(gb_dev_free(tk[1]));
+  GB_HIP(gb_dev_free(tv[0]));
+  if (tv[1]) GB_HIP(gb_dev_free(tv[1]));
   return GOOSEBERRY_SUCCESS;
 }
 
@@ -391,7 +391,7 @@ int gbd_permute_sparse(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
   GB_HIP(hipStreamSynchronize(s));
   uint32_t *rename = nullptr;
   if (d_colperm && ncols) {
-    GB_HIP(hipMallocAsync((void **)&rename, (size_t)ncols * sizeof(uint32_t), s));
+    GB_HIP(gb_dev_alloc((void **)&rename, (size_t)ncols * sizeof(uint32_t)));
     gb_perm_rename<<<gb_div_up(ncols, GB_BLOCK), GB_BLOCK, 0, s>>>(d_colperm,
                                                                   ncols, rename);
     GB_LAUNCH_CHECK();
@@ -399,13 +399,13 @@ int gbd_permute_sparse(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
   if (d_rowperm) {
     if (nrows) {
       uint32_t *len = nullptr;
-      GB_HIP(hipMallocAsync((void **)&len, (size_t)nrows * sizeof(uint32_t), s));
+      GB_HIP(gb_dev_alloc((void **)&len, (size_t)nrows * sizeof(uint32_t)));
       gb_perm_rowlen<<<gb_div_up(nrows, GB_BLOCK), GB_BLOCK, 0, s>>>(
           d_rowptr, d_rowperm, nrows, len);
       GB_LAUNCH_CHECK();
       rv = gb_scan_u32(len, d_prowptr, nrows, d_prowptr + nrows);
       if (rv != GOOSEBERRY_SUCCESS) return rv;
-      GB_HIP(hipFreeAsync(len, s));
+      GB_HIP(gb_dev_free(len));
     } else {
       GB_HIP(hipMemsetAsync(d_prowptr, 0, sizeof(uint32_t), s));
     }
@@ -431,7 +431,7 @@ int gbd_permute_sparse(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
       }
     }
   }
-  if (rename) GB_HIP(hipFreeAsync(rename, s));
+  if (rename) GB_HIP(gb_dev_free(rename));
   return GOOSEBERRY_SUCCESS;
 }
 

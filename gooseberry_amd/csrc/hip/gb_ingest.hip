@@ -502,7 +502,7 @@ int gbd_load_file(const char *file, char **d_txt, uint64_t *r_len) {
   rv = gb_in_file_to_device(f, 0, len, d);
   fclose(f);
   if (rv != GOOSEBERRY_SUCCESS) {
-    gb_dev_free(d);
+    (void)gb_dev_free(d);
     return rv;
   }
   *d_txt = d;
@@ -633,7 +633,7 @@ int gbd_parse_csr(const char *d_txt, uint64_t len, uint32_t *r_nrows, uint32_t *
       GB_IN_TRY(hipMemcpyAsync(&hnfix, nfix, 4, hipMemcpyDeviceToHost, s));
       GB_IN_TRY(hipStreamSynchronize(s));
       if (hnfix <= fixcap) break;
-      gb_dev_free(fix);
+      (void)gb_dev_free(fix);
       fix = nullptr;
       fixcap = hnfix;
     }
@@ -657,9 +657,9 @@ int gbd_parse_csr(const char *d_txt, uint64_t len, uint32_t *r_nrows, uint32_t *
   }
   rv = GOOSEBERRY_SUCCESS;
 done:
-  gb_dev_free(ccnt); gb_dev_free(ls); gb_dev_free(rowof); gb_dev_free(cnt);
-  gb_dev_free(minmax); gb_dev_free(tot); gb_dev_free(fix); gb_dev_free(nfix);
-  gb_dev_free(rowptr); gb_dev_free(rowind); gb_dev_free(rowval);
+  (void)gb_dev_free(ccnt); (void)gb_dev_free(ls); (void)gb_dev_free(rowof); (void)gb_dev_free(cnt);
+  (void)gb_dev_free(minmax); (void)gb_dev_free(tot); (void)gb_dev_free(fix); (void)gb_dev_free(nfix);
+  (void)gb_dev_free(rowptr); (void)gb_dev_free(rowind); (void)gb_dev_free(rowval);
   return rv;
 #undef GB_IN_TRY
 #undef GB_IN_RV
@@ -758,7 +758,7 @@ int gbd_parse_mtx_body(const char *d_txt, uint64_t len, uint64_t off, uint32_t n
     GB_MX_TRY(hipMemcpyAsync(&hnfix, nfix, 4, hipMemcpyDeviceToHost, s));
     GB_MX_TRY(hipStreamSynchronize(s));
     if (hnfix <= fixcap) break;
-    gb_dev_free(fix);
+    (void)gb_dev_free(fix);
     fix = nullptr;
     fixcap = hnfix;
   }
@@ -797,9 +797,9 @@ int gbd_parse_mtx_body(const char *d_txt, uint64_t len, uint64_t off, uint32_t n
   rowval = nullptr;
   rv = GOOSEBERRY_SUCCESS;
 done:
-  gb_dev_free(ccnt); gb_dev_free(ls); gb_dev_free(ent); gb_dev_free(ei); gb_dev_free(ej);
-  gb_dev_free(ew); gb_dev_free(flags); gb_dev_free(nfix); gb_dev_free(fix); gb_dev_free(key);
-  gb_dev_free(ord); gb_dev_free(rowptr); gb_dev_free(rowind); gb_dev_free(rowval);
+  (void)gb_dev_free(ccnt); (void)gb_dev_free(ls); (void)gb_dev_free(ent); (void)gb_dev_free(ei); (void)gb_dev_free(ej);
+  (void)gb_dev_free(ew); (void)gb_dev_free(flags); (void)gb_dev_free(nfix); (void)gb_dev_free(fix); (void)gb_dev_free(key);
+  (void)gb_dev_free(ord); (void)gb_dev_free(rowptr); (void)gb_dev_free(rowind); (void)gb_dev_free(rowval);
   return rv;
 #undef GB_MX_TRY
 #undef GB_MX_RV

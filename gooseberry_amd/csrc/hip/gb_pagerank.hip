@@ -265,19 +265,19 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   if (r_nsinks) *r_nsinks = nsinks;
   rv = GOOSEBERRY_SUCCESS;
 done:
-  gb_dev_free(indeg);
-  gb_dev_free(nval);
-  gb_dev_free(snval);
-  gb_dev_free(perm);
-  gb_dev_free(dist);
-  gb_dev_free(flag);
-  gb_dev_free(idx);
-  gb_dev_free(sinks);
-  gb_dev_free(rank);
-  gb_dev_free(out);
-  gb_dev_free(c);
-  gb_dev_free(part);
-  gb_dev_free(err);
+  (void)gb_dev_free(indeg);
+  (void)gb_dev_free(nval);
+  (void)gb_dev_free(snval);
+  (void)gb_dev_free(perm);
+  (void)gb_dev_free(dist);
+  (void)gb_dev_free(flag);
+  (void)gb_dev_free(idx);
+  (void)gb_dev_free(sinks);
+  (void)gb_dev_free(rank);
+  (void)gb_dev_free(out);
+  (void)gb_dev_free(c);
+  (void)gb_dev_free(part);
+  (void)gb_dev_free(err);
   return rv;
 #undef GB_PR_TRY
 #undef GB_PR_RV

@@ -3,12 +3,19 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <list>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+
 #include "gb_common.h"
 
 static hipStream_t g_stream = nullptr;
 static pthread_once_t g_once = PTHREAD_ONCE_INIT;
 static int g_init_rv = GOOSEBERRY_SUCCESS;
 static char g_last_error[512] = "";
+static size_t g_cache_cap = 0;  // bytes the large-block cache may hold
 
 int gb_hip_fail(hipError_t e, const char *what, const char *file, int line) {
   snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)",
@@ -50,15 +57,104 @@ static void gb_do_init() {
     g_init_rv = gb_hip_fail(e, "default memory pool", __FILE__, __LINE__);
     return;
   }
+  // the block cache holds at most an eighth of the device, and 32 GiB
+  size_t fr = 0, tot = 0;
+  e = hipMemGetInfo(&fr, &tot);
+  if (e != hipSuccess) {
+    g_init_rv = gb_hip_fail(e, "hipMemGetInfo", __FILE__, __LINE__);
+    return;
+  }
+  g_cache_cap = tot / 8 < (32ull << 30) ? tot / 8 : (32ull << 30);
+}
+
+// Large blocks (4 MiB and up, sizes rounded to 2 MiB) that the library
+// frees are kept here and handed back to a later allocation of the same
+// rounded size.  The default pool gave such blocks back to the driver at
+// free despite its release threshold: a free of a few hundred MiB took
+// 0.65-2.5 ms and the next allocation of that size 50-350 us, several
+// milliseconds per sparse product.  Every library launch and copy runs on
+// g_stream, so a block's reuse is ordered after its last use exactly as a
+// pool block's is.  The cache holds at most g_cache_cap bytes (oldest
+// blocks go back first), is emptied when an allocation finds the device
+// out of memory, and its bytes count as free in gbd_device_memory.
+#define GB_CACHE_MIN (4ull << 20)
+#define GB_CACHE_ROUND (2ull << 20)
+
+static std::mutex g_cache_mu;
+static std::unordered_map<void *, size_t> g_cache_live;  // handed out: size
+static std::list<std::pair<size_t, void *>> g_cache_fifo;  // free, oldest first
+static std::multimap<size_t, std::list<std::pair<size_t, void *>>::iterator> g_cache_free;
+static size_t g_cache_bytes = 0;
+
+static void gb_cache_evict_oldest() {  // g_cache_mu held
+  auto &front = g_cache_fifo.front();
+  auto range = g_cache_free.equal_range(front.first);
+  for (auto it = range.first; it != range.second; ++it)
+    if (it->second == g_cache_fifo.begin()) {
+      g_cache_free.erase(it);
+      break;
+    }
+  g_cache_bytes -= front.first;
+  (void)hipFreeAsync(front.second, g_stream);
+  g_cache_fifo.pop_front();
+}
+
+static void gb_cache_trim() {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  while (!g_cache_fifo.empty()) gb_cache_evict_oldest();
+}
+
+size_t gb_cache_cached_bytes() {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  return g_cache_bytes;
 }
 
 hipError_t gb_dev_alloc(void **p, size_t bytes) {
   *p = nullptr;
-  return hipMallocAsync(p, bytes ? bytes : 16, g_stream);
+  if (bytes < GB_CACHE_MIN) return hipMallocAsync(p, bytes ? bytes : 16, g_stream);
+  const size_t r = (bytes + GB_CACHE_ROUND - 1) & ~(GB_CACHE_ROUND - 1);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_cache_free.find(r);
+    if (it != g_cache_free.end()) {
+      *p = it->second->second;
+      g_cache_fifo.erase(it->second);
+      g_cache_free.erase(it);
+      g_cache_bytes -= r;
+      g_cache_live[*p] = r;
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipMallocAsync(p, r, g_stream);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    gb_cache_trim();
+    e = hipMallocAsync(p, r, g_stream);
+  }
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache_live[*p] = r;
+  }
+  return e;
 }
 
 hipError_t gb_dev_free(void *p) {
   if (!p) return hipSuccess;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_cache_live.find(p);
+    if (it != g_cache_live.end()) {
+      const size_t r = it->second;
+      g_cache_live.erase(it);
+      if (r <= g_cache_cap) {
+        while (g_cache_bytes + r > g_cache_cap) gb_cache_evict_oldest();
+        g_cache_fifo.emplace_back(r, p);
+        g_cache_free.emplace(r, std::prev(g_cache_fifo.end()));
+        g_cache_bytes += r;
+        return hipSuccess;
+      }
+    }
+  }
   return hipFreeAsync(p, g_stream);
 }
 
@@ -207,6 +303,8 @@ int gbd_device_memory(size_t *free_bytes, size_t *total_bytes) {
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   size_t f = 0, t = 0;
   GB_HIP(hipMemGetInfo(&f, &t));
+  f += gb_cache_cached_bytes();  // cached blocks are given back on demand
+  if (f > t) f = t;
   if (free_bytes) *free_bytes = f;
   if (total_bytes) *total_bytes = t;
   return GOOSEBERRY_SUCCESS;

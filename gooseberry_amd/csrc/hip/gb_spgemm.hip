@@ -544,13 +544,13 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
   GB_SG_TRY(hipStreamSynchronize(s));
   rv = GOOSEBERRY_SUCCESS;
 done:
-  gb_dev_free(len); gb_dev_free(off); gb_dev_free(isshort); gb_dev_free(islong);
-  gb_dev_free(slist); gb_dev_free(len2); gb_dev_free(off2); gb_dev_free(erow);
-  gb_dev_free(ecol); gb_dev_free(eval); gb_dev_free(perm1); gb_dev_free(row1);
-  gb_dev_free(ord); gb_dev_free(flag); gb_dev_free(rid); gb_dev_free(runstart);
-  gb_dev_free(rrow); gb_dev_free(rcol); gb_dev_free(rfirst); gb_dev_free(rval);
-  gb_dev_free(ordr); gb_dev_free(rbeg); gb_dev_free(cnt); gb_dev_free(crowptr);
-  gb_dev_free(crowind); gb_dev_free(crowval); gb_dev_free(tot);
+  (void)gb_dev_free(len); (void)gb_dev_free(off); (void)gb_dev_free(isshort); (void)gb_dev_free(islong);
+  (void)gb_dev_free(slist); (void)gb_dev_free(len2); (void)gb_dev_free(off2); (void)gb_dev_free(erow);
+  (void)gb_dev_free(ecol); (void)gb_dev_free(eval); (void)gb_dev_free(perm1); (void)gb_dev_free(row1);
+  (void)gb_dev_free(ord); (void)gb_dev_free(flag); (void)gb_dev_free(rid); (void)gb_dev_free(runstart);
+  (void)gb_dev_free(rrow); (void)gb_dev_free(rcol); (void)gb_dev_free(rfirst); (void)gb_dev_free(rval);
+  (void)gb_dev_free(ordr); (void)gb_dev_free(rbeg); (void)gb_dev_free(cnt); (void)gb_dev_free(crowptr);
+  (void)gb_dev_free(crowind); (void)gb_dev_free(crowval); (void)gb_dev_free(tot);
   return rv;
 #undef GB_SG_TRY
 #undef GB_SG_RV

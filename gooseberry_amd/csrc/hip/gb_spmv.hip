@@ -467,11 +467,11 @@ static void gb_split_free(gbd_matrix *m) {
   free(m->split);
   m->split = nullptr;
   m->nsplit = 0;
-  gb_dev_free(m->srowptr);
-  gb_dev_free(m->srowind);
-  gb_dev_free(m->sval);
-  gb_dev_free(m->sval32);
-  gb_dev_free(m->ypart);
+  (void)gb_dev_free(m->srowptr);
+  (void)gb_dev_free(m->srowind);
+  (void)gb_dev_free(m->sval);
+  (void)gb_dev_free(m->sval32);
+  (void)gb_dev_free(m->ypart);
   m->srowptr = m->srowind = nullptr;
   m->sval = m->ypart = nullptr;
   m->sval32 = nullptr;
@@ -492,13 +492,13 @@ static int gb_split_build(gbd_matrix *m, uint32_t P) {
   GB_HIP(gb_dev_alloc((void **)&m->srowind, (m->nnz ? m->nnz : 1) * 4));
   GB_HIP(gb_dev_alloc((void **)&m->sval, (m->nnz ? m->nnz : 1) * 8));
   GB_HIP(gb_dev_alloc((void **)&m->ypart, tot * 8));
-  GB_HIP(hipMallocAsync((void **)&cnt, tot * 4, s));
+  GB_HIP(gb_dev_alloc((void **)&cnt, tot * 4));
   const uint32_t g = gb_div_up(n, GB_BLOCK);
   gb_split_count<<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, n, width, P, cnt);
   GB_LAUNCH_CHECK();
   int rv = gbd_scan_u32(cnt, m->srowptr, tot, m->srowptr + tot);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  GB_HIP(hipFreeAsync(cnt, s));
+  GB_HIP(gb_dev_free(cnt));
   gb_split_fill<double><<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->val, n, width, P,
                                                m->srowptr, m->srowind, m->sval);
   GB_LAUNCH_CHECK();
@@ -546,11 +546,11 @@ static int gb_plan_build(gbd_matrix *m) {
 
   uint32_t *cls = nullptr, *flag = nullptr, *idx = nullptr, *dist = nullptr;
   unsigned long long *hist = nullptr;
-  GB_HIP(hipMallocAsync((void **)&cls, (size_t)n * 4, s));
-  GB_HIP(hipMallocAsync((void **)&flag, (size_t)n * 4, s));
-  GB_HIP(hipMallocAsync((void **)&idx, ((size_t)n + 1) * 4, s));
-  GB_HIP(hipMallocAsync((void **)&dist, 4 * 4, s));
-  GB_HIP(hipMallocAsync((void **)&hist, 33 * 8, s));
+  GB_HIP(gb_dev_alloc((void **)&cls, (size_t)n * 4));
+  GB_HIP(gb_dev_alloc((void **)&flag, (size_t)n * 4));
+  GB_HIP(gb_dev_alloc((void **)&idx, ((size_t)n + 1) * 4));
+  GB_HIP(gb_dev_alloc((void **)&dist, 4 * 4));
+  GB_HIP(gb_dev_alloc((void **)&hist, 33 * 8));
   GB_HIP(hipMemsetAsync(hist, 0, 33 * 8, s));
   gb_plan_classify<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(
       m->rowptr, n, m->params, cls, flag, hist);
@@ -576,11 +576,11 @@ static int gb_plan_build(gbd_matrix *m) {
                                                               n, m->blk);
     GB_LAUNCH_CHECK();
   }
-  GB_HIP(hipFreeAsync(cls, s));
-  GB_HIP(hipFreeAsync(flag, s));
-  GB_HIP(hipFreeAsync(idx, s));
-  GB_HIP(hipFreeAsync(dist, s));
-  GB_HIP(hipFreeAsync(hist, s));
+  GB_HIP(gb_dev_free(cls));
+  GB_HIP(gb_dev_free(flag));
+  GB_HIP(gb_dev_free(idx));
+  GB_HIP(gb_dev_free(dist));
+  GB_HIP(gb_dev_free(hist));
   if (!m->view && m->params.csplit > 1) {
     int rv2 = gb_split_build(m, m->params.csplit);
     if (rv2 != GOOSEBERRY_SUCCESS) {
@@ -617,7 +617,7 @@ int gbd_csr_check(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
   uint32_t *bad = nullptr, hbad = 0, nnz = 0;
   GB_HIP(hipMemcpyAsync(&nnz, d_rowptr + nrows, 4, hipMemcpyDeviceToHost, s));
   GB_HIP(hipStreamSynchronize(s));
-  GB_HIP(hipMallocAsync((void **)&bad, 4, s));
+  GB_HIP(gb_dev_alloc((void **)&bad, 4));
   GB_HIP(hipMemsetAsync(bad, 0, 4, s));
   uint64_t work = (uint64_t)nrows > nnz ? nrows : nnz;
   uint32_t g = gb_div_up(work ? work : 1, GB_BLOCK);
@@ -625,7 +625,7 @@ int gbd_csr_check(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
       d_rowptr, d_rowind, nrows, ncols, nnz, bad);
   GB_LAUNCH_CHECK();
   GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
-  GB_HIP(hipFreeAsync(bad, s));
+  GB_HIP(gb_dev_free(bad));
   GB_HIP(hipStreamSynchronize(s));
   if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
   *r_nnz = nnz;
@@ -695,18 +695,18 @@ void gbd_matrix_destroy(gbd_matrix *m) {
   if (!m) return;
   // pool frees are ordered after the queued work that reads the arrays
   if (m->view) {
-    gb_dev_free(m->blk);
-    gb_dev_free(m->rowbins);
+    (void)gb_dev_free(m->blk);
+    (void)gb_dev_free(m->rowbins);
     free(m);
     return;
   }
   gb_split_free(m);
-  if (m->rowptr) gb_dev_free(m->rowptr);
-  if (m->rowind) gb_dev_free(m->rowind);
-  if (m->val) gb_dev_free(m->val);
-  if (m->val32) gb_dev_free(m->val32);
-  if (m->blk) gb_dev_free(m->blk);
-  if (m->rowbins) gb_dev_free(m->rowbins);
+  if (m->rowptr) (void)gb_dev_free(m->rowptr);
+  if (m->rowind) (void)gb_dev_free(m->rowind);
+  if (m->val) (void)gb_dev_free(m->val);
+  if (m->val32) (void)gb_dev_free(m->val32);
+  if (m->blk) (void)gb_dev_free(m->blk);
+  if (m->rowbins) (void)gb_dev_free(m->rowbins);
   free(m);
 }
 
