@@ -69,7 +69,7 @@ static void gb_do_init() {
 
 // Large blocks (4 MiB and up, sizes rounded to 2 MiB) that the library
 // frees are kept here and handed back to a later allocation of the same
-// rounded size.  The default pool gave such blocks back to the driver at
+// rounded size, the most recently freed first.  The default pool gave such blocks back to the driver at
 // free despite its release threshold: a free of a few hundred MiB took
 // 0.65-2.5 ms and the next allocation of that size 50-350 us, several
 // milliseconds per sparse product.  Every library launch and copy runs on
@@ -115,8 +115,11 @@ hipError_t gb_dev_alloc(void **p, size_t bytes) {
   const size_t r = (bytes + GB_CACHE_ROUND - 1) & ~(GB_CACHE_ROUND - 1);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    auto it = g_cache_free.find(r);
-    if (it != g_cache_free.end()) {
+    // the block of this size freed last (its lines are the likeliest to
+    // be in the caches still)
+    auto range = g_cache_free.equal_range(r);
+    if (range.first != range.second) {
+      auto it = std::prev(range.second);
       *p = it->second->second;
       g_cache_fifo.erase(it->second);
       g_cache_free.erase(it);
