@@ -26,15 +26,33 @@
 // Both paths add a column's products one after another in t order: the
 // first product, then += the others (the dense variant starts from +0.0 as
 // the reference's zeroed row does, so a lone -0.0 becomes +0.0).  The sparse
-// result is built in two passes: counts per row (their scan gives the row
-// pointers), then the entries.
+// result needs each row's entry count before the rows can be placed: the
+// short rows write their entries in first-appearance order to a staging
+// area at the row's first product position, with their counts; the scan of
+// the counts gives the row pointers and a copy puts the staged rows in
+// place.  (When the staging area would pass its limit, 8 GiB unless
+// GOOSEBERRY_SPGEMM_STAGE_MAX says otherwise, the short rows are counted in
+// one pass and written in another instead.)
 #include <stdlib.h>
 
 #include "gb_common.h"
 
 #define GB_SG_SHORT 64  // products (and A entries) of a wavefront row
 
-enum { GB_SG_COUNT = 0, GB_SG_FILL = 1, GB_SG_DENSE = 2 };
+enum { GB_SG_COUNT = 0, GB_SG_FILL = 1, GB_SG_DENSE = 2, GB_SG_STAGE = 3 };
+#define GB_SG_STAGE_MAX (8ull << 30)  // bytes of staged short-row entries
+
+// the staging limit: GOOSEBERRY_SPGEMM_STAGE_MAX (bytes; 0 turns staging
+// off) when set, read at every product
+static uint64_t gb_sg_stage_max() {
+  const char *e = getenv("GOOSEBERRY_SPGEMM_STAGE_MAX");
+  if (e && *e) {
+    char *end = nullptr;
+    const unsigned long long v = strtoull(e, &end, 10);
+    if (end != e) return v;
+  }
+  return GB_SG_STAGE_MAX;
+}
 
 // products per A entry: the length of the B row it selects
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -103,7 +121,8 @@ gb_sg_wave(uint32_t nlist, const uint32_t *__restrict__ list,
            const uint32_t *__restrict__ browind, const double *__restrict__ browval,
            const uint32_t *__restrict__ crowptr, uint32_t *__restrict__ cnt,
            uint32_t *__restrict__ crowind, double *__restrict__ crowval,
-           double *__restrict__ cdense, uint32_t bncols) {
+           double *__restrict__ cdense, uint32_t bncols,
+           const uint32_t *__restrict__ pbase) {
   __shared__ uint32_t hkey[GB_WAVES_PER_BLOCK][GB_SG_HASH];
   __shared__ unsigned long long hmask[GB_WAVES_PER_BLOCK][GB_SG_HASH];
   __shared__ double hval[GB_WAVES_PER_BLOCK][GB_WAVE];
@@ -182,17 +201,63 @@ gb_sg_wave(uint32_t nlist, const uint32_t *__restrict__ list,
     }
   }
   const uint64_t heads = __ballot(head);
-  if (MODE == GB_SG_COUNT) {
+  if (MODE == GB_SG_COUNT || MODE == GB_SG_STAGE) {
     if (lane == 0) cnt[i] = (uint32_t)__popcll(heads);
-  } else if (head) {
-    if (MODE == GB_SG_FILL) {
+  }
+  if (MODE != GB_SG_COUNT && head) {
+    if (MODE == GB_SG_FILL || MODE == GB_SG_STAGE) {
+      // STAGE: the row's staging slots start at its first product position
+      // (pbase[j0]) and it has at most P <= its product count entries
       const uint32_t rank = (uint32_t)__popcll(heads & ((1ull << lane) - 1ull));
-      const uint32_t o = crowptr[i] + rank;
+      const uint32_t o = (MODE == GB_SG_FILL ? crowptr[i] : pbase[j0]) + rank;
       crowind[o] = col;
       crowval[o] = sum;
     } else {
       cdense[(size_t)i * bncols + col] = sum;
     }
+  }
+}
+
+// the staged short rows to their places, GB_BLOCK list rows per workgroup:
+// row i's cnt[i] entries from its staging slots (pbase[arowptr[i]] on) to
+// crowptr[i] on.  The workgroup numbers its rows' entries k = 0, 1, ...
+// (a scan of the counts); thread t copies entries t, t + GB_BLOCK, ...,
+// finding each one's row by binary search over the scan, so consecutive
+// entries go to consecutive places.
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_sg_unstage(uint32_t nlist, const uint32_t *__restrict__ list,
+              const uint32_t *__restrict__ arowptr, const uint32_t *__restrict__ pbase,
+              const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ sind,
+              const double *__restrict__ sval, const uint32_t *__restrict__ crowptr,
+              uint32_t *__restrict__ crowind, double *__restrict__ crowval) {
+  __shared__ uint32_t lds4[GB_WAVES_PER_BLOCK];
+  __shared__ uint32_t first[GB_BLOCK + 1];  // block entry number of each row's first
+  __shared__ uint32_t src[GB_BLOCK], dst[GB_BLOCK];
+  const uint32_t w = blockIdx.x * GB_BLOCK + threadIdx.x;
+  uint32_t c = 0;
+  if (w < nlist) {
+    const uint32_t i = list[w];
+    c = cnt[i];
+    src[threadIdx.x] = pbase[arowptr[i]];
+    dst[threadIdx.x] = crowptr[i];
+  }
+  uint32_t total = 0;
+  const uint32_t e = gb_block_scan_excl(c, lds4, &total);
+  first[threadIdx.x] = e;
+  if (threadIdx.x == 0) first[GB_BLOCK] = total;
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < total; k += GB_BLOCK) {
+    // the last row r with first[r] <= k (rows without entries share their
+    // first with the next row, and the search skips them)
+    uint32_t lo = 0, hi = GB_BLOCK - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (first[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t d = k - first[lo];
+    crowind[dst[lo] + d] = sind[src[lo] + d];
+    crowval[dst[lo] + d] = sval[src[lo] + d];
   }
 }
 
@@ -372,9 +437,10 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
            *ecol = nullptr, *perm1 = nullptr, *row1 = nullptr, *ord = nullptr,
            *flag = nullptr, *rid = nullptr, *runstart = nullptr, *rrow = nullptr,
            *rcol = nullptr, *rfirst = nullptr, *ordr = nullptr, *rbeg = nullptr,
-           *cnt = nullptr, *crowptr = nullptr, *crowind = nullptr;
-  double *eval = nullptr, *rval = nullptr, *crowval = nullptr;
+           *cnt = nullptr, *crowptr = nullptr, *crowind = nullptr, *sind = nullptr;
+  double *eval = nullptr, *rval = nullptr, *crowval = nullptr, *sval = nullptr;
   unsigned long long e64 = 0, *tot = nullptr;
+  bool staged = false;
   hipError_t he = hipSuccess;
 #define GB_SG_TRY(call)                                         \
   do {                                                          \
@@ -440,16 +506,24 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
   }
   if (!dense) {
     GB_SG_RV(gbd_zero(cnt, ((size_t)anrows + 1) * 4));
-    if (nshort) {
+    staged = e64 * 12 <= gb_sg_stage_max();
+    if (nshort && staged) {
+      GB_SG_TRY(gb_dev_alloc((void **)&sind, (size_t)e64 * 4));
+      GB_SG_TRY(gb_dev_alloc((void **)&sval, (size_t)e64 * 8));
+      gb_sg_wave<GB_SG_STAGE><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
+          nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
+          nullptr, cnt, sind, sval, nullptr, 0, off);
+      GB_SG_TRY(hipGetLastError());
+    } else if (nshort) {
       gb_sg_wave<GB_SG_COUNT><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
           nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
-          nullptr, cnt, nullptr, nullptr, nullptr, 0);
+          nullptr, cnt, nullptr, nullptr, nullptr, 0, nullptr);
       GB_SG_TRY(hipGetLastError());
     }
   } else if (nshort) {
     gb_sg_wave<GB_SG_DENSE><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
         nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
-        nullptr, nullptr, nullptr, nullptr, d_cdense, bncols);
+        nullptr, nullptr, nullptr, nullptr, d_cdense, bncols, nullptr);
     GB_SG_TRY(hipGetLastError());
   }
   // the sort path's products, numbered over its rows
@@ -523,10 +597,14 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
     GB_SG_TRY(hipStreamSynchronize(s));
     GB_SG_TRY(gb_dev_alloc((void **)&crowind, ((size_t)cnnz + 1) * 4));
     GB_SG_TRY(gb_dev_alloc((void **)&crowval, ((size_t)cnnz + 1) * 8));
-    if (nshort) {
+    if (nshort && staged) {
+      gb_sg_unstage<<<gb_div_up(nshort, GB_BLOCK), GB_BLOCK, 0, s>>>(
+          nshort, slist, d_arowptr, off, cnt, sind, sval, crowptr, crowind, crowval);
+      GB_SG_TRY(hipGetLastError());
+    } else if (nshort) {
       gb_sg_wave<GB_SG_FILL><<<gb_div_up(nshort, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(
           nshort, slist, d_arowptr, d_arowind, d_arowval, d_browptr, d_browind, d_browval,
-          crowptr, nullptr, crowind, crowval, nullptr, 0);
+          crowptr, nullptr, crowind, crowval, nullptr, 0, nullptr);
       GB_SG_TRY(hipGetLastError());
     }
     if (nruns) {
@@ -544,13 +622,12 @@ int gbd_spgemm(uint32_t anrows, uint32_t ancols, uint32_t bncols,
   GB_SG_TRY(hipStreamSynchronize(s));
   rv = GOOSEBERRY_SUCCESS;
 done:
-  (void)gb_dev_free(len); (void)gb_dev_free(off); (void)gb_dev_free(isshort); (void)gb_dev_free(islong);
-  (void)gb_dev_free(slist); (void)gb_dev_free(len2); (void)gb_dev_free(off2); (void)gb_dev_free(erow);
-  (void)gb_dev_free(ecol); (void)gb_dev_free(eval); (void)gb_dev_free(perm1); (void)gb_dev_free(row1);
-  (void)gb_dev_free(ord); (void)gb_dev_free(flag); (void)gb_dev_free(rid); (void)gb_dev_free(runstart);
-  (void)gb_dev_free(rrow); (void)gb_dev_free(rcol); (void)gb_dev_free(rfirst); (void)gb_dev_free(rval);
-  (void)gb_dev_free(ordr); (void)gb_dev_free(rbeg); (void)gb_dev_free(cnt); (void)gb_dev_free(crowptr);
-  (void)gb_dev_free(crowind); (void)gb_dev_free(crowval); (void)gb_dev_free(tot);
+  {
+    void *tmp[] = {len, off, isshort, islong, slist, len2, off2, erow, ecol, eval,
+                   perm1, row1, ord, flag, rid, runstart, rrow, rcol, rfirst, rval,
+                   ordr, rbeg, cnt, crowptr, crowind, crowval, tot, sind, sval};
+    for (void *q : tmp) (void)gb_dev_free(q);
+  }
   return rv;
 #undef GB_SG_TRY
 #undef GB_SG_RV

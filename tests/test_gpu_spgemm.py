@@ -28,10 +28,15 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("stage", ["staged", "two_pass"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%dx%d" % c[:3])
-def test_spgemm_bitwise_with_oracle(gb, oracle, case):
+def test_spgemm_bitwise_with_oracle(gb, oracle, case, stage, monkeypatch):
+    # short rows staged in one pass (the default), or counted and then
+    # written (GOOSEBERRY_SPGEMM_STAGE_MAX=0, the path past the staging limit)
+    if stage == "two_pass":
+        monkeypatch.setenv("GOOSEBERRY_SPGEMM_STAGE_MAX", "0")
     anr, anc, bnc, da, db, seed, dups, emp = case
-    print("spgemm case", case, flush=True)
+    print("spgemm case", case, stage, flush=True)
     a = random_csr(anr, anc, da, seed, empty_every=emp, dups=dups)
     b = random_csr(anc, bnc, db, seed + 100, empty_every=emp, dups=dups)
     rp, ri, rv = gb.spmultsp_sp(anr, anc, bnc, a, b)
