@@ -168,12 +168,14 @@ gb_cm_placelist(uint32_t L, uint32_t p0, uint32_t start, const uint32_t *__restr
 // LDS gives the exact order; the vertex is S[srank]
 #define GB_CM_LDS_SORT 8192
 #define GB_CM_SORT_THREADS 1024
-__global__ void __launch_bounds__(GB_CM_SORT_THREADS)
-gb_cm_sortplace(uint32_t L, uint32_t b0, uint32_t p0, uint32_t start,
-                const uint32_t *__restrict__ list, const uint32_t *__restrict__ claim,
-                const uint32_t *__restrict__ srank, const uint32_t *__restrict__ S,
-                uint32_t *__restrict__ posof, uint32_t *__restrict__ perm) {
-  __shared__ uint64_t key[GB_CM_LDS_SORT];
+
+__device__ void gb_cm_bitonic_place(uint32_t L, uint32_t b0, uint32_t p0, uint32_t start,
+                                    const uint32_t *__restrict__ list,
+                                    const uint32_t *__restrict__ claim,
+                                    const uint32_t *__restrict__ srank,
+                                    const uint32_t *__restrict__ S,
+                                    uint32_t *__restrict__ posof, uint32_t *__restrict__ perm,
+                                    uint64_t *key) {
   uint32_t P2 = 1;
   while (P2 < L) P2 <<= 1;
   for (uint32_t t = threadIdx.x; t < P2; t += GB_CM_SORT_THREADS) {
@@ -203,6 +205,88 @@ gb_cm_sortplace(uint32_t L, uint32_t b0, uint32_t p0, uint32_t start,
   }
   for (uint32_t t = threadIdx.x; t < L; t += GB_CM_SORT_THREADS) {
     const uint32_t v = S[(uint32_t)key[t]];
+    posof[v] = p0 + t;
+    perm[p0 + t] = start + v;
+  }
+}
+
+__global__ void __launch_bounds__(GB_CM_SORT_THREADS)
+gb_cm_sortplace(uint32_t L, uint32_t b0, uint32_t p0, uint32_t start,
+                const uint32_t *__restrict__ list, const uint32_t *__restrict__ claim,
+                const uint32_t *__restrict__ srank, const uint32_t *__restrict__ S,
+                uint32_t *__restrict__ posof, uint32_t *__restrict__ perm) {
+  __shared__ uint64_t key[GB_CM_LDS_SORT];
+  gb_cm_bitonic_place(L, b0, p0, start, list, claim, srank, S, posof, perm, key);
+}
+
+// A step's placement by buckets, for at most GB_CM_LDS_SORT parents
+// [b0, b1) and new vertices, all in LDS: each new vertex goes to the bucket
+// of its claiming position (counts by LDS atomics, their scan, a scatter),
+// and each bucket is put in srank order by one thread (insertion sort).
+// Parents keep their order and siblings go by srank: the (claiming
+// position, srank) order of the sort.  A bucket of more than
+// GB_CM_SEG_BUCKET vertices sends the step to the sort instead.
+#define GB_CM_SEG_BUCKET 32
+__global__ void __launch_bounds__(GB_CM_SORT_THREADS)
+gb_cm_segplace(uint32_t L, uint32_t b0, uint32_t b1, uint32_t p0, uint32_t start,
+               const uint32_t *__restrict__ list, const uint32_t *__restrict__ claim,
+               const uint32_t *__restrict__ srank, const uint32_t *__restrict__ S,
+               uint32_t *__restrict__ posof, uint32_t *__restrict__ perm) {
+  __shared__ uint64_t key[GB_CM_LDS_SORT];  // the sort's keys, if it is needed
+  // per bucket: its count, then its first slot, then (slots taken) its end
+  __shared__ uint32_t base[GB_CM_LDS_SORT];
+  __shared__ uint32_t out[GB_CM_LDS_SORT];  // sranks in bucket order
+  __shared__ uint32_t big;
+  const uint32_t np = b1 - b0;
+  for (uint32_t q = threadIdx.x; q < np; q += GB_CM_SORT_THREADS) base[q] = 0;
+  if (threadIdx.x == 0) big = 0;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < L; t += GB_CM_SORT_THREADS) {
+    const uint32_t v = list[t];
+    const uint32_t c = claim[v] - b0;  // < np: v was claimed in this step
+    const uint32_t r = srank[v];
+    key[t] = ((uint64_t)c << 32) | r;
+    const uint32_t had = atomicAdd(&base[c], 1u);
+    if (had == GB_CM_SEG_BUCKET) big = 1;
+  }
+  __syncthreads();
+  if (big) {  // uniform: read after the barrier
+    gb_cm_bitonic_place(L, b0, p0, start, list, claim, srank, S, posof, perm, key);
+    return;
+  }
+  // exclusive scan of the bucket counts by the first wavefront
+  if (threadIdx.x < GB_WAVE) {
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < np; c0 += GB_WAVE) {
+      const uint32_t q = c0 + threadIdx.x;
+      const uint32_t n = q < np ? base[q] : 0u;
+      const uint32_t inc = gb_wave_scan_incl(n);
+      if (q < np) base[q] = carry + inc - n;
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, GB_WAVE - 1);
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < L; t += GB_CM_SORT_THREADS) {
+    const uint64_t k = key[t];
+    const uint32_t c = (uint32_t)(k >> 32);
+    out[atomicAdd(&base[c], 1u)] = (uint32_t)k;
+  }
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < np; q += GB_CM_SORT_THREADS) {
+    const uint32_t a = q ? base[q - 1] : 0u, n = base[q] - a;
+    for (uint32_t x = 1; x < n; ++x) {  // at most GB_CM_SEG_BUCKET
+      const uint32_t r = out[a + x];
+      uint32_t y = x;
+      while (y > 0 && out[a + y - 1] > r) {
+        out[a + y] = out[a + y - 1];
+        --y;
+      }
+      out[a + y] = r;
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < L; t += GB_CM_SORT_THREADS) {
+    const uint32_t v = S[out[t]];
     posof[v] = p0 + t;
     perm[p0 + t] = start + v;
   }
@@ -408,7 +492,11 @@ static int gb_cm_block(uint32_t start, uint32_t end, const uint32_t *rowptr,
       uint32_t L = 0;
       GB_CM_TRY(hipMemcpyAsync(&L, dv + 1, 4, hipMemcpyDeviceToHost, s));
       GB_CM_TRY(hipStreamSynchronize(s));
-      if (L && L <= GB_CM_LDS_SORT) {
+      if (L && L <= GB_CM_LDS_SORT && no - sr <= GB_CM_LDS_SORT) {
+        gb_cm_segplace<<<1, GB_CM_SORT_THREADS, 0, s>>>(L, sr, no, no, start, list, claim,
+                                                       srank, S, posof, perm);
+        GB_CM_TRY(hipGetLastError());
+      } else if (L && L <= GB_CM_LDS_SORT) {
         gb_cm_sortplace<<<1, GB_CM_SORT_THREADS, 0, s>>>(L, sr, no, start, list, claim, srank,
                                                         S, posof, perm);
         GB_CM_TRY(hipGetLastError());
