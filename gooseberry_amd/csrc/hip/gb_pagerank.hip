@@ -30,15 +30,54 @@
 // bound, this build stops after this many iterations.
 #define GB_PR_MAX_ITER 1000000ull
 
+// Column sums in row order, s = ((0 + v0) + v1) + ..., one dependent add
+// after another as the reference's indeg[col] += val loop.  Columns of up to
+// GB_PR_SHORT entries take one thread each, the loads issued eight at a time
+// ahead of their adds; longer ones take a wavefront each (below), so a long
+// column's loads do not serialise behind its adds.
+#define GB_PR_SHORT 64
+
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_pr_colsum(uint32_t ncols, const uint32_t *__restrict__ dist,
              const uint32_t *__restrict__ perm, const double *__restrict__ val,
              double *__restrict__ indeg) {
   const uint32_t c = blockIdx.x * GB_BLOCK + threadIdx.x;
   if (c >= ncols) return;
+  const uint32_t p0 = dist[c], p1 = dist[c + 1];
+  if (p1 - p0 > GB_PR_SHORT) return;
   double s = 0.0;
-  for (uint32_t p = dist[c]; p < dist[c + 1]; ++p) s += val[perm[p]];
+  for (uint32_t p = p0; p < p1; p += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p + u < p1 ? val[perm[p + u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (p + u < p1) s += v[u];
+  }
   indeg[c] = s;
+}
+
+// the columns longer than GB_PR_SHORT: one wavefront each, 64 values loaded
+// at once (the next 64 while the current ones are added), added in order
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_colsum_long(uint32_t ncols, const uint32_t *__restrict__ dist,
+                  const uint32_t *__restrict__ perm, const double *__restrict__ val,
+                  double *__restrict__ indeg) {
+  const uint32_t c = blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE;
+  const uint32_t lane = threadIdx.x & (GB_WAVE - 1);
+  if (c >= ncols) return;
+  const uint32_t p0 = dist[c], p1 = dist[c + 1];
+  if (p1 - p0 <= GB_PR_SHORT) return;
+  double s = 0.0;
+  double cur = p0 + lane < p1 ? val[perm[p0 + lane]] : 0.0;
+  for (uint32_t p = p0; p < p1; p += GB_WAVE) {
+    const uint32_t q = p + GB_WAVE + lane;
+    const double next = q < p1 ? val[perm[q]] : 0.0;
+    const uint32_t m = p1 - p < GB_WAVE ? p1 - p : GB_WAVE;
+    for (uint32_t u = 0; u < m; ++u) s += __shfl(cur, (int)u, GB_WAVE);
+    cur = next;
+  }
+  if (lane == 0) indeg[c] = s;
 }
 
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -185,6 +224,9 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   // column sums in row order (column indices were validated with the matrix)
   GB_PR_RV(gbd_countingsort_kv_unchecked(m->rowind, nullptr, 0, n - 1, nnz, perm, dist));
   gb_pr_colsum<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm, m->val, indeg);
+  GB_PR_TRY(hipGetLastError());
+  gb_pr_colsum_long<<<gb_div_up(n, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm,
+                                                                         m->val, indeg);
   GB_PR_TRY(hipGetLastError());
   if (nnz) {
     gb_pr_normalise<<<gb_pr_grid(nnz), GB_BLOCK, 0, s>>>(nnz, m->rowind, m->val, indeg,

@@ -60,6 +60,30 @@ def test_pagerank_long_rows_and_many_sinks(gb, oracle):
     assert np.max(np.abs(x - xo)) <= 1e-13 * np.max(np.abs(xo))
 
 
+def test_pagerank_long_columns_bitwise(gb, oracle):
+    # two hub columns with thousands of entries (summed by one wavefront
+    # each, in row order) among short columns; rows stay short, so the
+    # written vector is still bitwise the oracle's
+    n = 4000
+    rng = np.random.default_rng(12)
+    rows = []
+    for i in range(n):
+        cols = set(rng.choice(n, size=4, replace=False).tolist())
+        cols.add(7)
+        if i % 2:
+            cols.add(1234)
+        rows.append(sorted(cols))
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ri = np.array([c for r in rows for c in r], dtype=np.uint32)
+    rv = rng.uniform(0.1, 3.0, ri.size)
+    assert np.bincount(ri)[[7, 1234]].min() > 64
+    x, it, err, ns = gb.Matrix(rp, ri, rv).pagerank(0.85, 0, 1e-10)
+    xo, ito, erro, nso = oracle_pr(oracle, rp, ri, rv, n, 0.85, 0, 1e-10)
+    assert ns == nso == 0 and it == ito
+    assert np.array_equal(x, xo)
+
+
 @pytest.mark.parametrize("csplit", [2, 5])
 def test_pagerank_column_split_plan(gb, oracle, csplit):
     # the iteration through column slices: each row summed per slice, then
