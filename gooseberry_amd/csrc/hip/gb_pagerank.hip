@@ -74,7 +74,18 @@ gb_pr_colsum_long(uint32_t ncols, const uint32_t *__restrict__ dist,
     const uint32_t q = p + GB_WAVE + lane;
     const double next = q < p1 ? val[perm[q]] : 0.0;
     const uint32_t m = p1 - p < GB_WAVE ? p1 - p : GB_WAVE;
-    for (uint32_t u = 0; u < m; ++u) s += __shfl(cur, (int)u, GB_WAVE);
+    // lane u's value read into scalar registers: the reads do not wait on
+    // the adds, so only the adds form the chain
+    const long long bits = __double_as_longlong(cur);
+    const int lo = (int)(uint32_t)bits, hi = (int)(bits >> 32);
+#pragma unroll
+    for (int u = 0; u < GB_WAVE; ++u) {
+      if ((uint32_t)u < m) {
+        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane(lo, u);
+        const uint32_t h = (uint32_t)__builtin_amdgcn_readlane(hi, u);
+        s += __longlong_as_double((long long)(((uint64_t)h << 32) | l));
+      }
+    }
     cur = next;
   }
   if (lane == 0) indeg[c] = s;
