@@ -88,94 +88,134 @@ static int gb_scan_u32(const uint32_t *d_in, uint32_t *d_out, size_t n,
 
 // ------------------------------------------------------ counting sort
 
-#define CS_ITEMS 16
-#define CS_CHUNK (GB_WAVE * CS_ITEMS)
-#define CS_RADIX 256
+#define CS_RADIX 256                  // 8-bit digits
+#define CS_ITEMS 16                   // keys per thread
+#define CS_TILE (GB_BLOCK * CS_ITEMS)  // keys per workgroup tile (4096)
+static_assert(GB_BLOCK == CS_RADIX, "one thread per digit in the sort's kernels");
 
 __device__ __forceinline__ uint32_t cs_digit(uint32_t k, uint32_t kmin,
                                              int shift) {
   return ((k - kmin) >> shift) & (CS_RADIX - 1);
 }
 
-// One wavefront per chunk of CS_CHUNK consecutive elements; counts are
-// stored digit-major (counts[d * nch + chunk]) so that one exclusive scan
-// gives every (digit, chunk) its output offset in stable order.
-__global__ void __launch_bounds__(GB_WAVE)
+// Digit counts of each tile of CS_TILE consecutive keys, stored digit-major
+// (counts[d * ntiles + tile]) so that one exclusive scan gives every
+// (digit, tile) its first output position in stable order.
+__global__ void __launch_bounds__(GB_BLOCK)
 gb_cs_count(const uint32_t *__restrict__ keys, size_t n, uint32_t kmin,
-            int shift, uint32_t *__restrict__ counts, uint32_t nch) {
+            int shift, uint32_t *__restrict__ counts, uint32_t ntiles) {
   __shared__ uint32_t cnt[CS_RADIX];
-  for (int d = threadIdx.x; d < CS_RADIX; d += GB_WAVE) cnt[d] = 0;
+  cnt[threadIdx.x] = 0;  // GB_BLOCK == CS_RADIX
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * CS_CHUNK;
+  const size_t base = (size_t)blockIdx.x * CS_TILE;
+#pragma unroll
   for (int i = 0; i < CS_ITEMS; ++i) {
-    size_t k = base + (size_t)i * GB_WAVE + threadIdx.x;
+    const size_t k = base + (size_t)i * GB_BLOCK + threadIdx.x;
     if (k < n) atomicAdd(&cnt[cs_digit(keys[k], kmin, shift)], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < CS_RADIX; d += GB_WAVE)
-    counts[(size_t)d * nch + blockIdx.x] = cnt[d];
+  counts[(size_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(GB_WAVE)
+// One tile per workgroup.  Key i of thread t is key base + i * GB_BLOCK + t,
+// so the keys in (i, t) order are the tile's keys in order.  Step i ranks
+// its GB_BLOCK keys: within a wavefront by the mask of lanes with the same
+// digit (eight ballots), across wavefronts by the per-wave digit counts in
+// LDS, across steps by each digit's running count.  A scan of the tile's
+// digit counts then places the keys in LDS in (digit, order) order, and
+// thread t writes the LDS keys t, t + GB_BLOCK, ... to their places: a
+// digit's keys of the tile go to consecutive places, so the writes come in
+// runs (CS_TILE / CS_RADIX = 16 keys on average) instead of one place per
+// key.
+__global__ void __launch_bounds__(GB_BLOCK)
 gb_cs_scatter(const uint32_t *__restrict__ keys,
               const uint32_t *__restrict__ vals, size_t n, uint32_t kmin,
-              int shift, const uint32_t *__restrict__ offs, uint32_t nch,
+              int shift, const uint32_t *__restrict__ offs, uint32_t ntiles,
               uint32_t *__restrict__ okeys, uint32_t *__restrict__ ovals) {
-  __shared__ uint32_t base_of[CS_RADIX];
-  for (int d = threadIdx.x; d < CS_RADIX; d += GB_WAVE)
-    base_of[d] = offs[(size_t)d * nch + blockIdx.x];
-  __syncthreads();
-  const int lane = threadIdx.x;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const size_t base = (size_t)blockIdx.x * CS_CHUNK;
+  __shared__ uint32_t wcnt[GB_WAVES_PER_BLOCK][CS_RADIX];
+  __shared__ uint32_t run[CS_RADIX];     // keys of each digit ranked so far
+  __shared__ uint32_t dstart[CS_RADIX];  // first LDS place of each digit
+  __shared__ uint32_t gbase[CS_RADIX];   // first output place of each digit
+  __shared__ uint32_t skey[CS_TILE], sval[CS_TILE];
+  __shared__ uint32_t lds4[GB_WAVES_PER_BLOCK];
+  const uint32_t tid = threadIdx.x;
+  const int lane = tid & (GB_WAVE - 1);
+  const int wave = tid / GB_WAVE;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  gbase[tid] = offs[(size_t)tid * ntiles + blockIdx.x];
+  run[tid] = 0;
+  const size_t base = (size_t)blockIdx.x * CS_TILE;
+  uint32_t key[CS_ITEMS], val[CS_ITEMS], pos[CS_ITEMS];
+#pragma unroll
   for (int i = 0; i < CS_ITEMS; ++i) {
-    size_t k = base + (size_t)i * GB_WAVE + lane;
-    const bool valid = k < n;
-    uint32_t key = valid ? keys[k] : kmin;
-    uint32_t val = valid ? (vals ? vals[k] : (uint32_t)k) : 0u;
-    uint32_t d = cs_digit(key, kmin, shift);
+    const size_t k = base + (size_t)i * GB_BLOCK + tid;
+    key[i] = k < n ? keys[k] : kmin;
+    val[i] = k < n ? (vals ? vals[k] : (uint32_t)k) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < CS_ITEMS; ++i) {
+#pragma unroll
+    for (int w = 0; w < GB_WAVES_PER_BLOCK; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    const bool valid = base + (size_t)i * GB_BLOCK + tid < n;
+    const uint32_t d = cs_digit(key[i], kmin, shift);
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
-      uint64_t b = __ballot((d >> bit) & 1u);
-      peers &= ((d >> bit) & 1u) ? b : ~b;
+      const uint64_t bb = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? bb : ~bb;
     }
-    uint32_t pos = 0;
+    if (valid && (63 - __clzll(peers)) == lane) wcnt[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
     if (valid) {
-      pos = base_of[d] + (uint32_t)__popcll(peers & lt);
-      okeys[pos] = key;
-      ovals[pos] = val;
+      uint32_t o = run[d];
+      for (int w = 0; w < wave; ++w) o += wcnt[w][d];
+      pos[i] = o + (uint32_t)__popcll(peers & lt);
     }
     __syncthreads();
-    if (valid && (63 - __clzll(peers)) == lane)
-      base_of[d] += (uint32_t)__popcll(peers);
-    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < GB_WAVES_PER_BLOCK; ++w) add += wcnt[w][tid];
+    run[tid] += add;
+  }
+  __syncthreads();
+  uint32_t total = 0;
+  dstart[tid] = gb_block_scan_excl(run[tid], lds4, &total);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < CS_ITEMS; ++i) {
+    if (base + (size_t)i * GB_BLOCK + tid < n) {
+      const uint32_t p = dstart[cs_digit(key[i], kmin, shift)] + pos[i];
+      skey[p] = key[i];
+      sval[p] = val[i];
+    }
+  }
+  __syncthreads();
+  const uint32_t tn = n - base < (size_t)CS_TILE ? (uint32_t)(n - base) : (uint32_t)CS_TILE;
+  for (uint32_t j = tid; j < tn; j += GB_BLOCK) {
+    const uint32_t k = skey[j];
+    const uint32_t d = cs_digit(k, kmin, shift);
+    const uint32_t o = gbase[d] + (j - dstart[d]);
+    okeys[o] = k;
+    ovals[o] = sval[j];
   }
 }
 
-__global__ void gb_cs_hist(const uint32_t *__restrict__ keys, size_t n,
-                           uint32_t kmin, uint32_t *__restrict__ hist) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) atomicAdd(&hist[keys[i] - kmin], 1u);
-}
-
-// Few keys: count in LDS per workgroup, then one global add per key, so
-// that millions of increments of a handful of counters do not serialise
-// on the same global addresses.
-#define CS_HIST_LDS 4096
+// dist[k] = the number of sorted keys below kmin + k, for k in [0, range]
+// (range = kmax - kmin + 1): a lower bound in the sorted keys per k
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_cs_hist_lds(const uint32_t *__restrict__ keys, size_t n, uint32_t kmin,
-               uint32_t range, uint32_t *__restrict__ hist) {
-  __shared__ uint32_t h[CS_HIST_LDS];
-  for (uint32_t k = threadIdx.x; k < range; k += GB_BLOCK) h[k] = 0;
-  __syncthreads();
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) atomicAdd(&h[keys[i] - kmin], 1u);
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < range; k += GB_BLOCK)
-    if (h[k]) atomicAdd(&hist[k], h[k]);
+gb_cs_dist(const uint32_t *__restrict__ skeys, size_t n, uint32_t kmin, uint64_t range,
+           uint32_t *__restrict__ dist) {
+  for (uint64_t k = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; k <= range;
+       k += (uint64_t)gridDim.x * GB_BLOCK) {
+    size_t lo = 0, hi = n;  // first position whose key is >= kmin + k
+    while (lo < hi) {
+      const size_t mid = lo + (hi - lo) / 2;
+      if ((uint64_t)(skeys[mid] - kmin) < k) lo = mid + 1;
+      else hi = mid;
+    }
+    dist[k] = (uint32_t)lo;
+  }
 }
 
 __global__ void gb_cs_check_keys(const uint32_t *__restrict__ keys, size_t n,
@@ -310,30 +350,17 @@ static int gb_countingsort(const uint32_t *d_keys, const uint32_t *d_vals,
     GB_HIP(hipStreamSynchronize(s));
     if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
-  if (d_dist) {
-    if (range > 0xfffffffeull) return GOOSEBERRY_ERROR_INVALIDINPUT;
-    GB_HIP(hipMemsetAsync(d_dist, 0, (range + 1) * sizeof(uint32_t), s));
-    if (n && range <= CS_HIST_LDS) {
-      uint32_t g = gb_grid_stride_blocks(n);
-      if (g > 1024u) g = 1024u;
-      gb_cs_hist_lds<<<g, GB_BLOCK, 0, s>>>(d_keys, n, kmin, (uint32_t)range,
-                                            d_dist);
-      GB_LAUNCH_CHECK();
-    } else if (n) {
-      gb_cs_hist<<<gb_grid_stride_blocks(n), GB_BLOCK, 0, s>>>(d_keys, n, kmin,
-                                                              d_dist);
-      GB_LAUNCH_CHECK();
-    }
-    rv = gb_scan_u32(d_dist, d_dist, range + 1, nullptr);
-    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (d_dist && range > 0xfffffffeull) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (n == 0) {
+    if (d_dist) GB_HIP(hipMemsetAsync(d_dist, 0, (range + 1) * sizeof(uint32_t), s));
+    return GOOSEBERRY_SUCCESS;
   }
-  if (n == 0) return GOOSEBERRY_SUCCESS;
   int bits = 0;
   while (bits < 32 && (range - 1) >> bits) bits += 8;
   if (bits == 0) bits = 8;
   const int npass = bits / 8;
-  const uint32_t nch = gb_div_up(n, CS_CHUNK);
-  const size_t ncounts = (size_t)CS_RADIX * nch;
+  const uint32_t ntiles = gb_div_up(n, CS_TILE);
+  const size_t ncounts = (size_t)CS_RADIX * ntiles;
   uint32_t *counts = nullptr, *tk[2] = {nullptr, nullptr},
            *tv[2] = {nullptr, nullptr};
   GB_HIP(gb_dev_alloc((void **)&counts, ncounts * sizeof(uint32_t)));
@@ -347,15 +374,21 @@ static int gb_countingsort(const uint32_t *d_keys, const uint32_t *d_vals,
     const bool last = (p == npass - 1);
     uint32_t *ok = tk[p & 1];
     uint32_t *ov = last ? d_out : tv[p & 1];
-    gb_cs_count<<<nch, GB_WAVE, 0, s>>>(ck, n, kmin, shift, counts, nch);
+    gb_cs_count<<<ntiles, GB_BLOCK, 0, s>>>(ck, n, kmin, shift, counts, ntiles);
     GB_LAUNCH_CHECK();
     rv = gb_scan_u32(counts, counts, ncounts, nullptr);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
-    gb_cs_scatter<<<nch, GB_WAVE, 0, s>>>(ck, cv, n, kmin, shift, counts, nch,
-                                          ok, ov);
+    gb_cs_scatter<<<ntiles, GB_BLOCK, 0, s>>>(ck, cv, n, kmin, shift, counts, ntiles, ok, ov);
     GB_LAUNCH_CHECK();
     ck = ok;
     cv = ov;
+  }
+  // the distribution from the sorted keys
+  if (d_dist) {
+    uint64_t g = (range + GB_BLOCK) / GB_BLOCK;
+    if (g > 65536) g = 65536;
+    gb_cs_dist<<<(uint32_t)g, GB_BLOCK, 0, s>>>(ck, n, kmin, range, d_dist);
+    GB_LAUNCH_CHECK();
   }
   GB_HIP(gb_dev_free(counts));
   GB_HIP(gb_dev_free(tk[0]));

@@ -23,6 +23,46 @@ def test_countingsort_matches_oracle(gb, oracle, n, kmin, kmax):
     assert np.array_equal(out, rout) and np.array_equal(dist, rdist)
 
 
+@pytest.mark.parametrize("case", ["tile_edges", "all_equal", "four_passes", "skewed_gaps",
+                                  "no_values"])
+def test_countingsort_tiles_and_distributions(gb, oracle, case):
+    # tiles of 4096 keys: partial last tiles, a single digit everywhere,
+    # full 32-bit keys, heavy and empty key ranges, the index payload
+    rng = np.random.default_rng(len(case))
+    vals = None
+    if case == "tile_edges":
+        for n in (4095, 4096, 4097, 3 * 4096 + 1):
+            keys = rng.integers(0, 1000, n).astype(np.uint32)
+            v = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+            out, dist = gb.countingsort_kv(keys, 0, 999, vals=v)
+            rout, rdist = oracle.countingsort_kv(keys, 0, 999, vals=v)
+            assert np.array_equal(out, rout) and np.array_equal(dist, rdist)
+        return
+    if case == "all_equal":
+        keys, kmin, kmax = np.full(300000, 77, np.uint32), 0, 200
+    elif case == "four_passes":
+        keys = rng.integers(0, 1 << 32, 200000, dtype=np.uint64).astype(np.uint32)
+        kmin, kmax = 0, 0xffffffff - 1
+        keys = np.minimum(keys, kmax)
+    elif case == "skewed_gaps":
+        keys = (rng.zipf(1.5, 500000) % 70000).astype(np.uint32) * 3
+        kmin, kmax = 0, 3 * 70000
+    else:
+        keys = rng.integers(5, 50000, 123457).astype(np.uint32)
+        kmin, kmax = 5, 49999
+    if case != "no_values":
+        vals = rng.integers(0, 1 << 32, keys.size, dtype=np.uint64).astype(np.uint32)
+    want_dist = kmax - kmin < 1 << 24
+    out, dist = gb.countingsort_kv(keys, kmin, kmax, vals=vals, want_dist=want_dist)
+    # a stable sort's order, and dist[k] = keys below kmin + k
+    order = np.argsort(keys, kind="stable")
+    rout = order.astype(np.uint32) if vals is None else vals[order]
+    assert np.array_equal(out, rout)
+    if want_dist:
+        rdist = np.searchsorted(np.sort(keys), np.arange(kmin, kmax + 2, dtype=np.int64))
+        assert np.array_equal(dist, rdist.astype(np.uint32))
+
+
 def test_countingsort_rejects_out_of_range_keys(gb):
     with pytest.raises(gb.GooseberryError) as e:
         gb.countingsort_kv(np.array([0, 5, 9], np.uint32), 0, 8)
