@@ -671,7 +671,11 @@ typedef struct gooseberry_pagerank_info_t {
 /* The same products on resident matrices (B needs at least as many rows as
  * A has columns; only the first A-ncols rows are used).  The sparse form
  * returns C as a new resident matrix planned with `plan` (NULL: defaults);
- * the dense form writes row-major C (A-nrows x B-ncols) to device memory. */
+ * the dense form writes row-major C (A-nrows x B-ncols) to device memory.
+ * The sparse forms stage the short rows' entries in 12 bytes of device
+ * memory per product, up to 8 GiB; the environment variable
+ * GOOSEBERRY_SPGEMM_STAGE_MAX (bytes, 0 = off) changes that limit, past
+ * which the rows are counted and written in two passes (same result). */
 int gooseberry_matrix_spmultsp(
     gooseberry_matrix_t a,
     gooseberry_matrix_t b,
