@@ -100,6 +100,7 @@ struct gbd_batch {
   gb_blists *d_lists;      // device counts
   gb_blists *h_lists;      // pinned
   uint32_t *d_spmv, *d_elem, *d_sysl;  // device work lists
+  uint4 *d_desc;  // per SpMV list entry: a row block's r0, r1, first entry, entries
   uint32_t cap_spmv, cap_elem;
   uint32_t nseg;                       // SpMV launches per step
   uint32_t cap_seg[GB_SPLIT_MAX];      // their grids
@@ -141,22 +142,25 @@ __device__ __forceinline__ double gb_b_sum_chunks(const double *part, uint32_t n
 // One SpMV work item of system s (same bins and orders as gb_spmv_kernel).
 // EPI 0: y = A x, dot += x[row] * y   (CG step; T = value/vector type)
 // EPI 1: out = b - A x in FP64, dot += out^2   (refresh step; T = double)
+// A row block's rows and entries come from its descriptor (d), filled when
+// the work list is built, so the block's loads start without waiting on the
+// plan's block table and row pointers.
 template <typename T, int EPI>
 __device__ __forceinline__ double gb_b_spmv_item(const gb_bplan &S, uint32_t w,
                                                  const T *__restrict__ val,
                                                  const T *__restrict__ x,
                                                  T *__restrict__ y,
                                                  const double *__restrict__ b,
-                                                 T *prod, T *tred) {
+                                                 T *prod, T *tred, const uint4 d) {
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & (GB_WAVE - 1), wave = tid / GB_WAVE;
   const uint32_t *__restrict__ rowptr = S.rowptr;
   const uint32_t *__restrict__ rowind = S.rowind;
   double dotp = 0.0;
   if (w < S.nstream) {
-    const uint32_t r0 = S.blk[2 * w], r1 = S.blk[2 * w + 1];
-    const uint32_t k0 = rowptr[r0];
-    const uint32_t cnt = rowptr[r1] - k0;
+    const uint32_t r0 = d.x, r1 = d.y;
+    const uint32_t k0 = d.z;
+    const uint32_t cnt = d.w;
     const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
     uint32_t pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0;
     T xa = 0, xb = 0;
@@ -262,8 +266,32 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bplan &S, uint32_t w,
 
 // ----------------------------------------------------------------- kernels
 
+// The descriptors of the SpMV list entries [a, b) of segment seg: for a
+// row block (item below its plan's nstream), its first and last-plus-one
+// rows, its first entry and its entry count.
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_b_describe(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
+              uint32_t a, uint32_t b, uint32_t seg, const gb_bplan *__restrict__ slice_tab,
+              uint4 *__restrict__ desc) {
+  for (uint32_t it = a + blockIdx.x * GB_BLOCK + threadIdx.x; it < b;
+       it += gridDim.x * GB_BLOCK) {
+    const uint32_t s = items[2 * it], lw = items[2 * it + 1];
+    const gb_bsys &S = sys[s];
+    const gb_bplan &L = S.nsplit ? slice_tab[(size_t)s * GB_SPLIT_MAX + seg] : S.pl;
+    uint4 d = make_uint4(0u, 0u, 0u, 0u);
+    if (lw < L.nstream) {
+      d.x = L.blk[2 * lw];
+      d.y = L.blk[2 * lw + 1];
+      d.z = L.rowptr[d.x];
+      d.w = L.rowptr[d.y] - d.z;
+    }
+    desc[it] = d;
+  }
+}
+
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
+          const uint4 *__restrict__ desc,
           const gb_blists *__restrict__ lists, uint32_t seg,
           const gbd_bstate *__restrict__ states, const gb_bplan *__restrict__ slice_tab) {
   __shared__ double prodd[GB_STREAM_LDS];
@@ -272,6 +300,7 @@ gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
   const uint32_t end = lists->seg[seg + 1];
   for (uint32_t it = lists->seg[seg] + blockIdx.x; it < end; it += gridDim.x) {
     const uint32_t s = items[2 * it], lw = items[2 * it + 1];
+    const uint4 d = desc[it];
     // the system, its state and its slice plan are addressed from the slot
     // alone, so their loads go out together
     const gb_bsys &S = sys[s];
@@ -289,22 +318,22 @@ gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
       w = L.base + lw;
       if (mode == GB_MODE_REFRESH) {
         double *yp = (double *)S.ypart + (size_t)r * S.n;
-        dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.x, yp, nullptr, prodd, tredd);
+        dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.x, yp, nullptr, prodd, tredd, d);
       } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
         float *yp = (float *)S.ypart + (size_t)r * S.n;
         dotp = gb_b_spmv_item<float, 0>(L, lw, L.val32, S.p32, yp, nullptr,
-                                        (float *)prodd, (float *)tredd);
+                                        (float *)prodd, (float *)tredd, d);
       } else {
         double *yp = (double *)S.ypart + (size_t)r * S.n;
-        dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.p, yp, nullptr, prodd, tredd);
+        dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.p, yp, nullptr, prodd, tredd, d);
       }
     } else if (mode == GB_MODE_REFRESH) {
-      dotp = gb_b_spmv_item<double, 1>(S.pl, w, S.pl.val, S.x, S.r, S.b, prodd, tredd);
+      dotp = gb_b_spmv_item<double, 1>(S.pl, w, S.pl.val, S.x, S.r, S.b, prodd, tredd, d);
     } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
       dotp = gb_b_spmv_item<float, 0>(S.pl, w, S.pl.val32, S.p32, S.ap32, nullptr,
-                                      (float *)prodd, (float *)tredd);
+                                      (float *)prodd, (float *)tredd, d);
     } else {
-      dotp = gb_b_spmv_item<double, 0>(S.pl, w, S.pl.val, S.p, S.ap, nullptr, prodd, tredd);
+      dotp = gb_b_spmv_item<double, 0>(S.pl, w, S.pl.val, S.p, S.ap, nullptr, prodd, tredd, d);
     }
     // one partial per work item
     const double bs = gb_block_sum(dotp, red);
@@ -676,8 +705,8 @@ static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
   hipStream_t s = gb_stream();
   for (uint32_t i = 0; i < k; ++i) {
     for (uint32_t g = 0; g < B->nseg; ++g) {
-      gb_b_spmv<<<B->cap_seg[g], GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_lists, g,
-                                                    B->d_st, B->d_slices);
+      gb_b_spmv<<<B->cap_seg[g], GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_desc,
+                                                    B->d_lists, g, B->d_st, B->d_slices);
       GB_LAUNCH_CHECK();
     }
     gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
@@ -817,6 +846,7 @@ void gbd_batch_destroy(gbd_batch *B) {
   (void)gb_dev_free(B->d_lists);
   (void)gb_dev_free(B->d_slices);
   (void)gb_dev_free(B->d_spmv);
+  (void)gb_dev_free(B->d_desc);
   (void)gb_dev_free(B->d_elem);
   (void)gb_dev_free(B->d_sysl);
   if (B->h_st) (void)hipHostFree(B->h_st);
@@ -947,8 +977,10 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   if (resize(ns, B->cap_spmv) || !B->d_spmv) {
     uint32_t cap = (uint32_t)(ns + ns / 2 + GB_BLIST_MIN);
     if (B->d_spmv) GB_HIP(gb_dev_free(B->d_spmv));
+    if (B->d_desc) GB_HIP(gb_dev_free(B->d_desc));
     if (B->h_spmv) GB_HIP(hipHostFree(B->h_spmv));
     GB_HIP(gb_dev_alloc((void **)&B->d_spmv, (size_t)cap * 8));
+    GB_HIP(gb_dev_alloc((void **)&B->d_desc, (size_t)cap * sizeof(uint4)));
     GB_HIP(hipHostMalloc((void **)&B->h_spmv, (size_t)cap * 8, hipHostMallocDefault));
     B->cap_spmv = cap;
     gb_b_drop_graphs(B);  // the graphs hold the old list pointer
@@ -1016,7 +1048,19 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   B->h_lists->n_spmv = (uint32_t)ns;
   B->h_lists->n_elem = (uint32_t)ne;
   B->h_lists->n_sys = nsys;
-  if (ns) GB_HIP(hipMemcpyAsync(B->d_spmv, B->h_spmv, ns * 8, hipMemcpyHostToDevice, s));
+  if (ns) {
+    GB_HIP(hipMemcpyAsync(B->d_spmv, B->h_spmv, ns * 8, hipMemcpyHostToDevice, s));
+    for (uint32_t g = 0; g < nseg; ++g) {
+      const uint32_t a0 = B->h_lists->seg[g], a1 = B->h_lists->seg[g + 1];
+      if (a1 > a0) {
+        uint32_t gr = gb_div_up(a1 - a0, GB_BLOCK);
+        if (gr > 4096u) gr = 4096u;
+        gb_b_describe<<<gr, GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, a0, a1, g, B->d_slices,
+                                              B->d_desc);
+        GB_LAUNCH_CHECK();
+      }
+    }
+  }
   if (ne) GB_HIP(hipMemcpyAsync(B->d_elem, B->h_elem, ne * 8, hipMemcpyHostToDevice, s));
   if (nsys)
     GB_HIP(hipMemcpyAsync(B->d_sysl, B->h_sysl, (size_t)nsys * 4, hipMemcpyHostToDevice, s));
@@ -1028,6 +1072,7 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
 int gbd_batch_step(gbd_batch *B, uint32_t k) {
   hipStream_t s = gb_stream();
   if (!B->d_spmv || !B->d_elem) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!gb_graphs_enabled()) return gb_b_enqueue_raw(B, k);
   for (int i = GB_B_NGRAPH - 1; i >= 0 && k; --i) {
     const uint32_t len = 1u << i;
     while (k >= len) {

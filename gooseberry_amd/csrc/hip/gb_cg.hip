@@ -340,6 +340,7 @@ int gbd_cg_enqueue(gbd_cg *cg, uint32_t k) { return gb_cg_enqueue_raw(cg, k); }
 // per power of two, captured on first use and kept while x stays the same).
 int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k) {
   hipStream_t s = gb_stream();
+  if (!gb_graphs_enabled()) return gb_cg_enqueue_raw(cg, k);
   if (cg->graph_x != cg->x) {
     gb_cg_drop_graph(cg);
     cg->graph_x = cg->x;

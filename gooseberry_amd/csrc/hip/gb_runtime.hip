@@ -1,6 +1,7 @@
 // gb_runtime.hip -- device selection, the library stream, memory, events.
 #include <pthread.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <list>
@@ -16,6 +17,7 @@ static pthread_once_t g_once = PTHREAD_ONCE_INIT;
 static int g_init_rv = GOOSEBERRY_SUCCESS;
 static char g_last_error[512] = "";
 static size_t g_cache_cap = 0;  // bytes the large-block cache may hold
+static bool g_graphs = true;     // replay captured step graphs
 
 int gb_hip_fail(hipError_t e, const char *what, const char *file, int line) {
   snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)",
@@ -65,7 +67,14 @@ static void gb_do_init() {
     return;
   }
   g_cache_cap = tot / 8 < (32ull << 30) ? tot / 8 : (32ull << 30);
+  // GOOSEBERRY_NO_GRAPHS (set, not "0"): launch CG and batch steps one by
+  // one instead of replaying captured graphs, for tracing tools that do not
+  // follow graph launches
+  const char *ng = getenv("GOOSEBERRY_NO_GRAPHS");
+  g_graphs = !(ng && *ng && strcmp(ng, "0") != 0);
 }
+
+bool gb_graphs_enabled() { return g_graphs; }
 
 // Large blocks (4 MiB and up, sizes rounded to 2 MiB) that the library
 // frees are kept here and handed back to a later allocation of the same
