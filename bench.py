@@ -217,7 +217,8 @@ def run_laplacian(args, gb):
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
             "traffic": traffic,
-            "kernel": "gb_spmv_kernel<double,false>",
+            "kernel": "gb_spmv_kernel<double,false,%s>" % (
+                "true" if M.info()["index16"] else "false"),
             "algorithmic_bytes_per_launch": ab,
             "avg_launch_ms": avg_spmv_ms,
         },

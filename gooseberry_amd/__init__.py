@@ -102,14 +102,16 @@ class _Plan(C.Structure):
     _fields_ = [("short_max", C.c_uint32), ("wave_max", C.c_uint32),
                 ("window", C.c_uint32), ("rowcap", C.c_uint32),
                 ("exact", C.c_uint32), ("grid", C.c_uint32),
-                ("xcd_remap", C.c_uint32), ("csplit", C.c_uint32)]
+                ("xcd_remap", C.c_uint32), ("csplit", C.c_uint32),
+                ("wide_index", C.c_uint32)]
 
 
 class _MatrixInfo(C.Structure):
     _fields_ = [("nrows", C.c_uint32), ("ncols", C.c_uint32),
                 ("nnz", C.c_uint64), ("plan", _Plan),
                 ("nblocks_stream", C.c_uint32), ("nrows_wave", C.c_uint32),
-                ("nrows_block", C.c_uint32), ("hist", C.c_uint64 * 33)]
+                ("nrows_block", C.c_uint32), ("hist", C.c_uint64 * 33),
+                ("index16", C.c_uint32)]
 
 
 class CgdInfo(C.Structure):
@@ -362,7 +364,8 @@ def _plan(plan):
     p = _Plan(plan.get("short_max", 0), plan.get("wave_max", 0),
               plan.get("window", 0), plan.get("rowcap", 0),
               1 if plan.get("exact") else 0, plan.get("grid", 0),
-              int(plan.get("xcd_remap", 0)), plan.get("csplit", 0))
+              int(plan.get("xcd_remap", 0)), plan.get("csplit", 0),
+              1 if plan.get("wide_index") else 0)
     return C.byref(p)
 
 
@@ -414,9 +417,11 @@ class Matrix:
                 "plan": {"short_max": i.plan.short_max, "wave_max": i.plan.wave_max,
                          "window": i.plan.window, "rowcap": i.plan.rowcap,
                          "exact": i.plan.exact, "grid": i.plan.grid,
-                         "xcd_remap": i.plan.xcd_remap, "csplit": i.plan.csplit},
+                         "xcd_remap": i.plan.xcd_remap, "csplit": i.plan.csplit,
+                         "wide_index": i.plan.wide_index},
                 "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
-                "nrows_block": i.nrows_block, "hist": list(i.hist)}
+                "nrows_block": i.nrows_block, "hist": list(i.hist),
+                "index16": i.index16}
 
     def replan(self, plan=None):
         check(lib().gooseberry_matrix_replan(self.h, _plan(plan)), "gooseberry_matrix_replan")

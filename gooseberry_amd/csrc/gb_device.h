@@ -91,6 +91,9 @@ typedef struct gbd_plan_params {
                           each multiplied by its own launch into a partial
                           vector, the partials added after (x gathers stay
                           in one slice of x at a time); 0 or 1: off */
+  uint32_t wide_index; /* 1: the row blocks read the 32-bit column indices;
+                          0: 16-bit offsets from the block's first row when
+                          every block's offsets fit in int16 */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram
@@ -120,6 +123,8 @@ void gbd_matrix_destroy(gbd_matrix *m);
 int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
     uint64_t *nnz, gbd_plan_params *params, uint32_t *nblocks_stream,
     uint32_t *nrows_wave, uint32_t *nrows_block, uint64_t *hist33);
+/* 1 when the plan's row blocks read 16-bit column offsets */
+uint32_t gbd_matrix_index16(const gbd_matrix *m);
 const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m);
 const uint32_t *gbd_matrix_rowind(const gbd_matrix *m);
 const double *gbd_matrix_rowval(const gbd_matrix *m);

@@ -23,6 +23,8 @@ struct gbd_matrix {
   uint32_t nwave;       // rows processed one wavefront per row
   uint32_t nblockrows;  // rows processed one workgroup per row
   uint32_t *rowbins;    // stable bin permutation: short | wave | block rows
+  int16_t *dcol;        // nnz or null: row-block entries' column - block's
+                        // first row (valid only in the row blocks)
   uint64_t hist[33];
   // column split (params.csplit = P > 1): P views, view r holding the
   // entries of columns [r W, (r + 1) W), W = ceil(ncols / P), in row order;

@@ -136,6 +136,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   if (p->grid) d.grid = p->grid;
   d.xcd_remap = p->xcd_remap;
   d.csplit = p->csplit > 1 ? p->csplit : 0;
+  d.wide_index = p->wide_index ? 1 : 0;
   *out = d;
   *have = 1;
 }
@@ -291,6 +292,8 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.grid = p.grid;
   info->plan.xcd_remap = p.xcd_remap;
   info->plan.csplit = p.csplit;
+  info->plan.wide_index = p.wide_index;
+  info->index16 = gbd_matrix_index16(mat->m);
   return GOOSEBERRY_SUCCESS;
 }
 

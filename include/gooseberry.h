@@ -347,6 +347,10 @@ typedef struct gooseberry_plan_t {
                        launch per slice into a partial vector, partials
                        added after; keeps the gathered slice of x in L2 for
                        matrices with scattered columns (not with exact) */
+  dim_t wide_index; /* 1: the row blocks read the 32-bit column indices;
+                       0: they read each entry's column as a 16-bit offset
+                       from the block's first row (2 bytes an entry less),
+                       when every offset fits (same results either way) */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */
@@ -404,6 +408,7 @@ typedef struct gooseberry_matrix_info_t {
   dim_t nrows_wave;       /* rows handled one wavefront per row */
   dim_t nrows_block;      /* rows handled one workgroup per row */
   uint64_t hist[33];      /* rows with floor(log2(len+1)) == b */
+  dim_t index16;          /* 1: the row blocks read 16-bit column offsets */
 } gooseberry_matrix_info_t;
 
 int gooseberry_matrix_info(

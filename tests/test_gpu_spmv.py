@@ -282,3 +282,75 @@ def test_column_split_cg_and_batch(gb, oracle):
     assert B.info(s).status == 1
     r = b - oracle.spmv(rp, ri, rv, dx.numpy())
     assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
+
+
+# ---- 16-bit column offsets in the row blocks (plan wide_index = 0)
+
+def test_index16_used_and_same_bits_as_wide(gb, oracle):
+    rp, ri, rv = gb.laplacian(3, 60)
+    n = rp.size - 1
+    x = gb.uniform(5, n)
+    Mn, yn, ref = check(gb, oracle, rp, ri, rv, x)
+    Mw, yw, _ = check(gb, oracle, rp, ri, rv, x, plan={"wide_index": 1})
+    assert Mn.info()["index16"] == 1 and Mn.info()["plan"]["wide_index"] == 0
+    assert Mw.info()["index16"] == 0 and Mw.info()["plan"]["wide_index"] == 1
+    assert np.array_equal(yn, yw) and np.array_equal(yn, ref)
+    # FP32 values read the same offsets
+    xf = x.astype(np.float32)
+    assert np.array_equal(Mn.spmv_f32(xf), Mw.spmv_f32(xf))
+    # CG: same iterations and bits
+    b = gb.uniform(6, n)
+    xa, ia = Mn.cgd(b, 1e-10)
+    xb, ib = Mw.cgd(b, 1e-10)
+    assert ia.niter == ib.niter and np.array_equal(xa, xb)
+
+
+def one_per_row(n, ncols, extra):
+    """Diagonal matrix plus (row, col) entries, each row sorted."""
+    rows = [[i] if i < ncols else [] for i in range(n)]
+    for r, c in extra:
+        rows[r] = sorted(set(rows[r]) | {c})
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint32)
+    ri = np.array([c for r in rows for c in r], dtype=np.uint32)
+    rv = np.random.default_rng(n + len(extra)).uniform(-1, 1, ri.size)
+    return rp, ri, rv
+
+
+@pytest.mark.parametrize("col,fits", [(32767, True), (32768, False)])
+def test_index16_positive_limit(gb, oracle, col, fits):
+    # one row block starting at row 0: the offset is the column itself
+    rp, ri, rv = one_per_row(4, 40000, [(0, col), (2, 7)])
+    x = np.random.default_rng(1).uniform(-1, 1, 40000)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    assert M.info()["nblocks_stream"] == 1
+    assert M.info()["index16"] == (1 if fits else 0)
+    assert np.array_equal(y, ref)
+
+
+@pytest.mark.parametrize("col,fits", [(39000 - 32768, True), (39000 - 32769, False)])
+def test_index16_negative_limit(gb, oracle, col, fits):
+    # rowcap 1: every row is its own block, so row 39000's offset is col - 39000
+    rp, ri, rv = one_per_row(40000, 40000, [(39000, col)])
+    x = np.random.default_rng(2).uniform(-1, 1, 40000)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x, plan={"rowcap": 1})
+    assert M.info()["index16"] == (1 if fits else 0)
+    assert np.array_equal(y, ref)
+
+
+def test_index16_replan_and_scattered_columns(gb, oracle):
+    # uniform columns over 200000: offsets leave int16, the plan stays wide
+    rp, ri, rv = irregular(200000, 7, density=0.00002)
+    x = np.random.default_rng(8).uniform(-1, 1, 200000)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    assert M.info()["index16"] == 0
+    # a banded matrix switched between the two by replanning
+    rp, ri, rv = gb.laplacian(2, 300)
+    x = gb.uniform(9, rp.size - 1)
+    M = gb.Matrix(rp, ri, rv, plan={"wide_index": 1})
+    y0 = M.spmv(x)
+    M.replan({})
+    assert M.info()["index16"] == 1
+    assert np.array_equal(M.spmv(x), y0)
+    M.replan({"wide_index": 1, "rowcap": 64})
+    assert M.info()["index16"] == 0
+    assert np.array_equal(M.spmv(x), y0)
