@@ -55,6 +55,7 @@ enum { GB_MODE_CG = 0, GB_MODE_REFRESH = 1 };
 // one column slice of a split matrix (its items numbered from `base`).
 struct gb_bplan {
   const uint32_t *rowptr, *rowind, *blk, *wave_rows, *block_rows;
+  const int16_t *dcol;  // the row blocks' 16-bit column offsets, or null
   const double *val;
   const float *val32;
   uint32_t nstream, nwave, nwaveblocks, nblockrows, nwork, exact, base, pad;
@@ -175,19 +176,37 @@ __device__ __forceinline__ double gb_b_spmv_item(const gb_bplan &S, uint32_t w,
     }
     if (cnt > 0) {
       const uint32_t klast = k0 + cnt - 1;
+      // the matrix's plan decides between 16-bit offsets from r0 (plain
+      // loads; FP32 values plain too) and 32-bit indices, as in
+      // gb_spmv_kernel; the offsets are added at the gathers
+      const int16_t *__restrict__ dcol = S.dcol;
       uint32_t c[GB_STREAM_ITERS];
       T v[GB_STREAM_ITERS], xv[GB_STREAM_ITERS];
+      if (dcol) {
 #pragma unroll
-      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-        if (i * GB_BLOCK < cnt) {
-          uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
-          c[i] = __builtin_nontemporal_load(&rowind[k]);
-          v[i] = __builtin_nontemporal_load(&val[k]);
+        for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+          if (i * GB_BLOCK < cnt) {
+            uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
+            c[i] = (uint32_t)(int32_t)dcol[k];
+            if (sizeof(T) == 4) v[i] = val[k];
+            else v[i] = __builtin_nontemporal_load(&val[k]);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+          if (i * GB_BLOCK < cnt) {
+            uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
+            c[i] = __builtin_nontemporal_load(&rowind[k]);
+            v[i] = __builtin_nontemporal_load(&val[k]);
+          }
         }
       }
+      const T *xg = dcol ? x + r0 : x;
 #pragma unroll
       for (int i = 0; i < GB_STREAM_ITERS; ++i)
-        if (i * GB_BLOCK < cnt) xv[i] = x[c[i]];
+        if (i * GB_BLOCK < cnt)
+          xv[i] = xg[dcol ? (int64_t)(int32_t)c[i] : (int64_t)c[i]];
 #pragma unroll
       for (int i = 0; i < GB_STREAM_ITERS; ++i) {
         uint32_t e = i * GB_BLOCK + tid;
@@ -776,6 +795,7 @@ static gb_bplan gb_b_plan_of(const gbd_matrix *m, uint32_t base) {
   memset(&P, 0, sizeof(P));
   P.rowptr = m->rowptr;
   P.rowind = m->rowind;
+  P.dcol = m->dcol;
   P.blk = m->blk;
   P.wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
   P.block_rows = m->rowbins + (m->nrows - m->nblockrows);

@@ -313,3 +313,26 @@ def test_split_matrix_in_batch_alone_and_mix(gb, oracle, precision):
     # round less than whole-row FP32 sums, which can save mixed iterations)
     (x_u, i_u), = solve_batch(gb, [(M0, b)], 1e-10, precision)
     assert abs(int(i_u.niter) - int(i_alone.niter)) <= max(3, 0.15 * int(i_u.niter))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_index16_and_wide_plans_bitwise(gb, precision):
+    # the batched SpMV reads each matrix's 16-bit offsets when its plan has
+    # them, 32-bit indices otherwise; the same systems give the same bits
+    rng = np.random.default_rng(5)
+    shapes = [(2, 50), (3, 22), (2, 31), (3, 9)]
+    hosts = [gb.laplacian(d, s) for d, s in shapes]
+    bs = [rng.uniform(-1, 1, rp.size - 1) for rp, _, _ in hosts]
+    narrow = [gb.Matrix(*h) for h in hosts]
+    wide = [gb.Matrix(*h, plan={"wide_index": 1}) for h in hosts]
+    assert all(M.info()["index16"] == 1 for M in narrow)
+    assert all(M.info()["index16"] == 0 for M in wide)
+    # mixed: two narrow and two wide matrices in one batch
+    mixed = [narrow[0], wide[1], narrow[2], wide[3]]
+    ra = solve_batch(gb, list(zip(narrow, bs)), 1e-10, precision)
+    rb = solve_batch(gb, list(zip(wide, bs)), 1e-10, precision)
+    rc = solve_batch(gb, list(zip(mixed, bs)), 1e-10, precision)
+    for (xa, ia), (xb, ib), (xc, ic) in zip(ra, rb, rc):
+        assert ia.status == 1
+        assert ia.niter == ib.niter == ic.niter
+        assert np.array_equal(xa, xb) and np.array_equal(xa, xc)
