@@ -1,17 +1,23 @@
 // gb_cg.hip -- device side of the conjugate-gradient solver.
 //
-// Reference: src/cgd.c:40-129.  One iteration is three kernels on the
+// Reference: src/cgd.c:40-129.  One iteration is four kernels on the
 // library stream, all scalars kept on the device (no host round trip):
-//   K1 gb_spmv_kernel<DOT>: ap = A p and p.ap -> alpha = dr / p.ap
-//   K2 gb_cg_update_xr:     ps = alpha p; x += ps; r -= alpha ap; r.r
-//                           -> beta = r.r / dr, convergence test
-//   K3 gb_cg_update_p:      p = r + (beta/alpha) (alpha p)
+//   K1 gb_spmv_kernel<DOT>: ap = A p and the p.ap partials
+//   K2 gb_cg_alpha:         alpha = dr / p.ap
+//   K3 gb_cg_update_r:      r -= alpha ap; r.r partials
+//   K4 gb_cg_update_xp:     beta = r.r / dr, convergence test;
+//                           ps = alpha p; x += ps; p = r + (beta/alpha) ps
 // which is the reference's scale-then-add order (src/cgd.c:92-105) with
 // the same separately rounded products and sums (-ffp-contract=off); only
-// the order of the dot-product sums differs.  Every kernel first reads the
-// device `done` flag and returns at once when the solve has stopped, so
-// the host can enqueue iterations in batches (or replay them from a
-// hipGraph) and still stop exactly where the reference stops.
+// the order of the dot-product sums differs.  x += ps sits in K4, which
+// reads p anyway (one vector read less than in K3; x does not feed r).
+// K2-K4 first read the device `done` flag and return at once when the
+// solve has stopped, so the host can enqueue iterations in batches (or
+// replay them from a hipGraph) and still stop exactly where the reference
+// stops.  `done` is written only by the state's set-up and by K2 (one
+// workgroup): K4 records its stop in `status`, which every K4 workgroup
+// computes for itself, so all of them still add their share of x; the next
+// K2 turns a nonzero status into `done`.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -131,7 +137,11 @@ gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
             gbd_cg_state *st) {
   __shared__ double red[GB_RED_THREADS / GB_WAVE];
   __shared__ int32_t stop;
-  if (threadIdx.x == 0) stop = st->done;
+  if (threadIdx.x == 0) {
+    // a stop recorded by the previous iteration's K4 becomes `done` here
+    stop = st->done || st->status;
+    if (stop && !st->done) st->done = 1;
+  }
   __syncthreads();
   if (stop) return;
   double s = 0.0;
@@ -165,20 +175,17 @@ gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
   }
 }
 
-// src/cgd.c:92-97: ps = alpha p; x += ps; r -= alpha Ap; partial sums of r.r
+// src/cgd.c:96-97: r -= alpha Ap; partial sums of r.r
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_cg_update_xr(double *__restrict__ x, double *__restrict__ r,
-                const double *__restrict__ p, const double *__restrict__ ap,
-                uint32_t n, double *__restrict__ partials,
-                const gbd_cg_state *st) {
+gb_cg_update_r(double *__restrict__ r, const double *__restrict__ ap,
+               uint32_t n, double *__restrict__ partials,
+               const gbd_cg_state *st) {
   __shared__ double red[GB_WAVES_PER_BLOCK];
   if (gb_block_flag(&st->done)) return;
   const double alpha = st->alpha;
   double d = 0.0;
   for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
        i += gridDim.x * GB_BLOCK) {
-    double ps = alpha * p[i];
-    x[i] = x[i] + ps;
     double aps = alpha * ap[i];
     double rn = r[i] - aps;
     r[i] = rn;
@@ -188,10 +195,12 @@ gb_cg_update_xr(double *__restrict__ x, double *__restrict__ r,
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
-// src/cgd.c:100-119: beta = r.r / dr; p = nr + (beta/alpha) (alpha p);
-// stop when sqrt(r.r) <= tol or at the iteration cap
+// src/cgd.c:92-94, 100-119: beta = r.r / dr; stop when sqrt(r.r) <= tol or
+// at the iteration cap; ps = alpha p; x += ps; unless stopping,
+// p = nr + (beta/alpha) ps
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_cg_update_p(const double *__restrict__ r, double *__restrict__ p,
+gb_cg_update_xp(double *__restrict__ x, const double *__restrict__ r,
+               double *__restrict__ p,
                uint32_t n, const double *__restrict__ dnr_partials,
                uint32_t ndnr, gbd_cg_state *st, double *__restrict__ trace,
                uint32_t trace_cap) {
@@ -212,16 +221,20 @@ gb_cg_update_p(const double *__restrict__ r, double *__restrict__ p,
     st->rnorm = rnorm;
     if (trace_cap) trace[(it1 - 1) % trace_cap] = rnorm;
     st->iter = it1;
-    if (status) {
-      st->status = status;
-      st->done = 1;
-    }
+    // not `done`: the other workgroups of this launch may not have read it
+    if (status) st->status = status;
   }
-  if (status) return;
   const double boa = beta / alpha;
+  if (status) {
+    for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
+         i += gridDim.x * GB_BLOCK)
+      x[i] = x[i] + alpha * p[i];
+    return;
+  }
   for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
        i += gridDim.x * GB_BLOCK) {
     double ps = alpha * p[i];
+    x[i] = x[i] + ps;
     double t = ps * boa;
     p[i] = r[i] + t;
   }
@@ -236,11 +249,11 @@ static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
     if (rv != GOOSEBERRY_SUCCESS) return rv;
     gb_cg_alpha<<<1, GB_RED_THREADS, 0, s>>>(cg->partials, g1, cg->st);
     GB_LAUNCH_CHECK();
-    gb_cg_update_xr<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
-        cg->x, cg->r, cg->p, cg->ap, n, cg->partials2, cg->st);
+    gb_cg_update_r<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
+        cg->r, cg->ap, n, cg->partials2, cg->st);
     GB_LAUNCH_CHECK();
-    gb_cg_update_p<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
-        cg->r, cg->p, n, cg->partials2, cg->grid_elem, cg->st, cg->trace,
+    gb_cg_update_xp<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
+        cg->x, cg->r, cg->p, n, cg->partials2, cg->grid_elem, cg->st, cg->trace,
         cg->trace_cap);
     GB_LAUNCH_CHECK();
   }
@@ -378,6 +391,9 @@ int gbd_cg_poll(gbd_cg *cg, gbd_cg_state *st, double *h_trace) {
                           hipMemcpyDeviceToHost, s));
   GB_HIP(hipStreamSynchronize(s));
   *st = *cg->h_st;
+  // the last K4 of a stopped solve records its status before any K2 has
+  // turned it into `done`
+  if (st->status) st->done = 1;
   if (h_trace && cg->trace_cap)
     memcpy(h_trace, cg->h_trace, (size_t)cg->trace_cap * sizeof(double));
   return GOOSEBERRY_SUCCESS;

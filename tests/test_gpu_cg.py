@@ -16,6 +16,21 @@ def true_rel(oracle, rp, ri, rv, x, b):
     return np.linalg.norm(r) / np.linalg.norm(b)
 
 
+def test_stop_by_tolerance_and_by_cap_give_the_same_x(gb):
+    # the iteration that meets the tolerance still adds alpha p to x: capping
+    # the solve at that iteration count gives the same bits
+    rp, ri, rv = gb.laplacian(3, 30)
+    b = gb.uniform(8, rp.size - 1)
+    err = 1e-9 * np.linalg.norm(b)
+    x1, i1 = gb.cgd(rp, ri, rv, b, err)
+    assert i1.status == 1
+    x2, i2 = gb.cgd(rp, ri, rv, b, 0.0, niter=int(i1.niter))
+    assert i2.status == 2 and i2.niter == i1.niter
+    assert np.array_equal(x1, x2)
+    x3, _ = gb.cgd(rp, ri, rv, b, 0.0, niter=int(i1.niter) - 1)
+    assert not np.array_equal(x1, x3)
+
+
 def test_config1_cg_vs_oracle(gb, oracle):
     rp, ri, rv = gb.laplacian(2, 256)
     b = gb.uniform(0, 65536)
