@@ -40,14 +40,18 @@ __device__ __forceinline__ uint32_t gb_row_class(uint32_t len,
 }
 
 // Per row: bin (0 short / 1 wave / 2 block), the "starts a row block" flag
-// and the log2 row-length histogram.
+// and the log2 row-length histogram (hist[0..32]); hist[33] counts the short
+// rows whose first or last column rules out 16-bit offsets from any block
+// they can fall in (column - row above 32767, or below -32768 - (rowcap - 1)),
+// so that the plan skips building offsets that cannot fit.
 __global__ void __launch_bounds__(GB_BLOCK)
-gb_plan_classify(const uint32_t *__restrict__ rowptr, uint32_t nrows,
+gb_plan_classify(const uint32_t *__restrict__ rowptr,
+                 const uint32_t *__restrict__ rowind, uint32_t nrows,
                  gbd_plan_params p, uint32_t *__restrict__ cls,
                  uint32_t *__restrict__ flag,
                  unsigned long long *__restrict__ hist) {
-  __shared__ uint32_t h[33];
-  if (threadIdx.x < 33) h[threadIdx.x] = 0;
+  __shared__ uint32_t h[34];
+  if (threadIdx.x < 34) h[threadIdx.x] = 0;
   __syncthreads();
   uint32_t r = blockIdx.x * GB_BLOCK + threadIdx.x;
   if (r < nrows) {
@@ -58,6 +62,11 @@ gb_plan_classify(const uint32_t *__restrict__ rowptr, uint32_t nrows,
     // floor(log2(len + 1)); len + 1 wraps to 0 only for len = 2^32 - 1
     uint32_t bin = (len == 0xffffffffu) ? 32u : 31u - __clz(len + 1);
     atomicAdd(&h[bin], 1u);
+    if (c == 0 && len && !p.wide_index) {
+      const int64_t lo = -32768 - (int64_t)(p.rowcap - 1);
+      const int64_t d0 = (int64_t)rowind[a] - r, d1 = (int64_t)rowind[b - 1] - r;
+      if (d0 > 32767 || d0 < lo || d1 > 32767 || d1 < lo) atomicAdd(&h[33], 1u);
+    }
     uint32_t f = 0;
     if (c == 0) {
       if (r == 0) {
@@ -77,7 +86,7 @@ gb_plan_classify(const uint32_t *__restrict__ rowptr, uint32_t nrows,
     flag[r] = f;
   }
   __syncthreads();
-  if (threadIdx.x < 33 && h[threadIdx.x])
+  if (threadIdx.x < 34 && h[threadIdx.x])
     atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
@@ -116,12 +125,21 @@ gb_plan_blocks(const uint32_t *__restrict__ cls,
 }
 
 // dcol[k] = rowind[k] - r0 for the entries of each row block [r0, r1),
-// one workgroup per block; *bad is set when an offset leaves int16.
+// one workgroup per block; *bad is set when an offset leaves int16.  On a
+// matrix that does not fit, most blocks find offsets out of range: one
+// atomic per workgroup, and workgroups that start after the flag is set
+// (as their XCD sees it) return at once, so the atomics do not pile up on
+// the one word.
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_plan_narrow(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ rowind,
                const uint32_t *__restrict__ blk, int16_t *__restrict__ dcol,
                uint32_t *__restrict__ bad) {
+  __shared__ uint32_t s_bad;
+  if (threadIdx.x == 0)
+    s_bad = __hip_atomic_load(bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (s_bad) return;
   const uint32_t r0 = blk[2 * blockIdx.x], r1 = blk[2 * blockIdx.x + 1];
   const uint32_t k1 = rowptr[r1];
   bool out = false;
@@ -130,7 +148,7 @@ gb_plan_narrow(const uint32_t *__restrict__ rowptr,
     out |= d < -32768 || d > 32767;
     dcol[k] = (int16_t)d;
   }
-  if (__any(out) && (threadIdx.x & (GB_WAVE - 1)) == 0) atomicOr(bad, 1u);
+  if (__syncthreads_or(out) && threadIdx.x == 0) atomicOr(bad, 1u);
 }
 
 // --------------------------------------------------------------- kernels
@@ -596,10 +614,10 @@ static int gb_plan_build(gbd_matrix *m) {
   GB_HIP(gb_dev_alloc((void **)&flag, (size_t)n * 4));
   GB_HIP(gb_dev_alloc((void **)&idx, ((size_t)n + 1) * 4));
   GB_HIP(gb_dev_alloc((void **)&dist, 4 * 4));
-  GB_HIP(gb_dev_alloc((void **)&hist, 33 * 8));
-  GB_HIP(hipMemsetAsync(hist, 0, 33 * 8, s));
+  GB_HIP(gb_dev_alloc((void **)&hist, 34 * 8));
+  GB_HIP(hipMemsetAsync(hist, 0, 34 * 8, s));
   gb_plan_classify<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(
-      m->rowptr, n, m->params, cls, flag, hist);
+      m->rowptr, m->rowind, n, m->params, cls, flag, hist);
   GB_LAUNCH_CHECK();
   GB_HIP(gb_dev_alloc((void **)&m->rowbins, (size_t)n * 4));
   int rv = gbd_countingsort_kv_unchecked(cls, nullptr, 0, 2, n, m->rowbins, dist);
@@ -607,7 +625,7 @@ static int gb_plan_build(gbd_matrix *m) {
   rv = gbd_scan_u32(flag, idx, n, idx + n);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   uint32_t hd[4], nst = 0;
-  unsigned long long hh[33];
+  unsigned long long hh[34];
   GB_HIP(hipMemcpyAsync(hd, dist, sizeof(hd), hipMemcpyDeviceToHost, s));
   GB_HIP(hipMemcpyAsync(&nst, idx + n, 4, hipMemcpyDeviceToHost, s));
   GB_HIP(hipMemcpyAsync(hh, hist, sizeof(hh), hipMemcpyDeviceToHost, s));
@@ -627,7 +645,8 @@ static int gb_plan_build(gbd_matrix *m) {
   GB_HIP(gb_dev_free(idx));
   GB_HIP(gb_dev_free(dist));
   GB_HIP(gb_dev_free(hist));
-  if (!m->view && !m->params.wide_index && nst && m->nnz) {
+  // hh[33]: rows whose columns already rule the 16-bit offsets out
+  if (!m->view && !m->params.wide_index && nst && m->nnz && hh[33] == 0) {
     // 16-bit column offsets for the row blocks, kept only if all fit
     uint32_t *bad = nullptr, hbad = 0;
     GB_HIP(gb_dev_alloc((void **)&m->dcol, m->nnz * sizeof(int16_t)));

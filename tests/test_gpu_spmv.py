@@ -354,3 +354,19 @@ def test_index16_replan_and_scattered_columns(gb, oracle):
     M.replan({"wide_index": 1, "rowcap": 64})
     assert M.info()["index16"] == 0
     assert np.array_equal(M.spmv(x), y0)
+
+
+def test_index16_block_check_beyond_row_precheck(gb, oracle):
+    # row 39000's column 6000 is 33000 behind the row: within what its block
+    # might allow (blocks start up to 511 rows earlier), so only the block
+    # check finds that its block (rows from 38912) cannot hold it
+    rp, ri, rv = one_per_row(40000, 40000, [(39000, 6000)])
+    x = np.random.default_rng(3).uniform(-1, 1, 40000)
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    assert M.info()["index16"] == 0
+    assert np.array_equal(y, ref)
+    # column 6200 in the same row: -32712 from the block's first row, fits
+    rp, ri, rv = one_per_row(40000, 40000, [(39000, 6200)])
+    M, y, ref = check(gb, oracle, rp, ri, rv, x)
+    assert M.info()["index16"] == 1
+    assert np.array_equal(y, ref)
