@@ -163,7 +163,10 @@ gb_plan_narrow(const uint32_t *__restrict__ rowptr,
 // and writes it to partials[blockIdx.x]; the next CG kernel adds the
 // partials in a fixed order (no fences or atomics: deterministic).
 #ifndef GB_SPMV_WAVES
-#define GB_SPMV_WAVES 5  // wavefronts per SIMD the register budget must allow
+// wavefronts per SIMD the register budget must allow: 72 VGPRs (the FP64
+// variants spill 2 to scratch).  Measured on the 160^3 Laplacian with the
+// 16-bit offsets, per CG iteration: 5 waves 118.1 us, 6: 114.9, 7: 114.7.
+#define GB_SPMV_WAVES 7
 #endif
 // With NARROW the row blocks read dcol, each entry's column minus the
 // block's first row as a 16-bit signed offset, instead of rowind (the plan
