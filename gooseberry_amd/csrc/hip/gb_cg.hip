@@ -129,14 +129,21 @@ gb_cg_init_state(const double *__restrict__ partials, uint32_t np,
 
 // src/cgd.c:90: alpha = dr / p.Ap from the SpMV's per-row-block partials,
 // added in a fixed order by one 1024-thread workgroup; each thread issues
-// all of its loads before adding (one memory round trip per 8K partials).
+// all of its loads before adding (one memory round trip per 16K partials),
+// and the first round's loads go out before the stop flag is known.
 #define GB_RED_THREADS 1024
-#define GB_RED_UNROLL 8
+#define GB_RED_UNROLL 16
 __global__ void __launch_bounds__(GB_RED_THREADS)
 gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
             gbd_cg_state *st) {
   __shared__ double red[GB_RED_THREADS / GB_WAVE];
   __shared__ int32_t stop;
+  double v[GB_RED_UNROLL];
+#pragma unroll
+  for (int j = 0; j < GB_RED_UNROLL; ++j) {
+    uint32_t i = j * GB_RED_THREADS + threadIdx.x;
+    v[j] = i < npap ? pap_partials[i] : 0.0;
+  }
   if (threadIdx.x == 0) {
     // a stop recorded by the previous iteration's K4 becomes `done` here
     stop = st->done || st->status;
@@ -145,15 +152,16 @@ gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
   __syncthreads();
   if (stop) return;
   double s = 0.0;
-  for (uint32_t base = 0; base < npap; base += GB_RED_THREADS * GB_RED_UNROLL) {
-    double v[GB_RED_UNROLL];
+  for (uint32_t base = 0;;) {
+#pragma unroll
+    for (int j = 0; j < GB_RED_UNROLL; ++j) s += v[j];
+    base += GB_RED_THREADS * GB_RED_UNROLL;
+    if (base >= npap) break;
 #pragma unroll
     for (int j = 0; j < GB_RED_UNROLL; ++j) {
       uint32_t i = base + j * GB_RED_THREADS + threadIdx.x;
       v[j] = i < npap ? pap_partials[i] : 0.0;
     }
-#pragma unroll
-    for (int j = 0; j < GB_RED_UNROLL; ++j) s += v[j];
   }
   s = gb_wave_sum(s);
   if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = s;
