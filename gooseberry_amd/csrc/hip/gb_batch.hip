@@ -495,7 +495,7 @@ gb_b_update(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
             } else {
               ap = S.ap32[i];
             }
-            S.d32[i] = S.d32[i] + a * S.p32[i];
+            // d += a p is done by gb_b_pupdate, which reads p anyway
             float rn = S.r32[i] - a * ap;
             S.r32[i] = rn;
             d += (double)rn * (double)rn;
@@ -615,15 +615,25 @@ gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items
     }
     if (status) continue;
     if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      // this step's d += a p (the same float products and sums as beside
+      // the r update, where it was one more read of p), then p = r + beta p.
+      // A mixed system's CG step sets `done` above only on a non-finite
+      // residual, after which x never takes d: a workgroup that sees that
+      // flag early and skips this loop changes no result.
+      const float a = (float)alpha;
       const float boa = (float)beta;
       for (int j = 0; j < GB_BCHUNK_ITEMS; ++j) {
         uint32_t i = i0 + j * GB_BLOCK + threadIdx.x;
         if (i < S.n) {
-          S.p32[i] = S.r32[i] + boa * S.p32[i];
+          const float pv = S.p32[i];
+          const float dn = S.d32[i] + a * pv;
+          S.p32[i] = S.r32[i] + boa * pv;
           if (refresh) {
             // the FP64 solution update
-            S.x[i] = S.x[i] + (double)S.d32[i];
+            S.x[i] = S.x[i] + (double)dn;
             S.d32[i] = 0.0f;
+          } else {
+            S.d32[i] = dn;
           }
         }
       }
