@@ -104,6 +104,7 @@ struct gbd_batch {
   uint4 *d_desc;  // per SpMV list entry: a row block's r0, r1, first entry, entries
   uint32_t cap_spmv, cap_elem;
   uint32_t nseg;                       // SpMV launches per step
+  bool banded;                         // the SpMV build for banded batches
   uint32_t cap_seg[GB_SPLIT_MAX];      // their grids
   uint32_t *h_slice_nwork;             // host: max_slots x GB_SPLIT_MAX
   gb_bplan *d_slices;                  // max_slots x GB_SPLIT_MAX slice plans
@@ -308,14 +309,17 @@ gb_b_describe(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ item
   }
 }
 
-// wavefronts per SIMD the batched SpMV's register budget must allow.  On
-// 512 Laplacians (mixed precision): 1645 solves/s when the compiler chose
-// (87 VGPRs, five wavefronts), 1755 with seven; FP64 batches unchanged.
-#ifndef GB_B_SPMV_WAVES
-#define GB_B_SPMV_WAVES 7
-#endif
+// Wavefronts per SIMD the batched SpMV's register budget must allow, by
+// batch.  Measured: 512 Laplacians (mixed precision) 1645 solves/s at 5
+// (87 VGPRs, the compiler's own choice), 1755 at 7; the served stream of
+// random matrices 355-363 solves/s at 5, 344-346 at 7.  A step launches the
+// seven-wavefront build when every admitted system's matrix reads 16-bit
+// column offsets (banded rows, gathers that stay in L2), else the five.
+#define GB_B_SPMV_WAVES_BANDED 7
+#define GB_B_SPMV_WAVES_OTHER 5
+template <int WAVES>
 __global__ void __launch_bounds__(GB_BLOCK)
-__attribute__((amdgpu_waves_per_eu(GB_B_SPMV_WAVES)))
+__attribute__((amdgpu_waves_per_eu(WAVES)))
 gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
           const uint4 *__restrict__ desc,
           const gb_blists *__restrict__ lists, uint32_t seg,
@@ -741,8 +745,12 @@ static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
   hipStream_t s = gb_stream();
   for (uint32_t i = 0; i < k; ++i) {
     for (uint32_t g = 0; g < B->nseg; ++g) {
-      gb_b_spmv<<<B->cap_seg[g], GB_BLOCK, 0, s>>>(B->d_sys, B->d_spmv, B->d_desc,
-                                                    B->d_lists, g, B->d_st, B->d_slices);
+      if (B->banded)
+        gb_b_spmv<GB_B_SPMV_WAVES_BANDED><<<B->cap_seg[g], GB_BLOCK, 0, s>>>(
+            B->d_sys, B->d_spmv, B->d_desc, B->d_lists, g, B->d_st, B->d_slices);
+      else
+        gb_b_spmv<GB_B_SPMV_WAVES_OTHER><<<B->cap_seg[g], GB_BLOCK, 0, s>>>(
+            B->d_sys, B->d_spmv, B->d_desc, B->d_lists, g, B->d_st, B->d_slices);
       GB_LAUNCH_CHECK();
     }
     gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
@@ -1035,9 +1043,11 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   // matrices in segment 0
   uint32_t nseg = 1;
   uint64_t cnt[GB_SPLIT_MAX] = {0};
+  bool banded = true;  // every active system's matrix has 16-bit offsets
   for (uint32_t i = 0; i < B->max_slots; ++i) {
     if (!B->used[i] || (active && !active[i])) continue;
     const gb_bsys &S = B->h_sys[i];
+    banded = banded && !S.nsplit && S.pl.dcol;
     if (S.nsplit) {
       if (S.nsplit > nseg) nseg = S.nsplit;
       for (uint32_t r = 0; r < S.nsplit; ++r)
@@ -1045,6 +1055,10 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
     } else {
       cnt[0] += S.pl.nwork;
     }
+  }
+  if (banded != B->banded) {
+    B->banded = banded;
+    gb_b_drop_graphs(B);  // the graphs hold the SpMV build
   }
   bool regrid = nseg != B->nseg;
   for (uint32_t g = 0; g < nseg; ++g) regrid = regrid || resize(cnt[g], B->cap_seg[g]);
