@@ -525,8 +525,9 @@ class Matrix:
         return dx.numpy(), info
 
     def free(self):
+        # refused (INVALIDINPUT) while a batch or server slot holds the matrix
         if getattr(self, "h", None):
-            lib().gooseberry_matrix_free(self.h)
+            check(lib().gooseberry_matrix_free(self.h), "gooseberry_matrix_free")
             self.h = None
 
     def __del__(self):

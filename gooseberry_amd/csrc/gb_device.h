@@ -118,7 +118,10 @@ int gbd_matrix_create(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr,
 int gbd_matrix_create_trusted(uint32_t nrows, uint32_t ncols,
     uint32_t *d_rowptr, uint32_t *d_rowind, double *d_rowval, uint32_t nnz,
     const gbd_plan_params *params, gbd_matrix **out);
+/* INVALIDINPUT while a batch slot holds the matrix (gbd_matrix_holders) */
 int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params);
+/* batch slots currently holding pointers into the matrix's plan */
+uint32_t gbd_matrix_holders(const gbd_matrix *m);
 void gbd_matrix_destroy(gbd_matrix *m);
 int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
     uint64_t *nnz, gbd_plan_params *params, uint32_t *nblocks_stream,

@@ -98,6 +98,7 @@ struct gbd_batch {
   gbd_bstate *h_st;        // pinned copy
   uint8_t *used;           // host: slot in use
   void **slot_mem;         // host: per-slot workspace allocation
+  gbd_matrix **slot_mat;   // host: the matrix each used slot holds
   gb_blists *d_lists;      // device counts
   gb_blists *h_lists;      // pinned
   uint32_t *d_spmv, *d_elem, *d_sysl;  // device work lists
@@ -852,6 +853,7 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
   B->delta = delta > 0 && delta < 1 ? delta : 0.05;
   B->used = (uint8_t *)calloc(max_slots, 1);
   B->slot_mem = (void **)calloc(max_slots, sizeof(void *));
+  B->slot_mat = (gbd_matrix **)calloc(max_slots, sizeof(gbd_matrix *));
   B->h_slice_nwork = (uint32_t *)calloc((size_t)max_slots * GB_SPLIT_MAX, sizeof(uint32_t));
   B->h_sys = (gb_bsys *)calloc(max_slots, sizeof(gb_bsys));
   hipError_t e = gb_dev_alloc((void **)&B->d_sys, max_slots * sizeof(gb_bsys));
@@ -871,7 +873,8 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
     e = hipMemsetAsync(B->d_st, 0, max_slots * sizeof(gbd_bstate), gb_stream());
   if (e == hipSuccess)
     e = hipMemsetAsync(B->d_sys, 0, max_slots * sizeof(gb_bsys), gb_stream());
-  if (e != hipSuccess || !B->used || !B->slot_mem || !B->h_sys || !B->h_slice_nwork) {
+  if (e != hipSuccess || !B->used || !B->slot_mem || !B->slot_mat || !B->h_sys ||
+      !B->h_slice_nwork) {
     gbd_batch_destroy(B);
     return e != hipSuccess ? gb_hip_fail(e, "gbd_batch_create", __FILE__, __LINE__)
                            : GOOSEBERRY_ERROR_NOMEMORY;
@@ -886,6 +889,9 @@ void gbd_batch_destroy(gbd_batch *B) {
   gb_b_drop_graphs(B);
   if (B->slot_mem)
     for (uint32_t i = 0; i < B->max_slots; ++i) (void)gb_dev_free(B->slot_mem[i]);
+  if (B->slot_mat && B->used)
+    for (uint32_t i = 0; i < B->max_slots; ++i)
+      if (B->used[i] && B->slot_mat[i]) B->slot_mat[i]->holders--;
   (void)gb_dev_free(B->d_sys);
   (void)gb_dev_free(B->d_st);
   (void)gb_dev_free(B->d_lists);
@@ -901,6 +907,7 @@ void gbd_batch_destroy(gbd_batch *B) {
   if (B->h_sysl) (void)hipHostFree(B->h_sysl);
   free(B->used);
   free(B->slot_mem);
+  free(B->slot_mat);
   free(B->h_slice_nwork);
   free(B->h_sys);
   free(B);
@@ -990,6 +997,10 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   GB_HIP(hipStreamSynchronize(s));
   B->used[slot] = 1;
   B->slot_mem[slot] = mem;
+  // the slot's descriptor holds the plan's arrays: no replan or free of the
+  // matrix until the slot is removed
+  B->slot_mat[slot] = const_cast<gbd_matrix *>(m);
+  B->slot_mat[slot]->holders++;
   *slot_out = slot;
   return GOOSEBERRY_SUCCESS;
 }
@@ -1000,6 +1011,8 @@ int gbd_batch_remove(gbd_batch *B, uint32_t slot) {
   GB_HIP(gb_dev_free(B->slot_mem[slot]));
   B->slot_mem[slot] = nullptr;
   B->used[slot] = 0;
+  if (B->slot_mat[slot]) B->slot_mat[slot]->holders--;
+  B->slot_mat[slot] = nullptr;
   return GOOSEBERRY_SUCCESS;
 }
 

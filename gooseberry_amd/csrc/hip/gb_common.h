@@ -13,6 +13,8 @@
 
 // Record a HIP failure and map it to GOOSEBERRY_ERROR_DEVICE.
 int gb_hip_fail(hipError_t e, const char *what, const char *file, int line);
+// Record a refusal's reason as the last error text (stderr too).
+void gb_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 #define GB_HIP(call)                                                        \
   do {                                                                      \
     hipError_t gb_e_ = (call);                                              \

@@ -38,6 +38,8 @@ struct gbd_matrix {
   float *sval32;      // nnz, with val32
   double *ypart;      // P * nrows partial results (FP64, or FP32 in place)
   bool view;          // rowptr/rowind/val belong to a parent matrix
+  uint32_t holders;   // batch slots holding pointers into the plan: while
+                      // nonzero the plan must not be rebuilt or freed
 };
 
 #define GB_SPLIT_MAX 64

@@ -1,5 +1,6 @@
 // gb_runtime.hip -- device selection, the library stream, memory, events.
 #include <pthread.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,6 +26,14 @@ int gb_hip_fail(hipError_t e, const char *what, const char *file, int line) {
   fprintf(stderr, "gooseberry: HIP error %s\n", g_last_error);
   if (e == hipErrorOutOfMemory) return GOOSEBERRY_ERROR_NOMEMORY;
   return GOOSEBERRY_ERROR_DEVICE;
+}
+
+void gb_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "gooseberry: %s\n", g_last_error);
 }
 
 static void gb_do_init() {

@@ -768,6 +768,12 @@ int gbd_matrix_create_trusted(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr
 }
 
 int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
+  // a batch slot copied the plan's arrays into its descriptor
+  if (m->holders) {
+    gb_set_error("the matrix is held by %u batch or server slot(s): its plan cannot be "
+                 "rebuilt until they are removed", m->holders);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
   gbd_plan_params p;
   if (params) p = *params;
   else gbd_plan_defaults(m->hist, m->nrows, m->nnz, &p);
@@ -812,6 +818,8 @@ int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
 }
 
 uint32_t gbd_matrix_index16(const gbd_matrix *m) { return m->dcol ? 1u : 0u; }
+
+uint32_t gbd_matrix_holders(const gbd_matrix *m) { return m->holders; }
 
 const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m) { return m->rowptr; }
 const uint32_t *gbd_matrix_rowind(const gbd_matrix *m) { return m->rowind; }

@@ -3,6 +3,7 @@
  * on top of the thin device ABI (gb_device.h).
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <inttypes.h>
 #include <stdlib.h>
@@ -16,6 +17,30 @@ struct gooseberry_matrix_s {
   gbd_matrix *m;
   gbd_cg *cg;
 };
+
+static pthread_mutex_t g_api_mu;
+static pthread_once_t g_api_once = PTHREAD_ONCE_INIT;
+
+static void gb_api_mu_init(void)
+{
+  pthread_mutexattr_t a;
+  pthread_mutexattr_init(&a);
+  pthread_mutexattr_settype(&a, PTHREAD_MUTEX_RECURSIVE);
+  pthread_mutex_init(&g_api_mu, &a);
+  pthread_mutexattr_destroy(&a);
+}
+
+void gb_api_lock(void)
+{
+  pthread_once(&g_api_once, gb_api_mu_init);
+  pthread_mutex_lock(&g_api_mu);
+}
+
+void gb_api_unlock(int *scope)
+{
+  (void)scope;
+  pthread_mutex_unlock(&g_api_mu);
+}
 
 void gooseberry_free(void *ptr) { free(ptr); }
 
@@ -43,39 +68,65 @@ const char *gooseberry_last_device_error(void) { return gbd_last_error(); }
 
 int gooseberry_device_alloc(void **d_ptr, size_t bytes)
 {
+  GB_API_GUARD();
   if (!d_ptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
   return gbd_malloc(d_ptr, bytes);
 }
 
-int gooseberry_device_free(void *d_ptr) { return gbd_free(d_ptr); }
+int gooseberry_device_free(void *d_ptr)
+{
+  GB_API_GUARD();
+  return gbd_free(d_ptr);
+}
 
 int gooseberry_memcpy_htod(void *d_dst, const void *h_src, size_t bytes)
 {
+  GB_API_GUARD();
   return gbd_htod(d_dst, h_src, bytes);
 }
 
 int gooseberry_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes)
 {
+  GB_API_GUARD();
   return gbd_dtoh(h_dst, d_src, bytes);
 }
 
 int gooseberry_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes)
 {
+  GB_API_GUARD();
   return gbd_dtod(d_dst, d_src, bytes);
 }
 
-int gooseberry_device_synchronize(void) { return gbd_sync(); }
+int gooseberry_device_synchronize(void)
+{
+  GB_API_GUARD();
+  return gbd_sync();
+}
 
 int gooseberry_device_memory(size_t *free_bytes, size_t *total_bytes)
 {
+  GB_API_GUARD();
   return gbd_device_memory(free_bytes, total_bytes);
 }
 
-int gooseberry_event_create(void **ev) { return gbd_event_create(ev); }
-int gooseberry_event_destroy(void *ev) { return gbd_event_destroy(ev); }
-int gooseberry_event_record(void *ev) { return gbd_event_record(ev); }
+int gooseberry_event_create(void **ev)
+{
+  GB_API_GUARD();
+  return gbd_event_create(ev);
+}
+int gooseberry_event_destroy(void *ev)
+{
+  GB_API_GUARD();
+  return gbd_event_destroy(ev);
+}
+int gooseberry_event_record(void *ev)
+{
+  GB_API_GUARD();
+  return gbd_event_record(ev);
+}
 int gooseberry_event_elapsed_ms(void *a, void *b, float *ms)
 {
+  GB_API_GUARD();
   return gbd_event_elapsed_ms(a, b, ms);
 }
 
@@ -96,6 +147,7 @@ int gb_upload(void **d, const void *h, size_t bytes)
 
 int gooseberry_scale(ind_t n, real_t *val, real_t s)
 {
+  GB_API_GUARD();
   if (n == 0) return GOOSEBERRY_SUCCESS;
   if (!val) return GOOSEBERRY_ERROR_INVALIDINPUT;
   void *d = NULL;
@@ -108,6 +160,7 @@ int gooseberry_scale(ind_t n, real_t *val, real_t s)
 
 int gooseberry_add_scalar(ind_t n, real_t *val, real_t s)
 {
+  GB_API_GUARD();
   if (n == 0) return GOOSEBERRY_SUCCESS;
   if (!val) return GOOSEBERRY_ERROR_INVALIDINPUT;
   void *d = NULL;
@@ -194,6 +247,7 @@ int gooseberry_matrix_create(dim_t nrows, dim_t ncols, const ind_t *rowptr,
     const dim_t *rowind, const real_t *rowval, const gooseberry_plan_t *plan,
     gooseberry_matrix_t *r_mat)
 {
+  GB_API_GUARD();
   if (!r_mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
   int rv = gb_check_csr(nrows, ncols, rowptr, rowind);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
@@ -223,6 +277,7 @@ int gooseberry_matrix_create_device(dim_t nrows, dim_t ncols,
     const ind_t *d_rowptr, const dim_t *d_rowind, const real_t *d_rowval,
     const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
 {
+  GB_API_GUARD();
   if (!r_mat || !d_rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
   ind_t nnz = 0;
   int rv = gbd_dtoh(&nnz, d_rowptr + nrows, sizeof(ind_t));
@@ -255,7 +310,10 @@ int gooseberry_matrix_create_device(dim_t nrows, dim_t ncols,
 
 int gooseberry_matrix_free(gooseberry_matrix_t mat)
 {
+  GB_API_GUARD();
   if (!mat) return GOOSEBERRY_SUCCESS;
+  /* a batch or server still reads the matrix: remove its solve first */
+  if (gbd_matrix_holders(mat->m)) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gbd_cg_destroy(mat->cg);
   gbd_matrix_destroy(mat->m);
   free(mat);
@@ -265,7 +323,9 @@ int gooseberry_matrix_free(gooseberry_matrix_t mat)
 int gooseberry_matrix_replan(gooseberry_matrix_t mat,
     const gooseberry_plan_t *plan)
 {
+  GB_API_GUARD();
   if (!mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (gbd_matrix_holders(mat->m)) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gbd_plan_params pp;
   int have;
   plan_to_dev(plan, mat->m, &pp, &have);
@@ -278,6 +338,7 @@ int gooseberry_matrix_replan(gooseberry_matrix_t mat,
 int gooseberry_matrix_info(gooseberry_matrix_t mat,
     gooseberry_matrix_info_t *info)
 {
+  GB_API_GUARD();
   if (!mat || !info) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gbd_plan_params p;
   memset(info, 0, sizeof(*info));
@@ -300,6 +361,7 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
 int gooseberry_matrix_spmv(gooseberry_matrix_t mat, const real_t *d_x,
     real_t *d_y)
 {
+  GB_API_GUARD();
   if (!mat || !d_x || !d_y) return GOOSEBERRY_ERROR_INVALIDINPUT;
   return gbd_spmv(mat->m, d_x, d_y);
 }
@@ -307,6 +369,7 @@ int gooseberry_matrix_spmv(gooseberry_matrix_t mat, const real_t *d_x,
 int gooseberry_matrix_spmm(gooseberry_matrix_t mat, dim_t bncols,
     const real_t *d_b, real_t *d_c)
 {
+  GB_API_GUARD();
   if (!mat || !d_b || !d_c) return GOOSEBERRY_ERROR_INVALIDINPUT;
   return gbd_spmm(mat->m, bncols, d_b, d_c);
 }
@@ -324,6 +387,7 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
     const real_t *bcolval, real_t *crowval, const dim_t *blocks,
     dim_t nblocks)
 {
+  GB_API_GUARD();
   dim_t rlo = 0, rhi = anrows;
   if (blocks) {
     for (dim_t c = 0; c < nblocks; ++c) {
@@ -410,6 +474,7 @@ int gooseberry_spmultsp(dim_t anrows, dim_t ancols, dim_t bncols,
     const ind_t *browptr, const dim_t *browind, const real_t *browval,
     real_t *crowval, const dim_t *blocks, dim_t nblocks)
 {
+  GB_API_GUARD();
   dim_t rlo, rhi;
   int rv = spgemm_blocks(anrows, blocks, nblocks, &rlo, &rhi);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
@@ -439,6 +504,7 @@ int gooseberry_spmultsp_sp(dim_t anrows, dim_t ancols, dim_t bncols,
     ind_t **r_crowptr, dim_t **r_crowind, real_t **r_crowval,
     const dim_t *blocks, dim_t nblocks)
 {
+  GB_API_GUARD();
   if (!r_crowptr || !r_crowind || !r_crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
   /* the reference fills every row whatever the blocks, and its row
    * pointers are only whole when the blocks partition the rows */
@@ -489,6 +555,7 @@ int gooseberry_spmultsp_sp(dim_t anrows, dim_t ancols, dim_t bncols,
 int gooseberry_matrix_spmultsp(gooseberry_matrix_t a, gooseberry_matrix_t b,
     const gooseberry_plan_t *plan, gooseberry_matrix_t *r_c)
 {
+  GB_API_GUARD();
   if (!a || !b || !r_c) return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t anr, anc, bnr, bnc;
   gbd_matrix_info(a->m, &anr, &anc, NULL, NULL, NULL, NULL, NULL, NULL);
@@ -507,6 +574,7 @@ int gooseberry_matrix_spmultsp(gooseberry_matrix_t a, gooseberry_matrix_t b,
 int gooseberry_matrix_spmultsp_dense(gooseberry_matrix_t a,
     gooseberry_matrix_t b, real_t *d_c)
 {
+  GB_API_GUARD();
   if (!a || !b) return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t anr, anc, bnr, bnc;
   gbd_matrix_info(a->m, &anr, &anc, NULL, NULL, NULL, NULL, NULL, NULL);
@@ -534,6 +602,7 @@ int gooseberry_permute_cuthillmckee(dim_t nrows, dim_t ncols,
     const ind_t *rowptr, const dim_t *rowind, const real_t *rowval,
     const dim_t *blocks, dim_t nblocks, dim_t *perm)
 {
+  GB_API_GUARD();
   (void)rowval;
   if (nrows != ncols) {
     fprintf(stderr, "Cuthill-McKee requires a structurally symmetric matrix: "
@@ -564,6 +633,7 @@ int gooseberry_permute_cuthillmckee(dim_t nrows, dim_t ncols,
 int gooseberry_matrix_cuthillmckee(gooseberry_matrix_t mat,
     const dim_t *blocks, dim_t nblocks, dim_t *d_perm)
 {
+  GB_API_GUARD();
   if (!mat || !d_perm) return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t nrows, ncols;
   gbd_matrix_info(mat->m, &nrows, &ncols, NULL, NULL, NULL, NULL, NULL, NULL);
@@ -601,36 +671,26 @@ static uint32_t gb_cg_next_batch(uint32_t k, uint64_t it0, double r0,
   return (uint32_t)want;
 }
 
-int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
-    real_t *d_x, real_t tol, int tol_mode, size_t niter, void *log,
-    gooseberry_cgd_info_t *info)
+/* The solve loop on a workspace: start, then batches of iterations (graph
+ * replays, or single launches for a workspace in the scratch array) and a
+ * poll after each. */
+static int cgd_loop(gbd_cg *cg, const real_t *d_b, real_t *d_x, real_t tol,
+    int tol_mode, size_t niter, FILE *flog, gooseberry_cgd_info_t *info)
 {
-  if (!mat || !d_b || !d_x) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  if (tol_mode != GOOSEBERRY_TOL_ABSOLUTE && tol_mode != GOOSEBERRY_TOL_RELATIVE)
-    return GOOSEBERRY_ERROR_INVALIDINPUT;
-  uint32_t nrows, ncols;
-  gbd_matrix_info(mat->m, &nrows, &ncols, NULL, NULL, NULL, NULL, NULL, NULL);
-  if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  int rv;
-  if (!mat->cg) {
-    rv = gbd_cg_create(mat->m, &mat->cg);
-    if (rv != GOOSEBERRY_SUCCESS) return rv;
-  }
-  FILE *flog = (FILE *)log;
-  rv = gbd_cg_start(mat->cg, d_b, d_x, tol,
+  int rv = gbd_cg_start(cg, d_b, d_x, tol,
       tol_mode == GOOSEBERRY_TOL_RELATIVE, niter, GB_CG_TRACE);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   gbd_cg_state st;
   double trace[GB_CG_TRACE];
   uint64_t printed = 0;
-  rv = gbd_cg_poll(mat->cg, &st, NULL);
+  rv = gbd_cg_poll(cg, &st, NULL);
   uint32_t k = GB_CG_FIRST;
   while (rv == GOOSEBERRY_SUCCESS && !st.done) {
     const uint64_t it0 = st.iter;
     const double r0 = st.rnorm;
-    rv = gbd_cg_enqueue_graph(mat->cg, k);
+    rv = gbd_cg_enqueue_graph(cg, k);
     if (rv != GOOSEBERRY_SUCCESS) break;
-    rv = gbd_cg_poll(mat->cg, &st, flog ? trace : NULL);
+    rv = gbd_cg_poll(cg, &st, flog ? trace : NULL);
     if (rv != GOOSEBERRY_SUCCESS) break;
     if (flog) {
       for (; printed < st.iter; ++printed) {
@@ -654,10 +714,35 @@ int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
   return st.status == 3 ? GOOSEBERRY_ERROR_BREAKDOWN : GOOSEBERRY_SUCCESS;
 }
 
+static int cgd_args_ok(gooseberry_matrix_t mat, const real_t *d_b,
+    const real_t *d_x, int tol_mode)
+{
+  if (!mat || !d_b || !d_x) return 0;
+  if (tol_mode != GOOSEBERRY_TOL_ABSOLUTE && tol_mode != GOOSEBERRY_TOL_RELATIVE)
+    return 0;
+  uint32_t nrows, ncols;
+  gbd_matrix_info(mat->m, &nrows, &ncols, NULL, NULL, NULL, NULL, NULL, NULL);
+  return nrows == ncols;
+}
+
+int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
+    real_t *d_x, real_t tol, int tol_mode, size_t niter, void *log,
+    gooseberry_cgd_info_t *info)
+{
+  GB_API_GUARD();
+  if (!cgd_args_ok(mat, d_b, d_x, tol_mode)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!mat->cg) {
+    int rv = gbd_cg_create(mat->m, &mat->cg);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  return cgd_loop(mat->cg, d_b, d_x, tol, tol_mode, niter, (FILE *)log, info);
+}
+
 int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,
     const dim_t *rowind, const real_t *rowval, const real_t *b, real_t *x,
     real_t error, size_t niter, gooseberry_cgd_info_t *info)
 {
+  GB_API_GUARD();
   /* src/cgd.c:51-65 shape checks (b and x are nrows/ncols long here) */
   if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (!b || !x) return GOOSEBERRY_ERROR_INVALIDINPUT;
@@ -689,6 +774,7 @@ int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,
 int gooseberry_countingsort_kv(const dim_t *keys, const dim_t *vals,
     dim_t kmin, dim_t kmax, size_t n, dim_t *out, dim_t *dist)
 {
+  GB_API_GUARD();
   if (kmax < kmin || (n && (!keys || !out))) return GOOSEBERRY_ERROR_INVALIDINPUT;
   void *dk = NULL, *dv = NULL, *dout = NULL, *dd = NULL;
   size_t nd = (size_t)kmax - kmin + 2;
@@ -721,6 +807,7 @@ static int check_perm(const dim_t *perm, dim_t n)
 int gooseberry_permute_sparse(dim_t nrows, dim_t ncols, ind_t *rowptr,
     dim_t *rowind, real_t *rowval, const dim_t *rowperm, const dim_t *colperm)
 {
+  GB_API_GUARD();
   int rv = gb_check_csr(nrows, ncols, rowptr, rowind);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   if ((rv = check_perm(rowperm, nrows)) != GOOSEBERRY_SUCCESS) return rv;
@@ -753,6 +840,7 @@ int gooseberry_permute_sparse(dim_t nrows, dim_t ncols, ind_t *rowptr,
 int gooseberry_permute_dense(dim_t nrows, dim_t ncols, real_t *rowval,
     const dim_t *rowperm, const dim_t *colperm, int rev)
 {
+  GB_API_GUARD();
   int rv;
   if ((rv = check_perm(rowperm, nrows)) != GOOSEBERRY_SUCCESS) return rv;
   if ((rv = check_perm(colperm, ncols)) != GOOSEBERRY_SUCCESS) return rv;
@@ -779,6 +867,7 @@ int gooseberry_matrix_pagerank(gooseberry_matrix_t mat, real_t damping,
     size_t niter, real_t minerror, real_t *d_out,
     gooseberry_pagerank_info_t *info)
 {
+  GB_API_GUARD();
   if (!mat || !d_out) return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint64_t it = 0;
   double rmse = 0;
@@ -796,6 +885,7 @@ int gooseberry_pagerank(dim_t n, const ind_t *rowptr, const dim_t *rowind,
     const real_t *rowval, real_t damping, size_t niter, real_t minerror,
     real_t *out, gooseberry_pagerank_info_t *info)
 {
+  GB_API_GUARD();
   if (!out || n == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gooseberry_matrix_t m = NULL;
   void *d = NULL;

@@ -33,6 +33,7 @@ struct gooseberry_batch_s {
 int gooseberry_batch_create(uint32_t max_systems, double delta,
     gooseberry_batch_t *r_batch)
 {
+  GB_API_GUARD();
   if (!r_batch || max_systems == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
   struct gooseberry_batch_s *B = calloc(1, sizeof(*B));
   if (!B) return GOOSEBERRY_ERROR_NOMEMORY;
@@ -54,6 +55,7 @@ int gooseberry_batch_create(uint32_t max_systems, double delta,
 
 int gooseberry_batch_free(gooseberry_batch_t B)
 {
+  GB_API_GUARD();
   if (!B) return GOOSEBERRY_SUCCESS;
   gbd_batch_destroy(B->b);
   free(B->states);
@@ -64,6 +66,7 @@ int gooseberry_batch_free(gooseberry_batch_t B)
 
 size_t gooseberry_batch_workspace_bytes(gooseberry_matrix_t mat, int precision)
 {
+  GB_API_GUARD();
   return mat ? gbd_batch_workspace_bytes(gb_matrix_dev(mat), precision) : 0;
 }
 
@@ -71,6 +74,7 @@ int gooseberry_batch_add(gooseberry_batch_t B, gooseberry_matrix_t mat,
     const real_t *d_b, real_t *d_x, real_t tol, int tol_mode, size_t niter,
     int precision, uint32_t *r_slot)
 {
+  GB_API_GUARD();
   if (!B || !mat || !d_b || !d_x || !r_slot) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (tol_mode != GOOSEBERRY_TOL_ABSOLUTE && tol_mode != GOOSEBERRY_TOL_RELATIVE)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
@@ -92,6 +96,7 @@ int gooseberry_batch_add(gooseberry_batch_t B, gooseberry_matrix_t mat,
 
 int gooseberry_batch_remove(gooseberry_batch_t B, uint32_t slot)
 {
+  GB_API_GUARD();
   if (!B || slot >= B->max) return GOOSEBERRY_ERROR_INVALIDINPUT;
   int rv = gbd_batch_remove(B->b, slot);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
@@ -103,6 +108,7 @@ int gooseberry_batch_remove(gooseberry_batch_t B, uint32_t slot)
 int gooseberry_batch_run(gooseberry_batch_t B, uint32_t steps,
     uint32_t *r_running)
 {
+  GB_API_GUARD();
   if (!B) return GOOSEBERRY_ERROR_INVALIDINPUT;
   int rv;
   uint32_t running = 0;
@@ -131,6 +137,7 @@ int gooseberry_batch_run(gooseberry_batch_t B, uint32_t steps,
 
 int gooseberry_batch_solve(gooseberry_batch_t B)
 {
+  GB_API_GUARD();
   uint32_t running = 1;
   int rv = GOOSEBERRY_SUCCESS;
   uint32_t k = 8;
@@ -144,6 +151,7 @@ int gooseberry_batch_solve(gooseberry_batch_t B)
 int gooseberry_batch_info(gooseberry_batch_t B, uint32_t slot,
     gooseberry_cgd_info_t *info)
 {
+  GB_API_GUARD();
   if (!B || !info || slot >= B->max || !gbd_batch_slot_used(B->b, slot))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   const gbd_bstate *s = &B->states[slot];
@@ -169,6 +177,7 @@ int gooseberry_matrix_generate_spd(dim_t n, uint64_t seed, int law,
     double zipf_s, dim_t dmin, dim_t dmax, const gooseberry_plan_t *plan,
     gooseberry_matrix_t *r_mat)
 {
+  GB_API_GUARD();
   if (!r_mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t *rp = NULL, *ri = NULL, nnz = 0;
   double *rv_ = NULL;
@@ -180,6 +189,7 @@ int gooseberry_matrix_generate_spd(dim_t n, uint64_t seed, int law,
 int gooseberry_matrix_generate_sdd(dim_t n, uint64_t seed, dim_t dmin,
     dim_t dmax, const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
 {
+  GB_API_GUARD();
   return gooseberry_matrix_generate_spd(n, seed, GOOSEBERRY_DEGREE_UNIFORM,
       0.0, dmin, dmax, plan, r_mat);
 }
@@ -187,6 +197,7 @@ int gooseberry_matrix_generate_sdd(dim_t n, uint64_t seed, dim_t dmin,
 int gooseberry_matrix_download(gooseberry_matrix_t mat, ind_t *rowptr,
     dim_t *rowind, real_t *rowval)
 {
+  GB_API_GUARD();
   if (!mat || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gbd_matrix *m = gb_matrix_dev(mat);
   uint32_t nrows;
@@ -203,6 +214,7 @@ int gooseberry_matrix_download(gooseberry_matrix_t mat, ind_t *rowptr,
 int gooseberry_generate_uniform_device(uint64_t seed, uint64_t stream,
     size_t n, real_t lo, real_t hi, real_t *d_out)
 {
+  GB_API_GUARD();
   if (n && !d_out) return GOOSEBERRY_ERROR_INVALIDINPUT;
   return gbd_fill_uniform(d_out, n, seed, stream, lo, hi);
 }
@@ -270,6 +282,7 @@ static size_t req_transient(const req_t *r)
 int gooseberry_server_create(size_t mem_budget, uint32_t max_active,
     int precision, double delta, int keep_results, gooseberry_server_t *r)
 {
+  GB_API_GUARD();
   if (!r || max_active == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (precision != GOOSEBERRY_PRECISION_FP64 && precision != GOOSEBERRY_PRECISION_MIXED)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
@@ -305,13 +318,15 @@ static void req_release(req_t *r)
 
 int gooseberry_server_free(gooseberry_server_t S)
 {
+  GB_API_GUARD();
   if (!S) return GOOSEBERRY_SUCCESS;
   gooseberry_device_synchronize();
+  /* the batch first: its slots hold the active requests' matrices */
+  gooseberry_batch_free(S->batch);
   for (size_t i = S->qhead; i < S->qtail; ++i) free(S->queue[i]);
   for (uint32_t i = 0; i < S->max_active; ++i)
     if (S->slots[i]) req_release(S->slots[i]);
   for (size_t i = 0; i < S->nkept; ++i) req_release(S->kept[i]);
-  gooseberry_batch_free(S->batch);
   free(S->queue);
   free(S->slots);
   free(S->kept);
@@ -346,6 +361,7 @@ static int enqueue(struct gooseberry_server_s *S, req_t *r)
 int gooseberry_server_submit_sdd(gooseberry_server_t S, uint64_t id, dim_t n,
     uint64_t seed, dim_t dmin, dim_t dmax, real_t tol)
 {
+  GB_API_GUARD();
   if (!S || n < 2 || dmin < 1 || dmax < dmin || dmax > 65535 || !(tol > 0))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   req_t *r = calloc(1, sizeof(*r));
@@ -365,6 +381,7 @@ int gooseberry_server_submit_sdd(gooseberry_server_t S, uint64_t id, dim_t n,
 int gooseberry_server_submit(gooseberry_server_t S, uint64_t id,
     gooseberry_matrix_t mat, const real_t *d_b, real_t *d_x, real_t tol)
 {
+  GB_API_GUARD();
   if (!S || !mat || !d_b || !d_x || !(tol > 0)) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gooseberry_matrix_info_t mi;
   int rv = gooseberry_matrix_info(mat, &mi);
@@ -423,6 +440,7 @@ fail:
 int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
     gooseberry_request_result_t *done, uint32_t max_done, uint32_t *r_ndone)
 {
+  GB_API_GUARD();
   if (!S) return GOOSEBERRY_ERROR_INVALIDINPUT;
   int rv;
   /* 1. admission: first in, first out, while the head fits */
@@ -490,6 +508,7 @@ int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
 int gooseberry_server_status(gooseberry_server_t S, uint32_t *r_active,
     uint32_t *r_queued, size_t *r_mem)
 {
+  GB_API_GUARD();
   if (!S) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (r_active) *r_active = S->nactive;
   if (r_queued) *r_queued = (uint32_t)(S->qtail - S->qhead);
@@ -500,6 +519,7 @@ int gooseberry_server_status(gooseberry_server_t S, uint32_t *r_active,
 int gooseberry_server_take_result(gooseberry_server_t S, uint64_t id,
     real_t *h_x, size_t n)
 {
+  GB_API_GUARD();
   if (!S) return GOOSEBERRY_ERROR_INVALIDINPUT;
   for (size_t i = 0; i < S->nkept; ++i) {
     req_t *r = S->kept[i];
@@ -519,6 +539,7 @@ int gooseberry_server_take_result(gooseberry_server_t S, uint64_t id,
 int gooseberry_matrix_spmv_f32(gooseberry_matrix_t mat, const float *d_x,
     float *d_y)
 {
+  GB_API_GUARD();
   if (!mat || !d_x || !d_y) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gbd_matrix *m = gb_matrix_dev(mat);
   int rv = gbd_matrix_make_f32(m);
@@ -530,6 +551,7 @@ int gooseberry_matrix_cgd_mixed(gooseberry_matrix_t mat, const real_t *d_b,
     real_t *d_x, real_t tol, int tol_mode, size_t niter,
     gooseberry_cgd_info_t *info)
 {
+  GB_API_GUARD();
   if (!mat || !d_b || !d_x) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gooseberry_batch_t B = NULL;
   uint32_t slot = 0;
@@ -551,6 +573,7 @@ int gooseberry_spmult_f32(dim_t anrows, dim_t ancols, dim_t bncols,
     const ind_t *arowptr, const dim_t *arowind, const float *arowval,
     const float *bcolval, float *crowval, const dim_t *blocks, dim_t nblocks)
 {
+  GB_API_GUARD();
   dim_t rlo = 0, rhi = anrows;
   if (blocks) {
     for (dim_t c = 0; c < nblocks; ++c)

@@ -20,4 +20,14 @@ int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
  * it announces); leaves f at the row pointers */
 int gb_bcsr_header(FILE *f, dim_t *nrows, dim_t *ncols, uint64_t *nnz);
 
+/* One recursive library lock serialises the public entry points: the
+ * library stream, its cached graphs and workspaces and the block cache are
+ * shared by every caller.  GB_API_GUARD() takes it for the rest of the
+ * enclosing function and releases it on every return path. */
+void gb_api_lock(void);
+void gb_api_unlock(int *scope);
+#define GB_API_GUARD()                                                    \
+  int gb_api_scope_ __attribute__((cleanup(gb_api_unlock), unused)) =     \
+      (gb_api_lock(), 0)
+
 #endif

@@ -177,6 +177,7 @@ int gooseberry_read_sparse_matrix_device(int type, const char *file,
     dim_t *nrows, dim_t *ncols, ind_t **d_rowptr, dim_t **d_rowind,
     real_t **d_rowval)
 {
+  GB_API_GUARD();
   ind_t nnz;
   return read_device(type, file, nrows, ncols, d_rowptr, d_rowind, d_rowval,
       &nnz);
@@ -185,6 +186,7 @@ int gooseberry_read_sparse_matrix_device(int type, const char *file,
 int gooseberry_parse_csr_device(const char *d_text, size_t len, dim_t *nrows,
     dim_t *ncols, ind_t **d_rowptr, dim_t **d_rowind, real_t **d_rowval)
 {
+  GB_API_GUARD();
   if ((len && !d_text) || !nrows || !ncols || !d_rowptr || !d_rowind ||
       !d_rowval)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
@@ -196,6 +198,7 @@ int gooseberry_parse_csr_device(const char *d_text, size_t len, dim_t *nrows,
 int gooseberry_matrix_read(int type, const char *file,
     const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
 {
+  GB_API_GUARD();
   if (!r_mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
   dim_t nrows = 0, ncols = 0;
   ind_t *rp = NULL, nnz = 0;

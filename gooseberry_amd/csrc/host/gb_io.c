@@ -958,6 +958,7 @@ int gooseberry_read_partition(const char *const file, dim_t *const r_nrows,
     dim_t *const r_nparts, dim_t **const r_map, dim_t **const r_perm,
     dim_t **const r_dist)
 {
+  GB_API_GUARD();
   dim_t nrows = r_nrows ? *r_nrows : 0;
   dim_t *map = NULL;
   int rv = gooseberry_read_labels(file, &nrows, &map);

@@ -69,9 +69,12 @@ static uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
     uint32_t reps, gooseberry_plan_t *r_plan, double *r_ms)
 {
+  GB_API_GUARD();
   if (!mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (precision != GOOSEBERRY_PRECISION_FP64 && precision != GOOSEBERRY_PRECISION_MIXED)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* a batch slot holds the current plan's arrays */
+  if (gbd_matrix_holders(gb_matrix_dev(mat))) return GOOSEBERRY_ERROR_INVALIDINPUT;
   tune_ctx c;
   memset(&c, 0, sizeof(c));
   c.m = gb_matrix_dev(mat);

@@ -16,7 +16,15 @@
  * and copy out before they return.  Functions taking device pointers
  * (named d_*) enqueue work on the library stream; results are ready after
  * gooseberry_device_synchronize() or any function that returns host data.
- * Calls may come from one host thread at a time.
+ *
+ * Threads: any host thread may call any function.  One library lock
+ * serialises the calls (the stream, the block cache, the graphs and the
+ * workspaces kept with matrices are shared), so a call from a second
+ * thread starts when the running call returns; the work a d_* call enqueues
+ * is ordered on the one stream.  Host-pointer calls such as gooseberry_spmult
+ * own no state between calls and may be made from several threads at once
+ * (they then take turns on the GPU), as the reference's may.  Only the
+ * file readers and writers (host work) run without the lock.
  */
 #ifndef GOOSEBERRY_H
 #define GOOSEBERRY_H
@@ -103,10 +111,15 @@ int gooseberry_add_scalar(
 /* C = A x B (include/gooseberry.h:174-184; src/gooseberry.c:53-67,
  * src/blas.c:552-606).  A is CSR (anrows x ancols), B is column-major
  * (ancols x bncols, i.e. B^t rows), C row-major (anrows x bncols).  All
- * pointers are host pointers owned by the caller.  blocks/nblocks only
- * steer the reference's thread scheduling and do not change results; they
- * are validated and otherwise ignored.  Each row sum of a row with at most
- * 64 entries is bitwise the reference's left-to-right sum. */
+ * pointers are host pointers owned by the caller.  blocks (nblocks + 1
+ * non-decreasing row boundaries, the last at most anrows) select the rows
+ * the reference computes: rows [blocks[0], blocks[nblocks]) of C are
+ * written and the other rows of crowval are left as they were; NULL
+ * computes every row.  Each row sum of a row with at most 64 entries is
+ * bitwise the reference's left-to-right sum.  Every call uploads A, builds
+ * its SpMV plan and frees it again: callers that multiply by the same A
+ * repeatedly should keep it resident (gooseberry_matrix_create,
+ * gooseberry_matrix_spmv). */
 int gooseberry_spmult(
     dim_t anrows,
     dim_t ancols,
@@ -381,8 +394,12 @@ int gooseberry_matrix_read(
     const gooseberry_plan_t * plan,
     gooseberry_matrix_t * mat);
 
+/* GOOSEBERRY_ERROR_INVALIDINPUT (and nothing freed) while a batch or a
+ * server holds the matrix in one of its slots: remove the solve first. */
 int gooseberry_matrix_free(gooseberry_matrix_t mat);
 
+/* Rebuild the SpMV plan (NULL: defaults).  INVALIDINPUT while a batch or a
+ * server holds the matrix (its slot reads the plan's arrays). */
 int gooseberry_matrix_replan(
     gooseberry_matrix_t mat,
     const gooseberry_plan_t * plan);
@@ -391,8 +408,8 @@ int gooseberry_matrix_replan(
  * histogram (HIP events over `reps` SpMVs each, 0 = 5; FP64 SpMV, or FP32
  * for GOOSEBERRY_PRECISION_MIXED) and keep the fastest; *r_plan and *r_ms
  * (may be NULL) receive it and its SpMV time.  Results do not depend on the
- * plan beyond the SpMV's rounding tolerance.  Not while the matrix is in a
- * batch or a server (they hold its plan arrays). */
+ * plan beyond the SpMV's rounding tolerance.  INVALIDINPUT while the matrix
+ * is in a batch or a server (they hold its plan arrays). */
 int gooseberry_matrix_autotune(
     gooseberry_matrix_t mat,
     int precision,

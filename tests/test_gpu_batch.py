@@ -336,3 +336,44 @@ def test_batch_index16_and_wide_plans_bitwise(gb, precision):
         assert ia.status == 1
         assert ia.niter == ib.niter == ic.niter
         assert np.array_equal(xa, xb) and np.array_equal(xa, xc)
+
+
+def test_held_matrix_refuses_replan_autotune_and_free(gb):
+    # a batch slot copies the plan's arrays: while it holds the matrix the
+    # plan may not be rebuilt and the matrix may not be freed
+    M, _ = lap(gb, 2, 40)
+    n = M.nrows
+    db = gb.DeviceArray.from_host(gb.uniform(2, n))
+    dx = gb.DeviceArray(n)
+    B = gb.Batch(2)
+    slot = B.add(M, db, dx, 1e-8, gb.TOL_RELATIVE, 0, gb.PRECISION_FP64)
+    for call in (lambda: M.replan({"short_max": 32}),
+                 lambda: M.autotune(gb.PRECISION_FP64, 1),
+                 lambda: M.free()):
+        with pytest.raises(gb.GooseberryError) as e:
+            call()
+        assert e.value.name == "INVALIDINPUT"
+    B.solve()
+    assert B.info(slot).status == 1
+    x1 = dx.numpy()
+    # after the slot is removed, all three work again and the solve repeats
+    B.remove(slot)
+    M.replan({"short_max": 32})
+    M.replan(None)
+    dx2 = gb.DeviceArray(n)
+    slot = B.add(M, db, dx2, 1e-8, gb.TOL_RELATIVE, 0, gb.PRECISION_FP64)
+    B.solve()
+    assert np.array_equal(dx2.numpy(), x1)
+    B.free()  # releases its slots
+    M.free()
+
+
+def test_server_free_releases_held_matrices(gb):
+    # the server's active requests hold their matrices until it is freed
+    S = gb.Server(max_active=4, precision=gb.PRECISION_MIXED)
+    for i in range(3):
+        S.submit_sdd(i, 6000 + 100 * i, 50 + i, 3, 9, 1e-10)
+    S.progress(1)
+    a, _, _ = S.status()
+    assert a == 3
+    S.free()
