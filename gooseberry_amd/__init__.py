@@ -44,6 +44,13 @@ FORMAT_BCSR = 8
 TOL_ABSOLUTE = 0
 TOL_RELATIVE = 1
 
+FAULT_ALLOC = 0
+FAULT_LAUNCH = 1
+
+REQUEST_CANCELLED = 4
+REQUEST_TIMEDOUT = 5
+REQUEST_REJECTED = 6
+
 # every extern "C" symbol declared in include/gooseberry.h
 EXPORTED = [
     "gooseberry_scale", "gooseberry_spmult", "gooseberry_read_sparse_matrix",
@@ -76,10 +83,12 @@ EXPORTED = [
     "gooseberry_parse_csr_device", "gooseberry_matrix_read",
     "gooseberry_permute_cuthillmckee", "gooseberry_matrix_cuthillmckee",
     "gooseberry_matrix_spmv_f32", "gooseberry_matrix_cgd_mixed", "gooseberry_spmult_f32",
+    "gooseberry_inject_fault", "gooseberry_server_cancel", "gooseberry_server_set_timeout",
 ]
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_MIXED_JACOBI = 2
 DEGREE_UNIFORM = 0
 DEGREE_ZIPF = 1
 
@@ -200,6 +209,9 @@ def _sig(lib):
         "gooseberry_matrix_spmv": (i32, [_P, _P, _P]),
         "gooseberry_matrix_spmm": (i32, [_P, u32, _P, _P]),
         "gooseberry_matrix_cgd": (i32, [_P, _P, _P, f64, i32, sz, _P, C.POINTER(CgdInfo)]),
+        "gooseberry_inject_fault": (i32, [i32, C.c_longlong]),
+        "gooseberry_server_cancel": (i32, [_P, u64]),
+        "gooseberry_server_set_timeout": (i32, [_P, f64]),
         "gooseberry_cgd": (i32, [u32, u32, _P, _P, _P, _P, _P, f64, sz, C.POINTER(CgdInfo)]),
         "gooseberry_generate_laplacian": (i32, [i32, u32, u32, u32, _U32P, C.POINTER(_P),
                                                 C.POINTER(_P), C.POINTER(_P)]),
@@ -251,6 +263,12 @@ def lib():
         _LIB = C.CDLL(LIB_PATH)
         _sig(_LIB)
     return _LIB
+
+
+def inject_fault(kind, after):
+    """Test hook: the (after+1)-th device allocation (FAULT_ALLOC) or kernel
+    launch check (FAULT_LAUNCH) from now on fails once; after < 0 disarms."""
+    check(lib().gooseberry_inject_fault(kind, int(after)), "gooseberry_inject_fault")
 
 
 def check(rv, what):
@@ -940,6 +958,14 @@ class Server:
         for r in out:
             self._keep.pop(r["id"], None)
         return out
+
+    def cancel(self, rid):
+        check(lib().gooseberry_server_cancel(self.h, rid), "gooseberry_server_cancel")
+        self._keep.pop(rid, None)
+
+    def set_timeout(self, seconds):
+        check(lib().gooseberry_server_set_timeout(self.h, float(seconds)),
+              "gooseberry_server_set_timeout")
 
     def status(self):
         a, q, m = C.c_uint32(), C.c_uint32(), C.c_size_t()

@@ -36,6 +36,9 @@ int gbd_scale(double *d, size_t n, double s);
 int gbd_add_scalar(double *d, size_t n, double s);
 int gbd_sync(void);
 void *gbd_stream(void); /* the library stream as a hipStream_t */
+/* Test hook: the (after+1)-th device allocation (kind 0) or kernel launch
+ * check (kind 1) from now on reports a failure, once; after < 0 disarms. */
+int gbd_inject_fault(int kind, long long after);
 int gbd_device_memory(size_t *free_bytes, size_t *total_bytes);
 const char *gbd_last_error(void);
 
@@ -134,6 +137,9 @@ const double *gbd_matrix_rowval(const gbd_matrix *m);
 /* FP32 copy of the values for the mixed-precision solver (made on first
  * request, kept with the matrix). */
 int gbd_matrix_make_f32(gbd_matrix *m);
+/* FP32 values of the Jacobi-scaled matrix D^-1/2 A D^-1/2 and D^-1/2
+ * (square matrices with a positive, finite diagonal; INVALIDINPUT else). */
+int gbd_matrix_make_f32_jacobi(gbd_matrix *m);
 
 /* y = A x (FP64).  bitwise the reference's row-sequential sums for rows in
  * LDS-staged blocks; tree-reduced for longer rows. */

@@ -29,6 +29,11 @@
 //            FP64 solution x and the next step recomputes r = b - A x with
 //            the FP64 matrix; the Krylov direction p is kept.  Convergence
 //            is only declared on the FP64 residual.
+//   MIXED_JACOBI -- MIXED with the inner FP32 solve on the Jacobi-scaled
+//            system D^-1/2 A D^-1/2 (D = diag(A)): the inner vectors live in
+//            the scaled space (r^ = D^-1/2 r, x = D^-1/2 x^), the FP64
+//            refreshes use A itself, and the refresh test compares the
+//            scaled recurrence residual with its value at the last refresh.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -77,6 +82,7 @@ struct gb_bsys {
   double *part2;    // nchunks partials of K3
   double *part3;    // nchunks partials of ||b - A x||^2 (split matrices)
   gbd_bstate *st;
+  const double *jscale;  // MIXED_JACOBI: D^-1/2, else null
   int32_t precision;
   int32_t pad;
 };
@@ -350,7 +356,7 @@ gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
       if (mode == GB_MODE_REFRESH) {
         double *yp = (double *)S.ypart + (size_t)r * S.n;
         dotp = gb_b_spmv_item<double, 0>(L, lw, L.val, S.x, yp, nullptr, prodd, tredd, d);
-      } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+      } else if (S.precision != GOOSEBERRY_PRECISION_FP64) {
         float *yp = (float *)S.ypart + (size_t)r * S.n;
         dotp = gb_b_spmv_item<float, 0>(L, lw, L.val32, S.p32, yp, nullptr,
                                         (float *)prodd, (float *)tredd, d);
@@ -360,7 +366,7 @@ gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
       }
     } else if (mode == GB_MODE_REFRESH) {
       dotp = gb_b_spmv_item<double, 1>(S.pl, w, S.pl.val, S.x, S.r, S.b, prodd, tredd, d);
-    } else if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+    } else if (S.precision != GOOSEBERRY_PRECISION_FP64) {
       dotp = gb_b_spmv_item<float, 0>(S.pl, w, S.pl.val32, S.p32, S.ap32, nullptr,
                                       (float *)prodd, (float *)tredd, d);
     } else {
@@ -457,7 +463,7 @@ gb_b_update(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
     const int mode = st->mode;
     const uint32_t i0 = c * GB_BCHUNK;
     double d = 0.0;
-    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+    if (S.precision != GOOSEBERRY_PRECISION_FP64) {
       if (mode == GB_MODE_REFRESH) {
         // r32 = (float) r64 from the refresh SpMV (for a split matrix r64 =
         // b - A x from the slices' FP64 partials, and its squared norm goes
@@ -477,7 +483,7 @@ gb_b_update(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
             } else {
               rr = S.r[i];
             }
-            float rf = (float)rr;
+            float rf = S.jscale ? (float)(S.jscale[i] * rr) : (float)rr;
             S.r32[i] = rf;
             d += (double)rf * (double)rf;
           }
@@ -583,6 +589,8 @@ gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items
         st->rnorm = sqrt(dnr);
         st->inner = 0;
         st->mode_next = GB_MODE_CG;
+        // the refresh test of a scaled system compares scaled residuals
+        if (S.jscale) st->rnorm_ref = st->rnorm;
       }
       continue;
     }
@@ -594,9 +602,14 @@ gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items
     const bool cap = st->maxiter && it1 >= st->maxiter;
     int status = 0;        // FP64: stop here
     bool refresh = false;  // MIXED: fold d into x, FP64 residual next step
-    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+    if (S.precision != GOOSEBERRY_PRECISION_FP64) {
       if (!isfinite(dnr)) status = 3;
-      else if (rnorm <= st->tol || rnorm <= delta * st->rnorm_ref || cap)
+      // refresh after a drop by delta, or once the recurrence residual
+      // predicts convergence (for a scaled system: it has dropped by
+      // tol / ||b - A x|| since the last refresh)
+      else if (rnorm <= delta * st->rnorm_ref || cap ||
+               (S.jscale ? rnorm * st->rnorm64 <= st->tol * st->rnorm_ref
+                         : rnorm <= st->tol))
         refresh = true;
     } else {
       if (rnorm <= st->tol) status = 1;
@@ -619,7 +632,7 @@ gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items
       }
     }
     if (status) continue;
-    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+    if (S.precision != GOOSEBERRY_PRECISION_FP64) {
       // this step's d += a p (the same float products and sums as beside
       // the r update, where it was one more read of p), then p = r + beta p.
       // A mixed system's CG step sets `done` above only on a non-finite
@@ -635,7 +648,7 @@ gb_b_pupdate(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items
           S.p32[i] = S.r32[i] + boa * pv;
           if (refresh) {
             // the FP64 solution update
-            S.x[i] = S.x[i] + (double)dn;
+            S.x[i] = S.x[i] + (S.jscale ? S.jscale[i] * (double)dn : (double)dn);
             S.d32[i] = 0.0f;
           } else {
             S.d32[i] = dn;
@@ -672,8 +685,8 @@ gb_b_init(const gb_bsys *__restrict__ sys, uint32_t slot) {
         double v = S.b[i];
         S.x[i] = 0.0;
         d += v * v;
-        if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
-          float vf = (float)v;
+        if (S.precision != GOOSEBERRY_PRECISION_FP64) {
+          float vf = S.jscale ? (float)(S.jscale[i] * v) : (float)v;
           S.r32[i] = vf;
           S.p32[i] = vf;
           S.d32[i] = 0.0f;
@@ -712,11 +725,11 @@ gb_b_init_state(const gb_bsys *__restrict__ sys, uint32_t slot, double tol,
     st->precision = S.precision;
     st->mode = GB_MODE_CG;
     st->mode_next = GB_MODE_CG;
-    if (S.precision == GOOSEBERRY_PRECISION_MIXED) {
+    if (S.precision != GOOSEBERRY_PRECISION_FP64) {
       st->dr = ff;
       st->rnorm = sqrt(ff);
       st->rnorm64 = st->rnorm0;
-      st->rnorm_ref = st->rnorm0;
+      st->rnorm_ref = S.jscale ? st->rnorm : st->rnorm0;
     } else {
       st->dr = bb;
       st->rnorm = st->rnorm0;
@@ -797,7 +810,7 @@ static gb_b_layout gb_b_plan_layout(const gbd_matrix *m, int precision) {
   L.part1 = take(np1 * 8);
   L.part2 = take(np2 * 8);
   L.r = take(n * 8);
-  if (precision == GOOSEBERRY_PRECISION_MIXED) {
+  if (precision != GOOSEBERRY_PRECISION_FP64) {
     L.d32 = take(n * 4);
     L.r32 = take(n * 4);
     L.p32 = take(n * 4);
@@ -816,7 +829,7 @@ static gb_b_layout gb_b_plan_layout(const gbd_matrix *m, int precision) {
   return L;
 }
 
-static gb_bplan gb_b_plan_of(const gbd_matrix *m, uint32_t base) {
+static gb_bplan gb_b_plan_of(const gbd_matrix *m, uint32_t base, bool jacobi) {
   gb_bplan P;
   memset(&P, 0, sizeof(P));
   P.rowptr = m->rowptr;
@@ -826,7 +839,7 @@ static gb_bplan gb_b_plan_of(const gbd_matrix *m, uint32_t base) {
   P.wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
   P.block_rows = m->rowbins + (m->nrows - m->nblockrows);
   P.val = m->val;
-  P.val32 = m->val32;
+  P.val32 = jacobi ? m->val32j : m->val32;
   P.nstream = m->nstream;
   P.nwave = m->nwave;
   P.nwaveblocks = gb_spmv_nwaveblocks(m);
@@ -918,9 +931,13 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
                   int precision, uint32_t *slot_out) {
   if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (precision != GOOSEBERRY_PRECISION_FP64 &&
-      precision != GOOSEBERRY_PRECISION_MIXED)
+      precision != GOOSEBERRY_PRECISION_MIXED &&
+      precision != GOOSEBERRY_PRECISION_MIXED_JACOBI)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
+  const bool jac = precision == GOOSEBERRY_PRECISION_MIXED_JACOBI;
   if (precision == GOOSEBERRY_PRECISION_MIXED && !m->val32)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (jac && (!m->val32j || !m->jscale || (m->nsplit && !m->sval32j)))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t slot = 0;
   while (slot < B->max_slots && B->used[slot]) ++slot;
@@ -933,7 +950,8 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   GB_HIP(gb_dev_alloc((void **)&mem, L.total));
   gb_bsys S;
   memset(&S, 0, sizeof(S));
-  S.pl = gb_b_plan_of(m, 0);
+  S.pl = gb_b_plan_of(m, 0, jac);
+  S.jscale = jac ? m->jscale : nullptr;
   S.n = m->nrows;
   S.nchunks = nch;
   S.nsplit = m->nsplit;
@@ -943,7 +961,7 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   S.part1 = (double *)(mem + L.part1);
   S.part2 = (double *)(mem + L.part2);
   S.r = (double *)(mem + L.r);
-  if (precision == GOOSEBERRY_PRECISION_MIXED) {
+  if (precision != GOOSEBERRY_PRECISION_FP64) {
     S.d32 = (float *)(mem + L.d32);
     S.r32 = (float *)(mem + L.r32);
     S.p32 = (float *)(mem + L.p32);
@@ -968,7 +986,7 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
     }
     uint32_t base = 0;
     for (uint32_t r = 0; r < m->nsplit; ++r) {
-      hsl[r] = gb_b_plan_of(m->split[r], base);
+      hsl[r] = gb_b_plan_of(m->split[r], base, jac);
       base += hsl[r].nwork;
       B->h_slice_nwork[(size_t)slot * GB_SPLIT_MAX + r] = hsl[r].nwork;
     }

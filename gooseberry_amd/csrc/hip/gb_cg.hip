@@ -1,23 +1,26 @@
 // gb_cg.hip -- device side of the conjugate-gradient solver.
 //
-// Reference: src/cgd.c:40-129.  One iteration is four kernels on the
+// Reference: src/cgd.c:40-129.  One iteration is three kernels on the
 // library stream, all scalars kept on the device (no host round trip):
 //   K1 gb_spmv_kernel<DOT>: ap = A p and the p.ap partials
-//   K2 gb_cg_alpha:         alpha = dr / p.ap
-//   K3 gb_cg_update_r:      r -= alpha ap; r.r partials
-//   K4 gb_cg_update_xp:     beta = r.r / dr, convergence test;
+//   K2 gb_cg_update_r:      alpha = dr / p.ap; r -= alpha ap; r.r partials
+//   K3 gb_cg_update_xp:     beta = r.r / dr, convergence test;
 //                           ps = alpha p; x += ps; p = r + (beta/alpha) ps
 // which is the reference's scale-then-add order (src/cgd.c:92-105) with
 // the same separately rounded products and sums (-ffp-contract=off); only
-// the order of the dot-product sums differs.  x += ps sits in K4, which
-// reads p anyway (one vector read less than in K3; x does not feed r).
-// K2-K4 first read the device `done` flag and return at once when the
-// solve has stopped, so the host can enqueue iterations in batches (or
-// replay them from a hipGraph) and still stop exactly where the reference
-// stops.  `done` is written only by the state's set-up and by K2 (one
-// workgroup): K4 records its stop in `status`, which every K4 workgroup
-// computes for itself, so all of them still add their share of x; the next
-// K2 turns a nonzero status into `done`.
+// the order of the dot-product sums differs.  x += ps sits in K3, which
+// reads p anyway (one vector read less than in K2; x does not feed r).
+// K2 and K3 give each workgroup one contiguous chunk of rows; each
+// workgroup adds the previous kernel's partials itself (the same values in
+// the same order in every workgroup, so every workgroup gets the same
+// scalar) while the loads of its chunk are in flight.
+// K2 and K3 read the device `done` flag and return when the solve has
+// stopped, so the host can enqueue iterations in batches (or replay them
+// from a hipGraph) and still stop exactly where the reference stops.
+// `done` is written only by the state's set-up and by K2's workgroup 0: K3
+// records its stop in `status`, which every K3 workgroup computes for
+// itself, so all of them still add their share of x; the next K2 turns a
+// nonzero status into `done`.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -42,7 +45,9 @@ struct gbd_cg {
   double *h_trace;     // pinned, trace_cap entries
   const double *b;
   double *x;
-  uint32_t grid_elem;
+  uint32_t grid_elem;  // workgroups of the set-up kernel
+  uint32_t grid_vec;   // workgroups of the r and x/p updates
+  uint32_t chunk;      // rows per workgroup of the r and x/p updates
   hipGraph_t graph[GB_CG_NGRAPH];       // graph[i]: 2^i iterations
   hipGraphExec_t gexec[GB_CG_NGRAPH];
   double *graph_x;                      // x the graphs were captured with
@@ -127,94 +132,183 @@ gb_cg_init_state(const double *__restrict__ partials, uint32_t np,
   }
 }
 
-// src/cgd.c:90: alpha = dr / p.Ap from the SpMV's per-row-block partials,
-// added in a fixed order by one 1024-thread workgroup; each thread issues
-// all of its loads before adding (one memory round trip per 16K partials),
-// and the first round's loads go out before the stop flag is known.
-#define GB_RED_THREADS 1024
-#define GB_RED_UNROLL 16
-__global__ void __launch_bounds__(GB_RED_THREADS)
-gb_cg_alpha(const double *__restrict__ pap_partials, uint32_t npap,
-            gbd_cg_state *st) {
-  __shared__ double red[GB_RED_THREADS / GB_WAVE];
-  __shared__ int32_t stop;
-  double v[GB_RED_UNROLL];
-#pragma unroll
-  for (int j = 0; j < GB_RED_UNROLL; ++j) {
-    uint32_t i = j * GB_RED_THREADS + threadIdx.x;
-    v[j] = i < npap ? pap_partials[i] : 0.0;
-  }
-  if (threadIdx.x == 0) {
-    // a stop recorded by the previous iteration's K4 becomes `done` here
-    stop = st->done || st->status;
-    if (stop && !st->done) st->done = 1;
-  }
-  __syncthreads();
-  if (stop) return;
+// src/cgd.c:90, 96-97: alpha = dr / p.Ap; r -= alpha Ap; partial sums of
+// r.r.  Each workgroup owns one contiguous chunk of rows and first issues
+// the loads of its r and Ap values, then adds the SpMV's p.Ap partials
+// itself -- every workgroup the same values in the same fixed order, so
+// all of them get the same alpha and no separate reduction launch is
+// needed -- while those loads are in flight.  Workgroup 0 records alpha
+// and turns a stop recorded by the previous iteration's x/p update into
+// `done`.
+// Launch shape of K2 and K3: GB_CGV_THREADS threads, each with GB_CGV_U
+// rows per pass, at most one workgroup per CU.  On the 160^3 Laplacian
+// (n = 4,096,000) every workgroup then covers its 16,000 rows in one pass,
+// with all of its loads issued at once.
+#define GB_CGV_THREADS 512
+#define GB_CGV_GRID 256u
+#define GB_CGV_U 32
+// loads per thread of the partial sums: K2 adds up to 16K SpMV partials in
+// one round, K3 the r update's (at most GB_CGV_GRID)
+#define GB_CGV_RED1 32
+#define GB_CGV_RED2 1
+
+// Fixed-order sum of n partials by a workgroup of T threads: each thread
+// issues R loads before adding (one memory round trip per T R partials);
+// the result is returned to every thread.
+template <int T, int R>
+__device__ __forceinline__ double gb_cgv_sum(const double *__restrict__ part,
+                                             uint32_t n, double *red) {
   double s = 0.0;
-  for (uint32_t base = 0;;) {
+  for (uint32_t base = 0; base < n; base += T * R) {
+    double v[R];
 #pragma unroll
-    for (int j = 0; j < GB_RED_UNROLL; ++j) s += v[j];
-    base += GB_RED_THREADS * GB_RED_UNROLL;
-    if (base >= npap) break;
-#pragma unroll
-    for (int j = 0; j < GB_RED_UNROLL; ++j) {
-      uint32_t i = base + j * GB_RED_THREADS + threadIdx.x;
-      v[j] = i < npap ? pap_partials[i] : 0.0;
+    for (int j = 0; j < R; ++j) {
+      uint32_t i = base + j * T + threadIdx.x;
+      v[j] = i < n ? part[i] : 0.0;
     }
+#pragma unroll
+    for (int j = 0; j < R; ++j) s += v[j];
   }
   s = gb_wave_sum(s);
   if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double pap = red[0];
-    for (int w = 1; w < GB_RED_THREADS / GB_WAVE; ++w) pap += red[w];
-    const double dr = st->dr;
-    const double alpha = dr / pap;
-    st->pap = pap;
-    st->alpha = alpha;
-    if (!(pap != 0.0) || !isfinite(alpha)) {
-      st->done = 1;
-      st->status = 3;
-    } else {
-      st->dr_prev = dr;
-      st->iter_next = st->iter + 1;
-    }
-  }
+  double t = red[0];
+#pragma unroll
+  for (int w = 1; w < T / GB_WAVE; ++w) t += red[w];
+  __syncthreads();
+  return t;
 }
 
-// src/cgd.c:96-97: r -= alpha Ap; partial sums of r.r
-__global__ void __launch_bounds__(GB_BLOCK)
-gb_cg_update_r(double *__restrict__ r, const double *__restrict__ ap,
-               uint32_t n, double *__restrict__ partials,
-               const gbd_cg_state *st) {
-  __shared__ double red[GB_WAVES_PER_BLOCK];
-  if (gb_block_flag(&st->done)) return;
-  const double alpha = st->alpha;
-  double d = 0.0;
-  for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
-       i += gridDim.x * GB_BLOCK) {
-    double aps = alpha * ap[i];
-    double rn = r[i] - aps;
-    r[i] = rn;
-    d += rn * rn;
+// Sum of one value per thread over a workgroup of T threads (wave
+// butterflies, then the wave totals in wave order); valid on thread 0.
+template <int T>
+__device__ __forceinline__ double gb_cgv_block_sum(double v, double *red) {
+  v = gb_wave_sum(v);
+  if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    t = red[0];
+#pragma unroll
+    for (int w = 1; w < T / GB_WAVE; ++w) t += red[w];
   }
-  double bs = gb_block_sum(d, red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+  return t;
+}
+
+template <int T, int U>
+__global__ void __launch_bounds__(T)
+gb_cg_update_r(double *__restrict__ r, const double *__restrict__ ap, uint32_t n,
+               uint32_t chunk, const double *__restrict__ pap_partials, uint32_t npap,
+               double *__restrict__ partials, gbd_cg_state *st) {
+  __shared__ double red[T / GB_WAVE];
+  __shared__ int32_t s_stop, s_bad;
+  __shared__ double s_alpha;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t i0 = blockIdx.x * chunk;  // < n: the grid is ceil(n / chunk)
+  const uint32_t len = n - i0 < chunk ? n - i0 : chunk;
+  double rv[U], av[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const uint32_t o = j * T + tid;
+    if (o < len) {
+      const uint32_t i = i0 + o;
+      rv[j] = r[i];
+      av[j] = ap[i];
+    }
+  }
+  if (tid == 0) {
+    // a stop recorded by the previous iteration's x/p update becomes `done`
+    // (workgroups that read `done` while workgroup 0 sets it see the
+    // nonzero status as well)
+    const int32_t dn = *(volatile int32_t *)&st->done;
+    const int32_t stop = dn || *(volatile int32_t *)&st->status;
+    if (stop && !dn && blockIdx.x == 0) st->done = 1;
+    s_stop = stop;
+  }
+  const double pap = gb_cgv_sum<T, GB_CGV_RED1>(pap_partials, npap, red);  // barriers inside
+  if (s_stop) return;
+  if (tid == 0) {
+    const double dr = st->dr;
+    const double alpha = dr / pap;
+    const bool bad = !(pap != 0.0) || !isfinite(alpha);
+    if (blockIdx.x == 0) {
+      st->pap = pap;
+      st->alpha = alpha;
+      if (bad) {
+        st->done = 1;
+        st->status = 3;
+      } else {
+        st->dr_prev = dr;
+        st->iter_next = st->iter + 1;
+      }
+    }
+    s_alpha = bad ? 0.0 : alpha;
+    s_bad = bad;
+  }
+  __syncthreads();
+  if (s_bad) return;
+  const double alpha = s_alpha;
+  double d = 0.0;
+  for (uint32_t done = 0;;) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t o = done + j * T + tid;
+      if (o < len) {
+        const uint32_t i = i0 + o;
+        const double aps = alpha * av[j];
+        const double rn = rv[j] - aps;
+        r[i] = rn;
+        d += rn * rn;
+      }
+    }
+    if (len - done <= U * T) break;
+    done += U * T;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t o = done + j * T + tid;
+      if (o < len) {
+        const uint32_t i = i0 + o;
+        rv[j] = r[i];
+        av[j] = ap[i];
+      }
+    }
+  }
+  const double bs = gb_cgv_block_sum<T>(d, red);
+  if (tid == 0) partials[blockIdx.x] = bs;
 }
 
 // src/cgd.c:92-94, 100-119: beta = r.r / dr; stop when sqrt(r.r) <= tol or
 // at the iteration cap; ps = alpha p; x += ps; unless stopping,
-// p = nr + (beta/alpha) ps
-__global__ void __launch_bounds__(GB_BLOCK)
+// p = nr + (beta/alpha) ps.  The workgroup's x, p and r values are loaded
+// first; the r.r partials of the r update are added (the same order in
+// every workgroup) while they are in flight.
+template <int T, int U>
+__global__ void __launch_bounds__(T)
 gb_cg_update_xp(double *__restrict__ x, const double *__restrict__ r,
-               double *__restrict__ p,
-               uint32_t n, const double *__restrict__ dnr_partials,
-               uint32_t ndnr, gbd_cg_state *st, double *__restrict__ trace,
-               uint32_t trace_cap) {
-  __shared__ double red[GB_WAVES_PER_BLOCK];
-  if (gb_block_flag(&st->done)) return;
-  const double dnr = gb_sum_partials(dnr_partials, ndnr, red);
+                double *__restrict__ p, uint32_t n, uint32_t chunk,
+                const double *__restrict__ dnr_partials, uint32_t ndnr,
+                gbd_cg_state *st, double *__restrict__ trace, uint32_t trace_cap) {
+  __shared__ double red[T / GB_WAVE];
+  __shared__ int32_t s_done;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t i0 = blockIdx.x * chunk;  // < n: the grid is ceil(n / chunk)
+  const uint32_t len = n - i0 < chunk ? n - i0 : chunk;
+  double xv[U], pv[U], rv[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const uint32_t o = j * T + tid;
+    if (o < len) {
+      const uint32_t i = i0 + o;
+      xv[j] = x[i];
+      pv[j] = p[i];
+      rv[j] = r[i];
+    }
+  }
+  // `done` is written only by the r update and the set-up, never during
+  // this kernel
+  if (tid == 0) s_done = *(volatile int32_t *)&st->done;
+  const double dnr = gb_cgv_sum<T, GB_CGV_RED2>(dnr_partials, ndnr, red);  // barriers inside
+  if (s_done) return;
   const double alpha = st->alpha;
   const double beta = dnr / st->dr_prev;
   const double rnorm = sqrt(dnr);
@@ -223,7 +317,7 @@ gb_cg_update_xp(double *__restrict__ x, const double *__restrict__ r,
   if (rnorm <= st->tol) status = 1;
   else if (!isfinite(dnr)) status = 3;
   else if (st->maxiter && it1 >= st->maxiter) status = 2;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && tid == 0) {
     st->beta = beta;
     st->dr = dnr;
     st->rnorm = rnorm;
@@ -233,18 +327,32 @@ gb_cg_update_xp(double *__restrict__ x, const double *__restrict__ r,
     if (status) st->status = status;
   }
   const double boa = beta / alpha;
-  if (status) {
-    for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
-         i += gridDim.x * GB_BLOCK)
-      x[i] = x[i] + alpha * p[i];
-    return;
-  }
-  for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n;
-       i += gridDim.x * GB_BLOCK) {
-    double ps = alpha * p[i];
-    x[i] = x[i] + ps;
-    double t = ps * boa;
-    p[i] = r[i] + t;
+  for (uint32_t done = 0;;) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t o = done + j * T + tid;
+      if (o < len) {
+        const uint32_t i = i0 + o;
+        const double ps = alpha * pv[j];
+        x[i] = xv[j] + ps;
+        if (!status) {
+          const double t = ps * boa;
+          p[i] = rv[j] + t;
+        }
+      }
+    }
+    if (len - done <= U * T) break;
+    done += U * T;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t o = done + j * T + tid;
+      if (o < len) {
+        const uint32_t i = i0 + o;
+        xv[j] = x[i];
+        pv[j] = p[i];
+        rv[j] = r[i];
+      }
+    }
   }
 }
 
@@ -255,14 +363,12 @@ static int gb_cg_enqueue_raw(gbd_cg *cg, uint32_t k) {
   for (uint32_t it = 0; it < k; ++it) {
     int rv = gb_spmv_launch(cg->m, cg->p, cg->ap, 1, cg->partials, cg->st);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
-    gb_cg_alpha<<<1, GB_RED_THREADS, 0, s>>>(cg->partials, g1, cg->st);
+    gb_cg_update_r<GB_CGV_THREADS, GB_CGV_U><<<cg->grid_vec, GB_CGV_THREADS, 0, s>>>(
+        cg->r, cg->ap, n, cg->chunk, cg->partials, g1, cg->partials2, cg->st);
     GB_LAUNCH_CHECK();
-    gb_cg_update_r<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
-        cg->r, cg->ap, n, cg->partials2, cg->st);
-    GB_LAUNCH_CHECK();
-    gb_cg_update_xp<<<cg->grid_elem, GB_BLOCK, 0, s>>>(
-        cg->x, cg->r, cg->p, n, cg->partials2, cg->grid_elem, cg->st, cg->trace,
-        cg->trace_cap);
+    gb_cg_update_xp<GB_CGV_THREADS, GB_CGV_U><<<cg->grid_vec, GB_CGV_THREADS, 0, s>>>(
+        cg->x, cg->r, cg->p, n, cg->chunk, cg->partials2, cg->grid_vec, cg->st,
+        cg->trace, cg->trace_cap);
     GB_LAUNCH_CHECK();
   }
   return GOOSEBERRY_SUCCESS;
@@ -280,26 +386,42 @@ static void gb_cg_drop_graph(gbd_cg *cg) {
 
 extern "C" {
 
+// Launch geometry and partial-array sizes of a solve with m.
+static void gb_cg_geometry(gbd_cg *cg, const gbd_matrix *m) {
+  cg->m = m;
+  cg->n = m->nrows;
+  uint32_t ge = gb_div_up(cg->n ? cg->n : 1, GB_BLOCK);
+  cg->grid_elem = ge > GB_ELEM_GRID_MAX ? GB_ELEM_GRID_MAX : ge;
+  // the vector updates: at most one workgroup per CU, each a contiguous
+  // chunk of a multiple of 64 rows (aligned wavefront accesses)
+  const uint64_t nn = cg->n ? cg->n : 1;
+  uint32_t gv = gb_div_up(nn, GB_CGV_THREADS);
+  if (gv > GB_CGV_GRID) gv = GB_CGV_GRID;
+  cg->chunk = gb_div_up(gb_div_up(nn, gv), GB_WAVE) * GB_WAVE;
+  cg->grid_vec = gb_div_up(nn, cg->chunk);
+  uint32_t gs = gb_spmv_npartials(m);
+  cg->npartials = gs > cg->grid_elem ? gs : cg->grid_elem;
+}
+
+static uint32_t gb_cg_np2(const gbd_cg *cg) {
+  return cg->grid_elem > cg->grid_vec ? cg->grid_elem : cg->grid_vec;
+}
+
 int gbd_cg_create(const gbd_matrix *m, gbd_cg **out) {
   int rv = gb_ensure_init();
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
   gbd_cg *cg = (gbd_cg *)calloc(1, sizeof(gbd_cg));
   if (!cg) return GOOSEBERRY_ERROR_NOMEMORY;
-  cg->m = m;
-  cg->n = m->nrows;
+  gb_cg_geometry(cg, m);
   size_t nb = (size_t)(cg->n ? cg->n : 1) * sizeof(double);
-  uint32_t ge = gb_div_up(cg->n ? cg->n : 1, GB_BLOCK);
-  cg->grid_elem = ge > GB_ELEM_GRID_MAX ? GB_ELEM_GRID_MAX : ge;
-  uint32_t gs = gb_spmv_npartials(m);
-  cg->npartials = gs > cg->grid_elem ? gs : cg->grid_elem;
   hipError_t e = gb_dev_alloc((void **)&cg->r, nb);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&cg->p, nb);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&cg->ap, nb);
   if (e == hipSuccess)
     e = gb_dev_alloc((void **)&cg->partials, (size_t)cg->npartials * sizeof(double));
   if (e == hipSuccess)
-    e = gb_dev_alloc((void **)&cg->partials2, (size_t)cg->grid_elem * sizeof(double));
+    e = gb_dev_alloc((void **)&cg->partials2, (size_t)gb_cg_np2(cg) * sizeof(double));
   if (e == hipSuccess) e = gb_dev_alloc((void **)&cg->st, sizeof(gbd_cg_state));
   if (e == hipSuccess)
     e = hipHostMalloc((void **)&cg->h_st, sizeof(gbd_cg_state), hipHostMallocDefault);

@@ -21,7 +21,12 @@ void gb_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
     if (gb_e_ != hipSuccess)                                                \
       return gb_hip_fail(gb_e_, #call, __FILE__, __LINE__);                 \
   } while (0)
-#define GB_LAUNCH_CHECK() GB_HIP(hipGetLastError())
+// the last launch's error, or an injected launch failure (test hook)
+hipError_t gb_launch_status();
+#define GB_FAULT_ALLOC 0
+#define GB_FAULT_LAUNCH 1
+#define GB_FAULT_KINDS 2
+#define GB_LAUNCH_CHECK() GB_HIP(gb_launch_status())
 
 hipStream_t gb_stream();
 int gb_ensure_init();

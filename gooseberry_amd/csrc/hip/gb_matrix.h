@@ -16,6 +16,9 @@ struct gbd_matrix {
   uint32_t *rowind;   // nnz
   double *val;        // nnz
   float *val32;       // nnz, made on request (mixed precision)
+  float *val32j;      // nnz, made on request: FP32 values of the Jacobi-scaled
+                      // matrix D^-1/2 A D^-1/2 (D = diag(A))
+  double *jscale;     // nrows, with val32j: D^-1/2
   gbd_plan_params params;
   // plan
   uint32_t nstream;     // LDS-staged row blocks
@@ -36,6 +39,7 @@ struct gbd_matrix {
   uint32_t *srowind;  // nnz
   double *sval;       // nnz
   float *sval32;      // nnz, with val32
+  float *sval32j;     // nnz, with val32j
   double *ypart;      // P * nrows partial results (FP64, or FP32 in place)
   bool view;          // rowptr/rowind/val belong to a parent matrix
   uint32_t holders;   // batch slots holding pointers into the plan: while

@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <list>
 #include <map>
 #include <mutex>
@@ -127,8 +128,28 @@ size_t gb_cache_cached_bytes() {
   return g_cache_bytes;
 }
 
+// Injected faults (gbd_inject_fault, a test hook): with a countdown c >= 0
+// armed for a kind, the c+1-th device allocation (or launch check) from then
+// on reports a failure instead, once; c < 0 disarms.  An injected launch
+// failure is reported after the kernel was launched normally.
+static std::atomic<long long> g_fault[GB_FAULT_KINDS] = {{-1}, {-1}};
+
+static bool gb_fault_fires(int kind) {
+  long long v = g_fault[kind].load();
+  while (v >= 0)
+    if (g_fault[kind].compare_exchange_weak(v, v - 1)) return v == 0;
+  return false;
+}
+
+hipError_t gb_launch_status() {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && gb_fault_fires(GB_FAULT_LAUNCH)) e = hipErrorLaunchFailure;
+  return e;
+}
+
 hipError_t gb_dev_alloc(void **p, size_t bytes) {
   *p = nullptr;
+  if (gb_fault_fires(GB_FAULT_ALLOC)) return hipErrorOutOfMemory;
   if (bytes < GB_CACHE_MIN) return hipMallocAsync(p, bytes ? bytes : 16, g_stream);
   const size_t r = (bytes + GB_CACHE_ROUND - 1) & ~(GB_CACHE_ROUND - 1);
   {
@@ -236,6 +257,12 @@ int gbd_scale(double *d, size_t n, double s) {
 int gbd_init(void) { return gb_ensure_init(); }
 
 const char *gbd_last_error(void) { return g_last_error; }
+
+int gbd_inject_fault(int kind, long long after) {
+  if (kind < 0 || kind >= GB_FAULT_KINDS) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  g_fault[kind].store(after < 0 ? -1 : after);
+  return GOOSEBERRY_SUCCESS;
+}
 
 void *gbd_stream(void) {
   if (gb_ensure_init() != GOOSEBERRY_SUCCESS) return nullptr;

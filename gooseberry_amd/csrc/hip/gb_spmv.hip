@@ -512,6 +512,46 @@ __global__ void gb_to_f32(const double *__restrict__ in, float *__restrict__ out
   if (i < n) out[i] = (float)in[i];
 }
 
+// Jacobi scaling, one wavefront per row: s_i = 1 / sqrt(a_ii), a_ii the sum
+// of the row's entries in column i (added in a fixed order); *bad is set
+// when a diagonal is not positive and finite.
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_jacobi_scale(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
+                const double *__restrict__ val, uint32_t n, double *__restrict__ scale,
+                uint32_t *__restrict__ bad) {
+  const uint32_t lane = threadIdx.x & (GB_WAVE - 1);
+  const uint32_t nw = gridDim.x * GB_WAVES_PER_BLOCK;
+  uint32_t b = 0;
+  for (uint32_t i = blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE; i < n; i += nw) {
+    const uint32_t k0 = rowptr[i], k1 = rowptr[i + 1];
+    double d = 0.0;
+    for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE)
+      if (rowind[k] == i) d += val[k];
+    d = gb_wave_sum(d);
+    if (lane == 0) {
+      if (d > 0.0 && isfinite(d)) scale[i] = 1.0 / sqrt(d);
+      else { scale[i] = 1.0; b = 1; }
+    }
+  }
+  if (b) atomicOr(bad, 1u);
+}
+
+// FP32 values of D^-1/2 A D^-1/2: (a_ij s_i) s_j rounded once to FP32, one
+// wavefront per row
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_jacobi_vals(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
+               const double *__restrict__ val, uint32_t n, const double *__restrict__ scale,
+               float *__restrict__ out) {
+  const uint32_t lane = threadIdx.x & (GB_WAVE - 1);
+  const uint32_t nw = gridDim.x * GB_WAVES_PER_BLOCK;
+  for (uint32_t i = blockIdx.x * GB_WAVES_PER_BLOCK + threadIdx.x / GB_WAVE; i < n; i += nw) {
+    const uint32_t k0 = rowptr[i], k1 = rowptr[i + 1];
+    const double si = scale[i];
+    for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE)
+      out[k] = (float)((val[k] * si) * scale[rowind[k]]);
+  }
+}
+
 // ------------------------------------------------------------- host side
 
 static int gb_plan_check(const gbd_plan_params *p) {
@@ -537,10 +577,11 @@ static void gb_split_free(gbd_matrix *m) {
   (void)gb_dev_free(m->srowind);
   (void)gb_dev_free(m->sval);
   (void)gb_dev_free(m->sval32);
+  (void)gb_dev_free(m->sval32j);
   (void)gb_dev_free(m->ypart);
   m->srowptr = m->srowind = nullptr;
   m->sval = m->ypart = nullptr;
-  m->sval32 = nullptr;
+  m->sval32 = m->sval32j = nullptr;
 }
 
 static int gb_plan_build(gbd_matrix *m);
@@ -574,6 +615,12 @@ static int gb_split_build(gbd_matrix *m, uint32_t P) {
                                                 m->srowptr, nullptr, m->sval32);
     GB_LAUNCH_CHECK();
   }
+  if (m->val32j) {
+    GB_HIP(gb_dev_alloc((void **)&m->sval32j, (m->nnz ? m->nnz : 1) * 4));
+    gb_split_fill<float><<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->val32j, n, width, P,
+                                                m->srowptr, nullptr, m->sval32j);
+    GB_LAUNCH_CHECK();
+  }
   // slice boundaries for the views' nonzero counts
   for (uint32_t r = 0; r <= P; ++r)
     GB_HIP(hipMemcpyAsync(&bounds[r], m->srowptr + (size_t)r * n, 4, hipMemcpyDeviceToHost, s));
@@ -593,6 +640,7 @@ static int gb_split_build(gbd_matrix *m, uint32_t P) {
     v->rowind = m->srowind;
     v->val = m->sval;
     v->val32 = m->sval32;
+    v->val32j = m->sval32j;
     v->params = m->params;
     v->params.csplit = 0;
     if ((rv = gb_plan_build(v)) != GOOSEBERRY_SUCCESS) return rv;
@@ -796,6 +844,8 @@ void gbd_matrix_destroy(gbd_matrix *m) {
   if (m->rowind) (void)gb_dev_free(m->rowind);
   if (m->val) (void)gb_dev_free(m->val);
   if (m->val32) (void)gb_dev_free(m->val32);
+  if (m->val32j) (void)gb_dev_free(m->val32j);
+  if (m->jscale) (void)gb_dev_free(m->jscale);
   if (m->blk) (void)gb_dev_free(m->blk);
   if (m->rowbins) (void)gb_dev_free(m->rowbins);
   if (m->dcol) (void)gb_dev_free(m->dcol);
@@ -841,6 +891,49 @@ int gbd_matrix_make_f32(gbd_matrix *m) {
       GB_LAUNCH_CHECK();
     }
     for (uint32_t r = 0; r < m->nsplit; ++r) m->split[r]->val32 = m->sval32;
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_matrix_make_f32_jacobi(gbd_matrix *m) {
+  if (m->val32j) return GOOSEBERRY_SUCCESS;
+  hipStream_t s = gb_stream();
+  const uint32_t n = m->nrows;
+  if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint32_t *bad = nullptr, hbad = 0;
+  GB_HIP(gb_dev_alloc((void **)&m->jscale, (size_t)(n ? n : 1) * sizeof(double)));
+  GB_HIP(gb_dev_alloc((void **)&bad, 4));
+  GB_HIP(hipMemsetAsync(bad, 0, 4, s));
+  const uint32_t g = n ? (gb_div_up(n, GB_WAVES_PER_BLOCK) > 16384u
+                              ? 16384u : gb_div_up(n, GB_WAVES_PER_BLOCK)) : 0;
+  if (n) {
+    gb_jacobi_scale<<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->val, n, m->jscale, bad);
+    GB_LAUNCH_CHECK();
+  }
+  GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  GB_HIP(gb_dev_free(bad));
+  if (hbad) {
+    GB_HIP(gb_dev_free(m->jscale));
+    m->jscale = nullptr;
+    gb_set_error("Jacobi scaling needs a positive, finite diagonal in every row");
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  GB_HIP(gb_dev_alloc((void **)&m->val32j, (m->nnz ? m->nnz : 1) * sizeof(float)));
+  if (n && m->nnz) {
+    gb_jacobi_vals<<<g, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->val, n, m->jscale,
+                                          m->val32j);
+    GB_LAUNCH_CHECK();
+  }
+  if (m->nsplit) {
+    // the slices' copy in slice order, as the split build makes it
+    const uint32_t P = m->nsplit;
+    const uint32_t width = (m->ncols + P - 1) / P;
+    GB_HIP(gb_dev_alloc((void **)&m->sval32j, (m->nnz ? m->nnz : 1) * sizeof(float)));
+    gb_split_fill<float><<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(
+        m->rowptr, m->rowind, m->val32j, n, width, P, m->srowptr, nullptr, m->sval32j);
+    GB_LAUNCH_CHECK();
+    for (uint32_t r = 0; r < P; ++r) m->split[r]->val32j = m->sval32j;
   }
   return GOOSEBERRY_SUCCESS;
 }

@@ -64,6 +64,11 @@ const char *gooseberry_strerror(int err)
 
 const char *gooseberry_last_device_error(void) { return gbd_last_error(); }
 
+int gooseberry_inject_fault(int kind, long long after)
+{
+  return gbd_inject_fault(kind, after);
+}
+
 /* --------------------------------------------------------- device memory */
 
 int gooseberry_device_alloc(void **d_ptr, size_t bytes)
@@ -672,13 +677,13 @@ static uint32_t gb_cg_next_batch(uint32_t k, uint64_t it0, double r0,
 }
 
 /* The solve loop on a workspace: start, then batches of iterations (graph
- * replays, or single launches for a workspace in the scratch array) and a
- * poll after each. */
+ * replays) and a poll after each. */
 static int cgd_loop(gbd_cg *cg, const real_t *d_b, real_t *d_x, real_t tol,
     int tol_mode, size_t niter, FILE *flog, gooseberry_cgd_info_t *info)
 {
+  /* the per-iteration residuals are kept on the device only for a log */
   int rv = gbd_cg_start(cg, d_b, d_x, tol,
-      tol_mode == GOOSEBERRY_TOL_RELATIVE, niter, GB_CG_TRACE);
+      tol_mode == GOOSEBERRY_TOL_RELATIVE, niter, flog ? GB_CG_TRACE : 0);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   gbd_cg_state st;
   double trace[GB_CG_TRACE];

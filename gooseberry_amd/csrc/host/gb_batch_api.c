@@ -83,6 +83,9 @@ int gooseberry_batch_add(gooseberry_batch_t B, gooseberry_matrix_t mat,
   if (precision == GOOSEBERRY_PRECISION_MIXED &&
       (rv = gbd_matrix_make_f32(m)) != GOOSEBERRY_SUCCESS)
     return rv;
+  if (precision == GOOSEBERRY_PRECISION_MIXED_JACOBI &&
+      (rv = gbd_matrix_make_f32_jacobi(m)) != GOOSEBERRY_SUCCESS)
+    return rv;
   uint32_t slot;
   rv = gbd_batch_add(B->b, m, d_b, d_x, tol, tol_mode == GOOSEBERRY_TOL_RELATIVE,
       niter, precision, &slot);
@@ -161,7 +164,7 @@ int gooseberry_batch_info(gooseberry_batch_t B, uint32_t slot,
   info->bnorm = s->rnorm0;
   info->status = s->status;
   info->converged = s->status == 1;
-  if (s->precision == GOOSEBERRY_PRECISION_MIXED) {
+  if (s->precision != GOOSEBERRY_PRECISION_FP64) {
     info->rnorm = s->rnorm64;
     info->true_rel_residual = s->rnorm0 > 0 ? s->rnorm64 / s->rnorm0 : 0.0;
   } else {
@@ -251,6 +254,9 @@ struct gooseberry_server_s {
   req_t **kept;             /* finished generated requests holding x */
   size_t nkept, capkept;
   uint32_t nactive;
+  double timeout;           /* seconds after submission; <= 0: none */
+  gooseberry_request_result_t *pending;  /* ended outside a batch step */
+  size_t npending, cappending;
 };
 
 /* Device bytes a request holds while admitted: its workspace, and for a
@@ -259,11 +265,13 @@ struct gooseberry_server_s {
 static size_t req_footprint(const struct gooseberry_server_s *S, const req_t *r)
 {
   const size_t n = r->n;
-  size_t vec = (S->precision == GOOSEBERRY_PRECISION_MIXED ? 8 + 16 : 24) * n;
+  const int mixed = S->precision != GOOSEBERRY_PRECISION_FP64;
+  size_t vec = (mixed ? 8 + 16 : 24) * n;
   size_t ws = vec + 64 * 1024 + n / 4;   /* vectors + partials and slack */
   if (!r->generated) return ws;
   size_t nnz = n * (2 * (size_t)r->dmax - 1);
-  size_t mat = nnz * (4 + 8 + (S->precision == GOOSEBERRY_PRECISION_MIXED ? 4 : 0))
+  size_t mat = nnz * (4 + 8 + (mixed ? 4 : 0))
+             + (S->precision == GOOSEBERRY_PRECISION_MIXED_JACOBI ? 8 * n : 0)
              + 4 * (n + 1)       /* rowptr */
              + 4 * n + 8 * (n / 8 + 16); /* plan: row bins, blocks */
   return ws + mat + 16 * n;      /* b and x */
@@ -284,7 +292,8 @@ int gooseberry_server_create(size_t mem_budget, uint32_t max_active,
 {
   GB_API_GUARD();
   if (!r || max_active == 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  if (precision != GOOSEBERRY_PRECISION_FP64 && precision != GOOSEBERRY_PRECISION_MIXED)
+  if (precision != GOOSEBERRY_PRECISION_FP64 && precision != GOOSEBERRY_PRECISION_MIXED &&
+      precision != GOOSEBERRY_PRECISION_MIXED_JACOBI)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   struct gooseberry_server_s *S = calloc(1, sizeof(*S));
   if (!S) return GOOSEBERRY_ERROR_NOMEMORY;
@@ -330,7 +339,101 @@ int gooseberry_server_free(gooseberry_server_t S)
   free(S->queue);
   free(S->slots);
   free(S->kept);
+  free(S->pending);
   free(S);
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* record the result of a request that ends outside a batch step */
+static int push_pending(struct gooseberry_server_s *S, const req_t *r, int status)
+{
+  if (S->npending == S->cappending) {
+    size_t nc = S->cappending ? 2 * S->cappending : 64;
+    gooseberry_request_result_t *np = realloc(S->pending, nc * sizeof(*np));
+    if (!np) return GOOSEBERRY_ERROR_NOMEMORY;
+    S->pending = np;
+    S->cappending = nc;
+  }
+  gooseberry_request_result_t res = r->res;
+  res.status = status;
+  res.done_s = now_s();
+  S->pending[S->npending++] = res;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* take request i of the queue out (order of the others kept) */
+static void queue_remove(struct gooseberry_server_s *S, size_t i)
+{
+  memmove(S->queue + i, S->queue + i + 1, (S->qtail - i - 1) * sizeof(req_t *));
+  S->qtail--;
+}
+
+/* end the running request in batch slot i (its x keeps the last folded
+ * iterate) */
+static void drop_active(struct gooseberry_server_s *S, uint32_t i)
+{
+  req_t *r = S->slots[i];
+  gooseberry_batch_remove(S->batch, i);
+  S->slots[i] = NULL;
+  S->nactive--;
+  S->in_use -= r->footprint;
+  if (r->generated) req_release(r);
+  else free(r);
+}
+
+int gooseberry_server_cancel(gooseberry_server_t S, uint64_t id)
+{
+  GB_API_GUARD();
+  if (!S) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  for (size_t i = S->qhead; i < S->qtail; ++i) {
+    req_t *r = S->queue[i];
+    if (r->id != id) continue;
+    int rv = push_pending(S, r, GOOSEBERRY_REQUEST_CANCELLED);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+    queue_remove(S, i);
+    free(r);
+    return GOOSEBERRY_SUCCESS;
+  }
+  for (uint32_t i = 0; i < S->max_active; ++i) {
+    req_t *r = S->slots[i];
+    if (!r || r->id != id) continue;
+    int rv = push_pending(S, r, GOOSEBERRY_REQUEST_CANCELLED);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+    drop_active(S, i);
+    return GOOSEBERRY_SUCCESS;
+  }
+  return GOOSEBERRY_ERROR_INVALIDINPUT;
+}
+
+int gooseberry_server_set_timeout(gooseberry_server_t S, double seconds)
+{
+  GB_API_GUARD();
+  if (!S || seconds != seconds) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  S->timeout = seconds;
+  return GOOSEBERRY_SUCCESS;
+}
+
+/* cancel what is past the timeout, queued or running */
+static int expire(struct gooseberry_server_s *S)
+{
+  if (!(S->timeout > 0)) return GOOSEBERRY_SUCCESS;
+  const double t = now_s();
+  int rv;
+  for (size_t i = S->qhead; i < S->qtail;) {
+    req_t *r = S->queue[i];
+    if (t - r->res.submit_s <= S->timeout) { ++i; continue; }
+    if ((rv = push_pending(S, r, GOOSEBERRY_REQUEST_TIMEDOUT)) != GOOSEBERRY_SUCCESS)
+      return rv;
+    queue_remove(S, i);
+    free(r);
+  }
+  for (uint32_t i = 0; i < S->max_active; ++i) {
+    req_t *r = S->slots[i];
+    if (!r || t - r->res.submit_s <= S->timeout) continue;
+    if ((rv = push_pending(S, r, GOOSEBERRY_REQUEST_TIMEDOUT)) != GOOSEBERRY_SUCCESS)
+      return rv;
+    drop_active(S, i);
+  }
   return GOOSEBERRY_SUCCESS;
 }
 
@@ -443,6 +546,8 @@ int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
   GB_API_GUARD();
   if (!S) return GOOSEBERRY_ERROR_INVALIDINPUT;
   int rv;
+  /* 0. requests past the timeout leave */
+  if ((rv = expire(S)) != GOOSEBERRY_SUCCESS) return rv;
   /* 1. admission: first in, first out, while the head fits */
   while (S->qhead < S->qtail && S->nactive < S->max_active) {
     req_t *r = S->queue[S->qhead];
@@ -457,7 +562,15 @@ int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
       }
       break;
     }
-    if ((rv = admit(S, r)) != GOOSEBERRY_SUCCESS) return rv;
+    if ((rv = admit(S, r)) != GOOSEBERRY_SUCCESS) {
+      if (rv != GOOSEBERRY_ERROR_INVALIDINPUT) return rv;
+      /* the input itself is refused: report the request, go on */
+      if ((rv = push_pending(S, r, GOOSEBERRY_REQUEST_REJECTED)) != GOOSEBERRY_SUCCESS)
+        return rv;
+      S->qhead++;
+      free(r);
+      continue;
+    }
     S->in_use += r->footprint;
     S->qhead++;
   }
@@ -465,8 +578,13 @@ int gooseberry_server_progress(gooseberry_server_t S, uint32_t steps,
   uint32_t running = 0;
   if ((rv = gooseberry_batch_run(S->batch, steps, &running)) != GOOSEBERRY_SUCCESS)
     return rv;
-  /* 3. retire finished solves */
+  /* 3. report what ended outside the steps, then retire finished solves */
   uint32_t nd = 0;
+  for (size_t i = 0; i < S->npending; ++i) {
+    if (done && nd < max_done) done[nd] = S->pending[i];
+    ++nd;
+  }
+  S->npending = 0;
   double t = now_s();
   for (uint32_t i = 0; i < S->max_active; ++i) {
     req_t *r = S->slots[i];

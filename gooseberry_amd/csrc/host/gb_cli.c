@@ -171,6 +171,42 @@ static void mat_free(mat_t *m)
   memset(m, 0, sizeof(*m));
 }
 
+/* A dense row-major matrix as CSR of its nonzero entries (row order,
+ * columns ascending). */
+static int dense_to_csr(mat_t *m)
+{
+  const size_t nr = m->nrows, nc = m->ncols;
+  size_t nnz = 0;
+  for (size_t k = 0; k < nr * nc; ++k) nnz += m->rowval[k] != 0;
+  if (nnz > 0xffffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  ind_t *rp = malloc(sizeof(ind_t) * (nr + 1));
+  dim_t *ri = malloc(sizeof(dim_t) * (nnz ? nnz : 1));
+  real_t *rv = malloc(sizeof(real_t) * (nnz ? nnz : 1));
+  if (!rp || !ri || !rv) {
+    free(rp); free(ri); free(rv);
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  size_t e = 0;
+  rp[0] = 0;
+  for (size_t i = 0; i < nr; ++i) {
+    for (size_t j = 0; j < nc; ++j) {
+      const real_t w = m->rowval[i * nc + j];
+      if (w != 0) {
+        ri[e] = (dim_t)j;
+        rv[e] = w;
+        ++e;
+      }
+    }
+    rp[i + 1] = (ind_t)e;
+  }
+  free(m->rowval);
+  m->rowptr = rp;
+  m->rowind = ri;
+  m->rowval = rv;
+  m->dense = 0;
+  return GOOSEBERRY_SUCCESS;
+}
+
 static int read_input(const char *file, mat_t *m)
 {
   int t = file_type(file);
@@ -284,7 +320,7 @@ static int blas_sparse_sparse(mat_t *a, mat_t *b, const char *outfile,
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
   if (is_dense(otype)) {
-    dense = calloc((size_t)anrows * bncols ? (size_t)anrows * bncols : 1,
+    dense = calloc(((size_t)anrows * bncols) ? (size_t)anrows * bncols : 1,
         sizeof(real_t));
     if (!dense) return GOOSEBERRY_ERROR_NOMEMORY;
   }
@@ -875,11 +911,17 @@ static int cmd_pagerank(int argc, char **argv)
   }
   if (err != GOOSEBERRY_SUCCESS) goto END;
   if (mat.dense) {
-    /* the reference also accepts dense matrices; this build runs the
-     * sparse path only */
-    fprintf(stderr, "pagerank: dense input matrices are not supported\n");
-    err = GOOSEBERRY_ERROR_UNIMPLEMENTED;
-    goto END;
+    /* src/gooseberry_bin.c:2561-2565: a dense matrix must be square */
+    if (mat.nrows != mat.ncols) {
+      fprintf(stderr, "PageRank requires a square matrix, but input matrix '%s' is "
+          "%u""x""%u.\n", matfile, (unsigned)mat.nrows, (unsigned)mat.ncols);
+      err = GOOSEBERRY_ERROR_INVALIDINPUT;
+      goto END;
+    }
+    /* the weights the reference's dense normalisation counts are the
+     * nonzero ones (src/gooseberry_bin.c:2633-2644): run the sparse path on
+     * them, in row order with columns ascending */
+    if ((err = dense_to_csr(&mat)) != GOOSEBERRY_SUCCESS) goto END;
   }
   /* src/gooseberry_bin.c:2561-2577: a non-square sparse matrix is squared */
   if (mat.nrows > mat.ncols) {

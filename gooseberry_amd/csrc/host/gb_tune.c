@@ -71,8 +71,11 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
 {
   GB_API_GUARD();
   if (!mat) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  if (precision != GOOSEBERRY_PRECISION_FP64 && precision != GOOSEBERRY_PRECISION_MIXED)
+  if (precision != GOOSEBERRY_PRECISION_FP64 && precision != GOOSEBERRY_PRECISION_MIXED &&
+      precision != GOOSEBERRY_PRECISION_MIXED_JACOBI)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* the scaled FP32 values stream like the plain ones: time those */
+  if (precision == GOOSEBERRY_PRECISION_MIXED_JACOBI) precision = GOOSEBERRY_PRECISION_MIXED;
   /* a batch slot holds the current plan's arrays */
   if (gbd_matrix_holders(gb_matrix_dev(mat))) return GOOSEBERRY_ERROR_INVALIDINPUT;
   tune_ctx c;

@@ -288,6 +288,20 @@ const char * gooseberry_strerror(int err);
 /* text of the last device error seen by the library ("" if none) */
 const char * gooseberry_last_device_error(void);
 
+/* Fault injection, a test hook with no counterpart in the reference: after
+ * `after` more successful device allocations (GOOSEBERRY_FAULT_ALLOC) or
+ * kernel launch checks (GOOSEBERRY_FAULT_LAUNCH), the next one reports a
+ * failure -- GOOSEBERRY_ERROR_NOMEMORY or GOOSEBERRY_ERROR_DEVICE from the
+ * entry point that made it -- once; after < 0 disarms.  An injected launch
+ * failure is reported after the kernel has run normally, so no kernel is
+ * ever left half-launched.  The library stays usable after an injected
+ * failure; an interrupted solve must be started again. */
+#define GOOSEBERRY_FAULT_ALLOC 0
+#define GOOSEBERRY_FAULT_LAUNCH 1
+int gooseberry_inject_fault(
+    int kind,
+    long long after);
+
 /* ---------------------------------------------------- index work (GPU) */
 
 /* Stable counting sort of vals by keys in [kmin, kmax] (host pointers):
@@ -536,9 +550,16 @@ int gooseberry_cgd(
  * products accumulated in FP64, FP64 solution; when the FP32 recurrence
  * residual has dropped by `delta` (batch parameter) the solution update is
  * folded into the FP64 x and the residual recomputed as b - A x with the FP64
- * matrix (reliable update); convergence is declared on that FP64 residual. */
+ * matrix (reliable update); convergence is declared on that FP64 residual.
+ * GOOSEBERRY_PRECISION_MIXED_JACOBI: MIXED with the inner FP32 iterations on
+ * the Jacobi-scaled system D^-1/2 A D^-1/2 (D = diag(A)), i.e. Jacobi-
+ * preconditioned CG; the FP64 refreshes and the convergence test use A and
+ * b unscaled.  Needs a positive, finite diagonal in every row
+ * (GOOSEBERRY_ERROR_INVALIDINPUT otherwise); the matrix keeps the scaled
+ * FP32 values (4 bytes per entry) and D^-1/2 (8 bytes per row). */
 #define GOOSEBERRY_PRECISION_FP64 0
 #define GOOSEBERRY_PRECISION_MIXED 1
+#define GOOSEBERRY_PRECISION_MIXED_JACOBI 2
 
 typedef struct gooseberry_batch_s * gooseberry_batch_t;
 
@@ -602,9 +623,17 @@ size_t gooseberry_batch_workspace_bytes(
  * retired as they finish. */
 typedef struct gooseberry_server_s * gooseberry_server_t;
 
+/* request result status, beside 1 converged, 2 iteration cap, 3 breakdown */
+#define GOOSEBERRY_REQUEST_CANCELLED 4  /* gooseberry_server_cancel */
+#define GOOSEBERRY_REQUEST_TIMEDOUT 5   /* past the server's timeout */
+#define GOOSEBERRY_REQUEST_REJECTED 6   /* its admission refused the input (e.g.
+                                           no positive diagonal for
+                                           MIXED_JACOBI) */
+
 typedef struct gooseberry_request_result_t {
   uint64_t id;
-  int status;            /* 1 converged, 2 iteration cap, 3 breakdown */
+  int status;            /* 1 converged, 2 iteration cap, 3 breakdown, or
+                            one of the GOOSEBERRY_REQUEST_* codes above */
   uint64_t niter;        /* CG (inner) iterations */
   uint64_t nrefine;      /* FP64 refreshes (MIXED) */
   double rel_residual;   /* final residual norm / ||b|| */
@@ -651,8 +680,27 @@ int gooseberry_server_submit(
     real_t * d_x,
     real_t tol);
 
-/* Admit what fits, advance every admitted solve by up to `steps` steps,
- * retire the finished ones and report up to max_done of them. */
+/* Cancel a queued or running request: it leaves the queue or the batch at
+ * once (a running request's x holds its last folded iterate) and the next
+ * progress call reports it with status GOOSEBERRY_REQUEST_CANCELLED.
+ * INVALIDINPUT when no queued or running request has this id. */
+int gooseberry_server_cancel(
+    gooseberry_server_t server,
+    uint64_t id);
+
+/* Requests still queued or running `seconds` after their submission are
+ * cancelled by the next progress call and reported with status
+ * GOOSEBERRY_REQUEST_TIMEDOUT; seconds <= 0 (the default) disables. */
+int gooseberry_server_set_timeout(
+    gooseberry_server_t server,
+    double seconds);
+
+/* Time out overdue requests, admit what fits, advance every admitted solve
+ * by up to `steps` steps, retire the finished ones and report up to
+ * max_done of them (*r_ndone counts all).  A request whose admission
+ * refuses its input is reported with status GOOSEBERRY_REQUEST_REJECTED;
+ * other admission failures (device memory, device errors) return the error
+ * and leave the request at the head of the queue. */
 int gooseberry_server_progress(
     gooseberry_server_t server,
     uint32_t steps,

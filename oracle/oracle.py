@@ -116,8 +116,25 @@ def dot(a, b):
     return lib().orc_dot(a.size, _p(a), _p(b))
 
 
-def cgd(rowptr, rowind, rowval, b, error, niter=0, trace=False):
-    """Returns (x, iterations, final rmse, trace array (k x 3) or None)."""
+_DEVNULL = None
+
+
+def _devnull():
+    """A C stdio stream on /dev/null (for the reference's per-iteration
+    print, src/cgd.c:116, when the solve is timed)."""
+    global _DEVNULL
+    if _DEVNULL is None:
+        c = C.CDLL(None)
+        c.fopen.restype = C.c_void_p
+        c.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        _DEVNULL = c.fopen(b"/dev/null", b"w")
+    return _DEVNULL
+
+
+def cgd(rowptr, rowind, rowval, b, error, niter=0, trace=False, print_devnull=False):
+    """Returns (x, iterations, final rmse, trace array (k x 3) or None).
+    print_devnull: print each iteration's line to /dev/null, as the
+    reference prints it to stdout."""
     rowptr, rowind, rowval, b = _u32(rowptr), _u32(rowind), _f64(rowval), _f64(b)
     n = rowptr.size - 1
     x = np.zeros(n)
@@ -125,8 +142,9 @@ def cgd(rowptr, rowind, rowval, b, error, niter=0, trace=False):
     tr = np.zeros(3 * cap) if cap else None
     it = C.c_size_t()
     rm = C.c_double()
+    log = _devnull() if print_devnull else None
     rv = lib().orc_cgd(n, n, _p(rowptr), _p(rowind), _p(rowval), n, _p(b), n, _p(x),
-                       float(error), int(niter), None, _p(tr), cap, C.byref(it), C.byref(rm))
+                       float(error), int(niter), log, _p(tr), cap, C.byref(it), C.byref(rm))
     assert rv == 1, rv
     k = it.value
     return x, k, rm.value, (tr[:3 * min(k, cap)].reshape(-1, 3) if cap else None)

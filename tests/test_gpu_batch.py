@@ -118,7 +118,7 @@ def solve_batch(gb, systems, tol=1e-10, precision=0, niter=0, order=None):
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_batch_solves_converge(gb, oracle, precision):
     rng = np.random.default_rng(1)
     mats = [lap(gb, 2, 60), lap(gb, 3, 24), lap(gb, 2, 17)]
@@ -131,7 +131,7 @@ def test_batch_solves_converge(gb, oracle, precision):
         assert info.status == 1, info.as_dict()
         tr = true_rel(oracle, rp, ri, rv, x, b)
         assert tr <= 1.05e-10, tr
-        if precision == 1:
+        if precision >= 1:
             assert info.nrefine >= 1 and info.true_rel_residual <= 1e-10
 
 
@@ -145,7 +145,7 @@ def test_batch_fp64_matches_single_solver_iterations(gb, oracle):
     assert abs(int(info.niter) - it) <= 2
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_alone_and_in_the_mix_bitwise(gb, precision):
     rng = np.random.default_rng(2)
     target, _ = lap(gb, 3, 20)
@@ -270,7 +270,7 @@ def _row_bound(rp, ri, rv, x):
     return 2.0 * np.maximum(lens - 1, 0) * np.finfo(np.float64).eps * s
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_autotune_keeps_results(gb, oracle, precision):
     n = 30000
     M = gb.generate_spd(n, 4, gb.DEGREE_ZIPF, 1.5, 2, 4096)
@@ -291,7 +291,7 @@ def test_autotune_keeps_results(gb, oracle, precision):
     assert true_rel(oracle, rp, ri, rv, xs, b) <= 1.05e-10
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_split_matrix_in_batch_alone_and_mix(gb, oracle, precision):
     # a column-split matrix runs its CG steps over its slices (refresh steps
     # over its own plan); same bits alone and in a mix, residual met
@@ -315,7 +315,7 @@ def test_split_matrix_in_batch_alone_and_mix(gb, oracle, precision):
     assert abs(int(i_u.niter) - int(i_alone.niter)) <= max(3, 0.15 * int(i_u.niter))
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_batch_index16_and_wide_plans_bitwise(gb, precision):
     # the batched SpMV reads each matrix's 16-bit offsets when its plan has
     # them, 32-bit indices otherwise; the same systems give the same bits
@@ -377,3 +377,78 @@ def test_server_free_releases_held_matrices(gb):
     a, _, _ = S.status()
     assert a == 3
     S.free()
+
+
+# ------------------------------------------------ Jacobi-scaled mixed solves
+
+def test_jacobi_mixed_fewer_iterations_same_answer(gb, oracle):
+    # Zipf row degrees give diagonals from ~2 to a few thousand: the
+    # Jacobi-scaled inner solve needs far fewer iterations than plain CG,
+    # and its answer meets the FP64 tolerance against the FP64 oracle (the
+    # reference's double CG run to 1e-12)
+    n = 60000
+    M = gb.generate_spd(n, 5, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    rp, ri, rv = M.download()
+    b = gb.uniform(10, n)
+    xr, it_ref, _, _ = oracle.cgd(rp, ri, rv, b, 1e-12 * np.linalg.norm(b))
+    (x64, i64), (xm, im), (xj, ij) = [solve_batch(gb, [(M, b)], 1e-10, p)[0]
+                                      for p in (0, 1, 2)]
+    for x, info in ((x64, i64), (xm, im), (xj, ij)):
+        assert info.status == 1
+        assert true_rel(oracle, rp, ri, rv, x, b) <= 1.05e-10
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-8
+    assert ij.true_rel_residual <= 1e-10 and ij.nrefine >= 1
+    assert 3 * int(ij.niter) < int(i64.niter), (ij.niter, i64.niter)
+    assert 3 * int(ij.niter) < int(im.niter), (ij.niter, im.niter)
+
+
+def test_jacobi_laplacian_same_iterations_as_mixed(gb, oracle):
+    # a constant diagonal: the scaled system is A / 6, so the inner
+    # iterations match plain mixed precision up to FP32 rounding
+    M, (rp, ri, rv) = lap(gb, 3, 30)
+    b = gb.uniform(11, M.nrows)
+    (xm, im), = solve_batch(gb, [(M, b)], 1e-10, 1)
+    (xj, ij), = solve_batch(gb, [(M, b)], 1e-10, 2)
+    assert im.status == ij.status == 1
+    assert abs(int(im.niter) - int(ij.niter)) <= max(3, 0.1 * int(im.niter))
+    assert true_rel(oracle, rp, ri, rv, xj, b) <= 1.05e-10
+
+
+def test_jacobi_refuses_nonpositive_diagonal(gb):
+    # row 1 has no diagonal entry, row 2 a negative one
+    rp = np.array([0, 2, 3, 5], np.uint32)
+    ri = np.array([0, 1, 0, 1, 2], np.uint32)
+    rv = np.array([2.0, -1.0, -1.0, -1.0, -3.0])
+    M = gb.Matrix(rp, ri, rv)
+    db, dx = gb.DeviceArray.from_host(np.ones(3)), gb.DeviceArray(3)
+    B = gb.Batch(1)
+    with pytest.raises(gb.GooseberryError) as e:
+        B.add(M, db, dx, 1e-10, gb.TOL_RELATIVE, 0, gb.PRECISION_MIXED_JACOBI)
+    assert e.value.name == "INVALIDINPUT"
+    # the same matrix still solves in the other precisions' paths
+    B.add(M, db, dx, 1e-10, gb.TOL_RELATIVE, 5, gb.PRECISION_FP64)
+
+
+def test_server_jacobi_alone_equals_mix(gb, oracle):
+    S = gb.Server(max_active=6, precision=gb.PRECISION_MIXED_JACOBI, keep_results=True)
+    sizes = [4096, 50000, 9000, 120000, 20000, 70000]
+    for i, n in enumerate(sizes):
+        S.submit_sdd(i, n, 2000 + i, 3, 27, 1e-10)
+    done = []
+    for _ in range(2000):
+        done += S.progress(8)
+        if len(done) == len(sizes):
+            break
+    assert sorted(r["id"] for r in done) == list(range(len(sizes)))
+    assert all(r["status"] == 1 and r["rel_residual"] <= 1e-10 for r in done)
+    for rid in (1, 3):
+        n = sizes[rid]
+        x = S.take_result(rid, n)
+        rp, ri, rv = gb.generate_sdd(n, 2000 + rid, 3, 27).download()
+        b = gb.uniform_device(2000 + rid, n, stream=1).numpy()
+        assert true_rel(oracle, rp, ri, rv, x, b) <= 1.05e-10
+        S1 = gb.Server(max_active=1, precision=gb.PRECISION_MIXED_JACOBI, keep_results=True)
+        S1.submit_sdd(0, n, 2000 + rid, 3, 27, 1e-10)
+        while not S1.progress(8):
+            pass
+        assert np.array_equal(S1.take_result(0, n), x)

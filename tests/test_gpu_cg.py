@@ -85,6 +85,33 @@ def test_config2_full_size_true_residual(gb, oracle):
     assert info2.niter == info.niter and np.array_equal(x, x2)
 
 
+def test_config2_cg_vs_oracle(gb, oracle):
+    # the headline system at full size: 160^3 Laplacian, b from seed 1,
+    # relative residual 1e-10.  The oracle runs the reference's loop on 16
+    # threads (one thread would take minutes); its dot products then add in
+    # a thread-count dependent order, which the +-2 bar allows.
+    rp, ri, rv = gb.laplacian(3, 160)
+    n = rp.size - 1
+    b = gb.uniform(1, n, stream=1)
+    err = 1e-10 * float(np.linalg.norm(b))
+    M = gb.Matrix(rp, ri, rv)
+    x, info = M.cgd(b, err, gb.TOL_ABSOLUTE)
+    oracle.set_threads(16)
+    try:
+        xr, it, rmse, _ = oracle.cgd(rp, ri, rv, b, err)
+    finally:
+        oracle.set_threads(1)
+    assert info.converged == 1 and info.status == 1
+    assert abs(int(info.niter) - it) <= 2, (info.niter, it)
+    # the same stopping rule met on both sides, and the same true residual
+    assert info.rnorm <= err and rmse <= err
+    rel = true_rel(oracle, rp, ri, rv, x, b)
+    rel_ref = true_rel(oracle, rp, ri, rv, xr, b)
+    assert rel < 1.5e-10 and rel_ref < 1.5e-10
+    assert abs(rel - rel_ref) <= 0.5 * max(rel, rel_ref)
+    assert np.max(np.abs(x - xr)) <= 1e-6 * np.max(np.abs(xr))
+
+
 def test_irregular_spd_vs_oracle(gb, oracle):
     # symmetric, diagonally dominant, rows of very different lengths
     rng = np.random.default_rng(4)

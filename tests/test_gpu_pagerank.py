@@ -157,3 +157,39 @@ def test_pagerank_cli(gb, oracle, tmp_path):
                         os.path.join(str(tmp_path), "pr.csr")],
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
     assert r.returncode == 2
+
+
+def test_pagerank_cli_dense_input(gb, oracle, tmp_path):
+    # a dense (grid) matrix: the weights the reference's dense normalisation
+    # counts are its nonzero entries (src/gooseberry_bin.c:2633-2644); the
+    # build runs the sparse path on them.  (The reference passes the dense
+    # matrix's absent row pointers to gooseberry_spmult at :2676, so it has
+    # no vector of its own to compare with; the oracle runs the same
+    # nonzero entries as CSR.)
+    from gooseberry_amd import CLI_PATH
+    n = 300
+    rng = np.random.default_rng(21)
+    W = rng.uniform(0.0, 1.0, (n, n))
+    W[rng.uniform(0, 1, (n, n)) < 0.9] = 0.0
+    W[5, :] = 0.0  # a sink
+    a = os.path.join(str(tmp_path), "W.grid")
+    oracle.write_grid(a, W.ravel(), n, n)
+    nr, nc, vals = oracle.read_grid(a)
+    D = np.asarray(vals).reshape(nr, nc)
+    rows, cols = np.nonzero(D)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=nr))]).astype(np.uint32)
+    ri, rv = cols.astype(np.uint32), D[rows, cols]
+    out = os.path.join(str(tmp_path), "pr.vec")
+    r = subprocess.run([CLI_PATH, "pagerank", "-i", a, "-o", out, "-I", "15"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    xo, ito, erro, nsinks = oracle_pr(oracle, rp, ri, rv, nr, 0.85, 15, 0.0)
+    assert ito == 15 and nsinks == 1
+    ref = os.path.join(str(tmp_path), "ref.vec")
+    oracle.write_grid(ref, xo, nr, 1)
+    assert open(out).read() == open(ref).read()
+    # a non-square dense matrix is refused, as the reference refuses it
+    oracle.write_grid(a, W[:, :200].ravel(), n, 200)
+    r = subprocess.run([CLI_PATH, "pagerank", "-i", a, "-o", out, "-I", "3"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 2 and "requires a square matrix" in r.stderr

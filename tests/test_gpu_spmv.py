@@ -370,3 +370,24 @@ def test_index16_block_check_beyond_row_precheck(gb, oracle):
     M, y, ref = check(gb, oracle, rp, ri, rv, x)
     assert M.info()["index16"] == 1
     assert np.array_equal(y, ref)
+
+
+# ---- irregular rows at the SpMV contract's tolerance
+
+@pytest.mark.parametrize("plan", [None, {"csplit": 8}, "autotune"])
+def test_zipf_spmv_rel_inf(gb, oracle, plan):
+    # Zipf(1.5) row degrees clipped to [2, 4096] (rows up to a few thousand
+    # entries), uniform columns: every plan the autotuner can pick, the
+    # 8-slice column split among them, within 1e-14 relative inf-norm of the
+    # reference's left-to-right sums
+    n = 200000
+    M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096,
+                        plan=plan if isinstance(plan, dict) else None)
+    if plan == "autotune":
+        M.autotune(gb.PRECISION_FP64, 2)
+    rp, ri, rv = M.download()
+    assert np.diff(rp.astype(np.int64)).max() > 1000
+    x = gb.uniform(2, n, stream=0)
+    y = M.spmv(x)
+    ref = oracle.spmult(n, n, 1, rp, ri, rv, x)
+    assert rel_inf(y, ref) <= TOL, (M.info()["plan"], rel_inf(y, ref))

@@ -265,3 +265,13 @@ def test_device_reader_fails_loudly_without_gpu(gb, tmp_path):
     with pytest.raises(gb.GooseberryError) as e:
         gb.read_sparse_device(path)
     assert e.value.name == "DEVICE"
+
+
+def test_fault_hook_arguments(gb):
+    # the hook only sets a countdown: no device is touched
+    L = gb.lib()
+    assert L.gooseberry_inject_fault(7, 0) == gb.ERROR_CODES["INVALIDINPUT"]
+    assert L.gooseberry_inject_fault(-1, 0) == gb.ERROR_CODES["INVALIDINPUT"]
+    for kind in (gb.FAULT_ALLOC, gb.FAULT_LAUNCH):
+        assert L.gooseberry_inject_fault(kind, 3) == gb.SUCCESS
+        assert L.gooseberry_inject_fault(kind, -1) == gb.SUCCESS
