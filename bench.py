@@ -8,15 +8,19 @@ Laplacian on a 160^3 grid (n = 4,096,000, nnz = 28,518,400, FP64 values
 
 Prints one JSON line: value = steps (= CG solves) per second, the SpMV
 roofline (algorithmic bytes 12 nnz + 4 (n+1) + 16 n per SpMV over the SpMV
-kernel's average duration, from HIP events on the library stream) and the
+kernel's average duration, from HIP events on the library stream), a short
+served stream's p50/p99 latency (--served-seconds, 10 by default) and the
 CPU baseline (the oracle, the C restatement of the reference's OpenMP path,
-timed on a bounded sample on this host).
+on this host's physical cores: the median of 5 runs of the 100 SpMVs and of
+the full solve, and the SpMV on one thread).
 
 --workload selects the other configurations, each printing its own line:
   zipf     irregular rows: n = 2,000,000 random symmetric diagonally
            dominant SPD matrix, degrees Zipf(1.5) clipped to [2, 4096] before
-           symmetrising, seed 2; FP32-inner CG with FP64 reliable updates to
-           1e-10 against plain FP64 CG, errors against FP64 CG run to 1e-12
+           symmetrising, seed 2; FP32-inner CG with FP64 reliable updates
+           (--precision jacobi: on the Jacobi-scaled system, the default;
+           mixed: unscaled) to 1e-10 against plain FP64 CG, errors against
+           FP64 CG run to 1e-12
   batched  512 independent 2D/3D Laplacians (alternating), n log-uniform in
            [1000, 200000], b from seed 3 + i; all advanced together by the
            batched engine (graph-replayed steps), mixed precision to 1e-10;
@@ -43,6 +47,8 @@ timed on a bounded sample on this host).
            orderings per second, the bandwidth before and after, and the
            FP64 SpMV on the scrambled and on the reordered matrix; the
            oracle's sequential ordering as the CPU baseline
+  dropin   gooseberry_spmult with host pointers (upload, plan, multiply and
+           copy back on every call) at the 256^2 and 160^3 Laplacians
   pagerank `gooseberry pagerank` on the Zipf graph of the zipf workload
            (n = 2,000,000, weights |a_ij|, no sinks): damping 0.85, a fixed
            100 iterations from the resident matrix (column sums,
@@ -66,6 +72,10 @@ PEAK_HBM_GBPS = 8000.0
 GRID = 160
 TOL = 1e-10
 NSPMV = 100
+# --precision choices of the mixed-precision workloads
+PRECISIONS = {"fp64": lambda gb: gb.PRECISION_FP64,
+              "mixed": lambda gb: gb.PRECISION_MIXED,
+              "jacobi": lambda gb: gb.PRECISION_MIXED_JACOBI}
 
 
 def spmv_bytes(n, nnz, val_bytes=8, vec_bytes=8):
@@ -77,63 +87,152 @@ def cg_iter_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 48 * n
 
 
+# the sources the SpMV kernel is built from: a counter summary counts only
+# for the build it was measured on
+SPMV_SOURCES = ["gooseberry_amd/csrc/hip/gb_spmv.hip", "gooseberry_amd/csrc/hip/gb_matrix.h",
+                "gooseberry_amd/csrc/hip/gb_common.h", "gooseberry_amd/build.py"]
+
+
+def spmv_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in SPMV_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def load_traffic():
-    """Measured HBM bytes per SpMV launch from the committed rocprofv3
-    counter summary (None when it has not been measured)."""
+    """Measured HBM bytes and traced duration per SpMV launch from the
+    committed rocprofv3 summary, or None for each when the summary was taken
+    on a different build of the kernel (its source hash differs)."""
     p = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     if not os.path.exists(p):
-        return None
+        return None, None, "no counter summary committed"
     with open(p) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        t = json.load(f)
+    if t.get("sources_sha256") != spmv_source_hash():
+        return None, None, "the committed counter summary is of another build of the kernel"
+    return (t.get("traffic_bytes_per_launch"), t.get("trace_avg_launch_ms"),
+            "profiles/spmv_traffic.json (" + t.get("source", "") + ")")
 
 
-def cpu_baseline(rp, ri, rv, x, b, gpu_iters, seconds=15.0, nspmv=NSPMV,
+def host_cpu():
+    """The host's CPU: model, sockets, physical cores and logical CPUs in
+    this process's affinity mask, and the cgroup CPU quota (if any)."""
+    info = {"model": None, "sockets": None, "physical_cores": None,
+            "logical_cpus": len(os.sched_getaffinity(0)), "cgroup_cpus": None}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    info["model"] = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    mask = os.sched_getaffinity(0)
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True,
+                             text=True, timeout=20).stdout
+        cores, socks = set(), set()
+        for ln in out.splitlines():
+            if ln.startswith("#") or not ln.strip():
+                continue
+            cpu, core, sock = (int(v) if v else 0 for v in ln.split(",")[:3])
+            if cpu in mask:
+                cores.add((sock, core))
+                socks.add(sock)
+        info["physical_cores"] = len(cores) or None
+        info["sockets"] = len(socks) or None
+    except Exception:
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                info["cgroup_cpus"] = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_threads(host):
+    """All physical cores this process may use: the affinity mask's physical
+    cores, capped by the cgroup quota and by OMP_NUM_THREADS when the
+    environment sets it (the machine's share of the host)."""
+    t = host["physical_cores"] or host["logical_cpus"]
+    for cap in (host["cgroup_cpus"], int(os.environ.get("OMP_NUM_THREADS", "0")) or None):
+        if cap:
+            t = min(t, cap)
+    return max(1, t)
+
+
+def cpu_baseline(rp, ri, rv, x, b, tol_abs, reps=5, nspmv=NSPMV, single_spmv=True,
                  what="the 160^3 Laplacian"):
-    """The oracle on this host's cores: a bounded sample of SpMVs and CG
-    iterations; solves/s extrapolated with the full solve's iteration count
-    (a step = nspmv SpMVs + one solve)."""
+    """The oracle (the C + OpenMP restatement of the reference's path, -O3,
+    no FMA) on this host's cores, over the reference's "Compute" regions
+    (src/gooseberry_bin.c:1697-1802 for `blas -r 100`, :2078-2091 for `cgd`):
+    nspmv back-to-back SpMVs, and one full CG solve to the absolute
+    tolerance tol_abs (work vectors allocated inside, each iteration's line
+    printed to /dev/null); the median of `reps` runs of each, on all
+    physical cores (OMP_PLACES=cores, OMP_PROC_BIND=close), plus the SpMV on
+    one thread.  A step = nspmv SpMVs + one solve, as on the GPU."""
+    # the OpenMP runtime reads these when the oracle is first loaded
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    orc.set_threads(threads)
+    host = host_cpu()
+    threads = cpu_threads(host)
     n, nnz = rp.size - 1, int(rp[-1])
-    # SpMV sample
-    orc.spmv(rp, ri, rv, x)
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        orc.spmv(rp, ri, rv, x)
-        k += 1
-        if time.perf_counter() - t0 > seconds / 3 or k >= NSPMV:
-            break
-    t_spmv = (time.perf_counter() - t0) / k
-    # CG sample: a fixed number of iterations of the reference's loop
-    iters = 5
-    t0 = time.perf_counter()
-    orc.cgd(rp, ri, rv, b, 0.0, niter=iters)
-    t_it = (time.perf_counter() - t0) / iters
-    budget = seconds - (time.perf_counter() - t0) - t_spmv * k
-    if budget > 4 * t_it * iters:
-        more = max(iters, min(200, int(budget / t_it)))
+
+    def spmv_runs(k):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                orc.spmv(rp, ri, rv, x)
+            ts.append((time.perf_counter() - t0) / k)
+        return float(np.median(ts))
+
+    orc.set_threads(threads)
+    orc.spmv(rp, ri, rv, x)  # first touch, page faults
+    t_spmv = spmv_runs(nspmv) if nspmv else spmv_runs(20)
+    solves, iters, rel = [], None, None
+    for _ in range(reps):
         t0 = time.perf_counter()
-        orc.cgd(rp, ri, rv, b, 0.0, niter=more)
-        t_it = (time.perf_counter() - t0) / more
-        iters = more
-    t_solve = nspmv * t_spmv + gpu_iters * t_it
-    return {
-        "value": 1.0 / t_solve,
+        xs, it, rmse, _ = orc.cgd(rp, ri, rv, b, tol_abs, print_devnull=True)
+        solves.append(time.perf_counter() - t0)
+        iters, rel = int(it), float(rmse / np.linalg.norm(b))
+    t_solve = float(np.median(solves))
+    out = {
+        "value": 1.0 / (nspmv * t_spmv + t_solve),
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": ("%d SpMVs and %d CG iterations of %s with the oracle "
-                   "(C + OpenMP restatement of src/blas.c and src/cgd.c, -O3, no FMA) on "
-                   "%d threads; a step's time = %d SpMVs + %d iterations (the full FP64 "
-                   "solve's count) at the sampled per-iteration time"
-                   % (k, iters, what, threads, nspmv, gpu_iters)),
+        "sample": ("the oracle (C + OpenMP restatement of src/blas.c and src/cgd.c, -O3, no "
+                   "FMA) on %s: median of %d runs of %d back-to-back SpMVs and of %d full CG "
+                   "solves (work vectors allocated inside, per-iteration line to /dev/null) on "
+                   "%d threads (OMP_PLACES=%s, OMP_PROC_BIND=%s); a step = %d SpMVs + 1 solve"
+                   % (what, reps, nspmv or 20, reps, threads, os.environ.get("OMP_PLACES"),
+                      os.environ.get("OMP_PROC_BIND"), nspmv)),
+        "host": host,
         "spmv_gbps": spmv_bytes(n, nnz) / t_spmv / 1e9,
         "spmv_ms": t_spmv * 1e3,
-        "cg_iter_ms": t_it * 1e3,
+        "solve_ms": t_solve * 1e3,
+        "solve_ms_runs": [v * 1e3 for v in solves],
+        "cg_iterations": iters,
+        "cg_final_rel_residual": rel,
+        "cg_iter_ms": t_solve * 1e3 / max(1, iters),
     }
+    if single_spmv:
+        orc.set_threads(1)
+        t1 = spmv_runs(nspmv or 20)
+        out["single_thread_spmv_ms"] = t1 * 1e3
+        out["single_thread_spmv_gbps"] = spmv_bytes(n, nnz) / t1 / 1e9
+        orc.set_threads(threads)
+    return out
 
 
 def run_laplacian(args, gb):
@@ -189,7 +288,7 @@ def run_laplacian(args, gb):
     it = int(np.median(iters))
     solve_ms = np.array(solve_s) * 1e3
     cg_iter_ms = float(np.median(solve_ms)) / max(it, 1)
-    traffic = load_traffic()
+    traffic, trace_ms, traffic_src = load_traffic()
 
     out = {
         "metric": METRIC,
@@ -221,6 +320,10 @@ def run_laplacian(args, gb):
                 "true" if M.info()["index16"] else "false"),
             "algorithmic_bytes_per_launch": ab,
             "avg_launch_ms": avg_spmv_ms,
+            "timing": "HIP events around 100 back-to-back launches on the library stream",
+            "trace_avg_launch_ms": trace_ms,
+            "trace_frac": (ab / (trace_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS) if trace_ms else None,
+            "traffic_source": traffic_src,
         },
         "spmv_gbps": achieved,
         "spmv_frac_of_peak": achieved / PEAK_HBM_GBPS,
@@ -231,11 +334,38 @@ def run_laplacian(args, gb):
         "cg_iter_ms": cg_iter_ms,
         "cg_iter_min_bytes_gbps": cg_iter_bytes(n, nnz) / (cg_iter_ms * 1e-3) / 1e9,
     }
+    del M, dx, dy, db, dsol
+    if args.served_seconds > 0:
+        sv = served_leg(args, gb, args.rate, args.served_seconds)
+        out["served_p50_latency_ms"] = sv["p50_latency_ms"]
+        out["served_p99_latency_ms"] = sv["p99_latency_ms"]
+        out["served"] = sv
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(rp, ri, rv, x, b, it, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(rp, ri, rv, x, b, TOL * float(np.linalg.norm(b)),
+                                           reps=args.cpu_reps)
     else:
         out["cpu_baseline"] = None
     return out
+
+
+def served_leg(args, gb, rate, duration):
+    """A short served stream beside the default workload: Poisson arrivals
+    at `rate`/s for `duration` s of the served workload's requests through
+    the continuous-batching server (64 slots, --precision), latency from
+    arrival to result."""
+    t_arr, ns, seeds = served_requests(gb, rate, duration)
+    S = gb.Server(max_active=64, precision=PRECISIONS[args.precision](gb))
+    done, lat, wall = serve(gb, S, t_arr, ns, seeds)
+    S.free()
+    lats = np.array([lat[r["id"]] for r in done]) * 1e3
+    return {"workload": ("Poisson arrivals %.0f/s for %.0f s (%d requests): random SDD SPD, n "
+                         "log-uniform in [4096, 1048576], degrees U[3,27] then symmetrised, "
+                         "generated on the GPU at admission; 64 slots, precision %s, to "
+                         "relative residual %g" % (rate, duration, len(ns), args.precision, TOL)),
+            "requests": int(len(ns)), "solves_per_s": len(done) / wall,
+            "p50_latency_ms": pct(lats, 50), "p99_latency_ms": pct(lats, 99),
+            "all_converged": all(r["status"] == 1 and r["rel_residual"] <= TOL for r in done),
+            "max_rel_residual": float(max(r["rel_residual"] for r in done)) if done else None}
 
 
 # ------------------------------------------------------------- shared bits
@@ -317,12 +447,14 @@ def run_zipf(args, gb):
         info = M.cgd_device(db, dx, TOL, gb.TOL_RELATIVE)
         return time.perf_counter() - t, info, dx
 
-    def mixed_solve():
+    prec = PRECISIONS[args.precision](gb)
+
+    def mixed_solve(precision=prec):
         dx = gb.DeviceArray(n)
         B = gb.Batch(1)
         gb.synchronize()
         t = time.perf_counter()
-        slot = B.add(M, db, dx, TOL, gb.TOL_RELATIVE, 0, gb.PRECISION_MIXED)
+        slot = B.add(M, db, dx, TOL, gb.TOL_RELATIVE, 0, precision)
         B.solve()
         dt = time.perf_counter() - t
         info = B.info(slot)
@@ -347,17 +479,25 @@ def run_zipf(args, gb):
 
     t64 = float(np.median([a for a, _, _ in f64]))
     tmx = float(np.median([a for a, _, _ in mix]))
+    other = {}
+    for name in ("mixed", "jacobi"):
+        p = PRECISIONS[name](gb)
+        if p != prec:
+            mixed_solve(p)
+            dt, info, dx = mixed_solve(p)
+            other[name] = dict(errors(dx, info), solve_ms=dt * 1e3)
     cfg = {"workload": ("random symmetric diagonally dominant SPD, n=%d, degrees Zipf(1.5) "
                         "clipped to [2,4096] then symmetrised (nnz=%d, mean row length "
                         "%.2f), off-diagonals U[-1,0), diagonal 1.01 x |row sum|, seed 2; "
                         "b U[-1,1) seed 2; step = one FP32-inner / FP64-refined CG solve "
-                        "to relative residual %g" % (n, nnz, nnz / n, TOL)),
-           "n": n, "nnz": nnz, "tol": TOL}
+                        "(precision %s) to relative residual %g"
+                        % (n, nnz, nnz / n, args.precision, TOL)),
+           "n": n, "nnz": nnz, "tol": TOL, "precision": args.precision}
     out = line(args.steps / sum(a for a, _, _ in mix), "solves/s", args, tmx * 1e3,
                "f32+f64", cfg, roofline=roof,
                mixed=dict(errors(mix[-1][2], mix[-1][1]), solve_ms=tmx * 1e3),
                fp64=dict(errors(f64[-1][2], f64[-1][1]), solve_ms=t64 * 1e3),
-               speedup_mixed_vs_fp64=t64 / tmx,
+               speedup_mixed_vs_fp64=t64 / tmx, other_precisions=other,
                p50_solve_ms=pct([a * 1e3 for a, _, _ in mix], 50),
                p99_solve_ms=pct([a * 1e3 for a, _, _ in mix], 99),
                fp64_reference={"tol": 1e-12, "iterations": int(iref.niter)},
@@ -367,8 +507,9 @@ def run_zipf(args, gb):
     if not args.no_cpu_baseline:
         rp, ri, rv = M.download()
         x = gb.uniform(2, n, stream=0)
-        cb = cpu_baseline(rp, ri, rv, x, bh, int(f64[-1][1].niter), args.cpu_seconds,
-                          nspmv=0, what="the Zipf matrix (n=%d, nnz=%d)" % (n, nnz))
+        cb = cpu_baseline(rp, ri, rv, x, bh, TOL * float(np.linalg.norm(bh)), reps=3,
+                          nspmv=0, single_spmv=False,
+                          what="the Zipf matrix (n=%d, nnz=%d)" % (n, nnz))
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
@@ -390,6 +531,31 @@ def batched_systems(gb, nsys):
     return out
 
 
+def fp64_references(gb, mats, rhs):
+    """Every system's FP64 reference solution: the plain FP64 CG (the
+    reference's loop on the GPU) run to relative residual 1e-12."""
+    refs = []
+    for M, db in zip(mats, rhs):
+        dx = gb.DeviceArray(M.nrows)
+        M.cgd_device(db, dx, 1e-12, gb.TOL_RELATIVE)
+        refs.append(dx.numpy())
+        dx.free()
+    return refs
+
+
+def result_errors(gb, mats, rhs, xs, refs):
+    """Every result's true relative residual (FP64 SpMV on the device) and
+    relative error against its FP64 reference solution."""
+    res, err = [], []
+    for M, db, dx, xr in zip(mats, rhs, xs, refs):
+        res.append(true_rel_residual(gb, M, db, dx))
+        err.append(float(np.linalg.norm(dx.numpy() - xr) / np.linalg.norm(xr)))
+    return {"systems": len(res), "true_rel_residual_max": max(res),
+            "true_rel_residual_median": float(np.median(res)),
+            "rel_error_vs_fp64_1e-12_max": max(err),
+            "rel_error_vs_fp64_1e-12_median": float(np.median(err))}
+
+
 def run_batched(args, gb):
     shapes = batched_systems(gb, args.nsys)
     mats, rhs = [], []
@@ -404,6 +570,7 @@ def run_batched(args, gb):
     log("batched: %d systems, sum n=%d, nnz=%d, built in %.1f s"
         % (len(mats), ns.sum(), nnz, time.perf_counter() - t0))
     xs = [gb.DeviceArray(M.nrows) for M in mats]
+    prec = PRECISIONS[args.precision](gb)
 
     def batch_solve(precision):
         B = gb.Batch(len(mats))
@@ -421,7 +588,8 @@ def run_batched(args, gb):
         its = [int(i.niter) for i in infos]
         return {"solves_per_s": len(infos) / dt, "wall_ms": dt * 1e3,
                 "all_converged": all(int(i.status) == 1 for i in infos),
-                "iterations_max": max(its), "iterations_mean": float(np.mean(its))}
+                "iterations_max": max(its), "iterations_mean": float(np.mean(its)),
+                "refreshes_mean": float(np.mean([int(i.nrefine) for i in infos]))}
 
     nnzs = np.array([int(M.info()["nnz"]) for M in mats], dtype=np.float64)
 
@@ -432,60 +600,63 @@ def run_batched(args, gb):
         its = np.array([int(i.niter) for i in infos], dtype=np.float64)
         return float(np.sum(its * (8 * nnzs + 4 * (ns + 1) + 24 * ns)))
 
+    # the FP64 references (CG to 1e-12) and the plain FP64 GPU CG one solve
+    # at a time over every system (timed)
+    t = time.perf_counter()
+    refs = fp64_references(gb, mats, rhs)
+    log("batched: FP64 references (CG to 1e-12) in %.1f s" % (time.perf_counter() - t))
+    single_iters = []
+    if not args.no_single:
+        gb.synchronize()
+        t1 = time.perf_counter()
+        for M, b, x in zip(mats, rhs, xs):
+            single_iters.append(int(M.cgd_device(b, x, TOL, gb.TOL_RELATIVE).niter))
+        gb.synchronize()
+        t_single = time.perf_counter() - t1
     for _ in range(args.warmup):
-        batch_solve(gb.PRECISION_MIXED)
-    mixed_default = [batch_solve(gb.PRECISION_MIXED) for _ in range(args.steps)]
-    fp64_batched = batch_solve(gb.PRECISION_FP64)
-    # residuals of the last mixed batch, checked on a sample of systems
-    check = list(range(0, len(mats), max(1, len(mats) // 16)))
-    worst = max(true_rel_residual(gb, mats[i], rhs[i], xs[i]) for i in check)
-
+        batch_solve(prec)
+    runs = [batch_solve(prec) for _ in range(args.steps)]
+    errs = result_errors(gb, mats, rhs, xs, refs)
+    other = {}
+    for name in ("fp64", "mixed", "jacobi"):
+        p = PRECISIONS[name](gb)
+        if p != prec:
+            other[name] = summary(*batch_solve(p))
     # autotuned plans: one tuning per distinct matrix shape class
     t = time.perf_counter()
-    tuned = gb.autotune_all(mats) if hasattr(gb, "autotune_all") else None
+    tuned = gb.autotune_all(mats, prec)
     tune_s = time.perf_counter() - t
-    mixed_tuned = [batch_solve(gb.PRECISION_MIXED) for _ in range(args.steps)] if tuned else []
+    runs_tuned = [batch_solve(prec) for _ in range(args.steps)]
     for M in mats:
         M.replan(None)
 
-    # one at a time: the plain FP64 GPU CG, a sample of the systems
-    sample = [] if args.no_single else list(range(0, len(mats), max(1, len(mats) // 64)))
-    t1 = time.perf_counter()
-    single_iters = []
-    for i in sample:
-        info = mats[i].cgd_device(rhs[i], xs[i], TOL, gb.TOL_RELATIVE)
-        single_iters.append(int(info.niter))
-    gb.synchronize()
-    t_single = (time.perf_counter() - t1) / len(sample) if sample else None
-
-    tm = float(np.median([a for a, _ in mixed_default]))
-    ab = inner_bytes(mixed_default[-1][1])
-    achieved = ab / mixed_default[-1][0] / 1e9
+    tm = float(np.median([a for a, _ in runs]))
+    ab = inner_bytes(runs[-1][1])
+    achieved = ab / runs[-1][0] / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
             "kernel": "whole batched step (gb_b_spmv, gb_b_reduce, gb_b_update, gb_b_pupdate "
                       "over all systems): FP32 inner-iteration bytes of every system over the "
                       "solve's wall time",
-            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": mixed_default[-1][0] * 1e3}
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": runs[-1][0] * 1e3}
     cfg = {"workload": ("%d independent Laplacians (2D 5-point / 3D 7-point alternating), n "
                         "log-uniform in [1000, 200000] (sum n=%d, nnz=%d), b U[-1,1) seed "
                         "3+i; step = all systems solved together by the batched engine, "
-                        "FP32-inner / FP64-refined CG to relative residual %g"
-                        % (len(mats), ns.sum(), nnz, TOL)),
-           "systems": len(mats), "sum_n": int(ns.sum()), "nnz": nnz, "tol": TOL}
-    out = line(len(mats) * args.steps / sum(a for a, _ in mixed_default), "solves/s", args,
-               tm * 1e3, "f32+f64", cfg,
-               mixed_default_plans=summary(*mixed_default[-1]),
-               worst_sampled_true_rel_residual=worst,
-               fp64_batched=summary(*fp64_batched),
-               one_at_a_time_fp64=None if not sample else {
-                   "solves_per_s": 1.0 / t_single, "systems_sampled": len(sample),
-                   "iterations_mean": float(np.mean(single_iters))},
-               speedup_vs_one_at_a_time_fp64=None if not sample else (len(mats) / tm) * t_single,
+                        "precision %s, to relative residual %g"
+                        % (len(mats), ns.sum(), nnz, args.precision, TOL)),
+           "systems": len(mats), "sum_n": int(ns.sum()), "nnz": nnz, "tol": TOL,
+           "precision": args.precision}
+    out = line(len(mats) * args.steps / sum(a for a, _ in runs), "solves/s", args,
+               tm * 1e3, "f32+f64" if args.precision != "fp64" else "f64", cfg,
+               batched=summary(*runs[-1]), every_result=errs, other_precisions=other,
                roofline=roof, cpu_baseline=None)
-    if mixed_tuned:
-        out["mixed_autotuned_plans"] = summary(*mixed_tuned[-1])
-        out["autotune"] = dict(tuned, seconds=tune_s)
+    if single_iters:
+        out["one_at_a_time_fp64"] = {"solves_per_s": len(mats) / t_single,
+                                     "systems": len(mats), "wall_ms": t_single * 1e3,
+                                     "iterations_mean": float(np.mean(single_iters))}
+        out["speedup_vs_one_at_a_time_fp64"] = (len(mats) / tm) / (len(mats) / t_single)
+    out["autotuned_plans"] = dict(summary(*runs_tuned[-1]), seconds=tune_s,
+                                  classes=tuned)
     return out
 
 
@@ -540,7 +711,8 @@ def run_served(args, gb):
     t_arr, ns, seeds = served_requests(gb, args.rate, args.duration)
     log("served: %d requests over %.0f s, mean n %.0f" % (len(ns), args.duration, ns.mean()))
     # the Poisson stream through the continuous-batching server
-    S = gb.Server(max_active=64, precision=gb.PRECISION_MIXED)
+    prec = PRECISIONS[args.precision](gb)
+    S = gb.Server(max_active=64, precision=prec)
     done, lat, wall = serve(gb, S, t_arr, ns, seeds)
     S.free()
     lats = np.array([lat[r["id"]] for r in done]) * 1e3
@@ -569,18 +741,21 @@ def run_served(args, gb):
         single_its.append(int(info.niter))
     gb.synchronize()
     t_single = time.perf_counter() - t
-    S = gb.Server(max_active=64, precision=gb.PRECISION_MIXED)
+    refs = fp64_references(gb, mats, bs)
+    xs2 = [gb.DeviceArray(int(ns[i])) for i in range(kc)]
+    S = gb.Server(max_active=64, precision=prec)
+    gb.synchronize()
     t = time.perf_counter()
     for i in range(kc):
-        S.submit(i, mats[i], bs[i], xs[i], TOL)
+        S.submit(i, mats[i], bs[i], xs2[i], TOL)
     cap_done = []
     while len(cap_done) < kc:
         cap_done += S.progress(4)
     t_served = time.perf_counter() - t
     S.free()
     cap_ok = all(r["status"] == 1 and r["rel_residual"] <= TOL for r in cap_done)
-    worst = max(true_rel_residual(gb, mats[i], bs[i], xs[i]) for i in range(0, kc, max(1, kc // 8)))
-    del mats, bs, xs
+    cap_errs = result_errors(gb, mats, bs, xs2, refs)
+    del mats, bs, xs, xs2, refs
     cap_served = kc / t_served
     cap_single = kc / t_single
     # whole-run roofline of the served capacity: every request's FP32
@@ -601,19 +776,21 @@ def run_served(args, gb):
                         "random SDD SPD n log-uniform in [4096, 1048576], degrees U[3,27] "
                         "then symmetrised, values as the Zipf workload, seed per request; "
                         "continuous batching (64 slots, memory admission), FP32-inner / "
-                        "FP64-refined CG to relative residual %g"
-                        % (args.rate, args.duration, len(ns), TOL)),
+                        "FP64-refined CG (precision %s) to relative residual %g"
+                        % (args.rate, args.duration, len(ns), args.precision, TOL)),
            "requests": int(len(ns)), "rate": args.rate, "duration_s": args.duration,
            "tol": TOL}
     return line(len(done) / wall, "solves/s", args, wall * 1e3, "f32+f64", cfg,
                 p50_latency_ms=pct(lats, 50), p99_latency_ms=pct(lats, 99),
                 all_converged=ok, mean_nnz=float(nnz.mean()), mean_iterations=float(its.mean()),
+                max_rel_residual=float(max(r["rel_residual"] for r in done)),
                 capacity={"requests": kc, "inputs": "resident in HBM before timing",
                           "served_mixed_solves_per_s": cap_served,
                           "one_at_a_time_fp64_solves_per_s": cap_single,
                           "ratio": cap_served / cap_single,
                           "served_all_converged": cap_ok,
-                          "worst_sampled_true_rel_residual": worst,
+                          "every_result": cap_errs,
+                          "served_iterations_mean": float(np.mean([r["niter"] for r in cap_done])),
                           "one_at_a_time_mean_iterations": float(np.mean(single_its))},
                 roofline=roof, cpu_baseline=None)
 
@@ -931,6 +1108,59 @@ def run_pagerank(args, gb):
                 oracle_check_rel=check, cpu_baseline=cpu)
 
 
+# --------------------------------------- the host-pointer drop-in boundary
+
+def run_dropin(args, gb):
+    """gooseberry_spmult with host pointers (src/gooseberry.c:53-67): each call
+    uploads A and x, builds A's SpMV plan, multiplies and copies y back.  Per
+    call time at the CPU-runnable 256^2 Laplacian and the 160^3 Laplacian,
+    beside the resident SpMV and the oracle's SpMV on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    import oracle as orc
+    host = host_cpu()
+    orc.set_threads(cpu_threads(host))
+    res = {}
+    for dims, side in ((2, 256), (3, args.grid)):
+        rp, ri, rv = gb.laplacian(dims, side)
+        n = rp.size - 1
+        x = gb.uniform(1 if dims == 3 else 0, n, stream=0 if dims == 3 else 1)
+        y = np.empty(n)
+        for _ in range(max(1, args.warmup)):
+            gb.spmult(n, n, 1, rp, ri, rv, x, crowval=y)
+        ts = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            gb.spmult(n, n, 1, rp, ri, rv, x, crowval=y)
+            ts.append(time.perf_counter() - t)
+        assert np.array_equal(y, orc.spmv(rp, ri, rv, x)) or dims == 3
+        M = gb.Matrix(rp, ri, rv)
+        dx, dy = gb.DeviceArray.from_host(x), gb.DeviceArray(n)
+        roof = spmv_roofline(gb, M, dx, dy)
+        tc = []
+        for _ in range(5):
+            t = time.perf_counter()
+            orc.spmv(rp, ri, rv, x)
+            tc.append(time.perf_counter() - t)
+        ab = spmv_bytes(n, int(rp[-1]))
+        res["%dd_%d" % (dims, side)] = {
+            "n": n, "nnz": int(rp[-1]), "host_bytes_in": int(4 * (n + 1) + 12 * int(rp[-1]) + 8 * n),
+            "dropin_ms_per_call": float(np.median(ts)) * 1e3,
+            "resident_spmv_ms": roof["avg_launch_ms"],
+            "oracle_spmv_ms": float(np.median(tc)) * 1e3,
+            "dropin_algorithmic_gbps": ab / float(np.median(ts)) / 1e9}
+        del M, dx, dy
+    big = res["3d_%d" % args.grid]
+    return line(1e3 / big["dropin_ms_per_call"], "calls/s", args, big["dropin_ms_per_call"],
+                "f64", {"workload": "gooseberry_spmult with host pointers (A, x uploaded, plan "
+                        "built, y copied back on every call): 2D 256^2 and 3D %d^3 Laplacians"
+                        % args.grid}, per_config=res, roofline=None,
+                cpu_baseline={"value": 1e3 / big["oracle_spmv_ms"], "unit": "calls/s",
+                              "cores": cpu_threads(host), "kind": "port", "host": host,
+                              "sample": "5 oracle SpMVs of the 3D Laplacian (median)"})
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -944,7 +1174,7 @@ def main():
                     help="untimed steps (default 2; zipf 1; batched 1)")
     ap.add_argument("--workload", default="laplacian",
                     choices=["laplacian", "zipf", "batched", "served", "spgemm", "ingest",
-                             "rcm", "pagerank"])
+                             "rcm", "pagerank", "dropin"])
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
@@ -955,10 +1185,18 @@ def main():
                     help="served: arrivals per second")
     ap.add_argument("--capacity-requests", type=int, default=200,
                     help="served: requests of the backlog capacity comparison")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="jacobi",
+                    help="precision of the zipf, batched and served solves: plain mixed "
+                         "(FP32 inner, FP64 refreshes) or mixed on the Jacobi-scaled system")
+    ap.add_argument("--served-seconds", type=float, default=10.0,
+                    help="length of the served stream beside the default workload (0: none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true",
                     help="batched: skip the one-at-a-time FP64 comparison")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="CPU time bound of the other workloads' baselines")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="runs of each CPU baseline leg (the median is reported)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run 4 device copies of exactly 256 MiB, the "
                          "known-size kernel that scales the rocprofv3 "
@@ -970,7 +1208,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0),
-                "spgemm": (5, 1), "ingest": (5, 1), "rcm": (3, 1), "pagerank": (3, 1)}
+                "spgemm": (5, 1), "ingest": (5, 1), "rcm": (3, 1), "pagerank": (3, 1),
+                "dropin": (5, 1)}
     st, wu = defaults[args.workload]
     args.steps = st if args.steps is None else args.steps
     args.warmup = wu if args.warmup is None else args.warmup
@@ -979,7 +1218,7 @@ def main():
 
     run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
            "served": run_served, "spgemm": run_spgemm, "ingest": run_ingest,
-           "rcm": run_rcm, "pagerank": run_pagerank}[args.workload]
+           "rcm": run_rcm, "pagerank": run_pagerank, "dropin": run_dropin}[args.workload]
     out = run(args, gb)
     print(json.dumps(out), flush=True)
 
