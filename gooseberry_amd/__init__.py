@@ -83,6 +83,7 @@ EXPORTED = [
     "gooseberry_parse_csr_device", "gooseberry_matrix_read",
     "gooseberry_permute_cuthillmckee", "gooseberry_matrix_cuthillmckee",
     "gooseberry_matrix_spmv_f32", "gooseberry_matrix_cgd_mixed", "gooseberry_spmult_f32",
+    "gooseberry_matrix_cgd_noalloc", "gooseberry_cgd_noalloc_max_bytes",
     "gooseberry_inject_fault", "gooseberry_server_cancel", "gooseberry_server_set_timeout",
 ]
 
@@ -209,6 +210,8 @@ def _sig(lib):
         "gooseberry_matrix_spmv": (i32, [_P, _P, _P]),
         "gooseberry_matrix_spmm": (i32, [_P, u32, _P, _P]),
         "gooseberry_matrix_cgd": (i32, [_P, _P, _P, f64, i32, sz, _P, C.POINTER(CgdInfo)]),
+        "gooseberry_matrix_cgd_noalloc": (i32, [_P, _P, _P, f64, i32, sz, C.POINTER(CgdInfo)]),
+        "gooseberry_cgd_noalloc_max_bytes": (sz, []),
         "gooseberry_inject_fault": (i32, [i32, C.c_longlong]),
         "gooseberry_server_cancel": (i32, [_P, u64]),
         "gooseberry_server_set_timeout": (i32, [_P, f64]),

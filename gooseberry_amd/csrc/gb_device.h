@@ -187,6 +187,13 @@ int gbd_cg_poll(gbd_cg *cg, gbd_cg_state *st, double *h_trace);
 /* Capture k iterations into a hipGraph once and replay it for each
  * gbd_cg_enqueue_graph call. */
 int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k);
+/* The workspace of a solve with m in the library's file-scope device array
+ * (no device allocation; graphs are not used: gbd_cg_enqueue_graph launches
+ * kernel by kernel).  INVALIDINPUT when m needs more than
+ * gbd_cg_scratch_max() bytes.  The workspace is shared by every such solve
+ * on the device and is not destroyed; start it with trace_cap = 0. */
+int gbd_cg_scratch(const gbd_matrix *m, gbd_cg **out);
+size_t gbd_cg_scratch_max(void);
 
 /* ----------------------------------------------------- batched solves */
 /* Per-system state of a batched solve (device layout, polled to host). */

@@ -51,6 +51,7 @@ struct gbd_cg {
   hipGraph_t graph[GB_CG_NGRAPH];       // graph[i]: 2^i iterations
   hipGraphExec_t gexec[GB_CG_NGRAPH];
   double *graph_x;                      // x the graphs were captured with
+  bool noalloc;  // the workspace in `scratch`: single launches, no graphs
 };
 
 // Fixed-order sum of n partials, computed identically by every workgroup
@@ -386,6 +387,21 @@ static void gb_cg_drop_graph(gbd_cg *cg) {
 
 extern "C" {
 
+// ------------------------------------------------- allocation-free solves
+// The workspace of gbd_cg_scratch solves (r, p, Ap, the two partial arrays
+// and the state) lives in this file-scope device array: such a solve makes
+// no device allocation.  Each device holds its own copy at its own address,
+// looked up once per device with hipGetSymbolAddress; host code uses that
+// address, never the array's name.  Every such solve on a device shares the
+// array, so solves that overlap on one device would race (the library lock
+// serialises the public entry points of one process).
+#define GB_CG_SCRATCH_BYTES (128ull << 20)
+#define GB_CG_SCRATCH_MAXDEV 64
+static __device__ __align__(256) unsigned char scratch[GB_CG_SCRATCH_BYTES];
+static void *g_scratch_addr[GB_CG_SCRATCH_MAXDEV];
+static gbd_cg g_scratch_cg;            // host bookkeeping of the shared workspace
+static gbd_cg_state g_scratch_hst;     // its polled state (pageable host memory)
+
 // Launch geometry and partial-array sizes of a solve with m.
 static void gb_cg_geometry(gbd_cg *cg, const gbd_matrix *m) {
   cg->m = m;
@@ -435,7 +451,7 @@ int gbd_cg_create(const gbd_matrix *m, gbd_cg **out) {
 }
 
 void gbd_cg_destroy(gbd_cg *cg) {
-  if (!cg) return;
+  if (!cg || cg->noalloc) return;
   (void)hipStreamSynchronize(gb_stream());
   gb_cg_drop_graph(cg);
   (void)gb_dev_free(cg->r);
@@ -483,7 +499,7 @@ int gbd_cg_enqueue(gbd_cg *cg, uint32_t k) { return gb_cg_enqueue_raw(cg, k); }
 // per power of two, captured on first use and kept while x stays the same).
 int gbd_cg_enqueue_graph(gbd_cg *cg, uint32_t k) {
   hipStream_t s = gb_stream();
-  if (!gb_graphs_enabled()) return gb_cg_enqueue_raw(cg, k);
+  if (!gb_graphs_enabled() || cg->noalloc) return gb_cg_enqueue_raw(cg, k);
   if (cg->graph_x != cg->x) {
     gb_cg_drop_graph(cg);
     cg->graph_x = cg->x;
@@ -526,6 +542,52 @@ int gbd_cg_poll(gbd_cg *cg, gbd_cg_state *st, double *h_trace) {
   if (st->status) st->done = 1;
   if (h_trace && cg->trace_cap)
     memcpy(h_trace, cg->h_trace, (size_t)cg->trace_cap * sizeof(double));
+  return GOOSEBERRY_SUCCESS;
+}
+
+size_t gbd_cg_scratch_max() { return GB_CG_SCRATCH_BYTES; }
+
+int gbd_cg_scratch(const gbd_matrix *m, gbd_cg **out) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_cg *cg = &g_scratch_cg;
+  memset(cg, 0, sizeof(*cg));
+  gb_cg_geometry(cg, m);
+  auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+  const uint64_t nb = al((uint64_t)(cg->n ? cg->n : 1) * sizeof(double));
+  const uint64_t o_p = nb, o_ap = 2 * nb, o_pa = 3 * nb;
+  const uint64_t o_pa2 = o_pa + al((uint64_t)cg->npartials * sizeof(double));
+  const uint64_t o_st = o_pa2 + al((uint64_t)gb_cg_np2(cg) * sizeof(double));
+  const uint64_t need = o_st + al(sizeof(gbd_cg_state));
+  if (need > GB_CG_SCRATCH_BYTES) {
+    gb_set_error("the allocation-free CG needs %llu bytes of workspace for this matrix "
+                 "(%u rows); its device array holds %llu",
+                 (unsigned long long)need, m->nrows, (unsigned long long)GB_CG_SCRATCH_BYTES);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  int dev = 0;
+  GB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= GB_CG_SCRATCH_MAXDEV) {
+    gb_set_error("device %d is beyond the %d devices the scratch table covers", dev,
+                 GB_CG_SCRATCH_MAXDEV);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (!g_scratch_addr[dev]) {
+    void *a = nullptr;
+    GB_HIP(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    g_scratch_addr[dev] = a;
+  }
+  unsigned char *base = (unsigned char *)g_scratch_addr[dev];
+  cg->r = (double *)base;
+  cg->p = (double *)(base + o_p);
+  cg->ap = (double *)(base + o_ap);
+  cg->partials = (double *)(base + o_pa);
+  cg->partials2 = (double *)(base + o_pa2);
+  cg->st = (gbd_cg_state *)(base + o_st);
+  cg->h_st = &g_scratch_hst;
+  cg->noalloc = true;
+  *out = cg;
   return GOOSEBERRY_SUCCESS;
 }
 

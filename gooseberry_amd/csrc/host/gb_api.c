@@ -677,7 +677,8 @@ static uint32_t gb_cg_next_batch(uint32_t k, uint64_t it0, double r0,
 }
 
 /* The solve loop on a workspace: start, then batches of iterations (graph
- * replays) and a poll after each. */
+ * replays, or single launches for a workspace in the scratch array) and a
+ * poll after each. */
 static int cgd_loop(gbd_cg *cg, const real_t *d_b, real_t *d_x, real_t tol,
     int tol_mode, size_t niter, FILE *flog, gooseberry_cgd_info_t *info)
 {
@@ -741,6 +742,23 @@ int gooseberry_matrix_cgd(gooseberry_matrix_t mat, const real_t *d_b,
     if (rv != GOOSEBERRY_SUCCESS) return rv;
   }
   return cgd_loop(mat->cg, d_b, d_x, tol, tol_mode, niter, (FILE *)log, info);
+}
+
+int gooseberry_matrix_cgd_noalloc(gooseberry_matrix_t mat, const real_t *d_b,
+    real_t *d_x, real_t tol, int tol_mode, size_t niter,
+    gooseberry_cgd_info_t *info)
+{
+  GB_API_GUARD();
+  if (!cgd_args_ok(mat, d_b, d_x, tol_mode)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gbd_cg *cg = NULL;
+  int rv = gbd_cg_scratch(mat->m, &cg);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  return cgd_loop(cg, d_b, d_x, tol, tol_mode, niter, NULL, info);
+}
+
+size_t gooseberry_cgd_noalloc_max_bytes(void)
+{
+  return gbd_cg_scratch_max();
 }
 
 int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,

@@ -488,6 +488,29 @@ int gooseberry_matrix_cgd(
     void * log,
     gooseberry_cgd_info_t * info);
 
+/* gooseberry_matrix_cgd without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * work vectors (r, p, Ap: 24 bytes per row), the partial sums and the state
+ * live in one file-scope device array of
+ * gooseberry_cgd_noalloc_max_bytes() bytes (128 MiB: about 5.5 million
+ * rows) that each GPU holds until the process ends; a larger system returns
+ * GOOSEBERRY_ERROR_INVALIDINPUT before any launch.  Kernels are launched one
+ * by one (no graphs); no log.  Every call on a GPU shares that array, so
+ * calls that overlap on one GPU race (the library lock serialises the calls
+ * of one process).  Same results, bit for bit, as gooseberry_matrix_cgd. */
+int gooseberry_matrix_cgd_noalloc(
+    gooseberry_matrix_t mat,
+    const real_t * d_b,
+    real_t * d_x,
+    real_t tol,
+    int tol_mode,
+    size_t niter,
+    gooseberry_cgd_info_t * info);
+
+/* Bytes of the allocation-free CG's device array. */
+size_t gooseberry_cgd_noalloc_max_bytes(void);
+
 /* Mixed precision for one system (a one-system batch, see the batched
  * solves below): FP32 inner iterations, FP64 dots, solution and reliable
  * updates; converged on the FP64 residual.  info->niter counts inner
