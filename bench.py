@@ -516,6 +516,28 @@ def run_zipf(args, gb):
     return out
 
 
+def run_zipf_spmv(args, gb):
+    """The Zipf matrix's FP64 SpMV alone: the default plan, then the autotuned
+    plan (each 20 launches timed with HIP events) -- the short run the
+    counter profiles of the irregular gather use."""
+    n = args.zipf_n
+    M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    nnz = int(M.info()["nnz"])
+    dx = gb.uniform_device(2, n, stream=0)
+    dy = gb.DeviceArray(n)
+    roof_default = spmv_roofline(gb, M, dx, dy)
+    plan, _ = M.autotune(gb.PRECISION_FP64, 5)
+    roof = spmv_roofline(gb, M, dx, dy)
+    roof["kernel"] = "gb_spmv_kernel<double,false,false> over %d column slices + gb_split_sum" \
+        % plan["csplit"] if plan["csplit"] else "gb_spmv_kernel<double,false,false>"
+    cfg = {"workload": ("FP64 SpMV of the random SDD SPD matrix, n=%d, degrees Zipf(1.5) clipped "
+                        "to [2,4096] then symmetrised (nnz=%d), x U[-1,1) seed 2" % (n, nnz)),
+           "n": n, "nnz": nnz}
+    return line(roof["achieved"], "GB/s", args, roof["avg_launch_ms"], "f64", cfg,
+                roofline=roof, default_plan=roof_default, autotuned_plan=plan,
+                cpu_baseline=None)
+
+
 # ------------------------------------------------------ batched systems
 
 def batched_systems(gb, nsys):
@@ -1174,7 +1196,7 @@ def main():
                     help="untimed steps (default 2; zipf 1; batched 1)")
     ap.add_argument("--workload", default="laplacian",
                     choices=["laplacian", "zipf", "batched", "served", "spgemm", "ingest",
-                             "rcm", "pagerank", "dropin"])
+                             "rcm", "pagerank", "dropin", "zipf-spmv"])
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
@@ -1209,7 +1231,7 @@ def main():
         sys.exit(2)
     defaults = {"laplacian": (10, 2), "zipf": (3, 1), "batched": (3, 1), "served": (1, 0),
                 "spgemm": (5, 1), "ingest": (5, 1), "rcm": (3, 1), "pagerank": (3, 1),
-                "dropin": (5, 1)}
+                "dropin": (5, 1), "zipf-spmv": (1, 0)}
     st, wu = defaults[args.workload]
     args.steps = st if args.steps is None else args.steps
     args.warmup = wu if args.warmup is None else args.warmup
@@ -1218,7 +1240,8 @@ def main():
 
     run = {"laplacian": run_laplacian, "zipf": run_zipf, "batched": run_batched,
            "served": run_served, "spgemm": run_spgemm, "ingest": run_ingest,
-           "rcm": run_rcm, "pagerank": run_pagerank, "dropin": run_dropin}[args.workload]
+           "rcm": run_rcm, "pagerank": run_pagerank, "dropin": run_dropin,
+           "zipf-spmv": run_zipf_spmv}[args.workload]
     out = run(args, gb)
     print(json.dumps(out), flush=True)
 
