@@ -526,7 +526,11 @@ def run_zipf_spmv(args, gb):
     dx = gb.uniform_device(2, n, stream=0)
     dy = gb.DeviceArray(n)
     roof_default = spmv_roofline(gb, M, dx, dy)
-    plan, _ = M.autotune(gb.PRECISION_FP64, 5)
+    if args.csplit is None:
+        plan, _ = M.autotune(gb.PRECISION_FP64, 5)
+    else:  # a fixed plan (counter runs: the profiler's slowdown skews tuning)
+        M.replan({"csplit": args.csplit})
+        plan = M.info()["plan"]
     roof = spmv_roofline(gb, M, dx, dy)
     roof["kernel"] = "gb_spmv_kernel<double,false,false> over %d column slices + gb_split_sum" \
         % plan["csplit"] if plan["csplit"] else "gb_spmv_kernel<double,false,false>"
@@ -1200,6 +1204,8 @@ def main():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--zipf-n", type=int, default=2000000)
     ap.add_argument("--nsys", type=int, default=512)
+    ap.add_argument("--csplit", type=int, default=None,
+                    help="zipf-spmv: this many column slices instead of the autotuned plan")
     ap.add_argument("--spgemm-grid", type=int, default=150)
     ap.add_argument("--duration", type=float, default=60.0,
                     help="served: seconds of Poisson arrivals")

@@ -113,7 +113,7 @@ class _Plan(C.Structure):
                 ("window", C.c_uint32), ("rowcap", C.c_uint32),
                 ("exact", C.c_uint32), ("grid", C.c_uint32),
                 ("xcd_remap", C.c_uint32), ("csplit", C.c_uint32),
-                ("wide_index", C.c_uint32)]
+                ("wide_index", C.c_uint32), ("wide_values", C.c_uint32)]
 
 
 class _MatrixInfo(C.Structure):
@@ -121,7 +121,7 @@ class _MatrixInfo(C.Structure):
                 ("nnz", C.c_uint64), ("plan", _Plan),
                 ("nblocks_stream", C.c_uint32), ("nrows_wave", C.c_uint32),
                 ("nrows_block", C.c_uint32), ("hist", C.c_uint64 * 33),
-                ("index16", C.c_uint32)]
+                ("index16", C.c_uint32), ("values8", C.c_uint32)]
 
 
 class CgdInfo(C.Structure):
@@ -386,7 +386,7 @@ def _plan(plan):
               plan.get("window", 0), plan.get("rowcap", 0),
               1 if plan.get("exact") else 0, plan.get("grid", 0),
               int(plan.get("xcd_remap", 0)), plan.get("csplit", 0),
-              1 if plan.get("wide_index") else 0)
+              1 if plan.get("wide_index") else 0, 1 if plan.get("wide_values") else 0)
     return C.byref(p)
 
 
@@ -439,10 +439,11 @@ class Matrix:
                          "window": i.plan.window, "rowcap": i.plan.rowcap,
                          "exact": i.plan.exact, "grid": i.plan.grid,
                          "xcd_remap": i.plan.xcd_remap, "csplit": i.plan.csplit,
-                         "wide_index": i.plan.wide_index},
+                         "wide_index": i.plan.wide_index,
+                         "wide_values": i.plan.wide_values},
                 "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
                 "nrows_block": i.nrows_block, "hist": list(i.hist),
-                "index16": i.index16}
+                "index16": i.index16, "values8": i.values8}
 
     def replan(self, plan=None):
         check(lib().gooseberry_matrix_replan(self.h, _plan(plan)), "gooseberry_matrix_replan")

@@ -97,6 +97,9 @@ typedef struct gbd_plan_params {
   uint32_t wide_index; /* 1: the row blocks read the 32-bit column indices;
                           0: 16-bit offsets from the block's first row when
                           every block's offsets fit in int16 */
+  uint32_t wide_values;/* 1: the values are read as stored; 0: as 8-bit
+                          indices into a table of the distinct values when
+                          there are at most 256 of them (no column split) */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram
@@ -131,6 +134,10 @@ int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
     uint32_t *nrows_wave, uint32_t *nrows_block, uint64_t *hist33);
 /* 1 when the plan's row blocks read 16-bit column offsets */
 uint32_t gbd_matrix_index16(const gbd_matrix *m);
+/* number of distinct values of a value-indexed plan (0: plain values) */
+uint32_t gbd_matrix_values8(const gbd_matrix *m);
+/* number of distinct values of a value-indexed plan (0: plain values) */
+uint32_t gbd_matrix_values8(const gbd_matrix *m);
 const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m);
 const uint32_t *gbd_matrix_rowind(const gbd_matrix *m);
 const double *gbd_matrix_rowval(const gbd_matrix *m);

@@ -38,6 +38,7 @@ int gb_ensure_init();
 hipError_t gb_dev_alloc(void **p, size_t bytes);
 hipError_t gb_dev_free(void *p);
 bool gb_graphs_enabled();  // false when GOOSEBERRY_NO_GRAPHS is set
+uint32_t gb_num_cus();     // compute units of the device
 
 static inline uint32_t gb_div_up(uint64_t a, uint64_t b) {
   return (uint32_t)((a + b - 1) / b);

@@ -23,11 +23,23 @@ struct gbd_matrix {
   // plan
   uint32_t nstream;     // LDS-staged row blocks
   uint32_t *blk;        // 2 * nstream: [row start, row end) per block
+  uint4 *bdesc;         // nstream: a block's first row, end row, first entry
+                        // and entry count, read by the SpMV in one load
   uint32_t nwave;       // rows processed one wavefront per row
   uint32_t nblockrows;  // rows processed one workgroup per row
   uint32_t *rowbins;    // stable bin permutation: short | wave | block rows
   int16_t *dcol;        // nnz or null: row-block entries' column - block's
                         // first row (valid only in the row blocks)
+  // value-indexed plan (plan wide_values = 0, at most GB_VDICT_MAX distinct
+  // values, no column split): each entry's value as an 8-bit index into a
+  // table of the distinct values, read instead of val / val32 (vtab32 =
+  // the FP32 roundings of vtab)
+  uint8_t *vidx;        // nnz or null
+  double *vtab;         // GB_VDICT_MAX
+  float *vtab32;        // GB_VDICT_MAX, with val32
+  uint32_t nvals;
+  uint32_t pipe_grid;   // > 0: the SpMV runs the persistent pipelined
+                        // row-block kernel on this many workgroups
   uint64_t hist[33];
   // column split (params.csplit = P > 1): P views, view r holding the
   // entries of columns [r W, (r + 1) W), W = ceil(ncols / P), in row order;
@@ -47,6 +59,17 @@ struct gbd_matrix {
 };
 
 #define GB_SPLIT_MAX 64
+#define GB_VDICT_MAX 256
+
+// One matrix entry's value: val[k], or vtab[vidx[k]] on a value-indexed
+// plan (VD).  Plain FP64 values are streamed with non-temporal loads.
+template <typename T, bool VD>
+__device__ __forceinline__ T gb_ld_val(const T *__restrict__ val,
+                                       const uint8_t *__restrict__ vidx,
+                                       const T *__restrict__ vtab, uint32_t k) {
+  if (VD) return vtab[vidx[k]];
+  return __builtin_nontemporal_load(&val[k]);
+}
 
 // workgroups of the split sum (fixed, so the partial count is known)
 static inline uint32_t gb_split_sum_grid(const gbd_matrix *m) {
@@ -59,11 +82,13 @@ static inline uint32_t gb_split_sum_grid(const gbd_matrix *m) {
 // strides of index and value loads, then their four gathers, are issued
 // before the products are added, so a lane keeps four gathers in flight;
 // the additions are those of the plain strided loop, in the same order.
-template <typename T, uint32_t STRIDE>
+template <typename T, uint32_t STRIDE, bool VD = false>
 __device__ __forceinline__ T gb_row_lane_sum(const uint32_t *__restrict__ rowind,
                                              const T *__restrict__ val,
                                              const T *__restrict__ x, uint32_t k0,
-                                             uint32_t k1, uint32_t lane) {
+                                             uint32_t k1, uint32_t lane,
+                                             const uint8_t *__restrict__ vidx = nullptr,
+                                             const T *__restrict__ vtab = nullptr) {
   T s = 0;
   uint32_t k = k0 + lane;
   for (; (uint64_t)k + 3 * STRIDE < k1; k += 4 * STRIDE) {
@@ -71,10 +96,10 @@ __device__ __forceinline__ T gb_row_lane_sum(const uint32_t *__restrict__ rowind
     const uint32_t c1 = __builtin_nontemporal_load(&rowind[k + STRIDE]);
     const uint32_t c2 = __builtin_nontemporal_load(&rowind[k + 2 * STRIDE]);
     const uint32_t c3 = __builtin_nontemporal_load(&rowind[k + 3 * STRIDE]);
-    const T v0 = __builtin_nontemporal_load(&val[k]);
-    const T v1 = __builtin_nontemporal_load(&val[k + STRIDE]);
-    const T v2 = __builtin_nontemporal_load(&val[k + 2 * STRIDE]);
-    const T v3 = __builtin_nontemporal_load(&val[k + 3 * STRIDE]);
+    const T v0 = gb_ld_val<T, VD>(val, vidx, vtab, k);
+    const T v1 = gb_ld_val<T, VD>(val, vidx, vtab, k + STRIDE);
+    const T v2 = gb_ld_val<T, VD>(val, vidx, vtab, k + 2 * STRIDE);
+    const T v3 = gb_ld_val<T, VD>(val, vidx, vtab, k + 3 * STRIDE);
     const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
     s += v0 * x0;
     s += v1 * x1;
@@ -83,7 +108,7 @@ __device__ __forceinline__ T gb_row_lane_sum(const uint32_t *__restrict__ rowind
   }
   for (; k < k1; k += STRIDE) {
     const uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-    const T v = __builtin_nontemporal_load(&val[k]);
+    const T v = gb_ld_val<T, VD>(val, vidx, vtab, k);
     s += v * x[c];
   }
   return s;

@@ -257,6 +257,11 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   mm = *m;
   mm.val = nval;
   mm.val32 = nullptr;
+  // the normalised values are read as stored (no value index)
+  mm.vidx = nullptr;
+  mm.vtab = nullptr;
+  mm.vtab32 = nullptr;
+  mm.nvals = 0;
   mm.nsplit = 0;
   mm.split = nullptr;
   if (m->nsplit && nnz) {

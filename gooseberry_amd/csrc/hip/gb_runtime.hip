@@ -86,6 +86,19 @@ static void gb_do_init() {
 
 bool gb_graphs_enabled() { return g_graphs; }
 
+static uint32_t g_num_cus = 0;
+
+uint32_t gb_num_cus() {
+  if (!g_num_cus) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    g_num_cus = (uint32_t)v;
+  }
+  return g_num_cus;
+}
+
 // Large blocks (4 MiB and up, sizes rounded to 2 MiB) that the library
 // frees are kept here and handed back to a later allocation of the same
 // rounded size, the most recently freed first.  The default pool gave such blocks back to the driver at

@@ -19,6 +19,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include "gb_matrix.h"
 
 // ------------------------------------------------------------- plan build
@@ -124,6 +126,19 @@ gb_plan_blocks(const uint32_t *__restrict__ cls,
   if (end) blk[2 * id + 1] = r + 1;
 }
 
+// Each row block's first row, end row, first entry and entry count, so
+// that the SpMV starts a block's data loads after one load instead of the
+// block table and then the row pointers.
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_plan_desc(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ blk,
+             uint32_t nst, uint4 *__restrict__ desc) {
+  const uint32_t b = blockIdx.x * GB_BLOCK + threadIdx.x;
+  if (b >= nst) return;
+  const uint32_t r0 = blk[2 * b], r1 = blk[2 * b + 1];
+  const uint32_t k0 = rowptr[r0];
+  desc[b] = make_uint4(r0, r1, k0, rowptr[r1] - k0);
+}
+
 // dcol[k] = rowind[k] - r0 for the entries of each row block [r0, r1),
 // one workgroup per block; *bad is set when an offset leaves int16.  On a
 // matrix that does not fit, most blocks find offsets out of range: one
@@ -162,6 +177,9 @@ gb_plan_narrow(const uint32_t *__restrict__ rowptr,
 // With DOT the kernel also forms the block's share of p.Ap (x = p, y = Ap)
 // and writes it to partials[blockIdx.x]; the next CG kernel adds the
 // partials in a fixed order (no fences or atomics: deterministic).
+#ifndef GB_SPMV_PIPE_WAVES
+#define GB_SPMV_PIPE_WAVES 4
+#endif
 #ifndef GB_SPMV_WAVES
 // wavefronts per SIMD the register budget must allow: 72 VGPRs (the FP64
 // variants spill 2 to scratch).  Measured on the 160^3 Laplacian with the
@@ -171,14 +189,17 @@ gb_plan_narrow(const uint32_t *__restrict__ rowptr,
 // With NARROW the row blocks read dcol, each entry's column minus the
 // block's first row as a 16-bit signed offset, instead of rowind (the plan
 // made dcol only when every block's offsets fit): 2 bytes an entry less.
-template <typename T, bool DOT, bool NARROW>
+// With VD the values are read as 8-bit indices into the table vtab of the
+// matrix's distinct values (1 byte an entry instead of 8 or 4).
+template <typename T, bool DOT, bool NARROW, bool VD>
 __global__ void __launch_bounds__(GB_BLOCK) __attribute__((amdgpu_waves_per_eu(GB_SPMV_WAVES)))
 gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ rowind,
                const int16_t *__restrict__ dcol,
-               const T *__restrict__ val, const T *__restrict__ x,
+               const T *__restrict__ val, const uint8_t *__restrict__ vidx,
+               const T *__restrict__ vtab, const T *__restrict__ x,
                T *__restrict__ y, uint32_t ystride,
-               const uint32_t *__restrict__ blk, uint32_t nstream,
+               const uint4 *__restrict__ bdesc, uint32_t nstream,
                const uint32_t *__restrict__ wave_rows, uint32_t nwave,
                uint32_t nwaveblocks,
                const uint32_t *__restrict__ block_rows, uint32_t nwork,
@@ -202,9 +223,10 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
       const uint32_t lb = remap == 0 ? w
                           : remap == 1 ? gb_xcd_remap(w, nstream)
                                        : gb_xcd_chunk_remap(w, nstream, remap);
-      const uint32_t r0 = blk[2 * lb], r1 = blk[2 * lb + 1];
-      const uint32_t k0 = rowptr[r0];
-      const uint32_t cnt = rowptr[r1] - k0;
+      const uint4 bd = bdesc[lb];
+      const uint32_t r0 = bd.x, r1 = bd.y;
+      const uint32_t k0 = bd.z;
+      const uint32_t cnt = bd.w;
       // row bounds of this thread's (at most two) rows, loaded together
       // with the block's nonzeros so that all loads are in flight at once
       const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
@@ -237,11 +259,12 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
             uint32_t k = min(k0 + i * GB_BLOCK + tid, klast);
             if (NARROW) {
               c[i] = (uint32_t)(int32_t)dcol[k];
-              if (sizeof(T) == 4) v[i] = val[k];
+              if (VD) v[i] = vtab[vidx[k]];
+              else if (sizeof(T) == 4) v[i] = val[k];
               else v[i] = __builtin_nontemporal_load(&val[k]);
             } else {
               c[i] = __builtin_nontemporal_load(&rowind[k]);
-              v[i] = __builtin_nontemporal_load(&val[k]);
+              v[i] = gb_ld_val<T, VD>(val, vidx, vtab, k);
             }
           }
         }
@@ -287,14 +310,15 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
             T pr = 0;
             if (k < k1) {
               uint32_t c = __builtin_nontemporal_load(&rowind[k]);
-              T v = __builtin_nontemporal_load(&val[k]);
+              T v = gb_ld_val<T, VD>(val, vidx, vtab, k);
               pr = v * x[c];
             }
             const uint32_t cnt = min((uint32_t)GB_WAVE, k1 - base);
             for (uint32_t j = 0; j < cnt; ++j) s += gb_readlane(pr, j);
           }
         } else {
-          s = gb_wave_sum(gb_row_lane_sum<T, GB_WAVE>(rowind, val, x, k0, k1, lane));
+          s = gb_wave_sum(
+              gb_row_lane_sum<T, GB_WAVE, VD>(rowind, val, x, k0, k1, lane, vidx, vtab));
         }
         if (lane == 0) {
           y[(size_t)row * ystride] = s;
@@ -313,7 +337,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
           const uint32_t cnt = min((uint32_t)GB_STREAM_LDS, k1 - base);
           for (uint32_t e = tid; e < cnt; e += GB_BLOCK) {
             uint32_t c = __builtin_nontemporal_load(&rowind[base + e]);
-            T v = __builtin_nontemporal_load(&val[base + e]);
+            T v = gb_ld_val<T, VD>(val, vidx, vtab, base + e);
             prod[e] = v * x[c];
           }
           __syncthreads();
@@ -326,7 +350,8 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
           if (DOT) dotp += (double)x[row] * (double)s;
         }
       } else {
-        T s = gb_wave_sum(gb_row_lane_sum<T, GB_BLOCK>(rowind, val, x, k0, k1, tid));
+        T s = gb_wave_sum(
+            gb_row_lane_sum<T, GB_BLOCK, VD>(rowind, val, x, k0, k1, tid, vidx, vtab));
         if (lane == 0) tred[wave] = s;
         __syncthreads();
         if (tid == 0) {
@@ -357,12 +382,174 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
   }
 }
 
+// Persistent, software-pipelined form for matrices whose rows are all
+// short (every work item an LDS-staged row block): workgroup w takes blocks
+// w, w + G, w + 2G, ... and issues the next block's loads (its row bounds,
+// column offsets or indices, and values or value indices) before it
+// gathers, stages and adds the current block, so one block's memory
+// latency overlaps the previous block's work.  Same products and sums as
+// gb_spmv_kernel's row blocks; with DOT one p.Ap partial per workgroup.
+template <typename T, bool DOT, bool NARROW, bool VD>
+struct gb_pipe_blk {
+  uint32_t r0, r1, k0, cnt;
+  uint32_t pa0, pa1, pb0, pb1;
+  T xa, xb;
+  uint32_t c[GB_STREAM_ITERS];
+  // values, or (VD) value indices
+  typename std::conditional<VD, uint32_t, T>::type v[GB_STREAM_ITERS];
+};
+
+template <typename T, bool DOT, bool NARROW, bool VD>
+__device__ __forceinline__ void gb_pipe_load(gb_pipe_blk<T, DOT, NARROW, VD> &B, const uint4 d,
+                                             const uint32_t *__restrict__ rowptr,
+                                             const uint32_t *__restrict__ rowind,
+                                             const int16_t *__restrict__ dcol,
+                                             const T *__restrict__ val,
+                                             const uint8_t *__restrict__ vidx,
+                                             const T *__restrict__ x, uint32_t tid) {
+  B.r0 = d.x;
+  B.r1 = d.y;
+  B.k0 = d.z;
+  B.cnt = d.w;
+  const uint32_t ra = d.x + tid, rb = d.x + tid + GB_BLOCK;
+  B.pa0 = B.pa1 = B.pb0 = B.pb1 = 0;
+  B.xa = B.xb = 0;
+  if (ra < d.y) {
+    B.pa0 = rowptr[ra];
+    B.pa1 = rowptr[ra + 1];
+    if (DOT) B.xa = x[ra];
+  }
+  if (rb < d.y) {
+    B.pb0 = rowptr[rb];
+    B.pb1 = rowptr[rb + 1];
+    if (DOT) B.xb = x[rb];
+  }
+  if (d.w == 0) return;
+  const uint32_t klast = d.z + d.w - 1;
+#pragma unroll
+  for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+    if (i * GB_BLOCK < d.w) {
+      const uint32_t k = min(d.z + i * GB_BLOCK + tid, klast);
+      if (NARROW) B.c[i] = (uint32_t)(int32_t)dcol[k];
+      else B.c[i] = __builtin_nontemporal_load(&rowind[k]);
+      if (VD) B.v[i] = vidx[k];
+      else if (NARROW && sizeof(T) == 4) B.v[i] = val[k];
+      else B.v[i] = __builtin_nontemporal_load(&val[k]);
+    }
+  }
+}
+
+template <typename T, bool DOT, bool NARROW, bool VD>
+__global__ void __launch_bounds__(GB_BLOCK) __attribute__((amdgpu_waves_per_eu(GB_SPMV_PIPE_WAVES)))
+gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
+             const int16_t *__restrict__ dcol, const T *__restrict__ val,
+             const uint8_t *__restrict__ vidx, const T *__restrict__ vtab,
+             const T *__restrict__ x, T *__restrict__ y, uint32_t ystride,
+             const uint4 *__restrict__ bdesc, uint32_t nstream, uint32_t nvals,
+             double *__restrict__ partials) {
+  __shared__ T prod[GB_STREAM_LDS];
+  __shared__ T ltab[VD ? GB_VDICT_MAX : 1];
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t G = gridDim.x;
+  if (VD) {
+    for (uint32_t i = tid; i < nvals; i += GB_BLOCK) ltab[i] = vtab[i];
+  }
+  double dotp = 0.0;
+  gb_pipe_blk<T, DOT, NARROW, VD> cur, nxt;
+  uint32_t b = blockIdx.x;
+  uint4 dn = make_uint4(0u, 0u, 0u, 0u);
+  if (b < nstream) {
+    const uint4 d0 = bdesc[b];
+    if (b + G < nstream) dn = bdesc[b + G];
+    gb_pipe_load(cur, d0, rowptr, rowind, dcol, val, vidx, x, tid);
+  }
+  if (VD) __syncthreads();  // the value table
+  for (; b < nstream; b += G) {
+    // the next block's loads go out before this block's gathers
+    const bool more = b + G < nstream;
+    uint4 dnn = make_uint4(0u, 0u, 0u, 0u);
+    if (more) {
+      if (b + 2 * G < nstream) dnn = bdesc[b + 2 * G];
+      gb_pipe_load(nxt, dn, rowptr, rowind, dcol, val, vidx, x, tid);
+    }
+    const uint32_t r0 = cur.r0, k0 = cur.k0, cnt = cur.cnt;
+    if (cnt > 0) {
+      const T *xg = NARROW ? x + r0 : x;
+      T xv[GB_STREAM_ITERS];
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        if (i * GB_BLOCK < cnt) {
+          if (NARROW) xv[i] = xg[(int32_t)cur.c[i]];
+          else xv[i] = xg[cur.c[i]];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
+        const uint32_t e = i * GB_BLOCK + tid;
+        if (i * GB_BLOCK < cnt && e < cnt) {
+          T v;
+          if constexpr (VD) v = ltab[cur.v[i]];
+          else v = cur.v[i];
+          prod[e] = v * xv[i];
+        }
+      }
+    }
+    __syncthreads();
+    // one thread per row: the reference's left-to-right sum from +0.0
+    const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
+    if (ra < cur.r1) {
+      T s = 0;
+      for (uint32_t k = cur.pa0 - k0; k < cur.pa1 - k0; ++k) s += prod[k];
+      y[(size_t)ra * ystride] = s;
+      if (DOT) dotp += (double)cur.xa * (double)s;
+    }
+    if (rb < cur.r1) {
+      T s = 0;
+      for (uint32_t k = cur.pb0 - k0; k < cur.pb1 - k0; ++k) s += prod[k];
+      y[(size_t)rb * ystride] = s;
+      if (DOT) dotp += (double)cur.xb * (double)s;
+    }
+    __syncthreads();  // prod is reused by the next block
+    if (more) {
+      cur = nxt;
+      dn = dnn;
+    }
+  }
+  if (DOT) {
+    const double ws = gb_wave_sum(dotp);
+    if ((tid & (GB_WAVE - 1)) == 0) red[tid / GB_WAVE] = ws;
+    __syncthreads();
+    if (tid == 0) {
+      double t = red[0];
+#pragma unroll
+      for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += red[q];
+      partials[blockIdx.x] = t;
+    }
+  }
+}
+
 uint32_t gb_spmv_grid(const gbd_matrix *m) {
   const uint64_t nwork = gb_spmv_nwork(m);
-  // default: one workgroup per work item (measured faster than a capped,
-  // persistent grid for the LDS-staged blocks)
-  const uint64_t gmax = m->params.grid ? m->params.grid : 0xffffffffull;
+  // one workgroup per work item, the pipelined kernel's persistent grid, or
+  // the plan's cap
+  const uint64_t gmax = m->pipe_grid ? m->pipe_grid
+                        : m->params.grid ? m->params.grid : 0xffffffffull;
   return (uint32_t)(nwork < gmax ? nwork : gmax);
+}
+
+// The pipelined row-block kernel runs matrices whose rows are all short
+// (every work item an LDS-staged row block): on the plan's grid when it
+// sets one, and by default for value-indexed plans, four workgroups per CU
+// (its register budget).  Measured on the 160^3 Laplacian with 8-bit value
+// indices: 56 against 64 us per SpMV, 100 against 111 us per CG iteration;
+// with FP64 values read as stored it is slower (78 against 68 us), the
+// stream then being HBM-bound.
+static void gb_pipe_choose(gbd_matrix *m) {
+  m->pipe_grid = 0;
+  if (m->nwave || m->nblockrows || !m->nstream) return;
+  if (m->params.grid) m->pipe_grid = m->params.grid;
+  else if (m->vidx) m->pipe_grid = 4 * gb_num_cus();
 }
 
 // ------------------------------------------------------- column split
@@ -432,9 +619,9 @@ gb_split_sum(const T *__restrict__ yp, uint32_t n, uint32_t P, T *__restrict__ y
 }
 
 template <typename T>
-static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
-                            T *y, uint32_t ystride, double *partials,
-                            const gbd_cg_state *cg);
+static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const uint8_t *vidx,
+                            const T *vtab, const T *x, T *y, uint32_t ystride,
+                            double *partials, const gbd_cg_state *cg);
 
 // a split matrix: one launch per slice into its partial vector, then the sum
 template <typename T>
@@ -446,7 +633,8 @@ static int gb_spmv_launch_split(const gbd_matrix *m, const T *x, T *y, uint32_t 
     const T *vval;
     if constexpr (sizeof(T) == 8) vval = (const T *)v->val;
     else vval = (const T *)v->val32;
-    int rv = gb_spmv_launch_t<T>(v, vval, x, yp + (size_t)r * m->nrows, 1, nullptr, nullptr);
+    int rv = gb_spmv_launch_t<T>(v, vval, nullptr, nullptr, x, yp + (size_t)r * m->nrows, 1,
+                                 nullptr, nullptr);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
   }
   hipStream_t s = gb_stream();
@@ -461,49 +649,68 @@ static int gb_spmv_launch_split(const gbd_matrix *m, const T *x, T *y, uint32_t 
   return GOOSEBERRY_SUCCESS;
 }
 
+// the persistent pipelined kernel serves plans with a grid cap on matrices
+// whose rows are all short
+static inline bool gb_spmv_piped(const gbd_matrix *m) { return m->pipe_grid != 0; }
+
+template <typename T, bool DOT, bool NARROW, bool VD>
+static void gb_spmv_go(const gbd_matrix *m, uint32_t grid, hipStream_t s, const T *val,
+                       const uint8_t *vidx, const T *vtab, const T *x, T *y,
+                       uint32_t ystride, double *partials, const gbd_cg_state *cg) {
+  if (gb_spmv_piped(m)) {
+    gb_spmv_pipe<T, DOT, NARROW, VD><<<grid, GB_BLOCK, 0, s>>>(
+        m->rowptr, m->rowind, NARROW ? m->dcol : nullptr, val, vidx, vtab, x, y, ystride,
+        m->bdesc, m->nstream, m->nvals, partials);
+    return;
+  }
+  const uint32_t *wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
+  const uint32_t *block_rows = m->rowbins + (m->nrows - m->nblockrows);
+  gb_spmv_kernel<T, DOT, NARROW, VD><<<grid, GB_BLOCK, 0, s>>>(
+      m->rowptr, m->rowind, NARROW ? m->dcol : nullptr, val, vidx, vtab, x, y, ystride,
+      m->bdesc, m->nstream, wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows,
+      gb_spmv_nwork(m), m->params.exact, m->params.xcd_remap, partials, cg);
+}
+
+template <typename T, bool DOT, bool NARROW>
+static void gb_spmv_go_vd(const gbd_matrix *m, uint32_t grid, hipStream_t s, const T *val,
+                          const uint8_t *vidx, const T *vtab, const T *x, T *y,
+                          uint32_t ystride, double *partials, const gbd_cg_state *cg) {
+  if (vidx)
+    gb_spmv_go<T, DOT, NARROW, true>(m, grid, s, val, vidx, vtab, x, y, ystride, partials, cg);
+  else
+    gb_spmv_go<T, DOT, NARROW, false>(m, grid, s, val, vidx, vtab, x, y, ystride, partials, cg);
+}
+
 template <typename T>
-static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const T *x,
-                            T *y, uint32_t ystride, double *partials,
-                            const gbd_cg_state *cg) {
+static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const uint8_t *vidx,
+                            const T *vtab, const T *x, T *y, uint32_t ystride,
+                            double *partials, const gbd_cg_state *cg) {
   if (m->nsplit) return gb_spmv_launch_split<T>(m, x, y, ystride, partials, cg);
   const uint32_t grid = gb_spmv_grid(m);
   if (grid == 0) return GOOSEBERRY_SUCCESS;
   hipStream_t s = gb_stream();
-  const uint32_t *wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
-  const uint32_t *block_rows = m->rowbins + (m->nrows - m->nblockrows);
-  const uint32_t nwork = gb_spmv_nwork(m);
-  const uint32_t nwb = gb_spmv_nwaveblocks(m);
-  const uint32_t ex = m->params.exact, rm = m->params.xcd_remap;
-  if (cg && m->dcol) {
-    gb_spmv_kernel<T, true, true><<<grid, GB_BLOCK, 0, s>>>(
-        m->rowptr, m->rowind, m->dcol, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, nwb, block_rows, nwork, ex, rm, partials, cg);
-  } else if (cg) {
-    gb_spmv_kernel<T, true, false><<<grid, GB_BLOCK, 0, s>>>(
-        m->rowptr, m->rowind, nullptr, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, nwb, block_rows, nwork, ex, rm, partials, cg);
-  } else if (m->dcol) {
-    gb_spmv_kernel<T, false, true><<<grid, GB_BLOCK, 0, s>>>(
-        m->rowptr, m->rowind, m->dcol, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, nwb, block_rows, nwork, ex, rm, nullptr, nullptr);
-  } else {
-    gb_spmv_kernel<T, false, false><<<grid, GB_BLOCK, 0, s>>>(
-        m->rowptr, m->rowind, nullptr, val, x, y, ystride, m->blk, m->nstream,
-        wave_rows, m->nwave, nwb, block_rows, nwork, ex, rm, nullptr, nullptr);
-  }
+  if (!vtab) vidx = nullptr;
+  if (cg && m->dcol)
+    gb_spmv_go_vd<T, true, true>(m, grid, s, val, vidx, vtab, x, y, ystride, partials, cg);
+  else if (cg)
+    gb_spmv_go_vd<T, true, false>(m, grid, s, val, vidx, vtab, x, y, ystride, partials, cg);
+  else if (m->dcol)
+    gb_spmv_go_vd<T, false, true>(m, grid, s, val, vidx, vtab, x, y, ystride, nullptr, nullptr);
+  else
+    gb_spmv_go_vd<T, false, false>(m, grid, s, val, vidx, vtab, x, y, ystride, nullptr, nullptr);
   GB_LAUNCH_CHECK();
   return GOOSEBERRY_SUCCESS;
 }
 
 int gb_spmv_launch(const gbd_matrix *m, const double *x, double *y,
                    uint32_t ystride, double *partials, const gbd_cg_state *cg) {
-  return gb_spmv_launch_t<double>(m, m->val, x, y, ystride, partials, cg);
+  return gb_spmv_launch_t<double>(m, m->val, m->vidx, m->vtab, x, y, ystride, partials, cg);
 }
 
 int gb_spmv_launch_f32(const gbd_matrix *m, const float *x, float *y,
                        double *partials, const gbd_cg_state *cg) {
   if (!m->val32) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  return gb_spmv_launch_t<float>(m, m->val32, x, y, 1, partials, cg);
+  return gb_spmv_launch_t<float>(m, m->val32, m->vidx, m->vtab32, x, y, 1, partials, cg);
 }
 
 __global__ void gb_to_f32(const double *__restrict__ in, float *__restrict__ out,
@@ -550,6 +757,232 @@ gb_jacobi_vals(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__
     for (uint32_t k = k0 + lane; k < k1; k += GB_WAVE)
       out[k] = (float)((val[k] * si) * scale[rowind[k]]);
   }
+}
+
+// ------------------------------------------------- value-indexed plans
+// The distinct values of an array (64-bit patterns of FP64 values, or
+// 32-bit patterns of FP32 values zero-extended) are collected in an
+// open-addressing table: each workgroup first deduplicates its chunk in
+// LDS and merges only its distinct keys into the global table; more than
+// GB_VDICT_MAX distinct keys in a chunk, or the all-ones pattern (the empty
+// mark), sets *over and the plan keeps plain values.
+#define GB_VD_GTAB 1024
+#define GB_VD_LTAB 512
+#define GB_VD_EMPTY 0xffffffffffffffffull
+#define GB_VD_CHUNK 65536
+
+__device__ __forceinline__ uint32_t gb_vd_hash(unsigned long long k) {
+  // the 64-bit finaliser of MurmurHash3
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+template <typename K>
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_vd_collect(const K *__restrict__ vals, uint64_t nnz, unsigned long long *gkeys,
+              uint32_t *over) {
+  __shared__ unsigned long long lk[GB_VD_LTAB];
+  __shared__ uint32_t lcount;
+  __shared__ volatile uint32_t lover;
+  for (uint32_t i = threadIdx.x; i < GB_VD_LTAB; i += GB_BLOCK) lk[i] = GB_VD_EMPTY;
+  if (threadIdx.x == 0) {
+    lcount = 0;
+    lover = __hip_atomic_load(over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (lover) return;
+  const uint64_t b = (uint64_t)blockIdx.x * GB_VD_CHUNK;
+  const uint64_t e = nnz - b < GB_VD_CHUNK ? nnz : b + GB_VD_CHUNK;
+  for (uint64_t k = b + threadIdx.x; k < e; k += GB_BLOCK) {
+    const unsigned long long key = (unsigned long long)vals[k];
+    if (key == GB_VD_EMPTY) {
+      lover = 1;
+      break;
+    }
+    uint32_t h = gb_vd_hash(key) & (GB_VD_LTAB - 1);
+    bool placed = false;
+    for (uint32_t p = 0; p < GB_VD_LTAB && !placed; ++p) {
+      const unsigned long long cur = ((volatile unsigned long long *)lk)[h];
+      if (cur == key) {
+        placed = true;
+      } else if (cur == GB_VD_EMPTY) {
+        const unsigned long long prev = atomicCAS(&lk[h], GB_VD_EMPTY, key);
+        if (prev == GB_VD_EMPTY) {
+          if (atomicAdd(&lcount, 1u) >= GB_VDICT_MAX) lover = 1;
+          placed = true;
+        } else if (prev == key) {
+          placed = true;
+        }
+      }
+      if (!placed) h = (h + 1) & (GB_VD_LTAB - 1);
+    }
+    if (!placed) lover = 1;
+    if (lover) break;
+  }
+  __syncthreads();
+  if (lover) {
+    if (threadIdx.x == 0) atomicOr(over, 1u);
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < GB_VD_LTAB; i += GB_BLOCK) {
+    const unsigned long long key = lk[i];
+    if (key == GB_VD_EMPTY) continue;
+    uint32_t h = gb_vd_hash(key) & (GB_VD_GTAB - 1);
+    bool placed = false;
+    for (uint32_t p = 0; p < GB_VD_GTAB && !placed; ++p) {
+      const unsigned long long prev = atomicCAS(&gkeys[h], GB_VD_EMPTY, key);
+      if (prev == GB_VD_EMPTY || prev == key) placed = true;
+      else h = (h + 1) & (GB_VD_GTAB - 1);
+    }
+    if (!placed) atomicOr(over, 1u);
+  }
+}
+
+// idx[k] = the table position of vals[k] (every value is in the table)
+template <typename K>
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_vd_map(const K *__restrict__ vals, uint64_t nnz, const unsigned long long *__restrict__ gkeys,
+          const uint8_t *__restrict__ slot2id, uint8_t *__restrict__ idx) {
+  __shared__ unsigned long long lk[GB_VD_GTAB];
+  __shared__ uint8_t lid[GB_VD_GTAB];
+  for (uint32_t i = threadIdx.x; i < GB_VD_GTAB; i += GB_BLOCK) {
+    lk[i] = gkeys[i];
+    lid[i] = slot2id[i];
+  }
+  __syncthreads();
+  for (uint64_t k = (uint64_t)blockIdx.x * GB_BLOCK + threadIdx.x; k < nnz;
+       k += (uint64_t)gridDim.x * GB_BLOCK) {
+    const unsigned long long key = (unsigned long long)vals[k];
+    uint32_t h = gb_vd_hash(key) & (GB_VD_GTAB - 1);
+    for (uint32_t p = 0; p < GB_VD_GTAB && lk[h] != key; ++p) h = (h + 1) & (GB_VD_GTAB - 1);
+    idx[k] = lid[h];
+  }
+}
+
+// An 8-bit index per entry and the table of the distinct values of vals
+// (width 8: FP64, 4: FP32; the table ordered by bit pattern, so the arrays
+// are the same in every run).  *built = false, and nothing allocated, when
+// there are more than GB_VDICT_MAX distinct values.
+static int gb_vdict_build(const void *vals, int width, uint64_t nnz, uint8_t **r_idx,
+                          void **r_tab, uint32_t *r_nvals, bool *built) {
+  hipStream_t s = gb_stream();
+  *built = false;
+  *r_idx = nullptr;
+  *r_tab = nullptr;
+  if (nnz == 0) return GOOSEBERRY_SUCCESS;
+  unsigned long long *gkeys = nullptr;
+  uint32_t *over = nullptr;
+  GB_HIP(gb_dev_alloc((void **)&gkeys, GB_VD_GTAB * 8 + 256));
+  over = (uint32_t *)(gkeys + GB_VD_GTAB);
+  GB_HIP(hipMemsetAsync(gkeys, 0xff, GB_VD_GTAB * 8, s));
+  GB_HIP(hipMemsetAsync(over, 0, 4, s));
+  const uint64_t nwg = (nnz + GB_VD_CHUNK - 1) / GB_VD_CHUNK;
+  if (nwg > 0x7fffffffull) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (width == 8)
+    gb_vd_collect<uint64_t><<<(uint32_t)nwg, GB_BLOCK, 0, s>>>((const uint64_t *)vals, nnz,
+                                                               gkeys, over);
+  else
+    gb_vd_collect<uint32_t><<<(uint32_t)nwg, GB_BLOCK, 0, s>>>((const uint32_t *)vals, nnz,
+                                                               gkeys, over);
+  GB_LAUNCH_CHECK();
+  unsigned long long hk[GB_VD_GTAB];
+  uint32_t hover = 0;
+  GB_HIP(hipMemcpyAsync(hk, gkeys, sizeof(hk), hipMemcpyDeviceToHost, s));
+  GB_HIP(hipMemcpyAsync(&hover, over, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  unsigned long long keys[GB_VDICT_MAX];
+  uint32_t nk = 0;
+  for (uint32_t i = 0; i < GB_VD_GTAB && !hover; ++i) {
+    if (hk[i] == GB_VD_EMPTY) continue;
+    if (nk == GB_VDICT_MAX) hover = 1;
+    else keys[nk++] = hk[i];
+  }
+  if (hover) {
+    GB_HIP(gb_dev_free(gkeys));
+    return GOOSEBERRY_SUCCESS;
+  }
+  // table order: ascending bit pattern
+  for (uint32_t i = 1; i < nk; ++i)
+    for (uint32_t j = i; j > 0 && keys[j - 1] > keys[j]; --j) {
+      const unsigned long long t = keys[j];
+      keys[j] = keys[j - 1];
+      keys[j - 1] = t;
+    }
+  uint8_t hid[GB_VD_GTAB];
+  memset(hid, 0, sizeof(hid));
+  for (uint32_t i = 0; i < GB_VD_GTAB; ++i) {
+    if (hk[i] == GB_VD_EMPTY) continue;
+    uint32_t lo = 0, hi = nk;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (keys[mid] < hk[i]) lo = mid + 1; else hi = mid;
+    }
+    hid[i] = (uint8_t)lo;
+  }
+  unsigned char htab[GB_VDICT_MAX * 8];
+  memset(htab, 0, sizeof(htab));
+  for (uint32_t i = 0; i < nk; ++i) {
+    if (width == 8) {
+      const uint64_t b = keys[i];
+      memcpy(htab + 8 * i, &b, 8);
+    } else {
+      const uint32_t b = (uint32_t)keys[i];
+      memcpy(htab + 4 * i, &b, 4);
+    }
+  }
+  uint8_t *slot2id = nullptr, *idx = nullptr;
+  void *tab = nullptr;
+  GB_HIP(gb_dev_alloc((void **)&slot2id, GB_VD_GTAB));
+  GB_HIP(gb_dev_alloc(&tab, GB_VDICT_MAX * (size_t)width));
+  GB_HIP(gb_dev_alloc((void **)&idx, nnz));
+  GB_HIP(hipMemcpyAsync(slot2id, hid, GB_VD_GTAB, hipMemcpyHostToDevice, s));
+  GB_HIP(hipMemcpyAsync(tab, htab, GB_VDICT_MAX * (size_t)width, hipMemcpyHostToDevice, s));
+  const uint64_t g = (nnz + GB_BLOCK - 1) / GB_BLOCK;
+  const uint32_t grid = g > 16384 ? 16384u : (uint32_t)g;
+  if (width == 8)
+    gb_vd_map<uint64_t><<<grid, GB_BLOCK, 0, s>>>((const uint64_t *)vals, nnz, gkeys, slot2id, idx);
+  else
+    gb_vd_map<uint32_t><<<grid, GB_BLOCK, 0, s>>>((const uint32_t *)vals, nnz, gkeys, slot2id, idx);
+  GB_LAUNCH_CHECK();
+  // the host staging (hid, htab) must outlive the copies
+  GB_HIP(hipStreamSynchronize(s));
+  GB_HIP(gb_dev_free(slot2id));
+  GB_HIP(gb_dev_free(gkeys));
+  *r_idx = idx;
+  *r_tab = tab;
+  *r_nvals = nk;
+  *built = true;
+  return GOOSEBERRY_SUCCESS;
+}
+
+// vtab32 = the FP32 roundings of vtab (the FP32 copy of a value-indexed
+// matrix shares its indices)
+static int gb_vdict_f32(gbd_matrix *m) {
+  if (!m->vidx || m->vtab32) return GOOSEBERRY_SUCCESS;
+  hipStream_t s = gb_stream();
+  double h64[GB_VDICT_MAX];
+  float h32[GB_VDICT_MAX];
+  GB_HIP(hipMemcpyAsync(h64, m->vtab, sizeof(h64), hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < GB_VDICT_MAX; ++i) h32[i] = (float)h64[i];
+  GB_HIP(gb_dev_alloc((void **)&m->vtab32, sizeof(h32)));
+  GB_HIP(hipMemcpyAsync(m->vtab32, h32, sizeof(h32), hipMemcpyHostToDevice, s));
+  GB_HIP(hipStreamSynchronize(s));
+  return GOOSEBERRY_SUCCESS;
+}
+
+static void gb_vdict_free(gbd_matrix *m) {
+  (void)gb_dev_free(m->vidx);
+  (void)gb_dev_free(m->vtab);
+  (void)gb_dev_free(m->vtab32);
+  m->vidx = nullptr;
+  m->vtab = nullptr;
+  m->vtab32 = nullptr;
+  m->nvals = 0;
 }
 
 // ------------------------------------------------------------- host side
@@ -652,9 +1085,12 @@ static int gb_plan_build(gbd_matrix *m) {
   hipStream_t s = gb_stream();
   const uint32_t n = m->nrows;
   if (!m->view) gb_split_free(m);
+  gb_vdict_free(m);
+  m->pipe_grid = 0;
   memset(m->hist, 0, sizeof(m->hist));
   m->nstream = m->nwave = m->nblockrows = 0;
   if (m->blk) { GB_HIP(gb_dev_free(m->blk)); m->blk = nullptr; }
+  if (m->bdesc) { GB_HIP(gb_dev_free(m->bdesc)); m->bdesc = nullptr; }
   if (m->rowbins) { GB_HIP(gb_dev_free(m->rowbins)); m->rowbins = nullptr; }
   if (m->dcol) { GB_HIP(gb_dev_free(m->dcol)); m->dcol = nullptr; }
   if (n == 0) return GOOSEBERRY_SUCCESS;
@@ -690,6 +1126,10 @@ static int gb_plan_build(gbd_matrix *m) {
     gb_plan_blocks<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(cls, flag, idx,
                                                               n, m->blk);
     GB_LAUNCH_CHECK();
+    GB_HIP(gb_dev_alloc((void **)&m->bdesc, (size_t)nst * sizeof(uint4)));
+    gb_plan_desc<<<gb_div_up(nst, GB_BLOCK), GB_BLOCK, 0, s>>>(m->rowptr, m->blk, nst,
+                                                              m->bdesc);
+    GB_LAUNCH_CHECK();
   }
   GB_HIP(gb_dev_free(cls));
   GB_HIP(gb_dev_free(flag));
@@ -720,6 +1160,19 @@ static int gb_plan_build(gbd_matrix *m) {
       return rv2;
     }
   }
+  if (!m->view && !m->params.wide_values && !m->nsplit && m->nnz) {
+    // 8-bit value indices when the matrix has few distinct values
+    bool built = false;
+    void *tab = nullptr;
+    int rv3 = gb_vdict_build(m->val, 8, m->nnz, &m->vidx, &tab, &m->nvals, &built);
+    m->vtab = (double *)tab;
+    if (rv3 == GOOSEBERRY_SUCCESS && m->val32) rv3 = gb_vdict_f32(m);
+    if (rv3 != GOOSEBERRY_SUCCESS) {
+      gb_vdict_free(m);
+      return rv3;
+    }
+  }
+  gb_pipe_choose(m);
   return GOOSEBERRY_SUCCESS;
 }
 
@@ -739,6 +1192,7 @@ void gbd_plan_defaults(const uint64_t *hist, uint64_t nrows, uint64_t nnz,
   p->xcd_remap = 0;
   p->csplit = 0;
   p->wide_index = 0;
+  p->wide_values = 0;
 }
 
 int gbd_csr_check(uint32_t nrows, uint32_t ncols, const uint32_t *d_rowptr,
@@ -835,11 +1289,13 @@ void gbd_matrix_destroy(gbd_matrix *m) {
   // pool frees are ordered after the queued work that reads the arrays
   if (m->view) {
     (void)gb_dev_free(m->blk);
+    (void)gb_dev_free(m->bdesc);
     (void)gb_dev_free(m->rowbins);
     free(m);
     return;
   }
   gb_split_free(m);
+  gb_vdict_free(m);
   if (m->rowptr) (void)gb_dev_free(m->rowptr);
   if (m->rowind) (void)gb_dev_free(m->rowind);
   if (m->val) (void)gb_dev_free(m->val);
@@ -847,6 +1303,7 @@ void gbd_matrix_destroy(gbd_matrix *m) {
   if (m->val32j) (void)gb_dev_free(m->val32j);
   if (m->jscale) (void)gb_dev_free(m->jscale);
   if (m->blk) (void)gb_dev_free(m->blk);
+  if (m->bdesc) (void)gb_dev_free(m->bdesc);
   if (m->rowbins) (void)gb_dev_free(m->rowbins);
   if (m->dcol) (void)gb_dev_free(m->dcol);
   free(m);
@@ -868,6 +1325,8 @@ int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
 }
 
 uint32_t gbd_matrix_index16(const gbd_matrix *m) { return m->dcol ? 1u : 0u; }
+
+uint32_t gbd_matrix_values8(const gbd_matrix *m) { return m->vidx ? m->nvals : 0u; }
 
 uint32_t gbd_matrix_holders(const gbd_matrix *m) { return m->holders; }
 
@@ -892,7 +1351,7 @@ int gbd_matrix_make_f32(gbd_matrix *m) {
     }
     for (uint32_t r = 0; r < m->nsplit; ++r) m->split[r]->val32 = m->sval32;
   }
-  return GOOSEBERRY_SUCCESS;
+  return gb_vdict_f32(m);
 }
 
 int gbd_matrix_make_f32_jacobi(gbd_matrix *m) {

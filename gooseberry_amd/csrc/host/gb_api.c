@@ -195,6 +195,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   d.xcd_remap = p->xcd_remap;
   d.csplit = p->csplit > 1 ? p->csplit : 0;
   d.wide_index = p->wide_index ? 1 : 0;
+  d.wide_values = p->wide_values ? 1 : 0;
   *out = d;
   *have = 1;
 }
@@ -359,7 +360,9 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.xcd_remap = p.xcd_remap;
   info->plan.csplit = p.csplit;
   info->plan.wide_index = p.wide_index;
+  info->plan.wide_values = p.wide_values;
   info->index16 = gbd_matrix_index16(mat->m);
+  info->values8 = gbd_matrix_values8(mat->m);
   return GOOSEBERRY_SUCCESS;
 }
 

@@ -168,6 +168,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
   pub.xcd_remap = best.xcd_remap;
   pub.csplit = best.csplit;
   pub.wide_index = best.wide_index;
+  pub.wide_values = best.wide_values;
   /* through the public replan: it also drops the CG workspace, which is
    * sized by the plan */
   if ((rv = gooseberry_matrix_replan(mat, &pub)) != GOOSEBERRY_SUCCESS) goto done;

@@ -378,6 +378,11 @@ typedef struct gooseberry_plan_t {
                        0: they read each entry's column as a 16-bit offset
                        from the block's first row (2 bytes an entry less),
                        when every offset fits (same results either way) */
+  dim_t wide_values; /* 1: the values are read as stored; 0: when the matrix
+                       has at most 256 distinct values (and no column split),
+                       each entry's value is read as an 8-bit index into a
+                       table of them (1 byte an entry instead of 8; CSR-VI,
+                       the same values, so the same results) */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */
@@ -440,6 +445,8 @@ typedef struct gooseberry_matrix_info_t {
   dim_t nrows_block;      /* rows handled one workgroup per row */
   uint64_t hist[33];      /* rows with floor(log2(len+1)) == b */
   dim_t index16;          /* 1: the row blocks read 16-bit column offsets */
+  dim_t values8;          /* distinct values of a value-indexed plan (0: the
+                             values are read as stored) */
 } gooseberry_matrix_info_t;
 
 int gooseberry_matrix_info(

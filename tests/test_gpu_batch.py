@@ -452,3 +452,21 @@ def test_server_jacobi_alone_equals_mix(gb, oracle):
         while not S1.progress(8):
             pass
         assert np.array_equal(S1.take_result(0, n), x)
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_batch_value_index_same_bits(gb, precision):
+    # value-indexed plans (the Laplacians have 2 distinct values, their
+    # Jacobi-scaled FP32 values too) give the batched solves the same bits
+    rng = np.random.default_rng(12)
+    shapes = [(2, 45), (3, 20), (2, 28)]
+    hosts = [gb.laplacian(d, s) for d, s in shapes]
+    bs = [rng.uniform(-1, 1, rp.size - 1) for rp, _, _ in hosts]
+    vi = [gb.Matrix(*h) for h in hosts]
+    wide = [gb.Matrix(*h, plan={"wide_values": 1}) for h in hosts]
+    assert all(M.info()["values8"] == 2 for M in vi)
+    ra = solve_batch(gb, list(zip(vi, bs)), 1e-10, precision)
+    rb = solve_batch(gb, list(zip(wide, bs)), 1e-10, precision)
+    for (xa, ia), (xb, ib) in zip(ra, rb):
+        assert ia.status == ib.status == 1 and ia.niter == ib.niter
+        assert np.array_equal(xa, xb)

@@ -391,3 +391,95 @@ def test_zipf_spmv_rel_inf(gb, oracle, plan):
     y = M.spmv(x)
     ref = oracle.spmult(n, n, 1, rp, ri, rv, x)
     assert rel_inf(y, ref) <= TOL, (M.info()["plan"], rel_inf(y, ref))
+
+
+# ---- value-indexed plans (8-bit value indices into a table, CSR-VI)
+
+def _few_values(n, seed, nvals, lengths=(), density=0.002):
+    rp, ri, _ = irregular(n, seed, lengths, density)
+    rng = np.random.default_rng(seed + 100)
+    table = rng.uniform(-1, 1, nvals)
+    rv = table[rng.integers(0, nvals, int(rp[-1]))]
+    return rp, ri, rv
+
+
+def test_value_index_laplacian_same_bits(gb, oracle):
+    rp, ri, rv = gb.laplacian(3, 40)
+    n = rp.size - 1
+    x = gb.uniform(3, n)
+    Mv = gb.Matrix(rp, ri, rv)
+    Mw = gb.Matrix(rp, ri, rv, plan={"wide_values": 1})
+    assert Mv.info()["values8"] == 2 and Mw.info()["values8"] == 0
+    yv, yw = Mv.spmv(x), Mw.spmv(x)
+    assert np.array_equal(yv, yw) and np.array_equal(yv, oracle.spmv(rp, ri, rv, x))
+    xf = x.astype(np.float32)
+    assert np.array_equal(Mv.spmv_f32(xf), Mw.spmv_f32(xf))
+    b = gb.uniform(4, n)
+    xv, iv = Mv.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    xw, iw = Mw.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    assert iv.niter == iw.niter and np.array_equal(xv, xw)
+
+
+@pytest.mark.parametrize("nvals,expect", [(1, 1), (200, 200), (256, 256), (257, 0)])
+def test_value_index_table_sizes_and_long_rows(gb, oracle, nvals, expect):
+    # long rows take the wavefront and workgroup paths, which read the
+    # indexed values too
+    rp, ri, rv = _few_values(30000, 7, nvals, lengths=[(5, 3000), (6, 9000), (7, 0)])
+    x = np.random.default_rng(2).uniform(-1, 1, 30000)
+    M = gb.Matrix(rp, ri, rv)
+    Mw = gb.Matrix(rp, ri, rv, plan={"wide_values": 1})
+    assert M.info()["values8"] == expect
+    y, yw = M.spmv(x), Mw.spmv(x)
+    assert np.array_equal(y, yw)
+    assert rel_inf(y, oracle.spmv(rp, ri, rv, x)) <= TOL
+    Me = gb.Matrix(rp, ri, rv, plan={"exact": 1})
+    assert np.array_equal(Me.spmv(x), oracle.spmv(rp, ri, rv, x))
+
+
+def test_value_index_signed_zero_nan_and_replans(gb, oracle):
+    rp, ri, _ = irregular(5000, 9, density=0.004)
+    nnz = int(rp[-1])
+    rng = np.random.default_rng(10)
+    rv = np.where(rng.uniform(0, 1, nnz) < 0.5, 0.0, -0.0)
+    rv[::7] = 2.5
+    x = rng.uniform(-1, 1, 5000)
+    x[3] = np.inf  # 0 * inf = nan, -0 * inf = nan: the same in either plan
+    M = gb.Matrix(rp, ri, rv)
+    assert M.info()["values8"] == 3
+    Mw = gb.Matrix(rp, ri, rv, plan={"wide_values": 1})
+    y, yw = M.spmv(x), Mw.spmv(x)
+    assert np.array_equal(y.view(np.uint64), yw.view(np.uint64))
+    # a column split drops the index, the default plan brings it back
+    M.replan({"csplit": 4})
+    assert M.info()["values8"] == 0
+    M.replan(None)
+    assert M.info()["values8"] == 3
+    assert np.array_equal(M.spmv(x).view(np.uint64), yw.view(np.uint64))
+
+
+def test_value_index_random_values_not_indexed(gb):
+    M = gb.generate_sdd(20000, 5, 3, 27)
+    assert M.info()["values8"] == 0
+
+
+# ---- persistent pipelined row blocks (plan grid on matrices of short rows)
+
+@pytest.mark.parametrize("grid", [1, 7, 1024, 65536])
+@pytest.mark.parametrize("wide_values", [0, 1])
+def test_pipelined_row_blocks_same_bits(gb, oracle, grid, wide_values):
+    rp, ri, rv = gb.laplacian(3, 36)
+    n = rp.size - 1
+    x = gb.uniform(6, n)
+    ref = gb.Matrix(rp, ri, rv)
+    M = gb.Matrix(rp, ri, rv, plan={"grid": grid, "wide_values": wide_values})
+    y = M.spmv(x)
+    assert np.array_equal(y, ref.spmv(x)) and np.array_equal(y, oracle.spmv(rp, ri, rv, x))
+    xf = x.astype(np.float32)
+    assert np.array_equal(M.spmv_f32(xf), ref.spmv_f32(xf))
+    B = np.random.default_rng(8).uniform(-1, 1, (n, 3))
+    assert np.array_equal(M.spmm(B.T.ravel(), 3), ref.spmm(B.T.ravel(), 3))
+    b = gb.uniform(7, n)
+    x1, i1 = M.cgd(b, 1e-10, gb.TOL_RELATIVE)
+    xr, it, _, _ = oracle.cgd(rp, ri, rv, b, 1e-10 * np.linalg.norm(b))
+    assert i1.status == 1 and abs(int(i1.niter) - it) <= 2
+    assert np.linalg.norm(b - oracle.spmv(rp, ri, rv, x1)) <= 1.05e-10 * np.linalg.norm(b)
