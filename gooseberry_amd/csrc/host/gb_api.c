@@ -389,7 +389,11 @@ gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat) { return mat->m; }
 /* src/blas.c:552-606 through the GPU.  With blocks, the reference computes
  * only the rows inside [blocks[0], blocks[nblocks]) (the union of the
  * blocks of a non-decreasing blocks array); other rows of crowval are left
- * untouched, as there. */
+ * untouched, as there.  Only those rows are uploaded and planned, their row
+ * pointers rebased to start at 0 (so a rowptr that does not start at 0 is
+ * read as the reference reads it).  The matrix lives for this one call, so
+ * its plan skips the 16-bit column offsets and the value table: building
+ * them costs more than one SpMV saves (the products are the same). */
 int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
     const ind_t *arowptr, const dim_t *arowind, const real_t *arowval,
     const real_t *bcolval, real_t *crowval, const dim_t *blocks,
@@ -406,25 +410,44 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
     rhi = nblocks ? blocks[nblocks] : 0;
   }
   if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
-  if (!bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!arowptr || !bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  const dim_t nr = rhi - rlo;
+  const ind_t base = arowptr[rlo];
+  const ind_t *rp = arowptr + rlo;
+  ind_t *rebased = NULL;
+  if (base) {
+    rebased = malloc(((size_t)nr + 1) * sizeof(ind_t));
+    if (!rebased) return GOOSEBERRY_ERROR_NOMEMORY;
+    for (dim_t i = 0; i <= nr; ++i) {
+      if (arowptr[rlo + i] < base) {
+        free(rebased);
+        return GOOSEBERRY_ERROR_INVALIDINPUT;
+      }
+      rebased[i] = arowptr[rlo + i] - base;
+    }
+    rp = rebased;
+  }
+  gooseberry_plan_t once;
+  memset(&once, 0, sizeof(once));
+  once.wide_index = 1;
+  once.wide_values = 1;
   gooseberry_matrix_t mat = NULL;
-  int rv = gooseberry_matrix_create(anrows, ancols, arowptr, arowind, arowval,
-      NULL, &mat);
+  int rv = gooseberry_matrix_create(nr, ancols, rp,
+      arowind ? arowind + base : NULL, arowval ? arowval + base : NULL,
+      &once, &mat);
+  free(rebased);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   void *db = NULL, *dc = NULL;
   size_t bbytes = (size_t)ancols * bncols * sizeof(real_t);
-  size_t cbytes = (size_t)anrows * bncols * sizeof(real_t);
+  size_t cbytes = (size_t)nr * bncols * sizeof(real_t);
   rv = gb_upload(&db, bcolval, bbytes);
   if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dc, cbytes);
   if (rv == GOOSEBERRY_SUCCESS) {
     rv = (bncols == 1) ? gbd_spmv(mat->m, db, dc)
                        : gbd_spmm(mat->m, bncols, db, dc);
   }
-  if (rv == GOOSEBERRY_SUCCESS) {
-    size_t off = (size_t)rlo * bncols;
-    rv = gbd_dtoh(crowval + off, (real_t *)dc + off,
-        (size_t)(rhi - rlo) * bncols * sizeof(real_t));
-  }
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtoh(crowval + (size_t)rlo * bncols, dc, cbytes);
   gbd_free(db);
   gbd_free(dc);
   gooseberry_matrix_free(mat);

@@ -702,30 +702,50 @@ int gooseberry_spmult_f32(dim_t anrows, dim_t ancols, dim_t bncols,
   }
   if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
   if (!arowptr || !bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  const size_t nnz = arowptr[anrows];
+  /* rows [rlo, rhi) only, row pointers rebased to 0, one-call plan: as
+   * gooseberry_spmult */
+  const dim_t nr = rhi - rlo;
+  const ind_t base = arowptr[rlo];
+  if (arowptr[rhi] < base) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  const size_t nnz = arowptr[rhi] - base;
   if (nnz && !arowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
   /* FP64 copies of FP32 values are exact, and so is their FP32 copy back */
   real_t *v64 = malloc((nnz ? nnz : 1) * sizeof(real_t));
-  float *col = malloc((size_t)anrows * sizeof(float));
-  if (!v64 || !col) { free(v64); free(col); return GOOSEBERRY_ERROR_NOMEMORY; }
-  for (size_t k = 0; k < nnz; ++k) v64[k] = arowval[k];
+  float *col = malloc((size_t)nr * sizeof(float));
+  ind_t *rp = malloc(((size_t)nr + 1) * sizeof(ind_t));
+  if (!v64 || !col || !rp) {
+    free(v64); free(col); free(rp);
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  int rv = GOOSEBERRY_SUCCESS;
+  for (dim_t i = 0; i <= nr; ++i) {
+    if (arowptr[rlo + i] < base) rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+    rp[i] = arowptr[rlo + i] - base;
+  }
+  for (size_t k = 0; k < nnz; ++k) v64[k] = arowval[base + k];
+  gooseberry_plan_t once;
+  memset(&once, 0, sizeof(once));
+  once.wide_index = 1;
+  once.wide_values = 1;
   gooseberry_matrix_t mat = NULL;
   void *dx = NULL, *dy = NULL;
-  int rv = gooseberry_matrix_create(anrows, ancols, arowptr, arowind, v64, NULL,
-      &mat);
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gooseberry_matrix_create(nr, ancols, rp, arowind ? arowind + base : NULL,
+        v64, &once, &mat);
   if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dx, (size_t)ancols * sizeof(float));
-  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dy, (size_t)anrows * sizeof(float));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dy, (size_t)nr * sizeof(float));
   for (dim_t j = 0; rv == GOOSEBERRY_SUCCESS && j < bncols; ++j) {
     rv = gbd_htod(dx, bcolval + (size_t)j * ancols, (size_t)ancols * sizeof(float));
     if (rv == GOOSEBERRY_SUCCESS) rv = gooseberry_matrix_spmv_f32(mat, dx, dy);
-    if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(col, dy, (size_t)anrows * sizeof(float));
-    for (dim_t i = rlo; rv == GOOSEBERRY_SUCCESS && i < rhi; ++i)
-      crowval[(size_t)i * bncols + j] = col[i];
+    if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(col, dy, (size_t)nr * sizeof(float));
+    for (dim_t i = 0; rv == GOOSEBERRY_SUCCESS && i < nr; ++i)
+      crowval[((size_t)rlo + i) * bncols + j] = col[i];
   }
   gbd_free(dx);
   gbd_free(dy);
   gooseberry_matrix_free(mat);
   free(v64);
   free(col);
+  free(rp);
   return rv;
 }

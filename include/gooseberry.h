@@ -115,10 +115,13 @@ int gooseberry_add_scalar(
  * non-decreasing row boundaries, the last at most anrows) select the rows
  * the reference computes: rows [blocks[0], blocks[nblocks]) of C are
  * written and the other rows of crowval are left as they were; NULL
- * computes every row.  Each row sum of a row with at most 64 entries is
- * bitwise the reference's left-to-right sum.  Every call uploads A, builds
- * its SpMV plan and frees it again: callers that multiply by the same A
- * repeatedly should keep it resident (gooseberry_matrix_create,
+ * computes every row.  Row i is entries arowptr[i]..arowptr[i+1]-1, also
+ * when arowptr[0] is not 0, as in the reference's loop.  Each row sum of a
+ * row with at most 64 entries is bitwise the reference's left-to-right sum.
+ * Every call uploads the rows it computes, builds their SpMV plan (without
+ * the 16-bit offsets and the value table, which only pay back over many
+ * SpMVs) and frees it again: callers that multiply by the same A repeatedly
+ * should keep it resident (gooseberry_matrix_create,
  * gooseberry_matrix_spmv). */
 int gooseberry_spmult(
     dim_t anrows,

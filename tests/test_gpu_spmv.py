@@ -185,6 +185,16 @@ def test_spmult_dropin_host_pointers(gb, oracle):
     c2 = np.full(4000, 5.0)
     gb.spmult(4000, 4500, 1, rp, ri, rv, x, blocks=np.array([0], np.uint32), crowval=c2)
     assert np.all(c2 == 5.0)
+    # a rowptr that does not start at 0: row i is entries rowptr[i]..rowptr[i+1]-1,
+    # as the reference's loop reads them (the entries before rowptr[0] are never read)
+    pad = 3
+    rp3 = rp.astype(np.uint32) + pad
+    ri3 = np.concatenate([np.full(pad, 4499, np.uint32), ri])
+    rv3 = np.concatenate([np.full(pad, np.nan), rv])
+    assert np.array_equal(gb.spmult(4000, 4500, 1, rp3, ri3, rv3, x), y)
+    c3 = np.full(4000, 7.0)
+    gb.spmult(4000, 4500, 1, rp3, ri3, rv3, x, blocks=blocks, crowval=c3)
+    assert np.array_equal(c3, c)
 
 
 def test_spmm_bncols3(gb, oracle):
