@@ -85,6 +85,11 @@ EXPORTED = [
     "gooseberry_matrix_spmv_f32", "gooseberry_matrix_cgd_mixed", "gooseberry_spmult_f32",
     "gooseberry_matrix_cgd_noalloc", "gooseberry_cgd_noalloc_max_bytes",
     "gooseberry_inject_fault", "gooseberry_server_cancel", "gooseberry_server_set_timeout",
+    "gooseberry_scale_i64", "gooseberry_add_scalar_i64", "gooseberry_spmult_i64",
+    "gooseberry_spmultsp_i64", "gooseberry_spmultsp_sp_i64",
+    "gooseberry_read_sparse_matrix_i64", "gooseberry_write_sparse_matrix_i64",
+    "gooseberry_matrix_create_i64", "gooseberry_matrix_create_device_i64",
+    "gooseberry_cgd_i64", "gooseberry_matrix_parts",
 ]
 
 PRECISION_FP64 = 0
@@ -249,6 +254,26 @@ def _sig(lib):
         "gooseberry_pagerank": (i32, [u32, _P, _P, _P, f64, sz, f64, _P,
                                       C.POINTER(PagerankInfo)]),
         "gooseberry_generate_uniform_device": (i32, [u64, u64, sz, f64, f64, _P]),
+        "gooseberry_scale_i64": (i32, [u64, _P, f64]),
+        "gooseberry_add_scalar_i64": (i32, [u64, _P, f64]),
+        "gooseberry_spmult_i64": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, u32]),
+        "gooseberry_spmultsp_i64": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P, _P, _P,
+                                          u32]),
+        "gooseberry_spmultsp_sp_i64": (i32, [u32, u32, u32, _P, _P, _P, _P, _P, _P,
+                                             C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                                             _P, u32]),
+        "gooseberry_read_sparse_matrix_i64": (i32, [i32, C.c_char_p, _U32P, _U32P,
+                                                    C.POINTER(_P), C.POINTER(_P),
+                                                    C.POINTER(_P)]),
+        "gooseberry_write_sparse_matrix_i64": (i32, [i32, C.c_char_p, u32, u32, _P, _P,
+                                                     _P]),
+        "gooseberry_matrix_create_i64": (i32, [u32, u32, _P, _P, _P, C.POINTER(_Plan),
+                                               C.POINTER(_P)]),
+        "gooseberry_matrix_create_device_i64": (i32, [u32, u32, _P, _P, _P,
+                                                      C.POINTER(_Plan), C.POINTER(_P)]),
+        "gooseberry_cgd_i64": (i32, [u32, u32, _P, _P, _P, _P, _P, f64, sz,
+                                     C.POINTER(CgdInfo)]),
+        "gooseberry_matrix_parts": (i32, [_P, _U32P]),
     }
     for name, (res, args) in s.items():
         f = getattr(lib, name)
@@ -289,6 +314,10 @@ def _u32(a):
 
 def _f64(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint64)
 
 
 # ------------------------------------------------------------ device memory
@@ -405,6 +434,34 @@ class Matrix:
         self.h = h.value
         self.nrows, self.ncols = nrows, ncols
         self.nnz = int(rowptr[-1])
+
+    @classmethod
+    def from_rowptr64(cls, rowptr, rowind, rowval, ncols=None, plan=None):
+        """gooseberry_matrix_create_i64: host arrays with 64-bit row pointers
+        (held as row parts above GOOSEBERRY_PART_NNZ or 2^31 entries)."""
+        rowptr, rowind, rowval = _u64(rowptr), _u32(rowind), _f64(rowval)
+        nrows = rowptr.size - 1
+        h = _P()
+        check(lib().gooseberry_matrix_create_i64(nrows, nrows if ncols is None else ncols,
+                                                 _ptr(rowptr), _ptr(rowind), _ptr(rowval),
+                                                 _plan(plan), C.byref(h)),
+              "gooseberry_matrix_create_i64")
+        return cls._from_handle(h.value)
+
+    @classmethod
+    def from_device64(cls, nrows, ncols, d_rowptr, d_rowind, d_rowval, plan=None):
+        """gooseberry_matrix_create_device_i64 on device addresses (copied)."""
+        h = _P()
+        check(lib().gooseberry_matrix_create_device_i64(nrows, ncols, d_rowptr, d_rowind,
+                                                        d_rowval, _plan(plan), C.byref(h)),
+              "gooseberry_matrix_create_device_i64")
+        return cls._from_handle(h.value)
+
+    def parts(self):
+        """Row parts the matrix is held as (0: one ordinary matrix)."""
+        n = C.c_uint32()
+        check(lib().gooseberry_matrix_parts(self.h, C.byref(n)), "gooseberry_matrix_parts")
+        return n.value
 
     @classmethod
     def read(cls, path, fmt=FORMAT_CSR, plan=None):
@@ -574,6 +631,34 @@ def spmult(anrows, ancols, bncols, rowptr, rowind, rowval, bcolval, blocks=None,
                                   _ptr(rowval), _ptr(bcolval), _ptr(crowval),
                                   _ptr(blocks), nblocks), "gooseberry_spmult")
     return crowval
+
+
+def spmult_i64(anrows, ancols, bncols, rowptr, rowind, rowval, bcolval, blocks=None,
+               crowval=None):
+    """gooseberry_spmult_i64: gooseberry_spmult with 64-bit row pointers."""
+    rowptr, rowind, rowval = _u64(rowptr), _u32(rowind), _f64(rowval)
+    bcolval = _f64(bcolval)
+    if crowval is None:
+        crowval = np.zeros(anrows * bncols, dtype=np.float64)
+    blocks = _u32(blocks)
+    nblocks = 0 if blocks is None else blocks.size - 1
+    check(lib().gooseberry_spmult_i64(anrows, ancols, bncols, _ptr(rowptr), _ptr(rowind),
+                                      _ptr(rowval), _ptr(bcolval), _ptr(crowval),
+                                      _ptr(blocks), nblocks), "gooseberry_spmult_i64")
+    return crowval
+
+
+def cgd_i64(rowptr, rowind, rowval, b, error, niter=0):
+    """gooseberry_cgd_i64: cgd() with 64-bit row pointers."""
+    rowptr, rowind, rowval, b = _u64(rowptr), _u32(rowind), _f64(rowval), _f64(b)
+    n = rowptr.size - 1
+    x = np.zeros(n, dtype=np.float64)
+    info = CgdInfo()
+    rv = lib().gooseberry_cgd_i64(n, n, _ptr(rowptr), _ptr(rowind), _ptr(rowval), _ptr(b),
+                                  _ptr(x), float(error), int(niter), C.byref(info))
+    if rv not in (SUCCESS, ERROR_CODES["BREAKDOWN"]):
+        check(rv, "gooseberry_cgd_i64")
+    return x, info
 
 
 def spmultsp(anrows, ancols, bncols, a, b, blocks=None, crowval=None):

@@ -41,6 +41,8 @@ void *gbd_stream(void); /* the library stream as a hipStream_t */
 int gbd_inject_fault(int kind, long long after);
 int gbd_device_memory(size_t *free_bytes, size_t *total_bytes);
 const char *gbd_last_error(void);
+/* sets the text gbd_last_error returns (host-side refusals) */
+void gbd_set_error(const char *msg);
 
 /* events on the library stream (timing) */
 int gbd_event_create(void **ev);
@@ -136,8 +138,18 @@ int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
 uint32_t gbd_matrix_index16(const gbd_matrix *m);
 /* number of distinct values of a value-indexed plan (0: plain values) */
 uint32_t gbd_matrix_values8(const gbd_matrix *m);
-/* number of distinct values of a value-indexed plan (0: plain values) */
-uint32_t gbd_matrix_values8(const gbd_matrix *m);
+/* A CSR with 64-bit row pointers; takes ownership of the three arrays as
+ * gbd_matrix_create does.  With at most part_nnz entries (0 or above 2^31:
+ * 2^31) the row pointers are narrowed and the matrix is an ordinary one;
+ * otherwise it is held as row parts of at most part_nnz entries, each with
+ * 32-bit row pointers and a plan of its own (FP64 SpMV, SpMM and CG only;
+ * gbd_matrix_rowptr is then NULL). */
+int gbd_matrix_create64(uint32_t nrows, uint32_t ncols, uint64_t *d_rowptr64,
+    uint32_t *d_rowind, double *d_rowval, uint64_t part_nnz,
+    const gbd_plan_params *params, gbd_matrix **out);
+/* row parts of a matrix made by gbd_matrix_create64 (0: an ordinary one) */
+uint32_t gbd_matrix_nparts(const gbd_matrix *m);
+/* NULL for a matrix held as row parts */
 const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m);
 const uint32_t *gbd_matrix_rowind(const gbd_matrix *m);
 const double *gbd_matrix_rowval(const gbd_matrix *m);

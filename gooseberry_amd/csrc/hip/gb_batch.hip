@@ -930,6 +930,11 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
                   double *d_x, double tol, int rel, uint64_t maxiter,
                   int precision, uint32_t *slot_out) {
   if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (m->nparts) {
+    gb_set_error("batched solves of a matrix held as row parts (64-bit row pointers) are "
+                 "not supported");
+    return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+  }
   if (precision != GOOSEBERRY_PRECISION_FP64 &&
       precision != GOOSEBERRY_PRECISION_MIXED &&
       precision != GOOSEBERRY_PRECISION_MIXED_JACOBI)

@@ -28,8 +28,10 @@ struct gbd_matrix {
   uint32_t nwave;       // rows processed one wavefront per row
   uint32_t nblockrows;  // rows processed one workgroup per row
   uint32_t *rowbins;    // stable bin permutation: short | wave | block rows
-  int16_t *dcol;        // nnz or null: row-block entries' column - block's
-                        // first row (valid only in the row blocks)
+  int16_t *dcol;        // nnz or null: row-block entries' column - (block's
+                        // first row + cshift) (valid only in the row blocks)
+  uint32_t cshift;      // 0, or a row part's first row in the whole matrix
+                        // (its diagonal's column offset)
   // value-indexed plan (plan wide_values = 0, at most GB_VDICT_MAX distinct
   // values, no column split): each entry's value as an 8-bit index into a
   // table of the distinct values, read instead of val / val32 (vtab32 =
@@ -54,11 +56,27 @@ struct gbd_matrix {
   float *sval32j;     // nnz, with val32j
   double *ypart;      // P * nrows partial results (FP64, or FP32 in place)
   bool view;          // rowptr/rowind/val belong to a parent matrix
+  // row parts (more entries than one part may hold, GB_PART_NNZ by
+  // default): part q is rows [prow[q], prow[q + 1]) as a view whose 32-bit
+  // row pointers (prowptr + prow[q] + q) count from its first entry
+  // pbase[q], with a plan of its own; the parent keeps the 64-bit row
+  // pointers and owns the indices and values (rowptr is null)
+  bool part;          // this view is a row part (it owns its plan arrays)
+  uint32_t nparts;
+  gbd_matrix **parts;
+  uint32_t *prow;     // host, nparts + 1
+  uint64_t *pbase;    // host, nparts + 1
+  uint32_t *prowptr;  // nrows + nparts
+  uint64_t *rowptr64; // nrows + 1
   uint32_t holders;   // batch slots holding pointers into the plan: while
                       // nonzero the plan must not be rebuilt or freed
 };
 
 #define GB_SPLIT_MAX 64
+// entries per row part of a matrix with 64-bit row pointers: every entry
+// offset inside a part, and every offset a kernel forms past a part's last
+// entry, stays below 2^32
+#define GB_PART_NNZ (1ull << 31)
 #define GB_VDICT_MAX 256
 
 // One matrix entry's value: val[k], or vtab[vidx[k]] on a value-indexed
@@ -127,7 +145,7 @@ uint32_t gb_spmv_grid(const gbd_matrix *m);
 
 // p.Ap partials written by one SpMV launch with DOT (one per workgroup)
 static inline uint32_t gb_spmv_npartials(const gbd_matrix *m) {
-  return m->nsplit ? gb_split_sum_grid(m) : gb_spmv_grid(m);
+  return (m->nsplit || m->nparts) ? gb_split_sum_grid(m) : gb_spmv_grid(m);
 }
 
 // Enqueue y = A x; when cg != nullptr also write the per-workgroup partial

@@ -188,6 +188,11 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   if (rv != GOOSEBERRY_SUCCESS) return rv;
   if (!m || !d_out || m->nrows != m->ncols || m->nrows == 0)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (m->nparts) {
+    gb_set_error("PageRank of a matrix held as row parts (64-bit row pointers) is not "
+                 "supported");
+    return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+  }
   hipStream_t s = gb_stream();
   const uint32_t n = m->nrows;
   const uint64_t nnz = m->nnz;

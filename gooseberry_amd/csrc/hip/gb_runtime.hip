@@ -271,6 +271,8 @@ int gbd_init(void) { return gb_ensure_init(); }
 
 const char *gbd_last_error(void) { return g_last_error; }
 
+void gbd_set_error(const char *msg) { gb_set_error("%s", msg); }
+
 int gbd_inject_fault(int kind, long long after) {
   if (kind < 0 || kind >= GB_FAULT_KINDS) return GOOSEBERRY_ERROR_INVALIDINPUT;
   g_fault[kind].store(after < 0 ? -1 : after);

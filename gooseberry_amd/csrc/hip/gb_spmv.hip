@@ -48,7 +48,7 @@ __device__ __forceinline__ uint32_t gb_row_class(uint32_t len,
 // so that the plan skips building offsets that cannot fit.
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_plan_classify(const uint32_t *__restrict__ rowptr,
-                 const uint32_t *__restrict__ rowind, uint32_t nrows,
+                 const uint32_t *__restrict__ rowind, uint32_t nrows, uint32_t cshift,
                  gbd_plan_params p, uint32_t *__restrict__ cls,
                  uint32_t *__restrict__ flag,
                  unsigned long long *__restrict__ hist) {
@@ -66,7 +66,8 @@ gb_plan_classify(const uint32_t *__restrict__ rowptr,
     atomicAdd(&h[bin], 1u);
     if (c == 0 && len && !p.wide_index) {
       const int64_t lo = -32768 - (int64_t)(p.rowcap - 1);
-      const int64_t d0 = (int64_t)rowind[a] - r, d1 = (int64_t)rowind[b - 1] - r;
+      const int64_t rr = (int64_t)r + cshift;
+      const int64_t d0 = (int64_t)rowind[a] - rr, d1 = (int64_t)rowind[b - 1] - rr;
       if (d0 > 32767 || d0 < lo || d1 > 32767 || d1 < lo) atomicAdd(&h[33], 1u);
     }
     uint32_t f = 0;
@@ -111,6 +112,33 @@ __global__ void gb_validate_csr(const uint32_t *__restrict__ rowptr,
   if (b) atomicOr(bad, 1u);
 }
 
+// The same check of a CSR with 64-bit row pointers.
+__global__ void gb_validate_csr64(const uint64_t *__restrict__ rowptr,
+                                  const uint32_t *__restrict__ rowind,
+                                  uint32_t nrows, uint32_t ncols, uint64_t nnz,
+                                  uint32_t *__restrict__ bad) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  uint32_t b = 0;
+  for (size_t r = i; r < nrows; r += stride) {
+    uint64_t a = rowptr[r], e = rowptr[r + 1];
+    if (e < a || e > nnz) b = 1;
+    if (r == 0 && a != 0) b = 1;
+  }
+  for (size_t k = i; k < nnz; k += stride)
+    if (rowind[k] >= ncols) b = 1;
+  if (b) atomicOr(bad, 1u);
+}
+
+// out[i] = in[i] - base for i < n (64-bit row pointers to the 32-bit ones
+// of a row part, or of a whole matrix with fewer than 2^32 entries)
+__global__ void gb_rebase_rowptr(const uint64_t *__restrict__ in, uint64_t base, uint64_t n,
+                                 uint32_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)(in[i] - base);
+}
+
 // idx = exclusive scan of flag.  Block id of a short row r is
 // idx[r] + flag[r] - 1.
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -148,8 +176,8 @@ gb_plan_desc(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ b
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_plan_narrow(const uint32_t *__restrict__ rowptr,
                const uint32_t *__restrict__ rowind,
-               const uint32_t *__restrict__ blk, int16_t *__restrict__ dcol,
-               uint32_t *__restrict__ bad) {
+               const uint32_t *__restrict__ blk, uint32_t cshift,
+               int16_t *__restrict__ dcol, uint32_t *__restrict__ bad) {
   __shared__ uint32_t s_bad;
   if (threadIdx.x == 0)
     s_bad = __hip_atomic_load(bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -159,7 +187,7 @@ gb_plan_narrow(const uint32_t *__restrict__ rowptr,
   const uint32_t k1 = rowptr[r1];
   bool out = false;
   for (uint32_t k = rowptr[r0] + threadIdx.x; k < k1; k += GB_BLOCK) {
-    const int64_t d = (int64_t)rowind[k] - (int64_t)r0;
+    const int64_t d = (int64_t)rowind[k] - ((int64_t)r0 + cshift);
     out |= d < -32768 || d > 32767;
     dcol[k] = (int16_t)d;
   }
@@ -198,7 +226,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
                const int16_t *__restrict__ dcol,
                const T *__restrict__ val, const uint8_t *__restrict__ vidx,
                const T *__restrict__ vtab, const T *__restrict__ x,
-               T *__restrict__ y, uint32_t ystride,
+               T *__restrict__ y, uint32_t ystride, uint32_t cshift,
                const uint4 *__restrict__ bdesc, uint32_t nstream,
                const uint32_t *__restrict__ wave_rows, uint32_t nwave,
                uint32_t nwaveblocks,
@@ -268,7 +296,7 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
             }
           }
         }
-        const T *xg = NARROW ? x + r0 : x;
+        const T *xg = NARROW ? x + r0 + cshift : x;
 #pragma unroll
         for (int i = 0; i < GB_STREAM_ITERS; ++i) {
           if (i * GB_BLOCK < cnt) {
@@ -444,7 +472,7 @@ __global__ void __launch_bounds__(GB_BLOCK) __attribute__((amdgpu_waves_per_eu(G
 gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
              const int16_t *__restrict__ dcol, const T *__restrict__ val,
              const uint8_t *__restrict__ vidx, const T *__restrict__ vtab,
-             const T *__restrict__ x, T *__restrict__ y, uint32_t ystride,
+             const T *__restrict__ x, T *__restrict__ y, uint32_t ystride, uint32_t cshift,
              const uint4 *__restrict__ bdesc, uint32_t nstream, uint32_t nvals,
              double *__restrict__ partials) {
   __shared__ T prod[GB_STREAM_LDS];
@@ -475,7 +503,7 @@ gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ r
     }
     const uint32_t r0 = cur.r0, k0 = cur.k0, cnt = cur.cnt;
     if (cnt > 0) {
-      const T *xg = NARROW ? x + r0 : x;
+      const T *xg = NARROW ? x + r0 + cshift : x;
       T xv[GB_STREAM_ITERS];
 #pragma unroll
       for (int i = 0; i < GB_STREAM_ITERS; ++i) {
@@ -623,6 +651,45 @@ static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const uint8_t *vi
                             const T *vtab, const T *x, T *y, uint32_t ystride,
                             double *partials, const gbd_cg_state *cg);
 
+// per-workgroup partials of x.y over n rows, grid-stride in a fixed order
+// (the p.Ap of a row-parted matrix, whose part launches compute no dot)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_dot_partials(const double *__restrict__ x, const double *__restrict__ y, uint32_t n,
+                double *__restrict__ partials) {
+  double dotp = 0.0;
+  for (uint32_t i = blockIdx.x * GB_BLOCK + threadIdx.x; i < n; i += gridDim.x * GB_BLOCK)
+    dotp += x[i] * y[i];
+  __shared__ double red[GB_WAVES_PER_BLOCK];
+  const double ws = gb_wave_sum(dotp);
+  if ((threadIdx.x & (GB_WAVE - 1)) == 0) red[threadIdx.x / GB_WAVE] = ws;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+#pragma unroll
+    for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += red[q];
+    partials[blockIdx.x] = t;
+  }
+}
+
+// a row-parted matrix (FP64): one SpMV per part into its rows of y, then
+// the p.Ap partials over the whole vectors
+static int gb_spmv_launch_parts(const gbd_matrix *m, const double *x, double *y,
+                                uint32_t ystride, double *partials, const gbd_cg_state *cg) {
+  for (uint32_t q = 0; q < m->nparts; ++q) {
+    const gbd_matrix *v = m->parts[q];
+    int rv = gb_spmv_launch_t<double>(v, v->val, v->vidx, v->vtab, x,
+                                      y + (size_t)m->prow[q] * ystride, ystride, nullptr,
+                                      nullptr);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  if (cg) {
+    gb_dot_partials<<<gb_split_sum_grid(m), GB_BLOCK, 0, gb_stream()>>>(x, y, m->nrows,
+                                                                         partials);
+    GB_LAUNCH_CHECK();
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
 // a split matrix: one launch per slice into its partial vector, then the sum
 template <typename T>
 static int gb_spmv_launch_split(const gbd_matrix *m, const T *x, T *y, uint32_t ystride,
@@ -660,14 +727,14 @@ static void gb_spmv_go(const gbd_matrix *m, uint32_t grid, hipStream_t s, const 
   if (gb_spmv_piped(m)) {
     gb_spmv_pipe<T, DOT, NARROW, VD><<<grid, GB_BLOCK, 0, s>>>(
         m->rowptr, m->rowind, NARROW ? m->dcol : nullptr, val, vidx, vtab, x, y, ystride,
-        m->bdesc, m->nstream, m->nvals, partials);
+        m->cshift, m->bdesc, m->nstream, m->nvals, partials);
     return;
   }
   const uint32_t *wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);
   const uint32_t *block_rows = m->rowbins + (m->nrows - m->nblockrows);
   gb_spmv_kernel<T, DOT, NARROW, VD><<<grid, GB_BLOCK, 0, s>>>(
       m->rowptr, m->rowind, NARROW ? m->dcol : nullptr, val, vidx, vtab, x, y, ystride,
-      m->bdesc, m->nstream, wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows,
+      m->cshift, m->bdesc, m->nstream, wave_rows, m->nwave, gb_spmv_nwaveblocks(m), block_rows,
       gb_spmv_nwork(m), m->params.exact, m->params.xcd_remap, partials, cg);
 }
 
@@ -686,6 +753,12 @@ static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const uint8_t *vi
                             const T *vtab, const T *x, T *y, uint32_t ystride,
                             double *partials, const gbd_cg_state *cg) {
   if (m->nsplit) return gb_spmv_launch_split<T>(m, x, y, ystride, partials, cg);
+  if (m->nparts) {
+    if constexpr (sizeof(T) == 8) return gb_spmv_launch_parts(m, x, y, ystride, partials, cg);
+    gb_set_error("FP32 SpMV of a matrix held as row parts (64-bit row pointers) is not "
+                 "supported");
+    return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+  }
   const uint32_t grid = gb_spmv_grid(m);
   if (grid == 0) return GOOSEBERRY_SUCCESS;
   hipStream_t s = gb_stream();
@@ -1081,6 +1154,26 @@ static int gb_split_build(gbd_matrix *m, uint32_t P) {
   return GOOSEBERRY_SUCCESS;
 }
 
+// the plans of a row-parted matrix: one per part; the parent's histogram
+// and bin counts are the parts' sums
+static int gb_parts_plan(gbd_matrix *m) {
+  if (m->params.csplit > 1) {
+    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split");
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  for (uint32_t q = 0; q < m->nparts; ++q) {
+    gbd_matrix *v = m->parts[q];
+    v->params = m->params;
+    int rv = gb_plan_build(v);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+    for (int i = 0; i < 33; ++i) m->hist[i] += v->hist[i];
+    m->nstream += v->nstream;
+    m->nwave += v->nwave;
+    m->nblockrows += v->nblockrows;
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
 static int gb_plan_build(gbd_matrix *m) {
   hipStream_t s = gb_stream();
   const uint32_t n = m->nrows;
@@ -1089,6 +1182,7 @@ static int gb_plan_build(gbd_matrix *m) {
   m->pipe_grid = 0;
   memset(m->hist, 0, sizeof(m->hist));
   m->nstream = m->nwave = m->nblockrows = 0;
+  if (m->nparts) return gb_parts_plan(m);
   if (m->blk) { GB_HIP(gb_dev_free(m->blk)); m->blk = nullptr; }
   if (m->bdesc) { GB_HIP(gb_dev_free(m->bdesc)); m->bdesc = nullptr; }
   if (m->rowbins) { GB_HIP(gb_dev_free(m->rowbins)); m->rowbins = nullptr; }
@@ -1104,7 +1198,7 @@ static int gb_plan_build(gbd_matrix *m) {
   GB_HIP(gb_dev_alloc((void **)&hist, 34 * 8));
   GB_HIP(hipMemsetAsync(hist, 0, 34 * 8, s));
   gb_plan_classify<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(
-      m->rowptr, m->rowind, n, m->params, cls, flag, hist);
+      m->rowptr, m->rowind, n, m->cshift, m->params, cls, flag, hist);
   GB_LAUNCH_CHECK();
   GB_HIP(gb_dev_alloc((void **)&m->rowbins, (size_t)n * 4));
   int rv = gbd_countingsort_kv_unchecked(cls, nullptr, 0, 2, n, m->rowbins, dist);
@@ -1137,13 +1231,17 @@ static int gb_plan_build(gbd_matrix *m) {
   GB_HIP(gb_dev_free(dist));
   GB_HIP(gb_dev_free(hist));
   // hh[33]: rows whose columns already rule the 16-bit offsets out
-  if (!m->view && !m->params.wide_index && nst && m->nnz && hh[33] == 0) {
+  // (a column slice's entries are addressed from the whole split array, so
+  // slices take neither the offsets nor the value index; row parts do)
+  const bool own = !m->view || m->part;
+  if (own && !m->params.wide_index && nst && m->nnz && hh[33] == 0) {
     // 16-bit column offsets for the row blocks, kept only if all fit
     uint32_t *bad = nullptr, hbad = 0;
     GB_HIP(gb_dev_alloc((void **)&m->dcol, m->nnz * sizeof(int16_t)));
     GB_HIP(gb_dev_alloc((void **)&bad, 4));
     GB_HIP(hipMemsetAsync(bad, 0, 4, s));
-    gb_plan_narrow<<<nst, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->blk, m->dcol, bad);
+    gb_plan_narrow<<<nst, GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, m->blk, m->cshift, m->dcol,
+                                            bad);
     GB_LAUNCH_CHECK();
     GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
     GB_HIP(hipStreamSynchronize(s));
@@ -1160,7 +1258,7 @@ static int gb_plan_build(gbd_matrix *m) {
       return rv2;
     }
   }
-  if (!m->view && !m->params.wide_values && !m->nsplit && m->nnz) {
+  if (own && !m->params.wide_values && !m->nsplit && m->nnz) {
     // 8-bit value indices when the matrix has few distinct values
     bool built = false;
     void *tab = nullptr;
@@ -1269,6 +1367,173 @@ int gbd_matrix_create_trusted(uint32_t nrows, uint32_t ncols, uint32_t *d_rowptr
                           params, out);
 }
 
+// One 64-bit row pointer read back (the part boundaries' binary search).
+static int gb_rowptr64_at(const uint64_t *d_rowptr64, uint32_t i, uint64_t *v) {
+  hipStream_t s = gb_stream();
+  GB_HIP(hipMemcpyAsync(v, d_rowptr64 + i, 8, hipMemcpyDeviceToHost, s));
+  GB_HIP(hipStreamSynchronize(s));
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_matrix_create64(uint32_t nrows, uint32_t ncols, uint64_t *d_rowptr64,
+                        uint32_t *d_rowind, double *d_rowval, uint64_t part_nnz,
+                        const gbd_plan_params *params, gbd_matrix **out) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  hipStream_t s = gb_stream();
+  if (part_nnz == 0 || part_nnz > GB_PART_NNZ) part_nnz = GB_PART_NNZ;
+  gbd_plan_params p;
+  if (params) p = *params;
+  else gbd_plan_defaults(nullptr, nrows, 0, &p);
+  if (!gb_plan_check(&p)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  // structure check, as for 32-bit row pointers
+  uint64_t nnz = 0;
+  if ((rv = gb_rowptr64_at(d_rowptr64, nrows, &nnz)) != GOOSEBERRY_SUCCESS) return rv;
+  uint32_t *bad = nullptr, hbad = 0;
+  GB_HIP(gb_dev_alloc((void **)&bad, 4));
+  GB_HIP(hipMemsetAsync(bad, 0, 4, s));
+  const uint64_t work = (uint64_t)nrows > nnz ? nrows : nnz;
+  const uint64_t gw = gb_div_up(work ? work : 1, GB_BLOCK);
+  gb_validate_csr64<<<gw > 8192u ? 8192u : (uint32_t)gw, GB_BLOCK, 0, s>>>(
+      d_rowptr64, d_rowind, nrows, ncols, nnz, bad);
+  GB_LAUNCH_CHECK();
+  GB_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+  GB_HIP(gb_dev_free(bad));
+  GB_HIP(hipStreamSynchronize(s));
+  if (hbad) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  const uint64_t gr = gb_div_up((uint64_t)nrows + 1, GB_BLOCK);
+  const uint32_t grid_rows = gr > 8192u ? 8192u : (uint32_t)gr;
+  if (nnz <= part_nnz) {
+    // an ordinary matrix: 32-bit row pointers
+    uint32_t *rp32 = nullptr;
+    GB_HIP(gb_dev_alloc((void **)&rp32, ((size_t)nrows + 1) * 4));
+    gb_rebase_rowptr<<<grid_rows, GB_BLOCK, 0, s>>>(d_rowptr64, 0, (uint64_t)nrows + 1, rp32);
+    GB_LAUNCH_CHECK();
+    rv = gb_matrix_create(nrows, ncols, rp32, d_rowind, d_rowval, true, (uint32_t)nnz, &p,
+                          out);
+    if (rv != GOOSEBERRY_SUCCESS) {
+      (void)gb_dev_free(rp32);
+      return rv;
+    }
+    (void)gb_dev_free(d_rowptr64);
+    return GOOSEBERRY_SUCCESS;
+  }
+  if (p.csplit > 1) {
+    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split");
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  // part boundaries: part q ends at the last row r1 with
+  // rowptr[r1] - rowptr[r0] <= part_nnz; two consecutive parts hold more
+  // than part_nnz entries, so there are at most 2 ceil(nnz / part_nnz)
+  const uint64_t cap = 2 * gb_div_up(nnz, part_nnz) + 2;
+  if (cap > 65536) {
+    gb_set_error("%llu entries in parts of at most %llu make too many row parts",
+                 (unsigned long long)nnz, (unsigned long long)part_nnz);
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  uint32_t *prow = (uint32_t *)malloc(cap * sizeof(uint32_t));
+  uint64_t *pbase = (uint64_t *)malloc(cap * sizeof(uint64_t));
+  if (!prow || !pbase) {
+    free(prow);
+    free(pbase);
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  uint32_t np = 0;
+  prow[0] = 0;
+  pbase[0] = 0;
+  while (prow[np] < nrows) {
+    const uint32_t r0 = prow[np];
+    const uint64_t b0 = pbase[np];
+    uint32_t lo = r0, hi = nrows;  // rowptr[lo] - b0 <= part_nnz holds for lo
+    uint64_t vlo = b0;
+    while (lo < hi) {
+      const uint32_t mid = lo + (uint32_t)(((uint64_t)hi - lo + 1) / 2);
+      uint64_t v = 0;
+      if ((rv = gb_rowptr64_at(d_rowptr64, mid, &v)) != GOOSEBERRY_SUCCESS) break;
+      if (v - b0 <= part_nnz) {
+        lo = mid;
+        vlo = v;
+      } else {
+        hi = mid - 1;
+      }
+    }
+    if (rv == GOOSEBERRY_SUCCESS && lo == r0) {
+      gb_set_error("row %u has more than %llu entries, the most one row part holds", r0,
+                   (unsigned long long)part_nnz);
+      rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+    if (rv != GOOSEBERRY_SUCCESS) {
+      free(prow);
+      free(pbase);
+      return rv;
+    }
+    ++np;
+    prow[np] = lo;
+    pbase[np] = vlo;
+  }
+  gbd_matrix *m = (gbd_matrix *)calloc(1, sizeof(gbd_matrix));
+  if (!m) {
+    free(prow);
+    free(pbase);
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  m->nrows = nrows;
+  m->ncols = ncols;
+  m->nnz = nnz;
+  m->params = p;
+  m->prow = prow;
+  m->pbase = pbase;
+  m->parts = (gbd_matrix **)calloc(np, sizeof(gbd_matrix *));
+  rv = m->parts ? GOOSEBERRY_SUCCESS : GOOSEBERRY_ERROR_NOMEMORY;
+  if (rv == GOOSEBERRY_SUCCESS &&
+      gb_dev_alloc((void **)&m->prowptr, ((size_t)nrows + np) * 4) != hipSuccess)
+    rv = GOOSEBERRY_ERROR_NOMEMORY;
+  for (uint32_t q = 0; rv == GOOSEBERRY_SUCCESS && q < np; ++q) {
+    const uint64_t rows = prow[q + 1] - prow[q];
+    const uint64_t g = gb_div_up(rows + 1, GB_BLOCK);
+    gb_rebase_rowptr<<<g > 8192u ? 8192u : (uint32_t)g, GB_BLOCK, 0, s>>>(
+        d_rowptr64 + prow[q], pbase[q], rows + 1, m->prowptr + prow[q] + q);
+    if (gb_launch_status() != hipSuccess) {
+      rv = GOOSEBERRY_ERROR_DEVICE;
+      break;
+    }
+    gbd_matrix *v = (gbd_matrix *)calloc(1, sizeof(gbd_matrix));
+    if (!v) {
+      rv = GOOSEBERRY_ERROR_NOMEMORY;
+      break;
+    }
+    m->parts[q] = v;
+    m->nparts = q + 1;
+    v->view = true;
+    v->part = true;
+    v->nrows = (uint32_t)rows;
+    v->ncols = ncols;
+    v->nnz = pbase[q + 1] - pbase[q];
+    v->rowptr = m->prowptr + prow[q] + q;
+    v->cshift = prow[q];
+    v->rowind = d_rowind + pbase[q];
+    v->val = d_rowval + pbase[q];
+  }
+  if (rv == GOOSEBERRY_SUCCESS) {
+    m->rowind = d_rowind;
+    m->val = d_rowval;
+    m->rowptr64 = d_rowptr64;
+    rv = gb_plan_build(m);
+  }
+  if (rv != GOOSEBERRY_SUCCESS) {
+    // the caller keeps the arrays it passed in
+    m->rowind = nullptr;
+    m->val = nullptr;
+    m->rowptr64 = nullptr;
+    gbd_matrix_destroy(m);
+    return rv;
+  }
+  *out = m;
+  return GOOSEBERRY_SUCCESS;
+}
+
+uint32_t gbd_matrix_nparts(const gbd_matrix *m) { return m->nparts; }
+
 int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
   // a batch slot copied the plan's arrays into its descriptor
   if (m->holders) {
@@ -1280,6 +1545,10 @@ int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
   if (params) p = *params;
   else gbd_plan_defaults(m->hist, m->nrows, m->nnz, &p);
   if (!gb_plan_check(&p)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (m->nparts && p.csplit > 1) {
+    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split");
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
   m->params = p;
   return gb_plan_build(m);
 }
@@ -1291,11 +1560,19 @@ void gbd_matrix_destroy(gbd_matrix *m) {
     (void)gb_dev_free(m->blk);
     (void)gb_dev_free(m->bdesc);
     (void)gb_dev_free(m->rowbins);
+    (void)gb_dev_free(m->dcol);
+    gb_vdict_free(m);
     free(m);
     return;
   }
   gb_split_free(m);
   gb_vdict_free(m);
+  for (uint32_t q = 0; q < m->nparts; ++q) gbd_matrix_destroy(m->parts[q]);
+  free(m->parts);
+  free(m->prow);
+  free(m->pbase);
+  if (m->prowptr) (void)gb_dev_free(m->prowptr);
+  if (m->rowptr64) (void)gb_dev_free(m->rowptr64);
   if (m->rowptr) (void)gb_dev_free(m->rowptr);
   if (m->rowind) (void)gb_dev_free(m->rowind);
   if (m->val) (void)gb_dev_free(m->val);
@@ -1324,9 +1601,24 @@ int gbd_matrix_info(const gbd_matrix *m, uint32_t *nrows, uint32_t *ncols,
   return GOOSEBERRY_SUCCESS;
 }
 
-uint32_t gbd_matrix_index16(const gbd_matrix *m) { return m->dcol ? 1u : 0u; }
+// row parts: 1 when every part reads 16-bit offsets; the largest value
+// table of the parts when every part is value-indexed
+uint32_t gbd_matrix_index16(const gbd_matrix *m) {
+  if (!m->nparts) return m->dcol ? 1u : 0u;
+  for (uint32_t q = 0; q < m->nparts; ++q)
+    if (!m->parts[q]->dcol) return 0u;
+  return 1u;
+}
 
-uint32_t gbd_matrix_values8(const gbd_matrix *m) { return m->vidx ? m->nvals : 0u; }
+uint32_t gbd_matrix_values8(const gbd_matrix *m) {
+  if (!m->nparts) return m->vidx ? m->nvals : 0u;
+  uint32_t nv = 0;
+  for (uint32_t q = 0; q < m->nparts; ++q) {
+    if (!m->parts[q]->vidx) return 0u;
+    if (m->parts[q]->nvals > nv) nv = m->parts[q]->nvals;
+  }
+  return nv;
+}
 
 uint32_t gbd_matrix_holders(const gbd_matrix *m) { return m->holders; }
 
@@ -1334,8 +1626,19 @@ const uint32_t *gbd_matrix_rowptr(const gbd_matrix *m) { return m->rowptr; }
 const uint32_t *gbd_matrix_rowind(const gbd_matrix *m) { return m->rowind; }
 const double *gbd_matrix_rowval(const gbd_matrix *m) { return m->val; }
 
+// the FP32 paths (mixed precision, batches, servers) take no row-parted
+// matrix
+static int gb_parts_refuse(const gbd_matrix *m, const char *what) {
+  if (!m->nparts) return GOOSEBERRY_SUCCESS;
+  gb_set_error("%s of a matrix held as row parts (64-bit row pointers, %u parts) is not "
+               "supported", what, m->nparts);
+  return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+}
+
 int gbd_matrix_make_f32(gbd_matrix *m) {
   if (m->val32) return GOOSEBERRY_SUCCESS;
+  int rp = gb_parts_refuse(m, "an FP32 copy");
+  if (rp != GOOSEBERRY_SUCCESS) return rp;
   GB_HIP(gb_dev_alloc((void **)&m->val32, (m->nnz ? m->nnz : 1) * sizeof(float)));
   if (m->nnz) {
     gb_to_f32<<<gb_div_up(m->nnz, GB_BLOCK), GB_BLOCK, 0, gb_stream()>>>(
@@ -1356,6 +1659,8 @@ int gbd_matrix_make_f32(gbd_matrix *m) {
 
 int gbd_matrix_make_f32_jacobi(gbd_matrix *m) {
   if (m->val32j) return GOOSEBERRY_SUCCESS;
+  int rp = gb_parts_refuse(m, "Jacobi scaling");
+  if (rp != GOOSEBERRY_SUCCESS) return rp;
   hipStream_t s = gb_stream();
   const uint32_t n = m->nrows;
   if (m->nrows != m->ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;

@@ -217,6 +217,15 @@ int gb_check_csr(dim_t nrows, dim_t ncols, const ind_t *rowptr,
   return GOOSEBERRY_SUCCESS;
 }
 
+/* what a matrix held as row parts (64-bit row pointers, more than 2^31
+ * entries) does not take */
+int gb_parts_unsupported(void)
+{
+  gbd_set_error("this operation does not take a matrix held as row parts (64-bit row "
+                "pointers)");
+  return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+}
+
 static int wrap_matrix(gbd_matrix *m, gooseberry_matrix_t *r_mat)
 {
   struct gooseberry_matrix_s *h = calloc(1, sizeof(*h));
@@ -386,6 +395,53 @@ gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat) { return mat->m; }
 
 /* -------------------------------------------------------------- spmult */
 
+/* the plan of a matrix made for one host-pointer product: no 16-bit
+ * offsets and no value table, whose build costs more than one SpMV saves */
+static void spmult_once_plan(gooseberry_plan_t *p)
+{
+  memset(p, 0, sizeof(*p));
+  p->wide_index = 1;
+  p->wide_values = 1;
+}
+
+/* C rows = A B for a matrix made for this one call (freed here): B (host,
+ * ancols x bncols, column-major) in, the nr rows of C (row-major) out */
+static int spmult_once(gooseberry_matrix_t mat, dim_t nr, dim_t ancols,
+    dim_t bncols, const real_t *bcolval, real_t *crows)
+{
+  void *db = NULL, *dc = NULL;
+  size_t bbytes = (size_t)ancols * bncols * sizeof(real_t);
+  size_t cbytes = (size_t)nr * bncols * sizeof(real_t);
+  int rv = gb_upload(&db, bcolval, bbytes);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dc, cbytes);
+  if (rv == GOOSEBERRY_SUCCESS) {
+    rv = (bncols == 1) ? gbd_spmv(mat->m, db, dc)
+                       : gbd_spmm(mat->m, bncols, db, dc);
+  }
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtoh(crows, dc, cbytes);
+  gbd_free(db);
+  gbd_free(dc);
+  gooseberry_matrix_free(mat);
+  return rv;
+}
+
+/* blocks (nblocks + 1 non-decreasing row boundaries, the last at most
+ * anrows) to the row range the reference computes */
+static int spmult_rows(dim_t anrows, const dim_t *blocks, dim_t nblocks,
+    dim_t *rlo, dim_t *rhi)
+{
+  *rlo = 0;
+  *rhi = anrows;
+  if (!blocks) return GOOSEBERRY_SUCCESS;
+  for (dim_t c = 0; c < nblocks; ++c) {
+    if (blocks[c + 1] < blocks[c]) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  }
+  if (nblocks && blocks[nblocks] > anrows) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  *rlo = nblocks ? blocks[0] : 0;
+  *rhi = nblocks ? blocks[nblocks] : 0;
+  return GOOSEBERRY_SUCCESS;
+}
+
 /* src/blas.c:552-606 through the GPU.  With blocks, the reference computes
  * only the rows inside [blocks[0], blocks[nblocks]) (the union of the
  * blocks of a non-decreasing blocks array); other rows of crowval are left
@@ -400,15 +456,9 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
     dim_t nblocks)
 {
   GB_API_GUARD();
-  dim_t rlo = 0, rhi = anrows;
-  if (blocks) {
-    for (dim_t c = 0; c < nblocks; ++c) {
-      if (blocks[c + 1] < blocks[c]) return GOOSEBERRY_ERROR_INVALIDINPUT;
-    }
-    if (nblocks && blocks[nblocks] > anrows) return GOOSEBERRY_ERROR_INVALIDINPUT;
-    rlo = nblocks ? blocks[0] : 0;
-    rhi = nblocks ? blocks[nblocks] : 0;
-  }
+  dim_t rlo, rhi;
+  if (spmult_rows(anrows, blocks, nblocks, &rlo, &rhi) != GOOSEBERRY_SUCCESS)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
   if (!arowptr || !bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
   const dim_t nr = rhi - rlo;
@@ -428,30 +478,14 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
     rp = rebased;
   }
   gooseberry_plan_t once;
-  memset(&once, 0, sizeof(once));
-  once.wide_index = 1;
-  once.wide_values = 1;
+  spmult_once_plan(&once);
   gooseberry_matrix_t mat = NULL;
   int rv = gooseberry_matrix_create(nr, ancols, rp,
       arowind ? arowind + base : NULL, arowval ? arowval + base : NULL,
       &once, &mat);
   free(rebased);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  void *db = NULL, *dc = NULL;
-  size_t bbytes = (size_t)ancols * bncols * sizeof(real_t);
-  size_t cbytes = (size_t)nr * bncols * sizeof(real_t);
-  rv = gb_upload(&db, bcolval, bbytes);
-  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dc, cbytes);
-  if (rv == GOOSEBERRY_SUCCESS) {
-    rv = (bncols == 1) ? gbd_spmv(mat->m, db, dc)
-                       : gbd_spmm(mat->m, bncols, db, dc);
-  }
-  if (rv == GOOSEBERRY_SUCCESS)
-    rv = gbd_dtoh(crowval + (size_t)rlo * bncols, dc, cbytes);
-  gbd_free(db);
-  gbd_free(dc);
-  gooseberry_matrix_free(mat);
-  return rv;
+  return spmult_once(mat, nr, ancols, bncols, bcolval, crowval + (size_t)rlo * bncols);
 }
 
 /* ------------------------------------------------------ sparse x sparse */
@@ -592,6 +626,7 @@ int gooseberry_matrix_spmultsp(gooseberry_matrix_t a, gooseberry_matrix_t b,
   gbd_matrix_info(a->m, &anr, &anc, NULL, NULL, NULL, NULL, NULL, NULL);
   gbd_matrix_info(b->m, &bnr, &bnc, NULL, NULL, NULL, NULL, NULL, NULL);
   if (bnr < anc) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (gbd_matrix_nparts(a->m) || gbd_matrix_nparts(b->m)) return gb_parts_unsupported();
   uint32_t *dptr = NULL, *dind = NULL, cnnz = 0;
   double *dval = NULL;
   int rv = gbd_spgemm(anr, anc, bnc, gbd_matrix_rowptr(a->m),
@@ -613,6 +648,7 @@ int gooseberry_matrix_spmultsp_dense(gooseberry_matrix_t a,
   if (bnr < anc) return GOOSEBERRY_ERROR_INVALIDINPUT;
   if ((size_t)anr * bnc == 0) return GOOSEBERRY_SUCCESS;
   if (!d_c) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (gbd_matrix_nparts(a->m) || gbd_matrix_nparts(b->m)) return gb_parts_unsupported();
   return gbd_spgemm(anr, anc, bnc, gbd_matrix_rowptr(a->m),
       gbd_matrix_rowind(a->m), gbd_matrix_rowval(a->m),
       gbd_matrix_rowptr(b->m), gbd_matrix_rowind(b->m),
@@ -670,6 +706,7 @@ int gooseberry_matrix_cuthillmckee(gooseberry_matrix_t mat,
   gbd_matrix_info(mat->m, &nrows, &ncols, NULL, NULL, NULL, NULL, NULL, NULL);
   if (nrows != ncols || !cm_blocks_ok(nrows, blocks, nblocks))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (gbd_matrix_nparts(mat->m)) return gb_parts_unsupported();
   return gbd_cuthillmckee(nrows, gbd_matrix_rowptr(mat->m),
       gbd_matrix_rowind(mat->m), blocks, nblocks, d_perm);
 }
@@ -787,21 +824,14 @@ size_t gooseberry_cgd_noalloc_max_bytes(void)
   return gbd_cg_scratch_max();
 }
 
-int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,
-    const dim_t *rowind, const real_t *rowval, const real_t *b, real_t *x,
-    real_t error, size_t niter, gooseberry_cgd_info_t *info)
+/* the host-array CG on a matrix made for this call (freed here) */
+static int cgd_resident_host(gooseberry_matrix_t mat, dim_t nrows,
+    const real_t *b, real_t *x, real_t error, size_t niter,
+    gooseberry_cgd_info_t *info)
 {
-  GB_API_GUARD();
-  /* src/cgd.c:51-65 shape checks (b and x are nrows/ncols long here) */
-  if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  if (!b || !x) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  gooseberry_matrix_t mat = NULL;
-  int rv = gooseberry_matrix_create(nrows, ncols, rowptr, rowind, rowval, NULL,
-      &mat);
-  if (rv != GOOSEBERRY_SUCCESS) return rv;
   void *db = NULL, *dx = NULL;
   size_t vb = (size_t)nrows * sizeof(real_t);
-  rv = gb_upload(&db, b, vb);
+  int rv = gb_upload(&db, b, vb);
   if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dx, vb);
   int rs = GOOSEBERRY_SUCCESS;
   if (rv == GOOSEBERRY_SUCCESS) {
@@ -816,6 +846,21 @@ int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,
   gbd_free(dx);
   gooseberry_matrix_free(mat);
   return rv != GOOSEBERRY_SUCCESS ? rv : rs;
+}
+
+int gooseberry_cgd(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval, const real_t *b, real_t *x,
+    real_t error, size_t niter, gooseberry_cgd_info_t *info)
+{
+  GB_API_GUARD();
+  /* src/cgd.c:51-65 shape checks (b and x are nrows/ncols long here) */
+  if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!b || !x) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gooseberry_matrix_t mat = NULL;
+  int rv = gooseberry_matrix_create(nrows, ncols, rowptr, rowind, rowval, NULL,
+      &mat);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  return cgd_resident_host(mat, nrows, b, x, error, niter, info);
 }
 
 /* --------------------------------------------------- index work (host) */
@@ -946,4 +991,247 @@ int gooseberry_pagerank(dim_t n, const ind_t *rowptr, const dim_t *rowind,
   gbd_free(d);
   gooseberry_matrix_free(m);
   return rv;
+}
+
+/* ------------------------------------------------- 64-bit row pointers */
+/* The reference's GOOSEBERRY_64BIT_INDICES build (include/gooseberry.h:
+ * 65-71) gives ind_t 64 bits.  Resident matrices of more than 2^31 entries
+ * are held as row parts (gbd_matrix_create64); the readers, writers and
+ * sparse products keep 32-bit storage and narrow or widen at the boundary. */
+
+/* GOOSEBERRY_PART_NNZ: entries per row part (0 or unset: the default) */
+static uint64_t part_nnz_env(void)
+{
+  const char *e = getenv("GOOSEBERRY_PART_NNZ");
+  if (!e || !*e) return 0;
+  char *end = NULL;
+  unsigned long long v = strtoull(e, &end, 10);
+  return (end && *end == '\0') ? (uint64_t)v : 0;
+}
+
+/* 64-bit row pointers narrowed to a new 32-bit array (NULL: too many
+ * entries, or no memory: *rv says which) */
+static ind_t *narrow_rowptr(const uint64_t *rp, dim_t nrows, int *rv)
+{
+  *rv = GOOSEBERRY_SUCCESS;
+  if (rp[nrows] > UINT32_MAX) {
+    gbd_set_error("more than 2^32 - 1 entries: this operation keeps 32-bit row pointers");
+    *rv = GOOSEBERRY_ERROR_UNIMPLEMENTED;
+    return NULL;
+  }
+  ind_t *out = malloc(((size_t)nrows + 1) * sizeof(ind_t));
+  if (!out) {
+    *rv = GOOSEBERRY_ERROR_NOMEMORY;
+    return NULL;
+  }
+  for (size_t i = 0; i <= nrows; ++i) {
+    if (rp[i] > UINT32_MAX) {
+      free(out);
+      *rv = GOOSEBERRY_ERROR_INVALIDINPUT;
+      return NULL;
+    }
+    out[i] = (ind_t)rp[i];
+  }
+  return out;
+}
+
+/* takes the three device arrays (freed here on failure) */
+static int create64_owned(dim_t nrows, dim_t ncols, void *dp, void *di,
+    void *dv, const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
+{
+  gbd_plan_params pp;
+  int have;
+  plan_to_dev(plan, NULL, &pp, &have);
+  gbd_matrix *m = NULL;
+  int rv = gbd_matrix_create64(nrows, ncols, dp, di, dv, part_nnz_env(),
+      have ? &pp : NULL, &m);
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  return wrap_matrix(m, r_mat);
+}
+
+int gooseberry_matrix_create_i64(dim_t nrows, dim_t ncols,
+    const uint64_t *rowptr, const dim_t *rowind, const real_t *rowval,
+    const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
+{
+  GB_API_GUARD();
+  if (!r_mat || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  const uint64_t nnz = rowptr[nrows];
+  if (nnz && (!rowind || !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *dp = NULL, *di = NULL, *dv = NULL;
+  int rv = gb_upload(&dp, rowptr, ((size_t)nrows + 1) * sizeof(uint64_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&di, rowind, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dv, rowval, nnz * sizeof(real_t));
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  return create64_owned(nrows, ncols, dp, di, dv, plan, r_mat);
+}
+
+int gooseberry_matrix_create_device_i64(dim_t nrows, dim_t ncols,
+    const uint64_t *d_rowptr, const dim_t *d_rowind, const real_t *d_rowval,
+    const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat)
+{
+  GB_API_GUARD();
+  if (!r_mat || !d_rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  uint64_t nnz = 0;
+  int rv = gbd_dtoh(&nnz, d_rowptr + nrows, sizeof(uint64_t));
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  if (nnz && (!d_rowind || !d_rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *dp = NULL, *di = NULL, *dv = NULL;
+  rv = gbd_malloc(&dp, ((size_t)nrows + 1) * sizeof(uint64_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&di, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dv, nnz * sizeof(real_t));
+  if (rv == GOOSEBERRY_SUCCESS)
+    rv = gbd_dtod(dp, d_rowptr, ((size_t)nrows + 1) * sizeof(uint64_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtod(di, d_rowind, nnz * sizeof(dim_t));
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_dtod(dv, d_rowval, nnz * sizeof(real_t));
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    return rv;
+  }
+  return create64_owned(nrows, ncols, dp, di, dv, plan, r_mat);
+}
+
+int gooseberry_matrix_parts(gooseberry_matrix_t mat, dim_t *r_nparts)
+{
+  GB_API_GUARD();
+  if (!mat || !r_nparts) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  *r_nparts = gbd_matrix_nparts(mat->m);
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_spmult_i64(dim_t anrows, dim_t ancols, dim_t bncols,
+    const uint64_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const real_t *bcolval, real_t *crowval, const dim_t *blocks,
+    dim_t nblocks)
+{
+  GB_API_GUARD();
+  dim_t rlo, rhi;
+  if (spmult_rows(anrows, blocks, nblocks, &rlo, &rhi) != GOOSEBERRY_SUCCESS)
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (anrows == 0 || bncols == 0 || rlo >= rhi) return GOOSEBERRY_SUCCESS;
+  if (!arowptr || !bcolval || !crowval) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  /* rows [rlo, rhi) with row pointers rebased to 0, as gooseberry_spmult */
+  const dim_t nr = rhi - rlo;
+  const uint64_t base = arowptr[rlo];
+  uint64_t *rp = malloc(((size_t)nr + 1) * sizeof(uint64_t));
+  if (!rp) return GOOSEBERRY_ERROR_NOMEMORY;
+  for (dim_t i = 0; i <= nr; ++i) {
+    if (arowptr[rlo + i] < base) {
+      free(rp);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+    rp[i] = arowptr[rlo + i] - base;
+  }
+  gooseberry_plan_t once;
+  spmult_once_plan(&once);
+  gooseberry_matrix_t mat = NULL;
+  int rv = gooseberry_matrix_create_i64(nr, ancols, rp,
+      arowind ? arowind + base : NULL, arowval ? arowval + base : NULL,
+      &once, &mat);
+  free(rp);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  return spmult_once(mat, nr, ancols, bncols, bcolval, crowval + (size_t)rlo * bncols);
+}
+
+int gooseberry_cgd_i64(dim_t nrows, dim_t ncols, const uint64_t *rowptr,
+    const dim_t *rowind, const real_t *rowval, const real_t *b, real_t *x,
+    real_t error, size_t niter, gooseberry_cgd_info_t *info)
+{
+  GB_API_GUARD();
+  if (nrows != ncols) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (!b || !x) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  gooseberry_matrix_t mat = NULL;
+  int rv = gooseberry_matrix_create_i64(nrows, ncols, rowptr, rowind, rowval,
+      NULL, &mat);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  return cgd_resident_host(mat, nrows, b, x, error, niter, info);
+}
+
+/* n values in chunks the 32-bit forms take */
+int gooseberry_scale_i64(uint64_t n, real_t *val, real_t s)
+{
+  GB_API_GUARD();
+  const uint64_t step = 1ull << 30;
+  for (uint64_t off = 0; off < n; off += step) {
+    const uint64_t k = n - off < step ? n - off : step;
+    int rv = gooseberry_scale((ind_t)k, val + off, s);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_add_scalar_i64(uint64_t n, real_t *val, real_t s)
+{
+  GB_API_GUARD();
+  const uint64_t step = 1ull << 30;
+  for (uint64_t off = 0; off < n; off += step) {
+    const uint64_t k = n - off < step ? n - off : step;
+    int rv = gooseberry_add_scalar((ind_t)k, val + off, s);
+    if (rv != GOOSEBERRY_SUCCESS) return rv;
+  }
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_spmultsp_i64(dim_t anrows, dim_t ancols, dim_t bncols,
+    const uint64_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const uint64_t *browptr, const dim_t *browind, const real_t *browval,
+    real_t *crowval, const dim_t *blocks, dim_t nblocks)
+{
+  GB_API_GUARD();
+  if (!arowptr || !browptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int rv;
+  ind_t *a32 = narrow_rowptr(arowptr, anrows, &rv);
+  if (!a32) return rv;
+  ind_t *b32 = narrow_rowptr(browptr, ancols, &rv);
+  if (!b32) {
+    free(a32);
+    return rv;
+  }
+  rv = gooseberry_spmultsp(anrows, ancols, bncols, a32, arowind, arowval, b32,
+      browind, browval, crowval, blocks, nblocks);
+  free(a32);
+  free(b32);
+  return rv;
+}
+
+int gooseberry_spmultsp_sp_i64(dim_t anrows, dim_t ancols, dim_t bncols,
+    const uint64_t *arowptr, const dim_t *arowind, const real_t *arowval,
+    const uint64_t *browptr, const dim_t *browind, const real_t *browval,
+    uint64_t **crowptr, dim_t **crowind, real_t **crowval,
+    const dim_t *blocks, dim_t nblocks)
+{
+  GB_API_GUARD();
+  if (!arowptr || !browptr || !crowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  int rv;
+  ind_t *a32 = narrow_rowptr(arowptr, anrows, &rv);
+  if (!a32) return rv;
+  ind_t *b32 = narrow_rowptr(browptr, ancols, &rv);
+  if (!b32) {
+    free(a32);
+    return rv;
+  }
+  ind_t *c32 = NULL;
+  rv = gooseberry_spmultsp_sp(anrows, ancols, bncols, a32, arowind, arowval,
+      b32, browind, browval, &c32, crowind, crowval, blocks, nblocks);
+  free(a32);
+  free(b32);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  uint64_t *c64 = malloc(((size_t)anrows + 1) * sizeof(uint64_t));
+  if (!c64) {
+    free(c32);
+    free(*crowind);
+    free(*crowval);
+    *crowind = NULL;
+    *crowval = NULL;
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  for (size_t i = 0; i <= anrows; ++i) c64[i] = c32[i];
+  free(c32);
+  *crowptr = c64;
+  return GOOSEBERRY_SUCCESS;
 }

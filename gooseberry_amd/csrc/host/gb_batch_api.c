@@ -206,6 +206,7 @@ int gooseberry_matrix_download(gooseberry_matrix_t mat, ind_t *rowptr,
   uint32_t nrows;
   uint64_t nnz;
   gbd_matrix_info(m, &nrows, NULL, &nnz, NULL, NULL, NULL, NULL, NULL);
+  if (gbd_matrix_nparts(m)) return gb_parts_unsupported();
   int rv = gbd_dtoh(rowptr, gbd_matrix_rowptr(m), ((size_t)nrows + 1) * 4);
   if (rv == GOOSEBERRY_SUCCESS && nnz && rowind)
     rv = gbd_dtoh(rowind, gbd_matrix_rowind(m), nnz * 4);

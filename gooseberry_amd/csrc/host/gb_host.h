@@ -12,6 +12,9 @@ int gb_upload(void **d, const void *h, size_t bytes);
 int gb_check_csr(dim_t nrows, dim_t ncols, const ind_t *rowptr,
     const dim_t *rowind);
 gbd_matrix *gb_matrix_dev(gooseberry_matrix_t mat);
+/* GOOSEBERRY_ERROR_UNIMPLEMENTED, with its error text, for operations that
+ * take no matrix held as row parts */
+int gb_parts_unsupported(void);
 int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
     dim_t *d_rowind, real_t *d_rowval, ind_t nnz,
     const gooseberry_plan_t *plan, gooseberry_matrix_t *r_mat);

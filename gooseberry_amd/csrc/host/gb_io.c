@@ -984,3 +984,51 @@ int gooseberry_read_partition(const char *const file, dim_t *const r_nrows,
   if (r_map) *r_map = map; else free(map);
   return GOOSEBERRY_SUCCESS;
 }
+
+/* The readers and writers with 64-bit row pointers (the reference's
+ * GOOSEBERRY_64BIT_INDICES build): the files are read and written through
+ * the 32-bit forms, the row pointers widened or narrowed at the boundary. */
+int gooseberry_read_sparse_matrix_i64(int type, const char *file,
+    dim_t *nrows, dim_t *ncols, uint64_t **rowptr, dim_t **rowind,
+    real_t **rowval)
+{
+  if (!nrows || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  ind_t *rp32 = NULL;
+  int rv = gooseberry_read_sparse_matrix(type, file, nrows, ncols, &rp32,
+      rowind, rowval);
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  uint64_t *rp = malloc(((size_t)*nrows + 1) * sizeof(uint64_t));
+  if (!rp) {
+    free(rp32);
+    free(*rowind);
+    free(*rowval);
+    *rowind = NULL;
+    *rowval = NULL;
+    return GOOSEBERRY_ERROR_NOMEMORY;
+  }
+  for (size_t i = 0; i <= *nrows; ++i) rp[i] = rp32[i];
+  free(rp32);
+  *rowptr = rp;
+  return GOOSEBERRY_SUCCESS;
+}
+
+int gooseberry_write_sparse_matrix_i64(int type, const char *file,
+    dim_t nrows, dim_t ncols, const uint64_t *rowptr, const dim_t *rowind,
+    const real_t *rowval)
+{
+  if (!file || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (rowptr[nrows] > UINT32_MAX) return GOOSEBERRY_ERROR_UNIMPLEMENTED;
+  ind_t *rp = malloc(((size_t)nrows + 1) * sizeof(ind_t));
+  if (!rp) return GOOSEBERRY_ERROR_NOMEMORY;
+  for (size_t i = 0; i <= nrows; ++i) {
+    if (rowptr[i] > rowptr[nrows]) {
+      free(rp);
+      return GOOSEBERRY_ERROR_INVALIDINPUT;
+    }
+    rp[i] = (ind_t)rowptr[i];
+  }
+  int rv = gooseberry_write_sparse_matrix(type, file, nrows, ncols, rp, rowind,
+      rowval);
+  free(rp);
+  return rv;
+}

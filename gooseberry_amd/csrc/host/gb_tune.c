@@ -78,6 +78,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
   if (precision == GOOSEBERRY_PRECISION_MIXED_JACOBI) precision = GOOSEBERRY_PRECISION_MIXED;
   /* a batch slot holds the current plan's arrays */
   if (gbd_matrix_holders(gb_matrix_dev(mat))) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  if (gbd_matrix_nparts(gb_matrix_dev(mat))) return gb_parts_unsupported();
   tune_ctx c;
   memset(&c, 0, sizeof(c));
   c.m = gb_matrix_dev(mat);

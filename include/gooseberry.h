@@ -38,11 +38,14 @@
 #define GOOSEBERRY_VER_SUBMINOR 7
 
 /* ---------------------------------------------------------------- types */
-/* include/gooseberry.h:43-82 (values FP64, indices uint32) */
+/* include/gooseberry.h:43-82: values FP64, dimensions uint32, and row
+ * pointers (ind_t) uint32 -- or uint64 when GOOSEBERRY_64BIT_INDICES is
+ * defined (include/gooseberry.h:65-71): the library exports both row-pointer
+ * widths, and under that macro the names of the functions taking ind_t map
+ * onto their _i64 forms (below, "64-bit row pointers"). */
 #ifndef BOWSTRING_TYPES_DEFINED
-#if defined(GOOSEBERRY_SIGNED_TYPES) || defined(GOOSEBERRY_64BIT_DIMENSIONS) \
-    || defined(GOOSEBERRY_64BIT_INDICES)
-#error "this library is built with unsigned 32-bit dim_t/ind_t"
+#if defined(GOOSEBERRY_SIGNED_TYPES) || defined(GOOSEBERRY_64BIT_DIMENSIONS)
+#error "this library is built with unsigned 32-bit dim_t and unsigned ind_t"
 #endif
 typedef uint64_t gooseberry_int64_t;
 #define PF_BSINT64_T "%" PRIu64
@@ -50,14 +53,34 @@ typedef uint32_t gooseberry_int32_t;
 #define PF_BSINT32_T "%" PRIu32
 typedef gooseberry_int32_t dim_t;
 #define PF_DIM_T PF_BSINT32_T
+#ifdef GOOSEBERRY_64BIT_INDICES
+typedef gooseberry_int64_t ind_t;
+#define PF_IND_T PF_BSINT64_T
+#else
 typedef gooseberry_int32_t ind_t;
 #define PF_IND_T PF_BSINT32_T
+#endif
 #ifndef GOOSEBERRY_DOUBLE_REALS
 #define GOOSEBERRY_DOUBLE_REALS 1
 #endif
 typedef double real_t;
 #define PF_REAL_T "%lf"
 #define PF_MINREAL_T "%g"
+#endif
+
+#ifdef GOOSEBERRY_64BIT_INDICES
+/* ind_t is 64-bit: the reference's names resolve to the _i64 entry points,
+ * whose prototypes below are the same with ind_t = uint64_t */
+#define gooseberry_scale gooseberry_scale_i64
+#define gooseberry_add_scalar gooseberry_add_scalar_i64
+#define gooseberry_spmult gooseberry_spmult_i64
+#define gooseberry_spmultsp gooseberry_spmultsp_i64
+#define gooseberry_spmultsp_sp gooseberry_spmultsp_sp_i64
+#define gooseberry_read_sparse_matrix gooseberry_read_sparse_matrix_i64
+#define gooseberry_write_sparse_matrix gooseberry_write_sparse_matrix_i64
+#define gooseberry_matrix_create gooseberry_matrix_create_i64
+#define gooseberry_matrix_create_device gooseberry_matrix_create_device_i64
+#define gooseberry_cgd gooseberry_cgd_i64
 #endif
 
 /* include/gooseberry.h:85-94; codes after GOOSEBERRY_ERROR_UNKNOWN are new */
@@ -187,6 +210,7 @@ int gooseberry_spmultsp_sp(
  * and each row places its unplaced neighbours by ascending degree (entries
  * in the block counted); equal degrees are taken in ascending row order.
  * Host arrays; computed on the GPU. */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_permute_cuthillmckee(
     dim_t nrows,
     dim_t ncols,
@@ -196,6 +220,7 @@ int gooseberry_permute_cuthillmckee(
     const dim_t * blocks,
     dim_t nblocks,
     dim_t * perm);
+#endif
 
 /* include/gooseberry.h IO prototypes; src/io.c:1464-1549.  Sparse readers:
  * CSR, MATRIXMARKET, GRAPH, CLU, SVM, POINT and BCSR; sparse writers: CSR,
@@ -249,6 +274,7 @@ int gooseberry_write_sparse_matrix(
  * pinned buffers and checked on the GPU (other formats:
  * GOOSEBERRY_ERROR_UNIMPLEMENTED).  The arrays are device memory (free with
  * gooseberry_device_free). */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_read_sparse_matrix_device(
     int type,
     const char * file,
@@ -257,8 +283,10 @@ int gooseberry_read_sparse_matrix_device(
     ind_t ** d_rowptr,
     dim_t ** d_rowind,
     real_t ** d_rowval);
+#endif
 
 /* The same parse of CSR text already in device memory (len bytes). */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_parse_csr_device(
     const char * d_text,
     size_t len,
@@ -267,6 +295,7 @@ int gooseberry_parse_csr_device(
     ind_t ** d_rowptr,
     dim_t ** d_rowind,
     real_t ** d_rowval);
+#endif
 
 /* src/io.c:1552-1639 */
 int gooseberry_read_labels(
@@ -323,6 +352,7 @@ int gooseberry_countingsort_kv(
 
 /* In-place CSR permutation, src/permute.c:120-205 (host pointers): new row
  * i is old row rowperm[i]; column c becomes i where colperm[i] == c. */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_permute_sparse(
     dim_t nrows,
     dim_t ncols,
@@ -331,6 +361,7 @@ int gooseberry_permute_sparse(
     real_t * rowval,
     const dim_t * rowperm,
     const dim_t * colperm);
+#endif
 
 /* In-place dense permutation, src/permute.c:208-245 (rev == 0) and its
  * inverse src/permute.c:248-285 (rev != 0). */
@@ -545,6 +576,7 @@ int gooseberry_matrix_spmv_f32(
 /* include/gooseberry.h:174-184 for the reference's float build (real_t =
  * float, the default without GOOSEBERRY_DOUBLE_REALS): the same call as
  * gooseberry_spmult with FP32 values and vectors, host pointers. */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_spmult_f32(
     dim_t anrows,
     dim_t ancols,
@@ -556,6 +588,7 @@ int gooseberry_spmult_f32(
     float * crowval,
     const dim_t * blocks,
     dim_t nblocks);
+#endif
 
 /* Host-pointer form of the reference's internal cgd() (src/cgd.h:29-34),
  * absolute threshold `error` as in the reference. */
@@ -809,6 +842,7 @@ int gooseberry_matrix_pagerank(
     gooseberry_pagerank_info_t * info);
 
 /* The same on host CSR arrays (n x n) and a host output vector. */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_pagerank(
     dim_t n,
     const ind_t * rowptr,
@@ -819,6 +853,7 @@ int gooseberry_pagerank(
     real_t minerror,
     real_t * out,
     gooseberry_pagerank_info_t * info);
+#endif
 
 /* ------------------------------------------------------- synthetic inputs */
 
@@ -853,11 +888,13 @@ int gooseberry_matrix_generate_sdd(
     gooseberry_matrix_t * r_mat);
 
 /* Copy a resident matrix back to host arrays (nrows+1, nnz, nnz). */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_matrix_download(
     gooseberry_matrix_t mat,
     ind_t * rowptr,
     dim_t * rowind,
     real_t * rowval);
+#endif
 
 /* d_out[i] uniform in [lo, hi) from a counter-based hash of (seed, stream,
  * i), filled on the device. */
@@ -870,6 +907,7 @@ int gooseberry_generate_uniform_device(
     real_t * d_out);
 /* 2D 5-point / 3D 7-point Dirichlet Laplacians on nx*ny(*nz) grids,
  * lexicographic order, columns ascending within a row (host arrays). */
+#ifndef GOOSEBERRY_64BIT_INDICES /* 32-bit row pointers only */
 int gooseberry_generate_laplacian(
     int dims,
     dim_t nx, dim_t ny, dim_t nz,
@@ -877,6 +915,7 @@ int gooseberry_generate_laplacian(
     ind_t ** rowptr,
     dim_t ** rowind,
     real_t ** rowval);
+#endif
 
 /* n values uniform in [lo, hi) from (seed, stream): xoshiro256** seeded
  * with SplitMix64, u = (x >> 11) * 2^-53. */
@@ -887,6 +926,134 @@ int gooseberry_generate_uniform(
     real_t lo,
     real_t hi,
     real_t * out);
+
+/* -------------------------------------------------- 64-bit row pointers */
+/* The functions above that take ind_t, with 64-bit row pointers: the
+ * reference's GOOSEBERRY_64BIT_INDICES build (include/gooseberry.h:65-71,
+ * CMakeLists.txt:59-60).  A program compiled with that macro calls these
+ * under the reference's names.  Semantics, errors and results are those of
+ * the 32-bit forms.
+ *
+ * A matrix with at most 2^31 entries is stored as with 32-bit row
+ * pointers.  A larger one (up to what HBM holds: about 24 billion entries
+ * of FP64 values and 32-bit indices in 288 GB) is held as row parts of at
+ * most 2^31 entries, each part with 32-bit row pointers counted from its
+ * first entry and an SpMV plan of its own; the SpMV is one launch per part
+ * into that part's rows of y, and CG adds p.Ap over the whole vectors.
+ * Such a matrix takes the FP64 SpMV, SpMM and CG; the FP32 and mixed paths,
+ * batches and servers, PageRank, column splits, the sparse products and
+ * Cuthill-McKee return GOOSEBERRY_ERROR_UNIMPLEMENTED for it.  The
+ * environment variable GOOSEBERRY_PART_NNZ (read when a matrix is created)
+ * sets a smaller part size, down to 1, as a test and tuning knob.  A single
+ * row of more entries than a part holds is GOOSEBERRY_ERROR_INVALIDINPUT.
+ *
+ * The readers and writers and the sparse products keep 32-bit storage
+ * inside: their _i64 forms widen or narrow the row pointers and return
+ * GOOSEBERRY_ERROR_UNIMPLEMENTED for more than 2^32 - 1 entries. */
+int gooseberry_scale_i64(
+    uint64_t n,
+    real_t * val,
+    real_t s);
+
+int gooseberry_add_scalar_i64(
+    uint64_t n,
+    real_t * val,
+    real_t s);
+
+int gooseberry_spmult_i64(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const uint64_t * arowptr,
+    const dim_t * arowind,
+    const real_t * arowval,
+    const real_t * bcolval,
+    real_t * crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
+int gooseberry_spmultsp_i64(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const uint64_t * arowptr,
+    const dim_t * arowind,
+    const real_t * arowval,
+    const uint64_t * browptr,
+    const dim_t * browind,
+    const real_t * browval,
+    real_t * crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
+int gooseberry_spmultsp_sp_i64(
+    dim_t anrows,
+    dim_t ancols,
+    dim_t bncols,
+    const uint64_t * arowptr,
+    const dim_t * arowind,
+    const real_t * arowval,
+    const uint64_t * browptr,
+    const dim_t * browind,
+    const real_t * browval,
+    uint64_t ** crowptr,
+    dim_t ** crowind,
+    real_t ** crowval,
+    const dim_t * blocks,
+    dim_t nblocks);
+
+int gooseberry_read_sparse_matrix_i64(
+    int type,
+    const char * file,
+    dim_t * r_nrows,
+    dim_t * r_ncols,
+    uint64_t ** rowptr,
+    dim_t ** rowind,
+    real_t ** rowval);
+
+int gooseberry_write_sparse_matrix_i64(
+    int type,
+    const char * file,
+    dim_t nrows,
+    dim_t ncols,
+    const uint64_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval);
+
+int gooseberry_matrix_create_i64(
+    dim_t nrows,
+    dim_t ncols,
+    const uint64_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * r_mat);
+
+int gooseberry_matrix_create_device_i64(
+    dim_t nrows,
+    dim_t ncols,
+    const uint64_t * d_rowptr,
+    const dim_t * d_rowind,
+    const real_t * d_rowval,
+    const gooseberry_plan_t * plan,
+    gooseberry_matrix_t * r_mat);
+
+int gooseberry_cgd_i64(
+    dim_t nrows,
+    dim_t ncols,
+    const uint64_t * rowptr,
+    const dim_t * rowind,
+    const real_t * rowval,
+    const real_t * b,
+    real_t * x,
+    real_t error,
+    size_t niter,
+    gooseberry_cgd_info_t * info);
+
+/* row parts of a resident matrix (0: one ordinary matrix) */
+int gooseberry_matrix_parts(
+    gooseberry_matrix_t mat,
+    dim_t * r_nparts);
 
 #ifdef __cplusplus
 }
