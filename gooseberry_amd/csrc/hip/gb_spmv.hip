@@ -205,9 +205,13 @@ gb_plan_narrow(const uint32_t *__restrict__ rowptr,
 // With DOT the kernel also forms the block's share of p.Ap (x = p, y = Ap)
 // and writes it to partials[blockIdx.x]; the next CG kernel adds the
 // partials in a fixed order (no fences or atomics: deterministic).
-#ifndef GB_SPMV_PIPE_WAVES
-#define GB_SPMV_PIPE_WAVES 4
-#endif
+// The pipelined row-block kernel's workgroup: 512 threads (one row of a
+// block per thread), built for eight wavefronts per SIMD (at most 64
+// VGPRs), four workgroups per CU.  Measured on the 160^3 Laplacian with
+// value indices, per SpMV / CG iteration: 256 threads at four waves per SIMD
+// 55.5 / 100.4 us, 512 threads at eight 54.1 / 95.5 us.
+#define GB_SPMV_PIPE_THREADS 512
+#define GB_SPMV_PIPE_WAVES 8
 #ifndef GB_SPMV_WAVES
 // wavefronts per SIMD the register budget must allow: 72 VGPRs (the FP64
 // variants spill 2 to scratch).  Measured on the 160^3 Laplacian with the
@@ -417,29 +421,41 @@ gb_spmv_kernel(const uint32_t *__restrict__ rowptr,
 // gathers, stages and adds the current block, so one block's memory
 // latency overlaps the previous block's work.  Same products and sums as
 // gb_spmv_kernel's row blocks; with DOT one p.Ap partial per workgroup.
-template <typename T, bool DOT, bool NARROW, bool VD>
+
+// entries of one row block per thread of an NT-thread workgroup
+template <uint32_t NT>
+struct gb_pipe_iters {
+  static constexpr uint32_t n = (GB_STREAM_LDS + NT - 1) / NT;
+};
+
+template <typename T, bool DOT, bool NARROW, bool VD, uint32_t NT>
 struct gb_pipe_blk {
+  static constexpr uint32_t ITERS = gb_pipe_iters<NT>::n;
   uint32_t r0, r1, k0, cnt;
   uint32_t pa0, pa1, pb0, pb1;
   T xa, xb;
-  uint32_t c[GB_STREAM_ITERS];
+  uint32_t c[ITERS];
   // values, or (VD) value indices
-  typename std::conditional<VD, uint32_t, T>::type v[GB_STREAM_ITERS];
+  typename std::conditional<VD, uint32_t, T>::type v[ITERS];
 };
 
-template <typename T, bool DOT, bool NARROW, bool VD>
-__device__ __forceinline__ void gb_pipe_load(gb_pipe_blk<T, DOT, NARROW, VD> &B, const uint4 d,
+// rows per block are at most 2 GB_BLOCK (the plan's rowcap): one or two
+// rows per thread
+template <typename T, bool DOT, bool NARROW, bool VD, uint32_t NT>
+__device__ __forceinline__ void gb_pipe_load(gb_pipe_blk<T, DOT, NARROW, VD, NT> &B,
+                                             const uint4 d,
                                              const uint32_t *__restrict__ rowptr,
                                              const uint32_t *__restrict__ rowind,
                                              const int16_t *__restrict__ dcol,
                                              const T *__restrict__ val,
                                              const uint8_t *__restrict__ vidx,
                                              const T *__restrict__ x, uint32_t tid) {
+  constexpr bool TWO = NT < 2 * GB_BLOCK;
   B.r0 = d.x;
   B.r1 = d.y;
   B.k0 = d.z;
   B.cnt = d.w;
-  const uint32_t ra = d.x + tid, rb = d.x + tid + GB_BLOCK;
+  const uint32_t ra = d.x + tid, rb = d.x + tid + NT;
   B.pa0 = B.pa1 = B.pb0 = B.pb1 = 0;
   B.xa = B.xb = 0;
   if (ra < d.y) {
@@ -447,7 +463,7 @@ __device__ __forceinline__ void gb_pipe_load(gb_pipe_blk<T, DOT, NARROW, VD> &B,
     B.pa1 = rowptr[ra + 1];
     if (DOT) B.xa = x[ra];
   }
-  if (rb < d.y) {
+  if (TWO && rb < d.y) {
     B.pb0 = rowptr[rb];
     B.pb1 = rowptr[rb + 1];
     if (DOT) B.xb = x[rb];
@@ -455,9 +471,9 @@ __device__ __forceinline__ void gb_pipe_load(gb_pipe_blk<T, DOT, NARROW, VD> &B,
   if (d.w == 0) return;
   const uint32_t klast = d.z + d.w - 1;
 #pragma unroll
-  for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-    if (i * GB_BLOCK < d.w) {
-      const uint32_t k = min(d.z + i * GB_BLOCK + tid, klast);
+  for (uint32_t i = 0; i < B.ITERS; ++i) {
+    if (i * NT < d.w) {
+      const uint32_t k = min(d.z + i * NT + tid, klast);
       if (NARROW) B.c[i] = (uint32_t)(int32_t)dcol[k];
       else B.c[i] = __builtin_nontemporal_load(&rowind[k]);
       if (VD) B.v[i] = vidx[k];
@@ -467,24 +483,26 @@ __device__ __forceinline__ void gb_pipe_load(gb_pipe_blk<T, DOT, NARROW, VD> &B,
   }
 }
 
-template <typename T, bool DOT, bool NARROW, bool VD>
-__global__ void __launch_bounds__(GB_BLOCK) __attribute__((amdgpu_waves_per_eu(GB_SPMV_PIPE_WAVES)))
+template <typename T, bool DOT, bool NARROW, bool VD, uint32_t NT, int WAVES>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ rowind,
              const int16_t *__restrict__ dcol, const T *__restrict__ val,
              const uint8_t *__restrict__ vidx, const T *__restrict__ vtab,
              const T *__restrict__ x, T *__restrict__ y, uint32_t ystride, uint32_t cshift,
              const uint4 *__restrict__ bdesc, uint32_t nstream, uint32_t nvals,
              double *__restrict__ partials) {
+  constexpr bool TWO = NT < 2 * GB_BLOCK;
+  using blk_t = gb_pipe_blk<T, DOT, NARROW, VD, NT>;
   __shared__ T prod[GB_STREAM_LDS];
   __shared__ T ltab[VD ? GB_VDICT_MAX : 1];
-  __shared__ double red[GB_WAVES_PER_BLOCK];
+  __shared__ double red[NT / GB_WAVE];
   const uint32_t tid = threadIdx.x;
   const uint32_t G = gridDim.x;
   if (VD) {
-    for (uint32_t i = tid; i < nvals; i += GB_BLOCK) ltab[i] = vtab[i];
+    for (uint32_t i = tid; i < nvals; i += NT) ltab[i] = vtab[i];
   }
   double dotp = 0.0;
-  gb_pipe_blk<T, DOT, NARROW, VD> cur, nxt;
+  blk_t cur, nxt;
   uint32_t b = blockIdx.x;
   uint4 dn = make_uint4(0u, 0u, 0u, 0u);
   if (b < nstream) {
@@ -504,18 +522,18 @@ gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ r
     const uint32_t r0 = cur.r0, k0 = cur.k0, cnt = cur.cnt;
     if (cnt > 0) {
       const T *xg = NARROW ? x + r0 + cshift : x;
-      T xv[GB_STREAM_ITERS];
+      T xv[blk_t::ITERS];
 #pragma unroll
-      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-        if (i * GB_BLOCK < cnt) {
+      for (uint32_t i = 0; i < blk_t::ITERS; ++i) {
+        if (i * NT < cnt) {
           if (NARROW) xv[i] = xg[(int32_t)cur.c[i]];
           else xv[i] = xg[cur.c[i]];
         }
       }
 #pragma unroll
-      for (int i = 0; i < GB_STREAM_ITERS; ++i) {
-        const uint32_t e = i * GB_BLOCK + tid;
-        if (i * GB_BLOCK < cnt && e < cnt) {
+      for (uint32_t i = 0; i < blk_t::ITERS; ++i) {
+        const uint32_t e = i * NT + tid;
+        if (i * NT < cnt && e < cnt) {
           T v;
           if constexpr (VD) v = ltab[cur.v[i]];
           else v = cur.v[i];
@@ -525,14 +543,14 @@ gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ r
     }
     __syncthreads();
     // one thread per row: the reference's left-to-right sum from +0.0
-    const uint32_t ra = r0 + tid, rb = r0 + tid + GB_BLOCK;
+    const uint32_t ra = r0 + tid, rb = r0 + tid + NT;
     if (ra < cur.r1) {
       T s = 0;
       for (uint32_t k = cur.pa0 - k0; k < cur.pa1 - k0; ++k) s += prod[k];
       y[(size_t)ra * ystride] = s;
       if (DOT) dotp += (double)cur.xa * (double)s;
     }
-    if (rb < cur.r1) {
+    if (TWO && rb < cur.r1) {
       T s = 0;
       for (uint32_t k = cur.pb0 - k0; k < cur.pb1 - k0; ++k) s += prod[k];
       y[(size_t)rb * ystride] = s;
@@ -551,7 +569,7 @@ gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ r
     if (tid == 0) {
       double t = red[0];
 #pragma unroll
-      for (int q = 1; q < GB_WAVES_PER_BLOCK; ++q) t += red[q];
+      for (uint32_t q = 1; q < NT / GB_WAVE; ++q) t += red[q];
       partials[blockIdx.x] = t;
     }
   }
@@ -569,8 +587,9 @@ uint32_t gb_spmv_grid(const gbd_matrix *m) {
 // The pipelined row-block kernel runs matrices whose rows are all short
 // (every work item an LDS-staged row block): on the plan's grid when it
 // sets one, and by default for value-indexed plans, four workgroups per CU
-// (its register budget).  Measured on the 160^3 Laplacian with 8-bit value
-// indices: 56 against 64 us per SpMV, 100 against 111 us per CG iteration;
+// (the most that are resident at eight waves per SIMD).  Measured on the
+// 160^3 Laplacian with 8-bit value indices (256-thread workgroups): 56
+// against 64 us per SpMV, 100 against 111 us per CG iteration;
 // with FP64 values read as stored it is slower (78 against 68 us), the
 // stream then being HBM-bound.
 static void gb_pipe_choose(gbd_matrix *m) {
@@ -725,9 +744,10 @@ static void gb_spmv_go(const gbd_matrix *m, uint32_t grid, hipStream_t s, const 
                        const uint8_t *vidx, const T *vtab, const T *x, T *y,
                        uint32_t ystride, double *partials, const gbd_cg_state *cg) {
   if (gb_spmv_piped(m)) {
-    gb_spmv_pipe<T, DOT, NARROW, VD><<<grid, GB_BLOCK, 0, s>>>(
-        m->rowptr, m->rowind, NARROW ? m->dcol : nullptr, val, vidx, vtab, x, y, ystride,
-        m->cshift, m->bdesc, m->nstream, m->nvals, partials);
+    gb_spmv_pipe<T, DOT, NARROW, VD, GB_SPMV_PIPE_THREADS, GB_SPMV_PIPE_WAVES>
+        <<<grid, GB_SPMV_PIPE_THREADS, 0, s>>>(
+            m->rowptr, m->rowind, NARROW ? m->dcol : nullptr, val, vidx, vtab, x, y, ystride,
+            m->cshift, m->bdesc, m->nstream, m->nvals, partials);
     return;
   }
   const uint32_t *wave_rows = m->rowbins + (m->nrows - m->nwave - m->nblockrows);

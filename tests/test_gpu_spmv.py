@@ -424,10 +424,14 @@ def test_value_index_laplacian_same_bits(gb, oracle):
     assert np.array_equal(yv, yw) and np.array_equal(yv, oracle.spmv(rp, ri, rv, x))
     xf = x.astype(np.float32)
     assert np.array_equal(Mv.spmv_f32(xf), Mw.spmv_f32(xf))
+    # CG: the value-indexed plan runs the pipelined kernel, whose p.Ap partials
+    # are grouped by its workgroups rather than by row block, so the dot
+    # products (not the SpMV) round differently
     b = gb.uniform(4, n)
     xv, iv = Mv.cgd(b, 1e-10, gb.TOL_RELATIVE)
     xw, iw = Mw.cgd(b, 1e-10, gb.TOL_RELATIVE)
-    assert iv.niter == iw.niter and np.array_equal(xv, xw)
+    assert abs(int(iv.niter) - int(iw.niter)) <= 1
+    assert np.max(np.abs(xv - xw)) <= 1e-9 * np.max(np.abs(xw))
 
 
 @pytest.mark.parametrize("nvals,expect", [(1, 1), (200, 200), (256, 256), (257, 0)])
