@@ -118,7 +118,8 @@ class _Plan(C.Structure):
                 ("window", C.c_uint32), ("rowcap", C.c_uint32),
                 ("exact", C.c_uint32), ("grid", C.c_uint32),
                 ("xcd_remap", C.c_uint32), ("csplit", C.c_uint32),
-                ("wide_index", C.c_uint32), ("wide_values", C.c_uint32)]
+                ("wide_index", C.c_uint32), ("wide_values", C.c_uint32),
+                ("xtile", C.c_uint32)]
 
 
 class _MatrixInfo(C.Structure):
@@ -415,7 +416,8 @@ def _plan(plan):
               plan.get("window", 0), plan.get("rowcap", 0),
               1 if plan.get("exact") else 0, plan.get("grid", 0),
               int(plan.get("xcd_remap", 0)), plan.get("csplit", 0),
-              1 if plan.get("wide_index") else 0, 1 if plan.get("wide_values") else 0)
+              1 if plan.get("wide_index") else 0, 1 if plan.get("wide_values") else 0,
+              int(plan.get("xtile", 0)))
     return C.byref(p)
 
 
@@ -497,7 +499,8 @@ class Matrix:
                          "exact": i.plan.exact, "grid": i.plan.grid,
                          "xcd_remap": i.plan.xcd_remap, "csplit": i.plan.csplit,
                          "wide_index": i.plan.wide_index,
-                         "wide_values": i.plan.wide_values},
+                         "wide_values": i.plan.wide_values,
+                         "xtile": i.plan.xtile},
                 "nblocks_stream": i.nblocks_stream, "nrows_wave": i.nrows_wave,
                 "nrows_block": i.nrows_block, "hist": list(i.hist),
                 "index16": i.index16, "values8": i.values8}

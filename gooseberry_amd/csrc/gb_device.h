@@ -102,6 +102,10 @@ typedef struct gbd_plan_params {
   uint32_t wide_values;/* 1: the values are read as stored; 0: as 8-bit
                           indices into a table of the distinct values when
                           there are at most 256 of them (no column split) */
+  uint32_t xtile;      /* C > 0 (a power of two, 256..8192): the FP64 SpMV
+                          stages x through LDS in tiles of C columns, each
+                          workgroup owning a block of rows whose y it keeps
+                          in LDS (matrices with scattered columns); 0: off */
 } gbd_plan_params;
 
 /* Default parameters for a matrix with the given row-length histogram

@@ -42,6 +42,14 @@ struct gbd_matrix {
   uint32_t nvals;
   uint32_t pipe_grid;   // > 0: the SpMV runs the persistent pipelined
                         // row-block kernel on this many workgroups
+  // x tiles (params.xtile = C): the entries grouped by (block of xt_R rows,
+  // tile of C columns), each group in CSR order; entry j of the layout is
+  // xt_idx[j] = (row in block << xt_lc) | (column in tile) with value
+  // xt_val[j], group g = b xt_T + t holding [xt_ptr[g], xt_ptr[g + 1])
+  uint32_t *xt_ptr;     // xt_W xt_T + 1
+  uint32_t *xt_idx;     // nnz
+  double *xt_val;       // nnz
+  uint32_t xt_R, xt_W, xt_T, xt_lc;
   uint64_t hist[33];
   // column split (params.csplit = P > 1): P views, view r holding the
   // entries of columns [r W, (r + 1) W), W = ceil(ncols / P), in row order;
@@ -73,6 +81,8 @@ struct gbd_matrix {
 };
 
 #define GB_SPLIT_MAX 64
+// columns of x an x-tile plan stages in LDS at most
+#define GB_XT_CMAX 8192
 // entries per row part of a matrix with 64-bit row pointers: every entry
 // offset inside a part, and every offset a kernel forms past a part's last
 // entry, stays below 2^32
@@ -145,7 +155,7 @@ uint32_t gb_spmv_grid(const gbd_matrix *m);
 
 // p.Ap partials written by one SpMV launch with DOT (one per workgroup)
 static inline uint32_t gb_spmv_npartials(const gbd_matrix *m) {
-  return (m->nsplit || m->nparts) ? gb_split_sum_grid(m) : gb_spmv_grid(m);
+  return (m->nsplit || m->nparts || m->xt_ptr) ? gb_split_sum_grid(m) : gb_spmv_grid(m);
 }
 
 // Enqueue y = A x; when cg != nullptr also write the per-workgroup partial

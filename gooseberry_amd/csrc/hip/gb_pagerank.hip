@@ -269,6 +269,7 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   mm.nvals = 0;
   mm.nsplit = 0;
   mm.split = nullptr;
+  mm.xt_ptr = nullptr;  // the x-tile layout holds A's values: row blocks here
   if (m->nsplit && nnz) {
     // a column-split plan: the slices' copy of the values divided by the
     // same column sums (the same quotient per entry), and views over it

@@ -196,6 +196,7 @@ static void plan_to_dev(const gooseberry_plan_t *p, const gbd_matrix *m,
   d.csplit = p->csplit > 1 ? p->csplit : 0;
   d.wide_index = p->wide_index ? 1 : 0;
   d.wide_values = p->wide_values ? 1 : 0;
+  d.xtile = p->xtile;
   *out = d;
   *have = 1;
 }
@@ -370,6 +371,7 @@ int gooseberry_matrix_info(gooseberry_matrix_t mat,
   info->plan.csplit = p.csplit;
   info->plan.wide_index = p.wide_index;
   info->plan.wide_values = p.wide_values;
+  info->plan.xtile = p.xtile;
   info->index16 = gbd_matrix_index16(mat->m);
   info->values8 = gbd_matrix_values8(mat->m);
   return GOOSEBERRY_SUCCESS;

@@ -158,6 +158,14 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
       if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
       if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
     }
+    /* 5. x through LDS in column tiles (the FP64 SpMV only) */
+    static const uint32_t xtiles[] = {4096, 8192};
+    for (size_t i = 0; precision == GOOSEBERRY_PRECISION_FP64 && i < 2; ++i) {
+      gbd_plan_params p = base;
+      p.xtile = xtiles[i];
+      if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
+      if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
+    }
   }
   gooseberry_plan_t pub;
   pub.short_max = best.short_max;
@@ -170,6 +178,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
   pub.csplit = best.csplit;
   pub.wide_index = best.wide_index;
   pub.wide_values = best.wide_values;
+  pub.xtile = best.xtile;
   /* through the public replan: it also drops the CG workspace, which is
    * sized by the plan */
   if ((rv = gooseberry_matrix_replan(mat, &pub)) != GOOSEBERRY_SUCCESS) goto done;

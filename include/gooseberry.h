@@ -417,6 +417,13 @@ typedef struct gooseberry_plan_t {
                        each entry's value is read as an 8-bit index into a
                        table of them (1 byte an entry instead of 8; CSR-VI,
                        the same values, so the same results) */
+  dim_t xtile;      /* C (a power of two, 256..8192): the FP64 SpMV stages x
+                       through LDS in tiles of C columns; each workgroup owns
+                       a block of rows and keeps its y in LDS, adding one
+                       tile's products at a time (matrices with scattered
+                       columns; a row's sum is regrouped by tile, as with a
+                       column split, and the layout costs 12 bytes an entry
+                       more); 0: off.  Not with csplit or exact. */
 } gooseberry_plan_t;
 
 /* Upload a host CSR matrix to HBM and build its SpMV plan on the GPU. */

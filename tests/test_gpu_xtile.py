@@ -1,0 +1,108 @@
+"""The x-tile SpMV plan (plan xtile = C) through the C ABI.
+
+The FP64 SpMV stages x through LDS in tiles of C columns; each workgroup
+owns a block of rows and adds one tile's products at a time.  A row's sum
+is regrouped by tile (as with a column split), so the bar is the SpMV
+contract, 1e-14 relative inf-norm against the oracle (src/blas.c:67-92),
+bitwise run to run, and bitwise on integer-valued data (every product and
+partial sum exact).  The structural cases: rows whose run inside one tile
+spans many threads' ranges, empty rows and groups, row counts and column
+counts that are not multiples of the block and tile, SpMM strides, CG's
+p.Ap, and the paths that keep the row blocks (FP32, PageRank).
+"""
+import numpy as np
+import pytest
+
+from test_gpu_spmv import irregular, rel_inf
+
+pytestmark = pytest.mark.gpu
+
+
+def _scattered(n, ncols, seed, lengths=(), density=0.002, integer=False):
+    rp, ri, rv = irregular(n, seed, lengths, density=density, ncols=ncols)
+    if integer:
+        rv = np.round(rv * 8.0)
+    return rp, ri, rv
+
+
+@pytest.mark.parametrize("xtile", [256, 1024, 8192])
+def test_xtile_exact_on_integers(gb, oracle, xtile):
+    # integer values and x: every regrouped sum is exact, so the plan must
+    # give the oracle's bits; row 3's 30000 entries all sit in tile 0, so its
+    # run there spans about 30 entries of every thread's range
+    n, ncols = 40001, 70001
+    rp, ri, rv = _scattered(n, ncols, 31, [(3, 30000), (4, 5000), (40000, 2000), (10, 0)],
+                            integer=True)
+    k3 = slice(int(rp[3]), int(rp[4]))
+    ri[k3] = np.random.default_rng(32).integers(0, min(xtile, ncols), k3.stop - k3.start)
+    x = np.random.default_rng(33).integers(-8, 9, ncols).astype(np.float64)
+    M = gb.Matrix(rp, ri, rv, ncols=ncols, plan={"xtile": xtile})
+    assert M.info()["plan"]["xtile"] == xtile
+    y = M.spmv(x)
+    assert np.array_equal(y, oracle.spmult(n, ncols, 1, rp, ri, rv, x))
+    assert np.array_equal(M.spmv(x), y)
+
+
+@pytest.mark.parametrize("xtile", [512, 4096])
+def test_xtile_random_values_vs_oracle(gb, oracle, xtile):
+    n, ncols = 60000, 50000
+    rp, ri, rv = _scattered(n, ncols, 34, [(0, 1500), (59999, 900)], density=0.0006)
+    x = np.random.default_rng(35).uniform(-1, 1, ncols)
+    ref = oracle.spmult(n, ncols, 1, rp, ri, rv, x)
+    M = gb.Matrix(rp, ri, rv, ncols=ncols, plan={"xtile": xtile})
+    y = M.spmv(x)
+    assert rel_inf(y, ref) <= 1e-14
+    # rows of at most one entry per tile are summed in the reference's order
+    y2 = gb.Matrix(rp, ri, rv, ncols=ncols).spmv(x)
+    assert rel_inf(y, y2) <= 1e-14
+    # SpMM: the same columns with a row stride
+    B = np.random.default_rng(36).uniform(-1, 1, (ncols, 3))
+    C = M.spmm(B.T.ravel(), 3).reshape(n, 3)
+    for j in range(3):
+        assert rel_inf(C[:, j], oracle.spmult(n, ncols, 1, rp, ri, rv, B[:, j])) <= 1e-14
+
+
+def test_xtile_empty_and_tiny(gb, oracle):
+    # a matrix smaller than one tile and one row block per CU; empty rows
+    rp, ri, rv = _scattered(300, 200, 37, [(5, 0), (6, 150)], density=0.05, integer=True)
+    x = np.random.default_rng(38).integers(-4, 5, 200).astype(np.float64)
+    M = gb.Matrix(rp, ri, rv, ncols=200, plan={"xtile": 256})
+    assert np.array_equal(M.spmv(x), oracle.spmult(300, 200, 1, rp, ri, rv, x))
+    rp0 = np.zeros(1001, np.uint32)
+    M0 = gb.Matrix(rp0, np.zeros(0, np.uint32), np.zeros(0), ncols=1000, plan={"xtile": 1024})
+    assert np.array_equal(M0.spmv(np.ones(1000)), np.zeros(1000))
+
+
+def test_xtile_cg_and_other_paths(gb, oracle):
+    # CG adds p.Ap over the whole vectors after the tiled SpMV
+    M = gb.generate_spd(60000, 5, gb.DEGREE_ZIPF, 1.5, 2, 400)
+    rp, ri, rv = M.download()
+    n = rp.size - 1
+    b = gb.uniform(6, n)
+    err = 1e-10 * np.linalg.norm(b)
+    xr, it, rmse, _ = oracle.cgd(rp, ri, rv, b, err)
+    Mt = gb.Matrix(rp, ri, rv, plan={"xtile": 4096})
+    x, info = Mt.cgd(b, err, gb.TOL_ABSOLUTE)
+    assert info.converged == 1 and abs(int(info.niter) - it) <= 2, (info.niter, it)
+    assert np.linalg.norm(b - oracle.spmv(rp, ri, rv, x)) <= 5e-10 * np.linalg.norm(b)
+    # the FP32 SpMV and PageRank keep the row blocks: the same as the plain plan
+    Mp = gb.Matrix(rp, ri, rv)
+    xf = np.random.default_rng(7).uniform(-1, 1, n).astype(np.float32)
+    assert np.array_equal(Mt.spmv_f32(xf), Mp.spmv_f32(xf))
+    pt, pp = Mt.pagerank(niter=5)[0], Mp.pagerank(niter=5)[0]
+    assert np.array_equal(pt, pp)
+    # replans in and out of the layout
+    Mt.replan({"csplit": 4})
+    assert Mt.info()["plan"]["xtile"] == 0
+    Mt.replan({"xtile": 8192})
+    xv = gb.uniform(8, n)
+    assert rel_inf(Mt.spmv(xv), oracle.spmv(rp, ri, rv, xv)) <= 1e-14
+
+
+@pytest.mark.parametrize("plan", [{"xtile": 300}, {"xtile": 128}, {"xtile": 16384},
+                                  {"xtile": 1024, "csplit": 4}, {"xtile": 1024, "exact": 1}])
+def test_xtile_refused_plans(gb, plan):
+    rp, ri, rv = gb.laplacian(2, 32)
+    with pytest.raises(gb.GooseberryError) as e:
+        gb.Matrix(rp, ri, rv, plan=plan)
+    assert e.value.name == "INVALIDINPUT"
