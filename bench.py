@@ -102,6 +102,16 @@ def spmv_kernel_name(info):
     return "gb_spmv_kernel<double, false, %s, %s>" % (i16, vd)
 
 
+def plan_kernel_name(plan):
+    """The FP64 SpMV kernel(s) a tuned plan runs on a matrix with long rows."""
+    if plan.get("xtile"):
+        return "gb_spmv_xtile (x staged through LDS in tiles of %d columns)" % plan["xtile"]
+    if plan.get("csplit"):
+        return ("gb_spmv_kernel<double, false, false, false> over %d column slices + "
+                "gb_split_sum" % plan["csplit"])
+    return "gb_spmv_kernel<double, false, false, false>"
+
+
 def cg_iter_bytes(n, nnz):
     # minimum per iteration: A once, read/write x, r, p
     return 12 * nnz + 4 * (n + 1) + 48 * n
@@ -435,7 +445,7 @@ def spmv_roofline(gb, M, dx, dy, nrep=20, val_bytes=8):
     a = ab / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": a, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": a / PEAK_HBM_GBPS, "traffic": None,
-            "kernel": "gb_spmv_kernel<double,false>",
+            "kernel": plan_kernel_name(inf["plan"]),
             "algorithmic_bytes_per_launch": ab, "avg_launch_ms": ms}
 
 
@@ -470,6 +480,14 @@ def run_zipf(args, gb):
     xref = dref.numpy()
     log("zipf: FP64 CG to 1e-12: %d iterations, status %d" % (iref.niter, iref.status))
 
+    def errors_of(dx, info):
+        x = dx.numpy()
+        return {"status": int(info.status), "iterations": int(info.niter),
+                "refreshes": int(info.nrefine),
+                "true_rel_residual": true_rel_residual(gb, M, db, dx, bh),
+                "rel_error_vs_fp64_1e-12": float(np.linalg.norm(x - xref) /
+                                                 np.linalg.norm(xref))}
+
     def fp64_solve():
         dx = gb.DeviceArray(n)
         gb.synchronize()
@@ -491,21 +509,21 @@ def run_zipf(args, gb):
         B.free()
         return dt, info, dx
 
+    # each solve on the plan tuned for its own SpMV: FP64 CG on the FP64
+    # plan, the FP32-inner solves on the plan tuned for the FP32 SpMV (the
+    # batched engine's kernels)
     for _ in range(args.warmup):
         fp64_solve()
-        mixed_solve()
     f64 = [fp64_solve() for _ in range(args.steps)]
+    f64_errs = errors_of(f64[-1][2], f64[-1][1])
+    plan32, tuned32_ms = M.autotune(gb.PRECISION_MIXED, 5)
+    log("zipf: FP32-inner plan %s (FP32 SpMV %.1f us)" % (plan32, tuned32_ms * 1e3))
+    for _ in range(args.warmup):
+        mixed_solve()
     mix = [mixed_solve() for _ in range(args.steps)]
     log("zipf: FP64 %.1f ms, mixed %.1f ms" % (np.median([a for a, _, _ in f64]) * 1e3,
                                                np.median([a for a, _, _ in mix]) * 1e3))
 
-    def errors(dx, info):
-        x = dx.numpy()
-        return {"status": int(info.status), "iterations": int(info.niter),
-                "refreshes": int(info.nrefine),
-                "true_rel_residual": true_rel_residual(gb, M, db, dx, bh),
-                "rel_error_vs_fp64_1e-12": float(np.linalg.norm(x - xref) /
-                                                 np.linalg.norm(xref))}
 
     t64 = float(np.median([a for a, _, _ in f64]))
     tmx = float(np.median([a for a, _, _ in mix]))
@@ -515,7 +533,7 @@ def run_zipf(args, gb):
         if p != prec:
             mixed_solve(p)
             dt, info, dx = mixed_solve(p)
-            other[name] = dict(errors(dx, info), solve_ms=dt * 1e3)
+            other[name] = dict(errors_of(dx, info), solve_ms=dt * 1e3)
     cfg = {"workload": ("random symmetric diagonally dominant SPD, n=%d, degrees Zipf(1.5) "
                         "clipped to [2,4096] then symmetrised (nnz=%d, mean row length "
                         "%.2f), off-diagonals U[-1,0), diagonal 1.01 x |row sum|, seed 2; "
@@ -525,14 +543,15 @@ def run_zipf(args, gb):
            "n": n, "nnz": nnz, "tol": TOL, "precision": args.precision}
     out = line(args.steps / sum(a for a, _, _ in mix), "solves/s", args, tmx * 1e3,
                "f32+f64", cfg, roofline=roof,
-               mixed=dict(errors(mix[-1][2], mix[-1][1]), solve_ms=tmx * 1e3),
-               fp64=dict(errors(f64[-1][2], f64[-1][1]), solve_ms=t64 * 1e3),
+               mixed=dict(errors_of(mix[-1][2], mix[-1][1]), solve_ms=tmx * 1e3),
+               fp64=dict(f64_errs, solve_ms=t64 * 1e3),
                speedup_mixed_vs_fp64=t64 / tmx, other_precisions=other,
                p50_solve_ms=pct([a * 1e3 for a, _, _ in mix], 50),
                p99_solve_ms=pct([a * 1e3 for a, _, _ in mix], 99),
                fp64_reference={"tol": 1e-12, "iterations": int(iref.niter)},
                autotune={"plan": plan, "spmv_ms_default": roof_default["avg_launch_ms"],
-                         "spmv_ms_tuned": roof["avg_launch_ms"]},
+                         "spmv_ms_tuned": roof["avg_launch_ms"],
+                         "fp32_inner_plan": plan32, "fp32_spmv_ms_tuned": tuned32_ms},
                generate_s=gen_s)
     if not args.no_cpu_baseline:
         rp, ri, rv = M.download()
@@ -562,8 +581,7 @@ def run_zipf_spmv(args, gb):
         M.replan({"csplit": args.csplit})
         plan = M.info()["plan"]
     roof = spmv_roofline(gb, M, dx, dy)
-    roof["kernel"] = "gb_spmv_kernel<double,false,false> over %d column slices + gb_split_sum" \
-        % plan["csplit"] if plan["csplit"] else "gb_spmv_kernel<double,false,false>"
+    roof["kernel"] = plan_kernel_name(plan)
     cfg = {"workload": ("FP64 SpMV of the random SDD SPD matrix, n=%d, degrees Zipf(1.5) clipped "
                         "to [2,4096] then symmetrised (nnz=%d), x U[-1,1) seed 2" % (n, nnz)),
            "n": n, "nnz": nnz}

@@ -100,6 +100,21 @@ gb_pr_normalise(uint64_t nnz, const uint32_t *__restrict__ rowind,
     nval[l] = val[l] / indeg[rowind[l]];
 }
 
+// The same quotients on an x-tile layout: entry j of group g = b T + t has
+// column (t << lc) | (its word's low lc bits)
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_pr_normalise_xt(const uint32_t *__restrict__ ptr, uint32_t ngroups, uint32_t T,
+                   uint32_t lc, const uint32_t *__restrict__ word,
+                   const double *__restrict__ val, const double *__restrict__ indeg,
+                   double *__restrict__ nval) {
+  const uint32_t cm = (1u << lc) - 1;
+  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const uint32_t c0 = (g % T) << lc, e = ptr[g + 1];
+    for (uint32_t j = ptr[g] + threadIdx.x; j < e; j += GB_BLOCK)
+      nval[j] = val[j] / indeg[c0 | (word[j] & cm)];
+  }
+}
+
 __global__ void __launch_bounds__(GB_BLOCK)
 gb_pr_sink_flags(uint32_t n, const uint32_t *__restrict__ rowptr,
                  uint32_t *__restrict__ flag) {
@@ -269,7 +284,16 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   mm.nvals = 0;
   mm.nsplit = 0;
   mm.split = nullptr;
-  mm.xt_ptr = nullptr;  // the x-tile layout holds A's values: row blocks here
+  if (m->xt_ptr && nnz) {
+    // an x-tile plan: the layout's copy of the values divided by the same
+    // column sums (the same quotient per entry)
+    GB_PR_TRY(gb_dev_alloc((void **)&snval, (size_t)nnz * 8));
+    const uint32_t ng = m->xt_W * m->xt_T;
+    gb_pr_normalise_xt<<<ng < 65536u ? ng : 65536u, GB_BLOCK, 0, s>>>(
+        m->xt_ptr, ng, m->xt_T, m->xt_lc, m->xt_idx, m->xt_val, indeg, snval);
+    GB_PR_TRY(hipGetLastError());
+    mm.xt_val = snval;
+  }
   if (m->nsplit && nnz) {
     // a column-split plan: the slices' copy of the values divided by the
     // same column sums (the same quotient per entry), and views over it

@@ -8,7 +8,8 @@ bitwise run to run, and bitwise on integer-valued data (every product and
 partial sum exact).  The structural cases: rows whose run inside one tile
 spans many threads' ranges, empty rows and groups, row counts and column
 counts that are not multiples of the block and tile, SpMM strides, CG's
-p.Ap, and the paths that keep the row blocks (FP32, PageRank).
+p.Ap, the FP32 SpMV (which keeps the row blocks) and PageRank (the layout
+over the normalised values).
 """
 import numpy as np
 import pytest
@@ -85,12 +86,14 @@ def test_xtile_cg_and_other_paths(gb, oracle):
     x, info = Mt.cgd(b, err, gb.TOL_ABSOLUTE)
     assert info.converged == 1 and abs(int(info.niter) - it) <= 2, (info.niter, it)
     assert np.linalg.norm(b - oracle.spmv(rp, ri, rv, x)) <= 5e-10 * np.linalg.norm(b)
-    # the FP32 SpMV and PageRank keep the row blocks: the same as the plain plan
+    # the FP32 SpMV keeps the row blocks: the same bits as the plain plan
     Mp = gb.Matrix(rp, ri, rv)
     xf = np.random.default_rng(7).uniform(-1, 1, n).astype(np.float32)
     assert np.array_equal(Mt.spmv_f32(xf), Mp.spmv_f32(xf))
-    pt, pp = Mt.pagerank(niter=5)[0], Mp.pagerank(niter=5)[0]
-    assert np.array_equal(pt, pp)
+    # PageRank runs the layout over the normalised values (same quotients,
+    # sums regrouped by tile)
+    pt, pp = Mt.pagerank(niter=20)[0], Mp.pagerank(niter=20)[0]
+    assert np.max(np.abs(pt - pp)) <= 1e-12 * np.max(np.abs(pp))
     # replans in and out of the layout
     Mt.replan({"csplit": 4})
     assert Mt.info()["plan"]["xtile"] == 0
@@ -106,3 +109,18 @@ def test_xtile_refused_plans(gb, plan):
     with pytest.raises(gb.GooseberryError) as e:
         gb.Matrix(rp, ri, rv, plan=plan)
     assert e.value.name == "INVALIDINPUT"
+
+
+def test_autotune_fp64_then_fp32(gb, oracle):
+    # x of 2.4 MB: the tuner tries column splits and x tiles for the FP64
+    # SpMV; tuning the FP32 SpMV next starts from that plan without x tiles
+    M = gb.generate_sdd(300000, 9, 3, 27)
+    rp, ri, rv = M.download()
+    x = gb.uniform(10, 300000)
+    plan64, _ = M.autotune(gb.PRECISION_FP64, 2)
+    assert rel_inf(M.spmv(x), oracle.spmv(rp, ri, rv, x)) <= 1e-14
+    plan32, _ = M.autotune(gb.PRECISION_MIXED, 2)
+    assert plan32["xtile"] == 0
+    assert rel_inf(M.spmv(x), oracle.spmv(rp, ri, rv, x)) <= 1e-14
+    M.replan(dict(plan64))
+    assert M.info()["plan"]["xtile"] == plan64["xtile"]
