@@ -575,10 +575,10 @@ def run_zipf_spmv(args, gb):
     dx = gb.uniform_device(2, n, stream=0)
     dy = gb.DeviceArray(n)
     roof_default = spmv_roofline(gb, M, dx, dy)
-    if args.csplit is None:
+    if args.csplit is None and args.xtile is None:
         plan, _ = M.autotune(gb.PRECISION_FP64, 5)
     else:  # a fixed plan (counter runs: the profiler's slowdown skews tuning)
-        M.replan({"csplit": args.csplit})
+        M.replan({"csplit": args.csplit} if args.csplit else {"xtile": args.xtile})
         plan = M.info()["plan"]
     roof = spmv_roofline(gb, M, dx, dy)
     roof["kernel"] = plan_kernel_name(plan)
@@ -1254,6 +1254,8 @@ def main():
     ap.add_argument("--nsys", type=int, default=512)
     ap.add_argument("--csplit", type=int, default=None,
                     help="zipf-spmv: this many column slices instead of the autotuned plan")
+    ap.add_argument("--xtile", type=int, default=None,
+                    help="zipf-spmv: x tiles of this many columns instead of the autotuned plan")
     ap.add_argument("--spgemm-grid", type=int, default=150)
     ap.add_argument("--duration", type=float, default=60.0,
                     help="served: seconds of Poisson arrivals")
