@@ -1,7 +1,7 @@
 """GPU PageRank (src/gooseberry_bin.c:2622-2737) against the oracle.
 
 The reference adds the whole sink mass to every entry
-(src/gooseberry_bin.c:2689-2694), so with empty rows present the iteration
+(src/gooseberry_bin.c:2682-2688), so with empty rows present the iteration
 grows without bound and the error test never passes: runs with sinks use
 a fixed iteration count, runs to an error threshold use sink-free graphs.
 
