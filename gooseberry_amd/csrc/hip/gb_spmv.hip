@@ -617,38 +617,42 @@ __global__ void gb_xt_gather(const uint32_t *__restrict__ perm, uint64_t nnz,
 
 // Cross-lane moves for the segmented scan, as DPP (data-parallel
 // primitives: a lane reads another lane's register within the instruction,
-// with no LDS round trip).  A lane whose source is outside the pattern
-// (or outside the row mask) gets `old`.
-template <int CTRL, int RMASK>
-__device__ __forceinline__ uint32_t gb_dpp_u32(uint32_t old, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, RMASK, 0xf, false);
+// with no LDS round trip), with bound_ctrl set: a lane whose source lies
+// outside the pattern reads 0, so no register needs a fallback value first.
+template <int CTRL>
+__device__ __forceinline__ uint32_t gb_dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
 }
-template <int CTRL, int RMASK>
+template <int CTRL>
 __device__ __forceinline__ double gb_dpp_f64(double v) {
   const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = gb_dpp_u32<CTRL, RMASK>(0u, (uint32_t)b);
-  const uint32_t hi = gb_dpp_u32<CTRL, RMASK>(0u, (uint32_t)(b >> 32));
+  const uint32_t lo = gb_dpp_u32<CTRL>((uint32_t)b);
+  const uint32_t hi = gb_dpp_u32<CTRL>((uint32_t)(b >> 32));
   return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
-// one step of the segmented inclusive scan: add the partial sum of the
-// source lane when it holds the same row (rows of a group are contiguous,
-// so the lanes between hold it too); the earlier part is the left operand
-template <int CTRL, int RMASK>
-__device__ __forceinline__ void gb_seg_step(double &sc, uint32_t r) {
-  const double ps = gb_dpp_f64<CTRL, RMASK>(sc);
-  const uint32_t pr = gb_dpp_u32<CTRL, RMASK>(0xffffffffu, r);
-  if (pr == r) sc = ps + sc;
+// one step of the segmented inclusive scan: a lane taking part in the step
+// (`on`) adds the partial sum of its source lane when that lane holds the
+// same row (rows of a group are contiguous, so the lanes between hold it
+// too); the earlier part is the left operand.  A lane whose source lies
+// outside the pattern reads row 0 and +0.0: adding +0.0 changes no partial
+// sum except -0.0, which becomes +0.0, and y starts from +0.0, so every
+// result is the same either way.
+template <int CTRL>
+__device__ __forceinline__ void gb_seg_step(double &sc, uint32_t r, bool on) {
+  const double ps = gb_dpp_f64<CTRL>(sc);
+  const uint32_t pr = gb_dpp_u32<CTRL>(r);
+  if (on && pr == r) sc = ps + sc;
 }
 // the scan over 64 lanes: within each row of 16 lanes by shifts of 1, 2,
 // 4 and 8 lanes, then the last lane of row 0 into row 1 and of row 2 into
 // row 3, then the last lane of row 1 into rows 2 and 3
-__device__ __forceinline__ double gb_seg_scan64(double sc, uint32_t r) {
-  gb_seg_step<0x111, 0xf>(sc, r);  // row_shr:1
-  gb_seg_step<0x112, 0xf>(sc, r);  // row_shr:2
-  gb_seg_step<0x114, 0xf>(sc, r);  // row_shr:4
-  gb_seg_step<0x118, 0xf>(sc, r);  // row_shr:8
-  gb_seg_step<0x142, 0xa>(sc, r);  // row_bcast:15
-  gb_seg_step<0x143, 0xc>(sc, r);  // row_bcast:31
+__device__ __forceinline__ double gb_seg_scan64(double sc, uint32_t r, uint32_t lane) {
+  gb_seg_step<0x111>(sc, r, true);               // row_shr:1
+  gb_seg_step<0x112>(sc, r, true);               // row_shr:2
+  gb_seg_step<0x114>(sc, r, true);               // row_shr:4
+  gb_seg_step<0x118>(sc, r, true);               // row_shr:8
+  gb_seg_step<0x142>(sc, r, (lane & 16u) != 0);  // row_bcast:15, rows 1 and 3
+  gb_seg_step<0x143>(sc, r, lane >= 32);         // row_bcast:31, rows 2 and 3
   return sc;
 }
 __device__ __forceinline__ double gb_readlane_f64(double v, uint32_t l) {
@@ -761,7 +765,7 @@ gb_spmv_xtile(const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx
             const bool valid = lane < nvalid;
             // invalid lanes get rows no valid lane has (valid rows are < 2^13)
             const uint32_t r = valid ? (w >> lc) : (0x80000000u | lane);
-            const double sc = gb_seg_scan64(valid ? v * xs[w & (C - 1)] : 0.0, r);
+            const double sc = gb_seg_scan64(valid ? v * xs[w & (C - 1)] : 0.0, r, lane);
             const uint32_t rl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)r);
             if (crow != NONE && crow != rl0) {
               // the carried run ended with the previous chunk
@@ -771,7 +775,9 @@ gb_spmv_xtile(const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx
               }
               crow = NONE;
             }
-            const uint32_t rn = gb_dpp_u32<0x130, 0xf>(0xffffffffu, r);  // wave_shl:1
+            // the next lane's row (wave_shl:1; lane 63, always the last
+            // valid lane or past it, reads 0)
+            const uint32_t rn = gb_dpp_u32<0x130>(r);
             const bool last = lane == nvalid - 1;
             double tot = sc;
             if (r == rl0 && crow == rl0) tot = csum + tot;
