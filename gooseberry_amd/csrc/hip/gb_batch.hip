@@ -9,6 +9,9 @@
 //                    its partials.  A column-split matrix runs over its
 //                    slices into partial vectors (FP64 ones in refresh
 //                    steps), which K3 adds; its refresh norm is tested in K4.
+//                    A matrix with an x-tile plan has no items: its SpMV is a
+//                    launch of its own (gb_b_spmv_xt, one partial per row
+//                    block) after the list's.
 //   K2 gb_b_reduce   (system): alpha, or the refreshed residual norm and the
 //                    convergence test
 //   K3 gb_b_update   (system, chunk of GB_BCHUNK rows): x/r update and r.r
@@ -40,6 +43,7 @@
 #include <string.h>
 
 #include "gb_matrix.h"
+#include "gb_xtile.h"
 
 #define GB_BCHUNK 2048           // rows per elementwise work item
 #define GB_BCHUNK_ITEMS (GB_BCHUNK / GB_BLOCK)
@@ -83,6 +87,12 @@ struct gb_bsys {
   double *part3;    // nchunks partials of ||b - A x||^2 (split matrices)
   gbd_bstate *st;
   const double *jscale;  // MIXED_JACOBI: D^-1/2, else null
+  // an x-tile plan (xt_ptr non-null): the layout, its FP64 values and the
+  // FP32 values of the inner solve (plain or Jacobi-scaled)
+  const uint32_t *xt_ptr, *xt_idx;
+  const double *xt_val;
+  const float *xt_val32;
+  uint32_t xt_R, xt_W, xt_T, xt_lc;
   int32_t precision;
   int32_t pad;
 };
@@ -114,6 +124,9 @@ struct gbd_batch {
   bool banded;                         // the SpMV build for banded batches
   uint32_t cap_seg[GB_SPLIT_MAX];      // their grids
   uint32_t *h_slice_nwork;             // host: max_slots x GB_SPLIT_MAX
+  uint32_t *h_xt;                      // host: the active x-tile systems' slots
+  uint32_t *h_xt_grid;                 // and their launch grids
+  uint32_t n_xt;
   gb_bplan *d_slices;                  // max_slots x GB_SPLIT_MAX slice plans
   uint32_t *h_spmv, *h_elem, *h_sysl;  // pinned staging
   double delta;
@@ -379,6 +392,28 @@ gb_b_spmv(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ items,
   }
 }
 
+// The SpMV of a system on an x-tile plan (grid = its row blocks): Ap and
+// the p.Ap partials of a CG step in the system's precision, or in a
+// refresh step the FP64 residual b - A x and its squared-norm partials;
+// one partial per row block
+__global__ void __launch_bounds__(GB_XT_THREADS)
+gb_b_spmv_xt(const gb_bsys *__restrict__ sys, uint32_t slot,
+             const gbd_bstate *__restrict__ states) {
+  __shared__ gb_xt_lds L;
+  const gb_bsys &S = sys[slot];
+  const gbd_bstate *st = states + slot;
+  if (st->done) return;  // not written during this kernel: uniform
+  if (st->mode_next == GB_MODE_REFRESH)
+    gb_xt_spmv<double, GB_XT_RESID>(S.xt_ptr, S.xt_idx, S.xt_val, S.x, S.r, 1, S.b, S.part1,
+                                    S.n, S.n, S.xt_R, S.xt_T, S.xt_lc, L);
+  else if (S.precision != GOOSEBERRY_PRECISION_FP64)
+    gb_xt_spmv<float, GB_XT_DOT>(S.xt_ptr, S.xt_idx, S.xt_val32, S.p32, S.ap32, 1, nullptr,
+                                 S.part1, S.n, S.n, S.xt_R, S.xt_T, S.xt_lc, L);
+  else
+    gb_xt_spmv<double, GB_XT_DOT>(S.xt_ptr, S.xt_idx, S.xt_val, S.p, S.ap, 1, nullptr,
+                                  S.part1, S.n, S.n, S.xt_R, S.xt_T, S.xt_lc, L);
+}
+
 __global__ void __launch_bounds__(GB_BRED)
 gb_b_reduce(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ syslist,
             const gb_blists *__restrict__ lists) {
@@ -395,8 +430,8 @@ gb_b_reduce(const gb_bsys *__restrict__ sys, const uint32_t *__restrict__ syslis
       if (threadIdx.x == 0) st->mode = mode;
       continue;
     }
-    // one partial per SpMV work item
-    const uint32_t np = S.nsplit ? S.nwork_split : S.pl.nwork;
+    // one partial per SpMV work item (per row block on an x-tile plan)
+    const uint32_t np = S.xt_ptr ? S.xt_W : S.nsplit ? S.nwork_split : S.pl.nwork;
     const double *part = S.part1;
     double s = 0.0;
     for (uint32_t base = 0; base < np; base += GB_BRED * 8) {
@@ -767,6 +802,10 @@ static int gb_b_enqueue_raw(gbd_batch *B, uint32_t k) {
             B->d_sys, B->d_spmv, B->d_desc, B->d_lists, g, B->d_st, B->d_slices);
       GB_LAUNCH_CHECK();
     }
+    for (uint32_t j = 0; j < B->n_xt; ++j) {
+      gb_b_spmv_xt<<<B->h_xt_grid[j], GB_XT_THREADS, 0, s>>>(B->d_sys, B->h_xt[j], B->d_st);
+      GB_LAUNCH_CHECK();
+    }
     gb_b_reduce<<<GB_BGRID_SYS, GB_BRED, 0, s>>>(B->d_sys, B->d_sysl, B->d_lists);
     GB_LAUNCH_CHECK();
     gb_b_update<<<B->cap_elem, GB_BLOCK, 0, s>>>(B->d_sys, B->d_elem, B->d_lists);
@@ -795,7 +834,8 @@ static uint32_t gb_b_nwork_split(const gbd_matrix *m) {
 
 static gb_b_layout gb_b_plan_layout(const gbd_matrix *m, int precision) {
   const size_t n = m->nrows ? m->nrows : 1;
-  const size_t nwork = m->nsplit ? gb_b_nwork_split(m) : gb_spmv_nwork(m);
+  size_t nwork = m->nsplit ? gb_b_nwork_split(m) : gb_spmv_nwork(m);
+  if (m->xt_ptr && m->xt_W > nwork) nwork = m->xt_W;
   const size_t nch = gb_b_nchunks(m->nrows);
   const size_t np1 = (nwork > nch ? nwork : nch) + 1;
   const size_t np2 = nch + 1;
@@ -868,6 +908,8 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
   B->slot_mem = (void **)calloc(max_slots, sizeof(void *));
   B->slot_mat = (gbd_matrix **)calloc(max_slots, sizeof(gbd_matrix *));
   B->h_slice_nwork = (uint32_t *)calloc((size_t)max_slots * GB_SPLIT_MAX, sizeof(uint32_t));
+  B->h_xt = (uint32_t *)calloc(max_slots, sizeof(uint32_t));
+  B->h_xt_grid = (uint32_t *)calloc(max_slots, sizeof(uint32_t));
   B->h_sys = (gb_bsys *)calloc(max_slots, sizeof(gb_bsys));
   hipError_t e = gb_dev_alloc((void **)&B->d_sys, max_slots * sizeof(gb_bsys));
   if (e == hipSuccess) e = gb_dev_alloc((void **)&B->d_st, max_slots * sizeof(gbd_bstate));
@@ -887,7 +929,7 @@ int gbd_batch_create(uint32_t max_slots, double delta, gbd_batch **out) {
   if (e == hipSuccess)
     e = hipMemsetAsync(B->d_sys, 0, max_slots * sizeof(gb_bsys), gb_stream());
   if (e != hipSuccess || !B->used || !B->slot_mem || !B->slot_mat || !B->h_sys ||
-      !B->h_slice_nwork) {
+      !B->h_slice_nwork || !B->h_xt || !B->h_xt_grid) {
     gbd_batch_destroy(B);
     return e != hipSuccess ? gb_hip_fail(e, "gbd_batch_create", __FILE__, __LINE__)
                            : GOOSEBERRY_ERROR_NOMEMORY;
@@ -922,6 +964,8 @@ void gbd_batch_destroy(gbd_batch *B) {
   free(B->slot_mem);
   free(B->slot_mat);
   free(B->h_slice_nwork);
+  free(B->h_xt);
+  free(B->h_xt_grid);
   free(B->h_sys);
   free(B);
 }
@@ -943,6 +987,10 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   if (precision == GOOSEBERRY_PRECISION_MIXED && !m->val32)
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   if (jac && (!m->val32j || !m->jscale || (m->nsplit && !m->sval32j)))
+    return GOOSEBERRY_ERROR_INVALIDINPUT;
+  // an x-tile plan: the layout's copy of the inner solve's FP32 values
+  if (m->xt_ptr && precision != GOOSEBERRY_PRECISION_FP64 &&
+      !(jac ? m->xt_val32j : m->xt_val32))
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   uint32_t slot = 0;
   while (slot < B->max_slots && B->used[slot]) ++slot;
@@ -977,6 +1025,16 @@ int gbd_batch_add(gbd_batch *B, const gbd_matrix *m, const double *d_b,
   }
   S.st = B->d_st + slot;
   S.precision = precision;
+  if (m->xt_ptr) {
+    S.xt_ptr = m->xt_ptr;
+    S.xt_idx = m->xt_idx;
+    S.xt_val = m->xt_val;
+    S.xt_val32 = jac ? m->xt_val32j : m->xt_val32;
+    S.xt_R = m->xt_R;
+    S.xt_W = m->xt_W;
+    S.xt_T = m->xt_T;
+    S.xt_lc = m->xt_lc;
+  }
   hipStream_t s = gb_stream();
   gb_bplan *hsl = nullptr;
   if (m->nsplit) {
@@ -1046,7 +1104,8 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   uint32_t nsys = 0;
   for (uint32_t i = 0; i < B->max_slots; ++i) {
     if (!B->used[i] || (active && !active[i])) continue;
-    ns += B->h_sys[i].nsplit ? B->h_sys[i].nwork_split : B->h_sys[i].pl.nwork;
+    if (!B->h_sys[i].xt_ptr)
+      ns += B->h_sys[i].nsplit ? B->h_sys[i].nwork_split : B->h_sys[i].pl.nwork;
     ne += B->h_sys[i].nchunks;
     ++nsys;
   }
@@ -1083,6 +1142,7 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   for (uint32_t i = 0; i < B->max_slots; ++i) {
     if (!B->used[i] || (active && !active[i])) continue;
     const gb_bsys &S = B->h_sys[i];
+    if (S.xt_ptr) continue;  // launched on its own
     banded = banded && !S.nsplit && S.pl.dcol;
     if (S.nsplit) {
       if (S.nsplit > nseg) nseg = S.nsplit;
@@ -1112,7 +1172,8 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
       if (!B->used[i] || (active && !active[i])) continue;
       const gb_bsys &S = B->h_sys[i];
       uint32_t nw = 0;
-      if (S.nsplit) nw = g < S.nsplit ? B->h_slice_nwork[(size_t)i * GB_SPLIT_MAX + g] : 0;
+      if (S.xt_ptr) nw = 0;
+      else if (S.nsplit) nw = g < S.nsplit ? B->h_slice_nwork[(size_t)i * GB_SPLIT_MAX + g] : 0;
       else if (g == 0) nw = S.pl.nwork;
       for (uint32_t w = 0; w < nw; ++w) {
         B->h_spmv[2 * a] = i;
@@ -1123,6 +1184,21 @@ int gbd_batch_commit(gbd_batch *B, const uint8_t *active) {
   }
   B->h_lists->seg[nseg] = (uint32_t)a;
   B->h_lists->nseg = nseg;
+  // the x-tile systems' launches (the graphs hold them)
+  uint32_t nx = 0;
+  bool xchanged = false;
+  for (uint32_t i = 0; i < B->max_slots; ++i) {
+    if (!B->used[i] || (active && !active[i]) || !B->h_sys[i].xt_ptr) continue;
+    xchanged = xchanged || nx >= B->n_xt || B->h_xt[nx] != i ||
+               B->h_xt_grid[nx] != B->h_sys[i].xt_W;
+    B->h_xt[nx] = i;
+    B->h_xt_grid[nx] = B->h_sys[i].xt_W;
+    ++nx;
+  }
+  if (xchanged || nx != B->n_xt) {
+    B->n_xt = nx;
+    gb_b_drop_graphs(B);
+  }
   for (uint32_t i = 0; i < B->max_slots; ++i) {
     if (!B->used[i] || (active && !active[i])) continue;
     for (uint32_t k = 0; k < B->h_sys[i].nchunks; ++k) {

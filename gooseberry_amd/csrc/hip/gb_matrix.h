@@ -49,6 +49,8 @@ struct gbd_matrix {
   uint32_t *xt_ptr;     // xt_W xt_T + 1
   uint32_t *xt_idx;     // nnz
   double *xt_val;       // nnz
+  float *xt_val32;      // nnz, with val32: its values in the layout's order
+  float *xt_val32j;     // nnz, with val32j: its values in the layout's order
   uint32_t xt_R, xt_W, xt_T, xt_lc;
   uint64_t hist[33];
   // column split (params.csplit = P > 1): P views, view r holding the

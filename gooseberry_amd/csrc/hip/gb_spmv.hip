@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "gb_matrix.h"
+#include "gb_xtile.h"
 
 // ------------------------------------------------------------- plan build
 
@@ -577,16 +578,8 @@ gb_spmv_pipe(const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ r
 
 // ------------------------------------------------------------- x tiles
 // For matrices whose columns are scattered over an x larger than an XCD's
-// L2, every gather is an L2 request.  With x tiles the entries are grouped
-// by (block of R rows, tile of C columns), each group in CSR order.  One
-// workgroup owns a row block and keeps its y in LDS; tile by tile it copies
-// x's tile into LDS and adds its group's products, so the gathers read LDS
-// and the matrix and x stream.  Each row gets one sum per tile, added to
-// its y in tile order: a fixed regrouping of the row's sum (by tile, as
-// with a column split), bitwise reproducible.
-#define GB_XT_THREADS 1024
-#define GB_XT_RMAX 8192  // rows of y a workgroup keeps in LDS
-#define GB_XT_DEPTH 2    // chunks of 64 entries a wavefront has in flight
+// L2, every gather is an L2 request; the x-tile layout (gb_xtile.h) moves
+// the gathers into LDS.
 
 // the layout's keys (group of each entry) and packed (row, column) words
 __global__ void __launch_bounds__(GB_BLOCK)
@@ -615,223 +608,17 @@ __global__ void gb_xt_gather(const uint32_t *__restrict__ perm, uint64_t nnz,
   }
 }
 
-// Cross-lane moves for the segmented scan, as DPP (data-parallel
-// primitives: a lane reads another lane's register within the instruction,
-// with no LDS round trip), with bound_ctrl set: a lane whose source lies
-// outside the pattern reads 0, so no register needs a fallback value first.
-template <int CTRL>
-__device__ __forceinline__ uint32_t gb_dpp_u32(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
-}
-template <int CTRL>
-__device__ __forceinline__ double gb_dpp_f64(double v) {
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = gb_dpp_u32<CTRL>((uint32_t)b);
-  const uint32_t hi = gb_dpp_u32<CTRL>((uint32_t)(b >> 32));
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-// one step of the segmented inclusive scan: a lane taking part in the step
-// (`on`) adds the partial sum of its source lane when that lane holds the
-// same row (rows of a group are contiguous, so the lanes between hold it
-// too); the earlier part is the left operand.  A lane whose source lies
-// outside the pattern reads row 0 and +0.0: adding +0.0 changes no partial
-// sum except -0.0, which becomes +0.0, and y starts from +0.0, so every
-// result is the same either way.
-template <int CTRL>
-__device__ __forceinline__ void gb_seg_step(double &sc, uint32_t r, bool on) {
-  const double ps = gb_dpp_f64<CTRL>(sc);
-  const uint32_t pr = gb_dpp_u32<CTRL>(r);
-  if (on && pr == r) sc = ps + sc;
-}
-// the scan over 64 lanes: within each row of 16 lanes by shifts of 1, 2,
-// 4 and 8 lanes, then the last lane of row 0 into row 1 and of row 2 into
-// row 3, then the last lane of row 1 into rows 2 and 3
-__device__ __forceinline__ double gb_seg_scan64(double sc, uint32_t r, uint32_t lane) {
-  gb_seg_step<0x111>(sc, r, true);               // row_shr:1
-  gb_seg_step<0x112>(sc, r, true);               // row_shr:2
-  gb_seg_step<0x114>(sc, r, true);               // row_shr:4
-  gb_seg_step<0x118>(sc, r, true);               // row_shr:8
-  gb_seg_step<0x142>(sc, r, (lane & 16u) != 0);  // row_bcast:15, rows 1 and 3
-  gb_seg_step<0x143>(sc, r, lane >= 32);         // row_bcast:31, rows 2 and 3
-  return sc;
-}
-__device__ __forceinline__ double gb_readlane_f64(double v, uint32_t l) {
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
-  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
-// y = A x on the x-tile layout: one workgroup per row block (grid xt_W).
-// A group's entries are cut into one contiguous range per wavefront; a
-// wavefront reads its range 64 entries at a time (coalesced, GB_XT_DEPTH
-// chunks in flight; the next group's tile of x and first chunks are loaded
-// when this group starts) and adds each row's products with a segmented
-// scan across its lanes (rows are contiguous within a group, so equal rows
-// are one run).  The run that reaches a chunk's end is carried into the
-// next chunk; a run cut by the range's end is finished by the wavefront
-// where it starts, which adds the following ranges' parts in order.
+// y = A x on the x-tile layout (V = double, or float for the FP32 SpMV);
+// with `partials` (FP64, CG) the block's share of x.Ax as well
+template <typename V, int EPI>
 __global__ void __launch_bounds__(GB_XT_THREADS)
 gb_spmv_xtile(const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ idx,
-              const double *__restrict__ val, const double *__restrict__ x,
-              double *__restrict__ y, uint32_t ystride, uint32_t nrows, uint32_t ncols,
-              uint32_t R, uint32_t T, uint32_t lc) {
-  constexpr uint32_t NW = GB_XT_THREADS / GB_WAVE;
-  constexpr uint32_t XPT = GB_XT_CMAX / GB_XT_THREADS;  // x values a thread stages
-  constexpr uint32_t NONE = 0xffffffffu;
-  __shared__ double xs[GB_XT_CMAX];
-  __shared__ double ys[GB_XT_RMAX];
-  __shared__ double wcarry[NW];  // a range's first run, cut off from the one before
-  __shared__ double whead[NW];   // a range's last run, started in the range
-  __shared__ uint32_t wfrow[NW]; // first run's row | cut (bit 31) | the only run (bit 30)
-  __shared__ uint32_t wlrow[NW]; // last run's row | started here (bit 31)
-  const uint32_t tid = threadIdx.x, lane = tid & (GB_WAVE - 1), wv = tid / GB_WAVE;
-  const uint32_t r0 = blockIdx.x * R;
-  const uint32_t nr = min(R, nrows - r0);
-  const uint32_t C = 1u << lc;
-  for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) ys[i] = 0.0;
-  const uint32_t *gp = ptr + (size_t)blockIdx.x * T;
-  double xr[XPT];             // the next group's tile of x
-  uint32_t rw[GB_XT_DEPTH];   // chunks in flight: packed words and values
-  double rv[GB_XT_DEPTH];
-  uint32_t rw2[GB_XT_DEPTH];  // the next group's first chunks
-  double rv2[GB_XT_DEPTH];
-  uint32_t wprev = 0, wprev2 = 0;  // the entry before the range (is its first
-                                   // run cut?)
-  // this wavefront's range of group tt
-  auto range = [&](uint32_t tt, uint32_t &ra, uint32_t &rz) {
-    const uint32_t gs = gp[tt], ge = gp[tt + 1];
-    const uint32_t cnt = ge - gs;
-    const uint32_t ew = (cnt + NW - 1) / NW;
-    ra = gs + min(cnt, wv * ew);
-    rz = min(ra + ew, ge);
-  };
-  // group tt's tile of x into registers, and its first chunks.  Every
-  // load is unconditional, its index clamped into the array (the extra
-  // values are never used), so no load sits under a branch and the waits
-  // for them stay exact.
-  auto prefetch = [&](uint32_t tt, uint32_t (&pw)[GB_XT_DEPTH], double (&pv)[GB_XT_DEPTH],
-                      uint32_t &pprev) {
-    const uint32_t c0 = tt << lc;
-    const uint32_t cl = min(C, ncols - c0) - 1;
-#pragma unroll
-    for (uint32_t i = 0; i < XPT; ++i) xr[i] = x[c0 + min(i * GB_XT_THREADS + tid, cl)];
-    uint32_t ra, rz;
-    range(tt, ra, rz);
-    const uint32_t kl = rz > ra ? rz - 1 : ra > 0 ? ra - 1 : 0;
-    pprev = idx[ra > 0 ? ra - 1 : 0];
-#pragma unroll
-    for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
-      const uint32_t k = min(ra + q * GB_WAVE + lane, kl);
-      pw[q] = __builtin_nontemporal_load(&idx[k]);
-      pv[q] = __builtin_nontemporal_load(&val[k]);
-    }
-  };
-  uint32_t t = 0;
-  while (t < T && gp[t] == gp[t + 1]) ++t;
-  if (t < T) prefetch(t, rw, rv, wprev);
-  while (t < T) {
-    {
-      const uint32_t cw = min(C, ncols - (t << lc));
-#pragma unroll
-      for (uint32_t i = 0; i < XPT; ++i) {
-        const uint32_t c = i * GB_XT_THREADS + tid;
-        if (c < cw) xs[c] = xr[i];
-      }
-    }
-    __syncthreads();
-    // the next group's x and first chunks go out now: they have this
-    // group's work to arrive
-    uint32_t tn = t + 1;
-    while (tn < T && gp[tn] == gp[tn + 1]) ++tn;
-    if (tn < T) prefetch(tn, rw2, rv2, wprev2);
-    const uint32_t s = gp[t];
-    uint32_t a, z;
-    range(t, a, z);
-    uint32_t fw = 0, lw = 0;
-    if (a < z) {  // the same for the whole wavefront
-      const uint32_t rfirst = (uint32_t)__builtin_amdgcn_readfirstlane((int)rw[0]) >> lc;
-      const bool cut = a > s && (wprev >> lc) == rfirst;
-      uint32_t crow = NONE;  // the run carried from the previous chunk
-      double csum = 0.0;
-      for (uint32_t kb = a; kb < z; kb += GB_XT_DEPTH * GB_WAVE) {
-#pragma unroll
-        for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
-          const uint32_t k0 = kb + q * GB_WAVE;
-          if (k0 < z) {
-            const uint32_t w = rw[q];
-            const double v = rv[q];
-            const uint32_t nvalid = min((uint32_t)GB_WAVE, z - k0);
-            const bool valid = lane < nvalid;
-            // invalid lanes get rows no valid lane has (valid rows are < 2^13)
-            const uint32_t r = valid ? (w >> lc) : (0x80000000u | lane);
-            const double sc = gb_seg_scan64(valid ? v * xs[w & (C - 1)] : 0.0, r, lane);
-            const uint32_t rl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)r);
-            if (crow != NONE && crow != rl0) {
-              // the carried run ended with the previous chunk
-              if (lane == 0) {
-                if (crow == rfirst && cut) wcarry[wv] = csum;
-                else ys[crow] += csum;
-              }
-              crow = NONE;
-            }
-            // the next lane's row (wave_shl:1; lane 63, always the last
-            // valid lane or past it, reads 0)
-            const uint32_t rn = gb_dpp_u32<0x130>(r);
-            const bool last = lane == nvalid - 1;
-            double tot = sc;
-            if (r == rl0 && crow == rl0) tot = csum + tot;
-            if (valid && !last && rn != r) {
-              if (r == rfirst && cut) wcarry[wv] = tot;
-              else ys[r] += tot;
-            }
-            crow = (uint32_t)__builtin_amdgcn_readlane((int)r, nvalid - 1);
-            csum = gb_readlane_f64(tot, nvalid - 1);
-          }
-          // the chunk GB_XT_DEPTH further on takes this slot, once this
-          // chunk's values are used (the same registers, no copies)
-          const uint32_t kf = min(k0 + GB_XT_DEPTH * GB_WAVE + lane, z - 1);
-          rw[q] = __builtin_nontemporal_load(&idx[kf]);
-          rv[q] = __builtin_nontemporal_load(&val[kf]);
-        }
-      }
-      const bool single = crow == rfirst && cut;
-      if (single && lane == 0) wcarry[wv] = csum;
-      if (!single && lane == 0) whead[wv] = csum;
-      fw = rfirst | (cut ? 0x80000000u : 0u) | (single ? 0x40000000u : 0u);
-      lw = single ? 0u : (crow | 0x80000000u);
-    }
-    if (lane == 0) {
-      wfrow[wv] = fw;
-      wlrow[wv] = lw;
-    }
-    __syncthreads();
-    // a run started in this range and cut by its end takes the next
-    // ranges' parts, in order
-    if (lane == 0 && (lw & 0x80000000u)) {
-      const uint32_t r = lw & 0x7fffffffu;
-      double tot = whead[wv];
-      for (uint32_t q = wv + 1; q < NW; ++q) {
-        const uint32_t f = wfrow[q];
-        if (!(f & 0x80000000u) || (f & 0x3fffffffu) != r) break;
-        tot += wcarry[q];
-        if (!(f & 0x40000000u)) break;
-      }
-      ys[r] += tot;
-    }
-    // no barrier here: every read of xs ended before the barrier above, and
-    // the next group's range words and ys updates come after the next
-    // group's first barrier, which every wavefront reaches after this step
-#pragma unroll
-    for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
-      rw[q] = rw2[q];
-      rv[q] = rv2[q];
-    }
-    wprev = wprev2;
-    t = tn;
-  }
-  __syncthreads();  // the last group's chained runs are in ys
-  for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) y[(size_t)(r0 + i) * ystride] = ys[i];
+              const V *__restrict__ val, const V *__restrict__ x, V *__restrict__ y,
+              uint32_t ystride, double *__restrict__ partials, uint32_t nrows,
+              uint32_t ncols, uint32_t R, uint32_t T, uint32_t lc) {
+  __shared__ gb_xt_lds L;
+  gb_xt_spmv<V, EPI>(ptr, idx, val, x, y, ystride, nullptr, partials, nrows, ncols, R, T,
+                     lc, L);
 }
 
 uint32_t gb_spmv_grid(const gbd_matrix *m) {
@@ -1038,17 +825,23 @@ static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const uint8_t *vi
                  "supported");
     return GOOSEBERRY_ERROR_UNIMPLEMENTED;
   }
-  if constexpr (sizeof(T) == 8) {
-    // x tiles (FP64; the FP32 SpMV runs the plan's row blocks)
-    if (m->xt_ptr) {
+  if (m->xt_ptr) {
+    // x tiles: the FP64 values, or their FP32 copy in the layout's order
+    // (made with the matrix's FP32 copy; CG runs in FP64 only)
+    const T *xv;
+    if constexpr (sizeof(T) == 8) xv = m->xt_val;
+    else xv = cg ? nullptr : m->xt_val32;
+    if (xv) {
       hipStream_t s = gb_stream();
-      gb_spmv_xtile<<<m->xt_W, GB_XT_THREADS, 0, s>>>(m->xt_ptr, m->xt_idx, m->xt_val, x, y,
-                                                      ystride, m->nrows, m->ncols, m->xt_R,
-                                                      m->xt_T, m->xt_lc);
+      gb_spmv_xtile<T, GB_XT_STORE><<<m->xt_W, GB_XT_THREADS, 0, s>>>(
+          m->xt_ptr, m->xt_idx, xv, x, y, ystride, nullptr, m->nrows, m->ncols, m->xt_R,
+          m->xt_T, m->xt_lc);
       GB_LAUNCH_CHECK();
-      if (cg) {
-        gb_dot_partials<<<gb_split_sum_grid(m), GB_BLOCK, 0, s>>>(x, y, m->nrows, partials);
-        GB_LAUNCH_CHECK();
+      if constexpr (sizeof(T) == 8) {
+        if (cg) {
+          gb_dot_partials<<<gb_split_sum_grid(m), GB_BLOCK, 0, s>>>(x, y, m->nrows, partials);
+          GB_LAUNCH_CHECK();
+        }
       }
       return GOOSEBERRY_SUCCESS;
     }
@@ -1358,9 +1151,56 @@ static void gb_xtile_free(gbd_matrix *m) {
   (void)gb_dev_free(m->xt_ptr);
   (void)gb_dev_free(m->xt_idx);
   (void)gb_dev_free(m->xt_val);
+  (void)gb_dev_free(m->xt_val32);
+  (void)gb_dev_free(m->xt_val32j);
   m->xt_ptr = m->xt_idx = nullptr;
   m->xt_val = nullptr;
+  m->xt_val32 = m->xt_val32j = nullptr;
   m->xt_R = m->xt_W = m->xt_T = m->xt_lc = 0;
+}
+
+// The Jacobi-scaled FP32 values in the layout's order: the same
+// (a_ij s_i) s_j rounded once as the CSR-order copy, the row and column
+// taken from the group and the packed word
+__global__ void __launch_bounds__(GB_BLOCK)
+gb_xt_jacobi_vals(const uint32_t *__restrict__ ptr, uint32_t ngroups, uint32_t T, uint32_t R,
+                  uint32_t lc, const uint32_t *__restrict__ word,
+                  const double *__restrict__ val, const double *__restrict__ scale,
+                  float *__restrict__ out) {
+  const uint32_t cm = (1u << lc) - 1;
+  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const uint32_t rb = (g / T) * R, c0 = (g % T) << lc, e = ptr[g + 1];
+    for (uint32_t j = ptr[g] + threadIdx.x; j < e; j += GB_BLOCK) {
+      const uint32_t w = word[j];
+      out[j] = (float)((val[j] * scale[rb + (w >> lc)]) * scale[c0 | (w & cm)]);
+    }
+  }
+}
+
+// The layout's FP32 copies of the values the matrix has in FP32 (val32,
+// val32j), made with them or with the layout, whichever comes second
+static int gb_xtile_f32(gbd_matrix *m) {
+  if (!m->xt_ptr) return GOOSEBERRY_SUCCESS;
+  hipStream_t s = gb_stream();
+  const size_t nz = m->nnz;
+  if (m->val32 && !m->xt_val32) {
+    GB_HIP(gb_dev_alloc((void **)&m->xt_val32, (nz ? nz : 1) * sizeof(float)));
+    if (nz) {
+      gb_to_f32<<<gb_div_up(nz, GB_BLOCK), GB_BLOCK, 0, s>>>(m->xt_val, m->xt_val32, nz);
+      GB_LAUNCH_CHECK();
+    }
+  }
+  if (m->val32j && m->jscale && !m->xt_val32j) {
+    GB_HIP(gb_dev_alloc((void **)&m->xt_val32j, (nz ? nz : 1) * sizeof(float)));
+    const uint32_t ng = m->xt_W * m->xt_T;
+    if (nz && ng) {
+      gb_xt_jacobi_vals<<<ng < 65536u ? ng : 65536u, GB_BLOCK, 0, s>>>(
+          m->xt_ptr, ng, m->xt_T, m->xt_R, m->xt_lc, m->xt_idx, m->xt_val, m->jscale,
+          m->xt_val32j);
+      GB_LAUNCH_CHECK();
+    }
+  }
+  return GOOSEBERRY_SUCCESS;
 }
 
 // The x-tile layout: keys = (row block, column tile) per entry, a stable
@@ -1417,7 +1257,9 @@ static int gb_xtile_build(gbd_matrix *m) {
   m->xt_W = W;
   m->xt_T = T;
   m->xt_lc = lc;
-  return GOOSEBERRY_SUCCESS;
+  rv = gb_xtile_f32(m);
+  if (rv != GOOSEBERRY_SUCCESS) gb_xtile_free(m);
+  return rv;
 }
 
 static int gb_plan_check(const gbd_plan_params *p) {
@@ -2026,6 +1868,8 @@ int gbd_matrix_make_f32(gbd_matrix *m) {
     }
     for (uint32_t r = 0; r < m->nsplit; ++r) m->split[r]->val32 = m->sval32;
   }
+  int rx = gb_xtile_f32(m);
+  if (rx != GOOSEBERRY_SUCCESS) return rx;
   return gb_vdict_f32(m);
 }
 
@@ -2071,7 +1915,7 @@ int gbd_matrix_make_f32_jacobi(gbd_matrix *m) {
     GB_LAUNCH_CHECK();
     for (uint32_t r = 0; r < P; ++r) m->split[r]->val32j = m->sval32j;
   }
-  return GOOSEBERRY_SUCCESS;
+  return gb_xtile_f32(m);
 }
 
 int gbd_spmv(const gbd_matrix *m, const double *d_x, double *d_y) {

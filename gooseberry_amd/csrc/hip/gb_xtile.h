@@ -1,0 +1,300 @@
+// gb_xtile.h -- the SpMV over the x-tile layout, shared by the matrix path
+// (gb_spmv.hip) and the batched engine (gb_batch.hip).
+//
+// The layout groups a matrix's entries by (block of R rows, tile of
+// C = 2^lc columns), each group in CSR order: ptr[g] .. ptr[g + 1] for group
+// g = block * T + tile, each entry a packed word (row within the block << lc
+// | column within the tile) and a value.  One workgroup owns a row block and
+// keeps its y in LDS; tile by tile it copies x's tile into LDS and adds its
+// group's products, so the gathers read LDS and only the matrix and x
+// stream.  Each row gets one sum per tile, added to its y in tile order: a
+// fixed regrouping of the row's sum (by tile, as with a column split),
+// bitwise reproducible.  V is the value, vector and accumulation type
+// (double, or float for the FP32 SpMV of the mixed-precision solves).
+#pragma once
+
+#include "gb_matrix.h"
+
+#define GB_XT_THREADS 1024
+#define GB_XT_RMAX 8192  // rows of y a workgroup keeps in LDS
+#define GB_XT_DEPTH 2    // chunks of 64 entries a wavefront has in flight
+
+// what the workgroup does with its block of y = A x
+enum {
+  GB_XT_STORE = 0,  // y[row * ystride] = (A x)[row]
+  GB_XT_DOT = 1,    // y[row] = (A x)[row]; partials[block] = sum of x[row] y[row]
+                    // (x = p in a CG step: the block's part of p.Ap)
+  GB_XT_RESID = 2,  // V = double: y[row] = b[row] - (A x)[row];
+                    // partials[block] = sum of y[row]^2
+};
+
+// Cross-lane moves for the segmented scan, as DPP (data-parallel
+// primitives: a lane reads another lane's register within the instruction,
+// with no LDS round trip), with bound_ctrl set: a lane whose source lies
+// outside the pattern reads 0, so no register needs a fallback value first.
+template <int CTRL>
+__device__ __forceinline__ uint32_t gb_dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ __forceinline__ double gb_dpp_val(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = gb_dpp_u32<CTRL>((uint32_t)b);
+  const uint32_t hi = gb_dpp_u32<CTRL>((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int CTRL>
+__device__ __forceinline__ float gb_dpp_val(float v) {
+  return __uint_as_float(gb_dpp_u32<CTRL>(__float_as_uint(v)));
+}
+// one step of the segmented inclusive scan: a lane taking part in the step
+// (`on`) adds the partial sum of its source lane when that lane holds the
+// same row (rows of a group are contiguous, so the lanes between hold it
+// too); the earlier part is the left operand.  A lane whose source lies
+// outside the pattern reads row 0 and +0.0: adding +0.0 changes no partial
+// sum except -0.0, which becomes +0.0, and y starts from +0.0, so every
+// result is the same either way.
+template <int CTRL, typename V>
+__device__ __forceinline__ void gb_seg_step(V &sc, uint32_t r, bool on) {
+  const V ps = gb_dpp_val<CTRL>(sc);
+  const uint32_t pr = gb_dpp_u32<CTRL>(r);
+  if (on && pr == r) sc = ps + sc;
+}
+// the scan over 64 lanes: within each row of 16 lanes by shifts of 1, 2,
+// 4 and 8 lanes, then the last lane of row 0 into row 1 and of row 2 into
+// row 3, then the last lane of row 1 into rows 2 and 3
+template <typename V>
+__device__ __forceinline__ V gb_seg_scan64(V sc, uint32_t r, uint32_t lane) {
+  gb_seg_step<0x111>(sc, r, true);               // row_shr:1
+  gb_seg_step<0x112>(sc, r, true);               // row_shr:2
+  gb_seg_step<0x114>(sc, r, true);               // row_shr:4
+  gb_seg_step<0x118>(sc, r, true);               // row_shr:8
+  gb_seg_step<0x142>(sc, r, (lane & 16u) != 0);  // row_bcast:15, rows 1 and 3
+  gb_seg_step<0x143>(sc, r, lane >= 32);         // row_bcast:31, rows 2 and 3
+  return sc;
+}
+__device__ __forceinline__ double gb_readlane_val(double v, uint32_t l) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ float gb_readlane_val(float v, uint32_t l) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), l));
+}
+
+// The LDS a workgroup of the x-tile SpMV uses, declared once by each
+// kernel and viewed as V (the FP32 SpMV uses the first halves)
+struct gb_xt_lds {
+  double xs[GB_XT_CMAX];  // the current tile of x
+  double ys[GB_XT_RMAX];  // the block's y
+  double wcarry[GB_XT_THREADS / GB_WAVE];  // a range's first run, cut off
+                                           // from the one before
+  double whead[GB_XT_THREADS / GB_WAVE];   // a range's last run, started in it
+  uint32_t wfrow[GB_XT_THREADS / GB_WAVE]; // first run's row | cut (bit 31) |
+                                           // the only run (bit 30)
+  uint32_t wlrow[GB_XT_THREADS / GB_WAVE]; // last run's row | started here (bit 31)
+  double red[GB_XT_THREADS / GB_WAVE];     // the block's partial
+};
+
+// y = A x on the x-tile layout for the block blockIdx.x (grid = the row
+// blocks).  A group's entries are cut into one contiguous range per
+// wavefront; a wavefront reads its range 64 entries at a time (coalesced,
+// GB_XT_DEPTH chunks in flight; the next group's tile of x and first chunks
+// are loaded when this group starts) and adds each row's products with a
+// segmented scan across its lanes (rows are contiguous within a group, so
+// equal rows are one run).  The run that reaches a chunk's end is carried
+// into the next chunk; a run cut by the range's end is finished by the
+// wavefront where it starts, which adds the following ranges' parts in
+// order.
+template <typename V, int EPI>
+__device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
+                                           const uint32_t *__restrict__ idx,
+                                           const V *__restrict__ val, const V *__restrict__ x,
+                                           V *__restrict__ y, uint32_t ystride,
+                                           const double *__restrict__ b,
+                                           double *__restrict__ partials, uint32_t nrows,
+                                           uint32_t ncols, uint32_t R, uint32_t T, uint32_t lc,
+                                           gb_xt_lds &L) {
+  constexpr uint32_t NW = GB_XT_THREADS / GB_WAVE;
+  constexpr uint32_t XPT = GB_XT_CMAX / GB_XT_THREADS;  // x values a thread stages
+  constexpr uint32_t NONE = 0xffffffffu;
+  V *xs = (V *)L.xs;
+  V *ys = (V *)L.ys;
+  V *wcarry = (V *)L.wcarry;
+  V *whead = (V *)L.whead;
+  uint32_t *wfrow = L.wfrow, *wlrow = L.wlrow;
+  const uint32_t tid = threadIdx.x, lane = tid & (GB_WAVE - 1), wv = tid / GB_WAVE;
+  const uint32_t r0 = blockIdx.x * R;
+  const uint32_t nr = min(R, nrows - r0);
+  const uint32_t C = 1u << lc;
+  for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) ys[i] = (V)0;
+  const uint32_t *gp = ptr + (size_t)blockIdx.x * T;
+  V xr[XPT];                  // the next group's tile of x
+  uint32_t rw[GB_XT_DEPTH];   // chunks in flight: packed words and values
+  V rv[GB_XT_DEPTH];
+  uint32_t rw2[GB_XT_DEPTH];  // the next group's first chunks
+  V rv2[GB_XT_DEPTH];
+  uint32_t wprev = 0, wprev2 = 0;  // the entry before the range (is its first
+                                   // run cut?)
+  // this wavefront's range of group tt
+  auto range = [&](uint32_t tt, uint32_t &ra, uint32_t &rz) {
+    const uint32_t gs = gp[tt], ge = gp[tt + 1];
+    const uint32_t cnt = ge - gs;
+    const uint32_t ew = (cnt + NW - 1) / NW;
+    ra = gs + min(cnt, wv * ew);
+    rz = min(ra + ew, ge);
+  };
+  // group tt's tile of x into registers, and its first chunks.  Every
+  // load is unconditional, its index clamped into the array (the extra
+  // values are never used), so no load sits under a branch and the waits
+  // for them stay exact.
+  auto prefetch = [&](uint32_t tt, uint32_t (&pw)[GB_XT_DEPTH], V (&pv)[GB_XT_DEPTH],
+                      uint32_t &pprev) {
+    const uint32_t c0 = tt << lc;
+    const uint32_t cl = min(C, ncols - c0) - 1;
+#pragma unroll
+    for (uint32_t i = 0; i < XPT; ++i) xr[i] = x[c0 + min(i * GB_XT_THREADS + tid, cl)];
+    uint32_t ra, rz;
+    range(tt, ra, rz);
+    const uint32_t kl = rz > ra ? rz - 1 : ra > 0 ? ra - 1 : 0;
+    pprev = idx[ra > 0 ? ra - 1 : 0];
+#pragma unroll
+    for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
+      const uint32_t k = min(ra + q * GB_WAVE + lane, kl);
+      pw[q] = __builtin_nontemporal_load(&idx[k]);
+      pv[q] = __builtin_nontemporal_load(&val[k]);
+    }
+  };
+  uint32_t t = 0;
+  while (t < T && gp[t] == gp[t + 1]) ++t;
+  if (t < T) prefetch(t, rw, rv, wprev);
+  while (t < T) {
+    {
+      const uint32_t cw = min(C, ncols - (t << lc));
+#pragma unroll
+      for (uint32_t i = 0; i < XPT; ++i) {
+        const uint32_t c = i * GB_XT_THREADS + tid;
+        if (c < cw) xs[c] = xr[i];
+      }
+    }
+    __syncthreads();
+    // the next group's x and first chunks go out now: they have this
+    // group's work to arrive
+    uint32_t tn = t + 1;
+    while (tn < T && gp[tn] == gp[tn + 1]) ++tn;
+    if (tn < T) prefetch(tn, rw2, rv2, wprev2);
+    const uint32_t s = gp[t];
+    uint32_t a, z;
+    range(t, a, z);
+    uint32_t fw = 0, lw = 0;
+    if (a < z) {  // the same for the whole wavefront
+      const uint32_t rfirst = (uint32_t)__builtin_amdgcn_readfirstlane((int)rw[0]) >> lc;
+      const bool cut = a > s && (wprev >> lc) == rfirst;
+      uint32_t crow = NONE;  // the run carried from the previous chunk
+      V csum = (V)0;
+      for (uint32_t kb = a; kb < z; kb += GB_XT_DEPTH * GB_WAVE) {
+#pragma unroll
+        for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
+          const uint32_t k0 = kb + q * GB_WAVE;
+          if (k0 < z) {
+            const uint32_t w = rw[q];
+            const V v = rv[q];
+            const uint32_t nvalid = min((uint32_t)GB_WAVE, z - k0);
+            const bool valid = lane < nvalid;
+            // invalid lanes get rows no valid lane has (valid rows are < 2^13)
+            const uint32_t r = valid ? (w >> lc) : (0x80000000u | lane);
+            const V sc = gb_seg_scan64<V>(valid ? v * xs[w & (C - 1)] : (V)0, r, lane);
+            const uint32_t rl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)r);
+            if (crow != NONE && crow != rl0) {
+              // the carried run ended with the previous chunk
+              if (lane == 0) {
+                if (crow == rfirst && cut) wcarry[wv] = csum;
+                else ys[crow] += csum;
+              }
+              crow = NONE;
+            }
+            // the next lane's row (wave_shl:1; lane 63, always the last
+            // valid lane or past it, reads 0)
+            const uint32_t rn = gb_dpp_u32<0x130>(r);
+            const bool last = lane == nvalid - 1;
+            V tot = sc;
+            if (r == rl0 && crow == rl0) tot = csum + tot;
+            if (valid && !last && rn != r) {
+              if (r == rfirst && cut) wcarry[wv] = tot;
+              else ys[r] += tot;
+            }
+            crow = (uint32_t)__builtin_amdgcn_readlane((int)r, nvalid - 1);
+            csum = gb_readlane_val(tot, nvalid - 1);
+          }
+          // the chunk GB_XT_DEPTH further on takes this slot, once this
+          // chunk's values are used (the same registers, no copies)
+          const uint32_t kf = min(k0 + GB_XT_DEPTH * GB_WAVE + lane, z - 1);
+          rw[q] = __builtin_nontemporal_load(&idx[kf]);
+          rv[q] = __builtin_nontemporal_load(&val[kf]);
+        }
+      }
+      const bool single = crow == rfirst && cut;
+      if (single && lane == 0) wcarry[wv] = csum;
+      if (!single && lane == 0) whead[wv] = csum;
+      fw = rfirst | (cut ? 0x80000000u : 0u) | (single ? 0x40000000u : 0u);
+      lw = single ? 0u : (crow | 0x80000000u);
+    }
+    if (lane == 0) {
+      wfrow[wv] = fw;
+      wlrow[wv] = lw;
+    }
+    __syncthreads();
+    // a run started in this range and cut by its end takes the next
+    // ranges' parts, in order
+    if (lane == 0 && (lw & 0x80000000u)) {
+      const uint32_t r = lw & 0x7fffffffu;
+      V tot = whead[wv];
+      for (uint32_t q = wv + 1; q < NW; ++q) {
+        const uint32_t f = wfrow[q];
+        if (!(f & 0x80000000u) || (f & 0x3fffffffu) != r) break;
+        tot += wcarry[q];
+        if (!(f & 0x40000000u)) break;
+      }
+      ys[r] += tot;
+    }
+    // no barrier here: every read of xs ended before the barrier above, and
+    // the next group's range words and ys updates come after the next
+    // group's first barrier, which every wavefront reaches after this step
+#pragma unroll
+    for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
+      rw[q] = rw2[q];
+      rv[q] = rv2[q];
+    }
+    wprev = wprev2;
+    t = tn;
+  }
+  __syncthreads();  // the last group's chained runs are in ys
+  if constexpr (EPI == GB_XT_STORE) {
+    for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) y[(size_t)(r0 + i) * ystride] = ys[i];
+  } else {
+    // the block's partial: each thread's rows in order, then the
+    // wavefronts' sums in order
+    double d = 0.0;
+    for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) {
+      const V yi = ys[i];
+      if constexpr (EPI == GB_XT_DOT) {
+        y[r0 + i] = yi;
+        d += (double)x[r0 + i] * (double)yi;
+      } else {
+        const double o = b[r0 + i] - (double)yi;
+        ((double *)y)[r0 + i] = o;
+        d += o * o;
+      }
+    }
+    d = gb_wave_sum(d);
+    if (lane == 0) L.red[wv] = d;
+    __syncthreads();
+    if (tid == 0) {
+      double s = L.red[0];
+#pragma unroll
+      for (uint32_t q = 1; q < NW; ++q) s += L.red[q];
+      partials[blockIdx.x] = s;
+    }
+  }
+}

@@ -100,8 +100,7 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
       (rv = gbd_event_create(&c.e1)) != GOOSEBERRY_SUCCESS)
     goto done;
   /* candidates start from the current plan without x tiles (a column
-   * split and x tiles exclude each other; the FP32 SpMV does not read the
-   * x-tile layout) */
+   * split and x tiles exclude each other) */
   cur.xtile = 0;
   gbd_plan_params best = cur;
   double best_ms = DBL_MAX, ms;
@@ -162,9 +161,9 @@ int gooseberry_matrix_autotune(gooseberry_matrix_t mat, int precision,
       if ((rv = time_plan(&c, &p, &ms)) != GOOSEBERRY_SUCCESS) goto done;
       if (ms < GB_TUNE_GAIN * best_ms) { best_ms = ms; best = p; }
     }
-    /* 5. x through LDS in column tiles (the FP64 SpMV only) */
+    /* 5. x through LDS in column tiles */
     static const uint32_t xtiles[] = {4096, 8192};
-    for (size_t i = 0; precision == GOOSEBERRY_PRECISION_FP64 && i < 2; ++i) {
+    for (size_t i = 0; i < 2; ++i) {
       gbd_plan_params p = base;
       p.csplit = 0;
       p.xtile = xtiles[i];
