@@ -120,7 +120,8 @@ def cg_iter_bytes(n, nnz):
 # the sources the SpMV kernel is built from: a counter summary counts only
 # for the build it was measured on
 SPMV_SOURCES = ["gooseberry_amd/csrc/hip/gb_spmv.hip", "gooseberry_amd/csrc/hip/gb_matrix.h",
-                "gooseberry_amd/csrc/hip/gb_common.h", "gooseberry_amd/build.py"]
+                "gooseberry_amd/csrc/hip/gb_common.h", "gooseberry_amd/csrc/hip/gb_xtile.h",
+                "gooseberry_amd/build.py"]
 
 
 def spmv_source_hash():

@@ -28,9 +28,12 @@ def rel_res(gb, M, db, dx):
     return float(np.linalg.norm(b - dy.numpy()) / np.linalg.norm(b))
 
 
-def test_zipf_2m_every_precision(gb):
+@pytest.mark.parametrize("plan", [None, {"xtile": 8192}])
+def test_zipf_2m_every_precision(gb, plan):
+    # the default plan, and the x tiles the autotuner picks for this matrix
+    # (the batched engine's mixed solves then run on the layout too)
     n = 2000000
-    M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096)
+    M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096, plan=plan)
     assert int(M.info()["nnz"]) > 300000000
     db = gb.uniform_device(2, n, stream=1)
     dref = gb.DeviceArray(n)

@@ -384,12 +384,13 @@ def test_index16_block_check_beyond_row_precheck(gb, oracle):
 
 # ---- irregular rows at the SpMV contract's tolerance
 
-@pytest.mark.parametrize("plan", [None, {"csplit": 8}, "autotune"])
+@pytest.mark.parametrize("plan", [None, {"csplit": 8}, {"xtile": 4096}, {"xtile": 8192},
+                                  "autotune"])
 def test_zipf_spmv_rel_inf(gb, oracle, plan):
     # Zipf(1.5) row degrees clipped to [2, 4096] (rows up to a few thousand
     # entries), uniform columns: every plan the autotuner can pick, the
-    # 8-slice column split among them, within 1e-14 relative inf-norm of the
-    # reference's left-to-right sums
+    # 8-slice column split and the x tiles among them, within 1e-14 relative
+    # inf-norm of the reference's left-to-right sums
     n = 200000
     M = gb.generate_spd(n, 2, gb.DEGREE_ZIPF, 1.5, 2, 4096,
                         plan=plan if isinstance(plan, dict) else None)

@@ -42,6 +42,25 @@ def test_noalloc_cg_matches_allocating_entry_point(gb, oracle):
         assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
 
 
+def test_noalloc_cg_on_x_tiles(gb, oracle):
+    # a plan whose SpMV stages x through LDS in column tiles: the same
+    # kernels as the allocating entry point, no workspace beyond the array
+    M0 = gb.generate_spd(60000, 5, gb.DEGREE_ZIPF, 1.5, 2, 400)
+    rp, ri, rv = M0.download()
+    M = gb.Matrix(rp, ri, rv, plan={"xtile": 4096})
+    n = M.nrows
+    b = gb.uniform(4, n)
+    db = gb.DeviceArray.from_host(b)
+    dx1, dx2 = gb.DeviceArray(n), gb.DeviceArray(n)
+    i1 = M.cgd_device(db, dx1, 1e-10, gb.TOL_RELATIVE)
+    rv2, i2 = noalloc(gb, M, db, dx2, 1e-10, gb.TOL_RELATIVE)
+    assert rv2 == gb.SUCCESS
+    assert i2.status == i1.status == 1 and i2.niter == i1.niter
+    assert np.array_equal(dx1.numpy(), dx2.numpy())
+    r = b - oracle.spmv(rp, ri, rv, dx2.numpy())
+    assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
+
+
 def test_noalloc_cg_cap_and_absolute_tolerance(gb):
     rp, ri, rv = gb.laplacian(2, 64)
     M = gb.Matrix(rp, ri, rv)
