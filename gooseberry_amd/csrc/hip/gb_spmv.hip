@@ -1364,8 +1364,9 @@ static int gb_split_build(gbd_matrix *m, uint32_t P) {
 // the plans of a row-parted matrix: one per part; the parent's histogram
 // and bin counts are the parts' sums
 static int gb_parts_plan(gbd_matrix *m) {
-  if (m->params.csplit > 1) {
-    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split");
+  if (m->params.csplit > 1 || m->params.xtile) {
+    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split "
+                 "and no x tiles");
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
   for (uint32_t q = 0; q < m->nparts; ++q) {
@@ -1758,8 +1759,9 @@ int gbd_matrix_replan(gbd_matrix *m, const gbd_plan_params *params) {
   if (params) p = *params;
   else gbd_plan_defaults(m->hist, m->nrows, m->nnz, &p);
   if (!gb_plan_check(&p)) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  if (m->nparts && p.csplit > 1) {
-    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split");
+  if (m->nparts && (p.csplit > 1 || p.xtile)) {
+    gb_set_error("a matrix held as row parts (64-bit row pointers) takes no column split "
+                 "and no x tiles");
     return GOOSEBERRY_ERROR_INVALIDINPUT;
   }
   m->params = p;

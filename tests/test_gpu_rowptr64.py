@@ -81,9 +81,11 @@ def test_row_parts_plans(gb, part_nnz):
     info = M.info()
     assert info["index16"] == 0 and info["values8"] == 0
     assert np.array_equal(M.spmv(x), y32)
-    with pytest.raises(gb.GooseberryError) as e:
-        M.replan({"csplit": 4})
-    assert e.value.name == "INVALIDINPUT"
+    for plan in ({"csplit": 4}, {"xtile": 1024}):
+        with pytest.raises(gb.GooseberryError) as e:
+            M.replan(plan)
+        assert e.value.name == "INVALIDINPUT"
+    assert np.array_equal(M.spmv(x), y32)
 
 
 def test_row_parts_cg_vs_oracle(gb, oracle, part_nnz):
