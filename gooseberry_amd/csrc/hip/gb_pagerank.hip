@@ -21,6 +21,7 @@
 #include <math.h>
 
 #include "gb_matrix.h"
+#include "gb_xtile.h"
 
 // Sink ranks added one at a time in sink order up to this many sinks; above
 // it, lane-strided sums and a butterfly (fixed order, not the reference's).
@@ -287,7 +288,9 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   if (m->xt_ptr && nnz) {
     // an x-tile plan: the layout's copy of the values divided by the same
     // column sums (the same quotient per entry)
-    GB_PR_TRY(gb_dev_alloc((void **)&snval, (size_t)nnz * 8));
+    // with the layout's zero entries past the end (its SpMV reads pairs)
+    GB_PR_TRY(gb_dev_alloc((void **)&snval, ((size_t)nnz + GB_XT_PAD) * 8));
+    GB_PR_TRY(hipMemsetAsync(snval + nnz, 0, GB_XT_PAD * 8, s));
     const uint32_t ng = m->xt_W * m->xt_T;
     gb_pr_normalise_xt<<<ng < 65536u ? ng : 65536u, GB_BLOCK, 0, s>>>(
         m->xt_ptr, ng, m->xt_T, m->xt_lc, m->xt_idx, m->xt_val, indeg, snval);

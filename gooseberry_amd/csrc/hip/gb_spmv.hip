@@ -1184,14 +1184,16 @@ static int gb_xtile_f32(gbd_matrix *m) {
   hipStream_t s = gb_stream();
   const size_t nz = m->nnz;
   if (m->val32 && !m->xt_val32) {
-    GB_HIP(gb_dev_alloc((void **)&m->xt_val32, (nz ? nz : 1) * sizeof(float)));
+    GB_HIP(gb_dev_alloc((void **)&m->xt_val32, (nz + GB_XT_PAD) * sizeof(float)));
+    GB_HIP(hipMemsetAsync(m->xt_val32 + nz, 0, GB_XT_PAD * sizeof(float), s));
     if (nz) {
       gb_to_f32<<<gb_div_up(nz, GB_BLOCK), GB_BLOCK, 0, s>>>(m->xt_val, m->xt_val32, nz);
       GB_LAUNCH_CHECK();
     }
   }
   if (m->val32j && m->jscale && !m->xt_val32j) {
-    GB_HIP(gb_dev_alloc((void **)&m->xt_val32j, (nz ? nz : 1) * sizeof(float)));
+    GB_HIP(gb_dev_alloc((void **)&m->xt_val32j, (nz + GB_XT_PAD) * sizeof(float)));
+    GB_HIP(hipMemsetAsync(m->xt_val32j + nz, 0, GB_XT_PAD * sizeof(float), s));
     const uint32_t ng = m->xt_W * m->xt_T;
     if (nz && ng) {
       gb_xt_jacobi_vals<<<ng < 65536u ? ng : 65536u, GB_BLOCK, 0, s>>>(
@@ -1229,8 +1231,11 @@ static int gb_xtile_build(gbd_matrix *m) {
   if (e == hipSuccess) e = gb_dev_alloc((void **)&word, nz * 4);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&perm, nz * 4);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_ptr, (groups + 1) * 4);
-  if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_idx, nz * 4);
-  if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_val, nz * 8);
+  // two entries of zeros past the end: the SpMV reads the layout in pairs
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_idx, (nz + GB_XT_PAD) * 4);
+  if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_val, (nz + GB_XT_PAD) * 8);
+  if (e == hipSuccess) e = hipMemsetAsync(m->xt_idx + nz, 0, GB_XT_PAD * 4, s);
+  if (e == hipSuccess) e = hipMemsetAsync(m->xt_val + nz, 0, GB_XT_PAD * 8, s);
   int rv = e == hipSuccess ? GOOSEBERRY_SUCCESS : GOOSEBERRY_ERROR_NOMEMORY;
   if (rv == GOOSEBERRY_SUCCESS) {
     gb_xt_keys<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(m->rowptr, m->rowind, n, R, T, lc,

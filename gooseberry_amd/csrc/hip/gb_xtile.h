@@ -17,7 +17,8 @@
 
 #define GB_XT_THREADS 1024
 #define GB_XT_RMAX 8192  // rows of y a workgroup keeps in LDS
-#define GB_XT_DEPTH 2    // chunks of 64 entries a wavefront has in flight
+#define GB_XT_DEPTH 2    // chunks a wavefront has in flight
+#define GB_XT_PAD 2      // zero entries past the end of the layout's arrays
 
 // what the workgroup does with its block of y = A x
 enum {
@@ -99,14 +100,18 @@ struct gb_xt_lds {
 
 // y = A x on the x-tile layout for the block blockIdx.x (grid = the row
 // blocks).  A group's entries are cut into one contiguous range per
-// wavefront; a wavefront reads its range 64 entries at a time (coalesced,
+// wavefront; a wavefront reads its range 128 entries at a time, two
+// consecutive entries per lane (8- and 16-byte loads, coalesced;
 // GB_XT_DEPTH chunks in flight; the next group's tile of x and first chunks
-// are loaded when this group starts) and adds each row's products with a
-// segmented scan across its lanes (rows are contiguous within a group, so
-// equal rows are one run).  The run that reaches a chunk's end is carried
-// into the next chunk; a run cut by the range's end is finished by the
-// wavefront where it starts, which adds the following ranges' parts in
-// order.
+// are loaded when this group starts).  A lane adds its two products when
+// they share a row; a segmented scan across the lanes, keyed by each lane's
+// second row, sums the runs (rows are contiguous within a group, so equal
+// rows are one run), and a run ending at a lane's first entry takes the
+// previous lane's scan.  The run that reaches a chunk's end is carried into
+// the next chunk; a run cut by the range's end is finished by the wavefront
+// where it starts, which adds the following ranges' parts in order.  The
+// layout's arrays hold GB_XT_PAD zero entries past the end, so a pair never
+// reads past them.
 template <typename V, int EPI>
 __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
                                            const uint32_t *__restrict__ idx,
@@ -130,26 +135,38 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
   const uint32_t C = 1u << lc;
   for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) ys[i] = (V)0;
   const uint32_t *gp = ptr + (size_t)blockIdx.x * T;
-  V xr[XPT];                  // the next group's tile of x
-  uint32_t rw[GB_XT_DEPTH];   // chunks in flight: packed words and values
-  V rv[GB_XT_DEPTH];
-  uint32_t rw2[GB_XT_DEPTH];  // the next group's first chunks
-  V rv2[GB_XT_DEPTH];
+  // a lane holds two consecutive entries: the layout is read as pairs
+  typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+  typedef V v2 __attribute__((ext_vector_type(2)));
+  constexpr uint32_t CH = 2 * GB_WAVE;  // entries per chunk
+  const u2 *__restrict__ idx2 = (const u2 *)idx;
+  const v2 *__restrict__ val2 = (const v2 *)val;
+  V xr[XPT];                // the next group's tile of x
+  u2 rw[GB_XT_DEPTH];       // chunks in flight: packed words and values
+  v2 rv[GB_XT_DEPTH];
+  u2 rw2[GB_XT_DEPTH];      // the next group's first chunks
+  v2 rv2[GB_XT_DEPTH];
   uint32_t wprev = 0, wprev2 = 0;  // the entry before the range (is its first
                                    // run cut?)
+  // the group pointers are read through the constant address space: the
+  // same words for every lane, loaded by the scalar unit, so no vector load
+  // (and no wait on the vector loads in flight) comes between groups
+  typedef const __attribute__((address_space(4))) uint32_t gb_const_u32;
+  const gb_const_u32 *gpc = (const gb_const_u32 *)(uintptr_t)gp;
   // this wavefront's range of group tt
   auto range = [&](uint32_t tt, uint32_t &ra, uint32_t &rz) {
-    const uint32_t gs = gp[tt], ge = gp[tt + 1];
+    const uint32_t gs = gpc[tt], ge = gpc[tt + 1];
     const uint32_t cnt = ge - gs;
     const uint32_t ew = (cnt + NW - 1) / NW;
     ra = gs + min(cnt, wv * ew);
     rz = min(ra + ew, ge);
   };
-  // group tt's tile of x into registers, and its first chunks.  Every
-  // load is unconditional, its index clamped into the array (the extra
-  // values are never used), so no load sits under a branch and the waits
-  // for them stay exact.
-  auto prefetch = [&](uint32_t tt, uint32_t (&pw)[GB_XT_DEPTH], V (&pv)[GB_XT_DEPTH],
+  // group tt's tile of x into registers, and its first chunks (pairs from
+  // the even entry at or before the range's start).  Every load is
+  // unconditional, its pair clamped to the range's last (the extra values
+  // are never used), so no load sits under a branch and the waits for them
+  // stay exact.
+  auto prefetch = [&](uint32_t tt, u2 (&pw)[GB_XT_DEPTH], v2 (&pv)[GB_XT_DEPTH],
                       uint32_t &pprev) {
     const uint32_t c0 = tt << lc;
     const uint32_t cl = min(C, ncols - c0) - 1;
@@ -157,17 +174,22 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
     for (uint32_t i = 0; i < XPT; ++i) xr[i] = x[c0 + min(i * GB_XT_THREADS + tid, cl)];
     uint32_t ra, rz;
     range(tt, ra, rz);
-    const uint32_t kl = rz > ra ? rz - 1 : ra > 0 ? ra - 1 : 0;
+    const uint32_t pl = (rz > ra ? rz - 1 : ra) >> 1;
     pprev = idx[ra > 0 ? ra - 1 : 0];
 #pragma unroll
     for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
-      const uint32_t k = min(ra + q * GB_WAVE + lane, kl);
-      pw[q] = __builtin_nontemporal_load(&idx[k]);
-      pv[q] = __builtin_nontemporal_load(&val[k]);
+      const uint32_t p = min((ra >> 1) + q * GB_WAVE + lane, pl);
+      pw[q] = __builtin_nontemporal_load(&idx2[p]);
+      pv[q] = __builtin_nontemporal_load(&val2[p]);
     }
   };
-  uint32_t t = 0;
-  while (t < T && gp[t] == gp[t + 1]) ++t;
+  // the next group with entries after group tt (or T); the pointer words
+  // are read as the same for every lane, so the loop stays scalar
+  auto next_group = [&](uint32_t tt) {
+    while (tt < T && gpc[tt] == gpc[tt + 1]) ++tt;
+    return tt;
+  };
+  uint32_t t = next_group(0);
   if (t < T) prefetch(t, rw, rv, wprev);
   while (t < T) {
     {
@@ -181,57 +203,77 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
     __syncthreads();
     // the next group's x and first chunks go out now: they have this
     // group's work to arrive
-    uint32_t tn = t + 1;
-    while (tn < T && gp[tn] == gp[tn + 1]) ++tn;
+    const uint32_t tn = next_group(t + 1);
     if (tn < T) prefetch(tn, rw2, rv2, wprev2);
-    const uint32_t s = gp[t];
+    const uint32_t s = gpc[t];
     uint32_t a, z;
     range(t, a, z);
     uint32_t fw = 0, lw = 0;
     if (a < z) {  // the same for the whole wavefront
-      const uint32_t rfirst = (uint32_t)__builtin_amdgcn_readfirstlane((int)rw[0]) >> lc;
+      const uint32_t w0 = (a & 1u) ? rw[0].y : rw[0].x;
+      const uint32_t rfirst = (uint32_t)__builtin_amdgcn_readfirstlane((int)w0) >> lc;
       const bool cut = a > s && (wprev >> lc) == rfirst;
       uint32_t crow = NONE;  // the run carried from the previous chunk
       V csum = (V)0;
-      for (uint32_t kb = a; kb < z; kb += GB_XT_DEPTH * GB_WAVE) {
+      // a run's sum into y, or, for the range's first run when it continues
+      // a run of the range before, into this wavefront's carry word
+      auto put = [&](uint32_t row, V t) {
+        if (row == rfirst && cut) wcarry[wv] = t;
+        else ys[row] += t;
+      };
+      const uint32_t kb0 = a & ~1u;
+      for (uint32_t kb = kb0; kb < z; kb += GB_XT_DEPTH * CH) {
 #pragma unroll
         for (uint32_t q = 0; q < GB_XT_DEPTH; ++q) {
-          const uint32_t k0 = kb + q * GB_WAVE;
+          const uint32_t k0 = kb + q * CH;
           if (k0 < z) {
-            const uint32_t w = rw[q];
-            const V v = rv[q];
-            const uint32_t nvalid = min((uint32_t)GB_WAVE, z - k0);
-            const bool valid = lane < nvalid;
-            // invalid lanes get rows no valid lane has (valid rows are < 2^13)
-            const uint32_t r = valid ? (w >> lc) : (0x80000000u | lane);
-            const V sc = gb_seg_scan64<V>(valid ? v * xs[w & (C - 1)] : (V)0, r, lane);
-            const uint32_t rl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)r);
-            if (crow != NONE && crow != rl0) {
-              // the carried run ended with the previous chunk
-              if (lane == 0) {
-                if (crow == rfirst && cut) wcarry[wv] = csum;
-                else ys[crow] += csum;
+            const u2 w = rw[q];
+            const v2 v = rv[q];
+            const uint32_t e0 = k0 + 2 * lane;
+            const bool v0 = e0 >= a && e0 < z, v1 = e0 + 1 < z;
+            // entries outside the range get rows no entry has (valid rows
+            // are < 2^13); the scan moves sums only to higher lanes and an
+            // invalid entry is never combined with a valid one, so theirs
+            // (products of other ranges' entries) reach no valid run
+            const uint32_t r0 = v0 ? (w.x >> lc) : (0x80000000u | (2 * lane));
+            const uint32_t r1 = v1 ? (w.y >> lc) : (0x80000001u | (2 * lane));
+            V p0 = v.x * xs[w.x & (C - 1)];
+            const V p1 = v.y * xs[w.y & (C - 1)];
+            // the run carried from the previous chunk (whose entries end
+            // just before this chunk's first, lane 0's first entry)
+            const uint32_t rl0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)r0);
+            if (crow != NONE) {
+              if (crow == rl0) {
+                if (lane == 0) p0 = csum + p0;
+              } else if (lane == 0) {
+                put(crow, csum);  // it ended with the previous chunk
               }
               crow = NONE;
             }
-            // the next lane's row (wave_shl:1; lane 63, always the last
-            // valid lane or past it, reads 0)
-            const uint32_t rn = gb_dpp_u32<0x130>(r);
-            const bool last = lane == nvalid - 1;
-            V tot = sc;
-            if (r == rl0 && crow == rl0) tot = csum + tot;
-            if (valid && !last && rn != r) {
-              if (r == rfirst && cut) wcarry[wv] = tot;
-              else ys[r] += tot;
-            }
-            crow = (uint32_t)__builtin_amdgcn_readlane((int)r, nvalid - 1);
-            csum = gb_readlane_val(tot, nvalid - 1);
+            // the lane's pair, then the runs across lanes by their last row
+            const bool same = r0 == r1;
+            const V sc = gb_seg_scan64<V>(same ? p0 + p1 : p1, r1, lane);
+            // the run ending at the lane's first entry (when the second
+            // starts another): its part in earlier lanes, then p0
+            const V sp = gb_dpp_val<0x138>(sc);      // wave_shr:1
+            const uint32_t rp = gb_dpp_u32<0x138>(r1);
+            const V t0 = rp == r0 ? sp + p0 : p0;
+            const uint32_t rn = gb_dpp_u32<0x130>(r0);  // wave_shl:1
+            // the chunk's last entry in the range: its run goes on
+            const uint32_t el = min(k0 + CH, z) - 1 - k0;
+            const uint32_t ll = el >> 1;
+            if (v0 && !same && !(lane == ll && !(el & 1u))) put(r0, t0);
+            if (v1 && r1 != rn && lane != ll) put(r1, sc);
+            const uint32_t rlast = (el & 1u) ? r1 : r0;
+            const V slast = (el & 1u) ? sc : t0;
+            crow = (uint32_t)__builtin_amdgcn_readlane((int)rlast, ll);
+            csum = gb_readlane_val(slast, ll);
           }
           // the chunk GB_XT_DEPTH further on takes this slot, once this
           // chunk's values are used (the same registers, no copies)
-          const uint32_t kf = min(k0 + GB_XT_DEPTH * GB_WAVE + lane, z - 1);
-          rw[q] = __builtin_nontemporal_load(&idx[kf]);
-          rv[q] = __builtin_nontemporal_load(&val[kf]);
+          const uint32_t pf = min((k0 >> 1) + GB_XT_DEPTH * GB_WAVE + lane, (z - 1) >> 1);
+          rw[q] = __builtin_nontemporal_load(&idx2[pf]);
+          rv[q] = __builtin_nontemporal_load(&val2[pf]);
         }
       }
       const bool single = crow == rfirst && cut;
