@@ -288,7 +288,7 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   if (m->xt_ptr && nnz) {
     // an x-tile plan: the layout's copy of the values divided by the same
     // column sums (the same quotient per entry)
-    // with the layout's zero entries past the end (its SpMV reads pairs)
+    // with the layout's zero entries past the end (its SpMV reads vectors)
     GB_PR_TRY(gb_dev_alloc((void **)&snval, ((size_t)nnz + GB_XT_PAD) * 8));
     GB_PR_TRY(hipMemsetAsync(snval + nnz, 0, GB_XT_PAD * 8, s));
     const uint32_t ng = m->xt_W * m->xt_T;

@@ -1231,7 +1231,7 @@ static int gb_xtile_build(gbd_matrix *m) {
   if (e == hipSuccess) e = gb_dev_alloc((void **)&word, nz * 4);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&perm, nz * 4);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_ptr, (groups + 1) * 4);
-  // two entries of zeros past the end: the SpMV reads the layout in pairs
+  // zero entries past the end: the SpMV reads the layout E entries a lane
   if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_idx, (nz + GB_XT_PAD) * 4);
   if (e == hipSuccess) e = gb_dev_alloc((void **)&m->xt_val, (nz + GB_XT_PAD) * 8);
   if (e == hipSuccess) e = hipMemsetAsync(m->xt_idx + nz, 0, GB_XT_PAD * 4, s);
