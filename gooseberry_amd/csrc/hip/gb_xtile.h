@@ -17,7 +17,9 @@
 
 #define GB_XT_THREADS 1024
 #define GB_XT_RMAX 8192  // rows of y a workgroup keeps in LDS
-#define GB_XT_DEPTH 2    // chunks a wavefront has in flight
+#define GB_XT_DEPTH 1    // chunks a wavefront has in flight (on the Zipf matrix:
+                         // FP64 1.56-1.59 ms at 1, 1.60 at 2, 1.72 at 3;
+                         // FP32 1.08 ms at 1, 1.13 at 2)
 #define GB_XT_PAD 4      // zero entries past the end of the layout's arrays
 
 // what the workgroup does with its block of y = A x
