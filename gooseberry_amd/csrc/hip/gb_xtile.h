@@ -10,7 +10,10 @@
 // stream.  Each row gets one sum per tile, added to its y in tile order: a
 // fixed regrouping of the row's sum (by tile, as with a column split),
 // bitwise reproducible.  V is the value, vector and accumulation type
-// (double, or float for the FP32 SpMV of the mixed-precision solves).
+// (double, or float for the FP32 SpMV of the mixed-precision solves).  In
+// FP32 a step covers two adjacent tiles (their groups are adjacent in the
+// layout and their x fits the same LDS), whose runs add into two
+// accumulators, summed when the block is written.
 #pragma once
 
 #include "gb_matrix.h"
@@ -124,7 +127,12 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
                                            uint32_t ncols, uint32_t R, uint32_t T, uint32_t lc,
                                            gb_xt_lds &L) {
   constexpr uint32_t NW = GB_XT_THREADS / GB_WAVE;
-  constexpr uint32_t XPT = GB_XT_CMAX / GB_XT_THREADS;  // x values a thread stages
+  // tiles a step covers: two adjacent tiles in FP32 (their x fits the same
+  // 64 KB of LDS; on the Zipf matrix 1.01 ms against 1.07 ms a tile a
+  // step), one in FP64; the second tile's rows carry bit 15 in their keys,
+  // so that no run spans the two tiles
+  constexpr uint32_t TP = sizeof(V) == 4 ? 2 : 1;
+  constexpr uint32_t XPT = TP * GB_XT_CMAX / GB_XT_THREADS;  // x values a thread stages
   constexpr uint32_t NONE = 0xffffffffu;
   V *xs = (V *)L.xs;
   V *ys = (V *)L.ys;
@@ -135,7 +143,14 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
   const uint32_t r0 = blockIdx.x * R;
   const uint32_t nr = min(R, nrows - r0);
   const uint32_t C = 1u << lc;
-  for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) ys[i] = (V)0;
+  // with two tiles a step, the second tile's runs add into their own
+  // accumulator (the y region's unused half in FP32): a row can have a run
+  // in each tile, and two wavefronts must not update one word at once
+  V *ys2 = ys + GB_XT_RMAX;
+  for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) {
+    ys[i] = (V)0;
+    if (TP > 1) ys2[i] = (V)0;
+  }
   const uint32_t *gp = ptr + (size_t)blockIdx.x * T;
   // a lane holds E consecutive entries, read as one vector: 2 in FP64, 4 in
   // FP32 (measured on the Zipf matrix: FP64 1.61 ms at 2, 1.66 ms at 4;
@@ -158,9 +173,10 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
   // (and no wait on the vector loads in flight) comes between groups
   typedef const __attribute__((address_space(4))) uint32_t gb_const_u32;
   const gb_const_u32 *gpc = (const gb_const_u32 *)(uintptr_t)gp;
-  // this wavefront's range of group tt
+  const uint32_t U = (T + TP - 1) / TP;  // steps
+  // this wavefront's range of step tt
   auto range = [&](uint32_t tt, uint32_t &ra, uint32_t &rz) {
-    const uint32_t gs = gpc[tt], ge = gpc[tt + 1];
+    const uint32_t gs = gpc[tt * TP], ge = gpc[min(T, tt * TP + TP)];
     const uint32_t cnt = ge - gs;
     const uint32_t ew = (cnt + NW - 1) / NW;
     ra = gs + min(cnt, wv * ew);
@@ -173,8 +189,8 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
   // stay exact.
   auto prefetch = [&](uint32_t tt, uE (&pw)[GB_XT_DEPTH], vE (&pv)[GB_XT_DEPTH],
                       uint32_t &pprev) {
-    const uint32_t c0 = tt << lc;
-    const uint32_t cl = min(C, ncols - c0) - 1;
+    const uint32_t c0 = (tt * TP) << lc;
+    const uint32_t cl = min(TP * C, ncols - c0) - 1;
 #pragma unroll
     for (uint32_t i = 0; i < XPT; ++i) xr[i] = x[c0 + min(i * GB_XT_THREADS + tid, cl)];
     uint32_t ra, rz;
@@ -191,14 +207,14 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
   // the next group with entries after group tt (or T); the pointer words
   // are read as the same for every lane, so the loop stays scalar
   auto next_group = [&](uint32_t tt) {
-    while (tt < T && gpc[tt] == gpc[tt + 1]) ++tt;
+    while (tt < U && gpc[tt * TP] == gpc[min(T, tt * TP + TP)]) ++tt;
     return tt;
   };
   uint32_t t = next_group(0);
-  if (t < T) prefetch(t, rw, rv, wprev);
-  while (t < T) {
+  if (t < U) prefetch(t, rw, rv, wprev);
+  while (t < U) {
     {
-      const uint32_t cw = min(C, ncols - (t << lc));
+      const uint32_t cw = min(TP * C, ncols - ((t * TP) << lc));
 #pragma unroll
       for (uint32_t i = 0; i < XPT; ++i) {
         const uint32_t c = i * GB_XT_THREADS + tid;
@@ -209,8 +225,22 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
     // the next group's x and first chunks go out now: they have this
     // group's work to arrive
     const uint32_t tn = next_group(t + 1);
-    if (tn < T) prefetch(tn, rw2, rv2, wprev2);
-    const uint32_t s = gpc[t];
+    if (tn < U) prefetch(tn, rw2, rv2, wprev2);
+    const uint32_t s = gpc[t * TP];
+    // entries from mid on belong to the step's second tile
+    const uint32_t mid = gpc[min(T, t * TP + 1)];
+    auto key = [&](uint32_t w, uint32_t e) {
+      if constexpr (TP == 1) return w >> lc;
+      else return (w >> lc) | (e >= mid ? 0x8000u : 0u);
+    };
+    auto xcol = [&](uint32_t w, uint32_t e) {
+      if constexpr (TP == 1) return w & (C - 1);
+      else return (e >= mid ? C : 0u) | (w & (C - 1));
+    };
+    auto yat = [&](uint32_t k) -> V & {
+      if constexpr (TP == 1) return ys[k];
+      else return (k & 0x8000u) ? ys2[k & 0x7fffu] : ys[k];
+    };
     uint32_t a, z;
     range(t, a, z);
     uint32_t fw = 0, lw = 0;
@@ -219,15 +249,15 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
 #pragma unroll
       for (uint32_t i = 1; i < E; ++i)
         if (a % E == i) w0 = rw[0][i];
-      const uint32_t rfirst = (uint32_t)__builtin_amdgcn_readfirstlane((int)w0) >> lc;
-      const bool cut = a > s && (wprev >> lc) == rfirst;
+      const uint32_t rfirst = key((uint32_t)__builtin_amdgcn_readfirstlane((int)w0), a);
+      const bool cut = a > s && key(wprev, a - 1) == rfirst;
       uint32_t crow = NONE;  // the run carried from the previous chunk
       V csum = (V)0;
       // a run's sum into y, or, for the range's first run when it continues
       // a run of the range before, into this wavefront's carry word
       auto put = [&](uint32_t row, V t) {
         if (row == rfirst && cut) wcarry[wv] = t;
-        else ys[row] += t;
+        else yat(row) += t;
       };
       const uint32_t kb0 = a - a % E;
       for (uint32_t kb = kb0; kb < z; kb += GB_XT_DEPTH * CH) {
@@ -247,8 +277,8 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
 #pragma unroll
             for (uint32_t i = 0; i < E; ++i) {
               const bool ok = e0 + i >= a && e0 + i < z;
-              r[i] = ok ? (w[i] >> lc) : (0x80000000u | (E * lane + i));
-              p[i] = v[i] * xs[w[i] & (C - 1)];
+              r[i] = ok ? key(w[i], e0 + i) : (0x80000000u | (E * lane + i));
+              p[i] = v[i] * xs[xcol(w[i], e0 + i)];
             }
             // the run carried from the previous chunk (whose entries end
             // just before this chunk's first, lane 0's first entry)
@@ -329,7 +359,7 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
         tot += wcarry[q];
         if (!(f & 0x40000000u)) break;
       }
-      ys[r] += tot;
+      yat(r) += tot;
     }
     // no barrier here: every read of xs ended before the barrier above, and
     // the next group's range words and ys updates come after the next
@@ -344,13 +374,14 @@ __device__ __forceinline__ void gb_xt_spmv(const uint32_t *__restrict__ ptr,
   }
   __syncthreads();  // the last group's chained runs are in ys
   if constexpr (EPI == GB_XT_STORE) {
-    for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) y[(size_t)(r0 + i) * ystride] = ys[i];
+    for (uint32_t i = tid; i < nr; i += GB_XT_THREADS)
+      y[(size_t)(r0 + i) * ystride] = TP > 1 ? ys[i] + ys2[i] : ys[i];
   } else {
     // the block's partial: each thread's rows in order, then the
     // wavefronts' sums in order
     double d = 0.0;
     for (uint32_t i = tid; i < nr; i += GB_XT_THREADS) {
-      const V yi = ys[i];
+      const V yi = TP > 1 ? ys[i] + ys2[i] : ys[i];
       if constexpr (EPI == GB_XT_DOT) {
         y[r0 + i] = yi;
         d += (double)x[r0 + i] * (double)yi;

