@@ -1197,14 +1197,11 @@ def run_dropin(args, gb):
     """gooseberry_spmult with host pointers (src/gooseberry.c:53-67): each call
     uploads A and x, builds A's SpMV plan, multiplies and copies y back.  Per
     call time at the CPU-runnable 256^2 Laplacian and the 160^3 Laplacian,
-    beside the resident SpMV and the oracle's SpMV on this host."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    os.environ.setdefault("OMP_PLACES", "cores")
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    import oracle as orc
-    host = host_cpu()
-    orc.set_threads(cpu_threads(host))
-    res = {}
+    beside the resident SpMV and the oracle's SpMV on this host.  Every GPU
+    call is timed before the oracle's OpenMP runtime starts: under
+    OMP_PROC_BIND it binds this thread to one core, and a host-pointer call
+    made from a thread bound to one core is a different measurement."""
+    cases = []
     for dims, side in ((2, 256), (3, args.grid)):
         rp, ri, rv = gb.laplacian(dims, side)
         n = rp.size - 1
@@ -1217,10 +1214,21 @@ def run_dropin(args, gb):
             t = time.perf_counter()
             gb.spmult(n, n, 1, rp, ri, rv, x, crowval=y)
             ts.append(time.perf_counter() - t)
-        assert np.array_equal(y, orc.spmv(rp, ri, rv, x)) or dims == 3
         M = gb.Matrix(rp, ri, rv)
         dx, dy = gb.DeviceArray.from_host(x), gb.DeviceArray(n)
         roof = spmv_roofline(gb, M, dx, dy)
+        del M, dx, dy
+        cases.append((dims, side, rp, ri, rv, x, y, ts, roof))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    import oracle as orc
+    host = host_cpu()
+    orc.set_threads(cpu_threads(host))
+    res = {}
+    for dims, side, rp, ri, rv, x, y, ts, roof in cases:
+        n = rp.size - 1
+        assert np.array_equal(y, orc.spmv(rp, ri, rv, x)) or dims == 3
         tc = []
         for _ in range(5):
             t = time.perf_counter()
@@ -1233,7 +1241,6 @@ def run_dropin(args, gb):
             "resident_spmv_ms": roof["avg_launch_ms"],
             "oracle_spmv_ms": float(np.median(tc)) * 1e3,
             "dropin_algorithmic_gbps": ab / float(np.median(ts)) / 1e9}
-        del M, dx, dy
     big = res["3d_%d" % args.grid]
     return line(1e3 / big["dropin_ms_per_call"], "calls/s", args, big["dropin_ms_per_call"],
                 "f64", {"workload": "gooseberry_spmult with host pointers (A, x uploaded, plan "

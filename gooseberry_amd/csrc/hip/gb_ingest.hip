@@ -30,12 +30,12 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
-#include <pthread.h>
 #include <unistd.h>
 
 #define GB_HD __host__ __device__
 #include "gb_strtod.h"
 #include "gb_common.h"
+#include "gb_stage.h"
 
 #define GB_IN_CHUNK 4096                       // bytes per workgroup, line split
 #define GB_IN_PER_THREAD (GB_IN_CHUNK / GB_BLOCK)  // 16
@@ -348,135 +348,32 @@ static int gb_in_errno(int e) {
   }
 }
 
-#define GB_IN_STAGE (64ull << 20)  // bytes per pinned staging buffer, at most
-#define GB_IN_READERS 4            // host threads filling one staging buffer
-
-// Two pinned staging buffers, kept from the first file load to the end of
-// the process.  A freshly pinned page costs a page fault on first write and
-// a mapping on its first copy to the device; on a cold buffer those cost
-// several times the read itself.  Loads hold the mutex, so loads from
-// several host threads run one at a time.
-static pthread_mutex_t g_in_mu = PTHREAD_MUTEX_INITIALIZER;
-static char *g_in_stage[2];
-static hipEvent_t g_in_ev[2];
-static uint64_t g_in_stage_bytes;
-
-struct gb_in_read_job {
+// A file range read into a staging buffer: pread by the staging threads,
+// each over its own part of the range.
+struct gb_in_file_src {
   int fd;
-  char *dst;
-  uint64_t off, len;
-  int err;
+  uint64_t base;  // file offset of the range's first byte
 };
 
-static void *gb_in_read_part(void *p) {
-  gb_in_read_job *j = (gb_in_read_job *)p;
+static int gb_in_fill_file(void *ctx, uint64_t off, uint64_t len, char *buf) {
+  const gb_in_file_src *f = (const gb_in_file_src *)ctx;
   uint64_t done = 0;
-  while (done < j->len) {
-    const ssize_t r = pread(j->fd, j->dst + done, (size_t)(j->len - done),
-                            (off_t)(j->off + done));
+  while (done < len) {
+    const ssize_t r = pread(f->fd, buf + done, (size_t)(len - done),
+                            (off_t)(f->base + off + done));
     if (r < 0 && errno == EINTR) continue;
-    if (r <= 0) {  // an error, or the file ended early
-      j->err = GOOSEBERRY_ERROR_READ;
-      break;
-    }
+    if (r <= 0) return GOOSEBERRY_ERROR_READ;  // an error, or the file ended early
     done += (uint64_t)r;
   }
-  return nullptr;
-}
-
-// len bytes at file offset off into dst, split over GB_IN_READERS threads
-// (a part whose thread cannot be started is read by this thread)
-static int gb_in_read_parallel(int fd, uint64_t off, uint64_t len, char *dst) {
-  gb_in_read_job job[GB_IN_READERS];
-  pthread_t th[GB_IN_READERS];
-  bool started[GB_IN_READERS] = {};
-  const int nt = len >= (4ull << 20) ? GB_IN_READERS : 1;
-  for (int k = 0; k < nt; ++k) {
-    const uint64_t a = len * k / nt, b = len * (k + 1) / nt;
-    job[k] = {fd, dst + a, off + a, b - a, 0};
-  }
-  for (int k = 1; k < nt; ++k)
-    started[k] = pthread_create(&th[k], nullptr, gb_in_read_part, &job[k]) == 0;
-  gb_in_read_part(&job[0]);
-  int rv = job[0].err;
-  for (int k = 1; k < nt; ++k) {
-    if (started[k]) pthread_join(th[k], nullptr);
-    else gb_in_read_part(&job[k]);
-    if (job[k].err) rv = job[k].err;
-  }
-  return rv ? rv : GOOSEBERRY_SUCCESS;
-}
-
-// staging buffers of at least `want` bytes (called with g_in_mu held and
-// no copy from them in flight)
-static int gb_in_stage_reserve(uint64_t want) {
-  hipError_t he = hipSuccess;
-  for (int j = 0; j < 2 && he == hipSuccess; ++j)
-    if (!g_in_ev[j]) he = hipEventCreateWithFlags(&g_in_ev[j], hipEventDisableTiming);
-  if (he != hipSuccess) return gb_hip_fail(he, "hipEventCreateWithFlags", __FILE__, __LINE__);
-  if (g_in_stage_bytes >= want) return GOOSEBERRY_SUCCESS;
-  for (int j = 0; j < 2; ++j) {
-    if (g_in_stage[j]) (void)hipHostFree(g_in_stage[j]);
-    g_in_stage[j] = nullptr;
-  }
-  g_in_stage_bytes = 0;
-  for (int j = 0; j < 2 && he == hipSuccess; ++j)
-    he = hipHostMalloc((void **)&g_in_stage[j], want, hipHostMallocDefault);
-  if (he != hipSuccess) {
-    for (int j = 0; j < 2; ++j) {
-      if (g_in_stage[j]) (void)hipHostFree(g_in_stage[j]);
-      g_in_stage[j] = nullptr;
-    }
-    return gb_hip_fail(he, "hipHostMalloc (staging)", __FILE__, __LINE__);
-  }
-  g_in_stage_bytes = want;
   return GOOSEBERRY_SUCCESS;
 }
 
 // bytes [offset, offset + bytes) of an open file into device memory: the
-// reader threads fill one staging buffer while the other one's copy to the
-// device runs
+// staging threads read into one pinned buffer while the other one's copy to
+// the device runs
 static int gb_in_file_to_device(FILE *f, uint64_t offset, uint64_t bytes, char *d_dst) {
-  if (bytes == 0) return GOOSEBERRY_SUCCESS;
-  const int fd = fileno(f);
-  hipStream_t s = gb_stream();
-  uint64_t stage = bytes < GB_IN_STAGE ? (bytes + (1ull << 20) - 1) & ~((1ull << 20) - 1)
-                                       : GB_IN_STAGE;
-  if (stage > GB_IN_STAGE) stage = GB_IN_STAGE;
-  bool used[2] = {false, false};
-  uint64_t copied = 0;
-  int i = 0;
-  hipError_t he = hipSuccess;
-  pthread_mutex_lock(&g_in_mu);
-  int rv = gb_in_stage_reserve(stage);
-  if (rv != GOOSEBERRY_SUCCESS) goto done;
-  stage = g_in_stage_bytes;
-  while (copied < bytes) {
-    const uint64_t want = bytes - copied < stage ? bytes - copied : stage;
-    if (used[i]) {
-      he = hipEventSynchronize(g_in_ev[i]);
-      if (he != hipSuccess) {
-        rv = gb_hip_fail(he, "hipEventSynchronize", __FILE__, __LINE__);
-        goto done;
-      }
-    }
-    rv = gb_in_read_parallel(fd, offset + copied, want, g_in_stage[i]);
-    if (rv != GOOSEBERRY_SUCCESS) goto done;
-    he = hipMemcpyAsync(d_dst + copied, g_in_stage[i], want, hipMemcpyHostToDevice, s);
-    if (he == hipSuccess) he = hipEventRecord(g_in_ev[i], s);
-    if (he != hipSuccess) {
-      rv = gb_hip_fail(he, "hipMemcpyAsync", __FILE__, __LINE__);
-      goto done;
-    }
-    used[i] = true;
-    copied += want;
-    i ^= 1;
-  }
-done:
-  // the staging buffers are idle again before another load may use them
-  (void)hipStreamSynchronize(s);
-  pthread_mutex_unlock(&g_in_mu);
-  return rv;
+  gb_in_file_src src = {fileno(f), offset};
+  return gb_stage_htod(d_dst, bytes, gb_in_fill_file, &src);
 }
 
 extern "C" {
