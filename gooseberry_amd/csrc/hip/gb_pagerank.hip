@@ -5,7 +5,9 @@
 //   column sums  indeg[c] = the values of column c added in row order (a
 //                stable counting sort of the entries by column, then one
 //                sequential sum per column: the reference's order)
-//   normalise    nval[l] = val[l] / indeg[rowind[l]] (IEEE division)
+//   normalise    nval[l] = val[l] / indeg[rowind[l]] (IEEE division), over
+//                the copy of the values the SpMV plan reads (the CSR's, the
+//                x-tile layout's or the column slices')
 //   sinks        the rows without entries, ascending (scan + compaction)
 // One iteration, three launches:
 //   SpMV         out = N rank (the matrix's own plan over nval; with a
@@ -19,6 +21,8 @@
 // and every tenth iteration the squared difference of rank and out
 // (fixed-order two-level reduction) for the stopping test.
 #include <math.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include "gb_matrix.h"
 #include "gb_xtile.h"
@@ -113,6 +117,210 @@ gb_pr_normalise_xt(const uint32_t *__restrict__ ptr, uint32_t ngroups, uint32_t 
     const uint32_t c0 = (g % T) << lc, e = ptr[g + 1];
     for (uint32_t j = ptr[g] + threadIdx.x; j < e; j += GB_BLOCK)
       nval[j] = val[j] / indeg[c0 | (word[j] & cm)];
+  }
+}
+
+// Column sums in row order and the normalised values on an x-tile layout,
+// one workgroup per column tile, with no sort.  The tile's entries in layout order (row blocks in
+// order, each group in CSR order) are its columns' entries in row order.
+// The workgroup walks them in chunks of GB_PRX_THREADS positions, one entry
+// a thread (the next chunk's entries load while the current one is added),
+// taking the groups of up to GB_PRX_THREADS row blocks at a time laid end to
+// end.  A wavefront holds 64 consecutive positions: ballots of the column
+// bits give every lane the lanes of its column, and the first of them adds
+// their values to the column's sum in LDS in lane order.  A column met by
+// one wavefront of the chunk is added at once; a column met by several is
+// added by those wavefronts one after another, in wavefront order.  The
+// sums are bitwise those of the sort-based path.  Then the workgroup divides
+// every entry of the tile by its column's sum.
+#define GB_PRX_THREADS 1024
+#define GB_PRX_NORM 4
+
+// A workgroup barrier that orders LDS only: global loads still in flight
+// (the next chunk's entries) stay in flight across it, where __syncthreads
+// would wait for them.  Every exchange across it here goes through LDS.
+__device__ __forceinline__ void gb_prx_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// exclusive scan of one value per thread over GB_PRX_THREADS threads;
+// *total gets the sum
+__device__ __forceinline__ uint32_t gb_prx_scan_excl(uint32_t v, uint32_t *wsum,
+                                                     uint32_t *total) {
+  const uint32_t lane = threadIdx.x & (GB_WAVE - 1), wave = threadIdx.x / GB_WAVE;
+  const uint32_t inc = gb_wave_scan_incl(v);
+  if (lane == GB_WAVE - 1) wsum[wave] = inc;
+  __syncthreads();
+  uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < GB_PRX_THREADS / GB_WAVE; ++w) {
+    const uint32_t x = wsum[w];
+    if (w < wave) woff += x;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + inc - v;
+}
+
+// the layout position of window position q: the group k whose start
+// goff[k] is the last one <= q (goff[0] = 0, non-decreasing over nb groups)
+__device__ __forceinline__ uint32_t gb_prx_locate(const uint32_t *goff, const uint32_t *gst,
+                                                  uint32_t nb, uint32_t q) {
+  uint32_t lo = 0, hi = nb;  // goff[lo] <= q < goff[hi] (goff[nb] = the total)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (goff[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return gst[lo] + (q - goff[lo]);
+}
+
+// The tile's groups of row blocks b0 .. b0 + nb - 1 laid end to end: the
+// first entry (gst) and first window position (goff) of each, goff[nb] = the
+// window's entry count (returned).  Ends with a barrier.
+__device__ __forceinline__ uint32_t gb_prx_window(const uint32_t *__restrict__ ptr,
+                                                  uint32_t b0, uint32_t nb, uint32_t T,
+                                                  uint32_t t, uint32_t *gst, uint32_t *goff,
+                                                  uint32_t *wsum) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t st = 0, sz = 0;
+  if (tid < nb) {
+    const uint32_t g = (b0 + tid) * T + t;
+    st = ptr[g];
+    sz = ptr[g + 1] - st;
+  }
+  uint32_t tot;
+  const uint32_t off = gb_prx_scan_excl(sz, wsum, &tot);  // barriers inside
+  gst[tid] = st;
+  goff[tid] = off;
+  if (tid == 0) goff[nb] = tot;
+  __syncthreads();
+  return tot;
+}
+
+// s + the values of the lanes in `peers` above this lane, in lane order
+// (every lane of the wavefront takes part; `peers` is empty for most)
+__device__ __forceinline__ double gb_prx_chain(double s, double v, uint64_t peers) {
+  for (;;) {
+    const bool more = peers != 0;
+    if (!__ballot(more)) break;
+    const int src = more ? __builtin_ctzll(peers) : (int)(threadIdx.x & (GB_WAVE - 1));
+    const double w = __shfl(v, src, GB_WAVE);
+    if (more) {
+      s = s + w;
+      peers &= peers - 1;
+    }
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(GB_PRX_THREADS)
+gb_pr_colsum_xt(const uint32_t *__restrict__ ptr, uint32_t W, uint32_t T, uint32_t lc,
+                const uint32_t *__restrict__ word, const double *__restrict__ val,
+                uint32_t ncols, double *__restrict__ indeg, double *__restrict__ nval) {
+  __shared__ double cs[GB_XT_CMAX];           // the tile's column sums
+  __shared__ uint32_t wset[GB_XT_CMAX];       // a column's wavefronts in the chunk
+  __shared__ uint32_t gst[GB_PRX_THREADS];    // the window's groups: first entry
+  __shared__ uint32_t goff[GB_PRX_THREADS + 1];  // and first window position
+  __shared__ uint32_t wsum[GB_PRX_THREADS / GB_WAVE];
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  const uint32_t lane = tid & (GB_WAVE - 1), wave = tid / GB_WAVE;
+  const uint64_t below = (1ull << lane) - 1;
+  const uint32_t C = 1u << lc, cm = C - 1;
+  for (uint32_t j = tid; j < C; j += GB_PRX_THREADS) {
+    cs[j] = 0.0;
+    wset[j] = 0;
+  }
+  for (uint32_t b0 = 0; b0 < W; b0 += GB_PRX_THREADS) {
+    const uint32_t nb = W - b0 < GB_PRX_THREADS ? W - b0 : GB_PRX_THREADS;
+    const uint32_t tot = gb_prx_window(ptr, b0, nb, T, t, gst, goff, wsum);
+    // this thread's entry of the first chunk
+    bool have = tid < tot;
+    uint32_t col = 0;
+    double v = 0.0;
+    if (have) {
+      const uint32_t j = gb_prx_locate(goff, gst, nb, tid);
+      col = word[j] & cm;
+      v = val[j];
+    }
+    // the first chunk's loads complete here: otherwise the compiler counts
+    // them as pending at the top of the loop and waits for every load there,
+    // the next chunk's included
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+    for (uint32_t q0 = 0; q0 < tot; q0 += GB_PRX_THREADS) {
+      // the next chunk's entry, loading while this chunk is added
+      const uint32_t qn = q0 + GB_PRX_THREADS + tid;
+      const bool nhave = qn < tot;
+      uint32_t ncol = 0;
+      double nv = 0.0;
+      if (nhave) {
+        const uint32_t j = gb_prx_locate(goff, gst, nb, qn);
+        ncol = word[j];
+        nv = val[j];
+      }
+      // the lanes of this wavefront with this lane's column
+      uint64_t peers = __ballot(have);
+      for (uint32_t bit = 0; bit < lc; ++bit) {
+        const bool on = (col >> bit) & 1u;
+        const uint64_t bb = __ballot(on);
+        peers &= on ? bb : ~bb;
+      }
+      const bool first = have && (peers & below) == 0;
+      if (first) atomicOr(&wset[col], 1u << wave);
+      gb_prx_lds_barrier();
+      const uint32_t ws = first ? wset[col] : 0u;
+      const bool alone = first && ws == (1u << wave);
+      const uint64_t rest = peers & ~(below | (1ull << lane));
+      // columns of one wavefront: added now, every wavefront at once
+      {
+        double sum = alone ? cs[col] + v : 0.0;
+        sum = gb_prx_chain(sum, v, alone ? rest : 0);
+        if (alone) cs[col] = sum;
+      }
+      // columns of several wavefronts: wavefront by wavefront, in order
+      const bool shared_col = first && !alone;
+      for (uint32_t w = 0; w < GB_PRX_THREADS / GB_WAVE; ++w) {
+        if (wave == w) {
+          double sum = shared_col ? cs[col] + v : 0.0;
+          sum = gb_prx_chain(sum, v, shared_col ? rest : 0);
+          if (shared_col) cs[col] = sum;
+        }
+        gb_prx_lds_barrier();
+      }
+      if (first) wset[col] = 0;
+      have = nhave;
+      col = ncol & cm;
+      v = nv;
+      gb_prx_lds_barrier();
+    }
+  }
+  for (uint32_t j = tid; j < C; j += GB_PRX_THREADS)
+    if ((t << lc) + j < ncols) indeg[(t << lc) + j] = cs[j];
+  // the quotients: every entry of the tile divided by its column's sum,
+  // GB_PRX_NORM entries a thread loaded before any is divided
+  for (uint32_t b0 = 0; b0 < W; b0 += GB_PRX_THREADS) {
+    const uint32_t nb = W - b0 < GB_PRX_THREADS ? W - b0 : GB_PRX_THREADS;
+    const uint32_t tot = gb_prx_window(ptr, b0, nb, T, t, gst, goff, wsum);
+    for (uint32_t q0 = 0; q0 < tot; q0 += GB_PRX_NORM * GB_PRX_THREADS) {
+      uint32_t jj[GB_PRX_NORM], ww[GB_PRX_NORM];
+      double vv[GB_PRX_NORM];
+#pragma unroll
+      for (int u = 0; u < GB_PRX_NORM; ++u) {
+        const uint32_t q = q0 + u * GB_PRX_THREADS + tid;
+        if (q < tot) {
+          jj[u] = gb_prx_locate(goff, gst, nb, q);
+          ww[u] = word[jj[u]];
+          vv[u] = val[jj[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < GB_PRX_NORM; ++u)
+        if (q0 + u * GB_PRX_THREADS + tid < tot) nval[jj[u]] = vv[u] / cs[ww[u] & cm];
+    }
+    __syncthreads();  // gst and goff are rewritten by the next window
   }
 }
 
@@ -226,6 +434,15 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   gbd_matrix *svp[GB_SPLIT_MAX];
   double *snval = nullptr;
   hipError_t he = hipSuccess;
+  // An x-tile or column-split plan reads its own copy of the values, which
+  // is normalised below; the CSR's order is then never read.
+  const bool own_copy = m->xt_ptr || m->nsplit;
+  // An x-tile plan sums the columns on its own layout, without the sort;
+  // GOOSEBERRY_PAGERANK_SORTED_SUMS (set, not "0") takes the sort instead
+  // (the same bits: a check of one path against the other).
+  const char *sorted_env = getenv("GOOSEBERRY_PAGERANK_SORTED_SUMS");
+  const bool tile_sums = m->xt_ptr && nnz &&
+                         !(sorted_env && *sorted_env && strcmp(sorted_env, "0") != 0);
 #define GB_PR_TRY(call)                                         \
   do {                                                          \
     he = (call);                                                \
@@ -241,9 +458,11 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   } while (0)
 
   GB_PR_TRY(gb_dev_alloc((void **)&indeg, (size_t)n * 8));
-  GB_PR_TRY(gb_dev_alloc((void **)&nval, (size_t)(nnz ? nnz : 1) * 8));
-  GB_PR_TRY(gb_dev_alloc((void **)&perm, (size_t)(nnz ? nnz : 1) * 4));
-  GB_PR_TRY(gb_dev_alloc((void **)&dist, ((size_t)n + 1) * 4));
+  if (!own_copy) GB_PR_TRY(gb_dev_alloc((void **)&nval, (size_t)(nnz ? nnz : 1) * 8));
+  if (!tile_sums) {
+    GB_PR_TRY(gb_dev_alloc((void **)&perm, (size_t)(nnz ? nnz : 1) * 4));
+    GB_PR_TRY(gb_dev_alloc((void **)&dist, ((size_t)n + 1) * 4));
+  }
   GB_PR_TRY(gb_dev_alloc((void **)&flag, (size_t)n * 4));
   GB_PR_TRY(gb_dev_alloc((void **)&idx, ((size_t)n + 1) * 4));
   GB_PR_TRY(gb_dev_alloc((void **)&sinks, (size_t)n * 4));
@@ -253,14 +472,23 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   GB_PR_TRY(gb_dev_alloc((void **)&part, GB_PR_ERR_GRID * 8));
   GB_PR_TRY(gb_dev_alloc((void **)&err, 8));
 
-  // column sums in row order (column indices were validated with the matrix)
-  GB_PR_RV(gbd_countingsort_kv_unchecked(m->rowind, nullptr, 0, n - 1, nnz, perm, dist));
-  gb_pr_colsum<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm, m->val, indeg);
-  GB_PR_TRY(hipGetLastError());
-  gb_pr_colsum_long<<<gb_div_up(n, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm,
-                                                                         m->val, indeg);
-  GB_PR_TRY(hipGetLastError());
-  if (nnz) {
+  if (tile_sums) {
+    // column sums and the layout's quotients, one workgroup per column tile
+    GB_PR_TRY(gb_dev_alloc((void **)&snval, ((size_t)nnz + GB_XT_PAD) * 8));
+    GB_PR_TRY(hipMemsetAsync(snval + nnz, 0, GB_XT_PAD * 8, s));
+    gb_pr_colsum_xt<<<m->xt_T, GB_PRX_THREADS, 0, s>>>(m->xt_ptr, m->xt_W, m->xt_T, m->xt_lc,
+                                                     m->xt_idx, m->xt_val, n, indeg, snval);
+    GB_PR_TRY(hipGetLastError());
+  } else {
+    // column sums in row order (column indices were validated with the matrix)
+    GB_PR_RV(gbd_countingsort_kv_unchecked(m->rowind, nullptr, 0, n - 1, nnz, perm, dist));
+    gb_pr_colsum<<<gb_div_up(n, GB_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm, m->val, indeg);
+    GB_PR_TRY(hipGetLastError());
+    gb_pr_colsum_long<<<gb_div_up(n, GB_WAVES_PER_BLOCK), GB_BLOCK, 0, s>>>(n, dist, perm,
+                                                                           m->val, indeg);
+    GB_PR_TRY(hipGetLastError());
+  }
+  if (nnz && !own_copy) {
     gb_pr_normalise<<<gb_pr_grid(nnz), GB_BLOCK, 0, s>>>(nnz, m->rowind, m->val, indeg,
                                                          nval);
     GB_PR_TRY(hipGetLastError());
@@ -274,7 +502,8 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   GB_PR_TRY(hipMemcpyAsync(&nsinks, idx + n, 4, hipMemcpyDeviceToHost, s));
   GB_PR_TRY(hipStreamSynchronize(s));
 
-  // the normalised matrix: the same structure and plan over nval
+  // the normalised matrix: the same structure and plan over nval (null with
+  // a plan of its own copy: its launches read only that copy)
   mm = *m;
   mm.val = nval;
   mm.val32 = nullptr;
@@ -287,14 +516,17 @@ int gbd_pagerank(const gbd_matrix *m, double damping, uint64_t niter,
   mm.split = nullptr;
   if (m->xt_ptr && nnz) {
     // an x-tile plan: the layout's copy of the values divided by the same
-    // column sums (the same quotient per entry)
-    // with the layout's zero entries past the end (its SpMV reads vectors)
-    GB_PR_TRY(gb_dev_alloc((void **)&snval, ((size_t)nnz + GB_XT_PAD) * 8));
-    GB_PR_TRY(hipMemsetAsync(snval + nnz, 0, GB_XT_PAD * 8, s));
-    const uint32_t ng = m->xt_W * m->xt_T;
-    gb_pr_normalise_xt<<<ng < 65536u ? ng : 65536u, GB_BLOCK, 0, s>>>(
-        m->xt_ptr, ng, m->xt_T, m->xt_lc, m->xt_idx, m->xt_val, indeg, snval);
-    GB_PR_TRY(hipGetLastError());
+    // column sums (the same quotient per entry), with the layout's zero
+    // entries past the end (its SpMV reads vectors); made above with the
+    // sums unless they came from the sort
+    if (!tile_sums) {
+      GB_PR_TRY(gb_dev_alloc((void **)&snval, ((size_t)nnz + GB_XT_PAD) * 8));
+      GB_PR_TRY(hipMemsetAsync(snval + nnz, 0, GB_XT_PAD * 8, s));
+      const uint32_t ng = m->xt_W * m->xt_T;
+      gb_pr_normalise_xt<<<ng < 65536u ? ng : 65536u, GB_BLOCK, 0, s>>>(
+          m->xt_ptr, ng, m->xt_T, m->xt_lc, m->xt_idx, m->xt_val, indeg, snval);
+      GB_PR_TRY(hipGetLastError());
+    }
     mm.xt_val = snval;
   }
   if (m->nsplit && nnz) {

@@ -193,3 +193,55 @@ def test_pagerank_cli_dense_input(gb, oracle, tmp_path):
     r = subprocess.run([CLI_PATH, "pagerank", "-i", a, "-o", out, "-I", "3"],
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
     assert r.returncode == 2 and "requires a square matrix" in r.stderr
+
+
+def _hub_graph(n, seed):
+    # every row reaches column 7, odd rows column 1234 as well: the chunks
+    # the tile walk adds hold many entries of one column
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        cols = set(rng.choice(n, size=4, replace=False).tolist())
+        cols.add(7)
+        if i % 2:
+            cols.add(1234)
+        rows.append(sorted(cols))
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ri = np.array([c for r in rows for c in r], dtype=np.uint32)
+    return rp, ri, np.random.default_rng(seed + 1).uniform(0.1, 3.0, ri.size)
+
+
+def _wide_graph(n, seed):
+    # one or two entries a row over more rows than 1024 row blocks of the
+    # layout hold: the tile walk takes its groups in two windows
+    rng = np.random.default_rng(seed)
+    deg = rng.integers(1, 3, n)
+    rp = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint32)
+    ri = rng.integers(0, n, int(rp[-1])).astype(np.uint32)
+    return rp, ri, rng.uniform(0.5, 2.0, ri.size)
+
+
+@pytest.mark.parametrize("case", ["hubs", "sinks", "wide"])
+def test_pagerank_tile_sums_match_sorted_sums(gb, case, monkeypatch):
+    # an x-tile plan sums the columns on its layout, in row order; the sort
+    # by column (the path pinned to the oracle above) gives the same sums,
+    # so the written vectors are the same bits
+    if case == "hubs":
+        rp, ri, rv = _hub_graph(20000, 3)
+        xtile, niter = 1024, 10
+    elif case == "sinks":
+        rp, ri, rv = _random_graph(30000, 5, sink_every=7)
+        xtile, niter = 4096, 12
+    else:
+        rp, ri, rv = _wide_graph(8_500_000, 9)
+        xtile, niter = 8192, 2
+    M = gb.Matrix(rp, ri, rv, plan={"xtile": xtile})
+    assert M.info()["plan"]["xtile"] == xtile
+    monkeypatch.delenv("GOOSEBERRY_PAGERANK_SORTED_SUMS", raising=False)
+    x, it, err, ns = M.pagerank(0.85, niter, 0.0)
+    monkeypatch.setenv("GOOSEBERRY_PAGERANK_SORTED_SUMS", "1")
+    xs, its, errs, nss = M.pagerank(0.85, niter, 0.0)
+    assert (it, ns) == (its, nss) == (niter, ns)
+    assert np.array_equal(x, xs)
+    assert err == errs
