@@ -27,6 +27,10 @@ int gbd_free(void *d_ptr);
 int gbd_host_alloc(void **h_ptr, size_t bytes); /* pinned host memory */
 int gbd_host_free(void *h_ptr);
 int gbd_htod(void *d_dst, const void *h_src, size_t bytes);
+/* n host arrays to n device buffers in one staged pipeline (n <= 16 share
+ * it; more, or arrays in pinned memory, are copied one by one) */
+int gbd_htod_many(uint32_t n, void *const *d_dst, const void *const *h_src,
+                  const size_t *bytes);
 int gbd_dtoh(void *h_dst, const void *d_src, size_t bytes);
 int gbd_dtod(void *d_dst, const void *d_src, size_t bytes);
 int gbd_zero(void *d_dst, size_t bytes); /* fill kernel, safe under capture */

@@ -384,36 +384,49 @@ static uint64_t gb_stage_chunk(uint64_t bytes) {
   return c < GB_STAGE_MAX ? c : GB_STAGE_MAX;
 }
 
-int gb_stage_htod(char *d_dst, uint64_t bytes, gb_stage_fn fill, void *ctx) {
-  if (bytes == 0) return GOOSEBERRY_SUCCESS;
+// One pipeline over several copies to the device: the buffers keep filling
+// from one copy to the next, so only the first chunk's fill and the last
+// chunk's DMA go unoverlapped, once for all of them.
+struct gb_stage_seg {
+  char *d;
+  gb_stage_fn fill;
+  void *ctx;
+  uint64_t bytes;
+};
+
+static int gb_stage_htod_segs(const gb_stage_seg *seg, uint32_t nseg) {
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < nseg; ++k) total += seg[k].bytes;
+  if (total == 0) return GOOSEBERRY_SUCCESS;
   hipStream_t s = g_stream;
   bool used[2] = {false, false};
-  uint64_t copied = 0;
   int i = 0;
   hipError_t he = hipSuccess;
   pthread_mutex_lock(&g_stage_mu);
   int rv = gb_stage_reserve(GB_STAGE_MAX);
-  const uint64_t stage = gb_stage_chunk(bytes);
-  while (rv == GOOSEBERRY_SUCCESS && copied < bytes) {
-    const uint64_t want = bytes - copied < stage ? bytes - copied : stage;
-    if (used[i]) {  // the copy two chunks back has left this buffer
-      he = hipEventSynchronize(g_stage_ev[i]);
+  const uint64_t stage = gb_stage_chunk(total);
+  for (uint32_t k = 0; k < nseg && rv == GOOSEBERRY_SUCCESS; ++k) {
+    for (uint64_t copied = 0; copied < seg[k].bytes;) {
+      const uint64_t want = seg[k].bytes - copied < stage ? seg[k].bytes - copied : stage;
+      if (used[i]) {  // the copy two chunks back has left this buffer
+        he = hipEventSynchronize(g_stage_ev[i]);
+        if (he != hipSuccess) {
+          rv = gb_hip_fail(he, "hipEventSynchronize", __FILE__, __LINE__);
+          break;
+        }
+      }
+      rv = gb_stage_parallel(seg[k].fill, seg[k].ctx, copied, want, g_stage[i]);
+      if (rv != GOOSEBERRY_SUCCESS) break;
+      he = hipMemcpyAsync(seg[k].d + copied, g_stage[i], want, hipMemcpyHostToDevice, s);
+      if (he == hipSuccess) he = hipEventRecord(g_stage_ev[i], s);
       if (he != hipSuccess) {
-        rv = gb_hip_fail(he, "hipEventSynchronize", __FILE__, __LINE__);
+        rv = gb_hip_fail(he, "hipMemcpyAsync (staged)", __FILE__, __LINE__);
         break;
       }
+      used[i] = true;
+      copied += want;
+      i ^= 1;
     }
-    rv = gb_stage_parallel(fill, ctx, copied, want, g_stage[i]);
-    if (rv != GOOSEBERRY_SUCCESS) break;
-    he = hipMemcpyAsync(d_dst + copied, g_stage[i], want, hipMemcpyHostToDevice, s);
-    if (he == hipSuccess) he = hipEventRecord(g_stage_ev[i], s);
-    if (he != hipSuccess) {
-      rv = gb_hip_fail(he, "hipMemcpyAsync (staged)", __FILE__, __LINE__);
-      break;
-    }
-    used[i] = true;
-    copied += want;
-    i ^= 1;
   }
   // the staging buffers are idle again before another copy may use them
   he = hipStreamSynchronize(s);
@@ -421,6 +434,11 @@ int gb_stage_htod(char *d_dst, uint64_t bytes, gb_stage_fn fill, void *ctx) {
     rv = gb_hip_fail(he, "hipStreamSynchronize", __FILE__, __LINE__);
   pthread_mutex_unlock(&g_stage_mu);
   return rv;
+}
+
+int gb_stage_htod(char *d_dst, uint64_t bytes, gb_stage_fn fill, void *ctx) {
+  const gb_stage_seg seg = {d_dst, fill, ctx, bytes};
+  return gb_stage_htod_segs(&seg, 1);
 }
 
 int gb_stage_dtoh(uint64_t bytes, const char *d_src, gb_stage_fn drain, void *ctx) {
@@ -582,6 +600,27 @@ int gbd_htod(void *d_dst, const void *h_src, size_t bytes) {
   GB_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, g_stream));
   GB_HIP(hipStreamSynchronize(g_stream));
   return GOOSEBERRY_SUCCESS;
+}
+
+int gbd_htod_many(uint32_t n, void *const *d_dst, const void *const *h_src,
+                  const size_t *bytes) {
+  int rv = gb_ensure_init();
+  if (rv != GOOSEBERRY_SUCCESS) return rv;
+  uint64_t total = 0;
+  bool pinned = false;
+  for (uint32_t k = 0; k < n; ++k) {
+    total += bytes[k];
+    if (bytes[k] && gb_host_pinned(h_src[k])) pinned = true;
+  }
+  if (total < GB_STAGE_MIN_HTOD || pinned || n > 16) {
+    for (uint32_t k = 0; k < n && rv == GOOSEBERRY_SUCCESS; ++k)
+      rv = gbd_htod(d_dst[k], h_src[k], bytes[k]);
+    return rv;
+  }
+  gb_stage_seg seg[16];
+  for (uint32_t k = 0; k < n; ++k)
+    seg[k] = {(char *)d_dst[k], gb_stage_from_mem, (void *)h_src[k], bytes[k]};
+  return gb_stage_htod_segs(seg, n);
 }
 
 int gbd_dtoh(void *h_dst, const void *d_src, size_t bytes) {

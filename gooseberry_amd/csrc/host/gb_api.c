@@ -259,35 +259,52 @@ int gb_matrix_adopt(dim_t nrows, dim_t ncols, ind_t *d_rowptr,
   return wrap_matrix(m, r_mat);
 }
 
+/* A matrix from host CSR arrays: the three arrays, and one more array when
+ * extra_bytes is nonzero (left in *d_extra), uploaded in one staged
+ * pipeline into fresh device buffers.  Only rowptr[0] is checked here; the
+ * rest of the structure (non-decreasing row pointers, columns below ncols)
+ * is checked on the device once the arrays are there: a scan of the host
+ * arrays took about as long as their upload. */
+static int matrix_from_host(dim_t nrows, dim_t ncols, const ind_t *rowptr,
+    const dim_t *rowind, const real_t *rowval, const gooseberry_plan_t *plan,
+    const void *extra, size_t extra_bytes, void **d_extra,
+    gooseberry_matrix_t *r_mat)
+{
+  if (!r_mat || !rowptr || rowptr[0] != 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  size_t nnz = rowptr[nrows];
+  if (nnz && (!rowind || !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
+  void *d[4] = {NULL, NULL, NULL, NULL};
+  const void *h[4] = {rowptr, rowind, rowval, extra};
+  size_t b[4] = {((size_t)nrows + 1) * sizeof(ind_t), nnz * sizeof(dim_t),
+                 nnz * sizeof(real_t), extra_bytes};
+  uint32_t n = extra_bytes ? 4 : 3, k;
+  int rv = GOOSEBERRY_SUCCESS;
+  for (k = 0; k < n && rv == GOOSEBERRY_SUCCESS; ++k) rv = gbd_malloc(&d[k], b[k]);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_htod_many(n, d, h, b);
+  gbd_matrix *m = NULL;
+  if (rv == GOOSEBERRY_SUCCESS) {
+    gbd_plan_params pp;
+    int have;
+    plan_to_dev(plan, NULL, &pp, &have);
+    rv = gbd_matrix_create(nrows, ncols, d[0], d[1], d[2], have ? &pp : NULL, &m);
+  }
+  if (rv != GOOSEBERRY_SUCCESS) {
+    for (k = 0; k < 4; ++k) gbd_free(d[k]);
+    return rv;
+  }
+  rv = wrap_matrix(m, r_mat);  /* frees the matrix and its arrays on failure */
+  if (rv != GOOSEBERRY_SUCCESS || !d_extra) gbd_free(d[3]);
+  else *d_extra = d[3];
+  return rv;
+}
+
 int gooseberry_matrix_create(dim_t nrows, dim_t ncols, const ind_t *rowptr,
     const dim_t *rowind, const real_t *rowval, const gooseberry_plan_t *plan,
     gooseberry_matrix_t *r_mat)
 {
   GB_API_GUARD();
-  if (!r_mat || !rowptr || rowptr[0] != 0) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  /* the rest of the structure (non-decreasing row pointers, columns below
-   * ncols) is checked on the device once the arrays are there: a scan of
-   * the host arrays took about as long as their upload */
-  size_t nnz = rowptr[nrows];
-  if (nnz && (!rowind || !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  void *dp = NULL, *di = NULL, *dv = NULL;
-  int rv = gb_upload(&dp, rowptr, ((size_t)nrows + 1) * sizeof(ind_t));
-  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&di, rowind, nnz * sizeof(dim_t));
-  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dv, rowval, nnz * sizeof(real_t));
-  if (rv != GOOSEBERRY_SUCCESS) {
-    gbd_free(dp); gbd_free(di); gbd_free(dv);
-    return rv;
-  }
-  gbd_plan_params pp;
-  int have;
-  plan_to_dev(plan, NULL, &pp, &have);
-  gbd_matrix *m = NULL;
-  rv = gbd_matrix_create(nrows, ncols, dp, di, dv, have ? &pp : NULL, &m);
-  if (rv != GOOSEBERRY_SUCCESS) {
-    gbd_free(dp); gbd_free(di); gbd_free(dv);
-    return rv;
-  }
-  return wrap_matrix(m, r_mat);
+  return matrix_from_host(nrows, ncols, rowptr, rowind, rowval, plan, NULL, 0, NULL,
+      r_mat);
 }
 
 int gooseberry_matrix_create_device(dim_t nrows, dim_t ncols,
@@ -407,16 +424,15 @@ static void spmult_once_plan(gooseberry_plan_t *p)
   p->wide_values = 1;
 }
 
-/* C rows = A B for a matrix made for this one call (freed here): B (host,
- * ancols x bncols, column-major) in, the nr rows of C (row-major) out */
-static int spmult_once(gooseberry_matrix_t mat, dim_t nr, dim_t ancols,
-    dim_t bncols, const real_t *bcolval, real_t *crows)
+/* C rows = A B for a matrix made for this one call (freed here, with db):
+ * B (ancols x bncols, column-major, uploaded with A into db) in, the nr
+ * rows of C (row-major) out */
+static int spmult_once(gooseberry_matrix_t mat, dim_t nr, dim_t bncols, void *db,
+    real_t *crows)
 {
-  void *db = NULL, *dc = NULL;
-  size_t bbytes = (size_t)ancols * bncols * sizeof(real_t);
+  void *dc = NULL;
   size_t cbytes = (size_t)nr * bncols * sizeof(real_t);
-  int rv = gb_upload(&db, bcolval, bbytes);
-  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_malloc(&dc, cbytes);
+  int rv = gbd_malloc(&dc, cbytes);
   if (rv == GOOSEBERRY_SUCCESS) {
     rv = (bncols == 1) ? gbd_spmv(mat->m, db, dc)
                        : gbd_spmm(mat->m, bncols, db, dc);
@@ -483,12 +499,14 @@ int gooseberry_spmult(dim_t anrows, dim_t ancols, dim_t bncols,
   gooseberry_plan_t once;
   spmult_once_plan(&once);
   gooseberry_matrix_t mat = NULL;
-  int rv = gooseberry_matrix_create(nr, ancols, rp,
+  void *db = NULL;
+  /* A and B in one upload */
+  int rv = matrix_from_host(nr, ancols, rp,
       arowind ? arowind + base : NULL, arowval ? arowval + base : NULL,
-      &once, &mat);
+      &once, bcolval, (size_t)ancols * bncols * sizeof(real_t), &db, &mat);
   free(rebased);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  return spmult_once(mat, nr, ancols, bncols, bcolval, crowval + (size_t)rlo * bncols);
+  return spmult_once(mat, nr, bncols, db, crowval + (size_t)rlo * bncols);
 }
 
 /* ------------------------------------------------------ sparse x sparse */
@@ -1063,15 +1081,19 @@ int gooseberry_matrix_create_i64(dim_t nrows, dim_t ncols,
   if (!r_mat || !rowptr) return GOOSEBERRY_ERROR_INVALIDINPUT;
   const uint64_t nnz = rowptr[nrows];
   if (nnz && (!rowind || !rowval)) return GOOSEBERRY_ERROR_INVALIDINPUT;
-  void *dp = NULL, *di = NULL, *dv = NULL;
-  int rv = gb_upload(&dp, rowptr, ((size_t)nrows + 1) * sizeof(uint64_t));
-  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&di, rowind, nnz * sizeof(dim_t));
-  if (rv == GOOSEBERRY_SUCCESS) rv = gb_upload(&dv, rowval, nnz * sizeof(real_t));
+  /* the three arrays in one staged upload */
+  void *d[3] = {NULL, NULL, NULL};
+  const void *h[3] = {rowptr, rowind, rowval};
+  size_t b[3] = {((size_t)nrows + 1) * sizeof(uint64_t), nnz * sizeof(dim_t),
+                 nnz * sizeof(real_t)};
+  int rv = GOOSEBERRY_SUCCESS;
+  for (int k = 0; k < 3 && rv == GOOSEBERRY_SUCCESS; ++k) rv = gbd_malloc(&d[k], b[k]);
+  if (rv == GOOSEBERRY_SUCCESS) rv = gbd_htod_many(3, d, h, b);
   if (rv != GOOSEBERRY_SUCCESS) {
-    gbd_free(dp); gbd_free(di); gbd_free(dv);
+    gbd_free(d[0]); gbd_free(d[1]); gbd_free(d[2]);
     return rv;
   }
-  return create64_owned(nrows, ncols, dp, di, dv, plan, r_mat);
+  return create64_owned(nrows, ncols, d[0], d[1], d[2], plan, r_mat);
 }
 
 int gooseberry_matrix_create_device_i64(dim_t nrows, dim_t ncols,
@@ -1138,7 +1160,13 @@ int gooseberry_spmult_i64(dim_t anrows, dim_t ancols, dim_t bncols,
       &once, &mat);
   free(rp);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
-  return spmult_once(mat, nr, ancols, bncols, bcolval, crowval + (size_t)rlo * bncols);
+  void *db = NULL;
+  rv = gb_upload(&db, bcolval, (size_t)ancols * bncols * sizeof(real_t));
+  if (rv != GOOSEBERRY_SUCCESS) {
+    gooseberry_matrix_free(mat);
+    return rv;
+  }
+  return spmult_once(mat, nr, bncols, db, crowval + (size_t)rlo * bncols);
 }
 
 int gooseberry_cgd_i64(dim_t nrows, dim_t ncols, const uint64_t *rowptr,
