@@ -612,10 +612,22 @@ int gbd_htod_many(uint32_t n, void *const *d_dst, const void *const *h_src,
     total += bytes[k];
     if (bytes[k] && gb_host_pinned(h_src[k])) pinned = true;
   }
-  if (total < GB_STAGE_MIN_HTOD || pinned || n > 16) {
+  if (pinned || n > 16) {
     for (uint32_t k = 0; k < n && rv == GOOSEBERRY_SUCCESS; ++k)
       rv = gbd_htod(d_dst[k], h_src[k], bytes[k]);
     return rv;
+  }
+  if (total < GB_STAGE_MIN_HTOD) {
+    // the runtime's copies, one synchronisation (also after a failed one:
+    // no copy from the caller's arrays is left in flight)
+    hipError_t he = hipSuccess;
+    for (uint32_t k = 0; k < n && he == hipSuccess; ++k)
+      if (bytes[k])
+        he = hipMemcpyAsync(d_dst[k], h_src[k], bytes[k], hipMemcpyHostToDevice, g_stream);
+    const hipError_t hs = hipStreamSynchronize(g_stream);
+    if (he != hipSuccess) return gb_hip_fail(he, "hipMemcpyAsync", __FILE__, __LINE__);
+    if (hs != hipSuccess) return gb_hip_fail(hs, "hipStreamSynchronize", __FILE__, __LINE__);
+    return GOOSEBERRY_SUCCESS;
   }
   gb_stage_seg seg[16];
   for (uint32_t k = 0; k < n; ++k)
