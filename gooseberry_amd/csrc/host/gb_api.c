@@ -655,6 +655,15 @@ int gooseberry_matrix_spmultsp(gooseberry_matrix_t a, gooseberry_matrix_t b,
       gbd_matrix_rowptr(b->m), gbd_matrix_rowind(b->m),
       gbd_matrix_rowval(b->m), NULL, &dptr, &dind, &dval, &cnnz);
   if (rv != GOOSEBERRY_SUCCESS) return rv;
+  /* without a plan: the defaults but the value index, whose build reads
+   * C's values twice (0.36 ms of a 3.7 ms product on the 150^3 Laplacian's
+   * square) and pays off only over many SpMVs; a replan adds it */
+  gooseberry_plan_t product;
+  if (!plan) {
+    memset(&product, 0, sizeof(product));
+    product.wide_values = 1;
+    plan = &product;
+  }
   return gb_matrix_adopt(anr, bnc, dptr, dind, dval, cnnz, plan, r_c);
 }
 

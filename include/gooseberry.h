@@ -813,7 +813,9 @@ typedef struct gooseberry_pagerank_info_t {
 
 /* The same products on resident matrices (B needs at least as many rows as
  * A has columns; only the first A-ncols rows are used).  The sparse form
- * returns C as a new resident matrix planned with `plan` (NULL: defaults);
+ * returns C as a new resident matrix planned with `plan` (NULL: the defaults
+ * without the value index, whose build costs about a tenth of a product and
+ * pays off only over many SpMVs: gooseberry_matrix_replan(C, NULL) adds it);
  * the dense form writes row-major C (A-nrows x B-ncols) to device memory.
  * The sparse forms stage the short rows' entries in 12 bytes of device
  * memory per product, up to 8 GiB; the environment variable

@@ -186,3 +186,23 @@ def test_resident_spgemm(gb, oracle):
     S = gb.Matrix(*random_csr(100, 50, 3, 24), ncols=50)
     with pytest.raises(gb.GooseberryError):
         B.spmultsp(S)   # 2000 columns against 100 rows
+
+
+def test_resident_product_plan_without_value_index(gb, oracle):
+    # a product planned by default skips the value index (its build reads
+    # C's values twice); a replan with the defaults adds it, and the SpMV's
+    # bits stay those of the plan without it
+    rp, ri, rv = gb.laplacian(2, 60)
+    A = gb.Matrix(rp, ri, rv)
+    C = A.spmultsp(A)
+    assert C.info()["values8"] == 0
+    x = np.random.default_rng(31).uniform(-1, 1, C.ncols)
+    y = C.spmv(x)
+    crp, cri, crv = C.download()
+    assert np.array_equal(bits(y), bits(oracle.spmv(crp, cri, crv, x)))
+    C.replan(None)
+    assert C.info()["values8"] > 0   # the square of a stencil has few distinct values
+    assert np.array_equal(bits(C.spmv(x)), bits(y))
+    # a plan given to the product is taken as it is
+    Cv = A.spmultsp(A, plan={"short_max": 64})
+    assert Cv.info()["values8"] == C.info()["values8"]
