@@ -736,21 +736,33 @@ gb_dot_partials(const double *__restrict__ x, const double *__restrict__ y, uint
   }
 }
 
-// a row-parted matrix (FP64): one SpMV per part into its rows of y, then
-// the p.Ap partials over the whole vectors
-static int gb_spmv_launch_parts(const gbd_matrix *m, const double *x, double *y,
-                                uint32_t ystride, double *partials, const gbd_cg_state *cg) {
+// a row-parted matrix: one SpMV per part into its rows of y (FP64, or FP32
+// over the parts' views of the FP32 copy), then for CG (FP64) the p.Ap
+// partials over the whole vectors
+template <typename T>
+static int gb_spmv_launch_parts(const gbd_matrix *m, const T *x, T *y, uint32_t ystride,
+                                double *partials, const gbd_cg_state *cg) {
   for (uint32_t q = 0; q < m->nparts; ++q) {
     const gbd_matrix *v = m->parts[q];
-    int rv = gb_spmv_launch_t<double>(v, v->val, v->vidx, v->vtab, x,
-                                      y + (size_t)m->prow[q] * ystride, ystride, nullptr,
-                                      nullptr);
+    const T *vv, *vt;
+    if constexpr (sizeof(T) == 8) {
+      vv = v->val;
+      vt = v->vtab;
+    } else {
+      vv = v->val32;
+      vt = v->vtab32;
+    }
+    if (!vv) return GOOSEBERRY_ERROR_INVALIDINPUT;
+    int rv = gb_spmv_launch_t<T>(v, vv, v->vidx, vt, x, y + (size_t)m->prow[q] * ystride,
+                                 ystride, nullptr, nullptr);
     if (rv != GOOSEBERRY_SUCCESS) return rv;
   }
-  if (cg) {
-    gb_dot_partials<<<gb_split_sum_grid(m), GB_BLOCK, 0, gb_stream()>>>(x, y, m->nrows,
-                                                                         partials);
-    GB_LAUNCH_CHECK();
+  if constexpr (sizeof(T) == 8) {
+    if (cg) {
+      gb_dot_partials<<<gb_split_sum_grid(m), GB_BLOCK, 0, gb_stream()>>>(x, y, m->nrows,
+                                                                           partials);
+      GB_LAUNCH_CHECK();
+    }
   }
   return GOOSEBERRY_SUCCESS;
 }
@@ -819,12 +831,7 @@ static int gb_spmv_launch_t(const gbd_matrix *m, const T *val, const uint8_t *vi
                             const T *vtab, const T *x, T *y, uint32_t ystride,
                             double *partials, const gbd_cg_state *cg) {
   if (m->nsplit) return gb_spmv_launch_split<T>(m, x, y, ystride, partials, cg);
-  if (m->nparts) {
-    if constexpr (sizeof(T) == 8) return gb_spmv_launch_parts(m, x, y, ystride, partials, cg);
-    gb_set_error("FP32 SpMV of a matrix held as row parts (64-bit row pointers) is not "
-                 "supported");
-    return GOOSEBERRY_ERROR_UNIMPLEMENTED;
-  }
+  if (m->nparts) return gb_spmv_launch_parts<T>(m, x, y, ystride, partials, cg);
   if (m->xt_ptr) {
     // x tiles: the FP64 values, or their FP32 copy in the layout's order
     // (made with the matrix's FP32 copy; CG runs in FP64 only)
@@ -1858,13 +1865,22 @@ static int gb_parts_refuse(const gbd_matrix *m, const char *what) {
 
 int gbd_matrix_make_f32(gbd_matrix *m) {
   if (m->val32) return GOOSEBERRY_SUCCESS;
-  int rp = gb_parts_refuse(m, "an FP32 copy");
-  if (rp != GOOSEBERRY_SUCCESS) return rp;
   GB_HIP(gb_dev_alloc((void **)&m->val32, (m->nnz ? m->nnz : 1) * sizeof(float)));
   if (m->nnz) {
     gb_to_f32<<<gb_div_up(m->nnz, GB_BLOCK), GB_BLOCK, 0, gb_stream()>>>(
         m->val, m->val32, m->nnz);
     GB_LAUNCH_CHECK();
+  }
+  if (m->nparts) {
+    // each part views its share of the copy (as its values view the
+    // matrix's) and rounds its own value table
+    for (uint32_t q = 0; q < m->nparts; ++q) {
+      gbd_matrix *v = m->parts[q];
+      v->val32 = m->val32 + m->pbase[q];
+      int rv = gb_vdict_f32(v);
+      if (rv != GOOSEBERRY_SUCCESS) return rv;
+    }
+    return GOOSEBERRY_SUCCESS;
   }
   if (m->nsplit) {
     GB_HIP(gb_dev_alloc((void **)&m->sval32, (m->nnz ? m->nnz : 1) * sizeof(float)));

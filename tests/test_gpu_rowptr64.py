@@ -53,6 +53,18 @@ def test_row_parts_spmv_bitwise(gb, oracle, part_nnz):
         B = np.random.default_rng(23).uniform(-1, 1, (30000, 3))
         assert np.array_equal(M.spmm(B.T.ravel(), 3),
                               gb.Matrix(rp, ri, rv).spmm(B.T.ravel(), 3))
+    # the FP32 SpMV over the parts' views of one FP32 copy: the unparted
+    # matrix's bits (the same row kernels on the same rows)
+    xf = x.astype(np.float32)
+    y32f = gb.Matrix(rp, ri, rv).spmv_f32(xf)
+    for pn in (nnz // 3 + 1, 10000):
+        part_nnz(pn)
+        M = gb.Matrix.from_rowptr64(rp64, ri, rv)
+        assert M.parts() > 1
+        assert np.array_equal(M.spmv_f32(xf), y32f), pn
+        M.replan({"short_max": 32})   # new plans per part, the same copy
+        assert np.array_equal(M.spmv_f32(xf), gb.Matrix(rp, ri, rv, plan={"short_max": 32})
+                              .spmv_f32(xf)), pn
     # within one part: an ordinary matrix
     part_nnz(None)
     M = gb.Matrix.from_rowptr64(rp64, ri, rv)
@@ -149,6 +161,10 @@ def test_row_parts_refuse_what_they_do_not_take(gb, part_nnz):
         assert e.value.name == "UNIMPLEMENTED"
     # the matrix still works after the refusals
     assert np.array_equal(M.spmv(b), gb.Matrix(rp, ri, rv).spmv(b))
+    # and takes the FP32 SpMV: each part's value table rounded to FP32
+    bf = b.astype(np.float32)
+    assert M.info()["values8"] > 0
+    assert np.array_equal(M.spmv_f32(bf), gb.Matrix(rp, ri, rv).spmv_f32(bf))
 
 
 def _fill_periodic(gb, dst, chunk, total_bytes):
